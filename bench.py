@@ -1,0 +1,214 @@
+"""Time-steps/s of the periodic hot path at 128^3 Kolmogorov on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+Workload (one "step" = one time step of the driver loop: predict_and_correct,
+iterate_correction until the corrector error is below tolerance,
+correct_velocity): Kolmogorov flow, nx = ny = nz = 128 (192^3 physical on
+the 3/2 grid), 2 pi box, Re = 100, dt = 0.005, implicitness 0.5,
+corrector tolerance 1e-10, seeded solenoidal field (seed 1, RMS 0.1).
+The state is resident in HBM before the timed region.  `value` is the
+plain FP64 form; the mixed form (FP32 transforms / nonlinear product) is
+reported beside it.  Prints one JSON line.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "time-steps/s + HBM roofline % at 128³ Kolmogorov; mixed+learned vs FP64, 1 GPU"
+PEAK_GBS = 8000.0
+N = 128
+RE, DT, RMS, SEED, TOL = 100.0, 0.005, 0.1, 1, 1e-10
+
+
+def kernel_bytes(n: int, b: int) -> dict:
+    """Algorithmic HBM bytes per launch of each periodic kernel.
+
+    S = one spectral FP64 vector field; intermediates hold b-byte complex.
+    The y-forward kernel's epilogue traffic depends on its mode and is
+    given per mode.
+    """
+    Nx, Ny, Nz, P = n // 2, n - 1, n - 1, 3 * n // 2
+    S = 3 * Ny * Nz * Nx * 16
+    return {
+        "y_inverse": S + 5 * P * Nz * Nx * b,
+        "z_inverse": 5 * P * Nz * Nx * b + 6 * P * P * Nx * b,
+        "x_nonlinear": 9 * P * P * Nx * b,
+        "z_forward": 3 * P * P * Nx * b + 3 * P * Nz * Nx * b,
+        "y_forward:predict": 3 * P * Nz * Nx * b + 3 * S,
+        "y_forward:correct": 3 * P * Nz * Nx * b + 4 * S,
+        "S": S,
+    }
+
+
+def rhs_model_bytes(n: int, b: int) -> int:
+    """The fixed 5-pass pencil model of one RHS (stated with the build's design)."""
+    Nx, Ny, Nz, P = n // 2, n - 1, n - 1, 3 * n // 2
+    return b * Nx * (6 * Ny * Nz + 18 * P * Nz + 18 * P * P)
+
+
+def run_steps(flow, state, k):
+    from dnsjax_amd.driver import run
+
+    r = run(flow, state, k)
+    return r.state, r.counts
+
+
+def gpu_form(precision: str, steps: int, warmup: int):
+    import torch
+
+    from dnsjax_amd.flows import TriplyPeriodicFlow
+    from dnsjax_amd.params import kolmogorov_2pi
+
+    p = kolmogorov_2pi(N, RE, DT, corrector_tolerance=TOL, max_corrector_iterations=10,
+                       precision=precision)
+    flow = TriplyPeriodicFlow(p)
+    state = flow.init_state(seed=SEED, rms=RMS)
+    state, _ = run_steps(flow, state, warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    state, counts = run_steps(flow, state, steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    # second, identical region with per-kernel HIP events on the launch stream
+    flow._native.profile_begin()
+    state, counts2 = run_steps(flow, state, steps)
+    torch.cuda.synchronize()
+    prof = flow._native.profile_end()
+    return {
+        "steps_per_s": steps / elapsed,
+        "ms_per_step": 1e3 * elapsed / steps,
+        "corrector_iters_per_step": sum(counts) / len(counts),
+        "counts": counts,
+        "prof": prof,
+        "prof_counts": counts2,
+    }
+
+
+def roofline(form: dict, b: int) -> dict:
+    kb = kernel_bytes(N, b)
+    prof = form["prof"]
+    # y_forward launches: one predict per step, the rest correct
+    steps = len(form["prof_counts"])
+    per = {}
+    for name, (ms, cnt) in prof.items():
+        if name == "y_forward":
+            n_pred = steps
+            n_corr = cnt - n_pred
+            byts = (n_pred * kb["y_forward:predict"] + n_corr * kb["y_forward:correct"]) / cnt
+        elif name in kb:
+            byts = kb[name]
+        else:
+            continue
+        avg_s = ms / cnt / 1e3
+        per[name] = {"avg_us": avg_s * 1e6, "launches": cnt, "bytes": byts,
+                     "gbs": byts / avg_s / 1e9}
+    dom = max(per, key=lambda k: prof[k][0])
+    d = per[dom]
+    rhs_names = ("y_inverse", "z_inverse", "x_nonlinear", "z_forward", "y_forward")
+    t_rhs = sum(per[k]["avg_us"] for k in rhs_names if k in per) * 1e-6
+    rhs_gbs = rhs_model_bytes(N, b) / t_rhs / 1e9
+    return {
+        "dominant": {"bound": "hbm", "kernel": dom, "achieved": round(d["gbs"], 1), "peak": PEAK_GBS,
+                     "unit": "GB/s", "frac": round(d["gbs"] / PEAK_GBS, 4), "traffic": None,
+                     "avg_us": round(d["avg_us"], 2), "bytes_per_launch": int(d["bytes"])},
+        "rhs": {"model_bytes": rhs_model_bytes(N, b), "t_us": round(t_rhs * 1e6, 2),
+                "achieved": round(rhs_gbs, 1), "frac": round(rhs_gbs / PEAK_GBS, 4)},
+        "kernels": {k: {kk: (round(vv, 2) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                    for k, v in per.items()},
+    }
+
+
+def cpu_baseline(budget_s: float) -> dict:
+    """The FP64 oracle on the host, single thread, a bounded sample of steps."""
+    import numpy as np
+
+    from dnsjax_amd.fields import periodic_field
+    from oracle import periodic as P
+    from oracle import stepper as S
+
+    L = 2 * math.pi
+    op = P.PeriodicParams(nx=N, ny=N, nz=N, lx=L, ly=L, lz=L, re=RE, dt=DT)
+    of = P.Flow(op)
+    pc, ic = S.make_stepper(P.get_rhs, P.predict, P.correct, P.norm)
+    u = periodic_field(N, N, N, L, L, L, RMS, SEED)
+
+    def step(s):
+        pred, rhs, err = pc(s, of)
+        c = 0
+        while err > TOL and c < 10:
+            pred, rhs, err = ic(s, pred, rhs, of)
+            c += 1
+        return P.correct_velocity(pred, of), c
+
+    u, _ = step(u)  # first step excluded (__main__.py:130-134)
+    n, t0 = 0, time.perf_counter()
+    while True:
+        u, _ = step(u)
+        n += 1
+        if time.perf_counter() - t0 > budget_s or n >= 50:
+            break
+    el = time.perf_counter() - t0
+    return {"value": n / el, "unit": "time-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{n} steps of the same 128^3 workload after 1 warm-up step, numpy FP64 oracle, 1 thread"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); the multi-GPU slab "
+              "decomposition is a design note, not part of this build.", flush=True)
+        sys.exit(2)
+    os.environ.setdefault("OMP_NUM_THREADS", "1")
+    fp64 = gpu_form("fp64", a.steps, a.warmup)
+    mixed = gpu_form("mixed", a.steps, a.warmup)
+    rf64 = roofline(fp64, 16)
+    rmix = roofline(mixed, 8)
+    out = {
+        "metric": METRIC,
+        "value": round(fp64["steps_per_s"], 3),
+        "unit": "time-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(fp64["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded solenoidal field, E(k) ~ k^4 exp(-2 (k/4)^2))",
+        "config": {"workload": "kolmogorov 128^3 (192^3 padded), 2pi box, Re=100, dt=0.005, c=0.5, tol=1e-10, seed 1 RMS 0.1",
+                   "form": "plain fp64", "corrector_iters_per_step": round(fp64["corrector_iters_per_step"], 3)},
+        "roofline": rf64["dominant"],
+        "rhs_roofline": rf64["rhs"],
+        "kernels": rf64["kernels"],
+        "mixed": {"value": round(mixed["steps_per_s"], 3), "ms_per_step": round(mixed["ms_per_step"], 4),
+                  "speedup_vs_fp64": round(mixed["steps_per_s"] / fp64["steps_per_s"], 3),
+                  "corrector_iters_per_step": round(mixed["corrector_iters_per_step"], 3),
+                  "roofline": rmix["dominant"], "rhs_roofline": rmix["rhs"]},
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(a.cpu_seconds)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

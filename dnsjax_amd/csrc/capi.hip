@@ -1,0 +1,251 @@
+// extern "C" entry points: argument checks, dispatch to the periodic or the
+// wall-bounded implementation, error reporting.  No exception crosses the ABI.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "context.hpp"
+
+namespace dnsmi {
+
+thread_local std::string g_create_error;
+
+void* Ctx::alloc(size_t bytes) {
+  void* p = nullptr;
+  if (bytes == 0) bytes = 256;
+  if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+  allocations.push_back(p);
+  return p;
+}
+
+hipEvent_t Ctx::ev_get() {
+  if (ev_used == ev_pool.size()) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    ev_pool.push_back(e);
+  }
+  return ev_pool[ev_used++];
+}
+
+int check_launch(Ctx* c, const char* what) {
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return c->fail(DNSMI_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+  return DNSMI_OK;
+}
+
+// periodic.hip
+int periodic_setup(Ctx* c);
+int periodic_get_rhs(Ctx* c, const double* state, double* rhs, hipStream_t s);
+int periodic_predict(Ctx* c, const double* state, const double* rhs, double* pred, hipStream_t s);
+int periodic_correct(Ctx* c, const double* pred_in, const double* rhs_prev,
+                     const double* rhs_next, double* pred_out, double* corr, hipStream_t s);
+int periodic_norm(Ctx* c, const double* corr, double* err, hipStream_t s);
+int periodic_predict_and_correct(Ctx* c, const double* state, double* pred, double* rhs_next,
+                                 double* err, hipStream_t s);
+int periodic_iterate_correction(Ctx* c, double* pred, const double* rhs_prev, double* rhs_next,
+                                double* err, hipStream_t s);
+int periodic_correct_velocity(Ctx* c, double* state, double* err, hipStream_t s);
+
+// walls.hip
+int walls_setup(Ctx* c);
+void walls_release(Ctx* c);
+int walls_get_rhs(Ctx* c, const double* state, double* rhs, hipStream_t s);
+int walls_predict(Ctx* c, const double* state, const double* rhs, double* pred, hipStream_t s);
+int walls_correct(Ctx* c, const double* state_prev, const double* pred_in,
+                  const double* rhs_prev, const double* rhs_next, double* pred_out,
+                  double* corr, hipStream_t s);
+int walls_norm(Ctx* c, const double* corr, double* err, hipStream_t s);
+int walls_predict_and_correct(Ctx* c, const double* state, double* pred, double* rhs_next,
+                              double* err, hipStream_t s);
+int walls_iterate_correction(Ctx* c, const double* state_prev, double* pred,
+                             const double* rhs_prev, double* rhs_next, double* err,
+                             hipStream_t s);
+
+}  // namespace dnsmi
+
+using dnsmi::Ctx;
+
+static hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+#define DNSMI_CHECK_CTX(ctx)                 \
+  if (!(ctx)) {                              \
+    dnsmi::g_create_error = "null context";  \
+    return DNSMI_EINVAL;                     \
+  }                                          \
+  Ctx* c = &(ctx)->impl;
+
+#define DNSMI_REQUIRE(cond, msg) \
+  if (!(cond)) return c->fail(DNSMI_EINVAL, msg);
+
+extern "C" {
+
+int dnsmi_create(const dnsmi_config* cfg, dnsmi_ctx** out) {
+  if (!cfg || !out) {
+    dnsmi::g_create_error = "null argument";
+    return DNSMI_EINVAL;
+  }
+  *out = nullptr;
+  dnsmi_ctx* h = new (std::nothrow) dnsmi_ctx();
+  if (!h) {
+    dnsmi::g_create_error = "host allocation failed";
+    return DNSMI_ENOMEM;
+  }
+  Ctx* c = &h->impl;
+  c->cfg = *cfg;
+  int rc = DNSMI_OK;
+  if (hipGetDevice(&c->device) != hipSuccess) {
+    rc = c->fail(DNSMI_EHIP, "no HIP device");
+  } else if (cfg->precision != DNSMI_FP64 && cfg->precision != DNSMI_MIXED) {
+    rc = c->fail(DNSMI_EINVAL, "precision must be DNSMI_FP64 or DNSMI_MIXED");
+  } else if (!(cfg->re > 0) || !(cfg->dt > 0) || cfg->implicitness < 0 || cfg->implicitness > 1) {
+    rc = c->fail(DNSMI_EINVAL, "need re > 0, dt > 0, 0 <= implicitness <= 1");
+  } else {
+    c->fp32 = cfg->precision == DNSMI_MIXED;
+    if (cfg->tile_cols > 0) c->tile_cols = cfg->tile_cols;
+    if (cfg->system == DNSMI_PLANE_COUETTE) {
+      c->walls = true;
+      rc = dnsmi::walls_setup(c);
+    } else {
+      rc = dnsmi::periodic_setup(c);
+    }
+  }
+  if (rc != DNSMI_OK) {
+    dnsmi::g_create_error = c->err;
+    dnsmi_destroy(h);
+    return rc;
+  }
+  *out = h;
+  return DNSMI_OK;
+}
+
+void dnsmi_destroy(dnsmi_ctx* ctx) {
+  if (!ctx) return;
+  Ctx* c = &ctx->impl;
+  if (c->walls) dnsmi::walls_release(c);
+  for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+  for (void* p : c->allocations) (void)hipFree(p);
+  delete ctx;
+}
+
+const char* dnsmi_last_error(const dnsmi_ctx* ctx) {
+  return ctx ? ctx->impl.err.c_str() : dnsmi::g_create_error.c_str();
+}
+
+long long dnsmi_spec_elems(const dnsmi_ctx* ctx) { return ctx ? ctx->impl.spec_elems : -1; }
+
+int dnsmi_get_rhs(dnsmi_ctx* ctx, const double* state, double* rhs, void* stream) {
+  DNSMI_CHECK_CTX(ctx)
+  DNSMI_REQUIRE(state && rhs && (const void*)state != (const void*)rhs, "get_rhs: bad pointers");
+  return c->walls ? dnsmi::walls_get_rhs(c, state, rhs, as_stream(stream))
+                  : dnsmi::periodic_get_rhs(c, state, rhs, as_stream(stream));
+}
+
+int dnsmi_predict(dnsmi_ctx* ctx, const double* state, const double* rhs, double* pred,
+                  void* stream) {
+  DNSMI_CHECK_CTX(ctx)
+  DNSMI_REQUIRE(state && rhs && pred, "predict: null pointer");
+  return c->walls ? dnsmi::walls_predict(c, state, rhs, pred, as_stream(stream))
+                  : dnsmi::periodic_predict(c, state, rhs, pred, as_stream(stream));
+}
+
+int dnsmi_correct(dnsmi_ctx* ctx, const double* state_prev, const double* pred_in,
+                  const double* rhs_prev, const double* rhs_next, double* pred_out,
+                  double* correction_out, void* stream) {
+  DNSMI_CHECK_CTX(ctx)
+  DNSMI_REQUIRE(state_prev && pred_in && rhs_prev && rhs_next && pred_out,
+                "correct: null pointer");
+  return c->walls ? dnsmi::walls_correct(c, state_prev, pred_in, rhs_prev, rhs_next, pred_out,
+                                         correction_out, as_stream(stream))
+                  : dnsmi::periodic_correct(c, pred_in, rhs_prev, rhs_next, pred_out,
+                                            correction_out, as_stream(stream));
+}
+
+int dnsmi_norm(dnsmi_ctx* ctx, const double* correction, double* err_dev, void* stream) {
+  DNSMI_CHECK_CTX(ctx)
+  DNSMI_REQUIRE(correction && err_dev, "norm: null pointer");
+  return c->walls ? dnsmi::walls_norm(c, correction, err_dev, as_stream(stream))
+                  : dnsmi::periodic_norm(c, correction, err_dev, as_stream(stream));
+}
+
+int dnsmi_predict_and_correct(dnsmi_ctx* ctx, const double* state, double* pred_out,
+                              double* rhs_next_out, double* err_dev, void* stream) {
+  DNSMI_CHECK_CTX(ctx)
+  DNSMI_REQUIRE(state && pred_out && rhs_next_out && err_dev, "predict_and_correct: null pointer");
+  DNSMI_REQUIRE((const void*)state != (const void*)pred_out &&
+                    (const void*)state != (const void*)rhs_next_out &&
+                    (const void*)pred_out != (const void*)rhs_next_out,
+                "predict_and_correct: outputs must not alias the state or each other");
+  return c->walls ? dnsmi::walls_predict_and_correct(c, state, pred_out, rhs_next_out, err_dev,
+                                                     as_stream(stream))
+                  : dnsmi::periodic_predict_and_correct(c, state, pred_out, rhs_next_out,
+                                                        err_dev, as_stream(stream));
+}
+
+int dnsmi_iterate_correction(dnsmi_ctx* ctx, const double* state_prev, double* pred,
+                             const double* rhs_prev, double* rhs_next_out, double* err_dev,
+                             void* stream) {
+  DNSMI_CHECK_CTX(ctx)
+  DNSMI_REQUIRE(state_prev && pred && rhs_prev && rhs_next_out && err_dev,
+                "iterate_correction: null pointer");
+  DNSMI_REQUIRE((const void*)pred != (const void*)state_prev &&
+                    (const void*)pred != (const void*)rhs_prev &&
+                    (const void*)pred != (const void*)rhs_next_out,
+                "iterate_correction: pred must not alias the other arrays");
+  return c->walls ? dnsmi::walls_iterate_correction(c, state_prev, pred, rhs_prev, rhs_next_out,
+                                                    err_dev, as_stream(stream))
+                  : dnsmi::periodic_iterate_correction(c, pred, rhs_prev, rhs_next_out, err_dev,
+                                                       as_stream(stream));
+}
+
+int dnsmi_correct_velocity(dnsmi_ctx* ctx, double* state, double* err_dev, void* stream) {
+  DNSMI_CHECK_CTX(ctx)
+  DNSMI_REQUIRE(state, "correct_velocity: null pointer");
+  if (c->walls) {
+    DNSMI_REQUIRE(!err_dev, "correct_velocity: walls measure no correction");
+    return DNSMI_OK;
+  }
+  return dnsmi::periodic_correct_velocity(c, state, err_dev, as_stream(stream));
+}
+
+static const char* const kKernelNames[dnsmi::KID_COUNT] = {
+    "y_inverse", "z_inverse", "x_nonlinear", "z_forward", "y_forward", "pointwise", "finalize_norm",
+    "walls_z_inverse", "walls_x_nonlinear", "walls_z_forward", "walls_imm", "walls_norm"};
+
+int dnsmi_profile_begin(dnsmi_ctx* ctx) {
+  DNSMI_CHECK_CTX(ctx)
+  c->prof_recs.clear();
+  c->ev_used = 0;
+  c->prof = true;
+  return DNSMI_OK;
+}
+
+int dnsmi_profile_end(dnsmi_ctx* ctx, int n_ids, double* total_ms, long long* launches) {
+  DNSMI_CHECK_CTX(ctx)
+  DNSMI_REQUIRE(n_ids >= 0 && (n_ids == 0 || (total_ms && launches)), "profile_end: bad arrays");
+  c->prof = false;
+  for (int i = 0; i < n_ids; ++i) {
+    total_ms[i] = 0.0;
+    launches[i] = 0;
+  }
+  for (const dnsmi::ProfRec& r : c->prof_recs) {
+    if (hipEventSynchronize(r.b) != hipSuccess) return c->fail(DNSMI_EHIP, "profile_end: event sync");
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, r.a, r.b) != hipSuccess)
+      return c->fail(DNSMI_EHIP, "profile_end: elapsed time");
+    if (r.kid < n_ids) {
+      total_ms[r.kid] += ms;
+      launches[r.kid] += 1;
+    }
+  }
+  c->prof_recs.clear();
+  c->ev_used = 0;
+  return DNSMI_OK;
+}
+
+const char* dnsmi_kernel_name(int id) {
+  return (id >= 0 && id < dnsmi::KID_COUNT) ? kKernelNames[id] : "";
+}
+
+}  // extern "C"

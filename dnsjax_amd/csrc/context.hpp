@@ -1,0 +1,121 @@
+// Library-internal context shared by the periodic and wall-bounded paths.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/dnsmi.h"
+
+namespace dnsmi {
+
+struct WallTables;  // wall-bounded set-up data (walls.hip)
+
+// Kernel ids for the per-kernel event timing (dnsmi_profile_*).
+enum KernelId {
+  KID_Y_INVERSE = 0,
+  KID_Z_INVERSE = 1,
+  KID_X_NONLINEAR = 2,
+  KID_Z_FORWARD = 3,
+  KID_Y_FORWARD = 4,
+  KID_POINTWISE = 5,
+  KID_FINALIZE = 6,
+  KID_W_Z_INVERSE = 7,
+  KID_W_X_NONLINEAR = 8,
+  KID_W_Z_FORWARD = 9,
+  KID_W_IMM = 10,
+  KID_W_NORM = 11,
+  KID_COUNT = 12
+};
+
+struct ProfRec {
+  int kid;
+  hipEvent_t a, b;
+};
+
+struct Ctx {
+  dnsmi_config cfg{};
+  bool walls = false;
+  int device = 0;
+  int nx = 0, ny = 0, nz = 0;
+  int Nx1 = 0, Ny1 = 0, Nz1 = 0;  // stored modes: nx/2, ny-1, nz-1 (walls: Ny1 = ny)
+  int Px = 0, Py = 0, Pz = 0;     // padded grid (walls: Py = ny)
+  long long spec_elems = 0;       // complex elements of one vector field
+  bool fp32 = false;              // FFT/nonlinear precision
+
+  // FFT intermediates (T-typed complex), aliased across passes.
+  void* bufA = nullptr;
+  void* bufB = nullptr;
+  size_t bufA_bytes = 0, bufB_bytes = 0;
+
+  // exp(+2 pi i m / P) tables per axis (x, y, z) in both precisions.
+  void* tw64[3] = {nullptr, nullptr, nullptr};
+  void* tw32[3] = {nullptr, nullptr, nullptr};
+
+  // Stored-mode wavenumbers and base-flow profiles.
+  double* kx = nullptr;
+  double* ky = nullptr;
+  double* kz = nullptr;
+  double* base = nullptr;   // (3, Py)
+  double* cbase = nullptr;  // (3, Py)
+  double* nbase = nullptr;  // (3, Py)
+
+  double* rhs0 = nullptr;      // N^0 of the predictor (spectral vector field)
+  double* partials = nullptr;  // per-block partial sums of the norm
+  int max_partials = 0;
+
+  WallTables* wt = nullptr;
+
+  int tile_cols = 16;
+
+  // per-kernel timing (off unless dnsmi_profile_begin was called)
+  bool prof = false;
+  std::vector<ProfRec> prof_recs;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  hipEvent_t ev_get();
+
+  std::string err;
+  std::vector<void*> allocations;
+
+  int fail(int code, const std::string& msg) {
+    err = msg;
+    return code;
+  }
+  void* alloc(size_t bytes);
+};
+
+extern thread_local std::string g_create_error;
+
+int check_launch(Ctx* c, const char* what);
+
+// Records an event pair around the launches in its scope when profiling.
+struct ProfScope {
+  Ctx* c;
+  int kid;
+  hipStream_t s;
+  hipEvent_t a = nullptr;
+  ProfScope(Ctx* c_, int kid_, hipStream_t s_) : c(c_), kid(kid_), s(s_) {
+    if (c->prof) {
+      a = c->ev_get();
+      (void)hipEventRecord(a, s);
+    }
+  }
+  ~ProfScope() {
+    if (c->prof && a) {
+      hipEvent_t b = c->ev_get();
+      (void)hipEventRecord(b, s);
+      c->prof_recs.push_back({kid, a, b});
+    }
+  }
+};
+
+// Shared small kernels (periodic.hip).
+int finalize_norm(Ctx* c, int nblocks, double* err_dev, hipStream_t s);
+
+}  // namespace dnsmi
+
+struct dnsmi_ctx {
+  dnsmi::Ctx impl;
+};

@@ -1,0 +1,977 @@
+// Triply periodic RHS pipeline and spectral updates, gfx950.
+//
+// One RHS evaluation (flows/monochromatic.py:366-386 via rhs.py:34-92 and
+// fft.py:291-408) is five passes over HBM, each a batch of independent
+// length-P transforms done as register-DFT / LDS / register-DFT:
+//   K1 y-inverse  (pad ky, curl prep)   u (3,Ny1,Nz1,Nx1) f64 -> W1 (5,Py,Nz1,Nx1)
+//   K2 z-inverse  (pad kz, curl)        W1 -> W2 (6,Py,Pz,Nx1)
+//   K3 x pass     (c2r, u x w + base-flow terms, r2c, truncate)
+//                                      W2 -> W3 (3,Py,Pz,Nx1)
+//   K4 z-forward  (truncate kz)         W3 -> W4 (3,Py,Nz1,Nx1)
+//   K5 y-forward  (truncate ky) + Leray projection + fused epilogue:
+//        plain RHS / Euler predictor / CN corrector with FP64 norm partials.
+// W1..W4 hold T = double (plain) or float (mixed) complex; the state, the
+// RHS, the projection, the Helmholtz multiply and the norm stay FP64.
+//
+// The curl is split across passes: K1 transforms u and i*ky*u_{x,z} (the
+// only factors that vary along y), K2 applies the i*kz factors before its
+// transforms, K3 applies the i*kx factors on the kx bins before its c2r.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "context.hpp"
+#include "dft.hpp"
+
+namespace dnsmi {
+
+enum { MODE_RHS = 0, MODE_PREDICT = 1, MODE_CORRECT = 2 };
+
+struct StepArgs {
+  const double* state;     // MODE_PREDICT: u^n
+  const double* rhs_prev;  // MODE_CORRECT: N_prev
+  double* pred;            // MODE_PREDICT: out; MODE_CORRECT: in/out
+  double* rhs_out;         // N of this evaluation
+  double* partials;        // MODE_CORRECT: per-block sum k_metric |delta|^2
+  double dt, c, re;
+};
+
+// Stored index of padded index n along an axis of `nmodes` modes padded to
+// P (fft.py:78-86, 127-135); -1 for the zero band.
+__device__ __forceinline__ int stored_of_padded(int n, int nmodes, int P) {
+  const int half = nmodes / 2;
+  if (n < half) return n;
+  if (n > P - half) return n - (P - nmodes + 1);
+  return -1;
+}
+
+template <typename T>
+__device__ __forceinline__ cx<T> to_t(cx<double> d) {
+  return {static_cast<T>(d.x), static_cast<T>(d.y)};
+}
+
+// ------------------------------------------------------------------ K1
+template <typename T, int P, int TC>
+__global__ void __launch_bounds__(TC * Plan<P>::N1)
+    k_y_inverse(const cx<double>* __restrict__ u, cx<T>* __restrict__ w1,
+                const cx<T>* __restrict__ tw, const double* __restrict__ ky, int ny, int Nz1,
+                int Nx1) {
+  constexpr int N1 = Plan<P>::N1, N2 = Plan<P>::N2;
+  __shared__ cx<T> lds[TC * P];
+  const int tid = threadIdx.x, col = tid % TC, j = tid / TC;
+  const int ntile = (Nx1 + TC - 1) / TC;
+  const int kz = blockIdx.x / ntile;
+  const int kx = (blockIdx.x - kz * ntile) * TC + col;
+  const bool ok = kx < Nx1;
+  const size_t plane = (size_t)Nz1 * Nx1;
+  const size_t in_comp = (size_t)(ny - 1) * plane;
+  const size_t out_f = (size_t)P * plane;
+  const size_t off = (size_t)kz * Nx1 + kx;
+  // pass 0: u_x -> F0 = Y(u_x), F4 = Y(i ky u_x); pass 1: u_z -> F2, F3;
+  // pass 2: u_y -> F1.
+#pragma unroll 1
+  for (int pass = 0; pass < 3; ++pass) {
+    const int c = pass == 0 ? 0 : (pass == 1 ? 2 : 1);
+    const int fa = c;
+    const int fb = pass == 0 ? 4 : 3;
+    cx<T> xa[N1], xb[N1];
+    if (j < N2) {
+#pragma unroll
+      for (int a = 0; a < N1; ++a) {
+        const int s = stored_of_padded(N2 * a + j, ny, P);
+        cx<T> v{0, 0}, vk{0, 0};
+        if (ok && s >= 0) {
+          const cx<double> d = u[c * in_comp + (size_t)s * plane + off];
+          const double k = ky[s];
+          v = to_t<T>(d);
+          vk = {static_cast<T>(-k * d.y), static_cast<T>(k * d.x)};
+        }
+        xa[a] = v;
+        xb[a] = vk;
+      }
+      step_a_store<P, N1, N2, +1, TC>(xa, j, col, lds, tw);
+    }
+    __syncthreads();
+    if (j < N1 && ok) {
+      cx<T> y[N2];
+      step_b_load<P, N1, N2, +1, TC>(y, j, col, lds);
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) w1[fa * out_f + (size_t)(j + N1 * k2) * plane + off] = y[k2];
+    }
+    __syncthreads();
+    if (pass < 2) {
+      if (j < N2) step_a_store<P, N1, N2, +1, TC>(xb, j, col, lds, tw);
+      __syncthreads();
+      if (j < N1 && ok) {
+        cx<T> y[N2];
+        step_b_load<P, N1, N2, +1, TC>(y, j, col, lds);
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2)
+          w1[fb * out_f + (size_t)(j + N1 * k2) * plane + off] = y[k2];
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// ------------------------------------------------------------------ K2
+template <typename T, int P, int TC>
+__global__ void __launch_bounds__(TC * Plan<P>::N1)
+    k_z_inverse(const cx<T>* __restrict__ w1, cx<T>* __restrict__ w2, const cx<T>* __restrict__ tw,
+                const double* __restrict__ kz_tab, int nz, int Py, int Nx1) {
+  constexpr int N1 = Plan<P>::N1, N2 = Plan<P>::N2;
+  __shared__ cx<T> lds[TC * P];
+  const int tid = threadIdx.x, col = tid % TC, j = tid / TC;
+  const int ntile = (Nx1 + TC - 1) / TC;
+  const int y = blockIdx.x / ntile;
+  const int kx = (blockIdx.x - y * ntile) * TC + col;
+  const bool ok = kx < Nx1;
+  const int Nz1 = nz - 1;
+  const size_t in_f = (size_t)Py * Nz1 * Nx1;
+  const size_t in_off = (size_t)y * Nz1 * Nx1 + kx;
+  const size_t out_f = (size_t)Py * P * Nx1;
+  const size_t out_off = (size_t)y * P * Nx1 + kx;
+  // outputs: 0 u_x, 1 u_y, 2 u_z, 3 w_x, 4 Z(i kz Yu_x), 5 Z(-Y(i ky u_x))
+#pragma unroll 1
+  for (int pass = 0; pass < 4; ++pass) {
+    cx<T> xa[N1], xb[N1];
+    if (j < N2) {
+#pragma unroll
+      for (int a = 0; a < N1; ++a) {
+        const int s = stored_of_padded(N2 * a + j, nz, P);
+        cx<T> va{0, 0}, vb{0, 0};
+        if (ok && s >= 0) {
+          const T k = static_cast<T>(kz_tab[s]);
+          const size_t e = in_off + (size_t)s * Nx1;
+          if (pass == 0) {
+            const cx<T> f0 = w1[0 * in_f + e];
+            va = f0;
+            vb = {-k * f0.y, k * f0.x};
+          } else if (pass == 1) {
+            const cx<T> f1 = w1[1 * in_f + e];
+            const cx<T> f3 = w1[3 * in_f + e];
+            va = f1;
+            vb = {f3.x + k * f1.y, f3.y - k * f1.x};
+          } else if (pass == 2) {
+            va = w1[2 * in_f + e];
+          } else {
+            const cx<T> f4 = w1[4 * in_f + e];
+            va = {-f4.x, -f4.y};
+          }
+        }
+        xa[a] = va;
+        xb[a] = vb;
+      }
+      step_a_store<P, N1, N2, +1, TC>(xa, j, col, lds, tw);
+    }
+    const int fa = pass == 0 ? 0 : (pass == 1 ? 1 : (pass == 2 ? 2 : 5));
+    const int fb = pass == 0 ? 4 : 3;
+    __syncthreads();
+    if (j < N1 && ok) {
+      cx<T> o[N2];
+      step_b_load<P, N1, N2, +1, TC>(o, j, col, lds);
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2)
+        w2[fa * out_f + out_off + (size_t)(j + N1 * k2) * Nx1] = o[k2];
+    }
+    __syncthreads();
+    if (pass < 2) {
+      if (j < N2) step_a_store<P, N1, N2, +1, TC>(xb, j, col, lds, tw);
+      __syncthreads();
+      if (j < N1 && ok) {
+        cx<T> o[N2];
+        step_b_load<P, N1, N2, +1, TC>(o, j, col, lds);
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2)
+          w2[fb * out_f + out_off + (size_t)(j + N1 * k2) * Nx1] = o[k2];
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// ------------------------------------------------------------------ K3
+// One row (fixed y, z) per N1 threads; R rows per workgroup.  Inverse plan
+// (N1, N2), forward plan (N2, N1): the inverse's output distribution
+// x = j + N1 k2 is exactly the forward's input distribution, so the
+// physical values never leave registers.
+template <int N1, int N2>
+struct XRow {
+  static constexpr int a = RowLds<N1, N2>::value, b = RowLds<N2, N1>::value;
+  static constexpr int value = a > b ? a : b;
+};
+
+template <typename T, int P, int R>
+__global__ void __launch_bounds__(R * Plan<P>::N1)
+    k_x_nonlinear(const cx<T>* __restrict__ w2, cx<T>* __restrict__ w3,
+                  const cx<T>* __restrict__ tw, const double* __restrict__ kx_tab,
+                  const double* __restrict__ base, const double* __restrict__ cbase,
+                  const double* __restrict__ nbase, int nx, int Py, int Pz, int ybase_stride) {
+  constexpr int N1 = Plan<P>::N1, N2 = Plan<P>::N2;
+  constexpr int ROW = XRow<N1, N2>::value;
+  __shared__ cx<T> lds[R * ROW];
+  const int tid = threadIdx.x, r = tid / N1, j = tid - r * N1;
+  const int nrows = Py * Pz;
+  const int row = blockIdx.x * R + r;
+  const bool ok = row < nrows;
+  const int yrow = ok ? row / Pz : 0;
+  const int Nx1 = nx / 2;
+  const size_t fstride = (size_t)nrows * Nx1;
+  const cx<T>* in = w2 + (size_t)(ok ? row : 0) * Nx1;
+  cx<T>* rl = lds + r * ROW;
+
+  T ux[N2], uy[N2], uz[N2], wx[N2], wy[N2], wz[N2];
+#pragma unroll 1
+  for (int pair = 0; pair < 3; ++pair) {
+    if (j < N2) {
+      cx<T> z[N1];
+#pragma unroll
+      for (int a = 0; a < N1; ++a) {
+        const int m = N2 * a + j;
+        int k = -1;
+        bool cj = false;
+        if (m < Nx1) {
+          k = m;
+        } else if (m > P - Nx1) {
+          k = P - m;
+          cj = true;
+        }
+        cx<T> X{0, 0}, Y{0, 0};
+        if (ok && k >= 0) {
+          const T kk = static_cast<T>(kx_tab[k]);
+          if (pair == 0) {  // (u_z, w_y = Z(i kz Yu_x) - i kx u_z)
+            X = in[2 * fstride + k];
+            const cx<T> g = in[4 * fstride + k];
+            Y = {g.x + kk * X.y, g.y - kk * X.x};
+          } else if (pair == 1) {  // (u_y, w_z = Z(-Y(i ky u_x)) + i kx u_y)
+            X = in[1 * fstride + k];
+            const cx<T> g = in[5 * fstride + k];
+            Y = {g.x - kk * X.y, g.y + kk * X.x};
+          } else {  // (u_x, w_x)
+            X = in[0 * fstride + k];
+            Y = in[3 * fstride + k];
+          }
+          if (k == 0) {  // irfft ignores the imaginary part of the DC bin
+            X.y = 0;
+            Y.y = 0;
+          }
+        }
+        // Z_m = X_m + i Y_m with X_{P-k} = conj(X_k)
+        z[a] = cj ? cx<T>{X.x + Y.y, Y.x - X.y} : cx<T>{X.x - Y.y, X.y + Y.x};
+      }
+      row_step_a_store<P, N1, N2, +1>(z, j, rl, tw);
+    }
+    __syncthreads();
+    {
+      cx<T> o[N2];
+      row_step_b_load<P, N1, N2, +1>(o, j, rl);
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) {
+        if (pair == 0) {
+          uz[k2] = o[k2].x;
+          wy[k2] = o[k2].y;
+        } else if (pair == 1) {
+          uy[k2] = o[k2].x;
+          wz[k2] = o[k2].y;
+        } else {
+          ux[k2] = o[k2].x;
+          wx[k2] = o[k2].y;
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  // u' x w' + (u' x curl U + U x w') + U x curl U   (rhs.py:80-88)
+  const T U0 = static_cast<T>(base[0 * ybase_stride + yrow]);
+  const T U1 = static_cast<T>(base[1 * ybase_stride + yrow]);
+  const T U2 = static_cast<T>(base[2 * ybase_stride + yrow]);
+  const T C0 = static_cast<T>(cbase[0 * ybase_stride + yrow]);
+  const T C1 = static_cast<T>(cbase[1 * ybase_stride + yrow]);
+  const T C2 = static_cast<T>(cbase[2 * ybase_stride + yrow]);
+  const T B0 = static_cast<T>(nbase[0 * ybase_stride + yrow]);
+  const T B1 = static_cast<T>(nbase[1 * ybase_stride + yrow]);
+  const T B2 = static_cast<T>(nbase[2 * ybase_stride + yrow]);
+  T nl[3][N2];
+#pragma unroll
+  for (int k2 = 0; k2 < N2; ++k2) {
+    const T a0 = ux[k2], a1 = uy[k2], a2 = uz[k2];
+    const T b0 = wx[k2], b1 = wy[k2], b2 = wz[k2];
+    nl[0][k2] = (a1 * b2 - a2 * b1) + (((a1 * C2 - a2 * C1) + (U1 * b2 - U2 * b1)) + B0);
+    nl[1][k2] = (a2 * b0 - a0 * b2) + (((a2 * C0 - a0 * C2) + (U2 * b0 - U0 * b2)) + B1);
+    nl[2][k2] = (a0 * b1 - a1 * b0) + (((a0 * C1 - a1 * C0) + (U0 * b1 - U1 * b0)) + B2);
+  }
+
+  const T inv = static_cast<T>(1.0 / P);
+  cx<T>* out = w3 + (size_t)(ok ? row : 0) * Nx1;
+#pragma unroll 1
+  for (int c = 0; c < 3; ++c) {
+    {
+      cx<T> v[N2];
+#pragma unroll
+      for (int a = 0; a < N2; ++a) v[a] = {c == 0 ? nl[0][a] : (c == 1 ? nl[1][a] : nl[2][a]), 0};
+      row_step_a_store<P, N2, N1, -1>(v, j, rl, tw);
+    }
+    __syncthreads();
+    if (j < N2 && ok) {
+      cx<T> o[N1];
+      row_step_b_load<P, N2, N1, -1>(o, j, rl);
+#pragma unroll
+      for (int k2 = 0; k2 < N1; ++k2) {
+        const int k = j + N2 * k2;
+        if (k < Nx1) out[c * fstride + k] = cscale(o[k2], inv);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------ K4
+template <typename T, int P, int TC>
+__global__ void __launch_bounds__(TC * Plan<P>::N1)
+    k_z_forward(const cx<T>* __restrict__ w3, cx<T>* __restrict__ w4, const cx<T>* __restrict__ tw,
+                int nz, int Py, int Nx1) {
+  constexpr int N1 = Plan<P>::N1, N2 = Plan<P>::N2;
+  __shared__ cx<T> lds[TC * P];
+  const int tid = threadIdx.x, col = tid % TC, j = tid / TC;
+  const int ntile = (Nx1 + TC - 1) / TC;
+  const int y = blockIdx.x / ntile;
+  const int kx = (blockIdx.x - y * ntile) * TC + col;
+  const bool ok = kx < Nx1;
+  const int Nz1 = nz - 1;
+  const size_t in_f = (size_t)Py * P * Nx1;
+  const size_t in_off = (size_t)y * P * Nx1 + kx;
+  const size_t out_f = (size_t)Py * Nz1 * Nx1;
+  const size_t out_off = (size_t)y * Nz1 * Nx1 + kx;
+  const T inv = static_cast<T>(1.0 / P);
+#pragma unroll 1
+  for (int c = 0; c < 3; ++c) {
+    if (j < N2) {
+      cx<T> x[N1];
+#pragma unroll
+      for (int a = 0; a < N1; ++a)
+        x[a] = ok ? w3[c * in_f + in_off + (size_t)(N2 * a + j) * Nx1] : cx<T>{0, 0};
+      step_a_store<P, N1, N2, -1, TC>(x, j, col, lds, tw);
+    }
+    __syncthreads();
+    if (j < N1 && ok) {
+      cx<T> o[N2];
+      step_b_load<P, N1, N2, -1, TC>(o, j, col, lds);
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) {
+        const int s = stored_of_padded(j + N1 * k2, nz, P);
+        if (s >= 0) w4[c * out_f + out_off + (size_t)s * Nx1] = cscale(o[k2], inv);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------ K5
+// Deterministic block sum (fixed shuffle tree, then waves in order).
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int w = 0; w < nw; ++w) s += red[w];
+  }
+  return s;
+}
+
+template <typename T, int P, int TC, int MODE>
+__global__ void __launch_bounds__(TC * Plan<P>::N1)
+    k_y_forward(const cx<T>* __restrict__ w4, const cx<T>* __restrict__ tw,
+                const double* __restrict__ kx_tab, const double* __restrict__ ky_tab,
+                const double* __restrict__ kz_tab, int ny, int Nz1, int Nx1, StepArgs args) {
+  constexpr int N1 = Plan<P>::N1, N2 = Plan<P>::N2;
+  __shared__ cx<T> lds[TC * P];
+  __shared__ double red[16];
+  const int tid = threadIdx.x, col = tid % TC, j = tid / TC;
+  const int ntile = (Nx1 + TC - 1) / TC;
+  const int kz = blockIdx.x / ntile;
+  const int kx = (blockIdx.x - kz * ntile) * TC + col;
+  const bool ok = kx < Nx1;
+  const size_t plane = (size_t)Nz1 * Nx1;
+  const size_t in_f = (size_t)P * plane;
+  const size_t off = (size_t)kz * Nx1 + kx;
+  const size_t comp = (size_t)(ny - 1) * plane;
+  const T inv = static_cast<T>(1.0 / P);
+  cx<T> R[3][N2];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    if (j < N2) {
+      cx<T> x[N1];
+#pragma unroll
+      for (int a = 0; a < N1; ++a)
+        x[a] = ok ? w4[c * in_f + (size_t)(N2 * a + j) * plane + off] : cx<T>{0, 0};
+      step_a_store<P, N1, N2, -1, TC>(x, j, col, lds, tw);
+    }
+    __syncthreads();
+    if (j < N1) {
+      step_b_load<P, N1, N2, -1, TC>(R[c], j, col, lds);
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) R[c][k2] = cscale(R[c][k2], inv);
+    }
+    __syncthreads();
+  }
+
+  double acc = 0.0;
+  if (j < N1 && ok) {
+    const double kxv = kx_tab[kx], kzv = kz_tab[kz];
+    const double km = kx == 0 ? 1.0 : 2.0;
+#pragma unroll
+    for (int k2 = 0; k2 < N2; ++k2) {
+      const int s = stored_of_padded(j + N1 * k2, ny, P);
+      if (s < 0) continue;
+      const double kyv = ky_tab[s];
+      const double lapl = -((kxv * kxv + kyv * kyv) + kzv * kzv);
+      const double il = lapl < 0 ? 1.0 / lapl : 0.0;
+      const cx<double> n0 = {R[0][k2].x, R[0][k2].y};
+      const cx<double> n1 = {R[1][k2].x, R[1][k2].y};
+      const cx<double> n2 = {R[2][k2].x, R[2][k2].y};
+      // div = i kx N0 + i ky N1 + i kz N2 ; p = inv_lapl * div
+      const cx<double> dv = {((-kxv * n0.y) + (-kyv * n1.y)) + (-kzv * n2.y),
+                             ((kxv * n0.x) + (kyv * n1.x)) + (kzv * n2.x)};
+      const cx<double> pr = {il * dv.x, il * dv.y};
+      const double kk[3] = {kxv, kyv, kzv};
+      const cx<double> nn[3] = {n0, n1, n2};
+      const bool mean = (s == 0 && kz == 0 && kx == 0);
+      const double ldt1 = mean ? 0.0 : 1.0 / args.dt + (1.0 - args.c) * lapl / args.re;
+      const double ildt2 = mean ? 0.0 : 1.0 / (1.0 / args.dt - args.c * lapl / args.re);
+      const size_t pos = (size_t)s * plane + off;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        // rhs = N - i k p
+        const cx<double> rh = {nn[c].x + kk[c] * pr.y, nn[c].y - kk[c] * pr.x};
+        const size_t idx = c * comp + pos;
+        cx<double>* rout = reinterpret_cast<cx<double>*>(args.rhs_out);
+        if (MODE == MODE_RHS) {
+          rout[idx] = rh;
+        } else if (MODE == MODE_PREDICT) {
+          const cx<double> u = reinterpret_cast<const cx<double>*>(args.state)[idx];
+          cx<double>* pd = reinterpret_cast<cx<double>*>(args.pred);
+          pd[idx] = {(u.x * ldt1 + rh.x) * ildt2, (u.y * ldt1 + rh.y) * ildt2};
+          rout[idx] = rh;
+        } else {
+          const cx<double> np = reinterpret_cast<const cx<double>*>(args.rhs_prev)[idx];
+          cx<double>* pd = reinterpret_cast<cx<double>*>(args.pred);
+          const cx<double> d = {args.c * (rh.x - np.x) * ildt2, args.c * (rh.y - np.y) * ildt2};
+          const cx<double> p0 = pd[idx];
+          pd[idx] = {p0.x + d.x, p0.y + d.y};
+          rout[idx] = rh;
+          acc += km * (d.x * d.x) + km * (d.y * d.y);
+        }
+      }
+    }
+  }
+  if (MODE == MODE_CORRECT) {
+    const double s = block_sum(acc, red);
+    if (threadIdx.x == 0) args.partials[blockIdx.x] = s;
+  }
+}
+
+// ------------------------------------------------------------------ pointwise
+struct PointArgs {
+  const double* a;  // state / pred_in / correction
+  const double* b;  // rhs / rhs_prev
+  const double* c2; // rhs_next
+  double* out;      // pred_out / state out
+  double* out2;     // correction out
+  double* partials;
+  double dt, c, re;
+  int ny, Nz1, Nx1;
+  long long n;  // complex elements per component
+};
+
+enum { PW_PREDICT = 0, PW_CORRECT = 1, PW_NORM = 2, PW_DIVFREE = 3, PW_DIVFREE_NORM = 4 };
+
+template <int OP>
+__global__ void __launch_bounds__(256)
+    k_pointwise(PointArgs p, const double* __restrict__ kx_tab, const double* __restrict__ ky_tab,
+                const double* __restrict__ kz_tab) {
+  __shared__ double red[16];
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  double acc = 0.0;
+  if (i < p.n) {
+    const int kx = (int)(i % p.Nx1);
+    const long long t = i / p.Nx1;
+    const int kz = (int)(t % p.Nz1);
+    const int s = (int)(t / p.Nz1);
+    const double kxv = kx_tab[kx], kyv = ky_tab[s], kzv = kz_tab[kz];
+    const double km = kx == 0 ? 1.0 : 2.0;
+    const double lapl = -((kxv * kxv + kyv * kyv) + kzv * kzv);
+    const bool mean = (s == 0 && kz == 0 && kx == 0);
+    const cx<double>* A = reinterpret_cast<const cx<double>*>(p.a);
+    const cx<double>* B = reinterpret_cast<const cx<double>*>(p.b);
+    const cx<double>* C = reinterpret_cast<const cx<double>*>(p.c2);
+    cx<double>* O = reinterpret_cast<cx<double>*>(p.out);
+    cx<double>* O2 = reinterpret_cast<cx<double>*>(p.out2);
+    if (OP == PW_PREDICT || OP == PW_CORRECT) {
+      const double ldt1 = mean ? 0.0 : 1.0 / p.dt + (1.0 - p.c) * lapl / p.re;
+      const double ildt2 = mean ? 0.0 : 1.0 / (1.0 / p.dt - p.c * lapl / p.re);
+      for (int c = 0; c < 3; ++c) {
+        const long long idx = c * p.n + i;
+        if (OP == PW_PREDICT) {
+          const cx<double> u = A[idx], r = B[idx];
+          O[idx] = {(u.x * ldt1 + r.x) * ildt2, (u.y * ldt1 + r.y) * ildt2};
+        } else {
+          const cx<double> pr = A[idx], np = B[idx], nn = C[idx];
+          const cx<double> d = {p.c * (nn.x - np.x) * ildt2, p.c * (nn.y - np.y) * ildt2};
+          O[idx] = {pr.x + d.x, pr.y + d.y};
+          if (O2) O2[idx] = d;
+        }
+      }
+    } else if (OP == PW_NORM) {
+      for (int c = 0; c < 3; ++c) {
+        const cx<double> d = A[c * p.n + i];
+        acc += km * (d.x * d.x) + km * (d.y * d.y);
+      }
+    } else {
+      // u - grad(inv_lapl div u); mean mode zeroed (flows/monochromatic.py:516-560)
+      const double il = lapl < 0 ? 1.0 / lapl : 0.0;
+      const cx<double> u0 = A[i], u1 = A[p.n + i], u2 = A[2 * p.n + i];
+      const cx<double> dv = {((-kxv * u0.y) + (-kyv * u1.y)) + (-kzv * u2.y),
+                             ((kxv * u0.x) + (kyv * u1.x)) + (kzv * u2.x)};
+      const cx<double> pr = {il * dv.x, il * dv.y};
+      const double kk[3] = {kxv, kyv, kzv};
+      const cx<double> uu[3] = {u0, u1, u2};
+      for (int c = 0; c < 3; ++c) {
+        const cx<double> corr = {kk[c] * pr.y, -kk[c] * pr.x};  // -(i k p)
+        cx<double> v = {uu[c].x + corr.x, uu[c].y + corr.y};
+        if (OP == PW_DIVFREE_NORM) acc += km * (corr.x * corr.x) + km * (corr.y * corr.y);
+        if (mean) v = {0.0, 0.0};
+        O[c * p.n + i] = v;
+      }
+    }
+  }
+  if (OP == PW_NORM || OP == PW_DIVFREE_NORM) {
+    const double s = block_sum(acc, red);
+    if (threadIdx.x == 0) p.partials[blockIdx.x] = s;
+  }
+}
+
+__global__ void __launch_bounds__(64) k_zero_mean(double* __restrict__ u, long long n) {
+  if (threadIdx.x < 6) u[2 * (threadIdx.x / 2) * n + (threadIdx.x & 1)] = 0.0;
+}
+
+__global__ void __launch_bounds__(256) k_finalize_norm(const double* __restrict__ partials, int n,
+                                                       double* __restrict__ err) {
+  __shared__ double red[16];
+  double v = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) v += partials[i];
+  const double s = block_sum(v, red);
+  if (threadIdx.x == 0) *err = sqrt(s);
+}
+
+int finalize_norm(Ctx* c, int nblocks, double* err_dev, hipStream_t s) {
+  ProfScope ps(c, KID_FINALIZE, s);
+  hipLaunchKernelGGL(k_finalize_norm, dim3(1), dim3(256), 0, s, c->partials, nblocks, err_dev);
+  return check_launch(c, "finalize_norm");
+}
+
+// ------------------------------------------------------------------ launchers
+template <typename T>
+struct Bufs {
+  cx<T>* A;
+  cx<T>* B;
+  const cx<T>* twx;
+  const cx<T>* twy;
+  const cx<T>* twz;
+};
+
+template <typename T>
+static Bufs<T> bufs_of(Ctx* c) {
+  const int slot = std::is_same<T, float>::value ? 1 : 0;
+  void* const* tw = slot ? c->tw32 : c->tw64;
+  return {static_cast<cx<T>*>(c->bufA), static_cast<cx<T>*>(c->bufB),
+          static_cast<const cx<T>*>(tw[0]), static_cast<const cx<T>*>(tw[1]),
+          static_cast<const cx<T>*>(tw[2])};
+}
+
+constexpr int kTC = 16;  // columns per workgroup of the y/z passes
+constexpr int kRows = 16;  // rows per workgroup of the x pass
+
+template <typename T>
+static int launch_y_inverse(Ctx* c, const double* state, hipStream_t s) {
+  Bufs<T> b = bufs_of<T>(c);
+  const int ntile = (c->Nx1 + kTC - 1) / kTC;
+  const dim3 grid(c->Nz1 * ntile);
+  ProfScope ps(c, KID_Y_INVERSE, s);
+  switch (c->Py) {
+#define X(P)                                                                                 \
+  case P:                                                                                    \
+    hipLaunchKernelGGL((k_y_inverse<T, P, kTC>), grid, dim3(kTC * Plan<P>::N1), 0, s,        \
+                       reinterpret_cast<const cx<double>*>(state), b.A, b.twy, c->ky, c->ny, \
+                       c->Nz1, c->Nx1);                                                      \
+    break;
+    DNSMI_FOR_EACH_P(X)
+#undef X
+    default: return c->fail(DNSMI_EINVAL, "unsupported padded y length");
+  }
+  return check_launch(c, "y_inverse");
+}
+
+template <typename T>
+static int launch_z_inverse(Ctx* c, hipStream_t s) {
+  Bufs<T> b = bufs_of<T>(c);
+  const int ntile = (c->Nx1 + kTC - 1) / kTC;
+  const dim3 grid(c->Py * ntile);
+  ProfScope ps(c, KID_Z_INVERSE, s);
+  switch (c->Pz) {
+#define X(P)                                                                                 \
+  case P:                                                                                    \
+    hipLaunchKernelGGL((k_z_inverse<T, P, kTC>), grid, dim3(kTC * Plan<P>::N1), 0, s, b.A,   \
+                       b.B, b.twz, c->kz, c->nz, c->Py, c->Nx1);                             \
+    break;
+    DNSMI_FOR_EACH_P(X)
+#undef X
+    default: return c->fail(DNSMI_EINVAL, "unsupported padded z length");
+  }
+  return check_launch(c, "z_inverse");
+}
+
+template <typename T>
+static int launch_x_nonlinear(Ctx* c, hipStream_t s) {
+  Bufs<T> b = bufs_of<T>(c);
+  const int nrows = c->Py * c->Pz;
+  const dim3 grid((nrows + kRows - 1) / kRows);
+  ProfScope ps(c, KID_X_NONLINEAR, s);
+  switch (c->Px) {
+#define X(P)                                                                                  \
+  case P:                                                                                     \
+    hipLaunchKernelGGL((k_x_nonlinear<T, P, kRows>), grid, dim3(kRows * Plan<P>::N1), 0, s,   \
+                       b.B, b.A, b.twx, c->kx, c->base, c->cbase, c->nbase, c->nx, c->Py,     \
+                       c->Pz, c->Py);                                                         \
+    break;
+    DNSMI_FOR_EACH_P(X)
+#undef X
+    default: return c->fail(DNSMI_EINVAL, "unsupported padded x length");
+  }
+  return check_launch(c, "x_nonlinear");
+}
+
+template <typename T>
+static int launch_z_forward(Ctx* c, hipStream_t s) {
+  Bufs<T> b = bufs_of<T>(c);
+  const int ntile = (c->Nx1 + kTC - 1) / kTC;
+  const dim3 grid(c->Py * ntile);
+  ProfScope ps(c, KID_Z_FORWARD, s);
+  switch (c->Pz) {
+#define X(P)                                                                               \
+  case P:                                                                                  \
+    hipLaunchKernelGGL((k_z_forward<T, P, kTC>), grid, dim3(kTC * Plan<P>::N1), 0, s, b.A, \
+                       b.B, b.twz, c->nz, c->Py, c->Nx1);                                  \
+    break;
+    DNSMI_FOR_EACH_P(X)
+#undef X
+    default: return c->fail(DNSMI_EINVAL, "unsupported padded z length");
+  }
+  return check_launch(c, "z_forward");
+}
+
+template <typename T, int MODE>
+static int launch_y_forward(Ctx* c, const StepArgs& a, hipStream_t s, int* nblocks) {
+  Bufs<T> b = bufs_of<T>(c);
+  const int ntile = (c->Nx1 + kTC - 1) / kTC;
+  const int nb = c->Nz1 * ntile;
+  if (nblocks) *nblocks = nb;
+  ProfScope ps(c, KID_Y_FORWARD, s);
+  switch (c->Py) {
+#define X(P)                                                                                   \
+  case P:                                                                                      \
+    hipLaunchKernelGGL((k_y_forward<T, P, kTC, MODE>), dim3(nb), dim3(kTC * Plan<P>::N1), 0,   \
+                       s, b.B, b.twy, c->kx, c->ky, c->kz, c->ny, c->Nz1, c->Nx1, a);          \
+    break;
+    DNSMI_FOR_EACH_P(X)
+#undef X
+    default: return c->fail(DNSMI_EINVAL, "unsupported padded y length");
+  }
+  return check_launch(c, "y_forward");
+}
+
+template <typename T, int MODE>
+static int periodic_rhs_t(Ctx* c, const double* state, const StepArgs& a, hipStream_t s,
+                          int* nblocks) {
+  int rc;
+  if ((rc = launch_y_inverse<T>(c, state, s))) return rc;
+  if ((rc = launch_z_inverse<T>(c, s))) return rc;
+  if ((rc = launch_x_nonlinear<T>(c, s))) return rc;
+  if ((rc = launch_z_forward<T>(c, s))) return rc;
+  return launch_y_forward<T, MODE>(c, a, s, nblocks);
+}
+
+template <int MODE>
+static int periodic_rhs_mode(Ctx* c, const double* state, const StepArgs& a, hipStream_t s,
+                             int* nblocks) {
+  return c->fp32 ? periodic_rhs_t<float, MODE>(c, state, a, s, nblocks)
+                 : periodic_rhs_t<double, MODE>(c, state, a, s, nblocks);
+}
+
+static StepArgs step_args(Ctx* c) {
+  StepArgs a{};
+  a.dt = c->cfg.dt;
+  a.c = c->cfg.implicitness;
+  a.re = c->cfg.re;
+  a.partials = c->partials;
+  return a;
+}
+
+int periodic_get_rhs(Ctx* c, const double* state, double* rhs, hipStream_t s) {
+  StepArgs a = step_args(c);
+  a.rhs_out = rhs;
+  return periodic_rhs_mode<MODE_RHS>(c, state, a, s, nullptr);
+}
+
+int periodic_predict_and_correct(Ctx* c, const double* state, double* pred, double* rhs_next,
+                                 double* err, hipStream_t s) {
+  StepArgs a = step_args(c);
+  a.state = state;
+  a.pred = pred;
+  a.rhs_out = c->rhs0;
+  int rc = periodic_rhs_mode<MODE_PREDICT>(c, state, a, s, nullptr);
+  if (rc) return rc;
+  StepArgs b = step_args(c);
+  b.rhs_prev = c->rhs0;
+  b.pred = pred;
+  b.rhs_out = rhs_next;
+  int nb = 0;
+  rc = periodic_rhs_mode<MODE_CORRECT>(c, pred, b, s, &nb);
+  if (rc) return rc;
+  return finalize_norm(c, nb, err, s);
+}
+
+int periodic_iterate_correction(Ctx* c, double* pred, const double* rhs_prev, double* rhs_next,
+                                double* err, hipStream_t s) {
+  StepArgs b = step_args(c);
+  b.rhs_prev = rhs_prev;
+  b.pred = pred;
+  b.rhs_out = rhs_next;
+  int nb = 0;
+  int rc = periodic_rhs_mode<MODE_CORRECT>(c, pred, b, s, &nb);
+  if (rc) return rc;
+  return finalize_norm(c, nb, err, s);
+}
+
+static PointArgs point_args(Ctx* c) {
+  PointArgs p{};
+  p.dt = c->cfg.dt;
+  p.c = c->cfg.implicitness;
+  p.re = c->cfg.re;
+  p.ny = c->ny;
+  p.Nz1 = c->Nz1;
+  p.Nx1 = c->Nx1;
+  p.n = (long long)c->Ny1 * c->Nz1 * c->Nx1;
+  p.partials = c->partials;
+  return p;
+}
+
+static int pw_blocks(const PointArgs& p) { return (int)((p.n + 255) / 256); }
+
+int periodic_predict(Ctx* c, const double* state, const double* rhs, double* pred,
+                     hipStream_t s) {
+  PointArgs p = point_args(c);
+  p.a = state;
+  p.b = rhs;
+  p.out = pred;
+  ProfScope ps(c, KID_POINTWISE, s);
+  hipLaunchKernelGGL(k_pointwise<PW_PREDICT>, dim3(pw_blocks(p)), dim3(256), 0, s, p, c->kx,
+                     c->ky, c->kz);
+  return check_launch(c, "predict");
+}
+
+int periodic_correct(Ctx* c, const double* pred_in, const double* rhs_prev,
+                     const double* rhs_next, double* pred_out, double* corr, hipStream_t s) {
+  PointArgs p = point_args(c);
+  p.a = pred_in;
+  p.b = rhs_prev;
+  p.c2 = rhs_next;
+  p.out = pred_out;
+  p.out2 = corr;
+  ProfScope ps(c, KID_POINTWISE, s);
+  hipLaunchKernelGGL(k_pointwise<PW_CORRECT>, dim3(pw_blocks(p)), dim3(256), 0, s, p, c->kx,
+                     c->ky, c->kz);
+  return check_launch(c, "correct");
+}
+
+int periodic_norm(Ctx* c, const double* corr, double* err, hipStream_t s) {
+  PointArgs p = point_args(c);
+  p.a = corr;
+  const int nb = pw_blocks(p);
+  if (nb > c->max_partials) return c->fail(DNSMI_EINVAL, "norm: too many blocks");
+  ProfScope ps(c, KID_POINTWISE, s);
+  hipLaunchKernelGGL(k_pointwise<PW_NORM>, dim3(nb), dim3(256), 0, s, p, c->kx, c->ky, c->kz);
+  int rc = check_launch(c, "norm");
+  if (rc) return rc;
+  return finalize_norm(c, nb, err, s);
+}
+
+int periodic_correct_velocity(Ctx* c, double* state, double* err, hipStream_t s) {
+  PointArgs p = point_args(c);
+  p.a = state;
+  p.out = state;
+  const int nb = pw_blocks(p);
+  if (!c->cfg.correct_divergence) {
+    // only the mean mode is zeroed (a kernel, not a memset, so that the
+    // call can sit inside a captured graph)
+    if (err) return c->fail(DNSMI_EINVAL, "measure_corrections needs correct_divergence");
+    ProfScope ps(c, KID_POINTWISE, s);
+    hipLaunchKernelGGL(k_zero_mean, dim3(1), dim3(64), 0, s, state, p.n);
+    return check_launch(c, "zero_mean");
+  }
+  if (err) {
+    if (nb > c->max_partials) return c->fail(DNSMI_EINVAL, "too many blocks");
+    ProfScope ps(c, KID_POINTWISE, s);
+  hipLaunchKernelGGL(k_pointwise<PW_DIVFREE_NORM>, dim3(nb), dim3(256), 0, s, p, c->kx, c->ky,
+                       c->kz);
+    int rc = check_launch(c, "correct_velocity");
+    if (rc) return rc;
+    return finalize_norm(c, nb, err, s);
+  }
+  ProfScope ps(c, KID_POINTWISE, s);
+  hipLaunchKernelGGL(k_pointwise<PW_DIVFREE>, dim3(nb), dim3(256), 0, s, p, c->kx, c->ky, c->kz);
+  return check_launch(c, "correct_velocity");
+}
+
+// ------------------------------------------------------------------ set-up
+static bool supported_length(int P) {
+  switch (P) {
+#define X(P_) case P_:
+    DNSMI_FOR_EACH_P(X)
+#undef X
+    return true;
+    default: return false;
+  }
+}
+
+template <typename T>
+static std::vector<cx<T>> host_twiddles(int P) {
+  std::vector<cx<T>> t(P);
+  for (int m = 0; m < P; ++m) {
+    const CosSin v = ct_unit(m, P);
+    t[m] = {static_cast<T>(v.c), static_cast<T>(v.s)};
+  }
+  return t;
+}
+
+static int upload(Ctx* c, void** dst, const void* src, size_t bytes) {
+  *dst = c->alloc(bytes);
+  if (!*dst) return c->fail(DNSMI_ENOMEM, "device allocation failed");
+  hipError_t e = hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return c->fail(DNSMI_EHIP, hipGetErrorString(e));
+  return DNSMI_OK;
+}
+
+int setup_twiddles(Ctx* c) {
+  const int Ps[3] = {c->Px, c->Py, c->Pz};
+  for (int a = 0; a < 3; ++a) {
+    if (!Ps[a]) continue;
+    auto t64 = host_twiddles<double>(Ps[a]);
+    auto t32 = host_twiddles<float>(Ps[a]);
+    int rc = upload(c, &c->tw64[a], t64.data(), t64.size() * sizeof(t64[0]));
+    if (rc) return rc;
+    rc = upload(c, &c->tw32[a], t32.data(), t32.size() * sizeof(t32[0]));
+    if (rc) return rc;
+  }
+  return DNSMI_OK;
+}
+
+int upload_doubles(Ctx* c, double** dst, const std::vector<double>& v) {
+  return upload(c, reinterpret_cast<void**>(dst), v.data(), v.size() * sizeof(double));
+}
+
+// Harmonics (operators.py:18-56) times 2 pi / L, in the reference's
+// operation order ((h * 2) * pi / L).
+std::vector<double> real_wavenumbers(int n, double L) {
+  std::vector<double> k(n / 2);
+  for (int i = 0; i < n / 2; ++i) k[i] = (double)(i * 2) * kPi / L;
+  return k;
+}
+std::vector<double> complex_wavenumbers(int n, double L) {
+  std::vector<double> k(n - 1);
+  for (int i = 0; i < n - 1; ++i) {
+    const int h = i < n / 2 ? i : i - (n - 1);
+    k[i] = (double)(h * 2) * kPi / L;
+  }
+  return k;
+}
+
+int periodic_setup(Ctx* c) {
+  const dnsmi_config& g = c->cfg;
+  if (g.nx % 2 || g.ny % 2 || g.nz % 2 || g.nx < 4 || g.ny < 4 || g.nz < 4)
+    return c->fail(DNSMI_EINVAL, "nx, ny, nz must be even and >= 4");
+  if (g.oversampling_factor != 3) return c->fail(DNSMI_EINVAL, "oversampling_factor must be 3");
+  c->nx = g.nx;
+  c->ny = g.ny;
+  c->nz = g.nz;
+  c->Nx1 = g.nx / 2;
+  c->Ny1 = g.ny - 1;
+  c->Nz1 = g.nz - 1;
+  c->Px = 3 * g.nx / 2;
+  c->Pz = 3 * g.nz / 2;
+  c->Py = g.oversample_y ? 3 * g.ny / 2 : g.ny;
+  if (!supported_length(c->Px) || !supported_length(c->Py) || !supported_length(c->Pz))
+    return c->fail(DNSMI_EINVAL, "padded lengths must be one of 12,16,24,32,48,64,72,96,128,144,192");
+  c->spec_elems = 3LL * c->Ny1 * c->Nz1 * c->Nx1;
+  const size_t cxs = c->fp32 ? 8 : 16;
+  const size_t w1 = 5ull * c->Py * c->Nz1 * c->Nx1, w3 = 3ull * c->Py * c->Pz * c->Nx1;
+  const size_t w2 = 6ull * c->Py * c->Pz * c->Nx1, w4 = 3ull * c->Py * c->Nz1 * c->Nx1;
+  c->bufA_bytes = (w1 > w3 ? w1 : w3) * cxs;
+  c->bufB_bytes = (w2 > w4 ? w2 : w4) * cxs;
+  c->bufA = c->alloc(c->bufA_bytes);
+  c->bufB = c->alloc(c->bufB_bytes);
+  c->rhs0 = static_cast<double*>(c->alloc(c->spec_elems * 16));
+  const long long per_comp = (long long)c->Ny1 * c->Nz1 * c->Nx1;
+  c->max_partials = (int)((per_comp + 255) / 256) + c->Nz1 * ((c->Nx1 + kTC - 1) / kTC) + 1024;
+  c->partials = static_cast<double*>(c->alloc(sizeof(double) * c->max_partials));
+  if (!c->bufA || !c->bufB || !c->rhs0 || !c->partials)
+    return c->fail(DNSMI_ENOMEM, "device allocation failed");
+  int rc = setup_twiddles(c);
+  if (rc) return rc;
+  if ((rc = upload_doubles(c, &c->kx, real_wavenumbers(g.nx, g.lx)))) return rc;
+  if ((rc = upload_doubles(c, &c->ky, complex_wavenumbers(g.ny, g.ly)))) return rc;
+  if ((rc = upload_doubles(c, &c->kz, complex_wavenumbers(g.nz, g.lz)))) return rc;
+
+  // Base flow on the padded y grid (flows/monochromatic.py:116-208):
+  // U_x = sin(2 pi y / Ly) (Kolmogorov) or cos (Waleffe), evaluated as the
+  // inverse real transform of the single harmonic q_f = 1.
+  const int Py = c->Py;
+  std::vector<double> base(3 * Py, 0.0), dbase(3 * Py, 0.0), cbase(3 * Py, 0.0),
+      nbase(3 * Py, 0.0);
+  if (g.system == DNSMI_KOLMOGOROV || g.system == DNSMI_WALEFFE) {
+    const double kf = 2 * kPi / g.ly;
+    for (int y = 0; y < Py; ++y) {
+      const CosSin t = ct_unit(y, Py);
+      // irfft of coefficient -0.5j (sin) or 0.5 (cos) at q_f = 1, times 2
+      base[y] = g.system == DNSMI_KOLMOGOROV ? t.s : t.c;
+      dbase[y] = g.system == DNSMI_KOLMOGOROV ? kf * t.c : -kf * t.s;
+    }
+  } else if (g.system != DNSMI_DECAYING_BOX) {
+    return c->fail(DNSMI_EINVAL, "not a periodic system");
+  }
+  for (int y = 0; y < Py; ++y) {
+    cbase[2 * Py + y] = -dbase[y];
+    nbase[1 * Py + y] = base[y] * dbase[y];
+  }
+  const double tilt = kPi * g.tilt_degree / 180.0;
+  if (tilt != 0.0) {
+    const double sn = std::sin(tilt), cs = std::cos(tilt);
+    for (int y = 0; y < Py; ++y) {
+      base[2 * Py + y] = sn * base[y];
+      base[y] *= cs;
+      cbase[0 * Py + y] = -sn * cbase[2 * Py + y];
+      cbase[2 * Py + y] *= cs;
+    }
+  }
+  if ((rc = upload_doubles(c, &c->base, base))) return rc;
+  if ((rc = upload_doubles(c, &c->cbase, cbase))) return rc;
+  if ((rc = upload_doubles(c, &c->nbase, nbase))) return rc;
+  return DNSMI_OK;
+}
+
+}  // namespace dnsmi

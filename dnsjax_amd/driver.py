@@ -1,0 +1,58 @@
+"""The time loop around the hot path, with the reference driver's semantics.
+
+``run`` follows ``__main__.py:125-203``: per step one
+``predict_and_correct``, then ``iterate_correction`` while
+``error > corrector_tolerance`` and fewer than ``max_corrector_iterations``
+extra iterations ran, then ``correct_velocity``; the loop stops after a
+step whose corrector did not converge.  A NaN error is reported (the
+reference stops silently on it).  The error is read on the host once per
+corrector pass, as in the reference.
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+
+@dataclass
+class RunResult:
+    state: object
+    counts: list = field(default_factory=list)
+    errors: list = field(default_factory=list)
+    converged: bool = True
+    nan: bool = False
+
+    @property
+    def rhs_evals(self) -> int:  # __main__.py:199-201
+        return sum(c + 2 for c in self.counts)
+
+
+def run(flow, state, n_steps: int, callback=None) -> RunResult:
+    p = flow.params
+    tol, max_it = p.corrector_tolerance, p.max_corrector_iterations
+    res = RunResult(state=state)
+    last_error = 0.0
+    for it in range(n_steps):
+        if not last_error < tol:
+            break
+        state_prev = state
+        state, rhs, error = flow.predict_and_correct(state_prev)
+        err = float(error)
+        c = 0
+        while err > tol and c < max_it:
+            state, rhs, error = flow.iterate_correction(state_prev, state, rhs)
+            err = float(error)
+            c += 1
+        state, _ = flow.correct_velocity(state)
+        res.counts.append(c)
+        res.errors.append(err)
+        last_error = err
+        if callback is not None:
+            callback(it + 1, state)
+        if math.isnan(err):
+            res.nan = True
+            break
+    res.state = state
+    res.converged = not (last_error > tol) and not res.nan
+    return res

@@ -1,0 +1,136 @@
+"""A device context of libdnsmi for one configuration, over torch tensors.
+
+Tensors are the device buffers (complex128, contiguous, on the current HIP
+device); every call is enqueued on torch's current stream.  Nothing here
+computes on the CPU.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import _lib
+from .params import Params
+
+
+def _ptr(t: torch.Tensor | None):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class NativeContext:
+    """Owns one ``dnsmi_ctx`` (set-up tables, FFT work buffers)."""
+
+    def __init__(self, params: Params):
+        self.lib = _lib.load()
+        if not torch.cuda.is_available():
+            raise RuntimeError("dnsjax_amd needs a HIP device (torch.cuda.is_available() is False)")
+        self.params = p = params
+        cfg = _lib.DnsmiConfig(
+            system=_lib.SYSTEMS[p.system], nx=p.nx, ny=p.ny, nz=p.nz,
+            oversampling_factor=p.oversampling_factor, oversample_y=int(p.oversample_y),
+            lx=p.lx, ly=p.ly, lz=p.lz, re=p.re, dt=p.dt, implicitness=p.implicitness,
+            tilt_degree=p.tilt_degree, fd_order=p.fd_order,
+            precision=_lib.DNSMI_MIXED if p.precision == "mixed" else _lib.DNSMI_FP64,
+            correct_divergence=int(p.correct_divergence), tile_cols=p.tile_cols,
+        )
+        handle = ctypes.c_void_p()
+        torch.cuda.current_device()  # make the device current before any HIP call
+        _lib.check(self.lib.dnsmi_create(ctypes.byref(cfg), ctypes.byref(handle)), None)
+        self.handle = handle
+        self.shape = (3,) + p.spec_shape
+        self.device = torch.device("cuda", torch.cuda.current_device())
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            torch.cuda.synchronize()
+            self.lib.dnsmi_destroy(h)
+            self.handle = None
+
+    # ---------------------------------------------------------------- helpers
+    def _chk(self, rc: int) -> None:
+        _lib.check(rc, self.handle)
+
+    def check_field(self, t: torch.Tensor, name: str) -> None:
+        if t.dtype != torch.complex128 or tuple(t.shape) != self.shape or not t.is_contiguous():
+            raise ValueError(f"{name}: need contiguous complex128 {self.shape}, got {t.dtype} {tuple(t.shape)}")
+        if t.device != self.device:
+            raise ValueError(f"{name}: tensor on {t.device}, context on {self.device}")
+
+    def empty(self) -> torch.Tensor:
+        return torch.empty(self.shape, dtype=torch.complex128, device=self.device)
+
+    def scalar(self) -> torch.Tensor:
+        return torch.empty((), dtype=torch.float64, device=self.device)
+
+    # ---------------------------------------------------------------- calls
+    def get_rhs(self, state, out=None):
+        self.check_field(state, "state")
+        out = self.empty() if out is None else out
+        self._chk(self.lib.dnsmi_get_rhs(self.handle, _ptr(state), _ptr(out), _stream()))
+        return out
+
+    def predict(self, state, rhs, out=None):
+        self.check_field(state, "state")
+        self.check_field(rhs, "rhs")
+        out = self.empty() if out is None else out
+        self._chk(self.lib.dnsmi_predict(self.handle, _ptr(state), _ptr(rhs), _ptr(out), _stream()))
+        return out
+
+    def correct(self, state_prev, pred, rhs_prev, rhs_next, out=None, correction=True):
+        for t, n in ((state_prev, "state_prev"), (pred, "prediction"), (rhs_prev, "rhs_prev"), (rhs_next, "rhs_next")):
+            self.check_field(t, n)
+        out = self.empty() if out is None else out
+        corr = self.empty() if correction else None
+        self._chk(self.lib.dnsmi_correct(self.handle, _ptr(state_prev), _ptr(pred), _ptr(rhs_prev),
+                                         _ptr(rhs_next), _ptr(out), _ptr(corr), _stream()))
+        return out, corr
+
+    def norm(self, correction, out=None):
+        self.check_field(correction, "correction")
+        out = self.scalar() if out is None else out
+        self._chk(self.lib.dnsmi_norm(self.handle, _ptr(correction), _ptr(out), _stream()))
+        return out
+
+    def predict_and_correct(self, state, pred=None, rhs_next=None, err=None):
+        self.check_field(state, "state")
+        pred = self.empty() if pred is None else pred
+        rhs_next = self.empty() if rhs_next is None else rhs_next
+        err = self.scalar() if err is None else err
+        self._chk(self.lib.dnsmi_predict_and_correct(self.handle, _ptr(state), _ptr(pred),
+                                                     _ptr(rhs_next), _ptr(err), _stream()))
+        return pred, rhs_next, err
+
+    def iterate_correction(self, state_prev, pred, rhs_prev, rhs_next=None, err=None):
+        self.check_field(state_prev, "state_prev")
+        self.check_field(pred, "prediction")
+        self.check_field(rhs_prev, "rhs_prev")
+        rhs_next = self.empty() if rhs_next is None else rhs_next
+        err = self.scalar() if err is None else err
+        self._chk(self.lib.dnsmi_iterate_correction(self.handle, _ptr(state_prev), _ptr(pred),
+                                                    _ptr(rhs_prev), _ptr(rhs_next), _ptr(err), _stream()))
+        return pred, rhs_next, err
+
+    # ---------------------------------------------------------------- timing
+    def profile_begin(self) -> None:
+        self._chk(self.lib.dnsmi_profile_begin(self.handle))
+
+    def profile_end(self) -> dict:
+        """{kernel name: (total ms, launches)} of the launches since begin."""
+        n = _lib.KERNEL_IDS
+        ms = (ctypes.c_double * n)()
+        cnt = (ctypes.c_longlong * n)()
+        self._chk(self.lib.dnsmi_profile_end(self.handle, n, ms, cnt))
+        return {self.lib.dnsmi_kernel_name(i).decode(): (ms[i], cnt[i]) for i in range(n) if cnt[i]}
+
+    def correct_velocity(self, state, measure=False):
+        self.check_field(state, "state")
+        err = self.scalar() if measure else None
+        self._chk(self.lib.dnsmi_correct_velocity(self.handle, _ptr(state), _ptr(err), _stream()))
+        return state, err

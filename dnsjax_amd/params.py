@@ -1,0 +1,78 @@
+"""Run parameters: the subset of the reference's ``parameters.py`` the
+time-step path reads, as one flat dataclass.
+
+Field meanings follow ``parameters.py:26-133`` (sections phys, geo, res,
+step, debug).  ``ly`` is a parameter here; the reference fixes it to 4 for
+periodic systems and 2 for walls (``parameters.py:186-193``), which is
+what ``None`` selects.  ``precision`` selects the plain FP64 path or the
+mixed form (FP32 transforms and nonlinear product).
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, replace
+
+PERIODIC_SYSTEMS = ("decaying-box", "kolmogorov", "waleffe")  # parameters.py:19-20
+WALLED_SYSTEMS = ("plane-couette",)  # parameters.py:22-23
+
+
+@dataclass
+class Params:
+    system: str = "kolmogorov"
+    nx: int = 128
+    ny: int = 128
+    nz: int = 128
+    lx: float = 4.0
+    ly: float | None = None
+    lz: float = 4.0
+    re: float = 1000.0
+    oversampling_factor: int = 3
+    oversample_y: bool = True
+    tilt_degree: float = 0.0
+    fd_order: int = 4
+    dt: float = 0.01
+    implicitness: float = 0.5
+    corrector_tolerance: float = 1e-5
+    max_corrector_iterations: int = 10
+    correct_divergence: bool = True
+    measure_corrections: bool = False
+    precision: str = "fp64"
+    tile_cols: int = 0
+
+    def __post_init__(self) -> None:
+        if self.system not in PERIODIC_SYSTEMS + WALLED_SYSTEMS:
+            raise ValueError(f"unknown system {self.system!r}")
+        if self.ly is None:
+            self.ly = 4.0 if self.system in PERIODIC_SYSTEMS else 2.0
+        if self.precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        if not self.re > 0 or not self.dt > 0 or not 0 <= self.implicitness <= 1:
+            raise ValueError("need re > 0, dt > 0, 0 <= implicitness <= 1")
+
+    @property
+    def walls(self) -> bool:
+        return self.system in WALLED_SYSTEMS
+
+    @property
+    def spec_shape(self) -> tuple:
+        """``sharding.py:395-434``."""
+        if self.walls:
+            return (self.nz - 1, self.nx // 2, self.ny)
+        return (self.ny - 1, self.nz - 1, self.nx // 2)
+
+    @property
+    def padded(self) -> tuple:
+        """(Py, Pz, Px), ``parameters.py:233-254``."""
+        f = self.oversampling_factor
+        py = (f * self.ny // 2 if self.oversample_y else self.ny) if not self.walls else self.ny
+        return (py, f * self.nz // 2, f * self.nx // 2)
+
+    def with_(self, **kw) -> "Params":
+        return replace(self, **kw)
+
+
+def kolmogorov_2pi(n: int, re: float, dt: float, **kw) -> Params:
+    """A cubic 2 pi box Kolmogorov run (the benchmark boxes)."""
+    L = 2 * math.pi
+    return Params(system="kolmogorov", nx=n, ny=n, nz=n, lx=L, ly=L, lz=L, re=re, dt=dt, **kw)

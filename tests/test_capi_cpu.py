@@ -1,0 +1,123 @@
+"""CPU-only checks of the boundary: the C-ABI library loads and exports every
+symbol include/dnsmi.h declares; host-side logic (parameters, stepper
+composition, driver loop semantics) without touching a GPU."""
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from dnsjax_amd import _lib
+from dnsjax_amd.driver import run
+from dnsjax_amd.params import Params
+from dnsjax_amd.timestep import make_stepper
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _header_functions():
+    txt = open(os.path.join(ROOT, "include", "dnsmi.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(dnsmi_\w+)\s*\(", txt)))
+
+
+def test_header_declares_the_boundary():
+    fns = _header_functions()
+    for name in ("dnsmi_create", "dnsmi_destroy", "dnsmi_get_rhs", "dnsmi_predict", "dnsmi_correct",
+                 "dnsmi_norm", "dnsmi_predict_and_correct", "dnsmi_iterate_correction",
+                 "dnsmi_correct_velocity", "dnsmi_last_error"):
+        assert name in fns
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libdnsmi.so not built (run __graft_entry__.build())")
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    missing = [f for f in _header_functions() if not hasattr(lib, f)]
+    assert not missing, missing
+    # the ctypes table covers the same surface
+    assert set(_header_functions()) <= set(_lib.SIGNATURES)
+
+
+def test_create_rejects_null_and_reports():
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libdnsmi.so not built")
+    lib = _lib.load()
+    rc = lib.dnsmi_create(None, None)
+    assert rc == 1
+    assert b"null" in lib.dnsmi_last_error(None)
+
+
+def test_params_defaults_follow_reference():
+    p = Params()
+    assert p.ly == 4.0 and p.spec_shape == (127, 127, 64) and p.padded == (192, 192, 192)
+    w = Params(system="plane-couette", nx=64, ny=128, nz=64)
+    assert w.ly == 2.0 and w.spec_shape == (63, 32, 128) and w.padded == (128, 96, 96)
+    with pytest.raises(ValueError):
+        Params(system="pipe")
+    with pytest.raises(ValueError):
+        Params(precision="fp16")
+
+
+def test_generic_stepper_composition_matches_reference_order():
+    calls = []
+
+    def get_rhs(s):
+        calls.append("rhs")
+        return s * 2.0
+
+    def predict(s, r):
+        calls.append("predict")
+        return s + r
+
+    def correct(sp, p, rp, rn):
+        calls.append("correct")
+        return p + 0.5 * (rn - rp), 0.5 * (rn - rp)
+
+    def norm(c):
+        calls.append("norm")
+        return float(np.abs(c).sum())
+
+    pc, ic = make_stepper(get_rhs, predict, correct, norm)
+    assert pc.fused is False
+    s = np.ones(3)
+    pred, rhs, err = pc(s)
+    assert calls == ["rhs", "predict", "rhs", "correct", "norm"]
+    assert np.allclose(pred, 3 + 0.5 * (6 - 2)) and np.allclose(rhs, 6) and err == 6.0
+    calls.clear()
+    pred2, rhs2, err2 = ic(s, pred, rhs)
+    assert calls == ["rhs", "correct", "norm"]
+
+
+class _ScriptedFlow:
+    """Errors follow a script, so the loop's count semantics can be checked."""
+
+    def __init__(self, errs, tol=1e-3, max_it=3):
+        self.params = Params(corrector_tolerance=tol, max_corrector_iterations=max_it)
+        self.errs = list(errs)
+
+    def predict_and_correct(self, s):
+        return s, None, self.errs.pop(0)
+
+    def iterate_correction(self, sp, s, r):
+        return s, None, self.errs.pop(0)
+
+    def correct_velocity(self, s):
+        return s, None
+
+
+def test_driver_count_semantics():
+    # step 1: converges after 2 extra; step 2: converges at once; step 3 hits max -> stop
+    f = _ScriptedFlow([1.0, 0.5, 1e-4, 1e-5, 1.0, 1.0, 1.0, 1.0, 1e-9], tol=1e-3, max_it=3)
+    r = run(f, 0.0, 10)
+    assert r.counts == [2, 0, 3]
+    assert not r.converged
+    assert f.errs == [1e-9]  # nothing ran after the non-converged step
+
+
+def test_driver_detects_nan():
+    f = _ScriptedFlow([float("nan")], tol=1e-3, max_it=3)
+    r = run(f, 0.0, 5)
+    assert r.nan and not r.converged and r.counts == [0]

@@ -1,0 +1,159 @@
+"""GPU parity of the periodic path against the FP64 oracle (through the C ABI).
+
+Bar: relative L2 <= 1e-12 for FP64 fields, corrector errors to 1e-9
+relative, identical corrector iteration counts over a run; the mixed form
+to the tolerances stated in each test.
+"""
+
+import math
+import os
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+from dnsjax_amd.params import Params  # noqa: E402
+from oracle import periodic as P  # noqa: E402
+from oracle import stepper as S  # noqa: E402
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, "golden")
+
+CASES = {
+    "periodic_16_kolmogorov": dict(nx=16, ny=16, nz=16, L=2 * math.pi, system="kolmogorov", re=40.0, dt=0.01, tilt=0.0),
+    "periodic_16x8x32_waleffe_ly4": dict(nx=16, ny=8, nz=32, L=4.0, system="waleffe", re=60.0, dt=0.02, tilt=0.0),
+    "periodic_8_kolmogorov_tilt30": dict(nx=8, ny=8, nz=8, L=4.0, system="kolmogorov", re=30.0, dt=0.02, tilt=30.0),
+}
+
+
+def rel(a, b):
+    a = a.detach().cpu().numpy() if hasattr(a, "detach") else np.asarray(a)
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def make(case, **kw):
+    from dnsjax_amd.flows import TriplyPeriodicFlow
+
+    c = CASES[case]
+    p = Params(system=c["system"], nx=c["nx"], ny=c["ny"], nz=c["nz"], lx=c["L"], ly=c["L"], lz=c["L"],
+               re=c["re"], dt=c["dt"], tilt_degree=c["tilt"], **kw)
+    return TriplyPeriodicFlow(p)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_golden_rhs_step_and_projection(case):
+    g = np.load(os.path.join(GOLDEN, case + ".npz"))
+    flow = make(case)
+    u = flow.to_device(g["state"])
+    rhs = flow.get_rhs(u)
+    assert rel(rhs, g["rhs0"]) < 1e-12
+    pred, rhs1, err1 = flow.predict_and_correct(u)
+    assert rel(pred, g["pred1"]) < 1e-12
+    assert rel(rhs1, g["rhs1"]) < 1e-12
+    assert abs(float(err1) - float(g["err1"])) <= 1e-9 * float(g["err1"])
+    assert rel(u, g["state"]) == 0.0  # predict_and_correct leaves state untouched
+    pred2, rhs2, err2 = flow.iterate_correction(u, pred, rhs1)
+    assert rel(pred2, g["pred2"]) < 1e-12
+    assert rel(rhs2, g["rhs2"]) < 1e-12
+    assert abs(float(err2) - float(g["err2"])) <= 1e-9 * float(g["err2"])
+    cv, _ = flow.correct_velocity(pred2.clone())
+    assert rel(cv, g["corrected"]) < 1e-12
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_generic_composition_matches_fused(case):
+    from dnsjax_amd.timestep import make_stepper
+
+    g = np.load(os.path.join(GOLDEN, case + ".npz"))
+    flow = make(case)
+    u = flow.to_device(g["state"])
+    pc, ic = make_stepper(lambda s: flow.get_rhs(s), lambda s, r: flow.predict(s, r),
+                          lambda a, b, c, d: flow.correct(a, b, c, d), lambda c: flow.norm(c))
+    assert pc.fused is False and flow.predict_and_correct.fused is True
+    pred, rhs1, err1 = pc(u)
+    assert rel(pred, g["pred1"]) < 1e-12 and rel(rhs1, g["rhs1"]) < 1e-12
+    assert abs(float(err1) - float(g["err1"])) <= 1e-9 * float(g["err1"])
+    pred2, rhs2, err2 = ic(u, pred, rhs1)
+    assert rel(pred2, g["pred2"]) < 1e-12
+    assert abs(float(err2) - float(g["err2"])) <= 1e-9 * float(g["err2"])
+
+
+def test_norm_and_measure_corrections():
+    g = np.load(os.path.join(GOLDEN, "periodic_16_kolmogorov.npz"))
+    flow = make("periodic_16_kolmogorov", measure_corrections=True)
+    o = P.Flow(P.PeriodicParams(nx=16, ny=16, nz=16, lx=2 * math.pi, ly=2 * math.pi, lz=2 * math.pi, re=40.0, dt=0.01))
+    a = g["rhs1"]
+    n = flow.norm(flow.to_device(a))
+    assert abs(float(n) - P.get_norm(a, o.fourier.k_metric)) < 1e-13 * P.get_norm(a, o.fourier.k_metric)
+    # a gradient part for the projection to remove: u + grad(phi)
+    rng = np.random.default_rng(9)
+    phi = rng.standard_normal(g["state"].shape[1:]) + 1j * rng.standard_normal(g["state"].shape[1:])
+    f = o.fourier
+    noisy = P.correct_velocity(g["state"], o) + 0.01 * P.gradient(phi, f.kx, f.ky, f.kz)
+    noisy = P.phys_to_spec(P.spec_to_phys(noisy, o.p), o.p)  # Hermitian at kx = 0
+    st, meas = flow.correct_velocity(flow.to_device(noisy))
+    ref = P.correct_velocity(noisy, o)
+    assert rel(st, ref) < 1e-12
+    corr = ref - noisy
+    corr[:, 0, 0, 0] = 0
+    assert abs(float(meas["div"]) - P.get_norm(corr, o.fourier.k_metric)) < 1e-10 * P.get_norm(corr, o.fourier.k_metric)
+
+
+def test_run_32cubed_counts_and_per_step_error():
+    """Config 0 shape (32^3, Re 40, dt 0.01, tol 1e-10), 30 steps vs the oracle."""
+    from dnsjax_amd.driver import run as gpu_run
+    from dnsjax_amd.flows import TriplyPeriodicFlow
+    from dnsjax_amd.fields import periodic_field
+
+    L = 2 * math.pi
+    p = Params(system="kolmogorov", nx=32, ny=32, nz=32, lx=L, ly=L, lz=L, re=40.0, dt=0.01,
+               corrector_tolerance=1e-10, max_corrector_iterations=10)
+    flow = TriplyPeriodicFlow(p)
+    u0 = periodic_field(32, 32, 32, L, L, L, 1e-2, seed=0)
+    op = P.PeriodicParams(nx=32, ny=32, nz=32, lx=L, ly=L, lz=L, re=40.0, dt=0.01)
+    of = P.Flow(op)
+    pc, ic = S.make_stepper(P.get_rhs, P.predict, P.correct, P.norm)
+    ref_states = []
+    _, ref_counts, ref_errs = S.run(u0, 30, 1e-10, 10, lambda s: pc(s, of), lambda a, b, c: ic(a, b, c, of),
+                                    lambda s: P.correct_velocity(s, of), callback=lambda i, s: ref_states.append(s))
+    gpu_states = []
+    r = gpu_run(flow, flow.to_device(u0), 30, callback=lambda i, s: gpu_states.append(s.cpu().numpy()))
+    assert r.counts == ref_counts
+    worst = max(rel(a, b) for a, b in zip(gpu_states, ref_states))
+    assert worst < 1e-12, worst
+    for a, b in zip(r.errors, ref_errs):
+        assert abs(a - b) <= 1e-6 * b + 1e-300
+
+
+def test_mixed_precision_close_to_fp64():
+    g = np.load(os.path.join(GOLDEN, "periodic_16_kolmogorov.npz"))
+    flow = make("periodic_16_kolmogorov", precision="mixed")
+    u = flow.to_device(g["state"])
+    rhs = flow.get_rhs(u)
+    assert rel(rhs, g["rhs0"]) < 1e-5
+    pred, rhs1, err1 = flow.predict_and_correct(u)
+    assert rel(pred, g["pred1"]) < 1e-6
+
+
+def test_rejects_bad_configs_and_aliasing():
+    from dnsjax_amd._lib import DnsmiError
+    from dnsjax_amd.flows import TriplyPeriodicFlow
+
+    with pytest.raises(DnsmiError):
+        TriplyPeriodicFlow(Params(nx=10, ny=16, nz=16))  # 15 is not a supported padded length
+    flow = make("periodic_8_kolmogorov_tilt30")
+    u = flow.init_state(seed=1, rms=0.1)
+    with pytest.raises(DnsmiError):
+        flow._native.predict_and_correct(u, pred=u)
+    with pytest.raises(ValueError):
+        flow.get_rhs(u.to(torch.complex64))
