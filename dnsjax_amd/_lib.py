@@ -11,7 +11,7 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libdnsmi.so")
+LIB_PATH = os.environ.get("DNSMI_LIB") or os.path.join(HERE, "libdnsmi.so")
 
 DNSMI_KOLMOGOROV = 0
 DNSMI_WALEFFE = 1
@@ -71,6 +71,7 @@ SIGNATURES = {
     "dnsmi_profile_begin": (_I, [_P]),
     "dnsmi_profile_end": (_I, [_P, _I, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong)]),
     "dnsmi_kernel_name": (ctypes.c_char_p, [_I]),
+    "dnsmi_walls_table": (_I, [_P, _I, _P, ctypes.c_longlong]),
 }
 
 KERNEL_IDS = 12

@@ -62,6 +62,7 @@ int walls_predict_and_correct(Ctx* c, const double* state, double* pred, double*
 int walls_iterate_correction(Ctx* c, const double* state_prev, double* pred,
                              const double* rhs_prev, double* rhs_next, double* err,
                              hipStream_t s);
+int walls_table(Ctx* c, int which, double* out, long long n);
 
 }  // namespace dnsmi
 
@@ -207,6 +208,12 @@ int dnsmi_correct_velocity(dnsmi_ctx* ctx, double* state, double* err_dev, void*
     return DNSMI_OK;
   }
   return dnsmi::periodic_correct_velocity(c, state, err_dev, as_stream(stream));
+}
+
+int dnsmi_walls_table(dnsmi_ctx* ctx, int which, double* out, long long n) {
+  DNSMI_CHECK_CTX(ctx)
+  DNSMI_REQUIRE(out && n >= 0, "walls_table: bad output");
+  return dnsmi::walls_table(c, which, out, n);
 }
 
 static const char* const kKernelNames[dnsmi::KID_COUNT] = {

@@ -68,6 +68,7 @@ struct Ctx {
   WallTables* wt = nullptr;
 
   int tile_cols = 16;
+  bool xpass_split = false;  // x-pass form (both are exact; the faster is chosen)
 
   // per-kernel timing (off unless dnsmi_profile_begin was called)
   bool prof = false;

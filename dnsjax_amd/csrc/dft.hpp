@@ -214,6 +214,15 @@ DNSMI_PLAN(144, 12, 12)
 DNSMI_PLAN(192, 16, 12)
 #undef DNSMI_PLAN
 
+// Column passes (y and z) hold two step-A inputs at once, so they take the
+// smaller factor first: (N1, N2) with N1 <= N2.
+template <int P>
+struct ColPlan {
+  static constexpr int N1 = Plan<P>::N2;
+  static constexpr int N2 = Plan<P>::N1;
+  static constexpr int threads = N2;  // per column: max(N1, N2)
+};
+
 // The list of instantiated lengths, for host-side dispatch.
 #define DNSMI_FOR_EACH_P(X) X(12) X(16) X(24) X(32) X(48) X(64) X(72) X(96) X(128) X(144) X(192)
 
@@ -225,7 +234,7 @@ DNSMI_PLAN(192, 16, 12)
 // tw[m] = exp(+2 pi i m / P) in T precision; DIR < 0 uses its conjugate.
 template <int P, int N1, int N2, int DIR, int TC, typename T>
 __device__ __forceinline__ void step_a_store(cx<T>* x, int b, int col, cx<T>* lds,
-                                             const cx<T>* __restrict__ tw) {
+                                             const cx<T>* tw) {
   Dft<N1, DIR, T>::run(x);
 #pragma unroll
   for (int k1 = 0; k1 < N1; ++k1) {
@@ -256,7 +265,7 @@ struct RowPad {
 
 template <int P, int N1, int N2, int DIR, typename T>
 __device__ __forceinline__ void row_step_a_store(cx<T>* x, int b, cx<T>* row_lds,
-                                                 const cx<T>* __restrict__ tw) {
+                                                 const cx<T>* tw) {
   Dft<N1, DIR, T>::run(x);
 #pragma unroll
   for (int k1 = 0; k1 < N1; ++k1) {

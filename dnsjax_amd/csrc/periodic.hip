@@ -24,6 +24,7 @@
 
 #include "context.hpp"
 #include "dft.hpp"
+#include "passes.hpp"
 
 namespace dnsmi {
 
@@ -54,12 +55,15 @@ __device__ __forceinline__ cx<T> to_t(cx<double> d) {
 
 // ------------------------------------------------------------------ K1
 template <typename T, int P, int TC>
-__global__ void __launch_bounds__(TC * Plan<P>::N1)
+__global__ void __launch_bounds__(TC * ColPlan<P>::threads)
     k_y_inverse(const cx<double>* __restrict__ u, cx<T>* __restrict__ w1,
                 const cx<T>* __restrict__ tw, const double* __restrict__ ky, int ny, int Nz1,
                 int Nx1) {
-  constexpr int N1 = Plan<P>::N1, N2 = Plan<P>::N2;
+  constexpr int N1 = ColPlan<P>::N1, N2 = ColPlan<P>::N2;
   __shared__ cx<T> lds[TC * P];
+  __shared__ cx<T> tws[P];
+  for (int i = threadIdx.x; i < P; i += blockDim.x) tws[i] = tw[i];
+  __syncthreads();
   const int tid = threadIdx.x, col = tid % TC, j = tid / TC;
   const int ntile = (Nx1 + TC - 1) / TC;
   const int kz = blockIdx.x / ntile;
@@ -91,7 +95,7 @@ __global__ void __launch_bounds__(TC * Plan<P>::N1)
         xa[a] = v;
         xb[a] = vk;
       }
-      step_a_store<P, N1, N2, +1, TC>(xa, j, col, lds, tw);
+      step_a_store<P, N1, N2, +1, TC>(xa, j, col, lds, tws);
     }
     __syncthreads();
     if (j < N1 && ok) {
@@ -102,7 +106,7 @@ __global__ void __launch_bounds__(TC * Plan<P>::N1)
     }
     __syncthreads();
     if (pass < 2) {
-      if (j < N2) step_a_store<P, N1, N2, +1, TC>(xb, j, col, lds, tw);
+      if (j < N2) step_a_store<P, N1, N2, +1, TC>(xb, j, col, lds, tws);
       __syncthreads();
       if (j < N1 && ok) {
         cx<T> y[N2];
@@ -118,11 +122,14 @@ __global__ void __launch_bounds__(TC * Plan<P>::N1)
 
 // ------------------------------------------------------------------ K2
 template <typename T, int P, int TC>
-__global__ void __launch_bounds__(TC * Plan<P>::N1)
+__global__ void __launch_bounds__(TC * ColPlan<P>::threads)
     k_z_inverse(const cx<T>* __restrict__ w1, cx<T>* __restrict__ w2, const cx<T>* __restrict__ tw,
                 const double* __restrict__ kz_tab, int nz, int Py, int Nx1) {
-  constexpr int N1 = Plan<P>::N1, N2 = Plan<P>::N2;
+  constexpr int N1 = ColPlan<P>::N1, N2 = ColPlan<P>::N2;
   __shared__ cx<T> lds[TC * P];
+  __shared__ cx<T> tws[P];
+  for (int i = threadIdx.x; i < P; i += blockDim.x) tws[i] = tw[i];
+  __syncthreads();
   const int tid = threadIdx.x, col = tid % TC, j = tid / TC;
   const int ntile = (Nx1 + TC - 1) / TC;
   const int y = blockIdx.x / ntile;
@@ -164,7 +171,7 @@ __global__ void __launch_bounds__(TC * Plan<P>::N1)
         xa[a] = va;
         xb[a] = vb;
       }
-      step_a_store<P, N1, N2, +1, TC>(xa, j, col, lds, tw);
+      step_a_store<P, N1, N2, +1, TC>(xa, j, col, lds, tws);
     }
     const int fa = pass == 0 ? 0 : (pass == 1 ? 1 : (pass == 2 ? 2 : 5));
     const int fb = pass == 0 ? 4 : 3;
@@ -178,7 +185,7 @@ __global__ void __launch_bounds__(TC * Plan<P>::N1)
     }
     __syncthreads();
     if (pass < 2) {
-      if (j < N2) step_a_store<P, N1, N2, +1, TC>(xb, j, col, lds, tw);
+      if (j < N2) step_a_store<P, N1, N2, +1, TC>(xb, j, col, lds, tws);
       __syncthreads();
       if (j < N1 && ok) {
         cx<T> o[N2];
@@ -212,6 +219,9 @@ __global__ void __launch_bounds__(R * Plan<P>::N1)
   constexpr int N1 = Plan<P>::N1, N2 = Plan<P>::N2;
   constexpr int ROW = XRow<N1, N2>::value;
   __shared__ cx<T> lds[R * ROW];
+  __shared__ cx<T> tws[P];
+  for (int i = threadIdx.x; i < P; i += blockDim.x) tws[i] = tw[i];
+  __syncthreads();
   const int tid = threadIdx.x, r = tid / N1, j = tid - r * N1;
   const int nrows = Py * Pz;
   const int row = blockIdx.x * R + r;
@@ -223,7 +233,7 @@ __global__ void __launch_bounds__(R * Plan<P>::N1)
   cx<T>* rl = lds + r * ROW;
 
   T ux[N2], uy[N2], uz[N2], wx[N2], wy[N2], wz[N2];
-#pragma unroll 1
+#pragma unroll
   for (int pair = 0; pair < 3; ++pair) {
     if (j < N2) {
       cx<T> z[N1];
@@ -261,7 +271,7 @@ __global__ void __launch_bounds__(R * Plan<P>::N1)
         // Z_m = X_m + i Y_m with X_{P-k} = conj(X_k)
         z[a] = cj ? cx<T>{X.x + Y.y, Y.x - X.y} : cx<T>{X.x - Y.y, X.y + Y.x};
       }
-      row_step_a_store<P, N1, N2, +1>(z, j, rl, tw);
+      row_step_a_store<P, N1, N2, +1>(z, j, rl, tws);
     }
     __syncthreads();
     {
@@ -306,13 +316,13 @@ __global__ void __launch_bounds__(R * Plan<P>::N1)
 
   const T inv = static_cast<T>(1.0 / P);
   cx<T>* out = w3 + (size_t)(ok ? row : 0) * Nx1;
-#pragma unroll 1
+#pragma unroll
   for (int c = 0; c < 3; ++c) {
     {
       cx<T> v[N2];
 #pragma unroll
       for (int a = 0; a < N2; ++a) v[a] = {c == 0 ? nl[0][a] : (c == 1 ? nl[1][a] : nl[2][a]), 0};
-      row_step_a_store<P, N2, N1, -1>(v, j, rl, tw);
+      row_step_a_store<P, N2, N1, -1>(v, j, rl, tws);
     }
     __syncthreads();
     if (j < N2 && ok) {
@@ -328,13 +338,170 @@ __global__ void __launch_bounds__(R * Plan<P>::N1)
   }
 }
 
+// K3, split form (N2 even): 2 N1 threads per row.  The inverse step B
+// (a DFT-N2 per k1) is split by decimation in frequency between the thread
+// pair (k1, h): z_b = (y_b +- y_{b+N2/2}) w^{b h}, a DFT-(N2/2), outputs
+// k2 = 2m + h.  Each thread then holds N2/2 physical points per field
+// (half of the plain form), at x = (k1 + N1 h) + 2 N1 m, which is the input
+// distribution of the forward plan (N2/2, 2 N1); its step B (DFT-2N1) is
+// split the same way.  Both threads of a pair read the whole LDS column.
+template <typename T, int P, int R>
+__global__ void __launch_bounds__(R * 2 * Plan<P>::N1)
+    k_x_nonlinear_split(const cx<T>* __restrict__ w2, cx<T>* __restrict__ w3,
+                        const cx<T>* __restrict__ tw, const double* __restrict__ kx_tab,
+                        const double* __restrict__ base, const double* __restrict__ cbase,
+                        const double* __restrict__ nbase, int nx, int Py, int Pz,
+                        int ybase_stride) {
+  constexpr int N1 = Plan<P>::N1, N2 = Plan<P>::N2;
+  static_assert(N2 % 2 == 0, "split x pass needs an even second factor");
+  constexpr int H2 = N2 / 2;    // points per field per thread
+  constexpr int TPR = 2 * N1;   // threads per row
+  constexpr int F1 = H2, F2 = TPR, G2 = F2 / 2;  // forward plan (F1, F2)
+  constexpr int ROWA = N1 * (N2 + 1), ROWB = F1 * (F2 + 1);
+  constexpr int ROW = ROWA > ROWB ? ROWA : ROWB;
+  __shared__ cx<T> lds[R * ROW];
+  __shared__ cx<T> tws[P];
+  for (int i = threadIdx.x; i < P; i += blockDim.x) tws[i] = tw[i];
+  __syncthreads();
+  const int tid = threadIdx.x, r = tid / TPR, t = tid - r * TPR;
+  const int k1 = t >> 1, h = t & 1;
+  const int nrows = Py * Pz;
+  const int row = blockIdx.x * R + r;
+  const bool ok = row < nrows;
+  const int yrow = ok ? row / Pz : 0;
+  const int Nx1 = nx / 2;
+  const size_t fstride = (size_t)nrows * Nx1;
+  const cx<T>* in = w2 + (size_t)(ok ? row : 0) * Nx1;
+  cx<T>* rl = lds + r * ROW;
+
+  T ux[H2], uy[H2], uz[H2], wx[H2], wy[H2], wz[H2];
+#pragma unroll
+  for (int pair = 0; pair < 3; ++pair) {
+    if (t < N2) {
+      cx<T> z[N1];
+#pragma unroll
+      for (int a = 0; a < N1; ++a) {
+        const int m = N2 * a + t;
+        int k = -1;
+        bool cj = false;
+        if (m < Nx1) {
+          k = m;
+        } else if (m > P - Nx1) {
+          k = P - m;
+          cj = true;
+        }
+        cx<T> X{0, 0}, Y{0, 0};
+        if (ok && k >= 0) {
+          const T kk = static_cast<T>(kx_tab[k]);
+          if (pair == 0) {
+            X = in[2 * fstride + k];
+            const cx<T> g = in[4 * fstride + k];
+            Y = {g.x + kk * X.y, g.y - kk * X.x};
+          } else if (pair == 1) {
+            X = in[1 * fstride + k];
+            const cx<T> g = in[5 * fstride + k];
+            Y = {g.x - kk * X.y, g.y + kk * X.x};
+          } else {
+            X = in[0 * fstride + k];
+            Y = in[3 * fstride + k];
+          }
+          if (k == 0) {
+            X.y = 0;
+            Y.y = 0;
+          }
+        }
+        z[a] = cj ? cx<T>{X.x + Y.y, Y.x - X.y} : cx<T>{X.x - Y.y, X.y + Y.x};
+      }
+      row_step_a_store<P, N1, N2, +1>(z, t, rl, tws);
+    }
+    __syncthreads();
+    {
+      cx<T> zz[H2];
+      const cx<T>* col = rl + k1 * (N2 + 1);
+#pragma unroll
+      for (int b = 0; b < H2; ++b) {
+        const cx<T> y0 = col[b], y1 = col[b + H2];
+        zz[b] = h ? twiddle_const<N2, +1>(csub(y0, y1), b) : cadd(y0, y1);
+      }
+      Dft<H2, +1, T>::run(zz);
+#pragma unroll
+      for (int m = 0; m < H2; ++m) {
+        if (pair == 0) {
+          uz[m] = zz[m].x;
+          wy[m] = zz[m].y;
+        } else if (pair == 1) {
+          uy[m] = zz[m].x;
+          wz[m] = zz[m].y;
+        } else {
+          ux[m] = zz[m].x;
+          wx[m] = zz[m].y;
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  const T U0 = static_cast<T>(base[0 * ybase_stride + yrow]);
+  const T U1 = static_cast<T>(base[1 * ybase_stride + yrow]);
+  const T U2 = static_cast<T>(base[2 * ybase_stride + yrow]);
+  const T C0 = static_cast<T>(cbase[0 * ybase_stride + yrow]);
+  const T C1 = static_cast<T>(cbase[1 * ybase_stride + yrow]);
+  const T C2 = static_cast<T>(cbase[2 * ybase_stride + yrow]);
+  const T B0 = static_cast<T>(nbase[0 * ybase_stride + yrow]);
+  const T B1 = static_cast<T>(nbase[1 * ybase_stride + yrow]);
+  const T B2 = static_cast<T>(nbase[2 * ybase_stride + yrow]);
+  T nl[3][H2];
+#pragma unroll
+  for (int m = 0; m < H2; ++m) {
+    const T a0 = ux[m], a1 = uy[m], a2 = uz[m];
+    const T b0 = wx[m], b1 = wy[m], b2 = wz[m];
+    nl[0][m] = (a1 * b2 - a2 * b1) + (((a1 * C2 - a2 * C1) + (U1 * b2 - U2 * b1)) + B0);
+    nl[1][m] = (a2 * b0 - a0 * b2) + (((a2 * C0 - a0 * C2) + (U2 * b0 - U0 * b2)) + B1);
+    nl[2][m] = (a0 * b1 - a1 * b0) + (((a0 * C1 - a1 * C0) + (U0 * b1 - U1 * b0)) + B2);
+  }
+
+  const T inv = static_cast<T>(1.0 / P);
+  cx<T>* out = w3 + (size_t)(ok ? row : 0) * Nx1;
+  const int bq = k1 + N1 * h;  // forward input column: x = bq + F2 m
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    {
+      cx<T> v[F1];
+#pragma unroll
+      for (int m = 0; m < F1; ++m) v[m] = {nl[c][m], 0};
+      row_step_a_store<P, F1, F2, -1>(v, bq, rl, tws);
+    }
+    __syncthreads();
+    if (t < 2 * F1 && ok) {
+      const int kf = t >> 1, hh = t & 1;
+      const cx<T>* col = rl + kf * (F2 + 1);
+      cx<T> zz[G2];
+#pragma unroll
+      for (int b = 0; b < G2; ++b) {
+        const cx<T> y0 = col[b], y1 = col[b + G2];
+        zz[b] = hh ? twiddle_const<F2, -1>(csub(y0, y1), b) : cadd(y0, y1);
+      }
+      Dft<G2, -1, T>::run(zz);
+#pragma unroll
+      for (int m = 0; m < G2; ++m) {
+        const int k = kf + F1 * (2 * m + hh);
+        if (k < Nx1) out[c * fstride + k] = cscale(zz[m], inv);
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // ------------------------------------------------------------------ K4
 template <typename T, int P, int TC>
-__global__ void __launch_bounds__(TC * Plan<P>::N1)
+__global__ void __launch_bounds__(TC * ColPlan<P>::threads)
     k_z_forward(const cx<T>* __restrict__ w3, cx<T>* __restrict__ w4, const cx<T>* __restrict__ tw,
                 int nz, int Py, int Nx1) {
-  constexpr int N1 = Plan<P>::N1, N2 = Plan<P>::N2;
+  constexpr int N1 = ColPlan<P>::N1, N2 = ColPlan<P>::N2;
   __shared__ cx<T> lds[TC * P];
+  __shared__ cx<T> tws[P];
+  for (int i = threadIdx.x; i < P; i += blockDim.x) tws[i] = tw[i];
+  __syncthreads();
   const int tid = threadIdx.x, col = tid % TC, j = tid / TC;
   const int ntile = (Nx1 + TC - 1) / TC;
   const int y = blockIdx.x / ntile;
@@ -353,7 +520,7 @@ __global__ void __launch_bounds__(TC * Plan<P>::N1)
 #pragma unroll
       for (int a = 0; a < N1; ++a)
         x[a] = ok ? w3[c * in_f + in_off + (size_t)(N2 * a + j) * Nx1] : cx<T>{0, 0};
-      step_a_store<P, N1, N2, -1, TC>(x, j, col, lds, tw);
+      step_a_store<P, N1, N2, -1, TC>(x, j, col, lds, tws);
     }
     __syncthreads();
     if (j < N1 && ok) {
@@ -370,27 +537,16 @@ __global__ void __launch_bounds__(TC * Plan<P>::N1)
 }
 
 // ------------------------------------------------------------------ K5
-// Deterministic block sum (fixed shuffle tree, then waves in order).
-__device__ __forceinline__ double block_sum(double v, double* red) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (lane == 0) red[wave] = v;
-  __syncthreads();
-  double s = 0.0;
-  if (threadIdx.x == 0) {
-    const int nw = (blockDim.x + 63) >> 6;
-    for (int w = 0; w < nw; ++w) s += red[w];
-  }
-  return s;
-}
-
 template <typename T, int P, int TC, int MODE>
-__global__ void __launch_bounds__(TC * Plan<P>::N1)
+__global__ void __launch_bounds__(TC * ColPlan<P>::threads)
     k_y_forward(const cx<T>* __restrict__ w4, const cx<T>* __restrict__ tw,
                 const double* __restrict__ kx_tab, const double* __restrict__ ky_tab,
                 const double* __restrict__ kz_tab, int ny, int Nz1, int Nx1, StepArgs args) {
-  constexpr int N1 = Plan<P>::N1, N2 = Plan<P>::N2;
+  constexpr int N1 = ColPlan<P>::N1, N2 = ColPlan<P>::N2;
   __shared__ cx<T> lds[TC * P];
+  __shared__ cx<T> tws[P];
+  for (int i = threadIdx.x; i < P; i += blockDim.x) tws[i] = tw[i];
+  __syncthreads();
   __shared__ double red[16];
   const int tid = threadIdx.x, col = tid % TC, j = tid / TC;
   const int ntile = (Nx1 + TC - 1) / TC;
@@ -410,7 +566,7 @@ __global__ void __launch_bounds__(TC * Plan<P>::N1)
 #pragma unroll
       for (int a = 0; a < N1; ++a)
         x[a] = ok ? w4[c * in_f + (size_t)(N2 * a + j) * plane + off] : cx<T>{0, 0};
-      step_a_store<P, N1, N2, -1, TC>(x, j, col, lds, tw);
+      step_a_store<P, N1, N2, -1, TC>(x, j, col, lds, tws);
     }
     __syncthreads();
     if (j < N1) {
@@ -606,7 +762,7 @@ static int launch_y_inverse(Ctx* c, const double* state, hipStream_t s) {
   switch (c->Py) {
 #define X(P)                                                                                 \
   case P:                                                                                    \
-    hipLaunchKernelGGL((k_y_inverse<T, P, kTC>), grid, dim3(kTC * Plan<P>::N1), 0, s,        \
+    hipLaunchKernelGGL((k_y_inverse<T, P, kTC>), grid, dim3(kTC * ColPlan<P>::threads), 0, s,  \
                        reinterpret_cast<const cx<double>*>(state), b.A, b.twy, c->ky, c->ny, \
                        c->Nz1, c->Nx1);                                                      \
     break;
@@ -626,7 +782,7 @@ static int launch_z_inverse(Ctx* c, hipStream_t s) {
   switch (c->Pz) {
 #define X(P)                                                                                 \
   case P:                                                                                    \
-    hipLaunchKernelGGL((k_z_inverse<T, P, kTC>), grid, dim3(kTC * Plan<P>::N1), 0, s, b.A,   \
+    hipLaunchKernelGGL((k_z_inverse<T, P, kTC>), grid, dim3(kTC * ColPlan<P>::threads), 0, s, b.A,   \
                        b.B, b.twz, c->kz, c->nz, c->Py, c->Nx1);                             \
     break;
     DNSMI_FOR_EACH_P(X)
@@ -636,18 +792,46 @@ static int launch_z_inverse(Ctx* c, hipStream_t s) {
   return check_launch(c, "z_inverse");
 }
 
+template <int P>
+struct XSplit {
+  static constexpr bool value = Plan<P>::N2 % 2 == 0;  // split form available
+  static constexpr int tpr = value ? 2 * Plan<P>::N1 : Plan<P>::N1;
+  static constexpr int rows = (256 / tpr) > 0 ? 256 / tpr : 1;
+};
+
+template <typename T, int P>
+static void launch_x_len(Ctx* c, const Bufs<T>& b, const cx<T>* xin, cx<T>* xout,
+                         const double* base, const double* cbase, const double* nbase, int ny,
+                         hipStream_t s) {
+  constexpr int R = XSplit<P>::rows;
+  const int nrows = ny * c->Pz;
+  const dim3 grid((nrows + R - 1) / R);
+  bool split = false;
+  if constexpr (XSplit<P>::value) split = c->xpass_split;
+  if (split) {
+    if constexpr (XSplit<P>::value)
+      hipLaunchKernelGGL((k_x_nonlinear_split<T, P, R>), grid, dim3(R * XSplit<P>::tpr), 0, s,
+                         xin, xout, b.twx, c->kx, base, cbase, nbase, c->nx, ny, c->Pz, ny);
+  } else {
+    constexpr int R1 = kRows;
+    const dim3 grid1((nrows + R1 - 1) / R1);
+    hipLaunchKernelGGL((k_x_nonlinear<T, P, R1>), grid1, dim3(R1 * Plan<P>::N1), 0, s, xin, xout,
+                       b.twx, c->kx, base, cbase, nbase, c->nx, ny, c->Pz, ny);
+  }
+}
+
+// x pass over rows (y, z) of W[f][y][z][kx]: periodic (y = padded grid) and
+// wall-bounded (y = the grid points) share it; the base flow profiles are
+// given per y.
 template <typename T>
-static int launch_x_nonlinear(Ctx* c, hipStream_t s) {
+int launch_x_pass(Ctx* c, const cx<T>* xin, cx<T>* xout, const double* base, const double* cbase,
+                  const double* nbase, int ny, int kid, hipStream_t s) {
   Bufs<T> b = bufs_of<T>(c);
-  const int nrows = c->Py * c->Pz;
-  const dim3 grid((nrows + kRows - 1) / kRows);
-  ProfScope ps(c, KID_X_NONLINEAR, s);
+  ProfScope ps(c, kid, s);
   switch (c->Px) {
-#define X(P)                                                                                  \
-  case P:                                                                                     \
-    hipLaunchKernelGGL((k_x_nonlinear<T, P, kRows>), grid, dim3(kRows * Plan<P>::N1), 0, s,   \
-                       b.B, b.A, b.twx, c->kx, c->base, c->cbase, c->nbase, c->nx, c->Py,     \
-                       c->Pz, c->Py);                                                         \
+#define X(P)                                                                \
+  case P:                                                                   \
+    launch_x_len<T, P>(c, b, xin, xout, base, cbase, nbase, ny, s);        \
     break;
     DNSMI_FOR_EACH_P(X)
 #undef X
@@ -655,24 +839,46 @@ static int launch_x_nonlinear(Ctx* c, hipStream_t s) {
   }
   return check_launch(c, "x_nonlinear");
 }
+template int launch_x_pass<double>(Ctx*, const cx<double>*, cx<double>*, const double*,
+                                   const double*, const double*, int, int, hipStream_t);
+template int launch_x_pass<float>(Ctx*, const cx<float>*, cx<float>*, const double*,
+                                  const double*, const double*, int, int, hipStream_t);
 
 template <typename T>
-static int launch_z_forward(Ctx* c, hipStream_t s) {
+static int launch_x_nonlinear(Ctx* c, hipStream_t s) {
+  Bufs<T> b = bufs_of<T>(c);
+  return launch_x_pass<T>(c, b.B, b.A, c->base, c->cbase, c->nbase, c->Py, KID_X_NONLINEAR, s);
+}
+
+// z forward over columns (y, kx) of W[c][y][z][kx] -> W[c][y][kz][kx]; ny is
+// the number of y planes (padded y for periodic, grid points for walls).
+template <typename T>
+int launch_z_forward_buf(Ctx* c, const cx<T>* in, cx<T>* out, int ny, int kid, hipStream_t s) {
   Bufs<T> b = bufs_of<T>(c);
   const int ntile = (c->Nx1 + kTC - 1) / kTC;
-  const dim3 grid(c->Py * ntile);
-  ProfScope ps(c, KID_Z_FORWARD, s);
+  const dim3 grid(ny * ntile);
+  ProfScope ps(c, kid, s);
   switch (c->Pz) {
-#define X(P)                                                                               \
-  case P:                                                                                  \
-    hipLaunchKernelGGL((k_z_forward<T, P, kTC>), grid, dim3(kTC * Plan<P>::N1), 0, s, b.A, \
-                       b.B, b.twz, c->nz, c->Py, c->Nx1);                                  \
+#define X(P)                                                                                   \
+  case P:                                                                                      \
+    hipLaunchKernelGGL((k_z_forward<T, P, kTC>), grid, dim3(kTC * ColPlan<P>::threads), 0, s,  \
+                       in, out, b.twz, c->nz, ny, c->Nx1);                                     \
     break;
     DNSMI_FOR_EACH_P(X)
 #undef X
     default: return c->fail(DNSMI_EINVAL, "unsupported padded z length");
   }
   return check_launch(c, "z_forward");
+}
+template int launch_z_forward_buf<double>(Ctx*, const cx<double>*, cx<double>*, int, int,
+                                          hipStream_t);
+template int launch_z_forward_buf<float>(Ctx*, const cx<float>*, cx<float>*, int, int,
+                                         hipStream_t);
+
+template <typename T>
+static int launch_z_forward(Ctx* c, hipStream_t s) {
+  Bufs<T> b = bufs_of<T>(c);
+  return launch_z_forward_buf<T>(c, b.A, b.B, c->Py, KID_Z_FORWARD, s);
 }
 
 template <typename T, int MODE>
@@ -685,7 +891,7 @@ static int launch_y_forward(Ctx* c, const StepArgs& a, hipStream_t s, int* nbloc
   switch (c->Py) {
 #define X(P)                                                                                   \
   case P:                                                                                      \
-    hipLaunchKernelGGL((k_y_forward<T, P, kTC, MODE>), dim3(nb), dim3(kTC * Plan<P>::N1), 0,   \
+    hipLaunchKernelGGL((k_y_forward<T, P, kTC, MODE>), dim3(nb), dim3(kTC * ColPlan<P>::threads), 0,   \
                        s, b.B, b.twy, c->kx, c->ky, c->kz, c->ny, c->Nz1, c->Nx1, a);          \
     break;
     DNSMI_FOR_EACH_P(X)
@@ -839,7 +1045,7 @@ int periodic_correct_velocity(Ctx* c, double* state, double* err, hipStream_t s)
 }
 
 // ------------------------------------------------------------------ set-up
-static bool supported_length(int P) {
+bool supported_length(int P) {
   switch (P) {
 #define X(P_) case P_:
     DNSMI_FOR_EACH_P(X)

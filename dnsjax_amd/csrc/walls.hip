@@ -1,15 +1,1070 @@
-// Wall-bounded (plane Couette) path: placeholder until the IMM kernels land.
+// Plane Couette (wall-bounded) path, gfx950.
+//
+// Set-up (host C++, once per context; geometries/cartesian.py:575-801,
+// fd.py:21-108): CGL grid, Fornberg D1/D2 stencils, per-mode Lk (Neumann)
+// and Hk (Dirichlet) band matrices factored with partial pivoting in the
+// LAPACK ab layout (dgbtf2 order of operations), homogeneous solutions
+// p_i, v_i, q_i and the 2x2 influence matrix inverse, y-quadrature
+// weights.  Per-mode tables are stored mode-fastest ([j][row][mode]) so a
+// wavefront whose lanes are consecutive (kz, kx) modes reads them
+// coalesced.
+//
+// Per step:
+//   RHS (flows/plane_couette.py:274-289): WT1 FD curl + transpose
+//   (3,Nkz,Nkx,Ny) -> 6 fields (Ny,Nkz,Nkx); z-inverse; the shared x pass
+//   with base flow U = y; the shared z-forward; WT5 transpose back.
+//   IMM (flows/plane_couette.py:376-504): one thread per (kz, kx) mode
+//   streams over y: divergence and Lk d, the Lk forward sweep fed on the
+//   fly, back substitution, the three Hk right-hand sides fed into one
+//   forward sweep, back substitution, wall divergence, alpha = -M^-1 d,
+//   assembly, correction and the quadrature-weighted norm partials.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
 #include "context.hpp"
+#include "dft.hpp"
+#include "passes.hpp"
 
 namespace dnsmi {
-int walls_setup(Ctx* c) { return c->fail(DNSMI_EINVAL, "plane Couette not built yet"); }
-void walls_release(Ctx*) {}
-int walls_get_rhs(Ctx* c, const double*, double*, hipStream_t) { return c->fail(DNSMI_EINVAL, "walls"); }
-int walls_predict(Ctx* c, const double*, const double*, double*, hipStream_t) { return c->fail(DNSMI_EINVAL, "walls"); }
-int walls_correct(Ctx* c, const double*, const double*, const double*, const double*, double*, double*, hipStream_t) { return c->fail(DNSMI_EINVAL, "walls"); }
-int walls_norm(Ctx* c, const double*, double*, hipStream_t) { return c->fail(DNSMI_EINVAL, "walls"); }
-int walls_predict_and_correct(Ctx* c, const double*, double*, double*, double*, hipStream_t) { return c->fail(DNSMI_EINVAL, "walls"); }
-int walls_iterate_correction(Ctx* c, const double*, double*, const double*, double*, double*, hipStream_t) { return c->fail(DNSMI_EINVAL, "walls"); }
+
+struct WallTables {
+  int p = 0, Ny = 0, Nkz = 0, Nkx = 0, M = 0;
+  // device
+  double* d1w = nullptr;  // (Ny, p+1)
+  int* d1j0 = nullptr;    // (Ny)
+  double* d2w = nullptr;  // (Ny, p+2)
+  int* d2j0 = nullptr;
+  double* qw = nullptr;   // quadrature weight / Ly, (Ny)
+  double* lk_ab = nullptr;  // [j][row][mode]
+  int* lk_piv = nullptr;    // [j][mode]
+  double* hk_ab = nullptr;
+  int* hk_piv = nullptr;
+  double* v1 = nullptr;  // [y][mode]
+  double* v2 = nullptr;
+  double* q1 = nullptr;
+  double* q2 = nullptr;
+  double* minv = nullptr;  // [4][mode]
+  double* mkx = nullptr;   // per mode
+  double* mkz = nullptr;
+  double* scratch = nullptr;  // 5 complex vectors [vec][y][mode]
+  // host copies in the reference layouts, for inspection by tests
+  std::vector<double> h_lk_ab, h_hk_ab, h_v1, h_q1, h_minv, h_ys;
+  std::vector<int> h_lk_piv, h_hk_piv;
+};
+
+// ------------------------------------------------------------------ host set-up
+// Fornberg (1998), SIAM Rev. 40, 685-691: weights of derivatives 0..m at z
+// on nodes x[0..n].
+static void fornberg(double z, const double* x, int n1, int m, std::vector<double>& C) {
+  const int n = n1 - 1;
+  C.assign((size_t)n1 * (m + 1), 0.0);
+  auto at = [&](int i, int k) -> double& { return C[(size_t)i * (m + 1) + k]; };
+  at(0, 0) = 1.0;
+  double c1 = 1.0, c4 = x[0] - z;
+  for (int i = 1; i <= n; ++i) {
+    const int mn = std::min(i, m);
+    double c2 = 1.0;
+    const double c5 = c4;
+    c4 = x[i] - z;
+    for (int j = 0; j < i; ++j) {
+      const double c3 = x[i] - x[j];
+      c2 *= c3;
+      if (j == i - 1) {
+        for (int k = mn; k > 0; --k) at(i, k) = c1 * (k * at(i - 1, k - 1) - c5 * at(i - 1, k)) / c2;
+        at(i, 0) = -c1 * c5 * at(i - 1, 0) / c2;
+      }
+      for (int k = mn; k > 0; --k) at(j, k) = (c4 * at(j, k) - k * at(j, k - 1)) / c3;
+      at(j, 0) = c4 * at(j, 0) / c3;
+    }
+    c1 = c2;
+  }
+}
+
+struct Stencil {
+  int width;
+  std::vector<double> w;  // (Ny, width)
+  std::vector<int> j0;    // (Ny)
+};
+
+// fd.py:68-108: D1 with p+1 points, D2 with p+2 points, one-sided at walls.
+static void build_stencils(const std::vector<double>& y, int p, Stencil& d1, Stencil& d2) {
+  const int Ny = (int)y.size();
+  d1.width = p + 1;
+  d2.width = p + 2;
+  d1.w.assign((size_t)Ny * d1.width, 0.0);
+  d2.w.assign((size_t)Ny * d2.width, 0.0);
+  d1.j0.assign(Ny, 0);
+  d2.j0.assign(Ny, 0);
+  std::vector<double> C;
+  for (int i = 0; i < Ny; ++i) {
+    int j0 = std::max(0, std::min(i - d1.width / 2, Ny - d1.width));
+    fornberg(y[i], &y[j0], d1.width, 1, C);
+    d1.j0[i] = j0;
+    for (int k = 0; k < d1.width; ++k) d1.w[(size_t)i * d1.width + k] = C[(size_t)k * 2 + 1];
+    j0 = std::max(0, std::min(i - d2.width / 2, Ny - d2.width));
+    fornberg(y[i], &y[j0], d2.width, 2, C);
+    d2.j0[i] = j0;
+    for (int k = 0; k < d2.width; ++k) d2.w[(size_t)i * d2.width + k] = C[(size_t)k * 3 + 2];
+  }
+}
+
+// Dense row i of a stencil operator, as a function (j -> value).
+static double stencil_at(const Stencil& s, int i, int j) {
+  const int k = j - s.j0[i];
+  return (k >= 0 && k < s.width) ? s.w[(size_t)i * s.width + k] : 0.0;
+}
+
+// LAPACK band storage: A(i, j) = ab[(kl + ku + i - j) * n + j] (row-major
+// rows of length n here), 2 kl + ku + 1 rows.
+struct Band {
+  int n, kl, ku, rows;
+  std::vector<double> ab;
+  std::vector<int> piv;
+  Band(int n_, int kl_, int ku_) : n(n_), kl(kl_), ku(ku_), rows(2 * kl_ + ku_ + 1) {
+    ab.assign((size_t)rows * n, 0.0);
+    piv.assign(n, 0);
+  }
+  double& at(int i, int j) { return ab[(size_t)(kl + ku + i - j) * n + j]; }
+  double& raw(int r, int j) { return ab[(size_t)r * n + j]; }
+};
+
+// Partial-pivoting band LU in the order of LAPACK's unblocked dgbtf2
+// (the routine dgbtrf uses when its block size exceeds kl): pivot = first
+// largest |a| in the column, reciprocal-scaled multipliers, rank-1 update
+// up to the last column touched so far.  Returns 0 or the 1-based index of
+// a zero pivot.
+static int band_lu(Band& B) {
+  const int n = B.n, kl = B.kl, ku = B.ku, kv = kl + ku;
+  int ju = 0, info = 0;
+  for (int j = 0; j < n; ++j) {
+    const int km = std::min(kl, n - 1 - j);
+    int jp = 0;
+    double best = std::fabs(B.raw(kv, j));
+    for (int i = 1; i <= km; ++i) {
+      const double a = std::fabs(B.raw(kv + i, j));
+      if (a > best) {
+        best = a;
+        jp = i;
+      }
+    }
+    B.piv[j] = j + jp;
+    if (B.raw(kv + jp, j) != 0.0) {
+      ju = std::max(ju, std::min(j + ku + jp, n - 1));
+      if (jp != 0)
+        for (int c = j; c <= ju; ++c) std::swap(B.at(j, c), B.at(j + jp, c));
+      if (km > 0) {
+        const double r = 1.0 / B.raw(kv, j);
+        for (int i = 1; i <= km; ++i) B.raw(kv + i, j) *= r;
+        for (int c = j + 1; c <= ju; ++c) {
+          const double t = -B.at(j, c);
+          if (t != 0.0)
+            for (int i = 1; i <= km; ++i) B.at(j + i, c) += B.raw(kv + i, j) * t;
+        }
+      }
+    } else if (info == 0) {
+      info = j + 1;
+    }
+  }
+  return info;
+}
+
+// dgbtrs('N') on one real right-hand side, as the device sweep does it.
+static void band_solve(const Band& B, std::vector<double>& b) {
+  const int n = B.n, kl = B.kl, ku = B.ku, kuu = kl + ku;
+  auto raw = [&](int r, int j) { return B.ab[(size_t)r * n + j]; };
+  for (int j = 0; j < n - 1; ++j) {
+    const int pj = B.piv[j];
+    std::swap(b[j], b[pj]);
+    for (int i = 1; i <= kl && j + i < n; ++i) b[j + i] -= raw(kl + ku + i, j) * b[j];
+  }
+  for (int j = n - 1; j >= 0; --j) {
+    const double x = b[j] / raw(kl + ku, j);
+    b[j] = x;
+    for (int i = 1; i <= kuu && j - i >= 0; ++i) b[j - i] -= raw(kuu - i, j) * x;
+  }
+}
+
+// Linear weights of the y quadrature, divided by Ly.  Odd Ny: the
+// non-uniform composite Simpson rule (geometries/cartesian.py:125-140).
+// Even Ny (the reference exits there): Simpson panels over the first Ny-4
+// intervals plus the exact integral of the cubic through the last four
+// points.
+static std::vector<double> quadrature_weights(const std::vector<double>& y, double Ly) {
+  const int n = (int)y.size();
+  std::vector<double> w(n, 0.0);
+  const int m = (n % 2 == 1) ? n : n - 3;
+  for (int i = 0; i + 2 < m; i += 2) {
+    const double h0 = y[i + 1] - y[i], h1 = y[i + 2] - y[i + 1];
+    const double hs = h0 + h1, hp = h0 * h1, r = h0 / h1;
+    w[i] += hs / 6 * (2 - 1 / r);
+    w[i + 1] += hs / 6 * (hs * hs / hp);
+    w[i + 2] += hs / 6 * (2 - r);
+  }
+  if (n % 2 == 0) {
+    // solve V w = moments, V[m][i] = (x_i - a)^m, on the last 4 points
+    const double a = y[n - 4], b = y[n - 1];
+    double V[4][5];
+    for (int r = 0; r < 4; ++r) {
+      for (int i = 0; i < 4; ++i) V[r][i] = std::pow(y[n - 4 + i] - a, r);
+      V[r][4] = std::pow(b - a, r + 1) / (r + 1);
+    }
+    for (int c = 0; c < 4; ++c) {  // Gaussian elimination with partial pivoting
+      int pr = c;
+      for (int r = c + 1; r < 4; ++r)
+        if (std::fabs(V[r][c]) > std::fabs(V[pr][c])) pr = r;
+      for (int k = 0; k < 5; ++k) std::swap(V[c][k], V[pr][k]);
+      for (int r = c + 1; r < 4; ++r) {
+        const double f = V[r][c] / V[c][c];
+        for (int k = c; k < 5; ++k) V[r][k] -= f * V[c][k];
+      }
+    }
+    double x[4];
+    for (int r = 3; r >= 0; --r) {
+      double s = V[r][4];
+      for (int k = r + 1; k < 4; ++k) s -= V[r][k] * x[k];
+      x[r] = s / V[r][r];
+    }
+    for (int i = 0; i < 4; ++i) w[n - 4 + i] += x[i];
+  }
+  for (double& v : w) v /= Ly;
+  return w;
+}
+
+template <typename V>
+static int upload_vec(Ctx* c, V** dst, const std::vector<V>& v) {
+  *dst = static_cast<V*>(c->alloc(v.size() * sizeof(V)));
+  if (!*dst) return c->fail(DNSMI_ENOMEM, "device allocation failed");
+  const hipError_t e = hipMemcpy(*dst, v.data(), v.size() * sizeof(V), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return c->fail(DNSMI_EHIP, hipGetErrorString(e));
+  return DNSMI_OK;
+}
+
+int walls_setup(Ctx* c) {
+  const dnsmi_config& g = c->cfg;
+  if (g.nx % 2 || g.nz % 2 || g.nx < 4 || g.nz < 4 || g.ny < 5)
+    return c->fail(DNSMI_EINVAL, "walls: nx, nz even >= 4 and ny >= 5");
+  if (g.oversampling_factor != 3) return c->fail(DNSMI_EINVAL, "oversampling_factor must be 3");
+  const int p = g.fd_order;
+  if (p != 2 && p != 4 && p != 6 && p != 8) return c->fail(DNSMI_EINVAL, "fd_order must be 2, 4, 6 or 8");
+  if (g.ny < p + 3) return c->fail(DNSMI_EINVAL, "ny too small for fd_order");
+  c->nx = g.nx;
+  c->ny = g.ny;
+  c->nz = g.nz;
+  c->Nx1 = g.nx / 2;
+  c->Nz1 = g.nz - 1;
+  c->Ny1 = g.ny;
+  c->Px = 3 * g.nx / 2;
+  c->Pz = 3 * g.nz / 2;
+  c->Py = g.ny;
+  if (!supported_length(c->Px) || !supported_length(c->Pz))
+    return c->fail(DNSMI_EINVAL, "padded lengths must be one of 12,16,24,32,48,64,72,96,128,144,192");
+  const int Ny = g.ny, Nkz = c->Nz1, Nkx = c->Nx1, M = Nkz * Nkx;
+  c->spec_elems = 3LL * M * Ny;
+  WallTables* t = new WallTables();
+  c->wt = t;
+  t->p = p;
+  t->Ny = Ny;
+  t->Nkz = Nkz;
+  t->Nkx = Nkx;
+  t->M = M;
+
+  // buffers: WT1 out (6,Ny,Nkz,Nkx) | z-inverse out (6,Ny,Pz,Nkx) | x-pass out
+  // (3,Ny,Pz,Nkx) | z-forward out (3,Ny,Nkz,Nkx)
+  const size_t cxs = c->fp32 ? 8 : 16;
+  const size_t a1 = 6ull * Ny * Nkz * Nkx, a3 = 3ull * Ny * c->Pz * Nkx;
+  const size_t b2 = 6ull * Ny * c->Pz * Nkx, b4 = 3ull * Ny * Nkz * Nkx;
+  c->bufA_bytes = std::max(a1, a3) * cxs;
+  c->bufB_bytes = std::max(b2, b4) * cxs;
+  c->bufA = c->alloc(c->bufA_bytes);
+  c->bufB = c->alloc(c->bufB_bytes);
+  c->rhs0 = static_cast<double*>(c->alloc(c->spec_elems * 16));
+  c->max_partials = (M + 63) / 64 + 1024;
+  c->partials = static_cast<double*>(c->alloc(sizeof(double) * c->max_partials));
+  t->scratch = static_cast<double*>(c->alloc(5ull * Ny * M * 16));
+  if (!c->bufA || !c->bufB || !c->rhs0 || !c->partials || !t->scratch)
+    return c->fail(DNSMI_ENOMEM, "device allocation failed");
+  int rc = setup_twiddles(c);
+  if (rc) return rc;
+
+  const std::vector<double> kx = real_wavenumbers(g.nx, g.lx);
+  const std::vector<double> kz = complex_wavenumbers(g.nz, g.lz);
+  if ((rc = upload_doubles(c, &c->kx, kx))) return rc;
+  if ((rc = upload_doubles(c, &c->kz, kz))) return rc;
+
+  // CGL grid and base flow U = y (flows/plane_couette.py:73-108)
+  std::vector<double> y(Ny);
+  for (int j = 0; j < Ny; ++j) y[j] = -std::cos((double)j * kPi / (double)(Ny - 1));
+  t->h_ys = y;
+  std::vector<double> base(3 * Ny, 0.0), cbase(3 * Ny, 0.0), nbase(3 * Ny, 0.0);
+  for (int j = 0; j < Ny; ++j) {
+    base[j] = y[j];
+    cbase[2 * Ny + j] = -1.0;
+    nbase[1 * Ny + j] = y[j] * 1.0;
+  }
+  if ((rc = upload_doubles(c, &c->base, base))) return rc;
+  if ((rc = upload_doubles(c, &c->cbase, cbase))) return rc;
+  if ((rc = upload_doubles(c, &c->nbase, nbase))) return rc;
+
+  Stencil d1, d2;
+  build_stencils(y, p, d1, d2);
+  if ((rc = upload_vec(c, &t->d1w, d1.w))) return rc;
+  if ((rc = upload_vec(c, &t->d1j0, d1.j0))) return rc;
+  if ((rc = upload_vec(c, &t->d2w, d2.w))) return rc;
+  if ((rc = upload_vec(c, &t->d2j0, d2.j0))) return rc;
+  if ((rc = upload_vec(c, &t->qw, quadrature_weights(y, g.ly > 0 ? g.ly : 2.0)))) return rc;
+
+  // per-mode IMM tables (geometries/cartesian.py:704-779)
+  const int kl = p, ku = p, rows = 3 * p + 1;
+  const double dt = g.dt, cc = g.implicitness, nu = 1.0 / g.re;
+  std::vector<double> lk_ab((size_t)Ny * rows * M), hk_ab((size_t)Ny * rows * M);
+  std::vector<int> lk_piv((size_t)Ny * M), hk_piv((size_t)Ny * M);
+  std::vector<double> v1((size_t)Ny * M), v2((size_t)Ny * M), q1((size_t)Ny * M), q2((size_t)Ny * M);
+  std::vector<double> minv(4ull * M), mkx(M), mkz(M);
+  t->h_lk_ab.assign((size_t)M * rows * Ny, 0.0);
+  t->h_hk_ab.assign((size_t)M * rows * Ny, 0.0);
+  t->h_lk_piv.assign((size_t)M * Ny, 0);
+  t->h_hk_piv.assign((size_t)M * Ny, 0);
+  t->h_v1.assign((size_t)M * Ny, 0.0);
+  t->h_q1.assign((size_t)M * Ny, 0.0);
+  t->h_minv.assign(4ull * M, 0.0);
+  for (int iz = 0; iz < Nkz; ++iz) {
+    for (int ix = 0; ix < Nkx; ++ix) {
+      const int m = iz * Nkx + ix;
+      const double k2 = kx[ix] * kx[ix] + kz[iz] * kz[iz];
+      mkx[m] = kx[ix];
+      mkz[m] = kz[iz];
+      Band L(Ny, kl, ku), H(Ny, kl, ku);
+      for (int i = 0; i < Ny; ++i) {
+        for (int j = std::max(0, i - kl); j <= std::min(Ny - 1, i + ku); ++j) {
+          const double d2ij = stencil_at(d2, i, j);
+          const double lraw = d2ij - (i == j ? k2 : 0.0);
+          double lv = lraw;
+          if (i == 0) lv = (k2 == 0.0) ? (j == 0 ? 1.0 : 0.0) : stencil_at(d1, 0, j);
+          if (i == Ny - 1) lv = stencil_at(d1, Ny - 1, j);
+          L.at(i, j) = lv;
+          double hv = (i == j ? 1.0 / dt : 0.0) - cc * nu * lraw;
+          if (i == 0 || i == Ny - 1) hv = (i == j) ? 1.0 : 0.0;
+          H.at(i, j) = hv;
+        }
+      }
+      // D1 rows of the walls are p+1 wide and lie inside the band by
+      // construction (one-sided stencils start at the wall).
+      if (band_lu(L) != 0 || band_lu(H) != 0)
+        return c->fail(DNSMI_ESINGULAR, "banded LU hit a zero pivot");
+      std::vector<double> p1(Ny, 0.0), p2(Ny, 0.0);
+      p1[0] = 1.0;
+      p2[Ny - 1] = 1.0;
+      band_solve(L, p1);
+      band_solve(L, p2);
+      auto d1_apply = [&](const std::vector<double>& v, int i) {
+        double s = 0.0;
+        for (int k = 0; k < d1.width; ++k) s += d1.w[(size_t)i * d1.width + k] * v[d1.j0[i] + k];
+        return s;
+      };
+      std::vector<double> r1(Ny), r2(Ny);
+      for (int i = 0; i < Ny; ++i) {
+        r1[i] = -d1_apply(p1, i);
+        r2[i] = -d1_apply(p2, i);
+      }
+      r1[0] = r1[Ny - 1] = r2[0] = r2[Ny - 1] = 0.0;
+      band_solve(H, r1);
+      band_solve(H, r2);
+      std::vector<double> s1 = p1, s2 = p2;
+      s1[0] = s1[Ny - 1] = s2[0] = s2[Ny - 1] = 0.0;
+      band_solve(H, s1);
+      band_solve(H, s2);
+      const double M00 = d1_apply(r1, 0), M01 = d1_apply(r2, 0);
+      const double M10 = d1_apply(r1, Ny - 1), M11 = d1_apply(r2, Ny - 1);
+      double Mi[4];
+      if (k2 == 0.0) {
+        Mi[0] = Mi[1] = Mi[2] = 0.0;
+        Mi[3] = 1.0 / M11;
+      } else {
+        const double det = M00 * M11 - M01 * M10;
+        Mi[0] = M11 / det;
+        Mi[1] = -M01 / det;
+        Mi[2] = -M10 / det;
+        Mi[3] = M00 / det;
+      }
+      for (int q = 0; q < 4; ++q) {
+        minv[(size_t)q * M + m] = Mi[q];
+        t->h_minv[(size_t)m * 4 + q] = Mi[q];
+      }
+      for (int j = 0; j < Ny; ++j) {
+        for (int r = 0; r < rows; ++r) {
+          lk_ab[((size_t)j * rows + r) * M + m] = L.raw(r, j);
+          hk_ab[((size_t)j * rows + r) * M + m] = H.raw(r, j);
+          t->h_lk_ab[((size_t)m * rows + r) * Ny + j] = L.raw(r, j);
+          t->h_hk_ab[((size_t)m * rows + r) * Ny + j] = H.raw(r, j);
+        }
+        lk_piv[(size_t)j * M + m] = L.piv[j];
+        hk_piv[(size_t)j * M + m] = H.piv[j];
+        t->h_lk_piv[(size_t)m * Ny + j] = L.piv[j];
+        t->h_hk_piv[(size_t)m * Ny + j] = H.piv[j];
+        v1[(size_t)j * M + m] = r1[j];
+        v2[(size_t)j * M + m] = r2[j];
+        q1[(size_t)j * M + m] = s1[j];
+        q2[(size_t)j * M + m] = s2[j];
+        t->h_v1[(size_t)m * Ny + j] = r1[j];
+        t->h_q1[(size_t)m * Ny + j] = s1[j];
+      }
+    }
+  }
+  if ((rc = upload_vec(c, &t->lk_ab, lk_ab))) return rc;
+  if ((rc = upload_vec(c, &t->hk_ab, hk_ab))) return rc;
+  if ((rc = upload_vec(c, &t->lk_piv, lk_piv))) return rc;
+  if ((rc = upload_vec(c, &t->hk_piv, hk_piv))) return rc;
+  if ((rc = upload_vec(c, &t->v1, v1))) return rc;
+  if ((rc = upload_vec(c, &t->v2, v2))) return rc;
+  if ((rc = upload_vec(c, &t->q1, q1))) return rc;
+  if ((rc = upload_vec(c, &t->q2, q2))) return rc;
+  if ((rc = upload_vec(c, &t->minv, minv))) return rc;
+  if ((rc = upload_vec(c, &t->mkx, mkx))) return rc;
+  if ((rc = upload_vec(c, &t->mkz, mkz))) return rc;
+  return DNSMI_OK;
+}
+
+void walls_release(Ctx* c) {
+  delete c->wt;
+  c->wt = nullptr;
+}
+
+// ------------------------------------------------------------------ RHS kernels
+// WT1: FD curl + transpose.  In: u (3, M, Ny) with M = Nkz*Nkx modes, FP64.
+// Out: V (6, Ny, Nkz, Nkx) in T: u_x, u_y, u_z, w_x = D1 u_z - i kz u_y,
+// i kz u_x, -D1 u_x (the i kx parts of w_y, w_z are added by the x pass).
+// A block handles TY y-points x TM modes; loads run along y, stores along
+// the mode index.
+constexpr int kTY = 16, kTM = 16;
+
+template <typename T, int S1>
+__global__ void __launch_bounds__(kTY* kTM)
+    k_walls_curl_t(const cx<double>* __restrict__ u, cx<T>* __restrict__ v,
+                   const double* __restrict__ d1w, const int* __restrict__ d1j0,
+                   const double* __restrict__ mkz, int Ny, int M) {
+  constexpr int HALO = S1;
+  __shared__ cx<double> tile[3][kTM][kTY + 2 * HALO + 1];
+  const int ntile_y = (Ny + kTY - 1) / kTY;
+  const int y0 = (blockIdx.x % ntile_y) * kTY;
+  const int m0 = (blockIdx.x / ntile_y) * kTM;
+  const int ylo = max(0, y0 - HALO);
+  const int yhi = min(Ny, y0 + kTY + HALO);
+  const int span = yhi - ylo;
+  const size_t comp = (size_t)M * Ny;
+  // load: lanes along y
+  for (int e = threadIdx.x; e < 3 * kTM * span; e += blockDim.x) {
+    const int c = e / (kTM * span);
+    const int r = e - c * kTM * span;
+    const int mi = r / span, yi = r - mi * span;
+    const int m = m0 + mi;
+    tile[c][mi][yi] = (m < M) ? u[c * comp + (size_t)m * Ny + ylo + yi] : cx<double>{0, 0};
+  }
+  __syncthreads();
+  const int mi = threadIdx.x % kTM, yi = threadIdx.x / kTM;
+  const int m = m0 + mi, y = y0 + yi;
+  if (m >= M || y >= Ny) return;
+  const cx<double> ux = tile[0][mi][y - ylo], uy = tile[1][mi][y - ylo], uz = tile[2][mi][y - ylo];
+  cx<double> dux{0, 0}, duz{0, 0};
+  const int j0 = d1j0[y];
+#pragma unroll
+  for (int k = 0; k < S1; ++k) {
+    const double w = d1w[y * S1 + k];
+    const cx<double> a = tile[0][mi][j0 + k - ylo], b = tile[2][mi][j0 + k - ylo];
+    dux.x += w * a.x;
+    dux.y += w * a.y;
+    duz.x += w * b.x;
+    duz.y += w * b.y;
+  }
+  const double k = mkz[m];
+  // flows/plane_couette.py:257-271
+  const cx<double> wx = {duz.x + k * uy.y, duz.y - k * uy.x};  // dy w - i kz v
+  const cx<double> wyp = {-k * ux.y, k * ux.x};                // i kz u
+  const cx<double> wzp = {-dux.x, -dux.y};                     // -dy u
+  const size_t f = (size_t)Ny * M;
+  const size_t o = (size_t)y * M + m;
+  v[0 * f + o] = {static_cast<T>(ux.x), static_cast<T>(ux.y)};
+  v[1 * f + o] = {static_cast<T>(uy.x), static_cast<T>(uy.y)};
+  v[2 * f + o] = {static_cast<T>(uz.x), static_cast<T>(uz.y)};
+  v[3 * f + o] = {static_cast<T>(wx.x), static_cast<T>(wx.y)};
+  v[4 * f + o] = {static_cast<T>(wyp.x), static_cast<T>(wyp.y)};
+  v[5 * f + o] = {static_cast<T>(wzp.x), static_cast<T>(wzp.y)};
+}
+
+// z-inverse of NF independent fields: (NF, Ny, Nkz, Nkx) -> (NF, Ny, Pz, Nkx).
+template <typename T, int P, int TC, int NF>
+__global__ void __launch_bounds__(TC * ColPlan<P>::threads)
+    k_z_inverse_plain(const cx<T>* __restrict__ in, cx<T>* __restrict__ out,
+                      const cx<T>* __restrict__ tw, int nz, int Ny, int Nx1) {
+  constexpr int N1 = ColPlan<P>::N1, N2 = ColPlan<P>::N2;
+  __shared__ cx<T> lds[TC * P];
+  __shared__ cx<T> tws[P];
+  for (int i = threadIdx.x; i < P; i += blockDim.x) tws[i] = tw[i];
+  __syncthreads();
+  const int tid = threadIdx.x, col = tid % TC, j = tid / TC;
+  const int ntile = (Nx1 + TC - 1) / TC;
+  const int y = blockIdx.x / ntile;
+  const int kx = (blockIdx.x - y * ntile) * TC + col;
+  const bool ok = kx < Nx1;
+  const int Nz1 = nz - 1;
+  const size_t in_f = (size_t)Ny * Nz1 * Nx1;
+  const size_t in_off = (size_t)y * Nz1 * Nx1 + kx;
+  const size_t out_f = (size_t)Ny * P * Nx1;
+  const size_t out_off = (size_t)y * P * Nx1 + kx;
+#pragma unroll 1
+  for (int f = 0; f < NF; ++f) {
+    if (j < N2) {
+      cx<T> x[N1];
+#pragma unroll
+      for (int a = 0; a < N1; ++a) {
+        const int n = N2 * a + j;
+        const int half = nz / 2;
+        const int s = n < half ? n : (n > P - half ? n - (P - nz + 1) : -1);
+        x[a] = (ok && s >= 0) ? in[f * in_f + in_off + (size_t)s * Nx1] : cx<T>{0, 0};
+      }
+      step_a_store<P, N1, N2, +1, TC>(x, j, col, lds, tws);
+    }
+    __syncthreads();
+    if (j < N1 && ok) {
+      cx<T> o[N2];
+      step_b_load<P, N1, N2, +1, TC>(o, j, col, lds);
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) out[f * out_f + out_off + (size_t)(j + N1 * k2) * Nx1] = o[k2];
+    }
+    __syncthreads();
+  }
+}
+
+// WT5: (3, Ny, Nkz, Nkx) T -> rhs (3, M, Ny) FP64, tiled transpose.
+template <typename T>
+__global__ void __launch_bounds__(kTY* kTM)
+    k_walls_to_spec(const cx<T>* __restrict__ w, cx<double>* __restrict__ rhs, int Ny, int M) {
+  __shared__ cx<double> tile[kTY][kTM + 1];
+  const int ntile_y = (Ny + kTY - 1) / kTY;
+  const int c = blockIdx.y;
+  const int y0 = (blockIdx.x % ntile_y) * kTY;
+  const int m0 = (blockIdx.x / ntile_y) * kTM;
+  const size_t f = (size_t)Ny * M;
+  {
+    const int mi = threadIdx.x % kTM, yi = threadIdx.x / kTM;
+    const int m = m0 + mi, y = y0 + yi;
+    if (m < M && y < Ny) {
+      const cx<T> v = w[c * f + (size_t)y * M + m];
+      tile[yi][mi] = {static_cast<double>(v.x), static_cast<double>(v.y)};
+    }
+  }
+  __syncthreads();
+  const int yi = threadIdx.x % kTY, mi = threadIdx.x / kTY;
+  const int m = m0 + mi, y = y0 + yi;
+  if (m < M && y < Ny) rhs[c * f + (size_t)m * Ny + y] = tile[yi][mi];
+}
+
+// ------------------------------------------------------------------ IMM kernel
+struct ImmArgs {
+  const cx<double>* un;   // velocity_n (3, M, Ny)
+  const cx<double>* uj;   // velocity_j
+  const cx<double>* nn;   // nonlin_n
+  const cx<double>* nj;   // nonlin_j
+  cx<double>* out;        // velocity_new (may alias uj)
+  cx<double>* corr;       // correction (may be null)
+  double* partials;       // per-block sum of quadrature-weighted k_metric |delta|^2
+  const double *d1w, *d2w, *qw;
+  const int *d1j0, *d2j0;
+  const double *lk_ab, *hk_ab;
+  const int *lk_piv, *hk_piv;
+  const double *v1, *v2, *q1, *q2, *minv, *mkx, *mkz;
+  cx<double>* scratch;  // 5 vectors [vec][y][mode]
+  int Ny, M, Nkx;
+  double dt, c, nu;
+};
+
+template <int PO>
+__global__ void __launch_bounds__(64) k_imm(ImmArgs a) {
+  constexpr int KL = PO, KU = PO, KUU = 2 * PO, ROWS = 3 * PO + 1, S1 = PO + 1, S2 = PO + 2;
+  __shared__ double red[2];
+  const int m = blockIdx.x * 64 + threadIdx.x;
+  const bool live = m < a.M;
+  const int Ny = a.Ny, M = a.M;
+  const size_t comp = (size_t)M * Ny;
+  double acc = 0.0;
+  if (live) {
+    const double kx = a.mkx[m], kz = a.mkz[m];
+    const double k2 = kx * kx + kz * kz;
+    const bool k2zero = (k2 == 0.0);
+    const double dt = a.dt, cc = a.c, nu = a.nu;
+    auto vec = [&](int v, int y) -> cx<double>& { return a.scratch[((size_t)v * Ny + y) * M + m]; };
+    auto at = [&](const cx<double>* f, int c, int y) { return f[c * comp + (size_t)m * Ny + y]; };
+    // D1 (or D2) stencil of component c of field f at row y
+    auto d1 = [&](const cx<double>* f, int c, int y) {
+      cx<double> s{0, 0};
+      const int j0 = a.d1j0[y];
+#pragma unroll
+      for (int k = 0; k < S1; ++k) {
+        const double w = a.d1w[y * S1 + k];
+        const cx<double> v = f[c * comp + (size_t)m * Ny + j0 + k];
+        s.x += w * v.x;
+        s.y += w * v.y;
+      }
+      return s;
+    };
+    auto d1v = [&](int v, int y) {
+      cx<double> s{0, 0};
+      const int j0 = a.d1j0[y];
+#pragma unroll
+      for (int k = 0; k < S1; ++k) {
+        const double w = a.d1w[y * S1 + k];
+        const cx<double> q = vec(v, j0 + k);
+        s.x += w * q.x;
+        s.y += w * q.y;
+      }
+      return s;
+    };
+    auto d2v = [&](int v, int y) {
+      cx<double> s{0, 0};
+      const int j0 = a.d2j0[y];
+#pragma unroll
+      for (int k = 0; k < S2; ++k) {
+        const double w = a.d2w[y * S2 + k];
+        const cx<double> q = vec(v, j0 + k);
+        s.x += w * q.x;
+        s.y += w * q.y;
+      }
+      return s;
+    };
+    auto d2f = [&](const cx<double>* f, int c, int y) {
+      cx<double> s{0, 0};
+      const int j0 = a.d2j0[y];
+#pragma unroll
+      for (int k = 0; k < S2; ++k) {
+        const double w = a.d2w[y * S2 + k];
+        const cx<double> q = f[c * comp + (size_t)m * Ny + j0 + k];
+        s.x += w * q.x;
+        s.y += w * q.y;
+      }
+      return s;
+    };
+    // i k * z
+    auto ik = [](double k, cx<double> z) { return cx<double>{-k * z.y, k * z.x}; };
+
+    // Stage 1a: d_hat^n (flows/plane_couette.py:427-428) -> vec 0
+    for (int y = 0; y < Ny; ++y) {
+      const cx<double> dv = d1(a.un, 1, y);
+      const cx<double> t0 = ik(kx, at(a.un, 0, y));
+      const cx<double> t2 = ik(kz, at(a.un, 2, y));
+      vec(0, y) = {(t0.x + dv.x) + t2.x, (t0.y + dv.y) + t2.y};
+    }
+    // Stage 1b + 2: f_hat (flows/plane_couette.py:431-445) streamed into the
+    // Lk forward sweep; forward result -> vec 1.
+    auto fhat = [&](int y) -> cx<double> {
+      if (y <= 0 || y >= Ny - 1) return {0, 0};
+      const cx<double> dh = vec(0, y);
+      const cx<double> dvj = d1(a.nj, 1, y), dvn = d1(a.nn, 1, y);
+      const cx<double> aj = ik(kx, at(a.nj, 0, y)), cj = ik(kz, at(a.nj, 2, y));
+      const cx<double> an = ik(kx, at(a.nn, 0, y)), cn = ik(kz, at(a.nn, 2, y));
+      const cx<double> divj = {(aj.x + dvj.x) + cj.x, (aj.y + dvj.y) + cj.y};
+      const cx<double> divn = {(an.x + dvn.x) + cn.x, (an.y + dvn.y) + cn.y};
+      const cx<double> d2d = d2v(0, y);
+      const cx<double> lkd = {d2d.x - k2 * dh.x, d2d.y - k2 * dh.y};
+      const double g = (1 - cc) * nu;
+      return {((dh.x / dt + cc * divj.x) + (1 - cc) * divn.x) + g * lkd.x,
+              ((dh.y / dt + cc * divj.y) + (1 - cc) * divn.y) + g * lkd.y};
+    };
+    {
+      cx<double> w[KL + 1];
+#pragma unroll
+      for (int i = 0; i <= KL; ++i) w[i] = (i < Ny) ? fhat(i) : cx<double>{0, 0};
+      for (int j = 0; j < Ny - 1; ++j) {
+        const int d = a.lk_piv[(size_t)j * M + m] - j;
+        const cx<double> w0 = w[0];
+#pragma unroll
+        for (int i = 1; i <= KL; ++i)
+          if (d == i) {
+            w[0] = w[i];
+            w[i] = w0;
+          }
+        const double* col = a.lk_ab + (size_t)j * ROWS * M + m;
+#pragma unroll
+        for (int i = 1; i <= KL; ++i) {
+          if (j + i < Ny) {
+            const double l = col[(size_t)(KL + KU + i) * M];
+            w[i].x -= l * w[0].x;
+            w[i].y -= l * w[0].y;
+          }
+        }
+        vec(1, j) = w[0];
+#pragma unroll
+        for (int i = 0; i < KL; ++i) w[i] = w[i + 1];
+        w[KL] = (j + KL + 1 < Ny) ? fhat(j + KL + 1) : cx<double>{0, 0};
+      }
+      vec(1, Ny - 1) = w[0];
+    }
+    // back substitution, in place (vec 1 = pP)
+    {
+      cx<double> z[KUU + 1];
+#pragma unroll
+      for (int i = 0; i <= KUU; ++i) z[KUU - i] = (Ny - 1 - i >= 0) ? vec(1, Ny - 1 - i) : cx<double>{0, 0};
+      for (int j = Ny - 1; j >= 0; --j) {
+        const double* col = a.lk_ab + (size_t)j * ROWS * M + m;
+        const double dg = col[(size_t)(KL + KU) * M];
+        const cx<double> x = {z[KUU].x / dg, z[KUU].y / dg};
+        vec(1, j) = x;
+#pragma unroll
+        for (int i = 1; i <= KUU; ++i) {
+          if (j - i >= 0) {
+            const double u = col[(size_t)(KUU - i) * M];
+            z[KUU - i].x -= u * x.x;
+            z[KUU - i].y -= u * x.y;
+          }
+        }
+#pragma unroll
+        for (int i = KUU; i > 0; --i) z[i] = z[i - 1];
+        z[0] = (j - KUU - 1 >= 0) ? vec(1, j - KUU - 1) : cx<double>{0, 0};
+      }
+    }
+    // Stage 3 (flows/plane_couette.py:449-472): R_c streamed into one Hk
+    // forward sweep for the three components -> vecs 2, 3, 4.
+    auto rhs3 = [&](int y, cx<double>* r) {
+      if (y <= 0 || y >= Ny - 1) {
+        r[0] = r[1] = r[2] = {0, 0};
+        return;
+      }
+      const cx<double> pp = vec(1, y);
+      const cx<double> dpp[3] = {ik(kx, pp), d1v(1, y), ik(kz, pp)};
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const cx<double> u = at(a.un, c, y);
+        const cx<double> du = d2f(a.un, c, y);
+        const cx<double> l = {du.x - k2 * u.x, du.y - k2 * u.y};
+        const double g = (1.0 - cc) * nu;
+        const cx<double> hm = {(1.0 / dt) * u.x + g * l.x, (1.0 / dt) * u.y + g * l.y};
+        const cx<double> Nj = at(a.nj, c, y), Nn = at(a.nn, c, y);
+        r[c] = {((hm.x - dpp[c].x) + cc * Nj.x) + (1 - cc) * Nn.x,
+                ((hm.y - dpp[c].y) + cc * Nj.y) + (1 - cc) * Nn.y};
+      }
+    };
+    {
+      cx<double> w[KL + 1][3];
+#pragma unroll
+      for (int i = 0; i <= KL; ++i) {
+        if (i < Ny) rhs3(i, w[i]);
+        else w[i][0] = w[i][1] = w[i][2] = {0, 0};
+      }
+      for (int j = 0; j < Ny - 1; ++j) {
+        const int d = a.hk_piv[(size_t)j * M + m] - j;
+        cx<double> w0[3] = {w[0][0], w[0][1], w[0][2]};
+#pragma unroll
+        for (int i = 1; i <= KL; ++i)
+          if (d == i) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+              w[0][c] = w[i][c];
+              w[i][c] = w0[c];
+            }
+          }
+        const double* col = a.hk_ab + (size_t)j * ROWS * M + m;
+#pragma unroll
+        for (int i = 1; i <= KL; ++i) {
+          if (j + i < Ny) {
+            const double l = col[(size_t)(KL + KU + i) * M];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+              w[i][c].x -= l * w[0][c].x;
+              w[i][c].y -= l * w[0][c].y;
+            }
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) vec(2 + c, j) = w[0][c];
+#pragma unroll
+        for (int i = 0; i < KL; ++i)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) w[i][c] = w[i + 1][c];
+        if (j + KL + 1 < Ny) rhs3(j + KL + 1, w[KL]);
+        else w[KL][0] = w[KL][1] = w[KL][2] = {0, 0};
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) vec(2 + c, Ny - 1) = w[0][c];
+    }
+    {
+      cx<double> z[KUU + 1][3];
+#pragma unroll
+      for (int i = 0; i <= KUU; ++i)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          z[KUU - i][c] = (Ny - 1 - i >= 0) ? vec(2 + c, Ny - 1 - i) : cx<double>{0, 0};
+      for (int j = Ny - 1; j >= 0; --j) {
+        const double* col = a.hk_ab + (size_t)j * ROWS * M + m;
+        const double dg = col[(size_t)(KL + KU) * M];
+        cx<double> x[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          x[c] = {z[KUU][c].x / dg, z[KUU][c].y / dg};
+          vec(2 + c, j) = x[c];
+        }
+#pragma unroll
+        for (int i = 1; i <= KUU; ++i) {
+          if (j - i >= 0) {
+            const double u = col[(size_t)(KUU - i) * M];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+              z[KUU - i][c].x -= u * x[c].x;
+              z[KUU - i][c].y -= u * x[c].y;
+            }
+          }
+        }
+#pragma unroll
+        for (int i = KUU; i > 0; --i)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) z[i][c] = z[i - 1][c];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          z[0][c] = (j - KUU - 1 >= 0) ? vec(2 + c, j - KUU - 1) : cx<double>{0, 0};
+      }
+    }
+    // Stages 4-5 (flows/plane_couette.py:476-486)
+    cx<double> dw0 = d1v(3, 0), dw1 = d1v(3, Ny - 1);
+    if (k2zero) dw0 = {0, 0};
+    const double m00 = a.minv[0 * (size_t)M + m], m01 = a.minv[1 * (size_t)M + m];
+    const double m10 = a.minv[2 * (size_t)M + m], m11 = a.minv[3 * (size_t)M + m];
+    const cx<double> al1 = {-(m00 * dw0.x + m01 * dw1.x), -(m00 * dw0.y + m01 * dw1.y)};
+    const cx<double> al2 = {-(m10 * dw0.x + m11 * dw1.x), -(m10 * dw0.y + m11 * dw1.y)};
+    // Stage 6 (flows/plane_couette.py:491-504) + norm partials
+    const double km = (m % a.Nkx == 0) ? 1.0 : 2.0;
+    for (int y = 0; y < Ny; ++y) {
+      const double v1 = a.v1[(size_t)y * M + m], v2 = a.v2[(size_t)y * M + m];
+      const double q1 = a.q1[(size_t)y * M + m], q2 = a.q2[(size_t)y * M + m];
+      const cx<double> va = vec(3, y);
+      const cx<double> vn = {(va.x + al1.x * v1) + al2.x * v2, (va.y + al1.y * v1) + al2.y * v2};
+      const cx<double> q = {al1.x * q1 + al2.x * q2, al1.y * q1 + al2.y * q2};
+      const cx<double> ua = vec(2, y), wa = vec(4, y);
+      const cx<double> iq_x = ik(kx, q), iq_z = ik(kz, q);
+      const cx<double> nw[3] = {{ua.x - iq_x.x, ua.y - iq_x.y}, vn, {wa.x - iq_z.x, wa.y - iq_z.y}};
+      double s = 0.0;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const size_t idx = c * comp + (size_t)m * Ny + y;
+        const cx<double> old = a.uj[idx];
+        const cx<double> d = {nw[c].x - old.x, nw[c].y - old.y};
+        a.out[idx] = nw[c];
+        if (a.corr) a.corr[idx] = d;
+        s += km * (d.x * d.x) + km * (d.y * d.y);
+      }
+      acc += a.qw[y] * s;
+    }
+  }
+  const double s = block_sum(acc, red);
+  if (threadIdx.x == 0 && a.partials) a.partials[blockIdx.x] = s;
+}
+
+// Quadrature-weighted norm of a field in (3, M, Ny) layout.
+__global__ void __launch_bounds__(256)
+    k_walls_norm(const cx<double>* __restrict__ f, const double* __restrict__ qw, int Ny, int M,
+                 int Nkx, double* __restrict__ partials) {
+  __shared__ double red[4];
+  const long long n = (long long)M * Ny;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  double acc = 0.0;
+  if (i < n) {
+    const int m = (int)(i / Ny), y = (int)(i - (long long)m * Ny);
+    const double km = (m % Nkx == 0) ? 1.0 : 2.0;
+    double s = 0.0;
+    for (int c = 0; c < 3; ++c) {
+      const cx<double> d = f[c * n + i];
+      s += km * (d.x * d.x) + km * (d.y * d.y);
+    }
+    acc = qw[y] * s;
+  }
+  const double s = block_sum(acc, red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// ------------------------------------------------------------------ launchers
+template <typename T>
+static int walls_rhs_t(Ctx* c, const double* state, double* rhs, hipStream_t s) {
+  WallTables* t = c->wt;
+  const int Ny = t->Ny, M = t->M;
+  cx<T>* A = static_cast<cx<T>*>(c->bufA);
+  cx<T>* B = static_cast<cx<T>*>(c->bufB);
+  const cx<T>* twz = static_cast<const cx<T>*>(c->fp32 ? c->tw32[2] : c->tw64[2]);
+  const int ntile_y = (Ny + kTY - 1) / kTY, ntile_m = (M + kTM - 1) / kTM;
+  {
+    ProfScope ps(c, KID_W_Z_INVERSE, s);
+    const dim3 grid(ntile_y * ntile_m);
+    const cx<double>* u = reinterpret_cast<const cx<double>*>(state);
+    switch (t->p) {
+#define CURL(PO)                                                                                \
+  case PO:                                                                                      \
+    hipLaunchKernelGGL((k_walls_curl_t<T, PO + 1>), grid, dim3(kTY * kTM), 0, s, u, A, t->d1w,  \
+                       t->d1j0, t->mkz, Ny, M);                                                 \
+    break;
+      CURL(2) CURL(4) CURL(6) CURL(8)
+#undef CURL
+      default: return c->fail(DNSMI_EINVAL, "fd_order");
+    }
+    int rc = check_launch(c, "walls_curl");
+    if (rc) return rc;
+    const int ntile = (c->Nx1 + 16 - 1) / 16;
+    switch (c->Pz) {
+#define X(P)                                                                                    \
+  case P:                                                                                       \
+    hipLaunchKernelGGL((k_z_inverse_plain<T, P, 16, 6>), dim3(Ny * ntile),                      \
+                       dim3(16 * ColPlan<P>::threads), 0, s, A, B, twz, c->nz, Ny, c->Nx1);     \
+    break;
+      DNSMI_FOR_EACH_P(X)
+#undef X
+      default: return c->fail(DNSMI_EINVAL, "unsupported padded z length");
+    }
+    rc = check_launch(c, "walls_z_inverse");
+    if (rc) return rc;
+  }
+  int rc = launch_x_pass<T>(c, B, A, c->base, c->cbase, c->nbase, Ny, KID_W_X_NONLINEAR, s);
+  if (rc) return rc;
+  rc = launch_z_forward_buf<T>(c, A, B, Ny, KID_W_Z_FORWARD, s);
+  if (rc) return rc;
+  {
+    ProfScope ps(c, KID_W_Z_FORWARD, s);
+    hipLaunchKernelGGL((k_walls_to_spec<T>), dim3(ntile_y * ntile_m, 3), dim3(kTY * kTM), 0, s, B,
+                       reinterpret_cast<cx<double>*>(rhs), Ny, M);
+  }
+  return check_launch(c, "walls_to_spec");
+}
+
+int walls_get_rhs(Ctx* c, const double* state, double* rhs, hipStream_t s) {
+  return c->fp32 ? walls_rhs_t<float>(c, state, rhs, s) : walls_rhs_t<double>(c, state, rhs, s);
+}
+
+static int walls_imm(Ctx* c, const double* un, const double* uj, const double* nn,
+                     const double* nj, double* out, double* corr, double* err, hipStream_t s) {
+  WallTables* t = c->wt;
+  const int nb = (t->M + 63) / 64;
+  if (nb > c->max_partials) return c->fail(DNSMI_EINVAL, "imm: too many blocks");
+  ImmArgs a{};
+  a.un = reinterpret_cast<const cx<double>*>(un);
+  a.uj = reinterpret_cast<const cx<double>*>(uj);
+  a.nn = reinterpret_cast<const cx<double>*>(nn);
+  a.nj = reinterpret_cast<const cx<double>*>(nj);
+  a.out = reinterpret_cast<cx<double>*>(out);
+  a.corr = reinterpret_cast<cx<double>*>(corr);
+  a.partials = c->partials;
+  a.d1w = t->d1w;
+  a.d2w = t->d2w;
+  a.qw = t->qw;
+  a.d1j0 = t->d1j0;
+  a.d2j0 = t->d2j0;
+  a.lk_ab = t->lk_ab;
+  a.hk_ab = t->hk_ab;
+  a.lk_piv = t->lk_piv;
+  a.hk_piv = t->hk_piv;
+  a.v1 = t->v1;
+  a.v2 = t->v2;
+  a.q1 = t->q1;
+  a.q2 = t->q2;
+  a.minv = t->minv;
+  a.mkx = t->mkx;
+  a.mkz = t->mkz;
+  a.scratch = reinterpret_cast<cx<double>*>(t->scratch);
+  a.Ny = t->Ny;
+  a.M = t->M;
+  a.Nkx = t->Nkx;
+  a.dt = c->cfg.dt;
+  a.c = c->cfg.implicitness;
+  a.nu = 1.0 / c->cfg.re;
+  {
+    ProfScope ps(c, KID_W_IMM, s);
+    switch (t->p) {
+      case 2: hipLaunchKernelGGL(k_imm<2>, dim3(nb), dim3(64), 0, s, a); break;
+      case 4: hipLaunchKernelGGL(k_imm<4>, dim3(nb), dim3(64), 0, s, a); break;
+      case 6: hipLaunchKernelGGL(k_imm<6>, dim3(nb), dim3(64), 0, s, a); break;
+      case 8: hipLaunchKernelGGL(k_imm<8>, dim3(nb), dim3(64), 0, s, a); break;
+      default: return c->fail(DNSMI_EINVAL, "fd_order");
+    }
+  }
+  int rc = check_launch(c, "imm");
+  if (rc || !err) return rc;
+  return finalize_norm(c, nb, err, s);
+}
+
+int walls_predict(Ctx* c, const double* state, const double* rhs, double* pred, hipStream_t s) {
+  return walls_imm(c, state, state, rhs, rhs, pred, nullptr, nullptr, s);
+}
+
+int walls_correct(Ctx* c, const double* state_prev, const double* pred_in,
+                  const double* rhs_prev, const double* rhs_next, double* pred_out,
+                  double* corr, hipStream_t s) {
+  return walls_imm(c, state_prev, pred_in, rhs_prev, rhs_next, pred_out, corr, nullptr, s);
+}
+
+int walls_norm(Ctx* c, const double* corr, double* err, hipStream_t s) {
+  WallTables* t = c->wt;
+  const long long n = (long long)t->M * t->Ny;
+  const int nb = (int)((n + 255) / 256);
+  if (nb > c->max_partials) return c->fail(DNSMI_EINVAL, "norm: too many blocks");
+  {
+    ProfScope ps(c, KID_W_NORM, s);
+    hipLaunchKernelGGL(k_walls_norm, dim3(nb), dim3(256), 0, s,
+                       reinterpret_cast<const cx<double>*>(corr), t->qw, t->Ny, t->M, t->Nkx,
+                       c->partials);
+  }
+  int rc = check_launch(c, "walls_norm");
+  if (rc) return rc;
+  return finalize_norm(c, nb, err, s);
+}
+
+int walls_predict_and_correct(Ctx* c, const double* state, double* pred, double* rhs_next,
+                              double* err, hipStream_t s) {
+  int rc = walls_get_rhs(c, state, c->rhs0, s);
+  if (rc) return rc;
+  if ((rc = walls_imm(c, state, state, c->rhs0, c->rhs0, pred, nullptr, nullptr, s))) return rc;
+  if ((rc = walls_get_rhs(c, pred, rhs_next, s))) return rc;
+  return walls_imm(c, state, pred, c->rhs0, rhs_next, pred, nullptr, err, s);
+}
+
+int walls_iterate_correction(Ctx* c, const double* state_prev, double* pred,
+                             const double* rhs_prev, double* rhs_next, double* err,
+                             hipStream_t s) {
+  // N_n := rhs_prev, the previous evaluation (flows/plane_couette.py:534-539)
+  if ((const void*)rhs_next == (const void*)rhs_prev) {
+    int rc = walls_get_rhs(c, pred, c->rhs0, s);
+    if (rc) return rc;
+    if ((rc = walls_imm(c, state_prev, pred, rhs_prev, c->rhs0, pred, nullptr, err, s))) return rc;
+    const hipError_t e = hipMemcpyAsync(rhs_next, c->rhs0, c->spec_elems * 16,
+                                        hipMemcpyDeviceToDevice, s);
+    return e == hipSuccess ? DNSMI_OK : c->fail(DNSMI_EHIP, hipGetErrorString(e));
+  }
+  int rc = walls_get_rhs(c, pred, rhs_next, s);
+  if (rc) return rc;
+  return walls_imm(c, state_prev, pred, rhs_prev, rhs_next, pred, nullptr, err, s);
+}
+
+// Host copies of the set-up tables in the reference layouts (for tests):
+// which = 0 Lk_ab, 1 Hk_ab, 2 v1, 3 q1, 4 M_inv, 5 ys (doubles) ; 6 Lk_piv,
+// 7 Hk_piv (ints written as doubles).
+int walls_table(Ctx* c, int which, double* out, long long n) {
+  WallTables* t = c->wt;
+  if (!t) return c->fail(DNSMI_EINVAL, "not a wall-bounded context");
+  const std::vector<double>* v = nullptr;
+  switch (which) {
+    case 0: v = &t->h_lk_ab; break;
+    case 1: v = &t->h_hk_ab; break;
+    case 2: v = &t->h_v1; break;
+    case 3: v = &t->h_q1; break;
+    case 4: v = &t->h_minv; break;
+    case 5: v = &t->h_ys; break;
+    case 6:
+    case 7: {
+      const std::vector<int>& pv = which == 6 ? t->h_lk_piv : t->h_hk_piv;
+      if (n != (long long)pv.size()) return c->fail(DNSMI_EINVAL, "table size mismatch");
+      for (size_t i = 0; i < pv.size(); ++i) out[i] = pv[i];
+      return DNSMI_OK;
+    }
+    default: return c->fail(DNSMI_EINVAL, "unknown table");
+  }
+  if (n != (long long)v->size()) return c->fail(DNSMI_EINVAL, "table size mismatch");
+  std::copy(v->begin(), v->end(), out);
+  return DNSMI_OK;
+}
+
 }  // namespace dnsmi

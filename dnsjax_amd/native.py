@@ -117,6 +117,15 @@ class NativeContext:
                                                     _ptr(rhs_prev), _ptr(rhs_next), _ptr(err), _stream()))
         return pred, rhs_next, err
 
+    def walls_table(self, which: int, shape: tuple):
+        """Host copy of a wall-bounded set-up table (see include/dnsmi.h)."""
+        import numpy as np
+
+        out = np.zeros(shape, dtype=np.float64)
+        self._chk(self.lib.dnsmi_walls_table(self.handle, which, out.ctypes.data_as(ctypes.c_void_p),
+                                             out.size))
+        return out
+
     # ---------------------------------------------------------------- timing
     def profile_begin(self) -> None:
         self._chk(self.lib.dnsmi_profile_begin(self.handle))

@@ -39,6 +39,12 @@ class Params:
     measure_corrections: bool = False
     precision: str = "fp64"
     tile_cols: int = 0
+    # Mixed form only: stop the corrector at the FP32 noise floor, i.e. when
+    # an iteration fails to shrink the error by this factor while the error
+    # is already below floor_cap * corrector_tolerance.  None keeps the
+    # reference's rule (iterate until error <= tolerance) unchanged.
+    corrector_floor_ratio: float | None = None
+    corrector_floor_cap: float = 100.0
 
     def __post_init__(self) -> None:
         if self.system not in PERIODIC_SYSTEMS + WALLED_SYSTEMS:
@@ -47,6 +53,8 @@ class Params:
             self.ly = 4.0 if self.system in PERIODIC_SYSTEMS else 2.0
         if self.precision not in ("fp64", "mixed"):
             raise ValueError("precision must be 'fp64' or 'mixed'")
+        if self.corrector_floor_ratio is None and self.precision == "mixed":
+            self.corrector_floor_ratio = 0.5
         if not self.re > 0 or not self.dt > 0 or not 0 <= self.implicitness <= 1:
             raise ValueError("need re > 0, dt > 0, 0 <= implicitness <= 1")
 

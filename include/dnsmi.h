@@ -115,6 +115,14 @@ int dnsmi_iterate_correction(dnsmi_ctx* ctx, const double* state_prev, double* p
  * the divergence correction (debug.measure_corrections). */
 int dnsmi_correct_velocity(dnsmi_ctx* ctx, double* state, double* err_dev, void* stream);
 
+/* Host copy (out: n host doubles) of a wall-bounded set-up table in the
+ * reference layout, for inspection (no counterpart in the reference, which
+ * keeps these as module attributes, flows/plane_couette.py:139-219):
+ * which = 0 Lk_ab (Nkz,Nkx,3p+1,Ny), 1 Hk_ab, 2 v1 (Nkz,Nkx,Ny), 3 q1,
+ * 4 M_inv (Nkz,Nkx,2,2), 5 y grid (Ny), 6 Lk_piv, 7 Hk_piv (0-based pivots
+ * as doubles).  Synchronous, host memory only. */
+int dnsmi_walls_table(dnsmi_ctx* ctx, int which, double* out, long long n);
+
 /* Per-kernel timing with HIP events on the launch stream (no counterpart in
  * the reference, whose bench.timer (bench.py:26-67) times whole calls).
  * Between begin and end every kernel launch of the context is bracketed by

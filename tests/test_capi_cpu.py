@@ -121,3 +121,15 @@ def test_driver_detects_nan():
     f = _ScriptedFlow([float("nan")], tol=1e-3, max_it=3)
     r = run(f, 0.0, 5)
     assert r.nan and not r.converged and r.counts == [0]
+
+
+def test_driver_noise_floor_rule_for_mixed():
+    f = _ScriptedFlow([1e-5, 1e-7, 2e-9, 1.6e-10, 1.3e-10, 1e-5, 1e-11], tol=1e-10, max_it=10)
+    f.params = Params(precision="mixed", corrector_tolerance=1e-10, max_corrector_iterations=10)
+    r = run(f, 0.0, 2)
+    assert r.counts == [4, 1] and r.floor_stops == 1 and r.converged
+    # the plain form keeps the reference rule
+    g = _ScriptedFlow([1e-5, 1e-7, 2e-9, 1.6e-10, 1.3e-10, 1.2e-10, 1.1e-10, 1.05e-10, 1.01e-10, 1.0e-10], tol=1e-10, max_it=10)
+    g.params = Params(corrector_tolerance=1e-10, max_corrector_iterations=10)
+    r = run(g, 0.0, 1)
+    assert r.counts == [9] and r.floor_stops == 0
