@@ -95,6 +95,18 @@ def gpu_form(precision: str, steps: int, warmup: int):
     }
 
 
+def measured_traffic(kernel: str, b: int, mode: str = "2"):
+    """HBM bytes per dispatch from the committed rocprofv3 PMC summary
+    (FETCH_SIZE / WRITE_SIZE passes scaled by a calibration copy), or None."""
+    path = os.path.join(ROOT, "profiles", "periodic_128_pmc_traffic.json")
+    if not os.path.exists(path):
+        return None
+    t = "double" if b == 16 else "float"
+    key = f"k_{kernel}<{t}, 192, 16>" if kernel != "y_forward" else f"k_y_forward<{t}, 192, 16, {mode}>"
+    ent = json.load(open(path))["kernels"].get(key)
+    return int(ent["traffic_bytes"]) if ent else None
+
+
 def roofline(form: dict, b: int) -> dict:
     kb = kernel_bytes(N, b)
     prof = form["prof"]
@@ -120,7 +132,8 @@ def roofline(form: dict, b: int) -> dict:
     rhs_gbs = rhs_model_bytes(N, b) / t_rhs / 1e9
     return {
         "dominant": {"bound": "hbm", "kernel": dom, "achieved": round(d["gbs"], 1), "peak": PEAK_GBS,
-                     "unit": "GB/s", "frac": round(d["gbs"] / PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(d["gbs"] / PEAK_GBS, 4),
+                     "traffic": measured_traffic(dom, b),
                      "avg_us": round(d["avg_us"], 2), "bytes_per_launch": int(d["bytes"])},
         "rhs": {"model_bytes": rhs_model_bytes(N, b), "t_us": round(t_rhs * 1e6, 2),
                 "achieved": round(rhs_gbs, 1), "frac": round(rhs_gbs / PEAK_GBS, 4)},
@@ -163,6 +176,36 @@ def cpu_baseline(budget_s: float) -> dict:
             "sample": f"{n} steps of the same 128^3 workload after 1 warm-up step, numpy FP64 oracle, 1 thread"}
 
 
+def profile_workload(steps: int) -> None:
+    """The workload profiled by rocprofv3 (kernel trace and PMC passes).
+
+    First a plain device-to-device copy of CAL_BYTES (the calibration
+    dispatch for the FETCH_SIZE / WRITE_SIZE counters: it must read and
+    write exactly CAL_BYTES), then `steps` steps of the FP64 and of the
+    mixed form at 128^3.
+    """
+    import torch
+
+    cal = 256 * 1024 * 1024
+    a = torch.ones(cal // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(f"calibration copy: {cal} bytes read, {cal} bytes written", flush=True)
+    del a, b
+    from dnsjax_amd.flows import TriplyPeriodicFlow
+    from dnsjax_amd.params import kolmogorov_2pi
+
+    for prec in ("fp64", "mixed"):
+        p = kolmogorov_2pi(N, RE, DT, corrector_tolerance=TOL, max_corrector_iterations=10,
+                           precision=prec)
+        flow = TriplyPeriodicFlow(p)
+        state = flow.init_state(seed=SEED, rms=RMS)
+        state, counts = run_steps(flow, state, steps)
+        torch.cuda.synchronize()
+        print(f"{prec}: {steps} steps, corrector iterations {counts}", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -170,11 +213,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--profile-workload", action="store_true",
+                    help="run the rocprofv3 workload (calibration copy + a few steps) and exit")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); the multi-GPU slab "
               "decomposition is a design note, not part of this build.", flush=True)
         sys.exit(2)
+    if a.profile_workload:
+        profile_workload(a.steps)
+        return
     os.environ.setdefault("OMP_NUM_THREADS", "1")
     fp64 = gpu_form("fp64", a.steps, a.warmup)
     mixed = gpu_form("mixed", a.steps, a.warmup)
