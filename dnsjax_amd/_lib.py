@@ -49,6 +49,7 @@ class DnsmiConfig(ctypes.Structure):
         ("precision", ctypes.c_int),
         ("correct_divergence", ctypes.c_int),
         ("tile_cols", ctypes.c_int),
+        ("variants", ctypes.c_int),
     ]
 
 
@@ -72,6 +73,7 @@ SIGNATURES = {
     "dnsmi_profile_end": (_I, [_P, _I, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_longlong)]),
     "dnsmi_kernel_name": (ctypes.c_char_p, [_I]),
     "dnsmi_walls_table": (_I, [_P, _I, _P, ctypes.c_longlong]),
+    "dnsmi_tuning": (ctypes.c_char_p, [_P]),
 }
 
 KERNEL_IDS = 12

@@ -104,7 +104,12 @@ int dnsmi_create(const dnsmi_config* cfg, dnsmi_ctx** out) {
     rc = c->fail(DNSMI_EINVAL, "need re > 0, dt > 0, 0 <= implicitness <= 1");
   } else {
     c->fp32 = cfg->precision == DNSMI_MIXED;
-    if (cfg->tile_cols > 0) c->tile_cols = cfg->tile_cols;
+    if (cfg->tile_cols > 0) {
+      c->tile_cols = cfg->tile_cols;
+      c->yinv_seq = (cfg->variants & 1) != 0;
+      c->zinv_seq = (cfg->variants & 2) != 0;
+      c->xpass_split = (cfg->variants & 4) != 0;
+    }
     if (cfg->system == DNSMI_PLANE_COUETTE) {
       c->walls = true;
       rc = dnsmi::walls_setup(c);
@@ -209,6 +214,8 @@ int dnsmi_correct_velocity(dnsmi_ctx* ctx, double* state, double* err_dev, void*
   }
   return dnsmi::periodic_correct_velocity(c, state, err_dev, as_stream(stream));
 }
+
+const char* dnsmi_tuning(const dnsmi_ctx* ctx) { return ctx ? ctx->impl.tuning.c_str() : ""; }
 
 int dnsmi_walls_table(dnsmi_ctx* ctx, int which, double* out, long long n) {
   DNSMI_CHECK_CTX(ctx)

@@ -68,7 +68,11 @@ struct Ctx {
   WallTables* wt = nullptr;
 
   int tile_cols = 16;
-  bool xpass_split = false;  // x-pass form (both are exact; the faster is chosen)
+  // kernel variants (all exact; the autotuner keeps the fastest)
+  bool xpass_split = false;
+  bool yinv_seq = false;
+  bool zinv_seq = false;
+  std::string tuning;
 
   // per-kernel timing (off unless dnsmi_profile_begin was called)
   bool prof = false;

@@ -20,6 +20,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <string>
 #include <vector>
 
 #include "context.hpp"
@@ -196,6 +197,132 @@ __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
       }
       __syncthreads();
     }
+  }
+}
+
+// K1 / K2, sequential form: one transform in flight per pass (fewer live
+// registers, higher occupancy); an input used by two outputs is read twice,
+// the second time from L2.  Same outputs as the paired forms above.
+template <typename T, int P, int TC>
+__global__ void __launch_bounds__(TC * ColPlan<P>::threads)
+    k_y_inverse_seq(const cx<double>* __restrict__ u, cx<T>* __restrict__ w1,
+                    const cx<T>* __restrict__ tw, const double* __restrict__ ky, int ny, int Nz1,
+                    int Nx1) {
+  constexpr int N1 = ColPlan<P>::N1, N2 = ColPlan<P>::N2;
+  __shared__ cx<T> lds[TC * P];
+  __shared__ cx<T> tws[P];
+  for (int i = threadIdx.x; i < P; i += blockDim.x) tws[i] = tw[i];
+  __syncthreads();
+  const int tid = threadIdx.x, col = tid % TC, j = tid / TC;
+  const int ntile = (Nx1 + TC - 1) / TC;
+  const int kz = blockIdx.x / ntile;
+  const int kx = (blockIdx.x - kz * ntile) * TC + col;
+  const bool ok = kx < Nx1;
+  const size_t plane = (size_t)Nz1 * Nx1;
+  const size_t in_comp = (size_t)(ny - 1) * plane;
+  const size_t out_f = (size_t)P * plane;
+  const size_t off = (size_t)kz * Nx1 + kx;
+  // pass -> (input component, multiply by i ky, output field)
+#pragma unroll 1
+  for (int pass = 0; pass < 5; ++pass) {
+    const int c = pass < 2 ? 0 : (pass < 4 ? 2 : 1);
+    const bool iky = (pass == 1 || pass == 3);
+    const int f = pass == 0 ? 0 : (pass == 1 ? 4 : (pass == 2 ? 2 : (pass == 3 ? 3 : 1)));
+    if (j < N2) {
+      cx<T> x[N1];
+#pragma unroll
+      for (int a = 0; a < N1; ++a) {
+        const int s = stored_of_padded(N2 * a + j, ny, P);
+        cx<T> v{0, 0};
+        if (ok && s >= 0) {
+          const cx<double> d = u[c * in_comp + (size_t)s * plane + off];
+          if (iky) {
+            const double k = ky[s];
+            v = {static_cast<T>(-k * d.y), static_cast<T>(k * d.x)};
+          } else {
+            v = to_t<T>(d);
+          }
+        }
+        x[a] = v;
+      }
+      step_a_store<P, N1, N2, +1, TC>(x, j, col, lds, tws);
+    }
+    __syncthreads();
+    if (j < N1 && ok) {
+      cx<T> y[N2];
+      step_b_load<P, N1, N2, +1, TC>(y, j, col, lds);
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) w1[f * out_f + (size_t)(j + N1 * k2) * plane + off] = y[k2];
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T, int P, int TC>
+__global__ void __launch_bounds__(TC * ColPlan<P>::threads)
+    k_z_inverse_seq(const cx<T>* __restrict__ w1, cx<T>* __restrict__ w2,
+                    const cx<T>* __restrict__ tw, const double* __restrict__ kz_tab, int nz, int Py,
+                    int Nx1) {
+  constexpr int N1 = ColPlan<P>::N1, N2 = ColPlan<P>::N2;
+  __shared__ cx<T> lds[TC * P];
+  __shared__ cx<T> tws[P];
+  for (int i = threadIdx.x; i < P; i += blockDim.x) tws[i] = tw[i];
+  __syncthreads();
+  const int tid = threadIdx.x, col = tid % TC, j = tid / TC;
+  const int ntile = (Nx1 + TC - 1) / TC;
+  const int y = blockIdx.x / ntile;
+  const int kx = (blockIdx.x - y * ntile) * TC + col;
+  const bool ok = kx < Nx1;
+  const int Nz1 = nz - 1;
+  const size_t in_f = (size_t)Py * Nz1 * Nx1;
+  const size_t in_off = (size_t)y * Nz1 * Nx1 + kx;
+  const size_t out_f = (size_t)Py * P * Nx1;
+  const size_t out_off = (size_t)y * P * Nx1 + kx;
+  // pass: 0 Z(F0)->0, 1 Z(i kz F0)->4, 2 Z(F1)->1, 3 Z(F3 - i kz F1)->3,
+  //       4 Z(F2)->2, 5 Z(-F4)->5
+#pragma unroll 1
+  for (int pass = 0; pass < 6; ++pass) {
+    const int fo = pass == 0 ? 0 : (pass == 1 ? 4 : (pass == 2 ? 1 : (pass == 3 ? 3 : (pass == 4 ? 2 : 5))));
+    if (j < N2) {
+      cx<T> x[N1];
+#pragma unroll
+      for (int a = 0; a < N1; ++a) {
+        const int s = stored_of_padded(N2 * a + j, nz, P);
+        cx<T> v{0, 0};
+        if (ok && s >= 0) {
+          const T k = static_cast<T>(kz_tab[s]);
+          const size_t e = in_off + (size_t)s * Nx1;
+          if (pass == 0) {
+            v = w1[0 * in_f + e];
+          } else if (pass == 1) {
+            const cx<T> f0 = w1[0 * in_f + e];
+            v = {-k * f0.y, k * f0.x};
+          } else if (pass == 2) {
+            v = w1[1 * in_f + e];
+          } else if (pass == 3) {
+            const cx<T> f1 = w1[1 * in_f + e];
+            const cx<T> f3 = w1[3 * in_f + e];
+            v = {f3.x + k * f1.y, f3.y - k * f1.x};
+          } else if (pass == 4) {
+            v = w1[2 * in_f + e];
+          } else {
+            const cx<T> f4 = w1[4 * in_f + e];
+            v = {-f4.x, -f4.y};
+          }
+        }
+        x[a] = v;
+      }
+      step_a_store<P, N1, N2, +1, TC>(x, j, col, lds, tws);
+    }
+    __syncthreads();
+    if (j < N1 && ok) {
+      cx<T> o[N2];
+      step_b_load<P, N1, N2, +1, TC>(o, j, col, lds);
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2)
+        w2[fo * out_f + out_off + (size_t)(j + N1 * k2) * Nx1] = o[k2];
+    }
+    __syncthreads();
   }
 }
 
@@ -758,13 +885,17 @@ static int launch_y_inverse(Ctx* c, const double* state, hipStream_t s) {
   Bufs<T> b = bufs_of<T>(c);
   const int ntile = (c->Nx1 + kTC - 1) / kTC;
   const dim3 grid(c->Nz1 * ntile);
+  const cx<double>* u = reinterpret_cast<const cx<double>*>(state);
   ProfScope ps(c, KID_Y_INVERSE, s);
   switch (c->Py) {
-#define X(P)                                                                                 \
-  case P:                                                                                    \
-    hipLaunchKernelGGL((k_y_inverse<T, P, kTC>), grid, dim3(kTC * ColPlan<P>::threads), 0, s,  \
-                       reinterpret_cast<const cx<double>*>(state), b.A, b.twy, c->ky, c->ny, \
-                       c->Nz1, c->Nx1);                                                      \
+#define X(P)                                                                                    \
+  case P:                                                                                       \
+    if (c->yinv_seq)                                                                            \
+      hipLaunchKernelGGL((k_y_inverse_seq<T, P, kTC>), grid, dim3(kTC * ColPlan<P>::threads), 0, \
+                         s, u, b.A, b.twy, c->ky, c->ny, c->Nz1, c->Nx1);                       \
+    else                                                                                        \
+      hipLaunchKernelGGL((k_y_inverse<T, P, kTC>), grid, dim3(kTC * ColPlan<P>::threads), 0, s,  \
+                         u, b.A, b.twy, c->ky, c->ny, c->Nz1, c->Nx1);                          \
     break;
     DNSMI_FOR_EACH_P(X)
 #undef X
@@ -780,10 +911,14 @@ static int launch_z_inverse(Ctx* c, hipStream_t s) {
   const dim3 grid(c->Py * ntile);
   ProfScope ps(c, KID_Z_INVERSE, s);
   switch (c->Pz) {
-#define X(P)                                                                                 \
-  case P:                                                                                    \
-    hipLaunchKernelGGL((k_z_inverse<T, P, kTC>), grid, dim3(kTC * ColPlan<P>::threads), 0, s, b.A,   \
-                       b.B, b.twz, c->kz, c->nz, c->Py, c->Nx1);                             \
+#define X(P)                                                                                    \
+  case P:                                                                                       \
+    if (c->zinv_seq)                                                                            \
+      hipLaunchKernelGGL((k_z_inverse_seq<T, P, kTC>), grid, dim3(kTC * ColPlan<P>::threads), 0, \
+                         s, b.A, b.B, b.twz, c->kz, c->nz, c->Py, c->Nx1);                      \
+    else                                                                                        \
+      hipLaunchKernelGGL((k_z_inverse<T, P, kTC>), grid, dim3(kTC * ColPlan<P>::threads), 0, s,  \
+                         b.A, b.B, b.twz, c->kz, c->nz, c->Py, c->Nx1);                         \
     break;
     DNSMI_FOR_EACH_P(X)
 #undef X
@@ -1107,6 +1242,65 @@ std::vector<double> complex_wavenumbers(int n, double L) {
   return k;
 }
 
+// ------------------------------------------------------------------ autotuner
+// Every variant below computes the same result; at context creation each
+// alternative is timed on the context's own work buffers (HIP events, a
+// warm-up launch then `reps` launches) and the fastest is kept.  The choices
+// are readable through dnsmi_tuning().
+template <typename F>
+static double time_launches(F&& launch, int reps) {
+  hipEvent_t a = nullptr, b = nullptr;
+  if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return 1e30;
+  launch();
+  (void)hipEventRecord(a, nullptr);
+  for (int i = 0; i < reps; ++i) launch();
+  (void)hipEventRecord(b, nullptr);
+  float ms = 0.f;
+  if (hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&ms, a, b) != hipSuccess) ms = 1e30f;
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return ms;
+}
+
+template <typename T>
+static int periodic_autotune_t(Ctx* c) {
+  const int reps = 3;
+  std::string log;
+  auto pick = [&](const char* what, bool& flag, auto&& launch) {
+    flag = false;
+    const double t0 = time_launches(launch, reps);
+    flag = true;
+    const double t1 = time_launches(launch, reps);
+    flag = t1 < t0;
+    char buf[160];
+    snprintf(buf, sizeof buf, "%s=%s(%.1f/%.1f us) ", what, flag ? "alt" : "base",
+             1e3 * t0 / reps, 1e3 * t1 / reps);
+    log += buf;
+  };
+  const double* state = c->rhs0;  // zeroed scratch stands in for a state
+  pick("y_inverse_seq", c->yinv_seq, [&] { launch_y_inverse<T>(c, state, nullptr); });
+  pick("z_inverse_seq", c->zinv_seq, [&] { launch_z_inverse<T>(c, nullptr); });
+  bool split_ok = false;
+  switch (c->Px) {
+#define X(P) case P: split_ok = XSplit<P>::value; break;
+    DNSMI_FOR_EACH_P(X)
+#undef X
+    default: break;
+  }
+  if (split_ok) pick("x_split", c->xpass_split, [&] { launch_x_nonlinear<T>(c, nullptr); });
+  c->tuning = log;
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? DNSMI_OK : c->fail(DNSMI_EHIP, std::string("autotune: ") + hipGetErrorString(e));
+}
+
+int periodic_autotune(Ctx* c) {
+  if (hipMemset(c->bufA, 0, c->bufA_bytes) != hipSuccess ||
+      hipMemset(c->bufB, 0, c->bufB_bytes) != hipSuccess ||
+      hipMemset(c->rhs0, 0, c->spec_elems * 16) != hipSuccess)
+    return c->fail(DNSMI_EHIP, "autotune: memset");
+  return c->fp32 ? periodic_autotune_t<float>(c) : periodic_autotune_t<double>(c);
+}
+
 int periodic_setup(Ctx* c) {
   const dnsmi_config& g = c->cfg;
   if (g.nx % 2 || g.ny % 2 || g.nz % 2 || g.nx < 4 || g.ny < 4 || g.nz < 4)
@@ -1177,6 +1371,7 @@ int periodic_setup(Ctx* c) {
   if ((rc = upload_doubles(c, &c->base, base))) return rc;
   if ((rc = upload_doubles(c, &c->cbase, cbase))) return rc;
   if ((rc = upload_doubles(c, &c->nbase, nbase))) return rc;
+  if (c->cfg.tile_cols == 0) return periodic_autotune(c);
   return DNSMI_OK;
 }
 

@@ -38,6 +38,7 @@ class NativeContext:
             tilt_degree=p.tilt_degree, fd_order=p.fd_order,
             precision=_lib.DNSMI_MIXED if p.precision == "mixed" else _lib.DNSMI_FP64,
             correct_divergence=int(p.correct_divergence), tile_cols=p.tile_cols,
+            variants=p.variants,
         )
         handle = ctypes.c_void_p()
         torch.cuda.current_device()  # make the device current before any HIP call
@@ -49,7 +50,10 @@ class NativeContext:
     def __del__(self):
         h = getattr(self, "handle", None)
         if h is not None and h.value:
-            torch.cuda.synchronize()
+            try:
+                torch.cuda.synchronize()
+            except Exception:  # interpreter teardown: the device is being released anyway
+                return
             self.lib.dnsmi_destroy(h)
             self.handle = None
 
@@ -116,6 +120,10 @@ class NativeContext:
         self._chk(self.lib.dnsmi_iterate_correction(self.handle, _ptr(state_prev), _ptr(pred),
                                                     _ptr(rhs_prev), _ptr(rhs_next), _ptr(err), _stream()))
         return pred, rhs_next, err
+
+    @property
+    def tuning(self) -> str:
+        return self.lib.dnsmi_tuning(self.handle).decode()
 
     def walls_table(self, which: int, shape: tuple):
         """Host copy of a wall-bounded set-up table (see include/dnsmi.h)."""

@@ -38,7 +38,8 @@ class Params:
     correct_divergence: bool = True
     measure_corrections: bool = False
     precision: str = "fp64"
-    tile_cols: int = 0
+    tile_cols: int = 0  # 0: autotune kernel variants; else use `variants`
+    variants: int = 0
     # Mixed form only: stop the corrector at the FP32 noise floor, i.e. when
     # an iteration fails to shrink the error by this factor while the error
     # is already below floor_cap * corrector_tolerance.  None keeps the

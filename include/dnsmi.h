@@ -71,7 +71,10 @@ typedef struct dnsmi_config {
   int fd_order;            /* walls: stencil order p; parameters.py:62 */
   int precision;           /* DNSMI_FP64 or DNSMI_MIXED */
   int correct_divergence;  /* parameters.py:108 */
-  int tile_cols;           /* columns per workgroup of the y/z passes; 0 = default */
+  int tile_cols;           /* 0: autotune the kernel variants at create;
+                              otherwise use `variants` as given */
+  int variants;            /* bit mask when tile_cols != 0: 1 sequential
+                              y-inverse, 2 sequential z-inverse, 4 split x pass */
 } dnsmi_config;
 
 typedef struct dnsmi_ctx dnsmi_ctx;
@@ -114,6 +117,10 @@ int dnsmi_iterate_correction(dnsmi_ctx* ctx, const double* state_prev, double* p
  * pass-through for walls.  If err_dev is not NULL it receives the norm of
  * the divergence correction (debug.measure_corrections). */
 int dnsmi_correct_velocity(dnsmi_ctx* ctx, double* state, double* err_dev, void* stream);
+
+/* The kernel variants the autotuner chose at create (tile_cols == 0), as
+ * "name=base|alt(us base/us alt) ..." (no counterpart in the reference). */
+const char* dnsmi_tuning(const dnsmi_ctx* ctx);
 
 /* Host copy (out: n host doubles) of a wall-bounded set-up table in the
  * reference layout, for inspection (no counterpart in the reference, which

@@ -157,3 +157,22 @@ def test_rejects_bad_configs_and_aliasing():
         flow._native.predict_and_correct(u, pred=u)
     with pytest.raises(ValueError):
         flow.get_rhs(u.to(torch.complex64))
+
+
+@pytest.mark.parametrize("variants", [0, 1, 2, 4, 7])
+@pytest.mark.parametrize("case", ["periodic_16_kolmogorov", "periodic_16x8x32_waleffe_ly4"])
+def test_every_kernel_variant_matches(case, variants):
+    """Each exact kernel variant (as the autotuner may pick) on the golden step."""
+    g = np.load(os.path.join(GOLDEN, case + ".npz"))
+    for prec, tol in (("fp64", 1e-12), ("mixed", 1e-5)):
+        flow = make(case, tile_cols=16, variants=variants, precision=prec)
+        u = flow.to_device(g["state"])
+        assert rel(flow.get_rhs(u), g["rhs0"]) < tol
+        pred, rhs1, err1 = flow.predict_and_correct(u)
+        assert rel(pred, g["pred1"]) < tol
+
+
+def test_autotuner_reports_choices():
+    flow = make("periodic_16_kolmogorov")
+    t = flow._native.tuning
+    assert "y_inverse_seq=" in t and "z_inverse_seq=" in t and "x_split=" in t
