@@ -74,6 +74,21 @@ SIGNATURES = {
     "dnsmi_kernel_name": (ctypes.c_char_p, [_I]),
     "dnsmi_walls_table": (_I, [_P, _I, _P, ctypes.c_longlong]),
     "dnsmi_tuning": (ctypes.c_char_p, [_P]),
+    "dnsmi_get_stats": (_I, [_P, _P, _P, _P]),
+    "dnsmi_step_device": (_I, [_P, _P, _P, _P, _P, _I, _P, _I, ctypes.c_double, _I, ctypes.c_double,
+                               ctypes.c_double, _P]),
+    "dnsmi_step_stats": (_I, [_P, ctypes.POINTER(ctypes.c_double)]),
+    "dnsmi_step_reset": (_I, [_P]),
+    "dnsmi_mlp_sizes": (_I, [_I, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong),
+                             ctypes.POINTER(ctypes.c_longlong)]),
+    "dnsmi_mlp_grad": (_I, [_I, _I, _P, _P, _P, _I, _P, _P, _P, _P]),
+    "dnsmi_mlp_adam": (_I, [_I, _P, _P, _P, _P, _I, ctypes.c_float, _P]),
+    "dnsmi_mlp_pack": (_I, [_I, _P, _P, _P, _P]),
+    "dnsmi_mlp_forward": (_I, [_I, _P, _I, _P, _P, _P, _P]),
+    "dnsmi_mlp_features": (_I, [_P, _P, _P, _P, _P, _P, _I, ctypes.c_float, ctypes.c_float,
+                                ctypes.c_float, _P, _P, _P]),
+    "dnsmi_guess": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _I, ctypes.c_float, ctypes.c_float,
+                         ctypes.c_float, _I, _P]),
 }
 
 KERNEL_IDS = 12

@@ -4,6 +4,7 @@
 
 #include <cstring>
 #include <new>
+#include <cstdint>
 #include <string>
 
 #include "context.hpp"
@@ -63,6 +64,26 @@ int walls_iterate_correction(Ctx* c, const double* state_prev, double* pred,
                              const double* rhs_prev, double* rhs_next, double* err,
                              hipStream_t s);
 int walls_table(Ctx* c, int which, double* out, long long n);
+int get_stats(Ctx* c, const double* state, double* out, hipStream_t s);
+int step_device(Ctx* c, const double* state, double* pred, double* rhs_a, double* rhs_b, int slots,
+                int* counts, int counts_cap, double tol, int max_iter, double floor_ratio,
+                double floor_cap, hipStream_t s);
+int step_read_ctl(Ctx* c, double* out6);
+int step_alloc_ctl(Ctx* c);
+int mlp_grad(int H, int B, const float* X, const float* T, const float* prm, int bf16, float* part,
+             float* grad, float* loss, hipStream_t s);
+int mlp_adam(int H, float* prm, const float* g, float* m, float* v, int step, float lr,
+             hipStream_t s);
+int mlp_pack(int H, const float* prm, uint16_t* wb, float* bias, hipStream_t s);
+int mlp_forward(int H, const float* X, int B, const uint16_t* wb, const float* bias, float* Y,
+                hipStream_t s);
+int mlp_features(Ctx* c, const double* nm2, const double* nm1, const double* ncur,
+                 const double* nnext, const long long* modes, int ns, float s_1, float s_2,
+                 float s_out, float* X, float* T, hipStream_t s);
+int guess_periodic(Ctx* c, const double* state, double* pred, double* rhs_prev, const float* h1,
+                   float* h2, const uint16_t* wb, const float* bias, int H, float s_1, float s_2,
+                   float s_out, int mode, hipStream_t s);
+int step_reset_ctl(Ctx* c);
 
 }  // namespace dnsmi
 
@@ -116,6 +137,7 @@ int dnsmi_create(const dnsmi_config* cfg, dnsmi_ctx** out) {
     } else {
       rc = dnsmi::periodic_setup(c);
     }
+    if (rc == DNSMI_OK) rc = dnsmi::step_alloc_ctl(c);
   }
   if (rc != DNSMI_OK) {
     dnsmi::g_create_error = c->err;
@@ -213,6 +235,110 @@ int dnsmi_correct_velocity(dnsmi_ctx* ctx, double* state, double* err_dev, void*
     return DNSMI_OK;
   }
   return dnsmi::periodic_correct_velocity(c, state, err_dev, as_stream(stream));
+}
+
+int dnsmi_get_stats(dnsmi_ctx* ctx, const double* state, double* stats_dev, void* stream) {
+  DNSMI_CHECK_CTX(ctx)
+  DNSMI_REQUIRE(state && stats_dev, "get_stats: null pointer");
+  return dnsmi::get_stats(c, state, stats_dev, as_stream(stream));
+}
+
+int dnsmi_step_device(dnsmi_ctx* ctx, const double* state, double* next, double* rhs_a,
+                      double* rhs_b, int slots, int* counts, int counts_cap, double tol,
+                      int max_iter, double floor_ratio, double floor_cap, void* stream) {
+  DNSMI_CHECK_CTX(ctx)
+  DNSMI_REQUIRE(state && next && rhs_a && rhs_b && slots >= 0 && max_iter >= 0,
+                "step_device: bad arguments");
+  DNSMI_REQUIRE((const void*)state != next && rhs_a != rhs_b && (void*)next != rhs_a &&
+                    (void*)next != rhs_b && (const void*)state != rhs_a &&
+                    (const void*)state != rhs_b,
+                "step_device: buffers must be distinct");
+  return dnsmi::step_device(c, state, next, rhs_a, rhs_b, slots, counts, counts_cap, tol, max_iter,
+                            floor_ratio, floor_cap, as_stream(stream));
+}
+
+int dnsmi_step_stats(dnsmi_ctx* ctx, double* out6) {
+  DNSMI_CHECK_CTX(ctx)
+  DNSMI_REQUIRE(out6, "step_stats: null output");
+  return dnsmi::step_read_ctl(c, out6);
+}
+
+int dnsmi_step_reset(dnsmi_ctx* ctx) {
+  DNSMI_CHECK_CTX(ctx)
+  return dnsmi::step_reset_ctl(c);
+}
+
+// Context-free entry points report through dnsmi_last_error(NULL).
+static int ctxfree(int rc, const char* what) {
+  if (rc == DNSMI_EINVAL)
+    dnsmi::g_create_error = std::string(what) + ": invalid argument (hidden width must be 32 or 64)";
+  else if (rc)
+    dnsmi::g_create_error = std::string(what) + ": kernel launch failed";
+  return rc;
+}
+
+int dnsmi_mlp_sizes(int H, long long* nparams, long long* packed_weight_bytes,
+                    long long* bias_floats) {
+  if (H != 32 && H != 64) return ctxfree(DNSMI_EINVAL, "mlp_sizes");
+  const long long nh = H / 16;
+  if (nparams) *nparams = 16LL * H + H + (long long)H * H + H + 6LL * H + 6;
+  if (packed_weight_bytes) *packed_weight_bytes = (2 * nh + nh * nh) * 64 * 4 * 2;
+  if (bias_floats) *bias_floats = 2LL * H + 16;
+  return DNSMI_OK;
+}
+
+int dnsmi_mlp_grad(int H, int B, const float* X, const float* T, const float* params, int bf16,
+                   float* part, float* grad, float* loss_dev, void* stream) {
+  if (!X || !T || !params || !part || !grad || B <= 0) return ctxfree(DNSMI_EINVAL, "mlp_grad");
+  return ctxfree(dnsmi::mlp_grad(H, B, X, T, params, bf16, part, grad, loss_dev, as_stream(stream)),
+                 "mlp_grad");
+}
+
+int dnsmi_mlp_adam(int H, float* params, const float* grad, float* m, float* v, int step, float lr,
+                   void* stream) {
+  if (!params || !grad || !m || !v || step < 1) return ctxfree(DNSMI_EINVAL, "mlp_adam");
+  return ctxfree(dnsmi::mlp_adam(H, params, grad, m, v, step, lr, as_stream(stream)), "mlp_adam");
+}
+
+int dnsmi_mlp_pack(int H, const float* params, void* packed_weights, float* bias, void* stream) {
+  if (!params || !packed_weights || !bias) return ctxfree(DNSMI_EINVAL, "mlp_pack");
+  return ctxfree(dnsmi::mlp_pack(H, params, static_cast<uint16_t*>(packed_weights), bias,
+                                 as_stream(stream)),
+                 "mlp_pack");
+}
+
+int dnsmi_mlp_forward(int H, const float* X, int B, const void* packed_weights, const float* bias,
+                      float* Y, void* stream) {
+  if (!X || !packed_weights || !bias || !Y || B < 0) return ctxfree(DNSMI_EINVAL, "mlp_forward");
+  return ctxfree(dnsmi::mlp_forward(H, X, B, static_cast<const uint16_t*>(packed_weights), bias, Y,
+                                    as_stream(stream)),
+                 "mlp_forward");
+}
+
+int dnsmi_mlp_features(dnsmi_ctx* ctx, const double* n_m2, const double* n_m1,
+                       const double* n_cur, const double* n_next, const long long* modes,
+                       int nsamples, float s_1, float s_2, float s_out, float* X, float* T,
+                       void* stream) {
+  DNSMI_CHECK_CTX(ctx)
+  DNSMI_REQUIRE(n_m2 && n_m1 && n_cur && modes && X && nsamples >= 0,
+                "mlp_features: bad arguments");
+  DNSMI_REQUIRE(!T == !n_next, "mlp_features: targets need n_next and T together");
+  return dnsmi::mlp_features(c, n_m2, n_m1, n_cur, n_next, modes, nsamples, s_1, s_2, s_out, X, T,
+                             as_stream(stream));
+}
+
+int dnsmi_guess(dnsmi_ctx* ctx, const double* state, double* pred, double* rhs_prev,
+                const float* hist1, float* hist2, const void* packed_weights, const float* bias,
+                int H, float s_1, float s_2, float s_out, int mode, void* stream) {
+  DNSMI_CHECK_CTX(ctx)
+  DNSMI_REQUIRE(state && pred && rhs_prev && hist1 && hist2, "guess: null pointer");
+  DNSMI_REQUIRE(mode != 2 || (packed_weights && bias), "guess: network weights missing");
+  DNSMI_REQUIRE((const void*)state != pred && (const void*)state != rhs_prev &&
+                    (void*)pred != rhs_prev && (const void*)hist1 != hist2,
+                "guess: buffers must be distinct");
+  return dnsmi::guess_periodic(c, state, pred, rhs_prev, hist1, hist2,
+                               static_cast<const uint16_t*>(packed_weights), bias, H, s_1, s_2,
+                               s_out, mode, as_stream(stream));
 }
 
 const char* dnsmi_tuning(const dnsmi_ctx* ctx) { return ctx ? ctx->impl.tuning.c_str() : ""; }

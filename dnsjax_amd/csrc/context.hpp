@@ -67,6 +67,11 @@ struct Ctx {
 
   WallTables* wt = nullptr;
 
+  // Device-side corrector control (graph-capturable steps): when `act` is
+  // set, every step kernel first reads *act and skips its launch if 0.
+  const int* act = nullptr;
+  int* ctl = nullptr;  // device control block (see step.hip)
+
   int tile_cols = 16;
   // kernel variants (all exact; the autotuner keeps the fastest)
   bool xpass_split = false;

@@ -279,6 +279,34 @@ __device__ __forceinline__ void row_step_a_store(cx<T>* x, int b, cx<T>* row_lds
   }
 }
 
+// Same, with the twiddles of this plan laid out per k1: tab[k1 * N2 + b] =
+// w^(b k1).  Lanes with consecutive b read consecutive entries, so the
+// reads do not collide in LDS banks (the flat table's tw[b k1] strides by k1).
+template <int P, int N1, int N2, int DIR, typename T>
+__device__ __forceinline__ void row_step_a_store_tab(cx<T>* x, int b, cx<T>* row_lds,
+                                                     const cx<T>* tab) {
+  Dft<N1, DIR, T>::run(x);
+#pragma unroll
+  for (int k1 = 0; k1 < N1; ++k1) {
+    cx<T> v = x[k1];
+    if (k1 != 0) {
+      cx<T> w = tab[k1 * N2 + b];
+      if (DIR < 0) w.y = -w.y;
+      v = cmul(v, w);
+    }
+    row_lds[k1 * (N2 + 1) + b] = v;
+  }
+}
+
+// Fill tab[k1 * N2 + b] = tw[b k1] (k1 < N1, b < N2) cooperatively.
+template <int N1, int N2, typename T>
+__device__ __forceinline__ void fill_plan_twiddles(cx<T>* tab, const cx<T>* __restrict__ tw) {
+  for (int i = threadIdx.x; i < N1 * N2; i += blockDim.x) {
+    const int k1 = i / N2, b = i - k1 * N2;
+    tab[i] = tw[b * k1];
+  }
+}
+
 template <int P, int N1, int N2, int DIR, typename T>
 __device__ __forceinline__ void row_step_b_load(cx<T>* y, int k1, const cx<T>* row_lds) {
 #pragma unroll

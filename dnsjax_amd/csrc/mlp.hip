@@ -1,0 +1,610 @@
+// Per-mode corrector-guess network: training and inference kernels.
+//
+// The network (no reference counterpart) maps 16 features of one spectral
+// mode to 6 outputs: x -> silu(W1^T x + b1) -> silu(W2^T h1 + b2) ->
+// W3^T h2 + b3, hidden width H (32 or 64).  Parameters are kept in FP32
+// (packed: W1 (16,H), b1 (H), W2 (H,H), b2 (H), W3 (H,6), b3 (6)); inference
+// feeds the matrix cores bf16 weights and bf16 layer inputs with FP32
+// accumulation (v_mfma_f32_16x16x16_bf16).
+//
+// Inference is computed transposed, Y^T = W^T X^T: weights are the A operand,
+// activations the B operand, 16 modes per wavefront.  The 16x16 FP32
+// accumulator of one layer (lane l: rows 4(l/16)+j, column l%16) has exactly
+// the B-operand layout the next layer needs for that K chunk, so layers chain
+// in registers with no exchange.
+//
+// Training runs one sample per thread in FP32 (forward with the same bf16
+// rounding as inference, straight-through gradients), keeps the per-sample
+// layer inputs and pre-activation gradients in LDS, and accumulates weight
+// gradients per block in a fixed order; a second kernel sums the block
+// partials in order (deterministic), and Adam updates the FP32 parameters.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "context.hpp"
+#include "dft.hpp"
+
+namespace dnsmi {
+
+constexpr int kDin = 16, kDout = 6;
+
+__host__ __device__ constexpr int mlp_nparams(int H) {
+  return kDin * H + H + H * H + H + H * kDout + kDout;
+}
+
+__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+__device__ __forceinline__ float dsilu(float z) {
+  const float s = 1.0f / (1.0f + __expf(-z));
+  return s * (1.0f + z * (1.0f - s));
+}
+__device__ __forceinline__ uint16_t bf16_bits(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (uint16_t)((u + (((u >> 16) & 1u) + 0x7FFFu)) >> 16);
+}
+__device__ __forceinline__ float bf16_round(float f) {
+  return __uint_as_float(((uint32_t)bf16_bits(f)) << 16);
+}
+
+// ------------------------------------------------------------------ training
+// One block = S samples (one per thread).  Inputs X (B, 16), targets T (B, 6).
+// grad_part[block][p] receives the block's sum of dL/dp, loss_part[block]
+// its sum of squared errors.  L = (1/B) sum (y - t)^2.
+template <int H, int S>
+__global__ void __launch_bounds__(S)
+    k_mlp_grad_block(const float* __restrict__ X, const float* __restrict__ T, int B,
+                     const float* __restrict__ prm, int bf16, float* __restrict__ grad_part,
+                     float* __restrict__ loss_part) {
+  constexpr int P = mlp_nparams(H);
+  __shared__ float w[P];
+  __shared__ float sx[S][kDin], sh1[S][H], sh2[S][H], sdz1[S][H], sdz2[S][H], sdy[S][kDout];
+  for (int i = threadIdx.x; i < P; i += S) w[i] = bf16 && (i < kDin * H || (i >= kDin * H + H && i < kDin * H + H + H * H) || (i >= kDin * H + 2 * H + H * H && i < kDin * H + 2 * H + H * H + H * kDout)) ? bf16_round(prm[i]) : prm[i];
+  __syncthreads();
+  const float* W1 = w;
+  const float* b1 = W1 + kDin * H;
+  const float* W2 = b1 + H;
+  const float* b2 = W2 + H * H;
+  const float* W3 = b2 + H;
+  const float* b3 = W3 + H * kDout;
+  const int t = threadIdx.x;
+  const int smp = blockIdx.x * S + t;
+  const bool live = smp < B;
+  float x[kDin], z1[H], z2[H], y[kDout];
+#pragma unroll
+  for (int k = 0; k < kDin; ++k) {
+    const float v = live ? X[(size_t)smp * kDin + k] : 0.f;
+    x[k] = bf16 ? bf16_round(v) : v;
+    sx[t][k] = x[k];
+  }
+#pragma unroll 4
+  for (int h = 0; h < H; ++h) {
+    float a = b1[h];
+#pragma unroll
+    for (int k = 0; k < kDin; ++k) a += x[k] * W1[k * H + h];
+    z1[h] = a;
+    const float s = silu(a);
+    sh1[t][h] = bf16 ? bf16_round(s) : s;
+  }
+#pragma unroll 4
+  for (int o = 0; o < H; ++o) {
+    float a = b2[o];
+    for (int h = 0; h < H; ++h) a += sh1[t][h] * W2[h * H + o];
+    z2[o] = a;
+    const float s = silu(a);
+    sh2[t][o] = bf16 ? bf16_round(s) : s;
+  }
+  float sq = 0.f;
+#pragma unroll
+  for (int o = 0; o < kDout; ++o) {
+    float a = b3[o];
+    for (int h = 0; h < H; ++h) a += sh2[t][h] * W3[h * kDout + o];
+    y[o] = a;
+    const float d = live ? a - T[(size_t)smp * kDout + o] : 0.f;
+    sq += d * d;
+    sdy[t][o] = live ? 2.0f * d / (float)B : 0.f;
+  }
+  // backward of the activations
+#pragma unroll 4
+  for (int h = 0; h < H; ++h) {
+    float a = 0.f;
+#pragma unroll
+    for (int o = 0; o < kDout; ++o) a += W3[h * kDout + o] * sdy[t][o];
+    sdz2[t][h] = a * dsilu(z2[h]);
+  }
+#pragma unroll 4
+  for (int h = 0; h < H; ++h) {
+    float a = 0.f;
+    for (int o = 0; o < H; ++o) a += W2[h * H + o] * sdz2[t][o];
+    sdz1[t][h] = a * dsilu(z1[h]);
+  }
+  __syncthreads();
+  // parameter gradients: each thread owns a strided set, sums samples in order
+  float* gp = grad_part + (size_t)blockIdx.x * P;
+  for (int p = t; p < P; p += S) {
+    float g = 0.f;
+    if (p < kDin * H) {
+      const int k = p / H, h = p % H;
+      for (int s = 0; s < S; ++s) g += sx[s][k] * sdz1[s][h];
+    } else if (p < kDin * H + H) {
+      const int h = p - kDin * H;
+      for (int s = 0; s < S; ++s) g += sdz1[s][h];
+    } else if (p < kDin * H + H + H * H) {
+      const int q = p - (kDin * H + H), h = q / H, o = q % H;
+      for (int s = 0; s < S; ++s) g += sh1[s][h] * sdz2[s][o];
+    } else if (p < kDin * H + 2 * H + H * H) {
+      const int o = p - (kDin * H + H + H * H);
+      for (int s = 0; s < S; ++s) g += sdz2[s][o];
+    } else if (p < kDin * H + 2 * H + H * H + H * kDout) {
+      const int q = p - (kDin * H + 2 * H + H * H), h = q / kDout, o = q % kDout;
+      for (int s = 0; s < S; ++s) g += sh2[s][h] * sdy[s][o];
+    } else {
+      const int o = p - (kDin * H + 2 * H + H * H + H * kDout);
+      for (int s = 0; s < S; ++s) g += sdy[s][o];
+    }
+    gp[p] = g;
+  }
+  // block loss (fixed order)
+  __shared__ float red[S];
+  red[t] = sq;
+  __syncthreads();
+  if (t == 0) {
+    float s = 0.f;
+    for (int i = 0; i < S; ++i) s += red[i];
+    loss_part[blockIdx.x] = s;
+  }
+}
+
+__global__ void k_mlp_reduce(const float* __restrict__ part, int nblocks, int P,
+                             float* __restrict__ grad, const float* __restrict__ loss_part,
+                             float* __restrict__ loss, int B) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < P) {
+    float g = 0.f;
+    for (int b = 0; b < nblocks; ++b) g += part[(size_t)b * P + p];
+    grad[p] = g;
+  }
+  if (p == 0 && loss) {
+    float s = 0.f;
+    for (int b = 0; b < nblocks; ++b) s += loss_part[b];
+    *loss = s / (float)B;
+  }
+}
+
+// Adam (Kingma & Ba 2015) on the FP32 parameters.
+__global__ void k_mlp_adam(float* __restrict__ prm, const float* __restrict__ g,
+                           float* __restrict__ m, float* __restrict__ v, int P, float lr,
+                           float b1, float b2, float eps, float c1, float c2) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  const float gi = g[p];
+  const float mi = b1 * m[p] + (1 - b1) * gi;
+  const float vi = b2 * v[p] + (1 - b2) * gi * gi;
+  m[p] = mi;
+  v[p] = vi;
+  prm[p] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// ------------------------------------------------------------------ packing
+// bf16 A operands in lane order: for each 16x16 tile, lane l holds
+// W[k0 + 4(l/16) + j][o0 + l%16], j = 0..3 (4 consecutive uint16).
+// Layout: W1 tiles (H/16), W2 tiles (H/16 out x H/16 in, out major),
+// W3 tiles (H/16, 6 outputs padded to 16); then FP32 biases b1 | b2 | b3.
+template <int H>
+__global__ void k_mlp_pack(const float* __restrict__ prm, uint16_t* __restrict__ wb,
+                           float* __restrict__ bias) {
+  constexpr int NH = H / 16;
+  const float* W1 = prm;
+  const float* b1 = W1 + kDin * H;
+  const float* W2 = b1 + H;
+  const float* b2 = W2 + H * H;
+  const float* W3 = b2 + H;
+  const float* b3 = W3 + H * kDout;
+  const int ntiles = NH + NH * NH + NH;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;  // one (tile, lane) per thread
+  if (e < ntiles * 64) {
+    const int tile = e / 64, l = e % 64;
+    for (int j = 0; j < 4; ++j) {
+      const int kk = 4 * (l / 16) + j, oo = l % 16;
+      float val = 0.f;
+      if (tile < NH) {
+        val = W1[kk * H + tile * 16 + oo];
+      } else if (tile < NH + NH * NH) {
+        const int t2 = tile - NH, o = t2 / NH, k = t2 % NH;
+        val = W2[(k * 16 + kk) * H + o * 16 + oo];
+      } else {
+        const int k = tile - NH - NH * NH;
+        val = oo < kDout ? W3[(k * 16 + kk) * kDout + oo] : 0.f;
+      }
+      wb[(size_t)e * 4 + j] = bf16_bits(val);
+    }
+  }
+  if (e < H) {
+    bias[e] = b1[e];
+    bias[H + e] = b2[e];
+  }
+  if (e < 16) bias[2 * H + e] = e < kDout ? b3[e] : 0.f;
+}
+
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ v4s pack_b(float a0, float a1, float a2, float a3) {
+  v4s r;
+  r[0] = (short)bf16_bits(a0);
+  r[1] = (short)bf16_bits(a1);
+  r[2] = (short)bf16_bits(a2);
+  r[3] = (short)bf16_bits(a3);
+  return r;
+}
+
+// The three layers for the 16 samples of this wave: xb = the wave's B
+// operand of layer 1 (lane: features 4(l/16)..+3 of sample l%16).  Returns
+// the lane's accumulator of the output layer (outputs 4(l/16)+j, sample l%16).
+template <int H>
+__device__ __forceinline__ v4f mlp_wave(v4s xb, const v4s* __restrict__ wt,
+                                        const float* __restrict__ bias, int lane) {
+  constexpr int NH = H / 16;
+  const int g = lane / 16;
+  v4s h1[NH];
+#pragma unroll
+  for (int o = 0; o < NH; ++o) {
+    v4f acc = {0.f, 0.f, 0.f, 0.f};
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wt[o * 64 + lane], xb, acc, 0, 0, 0);
+    float a[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) a[j] = silu(acc[j] + bias[o * 16 + 4 * g + j]);
+    h1[o] = pack_b(a[0], a[1], a[2], a[3]);
+  }
+  v4s h2[NH];
+#pragma unroll
+  for (int o = 0; o < NH; ++o) {
+    v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < NH; ++k)
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wt[(NH + o * NH + k) * 64 + lane], h1[k], acc,
+                                                      0, 0, 0);
+    float a[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) a[j] = silu(acc[j] + bias[H + o * 16 + 4 * g + j]);
+    h2[o] = pack_b(a[0], a[1], a[2], a[3]);
+  }
+  v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int k = 0; k < NH; ++k)
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wt[(NH + NH * NH + k) * 64 + lane], h2[k], acc,
+                                                    0, 0, 0);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) acc[j] += bias[2 * H + 4 * g + j];
+  return acc;
+}
+
+// Plain inference on X (B, 16) -> Y (B, 6): for checking the matrix-core
+// path against the oracle.
+template <int H>
+__global__ void __launch_bounds__(256)
+    k_mlp_forward(const float* __restrict__ X, int B, const uint16_t* __restrict__ wb,
+                  const float* __restrict__ bias, float* __restrict__ Y) {
+  constexpr int NT = 2 * (H / 16) + (H / 16) * (H / 16);
+  __shared__ v4s wt[NT * 64];
+  __shared__ float bs[2 * H + 16];
+  for (int i = threadIdx.x; i < NT * 64; i += blockDim.x) wt[i] = reinterpret_cast<const v4s*>(wb)[i];
+  for (int i = threadIdx.x; i < 2 * H + 16; i += blockDim.x) bs[i] = bias[i];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int s = (blockIdx.x * 4 + wave) * 16 + (lane % 16);
+  const int g = lane / 16;
+  float f[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) f[j] = (s < B) ? X[(size_t)s * kDin + 4 * g + j] : 0.f;
+  const v4f y = mlp_wave<H>(pack_b(f[0], f[1], f[2], f[3]), wt, bs, lane);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int o = 4 * g + j;
+    if (s < B && o < kDout) Y[(size_t)s * kDout + o] = y[j];
+  }
+}
+
+// ------------------------------------------------------------------ features
+// History: N^{n-1}, N^{n-2} kept in FP32 (interleaved re/im).  With
+//   D1 = N^n - N^{n-1},  D2 = N^n - 2 N^{n-1} + N^{n-2},
+// the quadratic extrapolation of the nonlinear term over one step is
+//   N^{n+1} - N^n ~ D1 + D2 = 2 N^n - 3 N^{n-1} + N^{n-2}.
+// Features of periodic mode i (flat index over (ky, kz, kx)):
+//   0..5  D1 components (re, im) * s_1
+//   6..11 D2 components * s_2
+//   12    min(dt |k|^2 / Re, 8) ; 13..15 kx/kmax, ky/kmax, kz/kmax
+// Target (training): w(k) (N^{n+1} - 3 N^n + 3 N^{n-1} - N^{n-2}) / s_out,
+// the residual of the quadratic extrapolation, weighted by
+// w = ildt_2 / dt (in (0, 1]) so that the squared loss weights each mode as
+// the corrector's error norm does.
+struct FeatArgs {
+  const double* kx;
+  const double* ky;
+  const double* kz;
+  int Nz1, Nx1;
+  long long n;
+  double kmax, dt, re, c;
+  float s_1, s_2, s_out;
+};
+
+__device__ __forceinline__ void mode_k(const FeatArgs& a, long long i, double& kx, double& ky,
+                                       double& kz) {
+  const int ix = (int)(i % a.Nx1);
+  const long long t = i / a.Nx1;
+  const int iz = (int)(t % a.Nz1);
+  const int iy = (int)(t / a.Nz1);
+  kx = a.kx[ix];
+  ky = a.ky[iy];
+  kz = a.kz[iz];
+}
+
+// ildt_2 / dt for mode i (0 for the mean mode, which the scheme holds fixed).
+__device__ __forceinline__ double mode_weight(const FeatArgs& a, double lapl, bool mean) {
+  return mean ? 0.0 : 1.0 / (1.0 - a.dt * a.c * lapl / a.re);
+}
+
+__device__ __forceinline__ float k_feature(const FeatArgs& a, long long i, int f) {
+  double kx, ky, kz;
+  mode_k(a, i, kx, ky, kz);
+  if (f == 12) return (float)fmin(a.dt * ((kx * kx + ky * ky) + kz * kz) / a.re, 8.0);
+  const double v = f == 13 ? kx : (f == 14 ? ky : kz);
+  return (float)(v / a.kmax);
+}
+
+// D1 and D2 of one real component from N^n (FP64) and the FP32 history.
+__device__ __forceinline__ void diffs(double cur, float h1, float h2, double& d1, double& d2) {
+  const double p1 = (double)h1, p2 = (double)h2;
+  d1 = cur - p1;
+  d2 = (cur - 2.0 * p1) + p2;
+}
+
+__device__ __forceinline__ float feature(const FeatArgs& a, const cx<double>* ncur,
+                                         const float* h1, const float* h2, long long i, int f) {
+  if (f >= 12) return k_feature(a, i, f);
+  const int q = f < 6 ? f : f - 6;
+  const cx<double> v = ncur[(q / 2) * a.n + i];
+  const long long e = ((q / 2) * a.n + i) * 2 + (q & 1);
+  double d1, d2;
+  diffs((q & 1) ? v.y : v.x, h1[e], h2[e], d1, d2);
+  return f < 6 ? (float)d1 * a.s_1 : (float)d2 * a.s_2;
+}
+
+// Training samples from four successive nonlinear terms (FP64 spectral
+// fields); the two older ones are rounded to FP32 as the history keeps them.
+__global__ void k_mlp_features(FeatArgs a, const cx<double>* __restrict__ nm2,
+                               const cx<double>* __restrict__ nm1,
+                               const cx<double>* __restrict__ ncur,
+                               const cx<double>* __restrict__ nnext,
+                               const long long* __restrict__ modes, int ns,
+                               float* __restrict__ X, float* __restrict__ T) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ns) return;
+  const long long i = modes[s];
+  for (int q = 0; q < 6; ++q) {
+    const long long e = (q / 2) * a.n + i;
+    const cx<double> v = ncur[e], p1 = nm1[e], p2 = nm2[e];
+    double d1, d2;
+    diffs((q & 1) ? v.y : v.x, (float)((q & 1) ? p1.y : p1.x), (float)((q & 1) ? p2.y : p2.x), d1, d2);
+    X[(size_t)s * kDin + q] = (float)d1 * a.s_1;
+    X[(size_t)s * kDin + 6 + q] = (float)d2 * a.s_2;
+  }
+  for (int f = 12; f < kDin; ++f) X[(size_t)s * kDin + f] = k_feature(a, i, f);
+  if (T && nnext) {
+    double kx, ky, kz;
+    mode_k(a, i, kx, ky, kz);
+    const double w = mode_weight(a, -((kx * kx + ky * ky) + kz * kz), i == 0);
+    for (int q = 0; q < 6; ++q) {
+      const long long e = (q / 2) * a.n + i;
+      const cx<double> p2 = nm2[e], p1 = nm1[e], c = ncur[e], x = nnext[e];
+      const double d3 = (q & 1) ? ((x.y - 3 * c.y) + (3 * p1.y - p2.y))
+                                : ((x.x - 3 * c.x) + (3 * p1.x - p2.x));
+      T[(size_t)s * kDout + q] = (float)(w * d3 / a.s_out);
+    }
+  }
+}
+
+// ------------------------------------------------------------------ guess
+// After the Euler predictor: pred holds u^1, ncur = N^n (FP64); h1 = N^{n-1}
+// and h2 = N^{n-2} (FP32 history).  Per mode, with
+//   mode 0: dN = 0                    (the plain predictor)
+//   mode 1: dN = D1                   (linear extrapolation of N)
+//   mode 3: dN = D1 + D2              (quadratic extrapolation)
+//   mode 2: dN = D1 + D2 + s_out net(x) / w   (learned residual)
+// and dN Leray-projected (zero at the mean mode):
+//   pred     <- u^1 + c ildt_2 dN           (the guess)
+//   rhs_prev <- N^n + dN                    (so that the incremental
+//               corrector from the guess is the same fixed-point map)
+//   h2       <- N^n                         (the caller then swaps h1, h2)
+template <int H>
+__global__ void __launch_bounds__(256)
+    k_guess_periodic(FeatArgs a, const cx<double>* __restrict__ ncur, const float* __restrict__ h1,
+                     float* __restrict__ h2, double* __restrict__ pred,
+                     double* __restrict__ rhs_prev, const uint16_t* __restrict__ wb,
+                     const float* __restrict__ bias, int mode) {
+  constexpr int NT = 2 * (H / 16) + (H / 16) * (H / 16);
+  __shared__ v4s wt[NT * 64];
+  __shared__ float bs[2 * H + 16];
+  __shared__ float outs[4][16][kDout];
+  const bool use_net = mode == 2;
+  if (use_net) {
+    for (int i = threadIdx.x; i < NT * 64; i += blockDim.x) wt[i] = reinterpret_cast<const v4s*>(wb)[i];
+    for (int i = threadIdx.x; i < 2 * H + 16; i += blockDim.x) bs[i] = bias[i];
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane / 16, ml = lane % 16;
+  const long long i = ((long long)blockIdx.x * 4 + wave) * 16 + ml;
+  const bool live = i < a.n;
+  if (use_net) {
+    float f[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) f[j] = live ? feature(a, ncur, h1, h2, i, 4 * g + j) : 0.f;
+    const v4f y = mlp_wave<H>(pack_b(f[0], f[1], f[2], f[3]), wt, bs, lane);
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (4 * g + j < kDout) outs[wave][ml][4 * g + j] = y[j];
+  }
+  __syncthreads();
+  if (!live || g == 3) return;
+  // lane (mode ml, component g)
+  double kx, ky, kz;
+  mode_k(a, i, kx, ky, kz);
+  const double lapl = -((kx * kx + ky * ky) + kz * kz);
+  const bool mean = (i == 0);
+  const double w = mode_weight(a, lapl, mean);
+  cx<double> dn[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const cx<double> nc = ncur[q * a.n + i];
+    const long long e = (q * a.n + i) * 2;
+    double d1x, d2x, d1y, d2y;
+    diffs(nc.x, h1[e], h2[e], d1x, d2x);
+    diffs(nc.y, h1[e + 1], h2[e + 1], d1y, d2y);
+    if (mode == 0)
+      dn[q] = {0.0, 0.0};
+    else if (mode == 1)
+      dn[q] = {d1x, d1y};
+    else
+      dn[q] = {d1x + d2x, d1y + d2y};
+    if (use_net && !mean) {
+      dn[q].x += (double)a.s_out * outs[wave][ml][2 * q] / w;
+      dn[q].y += (double)a.s_out * outs[wave][ml][2 * q + 1] / w;
+    }
+  }
+  const double il = lapl < 0 ? 1.0 / lapl : 0.0;
+  // projection: dn - k (k . dn) / |k|^2 = dn + k (k . dn) * inv_lapl
+  const cx<double> kd = {(kx * dn[0].x + ky * dn[1].x) + kz * dn[2].x,
+                         (kx * dn[0].y + ky * dn[1].y) + kz * dn[2].y};
+  const double kq = g == 0 ? kx : (g == 1 ? ky : kz);
+  // the mean mode keeps dN = 0: the guess then changes nothing there
+  const cx<double> d = mean ? cx<double>{0.0, 0.0}
+                            : cx<double>{dn[g].x + kq * kd.x * il, dn[g].y + kq * kd.y * il};
+  const double cil = a.c * a.dt * w;  // c ildt_2
+  const long long e = g * a.n + i;
+  cx<double>* pd = reinterpret_cast<cx<double>*>(pred);
+  cx<double>* rp = reinterpret_cast<cx<double>*>(rhs_prev);
+  const cx<double> u1 = pd[e];
+  const cx<double> nc = ncur[e];
+  if (mode != 0) pd[e] = {u1.x + cil * d.x, u1.y + cil * d.y};
+  rp[e] = mode == 0 ? nc : cx<double>{nc.x + d.x, nc.y + d.y};
+  // The 64 lanes of this wave hold all reads of its 16 modes' history
+  // (features and the loop above) and issue them before these stores, in
+  // program order; no other wave touches these modes.
+  h2[e * 2] = (float)nc.x;
+  h2[e * 2 + 1] = (float)nc.y;
+}
+
+// ------------------------------------------------------------------ launchers
+int mlp_grad(int H, int B, const float* X, const float* T, const float* prm, int bf16, float* part,
+             float* grad, float* loss, hipStream_t s) {
+  constexpr int S = 32;
+  const int nb = (B + S - 1) / S;
+  float* loss_part = part + (size_t)nb * mlp_nparams(H);
+  if (H == 32) {
+    hipLaunchKernelGGL((k_mlp_grad_block<32, S>), dim3(nb), dim3(S), 0, s, X, T, B, prm, bf16, part,
+                       loss_part);
+  } else if (H == 64) {
+    hipLaunchKernelGGL((k_mlp_grad_block<64, S>), dim3(nb), dim3(S), 0, s, X, T, B, prm, bf16, part,
+                       loss_part);
+  } else {
+    return DNSMI_EINVAL;
+  }
+  const int P = mlp_nparams(H);
+  hipLaunchKernelGGL(k_mlp_reduce, dim3((P + 255) / 256), dim3(256), 0, s, part, nb, P, grad,
+                     loss_part, loss, B);
+  return hipGetLastError() == hipSuccess ? DNSMI_OK : DNSMI_EHIP;
+}
+
+int mlp_adam(int H, float* prm, const float* g, float* m, float* v, int step, float lr,
+             hipStream_t s) {
+  const int P = mlp_nparams(H);
+  const float b1 = 0.9f, b2 = 0.999f;
+  const float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
+  hipLaunchKernelGGL(k_mlp_adam, dim3((P + 255) / 256), dim3(256), 0, s, prm, g, m, v, P, lr, b1,
+                     b2, 1e-8f, c1, c2);
+  return hipGetLastError() == hipSuccess ? DNSMI_OK : DNSMI_EHIP;
+}
+
+int mlp_pack(int H, const float* prm, uint16_t* wb, float* bias, hipStream_t s) {
+  const int NH = H / 16, nt = 2 * NH + NH * NH;
+  const int n = nt * 64 > H ? nt * 64 : H;
+  if (H == 32)
+    hipLaunchKernelGGL(k_mlp_pack<32>, dim3((n + 255) / 256), dim3(256), 0, s, prm, wb, bias);
+  else if (H == 64)
+    hipLaunchKernelGGL(k_mlp_pack<64>, dim3((n + 255) / 256), dim3(256), 0, s, prm, wb, bias);
+  else
+    return DNSMI_EINVAL;
+  return hipGetLastError() == hipSuccess ? DNSMI_OK : DNSMI_EHIP;
+}
+
+int mlp_forward(int H, const float* X, int B, const uint16_t* wb, const float* bias, float* Y,
+                hipStream_t s) {
+  const int nb = (B + 63) / 64;
+  if (H == 32)
+    hipLaunchKernelGGL(k_mlp_forward<32>, dim3(nb), dim3(256), 0, s, X, B, wb, bias, Y);
+  else if (H == 64)
+    hipLaunchKernelGGL(k_mlp_forward<64>, dim3(nb), dim3(256), 0, s, X, B, wb, bias, Y);
+  else
+    return DNSMI_EINVAL;
+  return hipGetLastError() == hipSuccess ? DNSMI_OK : DNSMI_EHIP;
+}
+
+static FeatArgs feat_args(Ctx* c, float s_1, float s_2, float s_out) {
+  FeatArgs a{};
+  a.kx = c->kx;
+  a.ky = c->ky;
+  a.kz = c->kz;
+  a.Nz1 = c->Nz1;
+  a.Nx1 = c->Nx1;
+  a.n = (long long)c->Ny1 * c->Nz1 * c->Nx1;
+  const double kxm = 2 * kPi / c->cfg.lx * (c->nx / 2), kym = 2 * kPi / c->cfg.ly * (c->ny / 2),
+               kzm = 2 * kPi / c->cfg.lz * (c->nz / 2);
+  a.kmax = fmax(kxm, fmax(kym, kzm));
+  a.dt = c->cfg.dt;
+  a.re = c->cfg.re;
+  a.c = c->cfg.implicitness;
+  a.s_1 = s_1;
+  a.s_2 = s_2;
+  a.s_out = s_out;
+  return a;
+}
+
+int mlp_features(Ctx* c, const double* nm2, const double* nm1, const double* ncur,
+                 const double* nnext, const long long* modes, int ns, float s_1, float s_2,
+                 float s_out, float* X, float* T, hipStream_t s) {
+  if (c->walls) return c->fail(DNSMI_EINVAL, "features: periodic contexts only");
+  const FeatArgs a = feat_args(c, s_1, s_2, s_out);
+  if (ns == 0) return DNSMI_OK;
+  hipLaunchKernelGGL(k_mlp_features, dim3((ns + 255) / 256), dim3(256), 0, s, a,
+                     reinterpret_cast<const cx<double>*>(nm2),
+                     reinterpret_cast<const cx<double>*>(nm1),
+                     reinterpret_cast<const cx<double>*>(ncur),
+                     reinterpret_cast<const cx<double>*>(nnext), modes, ns, X, T);
+  return check_launch(c, "mlp_features");
+}
+
+int periodic_predict_only(Ctx* c, const double* state, double* pred, double* rhs0, hipStream_t s);
+
+int guess_periodic(Ctx* c, const double* state, double* pred, double* rhs_prev, const float* h1,
+                   float* h2, const uint16_t* wb, const float* bias, int H, float s_1, float s_2,
+                   float s_out, int mode, hipStream_t s) {
+  if (c->walls) return c->fail(DNSMI_EINVAL, "guess: periodic contexts only");
+  if (mode < 0 || mode > 3) return c->fail(DNSMI_EINVAL, "guess: mode must be 0, 1, 2 or 3");
+  if (mode == 2 && H != 32 && H != 64)
+    return c->fail(DNSMI_EINVAL, "guess: hidden width must be 32 or 64");
+  int rc = periodic_predict_only(c, state, pred, c->rhs0, s);
+  if (rc) return rc;
+  const FeatArgs a = feat_args(c, s_1, s_2, s_out);
+  const int nb = (int)((a.n + 63) / 64);
+  const cx<double>* n0 = reinterpret_cast<const cx<double>*>(c->rhs0);
+  if (mode != 2 || H == 32)
+    hipLaunchKernelGGL(k_guess_periodic<32>, dim3(nb), dim3(256), 0, s, a, n0, h1, h2, pred,
+                       rhs_prev, wb, bias, mode);
+  else
+    hipLaunchKernelGGL(k_guess_periodic<64>, dim3(nb), dim3(256), 0, s, a, n0, h1, h2, pred,
+                       rhs_prev, wb, bias, mode);
+  return check_launch(c, "guess");
+}
+
+}  // namespace dnsmi

@@ -37,6 +37,7 @@ struct StepArgs {
   double* pred;            // MODE_PREDICT: out; MODE_CORRECT: in/out
   double* rhs_out;         // N of this evaluation
   double* partials;        // MODE_CORRECT: per-block sum k_metric |delta|^2
+  const int* act;          // if set and 0: the whole launch is skipped
   double dt, c, re;
 };
 
@@ -59,7 +60,8 @@ template <typename T, int P, int TC>
 __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
     k_y_inverse(const cx<double>* __restrict__ u, cx<T>* __restrict__ w1,
                 const cx<T>* __restrict__ tw, const double* __restrict__ ky, int ny, int Nz1,
-                int Nx1) {
+                int Nx1, const int* __restrict__ act) {
+  if (act && *act == 0) return;
   constexpr int N1 = ColPlan<P>::N1, N2 = ColPlan<P>::N2;
   __shared__ cx<T> lds[TC * P];
   __shared__ cx<T> tws[P];
@@ -125,7 +127,8 @@ __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
 template <typename T, int P, int TC>
 __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
     k_z_inverse(const cx<T>* __restrict__ w1, cx<T>* __restrict__ w2, const cx<T>* __restrict__ tw,
-                const double* __restrict__ kz_tab, int nz, int Py, int Nx1) {
+                const double* __restrict__ kz_tab, int nz, int Py, int Nx1, const int* __restrict__ act) {
+  if (act && *act == 0) return;
   constexpr int N1 = ColPlan<P>::N1, N2 = ColPlan<P>::N2;
   __shared__ cx<T> lds[TC * P];
   __shared__ cx<T> tws[P];
@@ -207,7 +210,8 @@ template <typename T, int P, int TC>
 __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
     k_y_inverse_seq(const cx<double>* __restrict__ u, cx<T>* __restrict__ w1,
                     const cx<T>* __restrict__ tw, const double* __restrict__ ky, int ny, int Nz1,
-                    int Nx1) {
+                    int Nx1, const int* __restrict__ act) {
+  if (act && *act == 0) return;
   constexpr int N1 = ColPlan<P>::N1, N2 = ColPlan<P>::N2;
   __shared__ cx<T> lds[TC * P];
   __shared__ cx<T> tws[P];
@@ -262,7 +266,8 @@ template <typename T, int P, int TC>
 __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
     k_z_inverse_seq(const cx<T>* __restrict__ w1, cx<T>* __restrict__ w2,
                     const cx<T>* __restrict__ tw, const double* __restrict__ kz_tab, int nz, int Py,
-                    int Nx1) {
+                    int Nx1, const int* __restrict__ act) {
+  if (act && *act == 0) return;
   constexpr int N1 = ColPlan<P>::N1, N2 = ColPlan<P>::N2;
   __shared__ cx<T> lds[TC * P];
   __shared__ cx<T> tws[P];
@@ -337,17 +342,24 @@ struct XRow {
   static constexpr int value = a > b ? a : b;
 };
 
+// The nonlinear term is formed in two stages to bound the live registers:
+// after the (u_z, w_y) and (u_y, w_z) transforms, component 0 is complete
+// and is transformed and stored; the (u_x, w_x) transform then completes
+// components 1 and 2.  At most four physical fields are held at once.
 template <typename T, int P, int R>
-__global__ void __launch_bounds__(R * Plan<P>::N1)
+__global__ void __launch_bounds__(R * Plan<P>::N1, 2)
     k_x_nonlinear(const cx<T>* __restrict__ w2, cx<T>* __restrict__ w3,
                   const cx<T>* __restrict__ tw, const double* __restrict__ kx_tab,
                   const double* __restrict__ base, const double* __restrict__ cbase,
-                  const double* __restrict__ nbase, int nx, int Py, int Pz, int ybase_stride) {
+                  const double* __restrict__ nbase, int nx, int Py, int Pz, int ybase_stride, const int* __restrict__ act) {
+  if (act && *act == 0) return;
   constexpr int N1 = Plan<P>::N1, N2 = Plan<P>::N2;
   constexpr int ROW = XRow<N1, N2>::value;
   __shared__ cx<T> lds[R * ROW];
-  __shared__ cx<T> tws[P];
-  for (int i = threadIdx.x; i < P; i += blockDim.x) tws[i] = tw[i];
+  __shared__ cx<T> tab_i[N1 * N2];  // inverse plan (N1, N2)
+  __shared__ cx<T> tab_f[N2 * N1];  // forward plan (N2, N1)
+  fill_plan_twiddles<N1, N2>(tab_i, tw);
+  fill_plan_twiddles<N2, N1>(tab_f, tw);
   __syncthreads();
   const int tid = threadIdx.x, r = tid / N1, j = tid - r * N1;
   const int nrows = Py * Pz;
@@ -357,11 +369,12 @@ __global__ void __launch_bounds__(R * Plan<P>::N1)
   const int Nx1 = nx / 2;
   const size_t fstride = (size_t)nrows * Nx1;
   const cx<T>* in = w2 + (size_t)(ok ? row : 0) * Nx1;
+  cx<T>* out = w3 + (size_t)(ok ? row : 0) * Nx1;
   cx<T>* rl = lds + r * ROW;
+  const T inv = static_cast<T>(1.0 / P);
 
-  T ux[N2], uy[N2], uz[N2], wx[N2], wy[N2], wz[N2];
-#pragma unroll
-  for (int pair = 0; pair < 3; ++pair) {
+  // Inverse transform of one (u, w) pair into physical values re/im.
+  auto inverse_pair = [&](int pair, T* re, T* im) {
     if (j < N2) {
       cx<T> z[N1];
 #pragma unroll
@@ -398,58 +411,26 @@ __global__ void __launch_bounds__(R * Plan<P>::N1)
         // Z_m = X_m + i Y_m with X_{P-k} = conj(X_k)
         z[a] = cj ? cx<T>{X.x + Y.y, Y.x - X.y} : cx<T>{X.x - Y.y, X.y + Y.x};
       }
-      row_step_a_store<P, N1, N2, +1>(z, j, rl, tws);
+      row_step_a_store_tab<P, N1, N2, +1>(z, j, rl, tab_i);
     }
     __syncthreads();
-    {
-      cx<T> o[N2];
-      row_step_b_load<P, N1, N2, +1>(o, j, rl);
+    cx<T> o[N2];
+    row_step_b_load<P, N1, N2, +1>(o, j, rl);
 #pragma unroll
-      for (int k2 = 0; k2 < N2; ++k2) {
-        if (pair == 0) {
-          uz[k2] = o[k2].x;
-          wy[k2] = o[k2].y;
-        } else if (pair == 1) {
-          uy[k2] = o[k2].x;
-          wz[k2] = o[k2].y;
-        } else {
-          ux[k2] = o[k2].x;
-          wx[k2] = o[k2].y;
-        }
-      }
+    for (int k2 = 0; k2 < N2; ++k2) {
+      re[k2] = o[k2].x;
+      im[k2] = o[k2].y;
     }
     __syncthreads();
-  }
+  };
 
-  // u' x w' + (u' x curl U + U x w') + U x curl U   (rhs.py:80-88)
-  const T U0 = static_cast<T>(base[0 * ybase_stride + yrow]);
-  const T U1 = static_cast<T>(base[1 * ybase_stride + yrow]);
-  const T U2 = static_cast<T>(base[2 * ybase_stride + yrow]);
-  const T C0 = static_cast<T>(cbase[0 * ybase_stride + yrow]);
-  const T C1 = static_cast<T>(cbase[1 * ybase_stride + yrow]);
-  const T C2 = static_cast<T>(cbase[2 * ybase_stride + yrow]);
-  const T B0 = static_cast<T>(nbase[0 * ybase_stride + yrow]);
-  const T B1 = static_cast<T>(nbase[1 * ybase_stride + yrow]);
-  const T B2 = static_cast<T>(nbase[2 * ybase_stride + yrow]);
-  T nl[3][N2];
-#pragma unroll
-  for (int k2 = 0; k2 < N2; ++k2) {
-    const T a0 = ux[k2], a1 = uy[k2], a2 = uz[k2];
-    const T b0 = wx[k2], b1 = wy[k2], b2 = wz[k2];
-    nl[0][k2] = (a1 * b2 - a2 * b1) + (((a1 * C2 - a2 * C1) + (U1 * b2 - U2 * b1)) + B0);
-    nl[1][k2] = (a2 * b0 - a0 * b2) + (((a2 * C0 - a0 * C2) + (U2 * b0 - U0 * b2)) + B1);
-    nl[2][k2] = (a0 * b1 - a1 * b0) + (((a0 * C1 - a1 * C0) + (U0 * b1 - U1 * b0)) + B2);
-  }
-
-  const T inv = static_cast<T>(1.0 / P);
-  cx<T>* out = w3 + (size_t)(ok ? row : 0) * Nx1;
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
+  // Forward transform of one real component, truncated to kx < nx/2.
+  auto forward_store = [&](int c, const T* v_re) {
     {
       cx<T> v[N2];
 #pragma unroll
-      for (int a = 0; a < N2; ++a) v[a] = {c == 0 ? nl[0][a] : (c == 1 ? nl[1][a] : nl[2][a]), 0};
-      row_step_a_store<P, N2, N1, -1>(v, j, rl, tws);
+      for (int a = 0; a < N2; ++a) v[a] = {v_re[a], 0};
+      row_step_a_store_tab<P, N2, N1, -1>(v, j, rl, tab_f);
     }
     __syncthreads();
     if (j < N2 && ok) {
@@ -462,7 +443,45 @@ __global__ void __launch_bounds__(R * Plan<P>::N1)
       }
     }
     __syncthreads();
+  };
+
+  // u' x w' + (u' x curl U + U x w') + U x curl U   (rhs.py:80-88)
+  const T U0 = static_cast<T>(base[0 * ybase_stride + yrow]);
+  const T U1 = static_cast<T>(base[1 * ybase_stride + yrow]);
+  const T U2 = static_cast<T>(base[2 * ybase_stride + yrow]);
+  const T C0 = static_cast<T>(cbase[0 * ybase_stride + yrow]);
+  const T C1 = static_cast<T>(cbase[1 * ybase_stride + yrow]);
+  const T C2 = static_cast<T>(cbase[2 * ybase_stride + yrow]);
+  const T B0 = static_cast<T>(nbase[0 * ybase_stride + yrow]);
+  const T B1 = static_cast<T>(nbase[1 * ybase_stride + yrow]);
+  const T B2 = static_cast<T>(nbase[2 * ybase_stride + yrow]);
+
+  T uz[N2], wy[N2], uy[N2], wz[N2];
+  inverse_pair(0, uz, wy);
+  inverse_pair(1, uy, wz);
+  {
+    T nl0[N2];
+#pragma unroll
+    for (int k2 = 0; k2 < N2; ++k2) {
+      const T a1 = uy[k2], a2 = uz[k2], b1 = wy[k2], b2 = wz[k2];
+      nl0[k2] = (a1 * b2 - a2 * b1) + (((a1 * C2 - a2 * C1) + (U1 * b2 - U2 * b1)) + B0);
+    }
+    forward_store(0, nl0);
   }
+  T nl1[N2], nl2[N2];
+  {
+    T ux[N2], wx[N2];
+    inverse_pair(2, ux, wx);
+#pragma unroll
+    for (int k2 = 0; k2 < N2; ++k2) {
+      const T a0 = ux[k2], a1 = uy[k2], a2 = uz[k2];
+      const T b0 = wx[k2], b1 = wy[k2], b2 = wz[k2];
+      nl1[k2] = (a2 * b0 - a0 * b2) + (((a2 * C0 - a0 * C2) + (U2 * b0 - U0 * b2)) + B1);
+      nl2[k2] = (a0 * b1 - a1 * b0) + (((a0 * C1 - a1 * C0) + (U0 * b1 - U1 * b0)) + B2);
+    }
+  }
+  forward_store(1, nl1);
+  forward_store(2, nl2);
 }
 
 // K3, split form (N2 even): 2 N1 threads per row.  The inverse step B
@@ -478,7 +497,8 @@ __global__ void __launch_bounds__(R * 2 * Plan<P>::N1)
                         const cx<T>* __restrict__ tw, const double* __restrict__ kx_tab,
                         const double* __restrict__ base, const double* __restrict__ cbase,
                         const double* __restrict__ nbase, int nx, int Py, int Pz,
-                        int ybase_stride) {
+                        int ybase_stride, const int* __restrict__ act) {
+  if (act && *act == 0) return;
   constexpr int N1 = Plan<P>::N1, N2 = Plan<P>::N2;
   static_assert(N2 % 2 == 0, "split x pass needs an even second factor");
   constexpr int H2 = N2 / 2;    // points per field per thread
@@ -623,7 +643,8 @@ __global__ void __launch_bounds__(R * 2 * Plan<P>::N1)
 template <typename T, int P, int TC>
 __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
     k_z_forward(const cx<T>* __restrict__ w3, cx<T>* __restrict__ w4, const cx<T>* __restrict__ tw,
-                int nz, int Py, int Nx1) {
+                int nz, int Py, int Nx1, const int* __restrict__ act) {
+  if (act && *act == 0) return;
   constexpr int N1 = ColPlan<P>::N1, N2 = ColPlan<P>::N2;
   __shared__ cx<T> lds[TC * P];
   __shared__ cx<T> tws[P];
@@ -669,6 +690,7 @@ __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
     k_y_forward(const cx<T>* __restrict__ w4, const cx<T>* __restrict__ tw,
                 const double* __restrict__ kx_tab, const double* __restrict__ ky_tab,
                 const double* __restrict__ kz_tab, int ny, int Nz1, int Nx1, StepArgs args) {
+  if (args.act && *args.act == 0) return;
   constexpr int N1 = ColPlan<P>::N1, N2 = ColPlan<P>::N2;
   __shared__ cx<T> lds[TC * P];
   __shared__ cx<T> tws[P];
@@ -844,7 +866,9 @@ __global__ void __launch_bounds__(64) k_zero_mean(double* __restrict__ u, long l
 }
 
 __global__ void __launch_bounds__(256) k_finalize_norm(const double* __restrict__ partials, int n,
-                                                       double* __restrict__ err) {
+                                                       double* __restrict__ err,
+                                                       const int* __restrict__ act) {
+  if (act && *act == 0) return;
   __shared__ double red[16];
   double v = 0.0;
   for (int i = threadIdx.x; i < n; i += blockDim.x) v += partials[i];
@@ -854,7 +878,7 @@ __global__ void __launch_bounds__(256) k_finalize_norm(const double* __restrict_
 
 int finalize_norm(Ctx* c, int nblocks, double* err_dev, hipStream_t s) {
   ProfScope ps(c, KID_FINALIZE, s);
-  hipLaunchKernelGGL(k_finalize_norm, dim3(1), dim3(256), 0, s, c->partials, nblocks, err_dev);
+  hipLaunchKernelGGL(k_finalize_norm, dim3(1), dim3(256), 0, s, c->partials, nblocks, err_dev, c->act);
   return check_launch(c, "finalize_norm");
 }
 
@@ -892,10 +916,10 @@ static int launch_y_inverse(Ctx* c, const double* state, hipStream_t s) {
   case P:                                                                                       \
     if (c->yinv_seq)                                                                            \
       hipLaunchKernelGGL((k_y_inverse_seq<T, P, kTC>), grid, dim3(kTC * ColPlan<P>::threads), 0, \
-                         s, u, b.A, b.twy, c->ky, c->ny, c->Nz1, c->Nx1);                       \
+                         s, u, b.A, b.twy, c->ky, c->ny, c->Nz1, c->Nx1, c->act);                       \
     else                                                                                        \
       hipLaunchKernelGGL((k_y_inverse<T, P, kTC>), grid, dim3(kTC * ColPlan<P>::threads), 0, s,  \
-                         u, b.A, b.twy, c->ky, c->ny, c->Nz1, c->Nx1);                          \
+                         u, b.A, b.twy, c->ky, c->ny, c->Nz1, c->Nx1, c->act);                          \
     break;
     DNSMI_FOR_EACH_P(X)
 #undef X
@@ -915,10 +939,10 @@ static int launch_z_inverse(Ctx* c, hipStream_t s) {
   case P:                                                                                       \
     if (c->zinv_seq)                                                                            \
       hipLaunchKernelGGL((k_z_inverse_seq<T, P, kTC>), grid, dim3(kTC * ColPlan<P>::threads), 0, \
-                         s, b.A, b.B, b.twz, c->kz, c->nz, c->Py, c->Nx1);                      \
+                         s, b.A, b.B, b.twz, c->kz, c->nz, c->Py, c->Nx1, c->act);                      \
     else                                                                                        \
       hipLaunchKernelGGL((k_z_inverse<T, P, kTC>), grid, dim3(kTC * ColPlan<P>::threads), 0, s,  \
-                         b.A, b.B, b.twz, c->kz, c->nz, c->Py, c->Nx1);                         \
+                         b.A, b.B, b.twz, c->kz, c->nz, c->Py, c->Nx1, c->act);                         \
     break;
     DNSMI_FOR_EACH_P(X)
 #undef X
@@ -946,12 +970,12 @@ static void launch_x_len(Ctx* c, const Bufs<T>& b, const cx<T>* xin, cx<T>* xout
   if (split) {
     if constexpr (XSplit<P>::value)
       hipLaunchKernelGGL((k_x_nonlinear_split<T, P, R>), grid, dim3(R * XSplit<P>::tpr), 0, s,
-                         xin, xout, b.twx, c->kx, base, cbase, nbase, c->nx, ny, c->Pz, ny);
+                         xin, xout, b.twx, c->kx, base, cbase, nbase, c->nx, ny, c->Pz, ny, c->act);
   } else {
     constexpr int R1 = kRows;
     const dim3 grid1((nrows + R1 - 1) / R1);
     hipLaunchKernelGGL((k_x_nonlinear<T, P, R1>), grid1, dim3(R1 * Plan<P>::N1), 0, s, xin, xout,
-                       b.twx, c->kx, base, cbase, nbase, c->nx, ny, c->Pz, ny);
+                       b.twx, c->kx, base, cbase, nbase, c->nx, ny, c->Pz, ny, c->act);
   }
 }
 
@@ -997,7 +1021,7 @@ int launch_z_forward_buf(Ctx* c, const cx<T>* in, cx<T>* out, int ny, int kid, h
 #define X(P)                                                                                   \
   case P:                                                                                      \
     hipLaunchKernelGGL((k_z_forward<T, P, kTC>), grid, dim3(kTC * ColPlan<P>::threads), 0, s,  \
-                       in, out, b.twz, c->nz, ny, c->Nx1);                                     \
+                       in, out, b.twz, c->nz, ny, c->Nx1, c->act);                                     \
     break;
     DNSMI_FOR_EACH_P(X)
 #undef X
@@ -1060,6 +1084,7 @@ static StepArgs step_args(Ctx* c) {
   a.c = c->cfg.implicitness;
   a.re = c->cfg.re;
   a.partials = c->partials;
+  a.act = c->act;
   return a;
 }
 
@@ -1085,6 +1110,15 @@ int periodic_predict_and_correct(Ctx* c, const double* state, double* pred, doub
   rc = periodic_rhs_mode<MODE_CORRECT>(c, pred, b, s, &nb);
   if (rc) return rc;
   return finalize_norm(c, nb, err, s);
+}
+
+// The Euler predictor alone: N^0 = rhs(state) into rhs0, u^1 into pred.
+int periodic_predict_only(Ctx* c, const double* state, double* pred, double* rhs0, hipStream_t s) {
+  StepArgs a = step_args(c);
+  a.state = state;
+  a.pred = pred;
+  a.rhs_out = rhs0;
+  return periodic_rhs_mode<MODE_PREDICT>(c, state, a, s, nullptr);
 }
 
 int periodic_iterate_correction(Ctx* c, double* pred, const double* rhs_prev, double* rhs_next,

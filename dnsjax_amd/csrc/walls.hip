@@ -445,7 +445,8 @@ template <typename T, int S1>
 __global__ void __launch_bounds__(kTY* kTM)
     k_walls_curl_t(const cx<double>* __restrict__ u, cx<T>* __restrict__ v,
                    const double* __restrict__ d1w, const int* __restrict__ d1j0,
-                   const double* __restrict__ mkz, int Ny, int M) {
+                   const double* __restrict__ mkz, int Ny, int M, const int* __restrict__ act) {
+  if (act && *act == 0) return;
   constexpr int HALO = S1;
   __shared__ cx<double> tile[3][kTM][kTY + 2 * HALO + 1];
   const int ntile_y = (Ny + kTY - 1) / kTY;
@@ -498,7 +499,9 @@ __global__ void __launch_bounds__(kTY* kTM)
 template <typename T, int P, int TC, int NF>
 __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
     k_z_inverse_plain(const cx<T>* __restrict__ in, cx<T>* __restrict__ out,
-                      const cx<T>* __restrict__ tw, int nz, int Ny, int Nx1) {
+                      const cx<T>* __restrict__ tw, int nz, int Ny, int Nx1,
+                      const int* __restrict__ act) {
+  if (act && *act == 0) return;
   constexpr int N1 = ColPlan<P>::N1, N2 = ColPlan<P>::N2;
   __shared__ cx<T> lds[TC * P];
   __shared__ cx<T> tws[P];
@@ -541,7 +544,9 @@ __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
 // WT5: (3, Ny, Nkz, Nkx) T -> rhs (3, M, Ny) FP64, tiled transpose.
 template <typename T>
 __global__ void __launch_bounds__(kTY* kTM)
-    k_walls_to_spec(const cx<T>* __restrict__ w, cx<double>* __restrict__ rhs, int Ny, int M) {
+    k_walls_to_spec(const cx<T>* __restrict__ w, cx<double>* __restrict__ rhs, int Ny, int M,
+                    const int* __restrict__ act) {
+  if (act && *act == 0) return;
   __shared__ cx<double> tile[kTY][kTM + 1];
   const int ntile_y = (Ny + kTY - 1) / kTY;
   const int c = blockIdx.y;
@@ -577,12 +582,14 @@ struct ImmArgs {
   const int *lk_piv, *hk_piv;
   const double *v1, *v2, *q1, *q2, *minv, *mkx, *mkz;
   cx<double>* scratch;  // 5 vectors [vec][y][mode]
+  const int* act;       // if set and 0: the launch is skipped
   int Ny, M, Nkx;
   double dt, c, nu;
 };
 
 template <int PO>
 __global__ void __launch_bounds__(64) k_imm(ImmArgs a) {
+  if (a.act && *a.act == 0) return;
   constexpr int KL = PO, KU = PO, KUU = 2 * PO, ROWS = 3 * PO + 1, S1 = PO + 1, S2 = PO + 2;
   __shared__ double red[2];
   const int m = blockIdx.x * 64 + threadIdx.x;
@@ -898,7 +905,7 @@ static int walls_rhs_t(Ctx* c, const double* state, double* rhs, hipStream_t s) 
 #define CURL(PO)                                                                                \
   case PO:                                                                                      \
     hipLaunchKernelGGL((k_walls_curl_t<T, PO + 1>), grid, dim3(kTY * kTM), 0, s, u, A, t->d1w,  \
-                       t->d1j0, t->mkz, Ny, M);                                                 \
+                       t->d1j0, t->mkz, Ny, M, c->act);                                         \
     break;
       CURL(2) CURL(4) CURL(6) CURL(8)
 #undef CURL
@@ -911,7 +918,8 @@ static int walls_rhs_t(Ctx* c, const double* state, double* rhs, hipStream_t s) 
 #define X(P)                                                                                    \
   case P:                                                                                       \
     hipLaunchKernelGGL((k_z_inverse_plain<T, P, 16, 6>), dim3(Ny * ntile),                      \
-                       dim3(16 * ColPlan<P>::threads), 0, s, A, B, twz, c->nz, Ny, c->Nx1);     \
+                       dim3(16 * ColPlan<P>::threads), 0, s, A, B, twz, c->nz, Ny, c->Nx1,     \
+                       c->act);                                                                 \
     break;
       DNSMI_FOR_EACH_P(X)
 #undef X
@@ -927,7 +935,7 @@ static int walls_rhs_t(Ctx* c, const double* state, double* rhs, hipStream_t s) 
   {
     ProfScope ps(c, KID_W_Z_FORWARD, s);
     hipLaunchKernelGGL((k_walls_to_spec<T>), dim3(ntile_y * ntile_m, 3), dim3(kTY * kTM), 0, s, B,
-                       reinterpret_cast<cx<double>*>(rhs), Ny, M);
+                       reinterpret_cast<cx<double>*>(rhs), Ny, M, c->act);
   }
   return check_launch(c, "walls_to_spec");
 }
@@ -966,6 +974,7 @@ static int walls_imm(Ctx* c, const double* un, const double* uj, const double* n
   a.mkx = t->mkx;
   a.mkz = t->mkz;
   a.scratch = reinterpret_cast<cx<double>*>(t->scratch);
+  a.act = c->act;
   a.Ny = t->Ny;
   a.M = t->M;
   a.Nkx = t->Nkx;

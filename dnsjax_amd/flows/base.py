@@ -50,6 +50,14 @@ class NativeFlow:
             state = torch.from_numpy(np.ascontiguousarray(state, dtype=np.complex128))
         return state.to(self._native.device).contiguous()
 
+    def get_stats(self, state) -> dict:
+        """E, I, D, E' as 0-d device tensors (``flows/monochromatic.py:508-510``,
+        ``flows/plane_couette.py:616-618``)."""
+        v = self._native.get_stats(state)
+        keys = ("E", "I", "D", "E'") if not self.params.walls else ("E'",)
+        idx = {"E": 0, "I": 1, "D": 2, "E'": 3}
+        return {k: v[idx[k]] for k in keys}
+
     def correct_velocity(self, state):
         """In place (the reference donates its input); returns
         ``(state, norm_corrections)``."""

@@ -1,0 +1,96 @@
+"""Captured time steps: one hipGraph launch per step, and ensembles.
+
+``GraphStepper`` captures ``dnsmi_step_device`` (a whole time step with the
+corrector loop decided on the device) into two hipGraphs that ping-pong
+between two state buffers, so a run of n steps is n graph launches and one
+synchronisation at the end.  Iteration counts are recorded on the device
+and read once.
+
+``Ensemble`` puts many independent trajectories (any mix of sizes, Reynolds
+numbers and precisions, each with its own autotuned kernel variants) into
+one captured graph per step parity: each member's step is a branch forked
+from the capture stream onto one of a few side streams, so members run
+concurrently and the whole ensemble advances by one graph launch.
+
+The reference has no counterpart (it runs one trajectory and reads the
+corrector error on the host every pass, ``__main__.py:161-191``); the
+per-member physics and iteration rule are unchanged.
+"""
+
+from __future__ import annotations
+
+import torch
+
+
+class _Member:
+    def __init__(self, flow, state, slots, capacity):
+        nat = flow._native
+        self.flow = flow
+        self.nat = nat
+        p = flow.params
+        self.slots = p.max_corrector_iterations if slots is None else min(slots, p.max_corrector_iterations)
+        self.bufs = [state.detach().clone().contiguous(), nat.empty()]
+        self.rhs = [nat.empty(), nat.empty()]
+        self.counts = torch.zeros(capacity, dtype=torch.int32, device=nat.device)
+        nat.step_reset()
+
+    def enqueue(self, parity: int) -> None:
+        self.nat.step_device(self.bufs[parity], self.bufs[1 - parity], self.rhs[0], self.rhs[1], self.slots,
+                             self.counts)
+
+
+class Ensemble:
+    """Trajectories advanced together by one captured graph launch per step."""
+
+    def __init__(self, flows, states, n_streams: int = 8, slots: int | None = None,
+                 capacity: int = 4096):
+        if len(flows) != len(states) or not flows:
+            raise ValueError("need one state per flow")
+        self.members = [_Member(f, s, slots, capacity) for f, s in zip(flows, states)]
+        self.capacity = capacity
+        self.parity = 0
+        self.steps = 0
+        self.graphs = [self._capture(0, n_streams), self._capture(1, n_streams)]
+
+    def _capture(self, parity: int, n_streams: int):
+        g = torch.cuda.CUDAGraph()
+        side = [torch.cuda.Stream() for _ in range(min(n_streams, len(self.members)))]
+        torch.cuda.synchronize()
+        with torch.cuda.graph(g):
+            main = torch.cuda.current_stream()
+            for s in side:
+                s.wait_stream(main)
+            for i, m in enumerate(self.members):
+                with torch.cuda.stream(side[i % len(side)]):
+                    m.enqueue(parity)
+            for s in side:
+                main.wait_stream(s)
+        return g
+
+    def step(self, n: int = 1) -> None:
+        if self.steps + n > self.capacity:
+            raise ValueError("iteration-count capacity exceeded")
+        for _ in range(n):
+            self.graphs[self.parity].replay()
+            self.parity ^= 1
+            self.steps += 1
+
+    def states(self):
+        return [m.bufs[self.parity] for m in self.members]
+
+    def counts(self):
+        return [m.counts[: self.steps].tolist() for m in self.members]
+
+    def step_stats(self):
+        return [m.nat.step_stats() for m in self.members]
+
+
+class GraphStepper(Ensemble):
+    """A single trajectory stepped by captured graphs (one launch per step)."""
+
+    def __init__(self, flow, state, slots: int | None = None, capacity: int = 4096):
+        super().__init__([flow], [state], n_streams=1, slots=slots, capacity=capacity)
+
+    @property
+    def state(self):
+        return self.states()[0]

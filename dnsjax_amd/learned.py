@@ -1,0 +1,273 @@
+"""Learned corrector guess for the triply periodic stepper.
+
+No counterpart in the reference.  The reference starts every step's
+corrector from the Euler predictor ``u^1 = (u^n ldt_1 + N^n) ildt_2``
+(``flows/monochromatic.py:320-333``) and iterates the incremental
+Crank-Nicolson correction ``u <- u + c ildt_2 (N(u) - N_prev)``
+(``flows/monochromatic.py:336-355``) until the correction's norm is below
+the tolerance (``__main__.py:161-191``).  The fixed point is
+``u* = u^1 + c ildt_2 (N(u*) - N^n)``.
+
+Here the first corrector pass starts from ``u^1 + c ildt_2 dN`` with
+``rhs_prev = N^n + dN``, where ``dN`` estimates ``N(u*) - N^n``; the
+incremental pass from there is exactly the full form
+``u <- ildt_2 (ldt_1 u^n + c N(u) + (1 - c) N^n)``.  ``dN`` is the
+quadratic extrapolation ``2 N^n - 3 N^{n-1} + N^{n-2}`` of the nonlinear
+term's time series, plus a per-mode network's estimate of the remaining
+third difference.  The iteration map, the fixed point and the stopping
+rule are the reference's; only the starting point moves, so a good guess
+saves whole RHS evaluations (the plain scheme spends ``2 + c`` per step,
+the guessed one ``1 + passes``).
+
+The network (``Net``) is trained on the GPU with the library's own FP32
+kernels on samples of the solver's nonlinear-term history (``collect``,
+``make_dataset``, ``train``) and runs in bf16 on the matrix cores inside
+the guess kernel.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass, field
+
+import torch
+
+from . import _lib
+from .native import _ptr, _stream
+
+
+class Net:
+    """Parameters, Adam moments and packed inference operands of one network."""
+
+    D_IN, D_OUT = 16, 6
+
+    def __init__(self, hidden: int = 32, seed: int = 0, device=None):
+        if hidden not in (32, 64):
+            raise ValueError("hidden width must be 32 or 64")
+        self.lib = _lib.load()
+        self.H = hidden
+        n, wbytes, nbias = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_longlong()
+        _lib.check(self.lib.dnsmi_mlp_sizes(hidden, ctypes.byref(n), ctypes.byref(wbytes), ctypes.byref(nbias)))
+        self.nparams = n.value
+        device = torch.device("cuda", torch.cuda.current_device()) if device is None else device
+        self.device = device
+        g = torch.Generator().manual_seed(seed)
+        H, di, do = hidden, self.D_IN, self.D_OUT
+        parts = [torch.randn(di, H, generator=g, dtype=torch.float64) / math.sqrt(di), torch.zeros(H),
+                 torch.randn(H, H, generator=g, dtype=torch.float64) / math.sqrt(H), torch.zeros(H),
+                 torch.randn(H, do, generator=g, dtype=torch.float64) * (0.1 / math.sqrt(H)), torch.zeros(do)]
+        self.params = torch.cat([p.reshape(-1).float() for p in parts]).to(device)
+        assert self.params.numel() == self.nparams
+        self.m = torch.zeros_like(self.params)
+        self.v = torch.zeros_like(self.params)
+        self.grad = torch.zeros_like(self.params)
+        self.loss = torch.zeros((), dtype=torch.float32, device=device)
+        self.steps = 0
+        self.wb = torch.zeros(wbytes.value, dtype=torch.uint8, device=device)
+        self.bias = torch.zeros(nbias.value, dtype=torch.float32, device=device)
+        self._part = torch.empty(0, dtype=torch.float32, device=device)
+
+    # ------------------------------------------------------------ parameters
+    def unpack(self):
+        """(W1, b1, W2, b2, W3, b3) views of the FP32 parameters."""
+        H, di, do = self.H, self.D_IN, self.D_OUT
+        shapes = [(di, H), (H,), (H, H), (H,), (H, do), (do,)]
+        out, o = [], 0
+        for s in shapes:
+            k = math.prod(s)
+            out.append(self.params[o:o + k].view(s))
+            o += k
+        return out
+
+    def set_params(self, flat: torch.Tensor) -> None:
+        self.params.copy_(flat.to(self.params))
+
+    # ------------------------------------------------------------ kernels
+    def compute_grad(self, X: torch.Tensor, T: torch.Tensor, bf16: bool = True) -> torch.Tensor:
+        """FP32 gradient of the mean squared error into ``self.grad``; returns the loss (device)."""
+        B = X.shape[0]
+        need = ((B + 31) // 32) * (self.nparams + 1)
+        if self._part.numel() < need:
+            self._part = torch.empty(need, dtype=torch.float32, device=self.device)
+        _lib.check(self.lib.dnsmi_mlp_grad(self.H, B, _ptr(X), _ptr(T), _ptr(self.params), int(bf16),
+                                           _ptr(self._part), _ptr(self.grad), _ptr(self.loss), _stream()))
+        return self.loss
+
+    def adam(self, lr: float) -> None:
+        self.steps += 1
+        _lib.check(self.lib.dnsmi_mlp_adam(self.H, _ptr(self.params), _ptr(self.grad), _ptr(self.m),
+                                           _ptr(self.v), self.steps, lr, _stream()))
+
+    def pack(self) -> None:
+        """Refresh the bf16 matrix-core operands from the FP32 parameters."""
+        _lib.check(self.lib.dnsmi_mlp_pack(self.H, _ptr(self.params), _ptr(self.wb), _ptr(self.bias),
+                                           _stream()))
+
+    def forward(self, X: torch.Tensor) -> torch.Tensor:
+        """Matrix-core inference on packed operands (call ``pack`` first)."""
+        Y = torch.empty((X.shape[0], self.D_OUT), dtype=torch.float32, device=self.device)
+        _lib.check(self.lib.dnsmi_mlp_forward(self.H, _ptr(X), X.shape[0], _ptr(self.wb), _ptr(self.bias),
+                                              _ptr(Y), _stream()))
+        return Y
+
+    @property
+    def operands(self):
+        return (self.H, self.wb, self.bias)
+
+
+# ---------------------------------------------------------------- data
+def collect(flow, state: torch.Tensor, n_steps: int):
+    """Run the plain stepper for n_steps and return the nonlinear terms
+    N^0..N^n at the start of every step (plus the final state)."""
+    from .driver import run
+
+    snaps = [flow.get_rhs(state).clone()]
+    res = run(flow, state.clone(), n_steps, callback=lambda i, s: snaps.append(flow.get_rhs(s).clone()))
+    if not res.converged:
+        raise RuntimeError("data run did not converge")
+    return snaps, res.state
+
+
+def _rms(t: torch.Tensor) -> float:
+    return float(torch.sqrt(torch.mean(t.real**2 + t.imag**2) / 2))
+
+
+@dataclass
+class Scales:
+    s_1: float
+    s_2: float
+    s_out: float
+
+    def astuple(self):
+        return (self.s_1, self.s_2, self.s_out)
+
+
+def fit_scales(snaps) -> Scales:
+    """Feature and target scales over the collected history: 1/rms(D1),
+    1/rms(D2) and rms(D3), D1..D3 the first to third differences of N."""
+    n = len(snaps)
+    if n < 4:
+        raise ValueError("need at least four nonlinear terms")
+    d1 = sum(_rms(snaps[i] - snaps[i - 1]) for i in range(1, n)) / (n - 1)
+    d2 = sum(_rms(snaps[i] - 2 * snaps[i - 1] + snaps[i - 2]) for i in range(2, n)) / (n - 2)
+    d3 = sum(_rms(snaps[i] - 3 * snaps[i - 1] + 3 * snaps[i - 2] - snaps[i - 3]) for i in range(3, n)) / (n - 3)
+    return Scales(1.0 / max(d1, 1e-300), 1.0 / max(d2, 1e-300), max(d3, 1e-300))
+
+
+def make_dataset(flow, snaps, scales: Scales, samples_per_window: int, seed: int = 0):
+    """Features X (S,16) and targets T (S,6) from every window of four
+    consecutive nonlinear terms, at uniformly drawn modes."""
+    nat = flow._native
+    n_modes = math.prod(flow.params.spec_shape)
+    g = torch.Generator(device=nat.device).manual_seed(seed)
+    Xs, Ts = [], []
+    for i in range(2, len(snaps) - 1):
+        modes = torch.randint(0, n_modes, (samples_per_window,), generator=g, device=nat.device,
+                              dtype=torch.int64)
+        X = torch.empty((samples_per_window, Net.D_IN), dtype=torch.float32, device=nat.device)
+        T = torch.empty((samples_per_window, Net.D_OUT), dtype=torch.float32, device=nat.device)
+        nat.mlp_features(snaps[i - 2], snaps[i - 1], snaps[i], snaps[i + 1], modes, *scales.astuple(), X, T)
+        Xs.append(X)
+        Ts.append(T)
+    return torch.cat(Xs), torch.cat(Ts)
+
+
+def train(net: Net, X: torch.Tensor, T: torch.Tensor, iters: int, batch: int = 8192, lr: float = 2e-3,
+          seed: int = 0, bf16: bool = True, log_every: int = 0):
+    """Mini-batch Adam with a cosine learning-rate decay; returns the loss history."""
+    g = torch.Generator(device=X.device).manual_seed(seed)
+    hist = []
+    for it in range(iters):
+        idx = torch.randint(0, X.shape[0], (batch,), generator=g, device=X.device)
+        net.compute_grad(X[idx].contiguous(), T[idx].contiguous(), bf16=bf16)
+        net.adam(lr * 0.5 * (1 + math.cos(math.pi * it / iters)))
+        if log_every and (it % log_every == 0 or it == iters - 1):
+            hist.append((it, float(net.loss)))
+    net.pack()
+    return hist
+
+
+def mean_loss(net: Net, X: torch.Tensor, T: torch.Tensor) -> float:
+    """Loss of the packed (bf16 matrix-core) network on a held-out set, next
+    to the loss of predicting zero (quadratic extrapolation alone)."""
+    Y = net.forward(X)
+    return float(torch.mean(torch.sum((Y - T) ** 2, dim=1))), float(torch.mean(torch.sum(T**2, dim=1)))
+
+
+# ---------------------------------------------------------------- stepping
+@dataclass
+class GuessRunResult:
+    state: object
+    passes: list = field(default_factory=list)   # corrector passes per step (1 RHS each)
+    errors: list = field(default_factory=list)
+    converged: bool = True
+    nan: bool = False
+    floor_stops: int = 0
+
+    @property
+    def rhs_evals(self) -> int:
+        return sum(c + 1 for c in self.passes)
+
+
+MODES = {"plain": 0, "linear": 1, "learned": 2, "quadratic": 3}
+
+
+def run_guessed(flow, state, n_steps: int, mode: int = 2, net: Net | None = None,
+                scales: Scales | None = None, callback=None) -> GuessRunResult:
+    """The reference's time loop (``__main__.py:125-203``) with the corrector
+    started from a guess: mode 1 linear, 3 quadratic extrapolation of N,
+    2 quadratic plus the network.  The first steps of a run lack history and
+    start from the plain predictor (step 0) and linear extrapolation (step
+    1).  Up to ``max_corrector_iterations + 1`` passes per step (the plain
+    scheme's RHS budget); the loop stops after a step that did not
+    converge, and the mixed form's noise-floor rule applies as in
+    ``driver.run``.  The state handed to ``callback`` is a buffer that
+    later steps reuse: copy it to keep it."""
+    if mode not in (0, 1, 2, 3):
+        raise ValueError("mode must be 0, 1, 2 or 3")
+    if mode == 2 and (net is None or scales is None):
+        raise ValueError("mode 2 needs a trained net and its scales")
+    p = flow.params
+    nat = flow._native
+    tol, max_pass = p.corrector_tolerance, p.max_corrector_iterations + 1
+    ratio = p.corrector_floor_ratio
+    cap = p.corrector_floor_cap * tol
+    sc = (1.0, 1.0, 1.0) if scales is None else scales.astuple()
+    ops = None if net is None else net.operands
+    hist = [torch.zeros(nat.shape + (2,), dtype=torch.float32, device=nat.device) for _ in range(2)]
+    res = GuessRunResult(state=state)
+    state = state.clone()
+    pred = nat.empty()
+    rhs = [nat.empty(), nat.empty()]
+    err_t = nat.scalar()
+    last_error = 0.0
+    for it in range(n_steps):
+        if not last_error < tol:
+            break
+        m = 0 if it == 0 else (min(mode, 1) if it == 1 else mode)
+        nat.guess(state, pred, rhs[0], hist[0], hist[1], m, ops, sc)
+        hist.reverse()  # hist[0] = N^n, hist[1] = N^{n-1}
+        c, err, floor, cur = 0, math.inf, False, 0
+        while err > tol and c < max_pass:
+            nat.iterate_correction(state, pred, rhs[cur], rhs[1 - cur], err_t)
+            cur = 1 - cur
+            prev, err = err, float(err_t)
+            c += 1
+            if ratio is not None and err <= cap and err > ratio * prev:
+                floor = True
+                break
+        pred, state = state, pred
+        state, _ = flow.correct_velocity(state)
+        res.passes.append(c)
+        res.errors.append(err)
+        res.floor_stops += int(floor)
+        last_error = 0.0 if floor else err
+        if callback is not None:
+            callback(it + 1, state)
+        if math.isnan(err):
+            res.nan = True
+            break
+    res.state = state
+    res.converged = not (last_error > tol) and not res.nan
+    return res

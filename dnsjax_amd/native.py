@@ -121,6 +121,52 @@ class NativeContext:
                                                     _ptr(rhs_prev), _ptr(rhs_next), _ptr(err), _stream()))
         return pred, rhs_next, err
 
+    def get_stats(self, state, out=None):
+        """Device tensor (4,) float64: E, I, D, E'."""
+        self.check_field(state, "state")
+        out = torch.empty(4, dtype=torch.float64, device=self.device) if out is None else out
+        self._chk(self.lib.dnsmi_get_stats(self.handle, _ptr(state), _ptr(out), _stream()))
+        return out
+
+    # ---------------------------------------------------------------- device-side steps
+    def step_device(self, state, nxt, rhs_a, rhs_b, slots, counts=None, tol=None, max_iter=None,
+                    floor_ratio=None, floor_cap=None):
+        """Enqueue one full time step with the corrector decided on the device
+        (capturable; no host synchronisation).  The new state goes to `nxt`."""
+        p = self.params
+        tol = p.corrector_tolerance if tol is None else tol
+        max_iter = p.max_corrector_iterations if max_iter is None else max_iter
+        ratio = p.corrector_floor_ratio if floor_ratio is None else floor_ratio
+        cap = (p.corrector_floor_cap * tol) if floor_cap is None else floor_cap
+        ncap = 0 if counts is None else counts.numel()
+        self._chk(self.lib.dnsmi_step_device(self.handle, _ptr(state), _ptr(nxt), _ptr(rhs_a), _ptr(rhs_b),
+                                             slots, _ptr(counts), ncap, tol, max_iter,
+                                             0.0 if ratio is None else ratio, cap, _stream()))
+
+    def mlp_features(self, n_m2, n_m1, n_cur, n_next, modes, s_1, s_2, s_out, X, T=None):
+        """Training features (and targets when n_next and T are given)."""
+        self._chk(self.lib.dnsmi_mlp_features(self.handle, _ptr(n_m2), _ptr(n_m1), _ptr(n_cur), _ptr(n_next),
+                                              _ptr(modes), modes.numel(), s_1, s_2, s_out, _ptr(X), _ptr(T),
+                                              _stream()))
+
+    def guess(self, state, pred, rhs_prev, hist1, hist2, mode, net=None, scales=(1.0, 1.0, 1.0)):
+        """Predictor with a corrector guess (mode 0 plain, 1 linear, 3
+        quadratic extrapolation, 2 quadratic plus the network `net` =
+        (H, packed weights, bias)).  hist2 receives N^n."""
+        H, wb, bias = (32, None, None) if net is None else net
+        s_1, s_2, s_out = scales
+        self._chk(self.lib.dnsmi_guess(self.handle, _ptr(state), _ptr(pred), _ptr(rhs_prev), _ptr(hist1),
+                                       _ptr(hist2), _ptr(wb), _ptr(bias), H, s_1, s_2, s_out, mode, _stream()))
+
+    def step_stats(self) -> dict:
+        out = (ctypes.c_double * 6)()
+        self._chk(self.lib.dnsmi_step_stats(self.handle, out))
+        return {"steps": int(out[0]), "total_iterations": int(out[1]), "unconverged": int(out[2]),
+                "nan": int(out[3]), "last_error": out[4], "last_floor_stop": bool(out[5])}
+
+    def step_reset(self) -> None:
+        self._chk(self.lib.dnsmi_step_reset(self.handle))
+
     @property
     def tuning(self) -> str:
         return self.lib.dnsmi_tuning(self.handle).decode()

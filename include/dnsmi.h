@@ -130,6 +130,87 @@ const char* dnsmi_tuning(const dnsmi_ctx* ctx);
  * as doubles).  Synchronous, host memory only. */
 int dnsmi_walls_table(dnsmi_ctx* ctx, int which, double* out, long long n);
 
+/* One full time step with the corrector loop decided on the device
+ * (__main__.py:161-191 without host reads): predict_and_correct from
+ * `state` into `next`, then up to `slots` iterate_correction passes, each
+ * skipped on the device once error <= tol or max_iter extra passes ran
+ * (and, if floor_ratio > 0, once a pass failed to shrink the error by that
+ * ratio while error <= floor_cap: the mixed form's noise floor), then
+ * correct_velocity(next).  rhs_a / rhs_b are two RHS work arrays.  No host
+ * synchronisation; the launch sequence is fixed, so it can be captured in
+ * a hipGraph.  counts (device int[counts_cap], may be NULL) receives each
+ * step's extra-iteration count in order.  `state` is not modified. */
+int dnsmi_step_device(dnsmi_ctx* ctx, const double* state, double* next, double* rhs_a,
+                      double* rhs_b, int slots, int* counts, int counts_cap, double tol,
+                      int max_iter, double floor_ratio, double floor_cap, void* stream);
+/* Synchronising read of the device step counters: out6 = {steps, total extra
+ * iterations, steps not converged, NaN steps, last error, last step stopped
+ * at the noise floor}; dnsmi_step_reset zeroes them. */
+int dnsmi_step_stats(dnsmi_ctx* ctx, double* out6);
+int dnsmi_step_reset(dnsmi_ctx* ctx);
+
+/* Diagnostics of a state into stats_dev[4] = {E, I, D, E'} (device doubles).
+ * Replaces get_stats: flows/monochromatic.py:433-510 (E' = |u|^2/2, input,
+ * dissipation, energy; the forcing constants use F = nu (2 pi/Ly)^2, equal
+ * to the reference's pi^2/(4 Re) at Ly = 4) and flows/plane_couette.py:591-618
+ * (E' only; E = E', I = D = 0). */
+int dnsmi_get_stats(dnsmi_ctx* ctx, const double* state, double* stats_dev, void* stream);
+
+/* ---- Learned corrector guess (no counterpart in the reference) ----------
+ * A per-mode network x (16 features) -> 6 outputs, hidden width H = 32 or
+ * 64, SiLU, parameters in FP32 packed as W1 (16,H) | b1 (H) | W2 (H,H) | b2 (H)
+ * | W3 (H,6) | b3 (6).  It predicts the second difference of the nonlinear
+ * term per mode so that the first corrector pass starts near the implicit
+ * solution; the corrector loop and its stopping rule are unchanged
+ * (__main__.py:161-191), only its starting point moves.  All buffers are
+ * device memory; entry points without a context report through
+ * dnsmi_last_error(NULL). */
+
+/* Sizes: parameter count, packed bf16 weight bytes and bias floats. */
+int dnsmi_mlp_sizes(int H, long long* nparams, long long* packed_weight_bytes,
+                    long long* bias_floats);
+/* Gradient of L = (1/B) sum (y - t)^2 over X (B,16), T (B,6), FP32, samples
+ * summed in a fixed order (deterministic).  part: ceil(B/32) * (nparams + 1)
+ * floats of scratch; grad: nparams floats; loss_dev: 1 float or NULL.
+ * bf16 != 0 rounds weights and layer inputs to bfloat16 as inference does
+ * (straight-through gradients). */
+int dnsmi_mlp_grad(int H, int B, const float* X, const float* T, const float* params, int bf16,
+                   float* part, float* grad, float* loss_dev, void* stream);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8; step counts from 1). */
+int dnsmi_mlp_adam(int H, float* params, const float* grad, float* m, float* v, int step, float lr,
+                   void* stream);
+/* Pack params into matrix-core operand tiles (bf16) and a bias block. */
+int dnsmi_mlp_pack(int H, const float* params, void* packed_weights, float* bias, void* stream);
+/* Matrix-core inference: Y (B,6) = net(X (B,16)). */
+int dnsmi_mlp_forward(int H, const float* X, int B, const void* packed_weights, const float* bias,
+                      float* Y, void* stream);
+/* Training samples from four successive nonlinear terms N^{n-2}, N^{n-1},
+ * N^n, N^{n+1} (spectral layout, complex128) of a periodic context: X (ns,16)
+ * for the flat mode indices `modes` (over (ky, kz, kx)) and, when n_next
+ * and T are both given, T (ns,6) = w(k) (N^{n+1} - 3 N^n + 3 N^{n-1} -
+ * N^{n-2}) / s_out, w = 1 / (1 + c dt |k|^2 / Re): the residual of quadratic
+ * extrapolation.  Features: D1 = N^n - N^{n-1} times s_1, D2 = N^n -
+ * 2 N^{n-1} + N^{n-2} times s_2 (older terms rounded to FP32 as the step
+ * history keeps them), min(dt |k|^2/Re, 8), k / k_max. */
+int dnsmi_mlp_features(dnsmi_ctx* ctx, const double* n_m2, const double* n_m1,
+                       const double* n_cur, const double* n_next, const long long* modes,
+                       int nsamples, float s_1, float s_2, float s_out, float* X, float* T,
+                       void* stream);
+/* Predictor with a guess, periodic contexts: N^n = rhs(state), u^1 =
+ * predict(state, N^n) (flows/monochromatic.py:320-333), then per mode
+ * dN = 0 (mode 0), D1 (mode 1, linear extrapolation), D1 + D2 (mode 3,
+ * quadratic) or D1 + D2 plus the network's residual (mode 2),
+ * Leray-projected and zero at the mean mode; pred = u^1 + c ildt_2 dN and
+ * rhs_prev = N^n + dN, so that iterate_correction(state, pred, rhs_prev)
+ * is the reference's fixed-point iteration in its full form
+ * u <- ildt_2 (ldt_1 u^n + c N(u) + (1 - c) N^n), started from the guess.
+ * hist1 = N^{n-1}, hist2 = N^{n-2}: 3 * spec_elems complex values each as
+ * FP32 re/im pairs; hist2 is overwritten with N^n, so the caller swaps the
+ * two for the next step.  Mode 0 (the plain predictor) needs no history. */
+int dnsmi_guess(dnsmi_ctx* ctx, const double* state, double* pred, double* rhs_prev,
+                const float* hist1, float* hist2, const void* packed_weights, const float* bias,
+                int H, float s_1, float s_2, float s_out, int mode, void* stream);
+
 /* Per-kernel timing with HIP events on the launch stream (no counterpart in
  * the reference, whose bench.timer (bench.py:26-67) times whole calls).
  * Between begin and end every kernel launch of the context is bracketed by

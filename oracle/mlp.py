@@ -1,0 +1,181 @@
+"""Oracle: the per-mode corrector-guess network, FP64 NumPy -- TEST INFRASTRUCTURE ONLY.
+
+The reference has no learned component (SURVEY a'); this restates the
+network this build trains and runs, so that the GPU kernels' forward
+values and parameter gradients can be checked:
+
+    x (D_IN) -> z1 = W1^T x + b1 -> h1 = silu(z1)
+             -> z2 = W2^T h1 + b2 -> h2 = silu(z2)
+             -> y  = W3^T h2 + b3            (D_OUT)
+    loss = (1/B) sum_samples sum_outputs (y - t)^2
+
+with W1 (D_IN, H), W2 (H, H), W3 (H, D_OUT).  ``bf16=True`` rounds the
+weights and the layer inputs to bfloat16 (round to nearest even) as the
+GPU inference kernel feeds its matrix cores.
+
+``features`` and ``guess`` restate the corrector-guess step built on the
+network for the periodic stepper (no reference counterpart; it reuses the
+reference's predictor ``flows/monochromatic.py:320-333`` and the Leray
+projection ``flows/monochromatic.py:516-565``).
+
+Pinned by: central finite differences of the loss (tests/test_oracle_mlp.py),
+which fix every gradient independently of the backward formulas; for
+``guess``: mode 0 reduces to the reference predictor exactly, and the
+fixed-point identity (the corrector pass from the guess equals the plain
+iteration map, tests/test_oracle_mlp.py) holds for any dN.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+
+def silu(z):
+    return z / (1.0 + np.exp(-z))
+
+
+def dsilu(z):
+    s = 1.0 / (1.0 + np.exp(-z))
+    return s * (1.0 + z * (1.0 - s))
+
+
+def to_bf16(a: np.ndarray) -> np.ndarray:
+    """Round float values to bfloat16 (nearest even), returned as float64."""
+    f = np.asarray(a, dtype=np.float32)
+    u = f.view(np.uint32).astype(np.uint64)
+    r = ((u >> 16) & 1) + 0x7FFF
+    u = ((u + r) >> 16) << 16
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def forward(params, x, bf16=False):
+    W1, b1, W2, b2, W3, b3 = params
+    q = to_bf16 if bf16 else (lambda a: a)
+    z1 = q(x) @ q(W1) + b1
+    h1 = silu(z1)
+    z2 = q(h1) @ q(W2) + b2
+    h2 = silu(z2)
+    y = q(h2) @ q(W3) + b3
+    return y, (x, z1, h1, z2, h2)
+
+
+def loss_and_grads(params, x, t, bf16=False):
+    """Loss and parameter gradients.  ``bf16`` rounds weights and layer
+    inputs as ``forward`` does and differentiates straight through the
+    rounding (the rounded values are used in the backward pass)."""
+    W1, b1, W2, b2, W3, b3 = params
+    q = to_bf16 if bf16 else (lambda a: a)
+    y, (x, z1, h1, z2, h2) = forward(params, x, bf16)
+    xq, h1q, h2q = q(x), q(h1), q(h2)
+    W2q, W3q = q(W2), q(W3)
+    B = x.shape[0]
+    loss = float(np.sum((y - t) ** 2) / B)
+    dy = 2.0 * (y - t) / B
+    gW3 = h2q.T @ dy
+    gb3 = dy.sum(0)
+    dz2 = (dy @ W3q.T) * dsilu(z2)
+    gW2 = h1q.T @ dz2
+    gb2 = dz2.sum(0)
+    dz1 = (dz2 @ W2q.T) * dsilu(z1)
+    gW1 = xq.T @ dz1
+    gb1 = dz1.sum(0)
+    return loss, (gW1, gb1, gW2, gb2, gW3, gb3)
+
+
+def init(d_in, h, d_out, seed):
+    rng = np.random.default_rng(seed)
+    W1 = rng.standard_normal((d_in, h)) / np.sqrt(d_in)
+    W2 = rng.standard_normal((h, h)) / np.sqrt(h)
+    W3 = rng.standard_normal((h, d_out)) / np.sqrt(h) * 0.1
+    return [W1, np.zeros(h), W2, np.zeros(h), W3, np.zeros(d_out)]
+
+
+def adam(params, grads, m, v, step, lr, b1=0.9, b2=0.999, eps=1e-8):
+    """One Adam update (Kingma & Ba 2015), in place."""
+    for i in range(len(params)):
+        m[i] = b1 * m[i] + (1 - b1) * grads[i]
+        v[i] = b2 * v[i] + (1 - b2) * grads[i] ** 2
+        mh = m[i] / (1 - b1**step)
+        vh = v[i] / (1 - b2**step)
+        params[i] = params[i] - lr * mh / (np.sqrt(vh) + eps)
+    return params
+
+
+# ------------------------------------------------------------ corrector guess
+def mode_weight(flow):
+    """w = ildt_2 / dt per mode, 0 for the mean mode (held fixed)."""
+    from .periodic import Fourier  # noqa: F401  (flow.fourier is one)
+
+    f, p = flow.fourier, flow.p
+    w = 1.0 / (1.0 - p.dt * p.implicitness * f.lapl / p.re)
+    w = np.broadcast_to(w, f.lapl.shape).copy()
+    w[0, 0, 0] = 0.0
+    return w
+
+
+def k_features(flow):
+    """Features 12..15 per mode: min(dt |k|^2/Re, 8), kx, ky, kz over k_max."""
+    f, p = flow.fourier, flow.p
+    kmax = max(2 * np.pi / p.lx * (p.nx // 2), 2 * np.pi / p.ly * (p.ny // 2), 2 * np.pi / p.lz * (p.nz // 2))
+    shape = f.lapl.shape
+    return [np.minimum(p.dt * (-f.lapl) / p.re, 8.0),
+            np.broadcast_to(f.kx / kmax, shape), np.broadcast_to(f.ky / kmax, shape),
+            np.broadcast_to(f.kz / kmax, shape)]
+
+
+def features(n_m2, n_m1, n_cur, n_next, modes, flow, s_1, s_2, s_out):
+    """X (S,16), T (S,6) for flat mode indices over (ky, kz, kx): D1, D2
+    (older terms rounded to FP32), k features; T the weighted residual of
+    quadratic extrapolation."""
+    f32 = lambda a: a.astype(np.float32).astype(np.float64)  # noqa: E731
+    X = np.zeros((len(modes), 16))
+    for q in range(6):
+        comp = (lambda a: a.real) if q % 2 == 0 else (lambda a: a.imag)
+        cur = comp(n_cur[q // 2].reshape(-1)[modes])
+        p1 = f32(comp(n_m1[q // 2].reshape(-1)[modes]))
+        p2 = f32(comp(n_m2[q // 2].reshape(-1)[modes]))
+        X[:, q] = (cur - p1) * s_1
+        X[:, 6 + q] = ((cur - 2 * p1) + p2) * s_2
+    for j, kf in enumerate(k_features(flow)):
+        X[:, 12 + j] = kf.reshape(-1)[modes]
+    T = None
+    if n_next is not None:
+        w = mode_weight(flow).reshape(-1)[modes]
+        T = np.zeros((len(modes), 6))
+        for q in range(6):
+            d3 = (n_next[q // 2] - 3 * n_cur[q // 2] + 3 * n_m1[q // 2] - n_m2[q // 2]).reshape(-1)[modes]
+            T[:, q] = w * (d3.real if q % 2 == 0 else d3.imag) / s_out
+    return X, T
+
+
+def guess(state, h1, h2, flow, mode, params=None, scales=(1.0, 1.0, 1.0)):
+    """(pred, rhs_prev, N^n) of the guessed predictor.
+
+    h1, h2: N^{n-1}, N^{n-2} as complex values (rounded to FP32 by the
+    caller as the device history keeps them).  mode 0: plain predictor;
+    1: dN = D1; 3: dN = D1 + D2; 2: plus s_out * net / w (bf16 inference)."""
+    from . import periodic as P
+
+    s_1, s_2, s_out = scales
+    f, p = flow.fourier, flow.p
+    n0 = P.get_rhs(state, flow)
+    u1 = P.predict(state, n0, flow)
+    if mode == 0:
+        return u1, n0, n0.copy()
+    d1 = n0 - h1
+    dn = d1 if mode == 1 else d1 + ((n0 - 2 * h1) + h2)
+    w = mode_weight(flow)
+    if mode == 2:
+        modes = np.arange(w.size)
+        X, _ = features(h2, h1, n0, None, modes, flow, s_1, s_2, s_out)
+        Y, _ = forward(params, X, bf16=True)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            for c in range(3):
+                net = (Y[:, 2 * c] + 1j * Y[:, 2 * c + 1]).reshape(w.shape)
+                dn[c] = dn[c] + np.where(w > 0, s_out * net / np.where(w > 0, w, 1.0), 0.0)
+    kd = f.kx * dn[0] + f.ky * dn[1] + f.kz * dn[2]
+    d = np.stack([dn[0] + f.kx * kd * f.inv_lapl, dn[1] + f.ky * kd * f.inv_lapl,
+                  dn[2] + f.kz * kd * f.inv_lapl])
+    d[:, 0, 0, 0] = 0.0
+    pred = u1 + p.implicitness * p.dt * w * d
+    return pred, n0 + d, n0.copy()

@@ -1,0 +1,175 @@
+"""Learned corrector guess on the GPU against the oracle.
+
+Training kernels (FP32 gradients, Adam) against the FP64 oracle, the
+matrix-core bf16 inference against the oracle's bf16 emulation, the guess
+kernel against ``oracle.mlp.guess``, and whole runs: a guessed run must stay
+on the plain run's trajectory to the corrector tolerance.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+from oracle import mlp as M  # noqa: E402
+from oracle import periodic as P  # noqa: E402
+from dnsjax_amd.params import Params  # noqa: E402
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _net_with(H, params_np):
+    from dnsjax_amd.learned import Net
+
+    net = Net(H, seed=0)
+    net.set_params(torch.from_numpy(np.concatenate([p.reshape(-1) for p in params_np])).float())
+    return net
+
+
+def _random_params(H, seed):
+    rng = np.random.default_rng(seed)
+    p = M.init(16, H, 6, seed=seed)
+    p[1] = rng.standard_normal(H) * 0.1
+    p[3] = rng.standard_normal(H) * 0.1
+    p[5] = rng.standard_normal(6) * 0.1
+    p[4] = p[4] * 10
+    return [a.astype(np.float32).astype(np.float64) for a in p]
+
+
+@pytest.mark.parametrize("H", [32, 64])
+@pytest.mark.parametrize("bf16", [False, True])
+def test_gradients_match_oracle(H, bf16):
+    rng = np.random.default_rng(H + int(bf16))
+    B = 77  # not a multiple of the 32-sample block
+    params = _random_params(H, 7)
+    x = rng.standard_normal((B, 16)).astype(np.float32).astype(np.float64)
+    t = rng.standard_normal((B, 6)).astype(np.float32).astype(np.float64)
+    loss, grads = M.loss_and_grads(params, x, t, bf16=bf16)
+    net = _net_with(H, params)
+    dev = net.device
+    gl = net.compute_grad(torch.from_numpy(x).float().to(dev), torch.from_numpy(t).float().to(dev), bf16=bf16)
+    got = net.grad.cpu().double().numpy()
+    tol = 3e-3 if bf16 else 1e-4  # bf16: an FP32 difference may flip one rounding
+    assert abs(float(gl) - loss) <= (1e-3 if bf16 else 1e-5) * loss
+    o = 0
+    for g in grads:
+        k = g.size
+        assert rel(got[o:o + k], g.reshape(-1)) <= tol, (H, bf16, o)
+        o += k
+
+
+def test_adam_matches_oracle():
+    H = 32
+    params = _random_params(H, 3)
+    rng = np.random.default_rng(5)
+    net = _net_with(H, params)
+    m = [np.zeros_like(p) for p in params]
+    v = [np.zeros_like(p) for p in params]
+    for step in range(1, 4):
+        grads = [rng.standard_normal(p.shape).astype(np.float32).astype(np.float64) for p in params]
+        net.grad.copy_(torch.from_numpy(np.concatenate([g.reshape(-1) for g in grads])).float())
+        net.adam(1e-2)
+        M.adam(params, grads, m, v, step, 1e-2)
+    got = net.params.cpu().double().numpy()
+    want = np.concatenate([p.reshape(-1) for p in params])
+    assert np.max(np.abs(got - want)) <= 2e-6 * np.max(np.abs(want))
+
+
+@pytest.mark.parametrize("H", [32, 64])
+def test_matrix_core_forward_matches_bf16_oracle(H):
+    rng = np.random.default_rng(11)
+    B = 1000
+    params = _random_params(H, 9)
+    x = rng.standard_normal((B, 16)).astype(np.float32).astype(np.float64)
+    want, _ = M.forward(params, x, bf16=True)
+    net = _net_with(H, params)
+    net.pack()
+    got = net.forward(torch.from_numpy(x).float().to(net.device)).cpu().double().numpy()
+    assert rel(got, want) <= 2e-3
+    full, _ = M.forward(params, x, bf16=False)
+    assert rel(got, full) <= 5e-2  # bf16 inference stays near the FP64 network
+
+
+def _periodic(n, re, precision="fp64", dt=0.01):
+    from dnsjax_amd.flows import TriplyPeriodicFlow
+
+    L = 2 * math.pi
+    return TriplyPeriodicFlow(Params(system="kolmogorov", nx=n, ny=n, nz=n, lx=L, ly=L, lz=L, re=re, dt=dt,
+                                     corrector_tolerance=1e-10, max_corrector_iterations=10,
+                                     precision=precision, tile_cols=16))
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+def test_guess_kernel_matches_oracle(mode):
+    from dnsjax_amd.fields import periodic_field
+
+    n, L = 16, 2 * math.pi
+    flow = _periodic(n, 40.0)
+    of = P.Flow(P.PeriodicParams(nx=n, ny=n, nz=n, lx=L, ly=L, lz=L, re=40.0, dt=0.01, system="kolmogorov"))
+    u = periodic_field(n, n, n, L, L, L, 0.1, 3)
+    h1 = P.get_rhs(periodic_field(n, n, n, L, L, L, 0.1, 4), of).astype(np.complex64)
+    h2 = P.get_rhs(periodic_field(n, n, n, L, L, L, 0.1, 6), of).astype(np.complex64)
+    params = _random_params(32, 2)
+    scales = (1.0, 3.0, 0.05)
+    want_pred, want_rhs, want_hist = M.guess(u, h1.astype(np.complex128), h2.astype(np.complex128), of, mode,
+                                             params, scales)
+    net = _net_with(32, params)
+    net.pack()
+    nat = flow._native
+    dev = lambda h: torch.from_numpy(np.stack([h.real, h.imag], -1).astype(np.float32)).to(nat.device)  # noqa: E731
+    d1, d2 = dev(h1).contiguous(), dev(h2).contiguous()
+    pred, rhs = nat.empty(), nat.empty()
+    nat.guess(flow.to_device(u), pred, rhs, d1, d2, mode, net.operands, scales)
+    got_pred, got_rhs = pred.cpu().numpy(), rhs.cpu().numpy()
+    u1 = P.predict(u, P.get_rhs(u, of), of)
+    assert rel(got_rhs, want_rhs) <= (1e-4 if mode == 2 else 1e-12)
+    if mode == 0:
+        assert rel(got_pred, want_pred) <= 1e-12
+    else:
+        assert rel(got_pred - u1, want_pred - u1) <= (5e-3 if mode == 2 else 1e-10)
+    h = d2.cpu().numpy().astype(np.float64)  # hist2 now holds N^n
+    w = np.stack([want_hist.real, want_hist.imag], -1)
+    assert np.max(np.abs(h - w)) <= 1e-7 * np.max(np.abs(w))  # kept as FP32
+    assert torch.equal(d1, dev(h1))  # hist1 is read only
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_guessed_runs_stay_on_the_plain_trajectory(precision):
+    """Extrapolated and learned guesses: every step converges and the states
+    stay within the corrector tolerance's reach of the plain run."""
+    from dnsjax_amd.driver import run
+    from dnsjax_amd.learned import Net, collect, fit_scales, make_dataset, mean_loss, run_guessed, train
+
+    flow = _periodic(32, 60.0, precision, dt=0.01)
+    u0 = flow.init_state(seed=5, rms=0.1)
+    snaps, u_mid = collect(flow, u0, 12)
+    scales = fit_scales(snaps)
+    X, T = make_dataset(flow, snaps, scales, 20000, seed=1)
+    assert X.shape == (10 * 20000, 16)  # 13 terms: 10 windows of four
+    net = Net(32, seed=0)
+    train(net, X, T, iters=300, batch=4096, lr=3e-3)
+    loss, base = mean_loss(net, X, T)
+    assert loss < base  # better than plain extrapolation on its own data
+
+    n = 10
+    plain = run(flow, u_mid.clone(), n)
+    assert plain.converged
+    for mode in (1, 3, 2):
+        g = run_guessed(flow, u_mid.clone(), n, mode=mode, net=net, scales=scales)
+        assert g.converged and len(g.passes) == n, (mode, g.passes)
+        d = rel(g.state.cpu().numpy(), plain.state.cpu().numpy())
+        assert d <= (1e-8 if precision == "fp64" else 1e-7), (mode, d)
+        if mode in (1, 3):
+            assert g.rhs_evals <= plain.rhs_evals, (mode, g.passes, plain.counts)

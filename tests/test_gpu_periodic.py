@@ -176,3 +176,16 @@ def test_autotuner_reports_choices():
     flow = make("periodic_16_kolmogorov")
     t = flow._native.tuning
     assert "y_inverse_seq=" in t and "z_inverse_seq=" in t and "x_split=" in t
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_stats_match_oracle(case):
+    g = np.load(os.path.join(GOLDEN, case + ".npz"))
+    c = CASES[case]
+    flow = make(case)
+    st = flow.get_stats(flow.to_device(g["pred2"]))
+    op = P.PeriodicParams(nx=c["nx"], ny=c["ny"], nz=c["nz"], lx=c["L"], ly=c["L"], lz=c["L"], re=c["re"],
+                          system=c["system"], dt=c["dt"], tilt_degree=c["tilt"])
+    ref = P.get_stats(g["pred2"], P.Flow(op))
+    for k in ("E", "I", "D", "E'"):
+        assert abs(float(st[k]) - ref[k]) <= 1e-12 * max(abs(ref[k]), 1e-30), (k, float(st[k]), ref[k])

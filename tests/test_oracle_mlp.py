@@ -1,0 +1,134 @@
+"""Pin the MLP oracle: analytic gradients against central finite differences."""
+
+import numpy as np
+import pytest
+
+from oracle import mlp as M
+
+
+def test_gradients_match_finite_differences():
+    rng = np.random.default_rng(0)
+    d_in, h, d_out, B = 16, 8, 6, 5
+    params = M.init(d_in, h, d_out, seed=1)
+    params[1] = rng.standard_normal(h) * 0.1
+    params[3] = rng.standard_normal(h) * 0.1
+    params[5] = rng.standard_normal(d_out) * 0.1
+    x = rng.standard_normal((B, d_in))
+    t = rng.standard_normal((B, d_out))
+    _, grads = M.loss_and_grads(params, x, t)
+    eps = 1e-6
+    for pi, (p, g) in enumerate(zip(params, grads)):
+        flat = p.reshape(-1)
+        for idx in rng.choice(flat.size, size=min(6, flat.size), replace=False):
+            old = flat[idx]
+            flat[idx] = old + eps
+            lp, _ = M.loss_and_grads(params, x, t)
+            flat[idx] = old - eps
+            lm, _ = M.loss_and_grads(params, x, t)
+            flat[idx] = old
+            fd = (lp - lm) / (2 * eps)
+            assert abs(fd - g.reshape(-1)[idx]) <= 1e-7 + 1e-6 * abs(fd), (pi, idx, fd, g.reshape(-1)[idx])
+
+
+def test_bf16_rounding():
+    v = np.array([1.0, 1.00390625, 1.0 + 2**-8 + 2**-9, -3.14159, 1e-20])
+    r = M.to_bf16(v)
+    assert r[0] == 1.0 and r[1] == 1.0  # tie to even
+    assert r[2] == 1.0078125
+    assert abs(r[3] + 3.140625) < 1e-12
+    assert abs(r[4] - 1e-20) / 1e-20 < 2**-8
+
+
+def test_adam_decreases_loss():
+    rng = np.random.default_rng(2)
+    params = M.init(4, 8, 2, seed=3)
+    x = rng.standard_normal((64, 4))
+    t = np.stack([np.sin(x[:, 0]), x[:, 1] * x[:, 2]], 1)
+    m = [np.zeros_like(p) for p in params]
+    v = [np.zeros_like(p) for p in params]
+    l0, _ = M.loss_and_grads(params, x, t)
+    for step in range(1, 301):
+        _, g = M.loss_and_grads(params, x, t)
+        M.adam(params, g, m, v, step, 1e-2)
+    l1, _ = M.loss_and_grads(params, x, t)
+    assert l1 < 0.3 * l0
+
+
+def test_bf16_gradients_track_full_precision():
+    """Straight-through bf16 gradients stay within bf16 precision of the exact ones."""
+    rng = np.random.default_rng(4)
+    params = M.init(16, 32, 6, seed=5)
+    x = rng.standard_normal((40, 16))
+    t = rng.standard_normal((40, 6))
+    _, g = M.loss_and_grads(params, x, t)
+    _, gb = M.loss_and_grads(params, x, t, bf16=True)
+    for a, b in zip(g, gb):
+        assert np.linalg.norm(a - b) <= 3e-2 * np.linalg.norm(a)
+
+
+# ---------------------------------------------------------------- corrector guess
+import math  # noqa: E402
+
+from oracle import periodic as P  # noqa: E402
+from dnsjax_amd.fields import periodic_field  # noqa: E402
+
+
+def _flow(n=8, re=40.0):
+    L = 2 * math.pi
+    return P.Flow(P.PeriodicParams(nx=n, ny=n, nz=n, lx=L, ly=L, lz=L, re=re, dt=0.01, system="kolmogorov"))
+
+
+def _f32(a):
+    return a.astype(np.complex64).astype(np.complex128)
+
+
+def test_guess_mode0_is_the_plain_predictor():
+    flow = _flow()
+    u = periodic_field(8, 8, 8, 2 * math.pi, 2 * math.pi, 2 * math.pi, 0.1, 3)
+    z = np.zeros_like(u)
+    pred, rhs, hist = M.guess(u, z, z, flow, 0)
+    n0 = P.get_rhs(u, flow)
+    assert np.array_equal(rhs, n0) and np.array_equal(hist, n0)
+    assert np.array_equal(pred, P.predict(u, n0, flow))
+
+
+@pytest.mark.parametrize("mode", [1, 2, 3])
+def test_guess_keeps_the_corrector_map(mode):
+    """One corrector pass from the guess equals the plain iteration map
+    u1 + c ildt_2 (N(pred) - N^n) at every mode, whatever dN is."""
+    flow = _flow()
+    L = 2 * math.pi
+    u = periodic_field(8, 8, 8, L, L, L, 0.1, 3)
+    h1 = _f32(P.get_rhs(periodic_field(8, 8, 8, L, L, L, 0.1, 4), flow))
+    h2 = _f32(P.get_rhs(periodic_field(8, 8, 8, L, L, L, 0.1, 5), flow))
+    params = M.init(16, 32, 6, seed=1)
+    pred, rhs_prev, _ = M.guess(u, h1, h2, flow, mode, params, (1.0, 2.0, 0.5))
+    n0 = P.get_rhs(u, flow)
+    u1 = P.predict(u, n0, flow)
+    assert np.linalg.norm(pred - u1) > 1e-3 * np.linalg.norm(u1 - u)  # the guess moved
+    nxt, _ = P.correct(u, pred, rhs_prev, P.get_rhs(pred, flow), flow)
+    plain = u1 + flow.p.implicitness * (P.get_rhs(pred, flow) - n0) * flow.ildt_2
+    assert np.max(np.abs(nxt - plain)) <= 1e-13 * np.max(np.abs(plain))
+    # ... which is the full form ildt_2 (ldt_1 u^n + c N(pred) + (1 - c) N^n)
+    c = flow.p.implicitness
+    full = flow.ildt_2 * (flow.ldt_1 * u + c * P.get_rhs(pred, flow) + (1 - c) * n0)
+    full[:, 0, 0, 0] = nxt[:, 0, 0, 0]  # the mean mode is held by correct_velocity
+    assert np.max(np.abs(nxt - full)) <= 1e-13 * np.max(np.abs(full))
+
+
+def test_extrapolation_is_exact_for_polynomial_histories():
+    """A nonlinear term moving quadratically in time is extrapolated exactly
+    by mode 3 (target zero), linearly by mode 1 only when linear."""
+    flow = _flow()
+    L = 2 * math.pi
+    a = _f32(P.get_rhs(periodic_field(8, 8, 8, L, L, L, 0.1, 3), flow))
+    b = _f32(P.get_rhs(periodic_field(8, 8, 8, L, L, L, 0.1, 9), flow)) * 0.25
+    e = _f32(P.get_rhs(periodic_field(8, 8, 8, L, L, L, 0.1, 11), flow)) * 0.125
+    seq = [a + k * b + k * k * e for k in range(4)]  # N^{n-2}, N^{n-1}, N^n, N^{n+1}
+    modes = np.arange(a[0].size)
+    X, T = M.features(seq[0], seq[1], seq[2], seq[3], modes, flow, 1.0, 1.0, 1.0)
+    scale = np.max(np.abs(b))
+    assert np.max(np.abs(T)) <= 1e-12 * scale
+    # D1 = N^n - N^{n-1} = b + 3 e ; D2 = 2 e
+    assert np.allclose(X[:, 0], (b[0] + 3 * e[0]).real.reshape(-1), atol=1e-6 * scale)
+    assert np.allclose(X[:, 6], (2 * e[0]).real.reshape(-1), atol=1e-6 * scale)
