@@ -176,6 +176,52 @@ def cpu_baseline(budget_s: float) -> dict:
             "sample": f"{n} steps of the same 128^3 workload after 1 warm-up step, numpy FP64 oracle, 1 thread"}
 
 
+def ensemble_form(steps: int, n_streams: int = 8) -> dict:
+    """64 Kolmogorov trajectories (16 each at 32^3, 48^3, 64^3, 96^3; Re drawn
+    uniformly in [20, 80] with seed 3; dt 0.01; seeds 300 + i, RMS 0.05;
+    mixed form, tol 1e-10), advanced by one captured graph launch per step,
+    against the same 64 trajectories run one after another with the eager
+    driver.  Aggregate time-steps/s over all members."""
+    import numpy as np
+    import torch
+
+    from dnsjax_amd.driver import run
+    from dnsjax_amd.flows import TriplyPeriodicFlow
+    from dnsjax_amd.graph import Ensemble
+    from dnsjax_amd.params import kolmogorov_2pi
+
+    sizes = [32] * 16 + [48] * 16 + [64] * 16 + [96] * 16
+    res = np.random.default_rng(3).uniform(20.0, 80.0, len(sizes))
+    t0 = time.perf_counter()
+    flows = [TriplyPeriodicFlow(kolmogorov_2pi(n, float(r), 0.01, corrector_tolerance=TOL,
+                                               max_corrector_iterations=10, precision="mixed"))
+             for n, r in zip(sizes, res)]
+    states = [f.init_state(seed=300 + i, rms=0.05) for i, f in enumerate(flows)]
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    m = len(flows)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    seq_counts = [run(f, s.clone(), steps).counts for f, s in zip(flows, states)]
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t0
+    ens = Ensemble(flows, states, n_streams=n_streams, capacity=steps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ens.step(steps)
+    torch.cuda.synchronize()
+    t_ens = time.perf_counter() - t0
+    same = ens.counts() == seq_counts
+    st = ens.step_stats()
+    return {"members": m, "sizes": "16 x {32,48,64,96}^3", "steps": steps, "n_streams": n_streams,
+            "value": round(m * steps / t_ens, 2), "unit": "member time-steps/s",
+            "sequential_value": round(m * steps / t_seq, 2),
+            "speedup_vs_sequential": round(t_seq / t_ens, 3),
+            "counts_match_sequential": same,
+            "unconverged": sum(x["unconverged"] for x in st), "setup_s": round(t_setup, 2),
+            "note": "the graph timing includes the first launch of each of its two graphs"}
+
+
 def profile_workload(steps: int) -> None:
     """The workload profiled by rocprofv3 (kernel trace and PMC passes).
 
@@ -213,6 +259,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--ensemble-steps", type=int, default=50,
+                    help="steps of the 64-member ensemble form (0 skips it)")
     ap.add_argument("--profile-workload", action="store_true",
                     help="run the rocprofv3 workload (calibration copy + a few steps) and exit")
     a = ap.parse_args()
@@ -251,6 +299,8 @@ def main():
                   "corrector_iters_per_step": round(mixed["corrector_iters_per_step"], 3),
                   "roofline": rmix["dominant"], "rhs_roofline": rmix["rhs"]},
     }
+    if a.ensemble_steps > 0:
+        out["ensemble"] = ensemble_form(a.ensemble_steps)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(a.cpu_seconds)
     else:
