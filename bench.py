@@ -8,8 +8,10 @@ correct_velocity): Kolmogorov flow, nx = ny = nz = 128 (192^3 physical on
 the 3/2 grid), 2 pi box, Re = 100, dt = 0.005, implicitness 0.5,
 corrector tolerance 1e-10, seeded solenoidal field (seed 1, RMS 0.1).
 The state is resident in HBM before the timed region.  `value` is the
-plain FP64 form; the mixed form (FP32 transforms / nonlinear product) is
-reported beside it.  Prints one JSON line.
+plain FP64 form; beside it: the mixed form (FP32 transforms / nonlinear
+product), the mixed form with the learned corrector guess (A/B over one
+trajectory segment, `mixed_learned_vs_fp64`), and the 64-member ensemble.
+Prints one JSON line.
 """
 
 from __future__ import annotations
@@ -176,6 +178,128 @@ def cpu_baseline(budget_s: float) -> dict:
             "sample": f"{n} steps of the same 128^3 workload after 1 warm-up step, numpy FP64 oracle, 1 thread"}
 
 
+def learned_form(steps: int, collect_steps: int = 30, hidden: int = 32, train_iters: int = 1500) -> dict:
+    """Mixed form with the learned corrector guess, against the plain forms
+    over the same trajectory segment.
+
+    The network is trained on the GPU (the library's FP32 training kernels)
+    on the nonlinear-term history of a `collect_steps` run of the same
+    workload; that run and the training are set-up, timed separately.
+    Then, from the state it ends in, `steps` steps of each form are timed:
+    plain FP64 (the reference's method), plain mixed, mixed with quadratic
+    extrapolation only, and mixed with the learned guess.  The end-state
+    energy of each mixed form is compared with the plain FP64 run's.
+    """
+    import torch
+
+    from dnsjax_amd.driver import run
+    from dnsjax_amd.flows import TriplyPeriodicFlow
+    from dnsjax_amd.learned import Net, collect, fit_scales, make_dataset, mean_loss, run_guessed, train
+    from dnsjax_amd.params import kolmogorov_2pi
+
+    def mk(prec):
+        return TriplyPeriodicFlow(kolmogorov_2pi(N, RE, DT, corrector_tolerance=TOL,
+                                                 max_corrector_iterations=10, precision=prec))
+
+    mixed, fp64 = mk("mixed"), mk("fp64")
+    u0 = mixed.init_state(seed=SEED, rms=RMS)
+    t0 = time.perf_counter()
+    snaps, u_mid = collect(mixed, u0, collect_steps)
+    sc = fit_scales(snaps)
+    X, T = make_dataset(mixed, snaps, sc, 100000, seed=1)
+    nv = 100000
+    Xv, Tv, X, T = X[-nv:], T[-nv:], X[:-nv], T[:-nv]
+    del snaps
+    torch.cuda.synchronize()
+    t_data = time.perf_counter() - t0
+    net = Net(hidden, seed=0)
+    t0 = time.perf_counter()
+    train(net, X, T, iters=train_iters, batch=16384, lr=3e-3)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    loss, base = mean_loss(net, Xv, Tv)
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        return r, time.perf_counter() - t
+
+    def energy(flow, state):
+        return float(flow.get_stats(state)["E'"])
+
+    rows = {}
+    ref, t_ref = timed(lambda: run(fp64, u_mid.clone(), steps))
+    e_ref = energy(fp64, ref.state)
+    rows["fp64_plain"] = (t_ref, ref.rhs_evals, ref.converged, ref.state)
+    r, t = timed(lambda: run(mixed, u_mid.clone(), steps))
+    rows["mixed_plain"] = (t, r.rhs_evals, r.converged, r.state)
+    for name, mode in (("mixed_quadratic", 3), ("mixed_learned", 2)):
+        g, t = timed(lambda: run_guessed(mixed, u_mid.clone(), steps, mode=mode, net=net, scales=sc))
+        rows[name] = (t, g.rhs_evals, g.converged, g.state)
+    # the same forms with the corrector decided on the device, one captured
+    # graph launch per step (no host read of the error per pass)
+    from dnsjax_amd.graph import GraphStepper
+    from dnsjax_amd.learned import GuessedGraphStepper
+
+    gp = GraphStepper(fp64, u_mid, capacity=steps + 4)
+    _, t = timed(lambda: gp.step(steps))
+    rows["fp64_plain_graph"] = (t, sum(c + 2 for c in gp.counts()[0]), gp.step_stats()[0]["unconverged"] == 0,
+                                gp.state)
+    gg = GuessedGraphStepper(mixed, u_mid, mode=2, net=net, scales=sc, capacity=steps + 4)
+    _, t = timed(lambda: gg.step(steps - 2))
+    t = t * steps / (steps - 2)  # the two eager start-up steps are not in the timed region
+    rows["mixed_learned_graph"] = (t, sum(c + 1 for c in gg.passes()[2:]) * steps / (steps - 2),
+                                   gg.step_stats()["unconverged"] == 0, gg.state)
+    out = {}
+    for name, (t, rhs, conv, st) in rows.items():
+        out[name] = {"steps_per_s": round(steps / t, 3), "ms_per_step": round(1e3 * t / steps, 4),
+                     "rhs_evals_per_step": round(rhs / steps, 3), "converged": conv,
+                     "energy_rel_err_vs_fp64": abs(energy(fp64, st) - e_ref) / abs(e_ref)}
+    sp = out["mixed_learned"]["steps_per_s"] / out["fp64_plain"]["steps_per_s"]
+    spg = out["mixed_learned_graph"]["steps_per_s"] / out["fp64_plain_graph"]["steps_per_s"]
+    return {"forms": out, "speedup_vs_fp64": round(sp, 3), "speedup_vs_fp64_graph": round(spg, 3),
+            "speedup_vs_mixed_quadratic": round(out["mixed_learned"]["steps_per_s"]
+                                                / out["mixed_quadratic"]["steps_per_s"], 3),
+            "net": {"hidden": hidden, "train_iters": train_iters, "train_s": round(t_train, 3),
+                    "data_s": round(t_data, 3), "samples": int(X.shape[0]),
+                    "heldout_loss": round(loss, 4), "heldout_loss_without_net": round(base, 4)},
+            "segment": f"{steps} steps after a {collect_steps}-step training run from the workload's initial state"}
+
+
+def walls_form(steps: int, warmup: int = 2) -> dict:
+    """Plane Couette 64 x 128 x 64 (CGL points in y), Lx = 2 pi, Lz = pi,
+    Re 400, fd_order 4, dt 0.01, seeded perturbation RMS 0.05 (seed 1), plain
+    FP64, the reference's default corrector tolerance (1e-5) and limit (10).
+    Time-steps/s of the driver loop, with per-kernel event times."""
+    import torch
+
+    from dnsjax_amd.driver import run
+    from dnsjax_amd.flows import PlaneCouetteFlow
+    from dnsjax_amd.params import Params
+
+    p = Params(system="plane-couette", nx=64, ny=128, nz=64, lx=2 * math.pi, lz=math.pi, re=400.0, dt=0.01,
+               fd_order=4, corrector_tolerance=1e-5, max_corrector_iterations=10)
+    flow = PlaneCouetteFlow(p)
+    state = flow.init_state(seed=SEED, rms=0.05)
+    state = run(flow, state, warmup).state
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = run(flow, state, steps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    flow._native.profile_begin()
+    r2 = run(flow, r.state, steps)
+    torch.cuda.synchronize()
+    prof = flow._native.profile_end()
+    return {"workload": "plane Couette 64x128x64, p=4, Re 400, dt 0.01, RMS 0.05, tol 1e-5 (reference default)",
+            "steps_per_s": round(steps / el, 3), "ms_per_step": round(1e3 * el / steps, 4),
+            "corrector_iters_per_step": round(sum(r.counts) / max(len(r.counts), 1), 3),
+            "converged": r.converged and r2.converged,
+            "kernels_avg_us": {k: round(1e3 * ms / n, 2) for k, (ms, n) in prof.items()}}
+
+
 def ensemble_form(steps: int, n_streams: int = 8) -> dict:
     """64 Kolmogorov trajectories (16 each at 32^3, 48^3, 64^3, 96^3; Re drawn
     uniformly in [20, 80] with seed 3; dt 0.01; seeds 300 + i, RMS 0.05;
@@ -228,7 +352,8 @@ def profile_workload(steps: int) -> None:
     First a plain device-to-device copy of CAL_BYTES (the calibration
     dispatch for the FETCH_SIZE / WRITE_SIZE counters: it must read and
     write exactly CAL_BYTES), then `steps` steps of the FP64 and of the
-    mixed form at 128^3.
+    mixed form at 128^3, then a short training run of the corrector-guess
+    network and `steps` steps of the mixed form with the learned guess.
     """
     import torch
 
@@ -250,6 +375,17 @@ def profile_workload(steps: int) -> None:
         state, counts = run_steps(flow, state, steps)
         torch.cuda.synchronize()
         print(f"{prec}: {steps} steps, corrector iterations {counts}", flush=True)
+    # the learned guess: a short training run, then `steps` guessed steps
+    from dnsjax_amd.learned import Net, collect, fit_scales, make_dataset, run_guessed, train
+
+    snaps, state = collect(flow, state, 6)
+    sc = fit_scales(snaps)
+    X, T = make_dataset(flow, snaps, sc, 16384, seed=1)
+    net = Net(32, seed=0)
+    train(net, X, T, iters=20, batch=16384)
+    g = run_guessed(flow, state, steps, mode=2, net=net, scales=sc)
+    torch.cuda.synchronize()
+    print(f"mixed + learned guess: {steps} steps, corrector passes {g.passes}", flush=True)
 
 
 def main():
@@ -259,6 +395,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--walls-steps", type=int, default=20,
+                    help="steps of the plane Couette 64x128x64 form (0 skips it)")
+    ap.add_argument("--learned-steps", type=int, default=40,
+                    help="steps of each form in the mixed+learned A/B (0 skips it)")
     ap.add_argument("--ensemble-steps", type=int, default=50,
                     help="steps of the 64-member ensemble form (0 skips it)")
     ap.add_argument("--profile-workload", action="store_true",
@@ -297,8 +437,14 @@ def main():
         "mixed": {"value": round(mixed["steps_per_s"], 3), "ms_per_step": round(mixed["ms_per_step"], 4),
                   "speedup_vs_fp64": round(mixed["steps_per_s"] / fp64["steps_per_s"], 3),
                   "corrector_iters_per_step": round(mixed["corrector_iters_per_step"], 3),
-                  "roofline": rmix["dominant"], "rhs_roofline": rmix["rhs"]},
+                  "roofline": rmix["dominant"], "rhs_roofline": rmix["rhs"], "kernels": rmix["kernels"]},
     }
+    if a.walls_steps > 0:
+        out["walls"] = walls_form(a.walls_steps)
+    if a.learned_steps > 0:
+        lf = learned_form(a.learned_steps)
+        out["mixed_learned"] = lf
+        out["mixed_learned_vs_fp64"] = lf["speedup_vs_fp64"]
     if a.ensemble_steps > 0:
         out["ensemble"] = ensemble_form(a.ensemble_steps)
     if not a.no_cpu:

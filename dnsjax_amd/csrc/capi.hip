@@ -69,6 +69,11 @@ int step_device(Ctx* c, const double* state, double* pred, double* rhs_a, double
                 int* counts, int counts_cap, double tol, int max_iter, double floor_ratio,
                 double floor_cap, hipStream_t s);
 int step_read_ctl(Ctx* c, double* out6);
+int step_device_guess(Ctx* c, const double* state, double* pred, double* rhs_a, double* rhs_b,
+                      const float* h1, float* h2, const uint16_t* wb, const float* bias, int H,
+                      float s_1, float s_2, float s_out, int mode, int slots, int* counts,
+                      int counts_cap, double tol, int max_pass, double floor_ratio,
+                      double floor_cap, hipStream_t s);
 int step_alloc_ctl(Ctx* c);
 int mlp_grad(int H, int B, const float* X, const float* T, const float* prm, int bf16, float* part,
              float* grad, float* loss, hipStream_t s);
@@ -339,6 +344,26 @@ int dnsmi_guess(dnsmi_ctx* ctx, const double* state, double* pred, double* rhs_p
   return dnsmi::guess_periodic(c, state, pred, rhs_prev, hist1, hist2,
                                static_cast<const uint16_t*>(packed_weights), bias, H, s_1, s_2,
                                s_out, mode, as_stream(stream));
+}
+
+int dnsmi_step_device_guess(dnsmi_ctx* ctx, const double* state, double* next, double* rhs_a,
+                            double* rhs_b, const float* hist1, float* hist2,
+                            const void* packed_weights, const float* bias, int H, float s_1,
+                            float s_2, float s_out, int mode, int slots, int* counts,
+                            int counts_cap, double tol, int max_pass, double floor_ratio,
+                            double floor_cap, void* stream) {
+  DNSMI_CHECK_CTX(ctx)
+  DNSMI_REQUIRE(state && next && rhs_a && rhs_b && hist1 && hist2, "step_device_guess: null pointer");
+  DNSMI_REQUIRE(mode != 2 || (packed_weights && bias), "step_device_guess: network weights missing");
+  DNSMI_REQUIRE(slots >= 0 && max_pass >= 1 && (counts || counts_cap == 0),
+                "step_device_guess: bad slots / max_pass / counts");
+  DNSMI_REQUIRE((const void*)state != next && (void*)rhs_a != rhs_b && (void*)next != rhs_a &&
+                    (void*)next != rhs_b && (const void*)hist1 != hist2,
+                "step_device_guess: buffers must be distinct");
+  return dnsmi::step_device_guess(c, state, next, rhs_a, rhs_b, hist1, hist2,
+                                  static_cast<const uint16_t*>(packed_weights), bias, H, s_1, s_2,
+                                  s_out, mode, slots, counts, counts_cap, tol, max_pass,
+                                  floor_ratio, floor_cap, as_stream(stream));
 }
 
 const char* dnsmi_tuning(const dnsmi_ctx* ctx) { return ctx ? ctx->impl.tuning.c_str() : ""; }

@@ -226,6 +226,16 @@ struct ColPlan {
 // The list of instantiated lengths, for host-side dispatch.
 #define DNSMI_FOR_EACH_P(X) X(12) X(16) X(24) X(32) X(48) X(64) X(72) X(96) X(128) X(144) X(192)
 
+// A copy of v that the compiler cannot treat as loop-invariant.  Multi-pass
+// kernels take their thread index through it at the top of each pass, so
+// that the indices, table values and LDS twiddles derived from it are
+// recomputed per pass instead of being hoisted out of the pass loop and
+// kept live in registers across all passes (which cost occupancy).
+__device__ __forceinline__ int opaque(int v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
 // ------------------------------------------------------------ two-step helpers
 // LDS layout for a tile of TC independent transforms (columns): element
 // (k1, b) of column col at ((k1 * N2 + b) * TC + col), col fastest, so a

@@ -67,7 +67,7 @@ __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
   __shared__ cx<T> tws[P];
   for (int i = threadIdx.x; i < P; i += blockDim.x) tws[i] = tw[i];
   __syncthreads();
-  const int tid = threadIdx.x, col = tid % TC, j = tid / TC;
+  const int tid = threadIdx.x, col = tid % TC, j_ = tid / TC;
   const int ntile = (Nx1 + TC - 1) / TC;
   const int kz = blockIdx.x / ntile;
   const int kx = (blockIdx.x - kz * ntile) * TC + col;
@@ -80,6 +80,7 @@ __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
   // pass 2: u_y -> F1.
 #pragma unroll 1
   for (int pass = 0; pass < 3; ++pass) {
+    const int j = opaque(j_);
     const int c = pass == 0 ? 0 : (pass == 1 ? 2 : 1);
     const int fa = c;
     const int fb = pass == 0 ? 4 : 3;
@@ -134,7 +135,7 @@ __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
   __shared__ cx<T> tws[P];
   for (int i = threadIdx.x; i < P; i += blockDim.x) tws[i] = tw[i];
   __syncthreads();
-  const int tid = threadIdx.x, col = tid % TC, j = tid / TC;
+  const int tid = threadIdx.x, col = tid % TC, j_ = tid / TC;
   const int ntile = (Nx1 + TC - 1) / TC;
   const int y = blockIdx.x / ntile;
   const int kx = (blockIdx.x - y * ntile) * TC + col;
@@ -147,6 +148,7 @@ __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
   // outputs: 0 u_x, 1 u_y, 2 u_z, 3 w_x, 4 Z(i kz Yu_x), 5 Z(-Y(i ky u_x))
 #pragma unroll 1
   for (int pass = 0; pass < 4; ++pass) {
+    const int j = opaque(j_);
     cx<T> xa[N1], xb[N1];
     if (j < N2) {
 #pragma unroll
@@ -217,7 +219,7 @@ __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
   __shared__ cx<T> tws[P];
   for (int i = threadIdx.x; i < P; i += blockDim.x) tws[i] = tw[i];
   __syncthreads();
-  const int tid = threadIdx.x, col = tid % TC, j = tid / TC;
+  const int tid = threadIdx.x, col = tid % TC, j_ = tid / TC;
   const int ntile = (Nx1 + TC - 1) / TC;
   const int kz = blockIdx.x / ntile;
   const int kx = (blockIdx.x - kz * ntile) * TC + col;
@@ -229,6 +231,7 @@ __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
   // pass -> (input component, multiply by i ky, output field)
 #pragma unroll 1
   for (int pass = 0; pass < 5; ++pass) {
+    const int j = opaque(j_);
     const int c = pass < 2 ? 0 : (pass < 4 ? 2 : 1);
     const bool iky = (pass == 1 || pass == 3);
     const int f = pass == 0 ? 0 : (pass == 1 ? 4 : (pass == 2 ? 2 : (pass == 3 ? 3 : 1)));
@@ -273,7 +276,7 @@ __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
   __shared__ cx<T> tws[P];
   for (int i = threadIdx.x; i < P; i += blockDim.x) tws[i] = tw[i];
   __syncthreads();
-  const int tid = threadIdx.x, col = tid % TC, j = tid / TC;
+  const int tid = threadIdx.x, col = tid % TC, j_ = tid / TC;
   const int ntile = (Nx1 + TC - 1) / TC;
   const int y = blockIdx.x / ntile;
   const int kx = (blockIdx.x - y * ntile) * TC + col;
@@ -287,6 +290,7 @@ __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
   //       4 Z(F2)->2, 5 Z(-F4)->5
 #pragma unroll 1
   for (int pass = 0; pass < 6; ++pass) {
+    const int j = opaque(j_);
     const int fo = pass == 0 ? 0 : (pass == 1 ? 4 : (pass == 2 ? 1 : (pass == 3 ? 3 : (pass == 4 ? 2 : 5))));
     if (j < N2) {
       cx<T> x[N1];
@@ -361,7 +365,7 @@ __global__ void __launch_bounds__(R * Plan<P>::N1, 2)
   fill_plan_twiddles<N1, N2>(tab_i, tw);
   fill_plan_twiddles<N2, N1>(tab_f, tw);
   __syncthreads();
-  const int tid = threadIdx.x, r = tid / N1, j = tid - r * N1;
+  const int tid = threadIdx.x, r = tid / N1, j_ = tid - r * N1;
   const int nrows = Py * Pz;
   const int row = blockIdx.x * R + r;
   const bool ok = row < nrows;
@@ -375,6 +379,7 @@ __global__ void __launch_bounds__(R * Plan<P>::N1, 2)
 
   // Inverse transform of one (u, w) pair into physical values re/im.
   auto inverse_pair = [&](int pair, T* re, T* im) {
+    const int j = opaque(j_);
     if (j < N2) {
       cx<T> z[N1];
 #pragma unroll
@@ -426,6 +431,7 @@ __global__ void __launch_bounds__(R * Plan<P>::N1, 2)
 
   // Forward transform of one real component, truncated to kx < nx/2.
   auto forward_store = [&](int c, const T* v_re) {
+    const int j = opaque(j_);
     {
       cx<T> v[N2];
 #pragma unroll
@@ -650,7 +656,7 @@ __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
   __shared__ cx<T> tws[P];
   for (int i = threadIdx.x; i < P; i += blockDim.x) tws[i] = tw[i];
   __syncthreads();
-  const int tid = threadIdx.x, col = tid % TC, j = tid / TC;
+  const int tid = threadIdx.x, col = tid % TC, j_ = tid / TC;
   const int ntile = (Nx1 + TC - 1) / TC;
   const int y = blockIdx.x / ntile;
   const int kx = (blockIdx.x - y * ntile) * TC + col;
@@ -663,6 +669,7 @@ __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
   const T inv = static_cast<T>(1.0 / P);
 #pragma unroll 1
   for (int c = 0; c < 3; ++c) {
+    const int j = opaque(j_);
     if (j < N2) {
       cx<T> x[N1];
 #pragma unroll
@@ -1361,7 +1368,11 @@ int periodic_setup(Ctx* c) {
   c->bufB = c->alloc(c->bufB_bytes);
   c->rhs0 = static_cast<double*>(c->alloc(c->spec_elems * 16));
   const long long per_comp = (long long)c->Ny1 * c->Nz1 * c->Nx1;
-  c->max_partials = (int)((per_comp + 255) / 256) + c->Nz1 * ((c->Nx1 + kTC - 1) / kTC) + 1024;
+  // norm partials (one per y-forward block) and the statistics' two per
+  // 256-mode block, whichever is larger
+  const long long stats_partials = 2 * ((per_comp + 255) / 256);
+  const long long norm_partials = (per_comp + 255) / 256 + (long long)c->Nz1 * ((c->Nx1 + kTC - 1) / kTC);
+  c->max_partials = (int)((stats_partials > norm_partials ? stats_partials : norm_partials) + 1024);
   c->partials = static_cast<double*>(c->alloc(sizeof(double) * c->max_partials));
   if (!c->bufA || !c->bufB || !c->rhs0 || !c->partials)
     return c->fail(DNSMI_ENOMEM, "device allocation failed");

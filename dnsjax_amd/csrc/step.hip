@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdint>
 
 #include "context.hpp"
 
@@ -102,6 +103,56 @@ int step_device(Ctx* c, const double* state, double* pred, double* rhs_a, double
   c->act = nullptr;
   if (rc) return rc;
   if (!c->walls && (rc = periodic_correct_velocity(c, pred, nullptr, s))) return rc;
+  hipLaunchKernelGGL(k_ctl_end, dim3(1), dim3(64), 0, s, ctl, counts, counts_cap, tol);
+  return check_launch(c, "ctl_end");
+}
+
+// The guessed step starts with no error: the first pass always runs.
+__global__ void k_ctl_begin_guess(StepCtl* ctl, int max_pass) {
+  if (threadIdx.x != 0) return;
+  ctl->c = 0;
+  ctl->floor = 0;
+  ctl->err = INFINITY;
+  ctl->prev = INFINITY;
+  ctl->active = max_pass > 0;
+}
+
+int guess_periodic(Ctx* c, const double* state, double* pred, double* rhs_prev, const float* h1,
+                   float* h2, const uint16_t* wb, const float* bias, int H, float s_1, float s_2,
+                   float s_out, int mode, hipStream_t s);
+
+// A step started from the corrector guess (periodic): the guessed predictor,
+// then up to `slots` passes, each one RHS, with the same device rule as
+// step_device (max_pass passes at most; the count recorded is the number of
+// passes).
+int step_device_guess(Ctx* c, const double* state, double* pred, double* rhs_a, double* rhs_b,
+                      const float* h1, float* h2, const uint16_t* wb, const float* bias, int H,
+                      float s_1, float s_2, float s_out, int mode, int slots, int* counts,
+                      int counts_cap, double tol, int max_pass, double floor_ratio,
+                      double floor_cap, hipStream_t s) {
+  if (!c->ctl) return c->fail(DNSMI_EINVAL, "control block missing");
+  if (c->walls) return c->fail(DNSMI_EINVAL, "guessed steps: periodic contexts only");
+  StepCtl* ctl = reinterpret_cast<StepCtl*>(c->ctl);
+  double* err = &ctl->err;
+  if (slots > max_pass) slots = max_pass;
+  c->act = nullptr;
+  int rc = guess_periodic(c, state, pred, rhs_a, h1, h2, wb, bias, H, s_1, s_2, s_out, mode, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_ctl_begin_guess, dim3(1), dim3(64), 0, s, ctl, max_pass);
+  if ((rc = check_launch(c, "ctl_begin_guess"))) return rc;
+  c->act = &ctl->active;
+  for (int i = 1; i <= slots && rc == DNSMI_OK; ++i) {
+    const double* rp = (i % 2) ? rhs_a : rhs_b;
+    double* rn = (i % 2) ? rhs_b : rhs_a;
+    rc = periodic_iterate_correction(c, pred, rp, rn, err, s);
+    if (rc) break;
+    hipLaunchKernelGGL(k_ctl_after_iteration, dim3(1), dim3(64), 0, s, ctl, tol, max_pass,
+                       floor_ratio, floor_cap);
+    rc = check_launch(c, "ctl_after_iteration");
+  }
+  c->act = nullptr;
+  if (rc) return rc;
+  if ((rc = periodic_correct_velocity(c, pred, nullptr, s))) return rc;
   hipLaunchKernelGGL(k_ctl_end, dim3(1), dim3(64), 0, s, ctl, counts, counts_cap, tol);
   return check_launch(c, "ctl_end");
 }

@@ -507,7 +507,7 @@ __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
   __shared__ cx<T> tws[P];
   for (int i = threadIdx.x; i < P; i += blockDim.x) tws[i] = tw[i];
   __syncthreads();
-  const int tid = threadIdx.x, col = tid % TC, j = tid / TC;
+  const int tid = threadIdx.x, col = tid % TC, j_ = tid / TC;
   const int ntile = (Nx1 + TC - 1) / TC;
   const int y = blockIdx.x / ntile;
   const int kx = (blockIdx.x - y * ntile) * TC + col;
@@ -519,6 +519,7 @@ __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
   const size_t out_off = (size_t)y * P * Nx1 + kx;
 #pragma unroll 1
   for (int f = 0; f < NF; ++f) {
+    const int j = opaque(j_);
     if (j < N2) {
       cx<T> x[N1];
 #pragma unroll

@@ -271,3 +271,66 @@ def run_guessed(flow, state, n_steps: int, mode: int = 2, net: Net | None = None
     res.state = state
     res.converged = not (last_error > tol) and not res.nan
     return res
+
+
+class GuessedGraphStepper:
+    """Guessed steps with the corrector decided on the device, one captured
+    graph launch per step (two graphs: state and history buffers swap each
+    step).  The first two steps (plain predictor, then linear extrapolation)
+    run eagerly through the same device step; the rest replay the graph of
+    the requested mode."""
+
+    def __init__(self, flow, state, mode: int = 2, net: Net | None = None, scales: Scales | None = None,
+                 slots: int | None = None, capacity: int = 4096):
+        if mode not in (1, 2, 3):
+            raise ValueError("mode must be 1, 2 or 3")
+        if mode == 2 and (net is None or scales is None):
+            raise ValueError("mode 2 needs a trained net and its scales")
+        nat = flow._native
+        self.nat, self.mode = nat, mode
+        p = flow.params
+        self.slots = p.max_corrector_iterations + 1 if slots is None else slots
+        self.ops = None if net is None else net.operands
+        self.sc = (1.0, 1.0, 1.0) if scales is None else scales.astuple()
+        self.bufs = [state.detach().clone().contiguous(), nat.empty()]
+        self.rhs = [nat.empty(), nat.empty()]
+        self.hist = [torch.zeros(nat.shape + (2,), dtype=torch.float32, device=nat.device) for _ in range(2)]
+        self.counts = torch.zeros(capacity, dtype=torch.int32, device=nat.device)
+        self.capacity = capacity
+        self.parity = 0
+        self.steps = 0
+        nat.step_reset()
+        self._enqueue(0, 0)  # plain predictor, records N^0
+        self._enqueue(1, min(mode, 1))  # linear extrapolation
+        self.parity, self.steps = 0, 2
+        self.graphs = [self._capture(0), self._capture(1)]
+
+    def _enqueue(self, parity: int, mode: int) -> None:
+        b, h = self.bufs, self.hist
+        self.nat.step_device_guess(b[parity], b[1 - parity], self.rhs[0], self.rhs[1], h[parity], h[1 - parity],
+                                   mode, self.slots, self.counts, self.ops, self.sc)
+
+    def _capture(self, parity: int):
+        g = torch.cuda.CUDAGraph()
+        torch.cuda.synchronize()
+        with torch.cuda.graph(g):
+            self._enqueue(parity, self.mode)
+        return g
+
+    def step(self, n: int = 1) -> None:
+        if self.steps + n > self.capacity:
+            raise ValueError("pass-count capacity exceeded")
+        for _ in range(n):
+            self.graphs[self.parity].replay()
+            self.parity ^= 1
+            self.steps += 1
+
+    @property
+    def state(self):
+        return self.bufs[self.parity]
+
+    def passes(self):
+        return self.counts[: self.steps].tolist()
+
+    def step_stats(self):
+        return self.nat.step_stats()

@@ -158,6 +158,21 @@ class NativeContext:
         self._chk(self.lib.dnsmi_guess(self.handle, _ptr(state), _ptr(pred), _ptr(rhs_prev), _ptr(hist1),
                                        _ptr(hist2), _ptr(wb), _ptr(bias), H, s_1, s_2, s_out, mode, _stream()))
 
+    def step_device_guess(self, state, nxt, rhs_a, rhs_b, hist1, hist2, mode, slots, counts=None, net=None,
+                          scales=(1.0, 1.0, 1.0)):
+        """Enqueue one time step started from the corrector guess, decided on
+        the device (capturable).  Up to max_corrector_iterations + 1 passes."""
+        p = self.params
+        H, wb, bias = (32, None, None) if net is None else net
+        s_1, s_2, s_out = scales
+        ratio = p.corrector_floor_ratio
+        ncap = 0 if counts is None else counts.numel()
+        self._chk(self.lib.dnsmi_step_device_guess(
+            self.handle, _ptr(state), _ptr(nxt), _ptr(rhs_a), _ptr(rhs_b), _ptr(hist1), _ptr(hist2), _ptr(wb),
+            _ptr(bias), H, s_1, s_2, s_out, mode, slots, _ptr(counts), ncap, p.corrector_tolerance,
+            p.max_corrector_iterations + 1, 0.0 if ratio is None else ratio,
+            p.corrector_floor_cap * p.corrector_tolerance, _stream()))
+
     def step_stats(self) -> dict:
         out = (ctypes.c_double * 6)()
         self._chk(self.lib.dnsmi_step_stats(self.handle, out))

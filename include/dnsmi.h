@@ -211,6 +211,21 @@ int dnsmi_guess(dnsmi_ctx* ctx, const double* state, double* pred, double* rhs_p
                 const float* hist1, float* hist2, const void* packed_weights, const float* bias,
                 int H, float s_1, float s_2, float s_out, int mode, void* stream);
 
+/* One time step started from the corrector guess, decided on the device
+ * (periodic contexts): dnsmi_guess from `state` into `next` (with rhs_a as
+ * rhs_prev), then up to `slots` corrector passes, each skipped on the device
+ * once error <= tol, max_pass passes ran, or the noise-floor rule of
+ * dnsmi_step_device fires; then correct_velocity(next).  counts[i] receives
+ * step i's number of passes (one RHS each).  hist1/hist2 as in dnsmi_guess
+ * (hist2 receives N^n; swap them, and state/next, for the next step).  A
+ * fixed launch sequence with no host synchronisation: capturable. */
+int dnsmi_step_device_guess(dnsmi_ctx* ctx, const double* state, double* next, double* rhs_a,
+                            double* rhs_b, const float* hist1, float* hist2,
+                            const void* packed_weights, const float* bias, int H, float s_1,
+                            float s_2, float s_out, int mode, int slots, int* counts,
+                            int counts_cap, double tol, int max_pass, double floor_ratio,
+                            double floor_cap, void* stream);
+
 /* Per-kernel timing with HIP events on the launch stream (no counterpart in
  * the reference, whose bench.timer (bench.py:26-67) times whole calls).
  * Between begin and end every kernel launch of the context is bracketed by

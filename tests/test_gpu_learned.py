@@ -173,3 +173,34 @@ def test_guessed_runs_stay_on_the_plain_trajectory(precision):
         assert d <= (1e-8 if precision == "fp64" else 1e-7), (mode, d)
         if mode in (1, 3):
             assert g.rhs_evals <= plain.rhs_evals, (mode, g.passes, plain.counts)
+
+
+@pytest.mark.parametrize("precision,mode", [("fp64", 3), ("mixed", 2)])
+def test_guessed_graph_steps_match_eager(precision, mode):
+    """Device-decided guessed steps replayed from captured graphs equal the
+    eager host loop bit for bit, with the same pass counts."""
+    from dnsjax_amd.learned import GuessedGraphStepper, Net, collect, fit_scales, make_dataset, run_guessed, train
+
+    flow = _periodic(32, 60.0, precision, dt=0.01)
+    u0 = flow.init_state(seed=7, rms=0.1)
+    snaps, u_mid = collect(flow, u0, 8)
+    sc = fit_scales(snaps)
+    X, T = make_dataset(flow, snaps, sc, 5000, seed=2)
+    net = Net(32, seed=1)
+    train(net, X, T, iters=50, batch=2048)
+    n = 6
+    ref = []
+    eager = run_guessed(flow, u_mid.clone(), n, mode=mode, net=net, scales=sc,
+                        callback=lambda i, s: ref.append(s.clone()))
+    gs = GuessedGraphStepper(flow, u_mid, mode=mode, net=net, scales=sc)
+    torch.cuda.synchronize()
+    assert torch.equal(gs.state, ref[1])  # the two eager start-up steps
+    for i in range(2, n):
+        gs.step(1)
+        torch.cuda.synchronize()
+        _ = flow.get_rhs(ref[i])  # eager work between replays
+        torch.cuda.synchronize()
+        assert torch.equal(gs.state, ref[i]), f"step {i} differs"
+    assert gs.passes() == eager.passes
+    st = gs.step_stats()
+    assert st["steps"] == n and st["unconverged"] == 0 and st["nan"] == 0

@@ -189,3 +189,18 @@ def test_stats_match_oracle(case):
     ref = P.get_stats(g["pred2"], P.Flow(op))
     for k in ("E", "I", "D", "E'"):
         assert abs(float(st[k]) - ref[k]) <= 1e-12 * max(abs(ref[k]), 1e-30), (k, float(st[k]), ref[k])
+
+
+def test_stats_at_full_size_agree_with_the_norm():
+    """128^3: the statistics' partial sums fit their buffer, and E' equals
+    half the squared corrector norm of the state (two device paths, each
+    pinned against the oracle at small sizes)."""
+    from dnsjax_amd.flows import TriplyPeriodicFlow
+    from dnsjax_amd.params import kolmogorov_2pi
+
+    flow = TriplyPeriodicFlow(kolmogorov_2pi(128, 100.0, 0.005, tile_cols=16))
+    u = flow.init_state(seed=1, rms=0.1)
+    st = flow.get_stats(u)
+    n = float(flow.norm(u))
+    assert abs(float(st["E'"]) - 0.5 * n * n) <= 1e-12 * 0.5 * n * n
+    assert all(math.isfinite(float(v)) for v in st.values())

@@ -178,3 +178,25 @@ def test_projection_is_idempotent():
     once = P.correct_velocity(u + 0.01 * rhs, flow)
     twice = P.correct_velocity(once, flow)
     assert np.max(np.abs(once - twice)) < 1e-17
+
+
+@pytest.mark.parametrize("system,L,tilt", [("kolmogorov", 2 * math.pi, 0.0), ("waleffe", 2 * math.pi, 0.0),
+                                           ("kolmogorov", 4.0, 30.0), ("waleffe", 4.0, 0.0)])
+def test_stats_power_balance_closed_forms(system, L, tilt):
+    """Diagnostics (flows/monochromatic.py:433-505) on states whose values the
+    physics fixes: the zero perturbation is the laminar flow (E = 1/4,
+    I = D = F/2, E' = 0); the perturbation minus the laminar profile is the
+    fluid at rest (E = I = D = 0, E' = 1/4)."""
+    p = P.PeriodicParams(nx=8, ny=8, nz=8, lx=L, ly=L, lz=L, re=40.0, system=system, tilt_degree=tilt)
+    flow = P.Flow(p)
+    F = flow.force_amplitude
+    z = np.zeros((3,) + flow.fourier.lapl.shape, dtype=np.complex128)
+    lam = P.get_stats(z, flow)
+    assert abs(lam["E"] - 0.25) < 1e-14 and abs(lam["E'"]) < 1e-14
+    assert abs(lam["I"] - F / 2) < 1e-14 * F and abs(lam["D"] - F / 2) < 1e-12 * F
+    idx, uf = P.forced_modes(flow)
+    rest = z.copy()
+    rest[idx] = -uf
+    st = P.get_stats(rest, flow)
+    assert abs(st["E"]) < 1e-14 and abs(st["I"]) < 1e-14 * F and abs(st["D"]) < 1e-12 * F
+    assert abs(st["E'"] - 0.25) < 1e-14
