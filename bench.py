@@ -401,6 +401,8 @@ def main():
                     help="steps of each form in the mixed+learned A/B (0 skips it)")
     ap.add_argument("--ensemble-steps", type=int, default=50,
                     help="steps of the 64-member ensemble form (0 skips it)")
+    ap.add_argument("--profile-walls", action="store_true",
+                    help="run a few steps of the plane Couette form for rocprofv3 and exit")
     ap.add_argument("--profile-workload", action="store_true",
                     help="run the rocprofv3 workload (calibration copy + a few steps) and exit")
     a = ap.parse_args()
@@ -410,6 +412,9 @@ def main():
         sys.exit(2)
     if a.profile_workload:
         profile_workload(a.steps)
+        return
+    if a.profile_walls:
+        print(json.dumps(walls_form(a.steps, warmup=1)), flush=True)
         return
     os.environ.setdefault("OMP_NUM_THREADS", "1")
     fp64 = gpu_form("fp64", a.steps, a.warmup)

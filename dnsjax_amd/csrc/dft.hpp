@@ -236,6 +236,28 @@ __device__ __forceinline__ int opaque(int v) {
   return v;
 }
 
+// Padded-axis index maps with the mode count NM known at compile time: NM
+// modes stored as NM - 1 (Nyquist dropped) at padded indices [0, NM/2) and
+// (P - NM/2, P).  The 3/2 rule gives NM = 2P/3 (Pad<P>); an axis without
+// oversampling has NM = P.  A step-A input (m = lo + lane over lanes
+// [lo, lo + W)) or step-B output whose whole lane range lies in the zero band
+// is known at compile time once the loops are unrolled, so its loads, stores
+// and the arithmetic that only feeds them drop out.
+template <int P, int NM>
+struct PadN {
+  static constexpr int n = NM, half = NM / 2;
+  // stored index of padded index m, or -1 in the zero band
+  static __device__ __forceinline__ int stored(int m) {
+    return m < half ? m : (m > P - half ? m - (P - n + 1) : -1);
+  }
+  static constexpr bool zero(int lo, int hi) { return lo >= half && hi <= P - half; }
+  static __device__ __forceinline__ int stored_in(int lo, int W, int lane) {
+    return zero(lo, lo + W - 1) ? -1 : stored(lo + lane);
+  }
+};
+template <int P>
+using Pad = PadN<P, 2 * P / 3>;
+
 // ------------------------------------------------------------ two-step helpers
 // LDS layout for a tile of TC independent transforms (columns): element
 // (k1, b) of column col at ((k1 * N2 + b) * TC + col), col fastest, so a

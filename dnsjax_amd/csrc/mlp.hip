@@ -155,13 +155,29 @@ __global__ void __launch_bounds__(S)
   }
 }
 
+// Deterministic two-level sum of the per-block partials: chunk q sums blocks
+// [32 q, 32 q + 32) in order into part2[q][p]; the final pass sums the
+// chunks in order.
+constexpr int kReduceChunk = 32;
+
 __global__ void k_mlp_reduce(const float* __restrict__ part, int nblocks, int P,
-                             float* __restrict__ grad, const float* __restrict__ loss_part,
-                             float* __restrict__ loss, int B) {
+                             float* __restrict__ part2) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  const int b0 = blockIdx.y * kReduceChunk;
+  const int b1 = b0 + kReduceChunk < nblocks ? b0 + kReduceChunk : nblocks;
+  float g = 0.f;
+  for (int b = b0; b < b1; ++b) g += part[(size_t)b * P + p];
+  part2[(size_t)blockIdx.y * P + p] = g;
+}
+
+__global__ void k_mlp_reduce_final(const float* __restrict__ part2, int nchunks, int P,
+                                   float* __restrict__ grad, const float* __restrict__ loss_part,
+                                   int nblocks, float* __restrict__ loss, int B) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p < P) {
     float g = 0.f;
-    for (int b = 0; b < nblocks; ++b) g += part[(size_t)b * P + p];
+    for (int q = 0; q < nchunks; ++q) g += part2[(size_t)q * P + p];
     grad[p] = g;
   }
   if (p == 0 && loss) {
@@ -416,6 +432,10 @@ __global__ void k_mlp_features(FeatArgs a, const cx<double>* __restrict__ nm2,
 //   rhs_prev <- N^n + dN                    (so that the incremental
 //               corrector from the guess is the same fixed-point map)
 //   h2       <- N^n                         (the caller then swaps h1, h2)
+// One lane per mode: each lane reads its mode's N^n and history once
+// (coalesced), forms the features and dN; the network runs on the wave's
+// 64 modes as four 16-mode matrix-core batches, with operands and outputs
+// exchanged through LDS.
 template <int H>
 __global__ void __launch_bounds__(256)
     k_guess_periodic(FeatArgs a, const cx<double>* __restrict__ ncur, const float* __restrict__ h1,
@@ -425,74 +445,91 @@ __global__ void __launch_bounds__(256)
   constexpr int NT = 2 * (H / 16) + (H / 16) * (H / 16);
   __shared__ v4s wt[NT * 64];
   __shared__ float bs[2 * H + 16];
-  __shared__ float outs[4][16][kDout];
+  __shared__ float feats[4][64][kDin + 1];
+  __shared__ float outs[4][64][kDout + 1];
   const bool use_net = mode == 2;
   if (use_net) {
     for (int i = threadIdx.x; i < NT * 64; i += blockDim.x) wt[i] = reinterpret_cast<const v4s*>(wb)[i];
     for (int i = threadIdx.x; i < 2 * H + 16; i += blockDim.x) bs[i] = bias[i];
   }
-  __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int g = lane / 16, ml = lane % 16;
-  const long long i = ((long long)blockIdx.x * 4 + wave) * 16 + ml;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = i < a.n;
+  const long long ii = live ? i : 0;
+  double kx, ky, kz;
+  mode_k(a, ii, kx, ky, kz);
+  cx<double> nc[3], d1[3], d2[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const long long e = q * a.n + ii;
+    nc[q] = live ? ncur[e] : cx<double>{0, 0};
+    const float2 p1 = live ? reinterpret_cast<const float2*>(h1)[e] : float2{0.f, 0.f};
+    const float2 p2 = live ? reinterpret_cast<const float2*>(h2)[e] : float2{0.f, 0.f};
+    diffs(nc[q].x, p1.x, p2.x, d1[q].x, d2[q].x);
+    diffs(nc[q].y, p1.y, p2.y, d1[q].y, d2[q].y);
+  }
   if (use_net) {
-    float f[4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) f[j] = live ? feature(a, ncur, h1, h2, i, 4 * g + j) : 0.f;
-    const v4f y = mlp_wave<H>(pack_b(f[0], f[1], f[2], f[3]), wt, bs, lane);
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
-      if (4 * g + j < kDout) outs[wave][ml][4 * g + j] = y[j];
+    for (int q = 0; q < 3; ++q) {
+      feats[wave][lane][2 * q] = (float)d1[q].x * a.s_1;
+      feats[wave][lane][2 * q + 1] = (float)d1[q].y * a.s_1;
+      feats[wave][lane][6 + 2 * q] = (float)d2[q].x * a.s_2;
+      feats[wave][lane][6 + 2 * q + 1] = (float)d2[q].y * a.s_2;
+    }
+    for (int f = 12; f < kDin; ++f) feats[wave][lane][f] = k_feature(a, ii, f);
   }
   __syncthreads();
-  if (!live || g == 3) return;
-  // lane (mode ml, component g)
-  double kx, ky, kz;
-  mode_k(a, i, kx, ky, kz);
+  if (use_net) {
+    const int g = lane / 16, ml = lane % 16;
+#pragma unroll
+    for (int sb = 0; sb < 4; ++sb) {
+      const float* fr = feats[wave][sb * 16 + ml];
+      const v4f y = mlp_wave<H>(pack_b(fr[4 * g], fr[4 * g + 1], fr[4 * g + 2], fr[4 * g + 3]), wt, bs, lane);
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (4 * g + j < kDout) outs[wave][sb * 16 + ml][4 * g + j] = y[j];
+    }
+  }
+  __syncthreads();
+  if (!live) return;
   const double lapl = -((kx * kx + ky * ky) + kz * kz);
   const bool mean = (i == 0);
   const double w = mode_weight(a, lapl, mean);
   cx<double> dn[3];
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
-    const cx<double> nc = ncur[q * a.n + i];
-    const long long e = (q * a.n + i) * 2;
-    double d1x, d2x, d1y, d2y;
-    diffs(nc.x, h1[e], h2[e], d1x, d2x);
-    diffs(nc.y, h1[e + 1], h2[e + 1], d1y, d2y);
     if (mode == 0)
       dn[q] = {0.0, 0.0};
     else if (mode == 1)
-      dn[q] = {d1x, d1y};
+      dn[q] = d1[q];
     else
-      dn[q] = {d1x + d2x, d1y + d2y};
+      dn[q] = {d1[q].x + d2[q].x, d1[q].y + d2[q].y};
     if (use_net && !mean) {
-      dn[q].x += (double)a.s_out * outs[wave][ml][2 * q] / w;
-      dn[q].y += (double)a.s_out * outs[wave][ml][2 * q + 1] / w;
+      dn[q].x += (double)a.s_out * outs[wave][lane][2 * q] / w;
+      dn[q].y += (double)a.s_out * outs[wave][lane][2 * q + 1] / w;
     }
   }
   const double il = lapl < 0 ? 1.0 / lapl : 0.0;
   // projection: dn - k (k . dn) / |k|^2 = dn + k (k . dn) * inv_lapl
   const cx<double> kd = {(kx * dn[0].x + ky * dn[1].x) + kz * dn[2].x,
                          (kx * dn[0].y + ky * dn[1].y) + kz * dn[2].y};
-  const double kq = g == 0 ? kx : (g == 1 ? ky : kz);
-  // the mean mode keeps dN = 0: the guess then changes nothing there
-  const cx<double> d = mean ? cx<double>{0.0, 0.0}
-                            : cx<double>{dn[g].x + kq * kd.x * il, dn[g].y + kq * kd.y * il};
+  const double kk[3] = {kx, ky, kz};
   const double cil = a.c * a.dt * w;  // c ildt_2
-  const long long e = g * a.n + i;
   cx<double>* pd = reinterpret_cast<cx<double>*>(pred);
   cx<double>* rp = reinterpret_cast<cx<double>*>(rhs_prev);
-  const cx<double> u1 = pd[e];
-  const cx<double> nc = ncur[e];
-  if (mode != 0) pd[e] = {u1.x + cil * d.x, u1.y + cil * d.y};
-  rp[e] = mode == 0 ? nc : cx<double>{nc.x + d.x, nc.y + d.y};
-  // The 64 lanes of this wave hold all reads of its 16 modes' history
-  // (features and the loop above) and issue them before these stores, in
-  // program order; no other wave touches these modes.
-  h2[e * 2] = (float)nc.x;
-  h2[e * 2 + 1] = (float)nc.y;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    // the mean mode keeps dN = 0: the guess then changes nothing there
+    const cx<double> d = mean ? cx<double>{0.0, 0.0}
+                              : cx<double>{dn[q].x + kk[q] * kd.x * il, dn[q].y + kk[q] * kd.y * il};
+    const long long e = q * a.n + i;
+    if (mode != 0) {
+      const cx<double> u1 = pd[e];
+      pd[e] = {u1.x + cil * d.x, u1.y + cil * d.y};
+    }
+    rp[e] = mode == 0 ? nc[q] : cx<double>{nc[q].x + d.x, nc[q].y + d.y};
+    reinterpret_cast<float2*>(h2)[e] = float2{(float)nc[q].x, (float)nc[q].y};
+  }
 }
 
 // ------------------------------------------------------------------ launchers
@@ -511,8 +548,11 @@ int mlp_grad(int H, int B, const float* X, const float* T, const float* prm, int
     return DNSMI_EINVAL;
   }
   const int P = mlp_nparams(H);
-  hipLaunchKernelGGL(k_mlp_reduce, dim3((P + 255) / 256), dim3(256), 0, s, part, nb, P, grad,
-                     loss_part, loss, B);
+  const int nq = (nb + kReduceChunk - 1) / kReduceChunk;
+  float* part2 = loss_part + nb;
+  hipLaunchKernelGGL(k_mlp_reduce, dim3((P + 255) / 256, nq), dim3(256), 0, s, part, nb, P, part2);
+  hipLaunchKernelGGL(k_mlp_reduce_final, dim3((P + 255) / 256), dim3(256), 0, s, part2, nq, P, grad,
+                     loss_part, nb, loss, B);
   return hipGetLastError() == hipSuccess ? DNSMI_OK : DNSMI_EHIP;
 }
 
@@ -596,7 +636,7 @@ int guess_periodic(Ctx* c, const double* state, double* pred, double* rhs_prev, 
   int rc = periodic_predict_only(c, state, pred, c->rhs0, s);
   if (rc) return rc;
   const FeatArgs a = feat_args(c, s_1, s_2, s_out);
-  const int nb = (int)((a.n + 63) / 64);
+  const int nb = (int)((a.n + 255) / 256);
   const cx<double>* n0 = reinterpret_cast<const cx<double>*>(c->rhs0);
   if (mode != 2 || H == 32)
     hipLaunchKernelGGL(k_guess_periodic<32>, dim3(nb), dim3(256), 0, s, a, n0, h1, h2, pred,

@@ -43,12 +43,6 @@ struct StepArgs {
 
 // Stored index of padded index n along an axis of `nmodes` modes padded to
 // P (fft.py:78-86, 127-135); -1 for the zero band.
-__device__ __forceinline__ int stored_of_padded(int n, int nmodes, int P) {
-  const int half = nmodes / 2;
-  if (n < half) return n;
-  if (n > P - half) return n - (P - nmodes + 1);
-  return -1;
-}
 
 template <typename T>
 __device__ __forceinline__ cx<T> to_t(cx<double> d) {
@@ -56,7 +50,7 @@ __device__ __forceinline__ cx<T> to_t(cx<double> d) {
 }
 
 // ------------------------------------------------------------------ K1
-template <typename T, int P, int TC>
+template <typename T, int P, int TC, bool PADY>
 __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
     k_y_inverse(const cx<double>* __restrict__ u, cx<T>* __restrict__ w1,
                 const cx<T>* __restrict__ tw, const double* __restrict__ ky, int ny, int Nz1,
@@ -88,7 +82,7 @@ __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
     if (j < N2) {
 #pragma unroll
       for (int a = 0; a < N1; ++a) {
-        const int s = stored_of_padded(N2 * a + j, ny, P);
+        const int s = PadN<P, PADY ? 2 * P / 3 : P>::stored_in(N2 * a, N2, j);
         cx<T> v{0, 0}, vk{0, 0};
         if (ok && s >= 0) {
           const cx<double> d = u[c * in_comp + (size_t)s * plane + off];
@@ -153,7 +147,7 @@ __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
     if (j < N2) {
 #pragma unroll
       for (int a = 0; a < N1; ++a) {
-        const int s = stored_of_padded(N2 * a + j, nz, P);
+        const int s = Pad<P>::stored_in(N2 * a, N2, j);
         cx<T> va{0, 0}, vb{0, 0};
         if (ok && s >= 0) {
           const T k = static_cast<T>(kz_tab[s]);
@@ -208,7 +202,7 @@ __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
 // K1 / K2, sequential form: one transform in flight per pass (fewer live
 // registers, higher occupancy); an input used by two outputs is read twice,
 // the second time from L2.  Same outputs as the paired forms above.
-template <typename T, int P, int TC>
+template <typename T, int P, int TC, bool PADY>
 __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
     k_y_inverse_seq(const cx<double>* __restrict__ u, cx<T>* __restrict__ w1,
                     const cx<T>* __restrict__ tw, const double* __restrict__ ky, int ny, int Nz1,
@@ -239,7 +233,7 @@ __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
       cx<T> x[N1];
 #pragma unroll
       for (int a = 0; a < N1; ++a) {
-        const int s = stored_of_padded(N2 * a + j, ny, P);
+        const int s = PadN<P, PADY ? 2 * P / 3 : P>::stored_in(N2 * a, N2, j);
         cx<T> v{0, 0};
         if (ok && s >= 0) {
           const cx<double> d = u[c * in_comp + (size_t)s * plane + off];
@@ -296,7 +290,7 @@ __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
       cx<T> x[N1];
 #pragma unroll
       for (int a = 0; a < N1; ++a) {
-        const int s = stored_of_padded(N2 * a + j, nz, P);
+        const int s = Pad<P>::stored_in(N2 * a, N2, j);
         cx<T> v{0, 0};
         if (ok && s >= 0) {
           const T k = static_cast<T>(kz_tab[s]);
@@ -370,7 +364,7 @@ __global__ void __launch_bounds__(R * Plan<P>::N1, 2)
   const int row = blockIdx.x * R + r;
   const bool ok = row < nrows;
   const int yrow = ok ? row / Pz : 0;
-  const int Nx1 = nx / 2;
+  constexpr int Nx1 = Pad<P>::half;  // nx / 2
   const size_t fstride = (size_t)nrows * Nx1;
   const cx<T>* in = w2 + (size_t)(ok ? row : 0) * Nx1;
   cx<T>* out = w3 + (size_t)(ok ? row : 0) * Nx1;
@@ -384,10 +378,20 @@ __global__ void __launch_bounds__(R * Plan<P>::N1, 2)
       cx<T> z[N1];
 #pragma unroll
       for (int a = 0; a < N1; ++a) {
-        const int m = N2 * a + j;
+        // padded index m = N2 a + j: the lane range's position relative to
+        // the zero band is fixed at compile time per (unrolled) a
+        const int lo = N2 * a, hi = N2 * a + N2 - 1;
+        const bool none = Pad<P>::zero(lo, hi), pos = hi < Nx1, neg = lo > P - Nx1;
+        const int m = lo + j;
         int k = -1;
         bool cj = false;
-        if (m < Nx1) {
+        if (none) {
+        } else if (pos) {
+          k = m;
+        } else if (neg) {
+          k = P - m;
+          cj = true;
+        } else if (m < Nx1) {
           k = m;
         } else if (m > P - Nx1) {
           k = P - m;
@@ -445,7 +449,8 @@ __global__ void __launch_bounds__(R * Plan<P>::N1, 2)
 #pragma unroll
       for (int k2 = 0; k2 < N1; ++k2) {
         const int k = j + N2 * k2;
-        if (k < Nx1) out[c * fstride + k] = cscale(o[k2], inv);
+        // N2 k2 >= Nx1: truncated for every lane (those DFT outputs are dead)
+        if (N2 * k2 < Nx1 && k < Nx1) out[c * fstride + k] = cscale(o[k2], inv);
       }
     }
     __syncthreads();
@@ -683,7 +688,7 @@ __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
       step_b_load<P, N1, N2, -1, TC>(o, j, col, lds);
 #pragma unroll
       for (int k2 = 0; k2 < N2; ++k2) {
-        const int s = stored_of_padded(j + N1 * k2, nz, P);
+        const int s = Pad<P>::stored_in(N1 * k2, N1, j);
         if (s >= 0) w4[c * out_f + out_off + (size_t)s * Nx1] = cscale(o[k2], inv);
       }
     }
@@ -692,7 +697,7 @@ __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
 }
 
 // ------------------------------------------------------------------ K5
-template <typename T, int P, int TC, int MODE>
+template <typename T, int P, int TC, int MODE, bool PADY>
 __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
     k_y_forward(const cx<T>* __restrict__ w4, const cx<T>* __restrict__ tw,
                 const double* __restrict__ kx_tab, const double* __restrict__ ky_tab,
@@ -739,7 +744,7 @@ __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
     const double km = kx == 0 ? 1.0 : 2.0;
 #pragma unroll
     for (int k2 = 0; k2 < N2; ++k2) {
-      const int s = stored_of_padded(j + N1 * k2, ny, P);
+      const int s = PadN<P, PADY ? 2 * P / 3 : P>::stored_in(N1 * k2, N1, j);
       if (s < 0) continue;
       const double kyv = ky_tab[s];
       const double lapl = -((kxv * kxv + kyv * kyv) + kzv * kzv);
@@ -917,16 +922,23 @@ static int launch_y_inverse(Ctx* c, const double* state, hipStream_t s) {
   const int ntile = (c->Nx1 + kTC - 1) / kTC;
   const dim3 grid(c->Nz1 * ntile);
   const cx<double>* u = reinterpret_cast<const cx<double>*>(state);
+  const bool pady = c->Py != c->ny;  // y padded by the 3/2 rule
   ProfScope ps(c, KID_Y_INVERSE, s);
   switch (c->Py) {
 #define X(P)                                                                                    \
   case P:                                                                                       \
-    if (c->yinv_seq)                                                                            \
-      hipLaunchKernelGGL((k_y_inverse_seq<T, P, kTC>), grid, dim3(kTC * ColPlan<P>::threads), 0, \
-                         s, u, b.A, b.twy, c->ky, c->ny, c->Nz1, c->Nx1, c->act);                       \
+    if (c->yinv_seq && pady)                                                                    \
+      hipLaunchKernelGGL((k_y_inverse_seq<T, P, kTC, true>), grid, dim3(kTC * ColPlan<P>::threads), \
+                         0, s, u, b.A, b.twy, c->ky, c->ny, c->Nz1, c->Nx1, c->act);            \
+    else if (c->yinv_seq)                                                                       \
+      hipLaunchKernelGGL((k_y_inverse_seq<T, P, kTC, false>), grid, dim3(kTC * ColPlan<P>::threads), \
+                         0, s, u, b.A, b.twy, c->ky, c->ny, c->Nz1, c->Nx1, c->act);            \
+    else if (pady)                                                                              \
+      hipLaunchKernelGGL((k_y_inverse<T, P, kTC, true>), grid, dim3(kTC * ColPlan<P>::threads), 0, \
+                         s, u, b.A, b.twy, c->ky, c->ny, c->Nz1, c->Nx1, c->act);               \
     else                                                                                        \
-      hipLaunchKernelGGL((k_y_inverse<T, P, kTC>), grid, dim3(kTC * ColPlan<P>::threads), 0, s,  \
-                         u, b.A, b.twy, c->ky, c->ny, c->Nz1, c->Nx1, c->act);                          \
+      hipLaunchKernelGGL((k_y_inverse<T, P, kTC, false>), grid, dim3(kTC * ColPlan<P>::threads), 0, \
+                         s, u, b.A, b.twy, c->ky, c->ny, c->Nz1, c->Nx1, c->act);               \
     break;
     DNSMI_FOR_EACH_P(X)
 #undef X
@@ -1057,8 +1069,14 @@ static int launch_y_forward(Ctx* c, const StepArgs& a, hipStream_t s, int* nbloc
   switch (c->Py) {
 #define X(P)                                                                                   \
   case P:                                                                                      \
-    hipLaunchKernelGGL((k_y_forward<T, P, kTC, MODE>), dim3(nb), dim3(kTC * ColPlan<P>::threads), 0,   \
-                       s, b.B, b.twy, c->kx, c->ky, c->kz, c->ny, c->Nz1, c->Nx1, a);          \
+    if (c->Py != c->ny)                                                                        \
+      hipLaunchKernelGGL((k_y_forward<T, P, kTC, MODE, true>), dim3(nb),                      \
+                         dim3(kTC * ColPlan<P>::threads), 0, s, b.B, b.twy, c->kx, c->ky, c->kz, \
+                         c->ny, c->Nz1, c->Nx1, a);                                            \
+    else                                                                                       \
+      hipLaunchKernelGGL((k_y_forward<T, P, kTC, MODE, false>), dim3(nb),                     \
+                         dim3(kTC * ColPlan<P>::threads), 0, s, b.B, b.twy, c->kx, c->ky, c->kz, \
+                         c->ny, c->Nz1, c->Nx1, a);                                            \
     break;
     DNSMI_FOR_EACH_P(X)
 #undef X

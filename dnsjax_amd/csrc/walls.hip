@@ -31,6 +31,11 @@
 
 namespace dnsmi {
 
+// Influence-matrix work vectors (complex, [vec][mode][y]): f_hat, the
+// pressure, the three R_c / velocity columns, three forward-sweep results.
+enum ImmVec { V_F = 0, V_P = 1, V_R = 2, V_T = 5, V_COUNT = 8 };
+constexpr int kImmT = 128;  // threads per mode in the parallel phases
+
 struct WallTables {
   int p = 0, Ny = 0, Nkz = 0, Nkx = 0, M = 0;
   // device
@@ -50,7 +55,7 @@ struct WallTables {
   double* minv = nullptr;  // [4][mode]
   double* mkx = nullptr;   // per mode
   double* mkz = nullptr;
-  double* scratch = nullptr;  // 5 complex vectors [vec][y][mode]
+  double* scratch = nullptr;  // IMM work vectors, complex [vec][mode][y]
   // host copies in the reference layouts, for inspection by tests
   std::vector<double> h_lk_ab, h_hk_ab, h_v1, h_q1, h_minv, h_ys;
   std::vector<int> h_lk_piv, h_hk_piv;
@@ -282,9 +287,9 @@ int walls_setup(Ctx* c) {
   c->bufA = c->alloc(c->bufA_bytes);
   c->bufB = c->alloc(c->bufB_bytes);
   c->rhs0 = static_cast<double*>(c->alloc(c->spec_elems * 16));
-  c->max_partials = (M + 63) / 64 + 1024;
+  c->max_partials = M + 1024;  // the IMM writes one norm partial per mode
   c->partials = static_cast<double*>(c->alloc(sizeof(double) * c->max_partials));
-  t->scratch = static_cast<double*>(c->alloc(5ull * Ny * M * 16));
+  t->scratch = static_cast<double*>(c->alloc((size_t)V_COUNT * Ny * M * 16));
   if (!c->bufA || !c->bufB || !c->rhs0 || !c->partials || !t->scratch)
     return c->fail(DNSMI_ENOMEM, "device allocation failed");
   int rc = setup_twiddles(c);
@@ -524,9 +529,7 @@ __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
       cx<T> x[N1];
 #pragma unroll
       for (int a = 0; a < N1; ++a) {
-        const int n = N2 * a + j;
-        const int half = nz / 2;
-        const int s = n < half ? n : (n > P - half ? n - (P - nz + 1) : -1);
+        const int s = Pad<P>::stored_in(N2 * a, N2, j);
         x[a] = (ok && s >= 0) ? in[f * in_f + in_off + (size_t)s * Nx1] : cx<T>{0, 0};
       }
       step_a_store<P, N1, N2, +1, TC>(x, j, col, lds, tws);
@@ -582,289 +585,293 @@ struct ImmArgs {
   const double *lk_ab, *hk_ab;
   const int *lk_piv, *hk_piv;
   const double *v1, *v2, *q1, *q2, *minv, *mkx, *mkz;
-  cx<double>* scratch;  // 5 vectors [vec][y][mode]
+  cx<double>* scratch;  // work vectors [vec][mode][y]
   const int* act;       // if set and 0: the launch is skipped
   int Ny, M, Nkx;
   double dt, c, nu;
 };
 
-template <int PO>
-__global__ void __launch_bounds__(64) k_imm(ImmArgs a) {
-  if (a.act && *a.act == 0) return;
-  constexpr int KL = PO, KU = PO, KUU = 2 * PO, ROWS = 3 * PO + 1, S1 = PO + 1, S2 = PO + 2;
-  __shared__ double red[2];
-  const int m = blockIdx.x * 64 + threadIdx.x;
-  const bool live = m < a.M;
-  const int Ny = a.Ny, M = a.M;
-  const size_t comp = (size_t)M * Ny;
-  double acc = 0.0;
-  if (live) {
-    const double kx = a.mkx[m], kz = a.mkz[m];
-    const double k2 = kx * kx + kz * kz;
-    const bool k2zero = (k2 == 0.0);
-    const double dt = a.dt, cc = a.c, nu = a.nu;
-    auto vec = [&](int v, int y) -> cx<double>& { return a.scratch[((size_t)v * Ny + y) * M + m]; };
-    auto at = [&](const cx<double>* f, int c, int y) { return f[c * comp + (size_t)m * Ny + y]; };
-    // D1 (or D2) stencil of component c of field f at row y
-    auto d1 = [&](const cx<double>* f, int c, int y) {
-      cx<double> s{0, 0};
-      const int j0 = a.d1j0[y];
-#pragma unroll
-      for (int k = 0; k < S1; ++k) {
-        const double w = a.d1w[y * S1 + k];
-        const cx<double> v = f[c * comp + (size_t)m * Ny + j0 + k];
-        s.x += w * v.x;
-        s.y += w * v.y;
-      }
-      return s;
-    };
-    auto d1v = [&](int v, int y) {
-      cx<double> s{0, 0};
-      const int j0 = a.d1j0[y];
-#pragma unroll
-      for (int k = 0; k < S1; ++k) {
-        const double w = a.d1w[y * S1 + k];
-        const cx<double> q = vec(v, j0 + k);
-        s.x += w * q.x;
-        s.y += w * q.y;
-      }
-      return s;
-    };
-    auto d2v = [&](int v, int y) {
-      cx<double> s{0, 0};
-      const int j0 = a.d2j0[y];
-#pragma unroll
-      for (int k = 0; k < S2; ++k) {
-        const double w = a.d2w[y * S2 + k];
-        const cx<double> q = vec(v, j0 + k);
-        s.x += w * q.x;
-        s.y += w * q.y;
-      }
-      return s;
-    };
-    auto d2f = [&](const cx<double>* f, int c, int y) {
-      cx<double> s{0, 0};
-      const int j0 = a.d2j0[y];
-#pragma unroll
-      for (int k = 0; k < S2; ++k) {
-        const double w = a.d2w[y * S2 + k];
-        const cx<double> q = f[c * comp + (size_t)m * Ny + j0 + k];
-        s.x += w * q.x;
-        s.y += w * q.y;
-      }
-      return s;
-    };
-    // i k * z
-    auto ik = [](double k, cx<double> z) { return cx<double>{-k * z.y, k * z.x}; };
+// The influence-matrix step in five phases.  The right-hand sides of the
+// banded systems are generated in parallel over (mode, y) (one workgroup
+// per mode, lanes along y: the fields' rows are contiguous in y), the
+// sequential band sweeps run one thread per (mode, right-hand side) on
+// precomputed rows with their loads issued a batch of rows ahead of the
+// recurrence, and the assembly is parallel again.  Work vectors are
+// [vec][mode][y]; per mode the arithmetic is the one-thread-per-mode
+// formulation's, operation for operation.
 
-    // Stage 1a: d_hat^n (flows/plane_couette.py:427-428) -> vec 0
-    for (int y = 0; y < Ny; ++y) {
-      const cx<double> dv = d1(a.un, 1, y);
-      const cx<double> t0 = ik(kx, at(a.un, 0, y));
-      const cx<double> t2 = ik(kz, at(a.un, 2, y));
-      vec(0, y) = {(t0.x + dv.x) + t2.x, (t0.y + dv.y) + t2.y};
+// Work vector v, mode m: element y at [y * VS] from the returned pointer.
+// The rows are contiguous in y (VS = 1): the parallel phases' lanes run
+// along y; a sweep lane walks its own row.
+constexpr int VS = 1;
+__device__ __forceinline__ cx<double>* svec(const ImmArgs& a, int v, int m) {
+  return a.scratch + ((size_t)v * a.M + m) * a.Ny;
+}
+__device__ __forceinline__ cx<double> ikm(double k, cx<double> z) { return {-k * z.y, k * z.x}; }
+
+// D1 / D2 stencils at row y of a row r (global or LDS) whose element y is
+// at r[y * stride]
+template <int S>
+__device__ __forceinline__ cx<double> stencil(const double* w, const int* j0t, const cx<double>* r,
+                                              int y, size_t stride = 1) {
+  cx<double> s{0, 0};
+  const int j0 = j0t[y];
+#pragma unroll
+  for (int k = 0; k < S; ++k) {
+    const double wk = w[y * S + k];
+    const cx<double> v = r[(j0 + k) * stride];
+    s.x += wk * v.x;
+    s.y += wk * v.y;
+  }
+  return s;
+}
+
+// Phase 1 (flows/plane_couette.py:427-445): d_hat^n into LDS, then f_hat
+// (zero on the walls) -> V_F.
+template <int PO>
+__global__ void __launch_bounds__(kImmT) k_imm_fhat(ImmArgs a) {
+  if (a.act && *a.act == 0) return;
+  constexpr int S1 = PO + 1, S2 = PO + 2;
+  extern __shared__ cx<double> dh[];
+  const int m = blockIdx.x, Ny = a.Ny;
+  const size_t comp = (size_t)a.M * Ny, row = (size_t)m * Ny;
+  const double kx = a.mkx[m], kz = a.mkz[m];
+  const double k2 = kx * kx + kz * kz;
+  const cx<double>* un0 = a.un + row;
+  const cx<double>* un1 = a.un + comp + row;
+  const cx<double>* un2 = a.un + 2 * comp + row;
+  for (int y = threadIdx.x; y < Ny; y += blockDim.x) {
+    const cx<double> dv = stencil<S1>(a.d1w, a.d1j0, un1, y);
+    const cx<double> t0 = ikm(kx, un0[y]);
+    const cx<double> t2 = ikm(kz, un2[y]);
+    dh[y] = {(t0.x + dv.x) + t2.x, (t0.y + dv.y) + t2.y};
+  }
+  __syncthreads();
+  const double dt = a.dt, cc = a.c, nu = a.nu;
+  cx<double>* f = svec(a, V_F, m);
+  const size_t M = a.M;
+  for (int y = threadIdx.x; y < Ny; y += blockDim.x) {
+    if (y <= 0 || y >= Ny - 1) {
+      f[y * VS] = {0, 0};
+      continue;
     }
-    // Stage 1b + 2: f_hat (flows/plane_couette.py:431-445) streamed into the
-    // Lk forward sweep; forward result -> vec 1.
-    auto fhat = [&](int y) -> cx<double> {
-      if (y <= 0 || y >= Ny - 1) return {0, 0};
-      const cx<double> dh = vec(0, y);
-      const cx<double> dvj = d1(a.nj, 1, y), dvn = d1(a.nn, 1, y);
-      const cx<double> aj = ik(kx, at(a.nj, 0, y)), cj = ik(kz, at(a.nj, 2, y));
-      const cx<double> an = ik(kx, at(a.nn, 0, y)), cn = ik(kz, at(a.nn, 2, y));
-      const cx<double> divj = {(aj.x + dvj.x) + cj.x, (aj.y + dvj.y) + cj.y};
-      const cx<double> divn = {(an.x + dvn.x) + cn.x, (an.y + dvn.y) + cn.y};
-      const cx<double> d2d = d2v(0, y);
-      const cx<double> lkd = {d2d.x - k2 * dh.x, d2d.y - k2 * dh.y};
-      const double g = (1 - cc) * nu;
-      return {((dh.x / dt + cc * divj.x) + (1 - cc) * divn.x) + g * lkd.x,
-              ((dh.y / dt + cc * divj.y) + (1 - cc) * divn.y) + g * lkd.y};
-    };
-    {
-      cx<double> w[KL + 1];
+    const cx<double> d = dh[y];
+    const cx<double> dvj = stencil<S1>(a.d1w, a.d1j0, a.nj + comp + row, y);
+    const cx<double> dvn = stencil<S1>(a.d1w, a.d1j0, a.nn + comp + row, y);
+    const cx<double> aj = ikm(kx, a.nj[row + y]), cj = ikm(kz, a.nj[2 * comp + row + y]);
+    const cx<double> an = ikm(kx, a.nn[row + y]), cn = ikm(kz, a.nn[2 * comp + row + y]);
+    const cx<double> divj = {(aj.x + dvj.x) + cj.x, (aj.y + dvj.y) + cj.y};
+    const cx<double> divn = {(an.x + dvn.x) + cn.x, (an.y + dvn.y) + cn.y};
+    const cx<double> d2d = stencil<S2>(a.d2w, a.d2j0, dh, y);
+    const cx<double> lkd = {d2d.x - k2 * d.x, d2d.y - k2 * d.y};
+    const double g = (1 - cc) * nu;
+    f[y * VS] = {((d.x / dt + cc * divj.x) + (1 - cc) * divn.x) + g * lkd.x,
+                ((d.y / dt + cc * divj.y) + (1 - cc) * divn.y) + g * lkd.y};
+  }
+}
+
+// Phases 2 and 4: solve ab x = rhs per (mode, right-hand side r =
+// blockIdx.y) with the LU factors and pivots in LAPACK band layout
+// (mode-fastest).  Forward sweep rhs vector vin + r -> vtmp + r, back
+// substitution -> vout + r.  Loads of U rows are issued before their
+// recurrence steps.
+template <int PO>
+struct SweepBatch {
+  static constexpr int value = PO <= 2 ? 32 : (PO <= 4 ? 16 : 8);
+};
+
+// Lanes per sweep workgroup (one wave; 16-lane workgroups over 4x the CUs
+// measured slower: a sweep is bound by its chain of batch latencies).
+constexpr int kSweepLanes = 64;
+
+template <int PO>
+__global__ void __launch_bounds__(kSweepLanes)
+    k_band_sweeps(ImmArgs a, const double* __restrict__ ab, const int* __restrict__ piv, int vin,
+                  int vtmp, int vout) {
+  if (a.act && *a.act == 0) return;
+  constexpr int KL = PO, KU = PO, KUU = 2 * PO, ROWS = 3 * PO + 1, U = SweepBatch<PO>::value;
+  const int m = blockIdx.x * kSweepLanes + threadIdx.x;
+  if (m >= a.M) return;
+  const int r = blockIdx.y, Ny = a.Ny, M = a.M;
+  // rin and out may be the same vector (the H sweeps solve in place): the
+  // forward sweep reads rin, the back substitution writes out.
+  const cx<double>* rin = svec(a, vin + r, m);
+  cx<double>* __restrict__ tmp = svec(a, vtmp + r, m);
+  cx<double>* out = svec(a, vout + r, m);
+  {
+    cx<double> w[KL + 1];
 #pragma unroll
-      for (int i = 0; i <= KL; ++i) w[i] = (i < Ny) ? fhat(i) : cx<double>{0, 0};
-      for (int j = 0; j < Ny - 1; ++j) {
-        const int d = a.lk_piv[(size_t)j * M + m] - j;
-        const cx<double> w0 = w[0];
+    for (int i = 0; i <= KL; ++i) w[i] = (i < Ny) ? rin[(size_t)i * VS] : cx<double>{0, 0};
+    for (int j0 = 0; j0 < Ny - 1; j0 += U) {
+      int dd[U];
+      double L[U][KL];
+      cx<double> nf[U];
+      // Loads from clamped (always valid) rows with no branches, so that the
+      // whole batch is in flight at once; the guards apply at use.
 #pragma unroll
-        for (int i = 1; i <= KL; ++i)
-          if (d == i) {
-            w[0] = w[i];
-            w[i] = w0;
-          }
-        const double* col = a.lk_ab + (size_t)j * ROWS * M + m;
+      for (int u = 0; u < U; ++u) {
+        const int j = j0 + u;
+        const int jj = j < Ny - 2 ? j : Ny - 2;
+        dd[u] = piv[(size_t)jj * M + m] - j;
+        const double* col = ab + (size_t)jj * ROWS * M + m;
 #pragma unroll
-        for (int i = 1; i <= KL; ++i) {
-          if (j + i < Ny) {
-            const double l = col[(size_t)(KL + KU + i) * M];
-            w[i].x -= l * w[0].x;
-            w[i].y -= l * w[0].y;
-          }
-        }
-        vec(1, j) = w[0];
-#pragma unroll
-        for (int i = 0; i < KL; ++i) w[i] = w[i + 1];
-        w[KL] = (j + KL + 1 < Ny) ? fhat(j + KL + 1) : cx<double>{0, 0};
+        for (int i = 1; i <= KL; ++i) L[u][i - 1] = col[(size_t)(KL + KU + i) * M];
+        const int jf = j + KL + 1 < Ny ? j + KL + 1 : Ny - 1;
+        nf[u] = rin[(size_t)jf * VS];
       }
-      vec(1, Ny - 1) = w[0];
-    }
-    // back substitution, in place (vec 1 = pP)
-    {
-      cx<double> z[KUU + 1];
 #pragma unroll
-      for (int i = 0; i <= KUU; ++i) z[KUU - i] = (Ny - 1 - i >= 0) ? vec(1, Ny - 1 - i) : cx<double>{0, 0};
-      for (int j = Ny - 1; j >= 0; --j) {
-        const double* col = a.lk_ab + (size_t)j * ROWS * M + m;
-        const double dg = col[(size_t)(KL + KU) * M];
-        const cx<double> x = {z[KUU].x / dg, z[KUU].y / dg};
-        vec(1, j) = x;
+      for (int u = 0; u < U; ++u) {
+        const int j = j0 + u;
+        if (j < Ny - 1) {
+          const cx<double> w0 = w[0];
 #pragma unroll
-        for (int i = 1; i <= KUU; ++i) {
-          if (j - i >= 0) {
-            const double u = col[(size_t)(KUU - i) * M];
-            z[KUU - i].x -= u * x.x;
-            z[KUU - i].y -= u * x.y;
-          }
-        }
+          for (int i = 1; i <= KL; ++i)
+            if (dd[u] == i) {
+              w[0] = w[i];
+              w[i] = w0;
+            }
 #pragma unroll
-        for (int i = KUU; i > 0; --i) z[i] = z[i - 1];
-        z[0] = (j - KUU - 1 >= 0) ? vec(1, j - KUU - 1) : cx<double>{0, 0};
-      }
-    }
-    // Stage 3 (flows/plane_couette.py:449-472): R_c streamed into one Hk
-    // forward sweep for the three components -> vecs 2, 3, 4.
-    auto rhs3 = [&](int y, cx<double>* r) {
-      if (y <= 0 || y >= Ny - 1) {
-        r[0] = r[1] = r[2] = {0, 0};
-        return;
-      }
-      const cx<double> pp = vec(1, y);
-      const cx<double> dpp[3] = {ik(kx, pp), d1v(1, y), ik(kz, pp)};
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const cx<double> u = at(a.un, c, y);
-        const cx<double> du = d2f(a.un, c, y);
-        const cx<double> l = {du.x - k2 * u.x, du.y - k2 * u.y};
-        const double g = (1.0 - cc) * nu;
-        const cx<double> hm = {(1.0 / dt) * u.x + g * l.x, (1.0 / dt) * u.y + g * l.y};
-        const cx<double> Nj = at(a.nj, c, y), Nn = at(a.nn, c, y);
-        r[c] = {((hm.x - dpp[c].x) + cc * Nj.x) + (1 - cc) * Nn.x,
-                ((hm.y - dpp[c].y) + cc * Nj.y) + (1 - cc) * Nn.y};
-      }
-    };
-    {
-      cx<double> w[KL + 1][3];
-#pragma unroll
-      for (int i = 0; i <= KL; ++i) {
-        if (i < Ny) rhs3(i, w[i]);
-        else w[i][0] = w[i][1] = w[i][2] = {0, 0};
-      }
-      for (int j = 0; j < Ny - 1; ++j) {
-        const int d = a.hk_piv[(size_t)j * M + m] - j;
-        cx<double> w0[3] = {w[0][0], w[0][1], w[0][2]};
-#pragma unroll
-        for (int i = 1; i <= KL; ++i)
-          if (d == i) {
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-              w[0][c] = w[i][c];
-              w[i][c] = w0[c];
+          for (int i = 1; i <= KL; ++i) {
+            if (j + i < Ny) {
+              const double l = L[u][i - 1];
+              w[i].x -= l * w[0].x;
+              w[i].y -= l * w[0].y;
             }
           }
-        const double* col = a.hk_ab + (size_t)j * ROWS * M + m;
+          tmp[(size_t)j * VS] = w[0];
 #pragma unroll
-        for (int i = 1; i <= KL; ++i) {
-          if (j + i < Ny) {
-            const double l = col[(size_t)(KL + KU + i) * M];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-              w[i][c].x -= l * w[0][c].x;
-              w[i][c].y -= l * w[0][c].y;
-            }
-          }
+          for (int i = 0; i < KL; ++i) w[i] = w[i + 1];
+          w[KL] = (j + KL + 1 < Ny) ? nf[u] : cx<double>{0, 0};
         }
-#pragma unroll
-        for (int c = 0; c < 3; ++c) vec(2 + c, j) = w[0][c];
-#pragma unroll
-        for (int i = 0; i < KL; ++i)
-#pragma unroll
-          for (int c = 0; c < 3; ++c) w[i][c] = w[i + 1][c];
-        if (j + KL + 1 < Ny) rhs3(j + KL + 1, w[KL]);
-        else w[KL][0] = w[KL][1] = w[KL][2] = {0, 0};
-      }
-#pragma unroll
-      for (int c = 0; c < 3; ++c) vec(2 + c, Ny - 1) = w[0][c];
-    }
-    {
-      cx<double> z[KUU + 1][3];
-#pragma unroll
-      for (int i = 0; i <= KUU; ++i)
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-          z[KUU - i][c] = (Ny - 1 - i >= 0) ? vec(2 + c, Ny - 1 - i) : cx<double>{0, 0};
-      for (int j = Ny - 1; j >= 0; --j) {
-        const double* col = a.hk_ab + (size_t)j * ROWS * M + m;
-        const double dg = col[(size_t)(KL + KU) * M];
-        cx<double> x[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          x[c] = {z[KUU][c].x / dg, z[KUU][c].y / dg};
-          vec(2 + c, j) = x[c];
-        }
-#pragma unroll
-        for (int i = 1; i <= KUU; ++i) {
-          if (j - i >= 0) {
-            const double u = col[(size_t)(KUU - i) * M];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-              z[KUU - i][c].x -= u * x[c].x;
-              z[KUU - i][c].y -= u * x[c].y;
-            }
-          }
-        }
-#pragma unroll
-        for (int i = KUU; i > 0; --i)
-#pragma unroll
-          for (int c = 0; c < 3; ++c) z[i][c] = z[i - 1][c];
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-          z[0][c] = (j - KUU - 1 >= 0) ? vec(2 + c, j - KUU - 1) : cx<double>{0, 0};
       }
     }
-    // Stages 4-5 (flows/plane_couette.py:476-486)
-    cx<double> dw0 = d1v(3, 0), dw1 = d1v(3, Ny - 1);
-    if (k2zero) dw0 = {0, 0};
-    const double m00 = a.minv[0 * (size_t)M + m], m01 = a.minv[1 * (size_t)M + m];
-    const double m10 = a.minv[2 * (size_t)M + m], m11 = a.minv[3 * (size_t)M + m];
-    const cx<double> al1 = {-(m00 * dw0.x + m01 * dw1.x), -(m00 * dw0.y + m01 * dw1.y)};
-    const cx<double> al2 = {-(m10 * dw0.x + m11 * dw1.x), -(m10 * dw0.y + m11 * dw1.y)};
-    // Stage 6 (flows/plane_couette.py:491-504) + norm partials
-    const double km = (m % a.Nkx == 0) ? 1.0 : 2.0;
-    for (int y = 0; y < Ny; ++y) {
-      const double v1 = a.v1[(size_t)y * M + m], v2 = a.v2[(size_t)y * M + m];
-      const double q1 = a.q1[(size_t)y * M + m], q2 = a.q2[(size_t)y * M + m];
-      const cx<double> va = vec(3, y);
-      const cx<double> vn = {(va.x + al1.x * v1) + al2.x * v2, (va.y + al1.y * v1) + al2.y * v2};
-      const cx<double> q = {al1.x * q1 + al2.x * q2, al1.y * q1 + al2.y * q2};
-      const cx<double> ua = vec(2, y), wa = vec(4, y);
-      const cx<double> iq_x = ik(kx, q), iq_z = ik(kz, q);
-      const cx<double> nw[3] = {{ua.x - iq_x.x, ua.y - iq_x.y}, vn, {wa.x - iq_z.x, wa.y - iq_z.y}};
-      double s = 0.0;
+    tmp[(size_t)(Ny - 1) * VS] = w[0];
+  }
+  {
+    cx<double> z[KUU + 1];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const size_t idx = c * comp + (size_t)m * Ny + y;
-        const cx<double> old = a.uj[idx];
-        const cx<double> d = {nw[c].x - old.x, nw[c].y - old.y};
-        a.out[idx] = nw[c];
-        if (a.corr) a.corr[idx] = d;
-        s += km * (d.x * d.x) + km * (d.y * d.y);
+    for (int i = 0; i <= KUU; ++i) z[KUU - i] = (Ny - 1 - i >= 0) ? tmp[(size_t)(Ny - 1 - i) * VS] : cx<double>{0, 0};
+    for (int j0 = Ny - 1; j0 >= 0; j0 -= U) {
+      double dg[U], Uv[U][KUU];
+      cx<double> nz[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int j = j0 - u;
+        const double* col = ab + (size_t)(j > 0 ? j : 0) * ROWS * M + m;
+        dg[u] = col[(size_t)(KL + KU) * M];
+#pragma unroll
+        for (int i = 1; i <= KUU; ++i) Uv[u][i - 1] = col[(size_t)(KUU - i) * M];
+        nz[u] = tmp[(size_t)(j - KUU - 1 > 0 ? j - KUU - 1 : 0) * VS];
       }
-      acc += a.qw[y] * s;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int j = j0 - u;
+        if (j >= 0) {
+          const cx<double> x = {z[KUU].x / dg[u], z[KUU].y / dg[u]};
+          out[(size_t)j * VS] = x;
+#pragma unroll
+          for (int i = 1; i <= KUU; ++i) {
+            if (j - i >= 0) {
+              const double uu = Uv[u][i - 1];
+              z[KUU - i].x -= uu * x.x;
+              z[KUU - i].y -= uu * x.y;
+            }
+          }
+#pragma unroll
+          for (int i = KUU; i > 0; --i) z[i] = z[i - 1];
+          z[0] = (j - KUU - 1 >= 0) ? nz[u] : cx<double>{0, 0};
+        }
+      }
     }
   }
+}
+
+// Phase 3 (flows/plane_couette.py:449-472): R_c for the three components
+// (zero on the walls) -> V_R + c, with the pressure row in LDS.
+template <int PO>
+__global__ void __launch_bounds__(kImmT) k_imm_rc(ImmArgs a) {
+  if (a.act && *a.act == 0) return;
+  constexpr int S1 = PO + 1, S2 = PO + 2;
+  extern __shared__ cx<double> pp[];
+  const int m = blockIdx.x, Ny = a.Ny;
+  const size_t comp = (size_t)a.M * Ny, row = (size_t)m * Ny;
+  const double kx = a.mkx[m], kz = a.mkz[m];
+  const double k2 = kx * kx + kz * kz;
+  const cx<double>* P = svec(a, V_P, m);
+  const size_t M = a.M;
+  for (int y = threadIdx.x; y < Ny; y += blockDim.x) pp[y] = P[y * VS];
+  __syncthreads();
+  const double dt = a.dt, cc = a.c, nu = a.nu;
+  for (int y = threadIdx.x; y < Ny; y += blockDim.x) {
+    if (y <= 0 || y >= Ny - 1) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) svec(a, V_R + c, m)[y * VS] = {0, 0};
+      continue;
+    }
+    const cx<double> p = pp[y];
+    const cx<double> dpp[3] = {ikm(kx, p), stencil<S1>(a.d1w, a.d1j0, pp, y), ikm(kz, p)};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const cx<double>* unc = a.un + c * comp + row;
+      const cx<double> u = unc[y];
+      const cx<double> du = stencil<S2>(a.d2w, a.d2j0, unc, y);
+      const cx<double> l = {du.x - k2 * u.x, du.y - k2 * u.y};
+      const double g = (1.0 - cc) * nu;
+      const cx<double> hm = {(1.0 / dt) * u.x + g * l.x, (1.0 / dt) * u.y + g * l.y};
+      const cx<double> Nj = a.nj[c * comp + row + y], Nn = a.nn[c * comp + row + y];
+      svec(a, V_R + c, m)[y * VS] = {((hm.x - dpp[c].x) + cc * Nj.x) + (1 - cc) * Nn.x,
+                                   ((hm.y - dpp[c].y) + cc * Nj.y) + (1 - cc) * Nn.y};
+    }
+  }
+}
+
+// Phase 5 (flows/plane_couette.py:476-504): alpha = -M^-1 (wall divergence
+// of v_arb), the corrected velocity, the correction, and this mode's
+// quadrature-weighted k_metric |delta|^2 into partials[m].
+template <int PO>
+__global__ void __launch_bounds__(kImmT) k_imm_final(ImmArgs a) {
+  if (a.act && *a.act == 0) return;
+  constexpr int S1 = PO + 1;
+  __shared__ double red[kImmT / 64];
+  const int m = blockIdx.x, Ny = a.Ny, M = a.M;
+  const size_t comp = (size_t)M * Ny, row = (size_t)m * Ny;
+  const double kx = a.mkx[m], kz = a.mkz[m];
+  const bool k2zero = (kx * kx + kz * kz == 0.0);
+  const cx<double>* ua = svec(a, V_R + 0, m);
+  const cx<double>* va = svec(a, V_R + 1, m);
+  const cx<double>* wa = svec(a, V_R + 2, m);
+  cx<double> dw0 = stencil<S1>(a.d1w, a.d1j0, va, 0, VS),
+             dw1 = stencil<S1>(a.d1w, a.d1j0, va, Ny - 1, VS);
+  if (k2zero) dw0 = {0, 0};
+  const double m00 = a.minv[0 * (size_t)M + m], m01 = a.minv[1 * (size_t)M + m];
+  const double m10 = a.minv[2 * (size_t)M + m], m11 = a.minv[3 * (size_t)M + m];
+  const cx<double> al1 = {-(m00 * dw0.x + m01 * dw1.x), -(m00 * dw0.y + m01 * dw1.y)};
+  const cx<double> al2 = {-(m10 * dw0.x + m11 * dw1.x), -(m10 * dw0.y + m11 * dw1.y)};
+  const double km = (m % a.Nkx == 0) ? 1.0 : 2.0;
+  double acc = 0.0;
+  for (int y = threadIdx.x; y < Ny; y += blockDim.x) {
+    const double v1 = a.v1[(size_t)y * M + m], v2 = a.v2[(size_t)y * M + m];
+    const double q1 = a.q1[(size_t)y * M + m], q2 = a.q2[(size_t)y * M + m];
+    const cx<double> vv = va[(size_t)y * VS];
+    const cx<double> vn = {(vv.x + al1.x * v1) + al2.x * v2, (vv.y + al1.y * v1) + al2.y * v2};
+    const cx<double> q = {al1.x * q1 + al2.x * q2, al1.y * q1 + al2.y * q2};
+    const cx<double> uu = ua[(size_t)y * VS], ww = wa[(size_t)y * VS];
+    const cx<double> iq_x = ikm(kx, q), iq_z = ikm(kz, q);
+    const cx<double> nw[3] = {{uu.x - iq_x.x, uu.y - iq_x.y}, vn, {ww.x - iq_z.x, ww.y - iq_z.y}};
+    double s = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const size_t idx = c * comp + row + y;
+      const cx<double> old = a.uj[idx];
+      const cx<double> d = {nw[c].x - old.x, nw[c].y - old.y};
+      a.out[idx] = nw[c];
+      if (a.corr) a.corr[idx] = d;
+      s += km * (d.x * d.x) + km * (d.y * d.y);
+    }
+    acc += a.qw[y] * s;
+  }
   const double s = block_sum(acc, red);
-  if (threadIdx.x == 0 && a.partials) a.partials[blockIdx.x] = s;
+  if (threadIdx.x == 0 && a.partials) a.partials[m] = s;
 }
 
 // Quadrature-weighted norm of a field in (3, M, Ny) layout.
@@ -948,7 +955,7 @@ int walls_get_rhs(Ctx* c, const double* state, double* rhs, hipStream_t s) {
 static int walls_imm(Ctx* c, const double* un, const double* uj, const double* nn,
                      const double* nj, double* out, double* corr, double* err, hipStream_t s) {
   WallTables* t = c->wt;
-  const int nb = (t->M + 63) / 64;
+  const int nb = t->M;  // one norm partial per mode
   if (nb > c->max_partials) return c->fail(DNSMI_EINVAL, "imm: too many blocks");
   ImmArgs a{};
   a.un = reinterpret_cast<const cx<double>*>(un);
@@ -984,11 +991,24 @@ static int walls_imm(Ctx* c, const double* un, const double* uj, const double* n
   a.nu = 1.0 / c->cfg.re;
   {
     ProfScope ps(c, KID_W_IMM, s);
+    const size_t lds = (size_t)t->Ny * sizeof(cx<double>);
+    const dim3 gm(t->M), gs((t->M + kSweepLanes - 1) / kSweepLanes, 1),
+        gs3((t->M + kSweepLanes - 1) / kSweepLanes, 3);
     switch (t->p) {
-      case 2: hipLaunchKernelGGL(k_imm<2>, dim3(nb), dim3(64), 0, s, a); break;
-      case 4: hipLaunchKernelGGL(k_imm<4>, dim3(nb), dim3(64), 0, s, a); break;
-      case 6: hipLaunchKernelGGL(k_imm<6>, dim3(nb), dim3(64), 0, s, a); break;
-      case 8: hipLaunchKernelGGL(k_imm<8>, dim3(nb), dim3(64), 0, s, a); break;
+#define IMM(PO)                                                                                   \
+  case PO:                                                                                        \
+    hipLaunchKernelGGL(k_imm_fhat<PO>, gm, dim3(kImmT), lds, s, a);                               \
+    hipLaunchKernelGGL(k_band_sweeps<PO>, gs, dim3(kSweepLanes), 0, s, a, t->lk_ab, t->lk_piv,     \
+                       (int)V_F,                                                                  \
+                       (int)V_T, (int)V_P);                                                       \
+    hipLaunchKernelGGL(k_imm_rc<PO>, gm, dim3(kImmT), lds, s, a);                                 \
+    hipLaunchKernelGGL(k_band_sweeps<PO>, gs3, dim3(kSweepLanes), 0, s, a, t->hk_ab, t->hk_piv,   \
+                       (int)V_R,                                                                  \
+                       (int)V_T, (int)V_R);                                                       \
+    hipLaunchKernelGGL(k_imm_final<PO>, gm, dim3(kImmT), 0, s, a);                                \
+    break;
+      IMM(2) IMM(4) IMM(6) IMM(8)
+#undef IMM
       default: return c->fail(DNSMI_EINVAL, "fd_order");
     }
   }

@@ -171,7 +171,8 @@ int dnsmi_mlp_sizes(int H, long long* nparams, long long* packed_weight_bytes,
                     long long* bias_floats);
 /* Gradient of L = (1/B) sum (y - t)^2 over X (B,16), T (B,6), FP32, samples
  * summed in a fixed order (deterministic).  part: ceil(B/32) * (nparams + 1)
- * floats of scratch; grad: nparams floats; loss_dev: 1 float or NULL.
+ * + ceil(B/1024) * nparams floats of scratch; grad: nparams floats;
+ * loss_dev: 1 float or NULL.
  * bf16 != 0 rounds weights and layer inputs to bfloat16 as inference does
  * (straight-through gradients). */
 int dnsmi_mlp_grad(int H, int B, const float* X, const float* T, const float* params, int bf16,

@@ -204,3 +204,25 @@ def test_stats_at_full_size_agree_with_the_norm():
     n = float(flow.norm(u))
     assert abs(float(st["E'"]) - 0.5 * n * n) <= 1e-12 * 0.5 * n * n
     assert all(math.isfinite(float(v)) for v in st.values())
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_unpadded_y_matches_oracle(precision):
+    """oversample_y = False (y transformed on ny points, no 3/2 padding):
+    RHS and one predictor-corrector pass against the oracle."""
+    from dnsjax_amd.fields import periodic_field
+    from dnsjax_amd.flows import TriplyPeriodicFlow
+
+    n, L = 16, 2 * math.pi
+    flow = TriplyPeriodicFlow(Params(system="kolmogorov", nx=n, ny=n, nz=n, lx=L, ly=L, lz=L, re=40.0, dt=0.01,
+                                     oversample_y=False, precision=precision))
+    op = P.PeriodicParams(nx=n, ny=n, nz=n, lx=L, ly=L, lz=L, re=40.0, dt=0.01, system="kolmogorov",
+                          oversample_y=False)
+    of = P.Flow(op)
+    u = periodic_field(n, n, n, L, L, L, 0.1, 11)
+    tol = 1e-12 if precision == "fp64" else 1e-5
+    assert rel(flow.get_rhs(flow.to_device(u)), P.get_rhs(u, of)) < tol
+    pc, _ = S.make_stepper(P.get_rhs, P.predict, P.correct, P.norm)
+    want, _, _ = pc(u, of)
+    got, _, _ = flow.predict_and_correct(flow.to_device(u))
+    assert rel(got, want) < tol
