@@ -120,6 +120,8 @@ __global__ void k_ctl_begin_guess(StepCtl* ctl, int max_pass) {
 int guess_periodic(Ctx* c, const double* state, double* pred, double* rhs_prev, const float* h1,
                    float* h2, const uint16_t* wb, const float* bias, int H, float s_1, float s_2,
                    float s_out, int mode, hipStream_t s);
+int guess_walls(Ctx* c, const double* state, double* pred, double* rhs_prev, const float* h1,
+                float* h2, int mode, hipStream_t s);
 
 // A step started from the corrector guess (periodic): the guessed predictor,
 // then up to `slots` passes, each one RHS, with the same device rule as
@@ -131,12 +133,13 @@ int step_device_guess(Ctx* c, const double* state, double* pred, double* rhs_a, 
                       int counts_cap, double tol, int max_pass, double floor_ratio,
                       double floor_cap, hipStream_t s) {
   if (!c->ctl) return c->fail(DNSMI_EINVAL, "control block missing");
-  if (c->walls) return c->fail(DNSMI_EINVAL, "guessed steps: periodic contexts only");
   StepCtl* ctl = reinterpret_cast<StepCtl*>(c->ctl);
   double* err = &ctl->err;
   if (slots > max_pass) slots = max_pass;
   c->act = nullptr;
-  int rc = guess_periodic(c, state, pred, rhs_a, h1, h2, wb, bias, H, s_1, s_2, s_out, mode, s);
+  int rc = c->walls ? guess_walls(c, state, pred, rhs_a, h1, h2, mode, s)
+                    : guess_periodic(c, state, pred, rhs_a, h1, h2, wb, bias, H, s_1, s_2, s_out,
+                                     mode, s);
   if (rc) return rc;
   hipLaunchKernelGGL(k_ctl_begin_guess, dim3(1), dim3(64), 0, s, ctl, max_pass);
   if ((rc = check_launch(c, "ctl_begin_guess"))) return rc;
@@ -144,7 +147,8 @@ int step_device_guess(Ctx* c, const double* state, double* pred, double* rhs_a, 
   for (int i = 1; i <= slots && rc == DNSMI_OK; ++i) {
     const double* rp = (i % 2) ? rhs_a : rhs_b;
     double* rn = (i % 2) ? rhs_b : rhs_a;
-    rc = periodic_iterate_correction(c, pred, rp, rn, err, s);
+    rc = c->walls ? walls_iterate_correction(c, state, pred, rp, rn, err, s)
+                  : periodic_iterate_correction(c, pred, rp, rn, err, s);
     if (rc) break;
     hipLaunchKernelGGL(k_ctl_after_iteration, dim3(1), dim3(64), 0, s, ctl, tol, max_pass,
                        floor_ratio, floor_cap);
@@ -152,7 +156,7 @@ int step_device_guess(Ctx* c, const double* state, double* pred, double* rhs_a, 
   }
   c->act = nullptr;
   if (rc) return rc;
-  if ((rc = periodic_correct_velocity(c, pred, nullptr, s))) return rc;
+  if (!c->walls && (rc = periodic_correct_velocity(c, pred, nullptr, s))) return rc;
   hipLaunchKernelGGL(k_ctl_end, dim3(1), dim3(64), 0, s, ctl, counts, counts_cap, tol);
   return check_launch(c, "ctl_end");
 }

@@ -11,8 +11,9 @@ corrector pass, as in the reference.
 The mixed form (FP32 transforms) has a noise floor near 1e-10 in the
 corrector error.  With ``params.corrector_floor_ratio`` set (the mixed
 default), a pass whose error did not shrink by that ratio while already
-below ``corrector_floor_cap * tolerance`` ends the corrector and counts as
-converged (``RunResult.floor_stops`` counts such steps).
+below ``params.floor_cap_value`` (min(100 tolerance, 1e-8)) ends the
+corrector and counts as converged (``RunResult.floor_stops`` counts such
+steps).
 """
 
 from __future__ import annotations
@@ -39,7 +40,7 @@ def run(flow, state, n_steps: int, callback=None) -> RunResult:
     p = flow.params
     tol, max_it = p.corrector_tolerance, p.max_corrector_iterations
     ratio = p.corrector_floor_ratio
-    cap = p.corrector_floor_cap * tol
+    cap = p.floor_cap_value
     res = RunResult(state=state)
     last_error = 0.0
     for it in range(n_steps):

@@ -217,7 +217,10 @@ def run_guessed(flow, state, n_steps: int, mode: int = 2, net: Net | None = None
                 scales: Scales | None = None, callback=None) -> GuessRunResult:
     """The reference's time loop (``__main__.py:125-203``) with the corrector
     started from a guess: mode 1 linear, 3 quadratic extrapolation of N,
-    2 quadratic plus the network.  The first steps of a run lack history and
+    2 quadratic plus the network (periodic flows; plane Couette takes modes
+    0, 1 and 3, its guess being the predictor's solve fed with the
+    extrapolated N).  Mode 0 is the plain predictor: the same steps as the
+    plain driver, counted as passes (= iterations + 1).  The first steps of a run lack history and
     start from the plain predictor (step 0) and linear extrapolation (step
     1).  Up to ``max_corrector_iterations + 1`` passes per step (the plain
     scheme's RHS budget); the loop stops after a step that did not
@@ -232,7 +235,7 @@ def run_guessed(flow, state, n_steps: int, mode: int = 2, net: Net | None = None
     nat = flow._native
     tol, max_pass = p.corrector_tolerance, p.max_corrector_iterations + 1
     ratio = p.corrector_floor_ratio
-    cap = p.corrector_floor_cap * tol
+    cap = p.floor_cap_value
     sc = (1.0, 1.0, 1.0) if scales is None else scales.astuple()
     ops = None if net is None else net.operands
     hist = [torch.zeros(nat.shape + (2,), dtype=torch.float32, device=nat.device) for _ in range(2)]

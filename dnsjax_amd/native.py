@@ -137,7 +137,7 @@ class NativeContext:
         tol = p.corrector_tolerance if tol is None else tol
         max_iter = p.max_corrector_iterations if max_iter is None else max_iter
         ratio = p.corrector_floor_ratio if floor_ratio is None else floor_ratio
-        cap = (p.corrector_floor_cap * tol) if floor_cap is None else floor_cap
+        cap = min(p.corrector_floor_cap * tol, p.corrector_floor_abs) if floor_cap is None else floor_cap
         ncap = 0 if counts is None else counts.numel()
         self._chk(self.lib.dnsmi_step_device(self.handle, _ptr(state), _ptr(nxt), _ptr(rhs_a), _ptr(rhs_b),
                                              slots, _ptr(counts), ncap, tol, max_iter,
@@ -171,7 +171,7 @@ class NativeContext:
             self.handle, _ptr(state), _ptr(nxt), _ptr(rhs_a), _ptr(rhs_b), _ptr(hist1), _ptr(hist2), _ptr(wb),
             _ptr(bias), H, s_1, s_2, s_out, mode, slots, _ptr(counts), ncap, p.corrector_tolerance,
             p.max_corrector_iterations + 1, 0.0 if ratio is None else ratio,
-            p.corrector_floor_cap * p.corrector_tolerance, _stream()))
+            p.floor_cap_value, _stream()))
 
     def step_stats(self) -> dict:
         out = (ctypes.c_double * 6)()

@@ -42,10 +42,15 @@ class Params:
     variants: int = 0
     # Mixed form only: stop the corrector at the FP32 noise floor, i.e. when
     # an iteration fails to shrink the error by this factor while the error
-    # is already below floor_cap * corrector_tolerance.  None keeps the
+    # is already below floor_cap (see `floor_cap_value`).  None keeps the
     # reference's rule (iterate until error <= tolerance) unchanged.
     corrector_floor_ratio: float | None = None
     corrector_floor_cap: float = 100.0
+    # Absolute ceiling of that window: the FP32 floor of the corrector error
+    # for O(1) fields is near 1e-10 (128^3); a stall above 1e-8 is slow
+    # contraction (the wall-bounded corrector contracts only ~2x per pass),
+    # not noise, and must not end the corrector.
+    corrector_floor_abs: float = 1e-8
 
     def __post_init__(self) -> None:
         if self.system not in PERIODIC_SYSTEMS + WALLED_SYSTEMS:
@@ -58,6 +63,11 @@ class Params:
             self.corrector_floor_ratio = 0.5
         if not self.re > 0 or not self.dt > 0 or not 0 <= self.implicitness <= 1:
             raise ValueError("need re > 0, dt > 0, 0 <= implicitness <= 1")
+
+    @property
+    def floor_cap_value(self) -> float:
+        """Error below which a stalled pass counts as the noise floor."""
+        return min(self.corrector_floor_cap * self.corrector_tolerance, self.corrector_floor_abs)
 
     @property
     def walls(self) -> bool:

@@ -179,3 +179,19 @@ def guess(state, h1, h2, flow, mode, params=None, scales=(1.0, 1.0, 1.0)):
     d[:, 0, 0, 0] = 0.0
     pred = u1 + p.implicitness * p.dt * w * d
     return pred, n0 + d, n0.copy()
+
+
+def guess_walls(state, h1, h2, flow, mode):
+    """(pred, rhs_prev, N^n) of the wall-bounded guess: the predictor's solve
+    (``flows/plane_couette.py:507-519``) fed with N_ext = N^n + dN, dN = 0
+    (mode 0), D1 (mode 1) or D1 + D2 (mode 3); rhs_prev = N_ext.  h1, h2
+    are N^{n-1}, N^{n-2} rounded to FP32 by the caller."""
+    from . import walls as W
+
+    n0 = W.get_rhs(state, flow)
+    if mode == 0:
+        ext = n0
+    else:
+        d1 = n0 - h1
+        ext = n0 + (d1 if mode == 1 else d1 + ((n0 - 2 * h1) + h2))
+    return W.predict(state, ext, flow), ext, n0.copy()

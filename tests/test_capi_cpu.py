@@ -133,3 +133,14 @@ def test_driver_noise_floor_rule_for_mixed():
     g.params = Params(corrector_tolerance=1e-10, max_corrector_iterations=10)
     r = run(g, 0.0, 1)
     assert r.counts == [9] and r.floor_stops == 0
+
+
+def test_noise_floor_rule_ignores_slow_contraction():
+    """A loose tolerance with a corrector contracting only ~2x per pass (the
+    wall-bounded one) must iterate to the tolerance: a stall above the
+    absolute ceiling (1e-8) is not the noise floor."""
+    f = _ScriptedFlow([1e-3, 6e-4, 3.5e-4, 2e-4, 1.1e-4, 6e-5, 3.2e-5, 1.7e-5, 9e-6], tol=1e-5, max_it=10)
+    f.params = Params(precision="mixed", corrector_tolerance=1e-5, max_corrector_iterations=10)
+    assert f.params.floor_cap_value == 1e-8
+    r = run(f, 0.0, 1)
+    assert r.counts == [8] and r.floor_stops == 0 and r.converged

@@ -204,3 +204,65 @@ def test_guessed_graph_steps_match_eager(precision, mode):
     assert gs.passes() == eager.passes
     st = gs.step_stats()
     assert st["steps"] == n and st["unconverged"] == 0 and st["nan"] == 0
+
+
+def _walls(nx, ny, nz, precision="fp64", dt=0.005):
+    from dnsjax_amd.flows import PlaneCouetteFlow
+
+    return PlaneCouetteFlow(Params(system="plane-couette", nx=nx, ny=ny, nz=nz, lx=2 * math.pi, lz=math.pi,
+                                   re=400.0, dt=dt, fd_order=4, corrector_tolerance=1e-10,
+                                   max_corrector_iterations=20, precision=precision, tile_cols=16))
+
+
+@pytest.mark.parametrize("mode", [0, 1, 3])
+def test_walls_guess_matches_oracle(mode):
+    from oracle import walls as W
+    from dnsjax_amd.fields import wall_field
+
+    flow = _walls(8, 17, 8, dt=0.01)
+    of = W.WallFlow(W.WallParams(nx=8, ny=17, nz=8, fd_order=4, re=400.0, dt=0.01))
+    u = wall_field(8, 17, 8, of.p.lx, of.p.lz, 0.05, seed=3)
+    h1 = W.get_rhs(wall_field(8, 17, 8, of.p.lx, of.p.lz, 0.05, seed=4), of).astype(np.complex64)
+    h2 = W.get_rhs(wall_field(8, 17, 8, of.p.lx, of.p.lz, 0.05, seed=5), of).astype(np.complex64)
+    want_pred, want_rhs, want_hist = M.guess_walls(u, h1.astype(np.complex128), h2.astype(np.complex128), of,
+                                                   mode)
+    nat = flow._native
+    dev = lambda h: torch.from_numpy(np.stack([h.real, h.imag], -1).astype(np.float32)).to(nat.device)  # noqa: E731
+    d1, d2 = dev(h1).contiguous(), dev(h2).contiguous()
+    pred, rhs = nat.empty(), nat.empty()
+    nat.guess(flow.to_device(u), pred, rhs, d1, d2, mode)
+    assert rel(rhs.cpu().numpy(), want_rhs) <= 1e-12
+    assert rel(pred.cpu().numpy(), want_pred) <= 1e-11  # IMM tolerance
+    h = d2.cpu().numpy().astype(np.float64)
+    w = np.stack([want_hist.real, want_hist.imag], -1)
+    assert np.max(np.abs(h - w)) <= 1e-7 * np.max(np.abs(w))
+
+
+def test_walls_guessed_runs_and_graph_steps():
+    """Plane Couette 16 x 33 x 16: mode 0 reproduces the plain driver bit for
+    bit; quadratic extrapolation stays on its trajectory with no more RHS
+    evaluations; device-decided guessed graph steps equal the eager loop."""
+    from dnsjax_amd.driver import run
+    from dnsjax_amd.fields import wall_field
+    from dnsjax_amd.learned import GuessedGraphStepper, run_guessed
+
+    flow = _walls(16, 33, 16)
+    u0 = flow.to_device(wall_field(16, 33, 16, 2 * math.pi, math.pi, 0.05, seed=2))
+    n = 8
+    plain = run(flow, u0.clone(), n)
+    assert plain.converged
+    g0 = run_guessed(flow, u0.clone(), n, mode=0)
+    assert torch.equal(g0.state, plain.state) and g0.passes == [c + 1 for c in plain.counts]
+    ref = []
+    g3 = run_guessed(flow, u0.clone(), n, mode=3, callback=lambda i, s: ref.append(s.clone()))
+    assert g3.converged and len(g3.passes) == n
+    assert rel(g3.state.cpu().numpy(), plain.state.cpu().numpy()) <= 1e-8
+    assert g3.rhs_evals <= plain.rhs_evals, (g3.passes, plain.counts)
+    gs = GuessedGraphStepper(flow, u0, mode=3)
+    torch.cuda.synchronize()
+    assert torch.equal(gs.state, ref[1])
+    for i in range(2, n):
+        gs.step(1)
+        torch.cuda.synchronize()
+        assert torch.equal(gs.state, ref[i]), f"step {i} differs"
+    assert gs.passes() == g3.passes

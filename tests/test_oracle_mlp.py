@@ -132,3 +132,26 @@ def test_extrapolation_is_exact_for_polynomial_histories():
     # D1 = N^n - N^{n-1} = b + 3 e ; D2 = 2 e
     assert np.allclose(X[:, 0], (b[0] + 3 * e[0]).real.reshape(-1), atol=1e-6 * scale)
     assert np.allclose(X[:, 6], (2 * e[0]).real.reshape(-1), atol=1e-6 * scale)
+
+
+def test_walls_guess_mode0_and_corrector_map():
+    """Walls: mode 0 is the reference predictor; one corrector pass from any
+    guess is the reference's correction S(u^n; u_g, N_ext, N(u_g))."""
+    from oracle import walls as W
+    from dnsjax_amd.fields import wall_field
+
+    flow = W.WallFlow(W.WallParams(nx=8, ny=17, nz=8, fd_order=4, re=400.0, dt=0.01))
+    u = wall_field(8, 17, 8, flow.p.lx, flow.p.lz, 0.05, seed=3)
+    n0 = W.get_rhs(u, flow)
+    z = np.zeros_like(u)
+    pred, rhs, hist = M.guess_walls(u, z, z, flow, 0)
+    assert np.array_equal(pred, W.predict(u, n0, flow)) and np.array_equal(rhs, n0)
+    assert np.array_equal(hist, n0)
+    h1 = _f32(W.get_rhs(wall_field(8, 17, 8, flow.p.lx, flow.p.lz, 0.05, seed=4), flow))
+    h2 = _f32(W.get_rhs(wall_field(8, 17, 8, flow.p.lx, flow.p.lz, 0.05, seed=5), flow))
+    for mode in (1, 3):
+        g, ext, _ = M.guess_walls(u, h1, h2, flow, mode)
+        d1 = n0 - h1
+        want_ext = n0 + (d1 if mode == 1 else 2 * d1 - (h1 - h2))
+        assert np.allclose(ext, want_ext, rtol=0, atol=1e-13 * np.max(np.abs(n0)))
+        assert np.array_equal(g, W.predict(u, ext, flow))
