@@ -104,7 +104,7 @@ def measured_traffic(kernel: str, b: int, mode: str = "2"):
     if not os.path.exists(path):
         return None
     t = "double" if b == 16 else "float"
-    key = f"k_{kernel}<{t}, 192, 16>" if kernel != "y_forward" else f"k_y_forward<{t}, 192, 16, {mode}>"
+    key = f"k_{kernel}<{t}, 192, 16>" if kernel != "y_forward" else f"k_y_forward<{t}, 192, 16, {mode}, true>"
     ent = json.load(open(path))["kernels"].get(key)
     return int(ent["traffic_bytes"]) if ent else None
 

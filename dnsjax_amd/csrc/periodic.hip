@@ -479,20 +479,52 @@ __global__ void __launch_bounds__(R * Plan<P>::N1, 2)
     }
     forward_store(0, nl0);
   }
-  T nl1[N2], nl2[N2];
+  // Components 1 and 2 are real: one complex transform of nl1 + i nl2,
+  // separated by Hermitian symmetry, X1[k] = (Z[k] + conj Z[P-k]) / 2 and
+  // X2[k] = (Z[k] - conj Z[P-k]) / 2i.  Z[P-k] for k < Nx1 lives in another
+  // lane of the row (lane (N2 - j) mod N2), so the needed high outputs are
+  // exchanged through the row's LDS.
   {
-    T ux[N2], wx[N2];
-    inverse_pair(2, ux, wx);
+    cx<T> v[N2];
+    {
+      T ux[N2], wx[N2];
+      inverse_pair(2, ux, wx);
 #pragma unroll
-    for (int k2 = 0; k2 < N2; ++k2) {
-      const T a0 = ux[k2], a1 = uy[k2], a2 = uz[k2];
-      const T b0 = wx[k2], b1 = wy[k2], b2 = wz[k2];
-      nl1[k2] = (a2 * b0 - a0 * b2) + (((a2 * C0 - a0 * C2) + (U2 * b0 - U0 * b2)) + B1);
-      nl2[k2] = (a0 * b1 - a1 * b0) + (((a0 * C1 - a1 * C0) + (U0 * b1 - U1 * b0)) + B2);
+      for (int k2 = 0; k2 < N2; ++k2) {
+        const T a0 = ux[k2], a1 = uy[k2], a2 = uz[k2];
+        const T b0 = wx[k2], b1 = wy[k2], b2 = wz[k2];
+        v[k2].x = (a2 * b0 - a0 * b2) + (((a2 * C0 - a0 * C2) + (U2 * b0 - U0 * b2)) + B1);
+        v[k2].y = (a0 * b1 - a1 * b0) + (((a0 * C1 - a1 * C0) + (U0 * b1 - U1 * b0)) + B2);
+      }
+    }
+    const int j = opaque(j_);
+    row_step_a_store_tab<P, N2, N1, -1>(v, j, rl, tab_f);
+    __syncthreads();
+    cx<T> o[N1];
+    if (j < N2) row_step_b_load<P, N2, N1, -1>(o, j, rl);
+    __syncthreads();
+    // publish Z[k] for the partners' indices: k = j + N2 k2 with k2 >= KH
+    constexpr int KH = N1 - 1 - (Nx1 - 1) / N2;
+    if (j < N2) {
+#pragma unroll
+      for (int k2 = KH; k2 < N1; ++k2) rl[j + N2 * k2] = o[k2];
+      if (j == 0) rl[0] = o[0];
+    }
+    __syncthreads();
+    if (j < N2 && ok) {
+#pragma unroll
+      for (int k2 = 0; k2 < N1; ++k2) {
+        const int k = j + N2 * k2;
+        if (N2 * k2 < Nx1 && k < Nx1) {
+          const cx<T> z = o[k2], w = rl[k == 0 ? 0 : P - k];
+          const cx<T> x1 = {(z.x + w.x) * (T)0.5, (z.y - w.y) * (T)0.5};
+          const cx<T> x2 = {(z.y + w.y) * (T)0.5, (w.x - z.x) * (T)0.5};
+          out[1 * fstride + k] = cscale(x1, inv);
+          out[2 * fstride + k] = cscale(x2, inv);
+        }
+      }
     }
   }
-  forward_store(1, nl1);
-  forward_store(2, nl2);
 }
 
 // K3, split form (N2 even): 2 N1 threads per row.  The inverse step B

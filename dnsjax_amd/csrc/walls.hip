@@ -694,11 +694,12 @@ __global__ void __launch_bounds__(kSweepLanes)
   if (m >= a.M) return;
   const int r = blockIdx.y, Ny = a.Ny, M = a.M;
   // rin and out may be the same vector (the H sweeps solve in place): the
-  // forward sweep reads rin, the back substitution writes out.
-  const cx<double>* rin = svec(a, vin + r, m);
-  cx<double>* __restrict__ tmp = svec(a, vtmp + r, m);
-  cx<double>* out = svec(a, vout + r, m);
+  // forward sweep only reads rin and writes tmp, the back substitution only
+  // reads tmp and writes out, so each sweep's pointers are restrict within
+  // its own scope (a batch's loads need not wait for the previous stores).
   {
+    const cx<double>* __restrict__ rin = svec(a, vin + r, m);
+    cx<double>* __restrict__ tmp = svec(a, vtmp + r, m);
     cx<double> w[KL + 1];
 #pragma unroll
     for (int i = 0; i <= KL; ++i) w[i] = (i < Ny) ? rin[(size_t)i * VS] : cx<double>{0, 0};
@@ -748,6 +749,8 @@ __global__ void __launch_bounds__(kSweepLanes)
     tmp[(size_t)(Ny - 1) * VS] = w[0];
   }
   {
+    const cx<double>* __restrict__ tmp = svec(a, vtmp + r, m);
+    cx<double>* __restrict__ out = svec(a, vout + r, m);
     cx<double> z[KUU + 1];
 #pragma unroll
     for (int i = 0; i <= KUU; ++i) z[KUU - i] = (Ny - 1 - i >= 0) ? tmp[(size_t)(Ny - 1 - i) * VS] : cx<double>{0, 0};
