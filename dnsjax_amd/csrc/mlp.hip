@@ -421,14 +421,14 @@ __global__ void k_mlp_features(FeatArgs a, const cx<double>* __restrict__ nm2,
 }
 
 // ------------------------------------------------------------------ guess
-// After the Euler predictor: pred holds u^1, ncur = N^n (FP64); h1 = N^{n-1}
+// ncur = N^n (FP64, from the RHS pipeline); h1 = N^{n-1}
 // and h2 = N^{n-2} (FP32 history).  Per mode, with
 //   mode 0: dN = 0                    (the plain predictor)
 //   mode 1: dN = D1                   (linear extrapolation of N)
 //   mode 3: dN = D1 + D2              (quadratic extrapolation)
 //   mode 2: dN = D1 + D2 + s_out net(x) / w   (learned residual)
 // and dN Leray-projected (zero at the mean mode):
-//   pred     <- u^1 + c ildt_2 dN           (the guess)
+//   pred     <- u^1 + c ildt_2 dN, u^1 = (u^n ldt_1 + N^n) ildt_2
 //   rhs_prev <- N^n + dN                    (so that the incremental
 //               corrector from the guess is the same fixed-point map)
 //   h2       <- N^n                         (the caller then swaps h1, h2)
@@ -438,7 +438,8 @@ __global__ void k_mlp_features(FeatArgs a, const cx<double>* __restrict__ nm2,
 // exchanged through LDS.
 template <int H>
 __global__ void __launch_bounds__(256)
-    k_guess_periodic(FeatArgs a, const cx<double>* __restrict__ ncur, const float* __restrict__ h1,
+    k_guess_periodic(FeatArgs a, const cx<double>* __restrict__ state,
+                     const cx<double>* __restrict__ ncur, const float* __restrict__ h1,
                      float* __restrict__ h2, double* __restrict__ pred,
                      double* __restrict__ rhs_prev, const uint16_t* __restrict__ wb,
                      const float* __restrict__ bias, int mode) {
@@ -515,6 +516,10 @@ __global__ void __launch_bounds__(256)
                          (kx * dn[0].y + ky * dn[1].y) + kz * dn[2].y};
   const double kk[3] = {kx, ky, kz};
   const double cil = a.c * a.dt * w;  // c ildt_2
+  // the Euler predictor, as the y-forward epilogue forms it
+  // (flows/monochromatic.py:320-333): u^1 = (u^n ldt_1 + N^n) ildt_2
+  const double ldt1 = mean ? 0.0 : 1.0 / a.dt + (1.0 - a.c) * lapl / a.re;
+  const double ildt2 = mean ? 0.0 : 1.0 / (1.0 / a.dt - a.c * lapl / a.re);
   cx<double>* pd = reinterpret_cast<cx<double>*>(pred);
   cx<double>* rp = reinterpret_cast<cx<double>*>(rhs_prev);
 #pragma unroll
@@ -523,10 +528,9 @@ __global__ void __launch_bounds__(256)
     const cx<double> d = mean ? cx<double>{0.0, 0.0}
                               : cx<double>{dn[q].x + kk[q] * kd.x * il, dn[q].y + kk[q] * kd.y * il};
     const long long e = q * a.n + i;
-    if (mode != 0) {
-      const cx<double> u1 = pd[e];
-      pd[e] = {u1.x + cil * d.x, u1.y + cil * d.y};
-    }
+    const cx<double> u = state[e];
+    const cx<double> u1 = {(u.x * ldt1 + nc[q].x) * ildt2, (u.y * ldt1 + nc[q].y) * ildt2};
+    pd[e] = mode == 0 ? u1 : cx<double>{u1.x + cil * d.x, u1.y + cil * d.y};
     rp[e] = mode == 0 ? nc[q] : cx<double>{nc[q].x + d.x, nc[q].y + d.y};
     reinterpret_cast<float2*>(h2)[e] = float2{(float)nc[q].x, (float)nc[q].y};
   }
@@ -624,7 +628,7 @@ int mlp_features(Ctx* c, const double* nm2, const double* nm1, const double* ncu
   return check_launch(c, "mlp_features");
 }
 
-int periodic_predict_only(Ctx* c, const double* state, double* pred, double* rhs0, hipStream_t s);
+int periodic_get_rhs(Ctx* c, const double* state, double* rhs, hipStream_t s);
 
 int guess_periodic(Ctx* c, const double* state, double* pred, double* rhs_prev, const float* h1,
                    float* h2, const uint16_t* wb, const float* bias, int H, float s_1, float s_2,
@@ -633,17 +637,19 @@ int guess_periodic(Ctx* c, const double* state, double* pred, double* rhs_prev, 
   if (mode < 0 || mode > 3) return c->fail(DNSMI_EINVAL, "guess: mode must be 0, 1, 2 or 3");
   if (mode == 2 && H != 32 && H != 64)
     return c->fail(DNSMI_EINVAL, "guess: hidden width must be 32 or 64");
-  int rc = periodic_predict_only(c, state, pred, c->rhs0, s);
+  int rc = periodic_get_rhs(c, state, c->rhs0, s);
   if (rc) return rc;
   const FeatArgs a = feat_args(c, s_1, s_2, s_out);
   const int nb = (int)((a.n + 255) / 256);
   const cx<double>* n0 = reinterpret_cast<const cx<double>*>(c->rhs0);
   if (mode != 2 || H == 32)
-    hipLaunchKernelGGL(k_guess_periodic<32>, dim3(nb), dim3(256), 0, s, a, n0, h1, h2, pred,
-                       rhs_prev, wb, bias, mode);
+    hipLaunchKernelGGL(k_guess_periodic<32>, dim3(nb), dim3(256), 0, s, a,
+                       reinterpret_cast<const cx<double>*>(state), n0, h1, h2, pred, rhs_prev, wb,
+                       bias, mode);
   else
-    hipLaunchKernelGGL(k_guess_periodic<64>, dim3(nb), dim3(256), 0, s, a, n0, h1, h2, pred,
-                       rhs_prev, wb, bias, mode);
+    hipLaunchKernelGGL(k_guess_periodic<64>, dim3(nb), dim3(256), 0, s, a,
+                       reinterpret_cast<const cx<double>*>(state), n0, h1, h2, pred, rhs_prev, wb,
+                       bias, mode);
   return check_launch(c, "guess");
 }
 

@@ -41,8 +41,6 @@ struct StepArgs {
   double dt, c, re;
 };
 
-// Stored index of padded index n along an axis of `nmodes` modes padded to
-// P (fft.py:78-86, 127-135); -1 for the zero band.
 
 template <typename T>
 __device__ __forceinline__ cx<T> to_t(cx<double> d) {
@@ -1167,15 +1165,6 @@ int periodic_predict_and_correct(Ctx* c, const double* state, double* pred, doub
   rc = periodic_rhs_mode<MODE_CORRECT>(c, pred, b, s, &nb);
   if (rc) return rc;
   return finalize_norm(c, nb, err, s);
-}
-
-// The Euler predictor alone: N^0 = rhs(state) into rhs0, u^1 into pred.
-int periodic_predict_only(Ctx* c, const double* state, double* pred, double* rhs0, hipStream_t s) {
-  StepArgs a = step_args(c);
-  a.state = state;
-  a.pred = pred;
-  a.rhs_out = rhs0;
-  return periodic_rhs_mode<MODE_PREDICT>(c, state, a, s, nullptr);
 }
 
 int periodic_iterate_correction(Ctx* c, double* pred, const double* rhs_prev, double* rhs_next,

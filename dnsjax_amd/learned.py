@@ -280,8 +280,12 @@ class GuessedGraphStepper:
     """Guessed steps with the corrector decided on the device, one captured
     graph launch per step (two graphs: state and history buffers swap each
     step).  The first two steps (plain predictor, then linear extrapolation)
-    run eagerly through the same device step; the rest replay the graph of
-    the requested mode."""
+    run eagerly through the same device step with the full pass budget; the
+    rest replay the graph of the requested mode with `slots` corrector slots
+    (default: the full budget, max_corrector_iterations + 1).  A step that
+    needs more passes than the graph holds is counted in
+    ``step_stats()["unconverged"]``; a run with none is identical to the
+    eager loop."""
 
     def __init__(self, flow, state, mode: int = 2, net: Net | None = None, scales: Scales | None = None,
                  slots: int | None = None, capacity: int = 4096):
@@ -303,15 +307,17 @@ class GuessedGraphStepper:
         self.parity = 0
         self.steps = 0
         nat.step_reset()
-        self._enqueue(0, 0)  # plain predictor, records N^0
-        self._enqueue(1, min(mode, 1))  # linear extrapolation
+        # the start-up steps run eagerly with the full pass budget
+        full = p.max_corrector_iterations + 1
+        self._enqueue(0, 0, full)  # plain predictor, records N^0
+        self._enqueue(1, min(mode, 1), full)  # linear extrapolation
         self.parity, self.steps = 0, 2
         self.graphs = [self._capture(0), self._capture(1)]
 
-    def _enqueue(self, parity: int, mode: int) -> None:
+    def _enqueue(self, parity: int, mode: int, slots: int | None = None) -> None:
         b, h = self.bufs, self.hist
         self.nat.step_device_guess(b[parity], b[1 - parity], self.rhs[0], self.rhs[1], h[parity], h[1 - parity],
-                                   mode, self.slots, self.counts, self.ops, self.sc)
+                                   mode, self.slots if slots is None else slots, self.counts, self.ops, self.sc)
 
     def _capture(self, parity: int):
         g = torch.cuda.CUDAGraph()
