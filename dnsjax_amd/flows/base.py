@@ -50,6 +50,14 @@ class NativeFlow:
             state = torch.from_numpy(np.ascontiguousarray(state, dtype=np.complex128))
         return state.to(self._native.device).contiguous()
 
+    def save_snapshot(self, path: str, state) -> None:
+        """Write ``state`` in the reference's snapshot format (``.npz``:
+        periodic ``velocity_phys``, the total velocity on the padded grid;
+        walls ``velocity_phys_nonexpanded``)."""
+        from ..snapshots import save_snapshot
+
+        save_snapshot(path, state, self.params)
+
     def get_stats(self, state) -> dict:
         """E, I, D, E' as 0-d device tensors (``flows/monochromatic.py:508-510``,
         ``flows/plane_couette.py:616-618``)."""

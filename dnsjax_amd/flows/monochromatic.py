@@ -17,6 +17,7 @@ import torch
 
 from ..fields import periodic_field
 from ..params import Params
+from ..snapshots import load_snapshot
 from .base import NativeFlow
 
 
@@ -26,10 +27,15 @@ class TriplyPeriodicFlow(NativeFlow):
             raise ValueError("TriplyPeriodicFlow needs a periodic system")
         super().__init__(params)
 
-    def init_state(self, seed: int | None = None, rms: float = 0.0) -> torch.Tensor:
-        """Laminar start (zeros, ``flows/monochromatic.py:296-301``) or the
-        seeded solenoidal field of ``dnsjax_amd.fields.periodic_field``."""
+    def init_state(self, seed: int | None = None, rms: float = 0.0,
+                   snapshot: str | None = None) -> torch.Tensor:
+        """Laminar start (zeros, ``flows/monochromatic.py:296-301``), a
+        snapshot's ``velocity_phys`` minus the base flow
+        (``flows/monochromatic.py:302-311``), or the seeded solenoidal field
+        of ``dnsjax_amd.fields.periodic_field``."""
         p = self.params
+        if snapshot is not None:
+            return load_snapshot(snapshot, p, self._native.device)
         if seed is None:
             return torch.zeros(self.spec_shape, dtype=torch.complex128, device=self._native.device)
         u = periodic_field(p.nx, p.ny, p.nz, p.lx, p.ly, p.lz, rms, seed)

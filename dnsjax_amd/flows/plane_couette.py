@@ -15,6 +15,7 @@ import torch
 
 from ..fields import wall_field
 from ..params import Params
+from ..snapshots import load_snapshot
 from .base import NativeFlow
 
 
@@ -24,8 +25,14 @@ class PlaneCouetteFlow(NativeFlow):
             raise ValueError("PlaneCouetteFlow needs a walled system")
         super().__init__(params)
 
-    def init_state(self, seed: int | None = None, rms: float = 0.0) -> torch.Tensor:
+    def init_state(self, seed: int | None = None, rms: float = 0.0,
+                   snapshot: str | None = None) -> torch.Tensor:
+        """Laminar start (zeros), a snapshot's ``velocity_phys_nonexpanded``
+        (``flows/plane_couette.py:225-248``), or the seeded no-slip field of
+        ``dnsjax_amd.fields.wall_field``."""
         p = self.params
+        if snapshot is not None:
+            return load_snapshot(snapshot, p, self._native.device)
         if seed is None:
             return torch.zeros(self.spec_shape, dtype=torch.complex128, device=self._native.device)
         return self.to_device(wall_field(p.nx, p.ny, p.nz, p.lx, p.lz, rms, seed))

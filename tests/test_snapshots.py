@@ -1,0 +1,159 @@
+"""Snapshot I/O in the reference's ``.npz`` formats (``init_state``,
+``flows/monochromatic.py:294-314``, ``flows/plane_couette.py:225-248``).
+
+Pinned by: the oracle's transforms (``oracle/periodic.py`` rfft3d/irfft3d,
+``oracle/walls.py`` rfft2d/irfft2d) and base flow, closed-form single
+harmonics (a physical sine lands on one stored mode with the forward
+normalisation's amplitude), and save -> load round trips.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from dnsjax_amd import snapshots as S
+from dnsjax_amd.fields import periodic_field, wall_field
+from dnsjax_amd.params import Params
+from oracle import periodic as OP
+from oracle import walls as OW
+
+PERIODIC_CASES = [
+    dict(system="kolmogorov", nx=16, ny=16, nz=16),
+    dict(system="waleffe", nx=16, ny=8, nz=32, oversample_y=False, tilt_degree=30.0),
+    dict(system="decaying-box", nx=8, ny=12, nz=8, oversampling_factor=4),
+]
+
+
+def _pair(case):
+    p = Params(lx=2 * math.pi, lz=math.pi, **case)
+    op = OP.PeriodicParams(nx=p.nx, ny=p.ny, nz=p.nz, lx=p.lx, ly=p.ly, lz=p.lz,
+                           system=p.system, oversampling_factor=p.oversampling_factor,
+                           oversample_y=p.oversample_y, tilt_degree=p.tilt_degree)
+    return p, op
+
+
+@pytest.mark.parametrize("case", PERIODIC_CASES)
+def test_periodic_transforms_match_oracle(case):
+    p, op = _pair(case)
+    rng = np.random.default_rng(3)
+    phys = rng.standard_normal((3, *op.phys_shape))
+    spec = S.phys_to_spec(torch.from_numpy(phys), p).numpy()
+    ref = OP.phys_to_spec(phys, op)
+    assert spec.shape == (3, *p.spec_shape)
+    np.testing.assert_allclose(spec, ref, rtol=0, atol=1e-14)
+    back = S.spec_to_phys(torch.from_numpy(ref), p).numpy()
+    np.testing.assert_allclose(back, OP.spec_to_phys(ref, op), rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("case", PERIODIC_CASES)
+def test_base_flow_matches_oracle(case):
+    p, op = _pair(case)
+    np.testing.assert_allclose(S.base_flow(p).numpy(), OP.Flow(op).base_flow,
+                               rtol=0, atol=1e-15)
+
+
+def test_single_harmonics_land_on_their_modes():
+    """u_x = sin(3 * 2 pi y / Ly) + cos(2 * 2 pi z / Lz) + cos(1 * 2 pi x / Lx):
+    -0.5i / +0.5i at ky = +-3, 0.5 at kz = +-2 and 0.5 at kx = 1 (the
+    forward normalisation divides by the grid size)."""
+    p = Params(system="decaying-box", nx=8, ny=12, nz=8)
+    py, pz, px = p.padded
+    y = np.arange(py)[:, None, None] / py
+    z = np.arange(pz)[None, :, None] / pz
+    x = np.arange(px)[None, None, :] / px
+    phys = np.zeros((3, py, pz, px))
+    phys[0] = np.sin(2 * np.pi * 3 * y) + np.cos(2 * np.pi * 2 * z) + np.cos(2 * np.pi * x)
+    spec = S.phys_to_spec(torch.from_numpy(phys), p).numpy()
+    want = np.zeros((3, *p.spec_shape), dtype=complex)
+    want[0, 3, 0, 0] = -0.5j
+    want[0, p.ny - 1 - 3, 0, 0] = 0.5j
+    want[0, 0, 2, 0] = 0.5
+    want[0, 0, p.nz - 1 - 2, 0] = 0.5
+    want[0, 0, 0, 1] = 0.5
+    np.testing.assert_allclose(spec, want, rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("case", PERIODIC_CASES)
+def test_periodic_save_load_round_trip(case, tmp_path):
+    p, op = _pair(case)
+    state = periodic_field(p.nx, p.ny, p.nz, p.lx, p.ly, p.lz, rms=0.3, seed=5)
+    path = str(tmp_path / "snap.npz")
+    S.save_snapshot(path, torch.from_numpy(state), p)
+    with np.load(path) as f:
+        assert list(f.keys()) == ["velocity_phys"]
+        total = f["velocity_phys"]
+    # The file holds the total velocity on the padded grid.
+    want = OP.spec_to_phys(state, op) + OP.Flow(op).base_flow
+    np.testing.assert_allclose(total, want, rtol=0, atol=1e-13)
+    back = S.init_state(p, snapshot=path).numpy()
+    np.testing.assert_allclose(back, state, rtol=0, atol=1e-14)
+
+
+def test_laminar_state_and_missing_snapshot(tmp_path):
+    p = Params(system="kolmogorov", nx=8, ny=8, nz=8)
+    zero = S.init_state(p, start_from_laminar=True)
+    assert zero.shape == (3, *p.spec_shape) and not zero.abs().any()
+    path = str(tmp_path / "lam.npz")
+    S.save_snapshot(path, zero, p)
+    with np.load(path) as f:
+        np.testing.assert_allclose(f["velocity_phys"],
+                                   np.broadcast_to(S.base_flow(p).numpy(), (3, *p.padded)),
+                                   rtol=0, atol=0)
+    assert not S.init_state(p, snapshot=path).abs().max() > 1e-16
+    with pytest.raises(ValueError, match="initial condition"):
+        S.init_state(p)
+
+
+def test_float32_snapshot_is_read_in_fp64(tmp_path):
+    """The reference casts the stored array to its float type on load."""
+    p = Params(system="kolmogorov", nx=8, ny=8, nz=8)
+    rng = np.random.default_rng(1)
+    phys = rng.standard_normal((3, *p.padded)).astype(np.float32)
+    path = str(tmp_path / "f32.npz")
+    np.savez(path, velocity_phys=phys)
+    got = S.init_state(p, snapshot=path)
+    assert got.dtype == torch.complex128
+    want = S.phys_to_spec(torch.from_numpy(phys.astype(np.float64)) - S.base_flow(p), p)
+    assert torch.equal(got, want)
+
+
+WALL_CASES = [dict(nx=8, ny=17, nz=8), dict(nx=16, ny=33, nz=8, oversampling_factor=4)]
+
+
+def _wall_pair(case):
+    p = Params(system="plane-couette", lx=2 * math.pi, lz=math.pi, **case)
+    op = OW.WallParams(nx=p.nx, ny=p.ny, nz=p.nz, lx=p.lx, lz=p.lz,
+                       oversampling_factor=p.oversampling_factor)
+    return p, op
+
+
+@pytest.mark.parametrize("case", WALL_CASES)
+def test_wall_transforms_match_oracle(case):
+    p, op = _wall_pair(case)
+    rng = np.random.default_rng(4)
+    phys = rng.standard_normal((3, *op.phys_shape))
+    spec = S.phys_to_spec_2d(torch.from_numpy(phys), p).numpy()
+    ref = OW.phys_to_spec_2d(phys, op)
+    assert spec.shape == (3, *p.spec_shape)
+    np.testing.assert_allclose(spec, ref, rtol=0, atol=1e-14)
+    back = S.spec_to_phys_2d(torch.from_numpy(ref), p).numpy()
+    np.testing.assert_allclose(back, OW.spec_to_phys_2d(ref, op), rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("case", WALL_CASES)
+def test_wall_save_load_round_trip(case, tmp_path):
+    p, op = _wall_pair(case)
+    state = wall_field(p.nx, p.ny, p.nz, p.lx, p.lz, rms=0.2, seed=9)
+    path = str(tmp_path / "walls.npz")
+    S.save_snapshot(path, torch.from_numpy(state), p)
+    with np.load(path) as f:
+        assert list(f.keys()) == ["velocity_phys_nonexpanded"]
+        phys = f["velocity_phys_nonexpanded"]
+    assert phys.shape == (3, *op.phys_shape)
+    np.testing.assert_allclose(phys, OW.spec_to_phys_2d(state, op), rtol=0, atol=1e-13)
+    back = S.init_state(p, snapshot=path).numpy()
+    np.testing.assert_allclose(back, state, rtol=0, atol=1e-14)
