@@ -53,7 +53,7 @@ def phys_to_spec(phys: torch.Tensor, p: Params) -> torch.Tensor:
     """(3, P_y, P_z, P_x) real -> (3, ny-1, nz-1, nx/2) complex (``_rfft3d``)."""
     a = torch.fft.rfft(phys, dim=-1, norm="forward")[..., : p.nx // 2]
     a = _truncate(torch.fft.fft(a, dim=-2, norm="forward"), p.nz, -2)
-    return _truncate(torch.fft.fft(a, dim=-3, norm="forward"), p.ny, -3)
+    return _truncate(torch.fft.fft(a, dim=-3, norm="forward"), p.ny, -3).contiguous()
 
 
 def spec_to_phys(spec: torch.Tensor, p: Params) -> torch.Tensor:
@@ -63,7 +63,7 @@ def spec_to_phys(spec: torch.Tensor, p: Params) -> torch.Tensor:
     a = torch.fft.ifft(_zeropad(a, pz, -2), dim=-2, norm="forward")
     full = torch.zeros(a.shape[:-1] + (px // 2 + 1,), dtype=a.dtype, device=a.device)
     full[..., : p.nx // 2] = a
-    return torch.fft.irfft(full, n=px, dim=-1, norm="forward")
+    return torch.fft.irfft(full, n=px, dim=-1, norm="forward").contiguous()
 
 
 def phys_to_spec_2d(phys: torch.Tensor, p: Params) -> torch.Tensor:
@@ -80,7 +80,7 @@ def spec_to_phys_2d(spec: torch.Tensor, p: Params) -> torch.Tensor:
     a = torch.fft.ifft(_zeropad(a, pz, -2), dim=-2, norm="forward")
     full = torch.zeros(a.shape[:-1] + (px // 2 + 1,), dtype=a.dtype, device=a.device)
     full[..., : p.nx // 2] = a
-    return torch.fft.irfft(full, n=px, dim=-1, norm="forward")
+    return torch.fft.irfft(full, n=px, dim=-1, norm="forward").contiguous()
 
 
 def base_flow(p: Params, device=None) -> torch.Tensor:
@@ -103,7 +103,8 @@ def base_flow(p: Params, device=None) -> torch.Tensor:
 
 
 def load_snapshot(path: str, p: Params, device=None) -> torch.Tensor:
-    """The spectral state (complex128, the reference layout) of a snapshot."""
+    """The spectral state (contiguous complex128, the reference layout) of a
+    snapshot."""
     with np.load(path, allow_pickle=False) as f:
         key = "velocity_phys_nonexpanded" if p.walls else "velocity_phys"
         phys = torch.from_numpy(np.asarray(f[key], dtype=np.float64)).to(device)

@@ -89,8 +89,9 @@ def test_periodic_save_load_round_trip(case, tmp_path):
     # The file holds the total velocity on the padded grid.
     want = OP.spec_to_phys(state, op) + OP.Flow(op).base_flow
     np.testing.assert_allclose(total, want, rtol=0, atol=1e-13)
-    back = S.init_state(p, snapshot=path).numpy()
-    np.testing.assert_allclose(back, state, rtol=0, atol=1e-14)
+    back = S.init_state(p, snapshot=path)
+    assert back.is_contiguous()
+    np.testing.assert_allclose(back.numpy(), state, rtol=0, atol=1e-14)
 
 
 def test_laminar_state_and_missing_snapshot(tmp_path):
@@ -155,5 +156,6 @@ def test_wall_save_load_round_trip(case, tmp_path):
         phys = f["velocity_phys_nonexpanded"]
     assert phys.shape == (3, *op.phys_shape)
     np.testing.assert_allclose(phys, OW.spec_to_phys_2d(state, op), rtol=0, atol=1e-13)
-    back = S.init_state(p, snapshot=path).numpy()
-    np.testing.assert_allclose(back, state, rtol=0, atol=1e-14)
+    back = S.init_state(p, snapshot=path)
+    assert back.is_contiguous()
+    np.testing.assert_allclose(back.numpy(), state, rtol=0, atol=1e-14)
