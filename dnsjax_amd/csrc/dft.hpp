@@ -236,6 +236,22 @@ __device__ __forceinline__ int opaque(int v) {
   return v;
 }
 
+// Ordering point for an LDS exchange among the lanes of one row.  When every
+// row lies inside one wavefront (WAVE), the exchange needs no hardware
+// barrier: LDS operations of a single wavefront complete in issue order, so a
+// wavefront-scope fence (no instructions; it only stops the compiler moving
+// LDS accesses across it) suffices, and the waves of a workgroup run
+// unsynchronised.  Rows that straddle wavefronts need the workgroup barrier.
+template <bool WAVE>
+__device__ __forceinline__ void row_sync() {
+  if constexpr (WAVE) {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  } else {
+    __syncthreads();
+  }
+}
+
 // Padded-axis index maps with the mode count NM known at compile time: NM
 // modes stored as NM - 1 (Nyquist dropped) at padded indices [0, NM/2) and
 // (P - NM/2, P).  The 3/2 rule gives NM = 2P/3 (Pad<P>); an axis without

@@ -347,10 +347,13 @@ __global__ void __launch_bounds__(R * Plan<P>::N1, 2)
     k_x_nonlinear(const cx<T>* __restrict__ w2, cx<T>* __restrict__ w3,
                   const cx<T>* __restrict__ tw, const double* __restrict__ kx_tab,
                   const double* __restrict__ base, const double* __restrict__ cbase,
-                  const double* __restrict__ nbase, int nx, int Py, int Pz, int ybase_stride, const int* __restrict__ act) {
+                  const double* __restrict__ /* nbase: U x curl U, implied */, int nx, int Py,
+                  int Pz, int ybase_stride, const int* __restrict__ act) {
   if (act && *act == 0) return;
   constexpr int N1 = Plan<P>::N1, N2 = Plan<P>::N2;
   constexpr int ROW = XRow<N1, N2>::value;
+  // 64 % N1 == 0: each row's N1 lanes sit in one wavefront
+  constexpr bool kWaveRows = 64 % N1 == 0;
   __shared__ cx<T> lds[R * ROW];
   __shared__ cx<T> tab_i[N1 * N2];  // inverse plan (N1, N2)
   __shared__ cx<T> tab_f[N2 * N1];  // forward plan (N2, N1)
@@ -369,35 +372,23 @@ __global__ void __launch_bounds__(R * Plan<P>::N1, 2)
   cx<T>* rl = lds + r * ROW;
   const T inv = static_cast<T>(1.0 / P);
 
-  // Inverse transform of one (u, w) pair into physical values re/im.
+  // Inverse transform of one (u, w) pair into physical values re/im.  The
+  // row's 16 lanes first load the pair's modes coalesced (kx = j + N1 i) and
+  // stage the combined input z_m = X_m + i Y_m (X_{P-k} = conj X_k) in the
+  // row's LDS; the step-A lanes read theirs from there.  This keeps a lane's
+  // loads in flight to 2 Nx1 / N1 values instead of 2 N1.  Step A's output
+  // reuses the buffer: within a wavefront its reads complete before its
+  // writes, and rows that span wavefronts synchronise in between.
   auto inverse_pair = [&](int pair, T* re, T* im) {
-    const int j = opaque(j_);
-    if (j < N2) {
-      cx<T> z[N1];
+    {
+      const int j = opaque(j_);
+      constexpr int PER = (Nx1 + N1 - 1) / N1;
 #pragma unroll
-      for (int a = 0; a < N1; ++a) {
-        // padded index m = N2 a + j: the lane range's position relative to
-        // the zero band is fixed at compile time per (unrolled) a
-        const int lo = N2 * a, hi = N2 * a + N2 - 1;
-        const bool none = Pad<P>::zero(lo, hi), pos = hi < Nx1, neg = lo > P - Nx1;
-        const int m = lo + j;
-        int k = -1;
-        bool cj = false;
-        if (none) {
-        } else if (pos) {
-          k = m;
-        } else if (neg) {
-          k = P - m;
-          cj = true;
-        } else if (m < Nx1) {
-          k = m;
-        } else if (m > P - Nx1) {
-          k = P - m;
-          cj = true;
-        }
-        cx<T> X{0, 0}, Y{0, 0};
-        if (ok && k >= 0) {
+      for (int i = 0; i < PER; ++i) {
+        const int k = j + N1 * i;
+        if (Nx1 % N1 == 0 || k < Nx1) {
           const T kk = static_cast<T>(kx_tab[k]);
+          cx<T> X, Y;
           if (pair == 0) {  // (u_z, w_y = Z(i kz Yu_x) - i kx u_z)
             X = in[2 * fstride + k];
             const cx<T> g = in[4 * fstride + k];
@@ -414,21 +405,44 @@ __global__ void __launch_bounds__(R * Plan<P>::N1, 2)
             X.y = 0;
             Y.y = 0;
           }
+          rl[k] = {X.x - Y.y, X.y + Y.x};
+          if (k != 0) rl[P - k] = {X.x + Y.y, Y.x - X.y};
         }
-        // Z_m = X_m + i Y_m with X_{P-k} = conj(X_k)
-        z[a] = cj ? cx<T>{X.x + Y.y, Y.x - X.y} : cx<T>{X.x - Y.y, X.y + Y.x};
       }
-      row_step_a_store_tab<P, N1, N2, +1>(z, j, rl, tab_i);
     }
-    __syncthreads();
+    row_sync<kWaveRows>();
+    const int j = opaque(j_);
+    cx<T> z[N1];
+    if (j < N2) {
+#pragma unroll
+      for (int a = 0; a < N1; ++a) {
+        // padded index m = N2 a + j: the lane range's position relative to
+        // the zero band is fixed at compile time per (unrolled) a
+        const int lo = N2 * a, hi = N2 * a + N2 - 1;
+        const bool none = Pad<P>::zero(lo, hi), pos = hi < Nx1, neg = lo > P - Nx1;
+        const int m = lo + j;
+        const bool live = !none && (pos || neg || m < Nx1 || m > P - Nx1);
+        z[a] = live ? rl[m] : cx<T>{0, 0};
+      }
+    }
+    if constexpr (!kWaveRows) __syncthreads();
+    if (j < N2) row_step_a_store_tab<P, N1, N2, +1>(z, j, rl, tab_i);
+    row_sync<kWaveRows>();
     cx<T> o[N2];
     row_step_b_load<P, N1, N2, +1>(o, j, rl);
+    // total velocity and vorticity: u + U and w + curl U, component
+    // (2, 1), (1, 2), (0, 0) for pairs 0, 1, 2
+    const int yr = opaque(yrow);
+    const int cu = pair == 0 ? 2 : (pair == 1 ? 1 : 0);
+    const int cw = pair == 0 ? 1 : (pair == 1 ? 2 : 0);
+    const T Uc = static_cast<T>(base[cu * ybase_stride + yr]);
+    const T Cc = static_cast<T>(cbase[cw * ybase_stride + yr]);
 #pragma unroll
     for (int k2 = 0; k2 < N2; ++k2) {
-      re[k2] = o[k2].x;
-      im[k2] = o[k2].y;
+      re[k2] = o[k2].x + Uc;
+      im[k2] = o[k2].y + Cc;
     }
-    __syncthreads();
+    row_sync<kWaveRows>();
   };
 
   // Forward transform of one real component, truncated to kx < nx/2.
@@ -440,7 +454,7 @@ __global__ void __launch_bounds__(R * Plan<P>::N1, 2)
       for (int a = 0; a < N2; ++a) v[a] = {v_re[a], 0};
       row_step_a_store_tab<P, N2, N1, -1>(v, j, rl, tab_f);
     }
-    __syncthreads();
+    row_sync<kWaveRows>();
     if (j < N2 && ok) {
       cx<T> o[N1];
       row_step_b_load<P, N2, N1, -1>(o, j, rl);
@@ -451,20 +465,15 @@ __global__ void __launch_bounds__(R * Plan<P>::N1, 2)
         if (N2 * k2 < Nx1 && k < Nx1) out[c * fstride + k] = cscale(o[k2], inv);
       }
     }
-    __syncthreads();
+    row_sync<kWaveRows>();
   };
 
-  // u' x w' + (u' x curl U + U x w') + U x curl U   (rhs.py:80-88)
-  const T U0 = static_cast<T>(base[0 * ybase_stride + yrow]);
-  const T U1 = static_cast<T>(base[1 * ybase_stride + yrow]);
-  const T U2 = static_cast<T>(base[2 * ybase_stride + yrow]);
-  const T C0 = static_cast<T>(cbase[0 * ybase_stride + yrow]);
-  const T C1 = static_cast<T>(cbase[1 * ybase_stride + yrow]);
-  const T C2 = static_cast<T>(cbase[2 * ybase_stride + yrow]);
-  const T B0 = static_cast<T>(nbase[0 * ybase_stride + yrow]);
-  const T B1 = static_cast<T>(nbase[1 * ybase_stride + yrow]);
-  const T B2 = static_cast<T>(nbase[2 * ybase_stride + yrow]);
-
+  // u' x w' + (u' x curl U + U x w') + U x curl U   (rhs.py:80-88) is
+  // formed as (u' + U) x (w' + curl U): the base-flow tables satisfy
+  // nbase = U x curl U (the tilt rotates U and curl U together and leaves
+  // (0, U dU/dy, 0) unchanged; plane Couette: (y,0,0) x (0,0,-1) = (0,y,0)),
+  // so the product is the same sum of four terms with no per-row constants
+  // held across the passes.
   T uz[N2], wy[N2], uy[N2], wz[N2];
   inverse_pair(0, uz, wy);
   inverse_pair(1, uy, wz);
@@ -472,8 +481,7 @@ __global__ void __launch_bounds__(R * Plan<P>::N1, 2)
     T nl0[N2];
 #pragma unroll
     for (int k2 = 0; k2 < N2; ++k2) {
-      const T a1 = uy[k2], a2 = uz[k2], b1 = wy[k2], b2 = wz[k2];
-      nl0[k2] = (a1 * b2 - a2 * b1) + (((a1 * C2 - a2 * C1) + (U1 * b2 - U2 * b1)) + B0);
+      nl0[k2] = uy[k2] * wz[k2] - uz[k2] * wy[k2];
     }
     forward_store(0, nl0);
   }
@@ -489,18 +497,16 @@ __global__ void __launch_bounds__(R * Plan<P>::N1, 2)
       inverse_pair(2, ux, wx);
 #pragma unroll
       for (int k2 = 0; k2 < N2; ++k2) {
-        const T a0 = ux[k2], a1 = uy[k2], a2 = uz[k2];
-        const T b0 = wx[k2], b1 = wy[k2], b2 = wz[k2];
-        v[k2].x = (a2 * b0 - a0 * b2) + (((a2 * C0 - a0 * C2) + (U2 * b0 - U0 * b2)) + B1);
-        v[k2].y = (a0 * b1 - a1 * b0) + (((a0 * C1 - a1 * C0) + (U0 * b1 - U1 * b0)) + B2);
+        v[k2].x = uz[k2] * wx[k2] - ux[k2] * wz[k2];
+        v[k2].y = ux[k2] * wy[k2] - uy[k2] * wx[k2];
       }
     }
     const int j = opaque(j_);
     row_step_a_store_tab<P, N2, N1, -1>(v, j, rl, tab_f);
-    __syncthreads();
+    row_sync<kWaveRows>();
     cx<T> o[N1];
     if (j < N2) row_step_b_load<P, N2, N1, -1>(o, j, rl);
-    __syncthreads();
+    row_sync<kWaveRows>();
     // publish Z[k] for the partners' indices: k = j + N2 k2 with k2 >= KH
     constexpr int KH = N1 - 1 - (Nx1 - 1) / N2;
     if (j < N2) {
@@ -508,7 +514,7 @@ __global__ void __launch_bounds__(R * Plan<P>::N1, 2)
       for (int k2 = KH; k2 < N1; ++k2) rl[j + N2 * k2] = o[k2];
       if (j == 0) rl[0] = o[0];
     }
-    __syncthreads();
+    row_sync<kWaveRows>();
     if (j < N2 && ok) {
 #pragma unroll
       for (int k2 = 0; k2 < N1; ++k2) {
