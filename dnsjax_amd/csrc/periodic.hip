@@ -71,7 +71,7 @@ __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
   // pass 0: u_x -> F0 = Y(u_x), F4 = Y(i ky u_x); pass 1: u_z -> F2, F3;
   // pass 2: u_y -> F1.
 #pragma unroll 1
-  for (int pass = 0; pass < 3; ++pass) {
+  for (int pass = blockIdx.y; pass < 3; pass += gridDim.y) {
     const int j = opaque(j_);
     const int c = pass == 0 ? 0 : (pass == 1 ? 2 : 1);
     const int fa = c;
@@ -139,7 +139,7 @@ __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
   const size_t out_off = (size_t)y * P * Nx1 + kx;
   // outputs: 0 u_x, 1 u_y, 2 u_z, 3 w_x, 4 Z(i kz Yu_x), 5 Z(-Y(i ky u_x))
 #pragma unroll 1
-  for (int pass = 0; pass < 4; ++pass) {
+  for (int pass = blockIdx.y; pass < 4; pass += gridDim.y) {
     const int j = opaque(j_);
     cx<T> xa[N1], xb[N1];
     if (j < N2) {
@@ -222,7 +222,7 @@ __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
   const size_t off = (size_t)kz * Nx1 + kx;
   // pass -> (input component, multiply by i ky, output field)
 #pragma unroll 1
-  for (int pass = 0; pass < 5; ++pass) {
+  for (int pass = blockIdx.y; pass < 5; pass += gridDim.y) {
     const int j = opaque(j_);
     const int c = pass < 2 ? 0 : (pass < 4 ? 2 : 1);
     const bool iky = (pass == 1 || pass == 3);
@@ -281,7 +281,7 @@ __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
   // pass: 0 Z(F0)->0, 1 Z(i kz F0)->4, 2 Z(F1)->1, 3 Z(F3 - i kz F1)->3,
   //       4 Z(F2)->2, 5 Z(-F4)->5
 #pragma unroll 1
-  for (int pass = 0; pass < 6; ++pass) {
+  for (int pass = blockIdx.y; pass < 6; pass += gridDim.y) {
     const int j = opaque(j_);
     const int fo = pass == 0 ? 0 : (pass == 1 ? 4 : (pass == 2 ? 1 : (pass == 3 ? 3 : (pass == 4 ? 2 : 5))));
     if (j < N2) {
@@ -709,7 +709,7 @@ __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
   const size_t out_off = (size_t)y * Nz1 * Nx1 + kx;
   const T inv = static_cast<T>(1.0 / P);
 #pragma unroll 1
-  for (int c = 0; c < 3; ++c) {
+  for (int c = blockIdx.y; c < 3; c += gridDim.y) {
     const int j = opaque(j_);
     if (j < N2) {
       cx<T> x[N1];
@@ -956,7 +956,9 @@ template <typename T>
 static int launch_y_inverse(Ctx* c, const double* state, hipStream_t s) {
   Bufs<T> b = bufs_of<T>(c);
   const int ntile = (c->Nx1 + kTC - 1) / kTC;
-  const dim3 grid(c->Nz1 * ntile);
+  // one pass per workgroup (grid.y): the passes read different inputs and
+  // write different outputs, and Nz1 * ntile workgroups alone underfill the GPU
+  const dim3 grid(c->Nz1 * ntile, c->yinv_seq ? 5 : 3);
   const cx<double>* u = reinterpret_cast<const cx<double>*>(state);
   const bool pady = c->Py != c->ny;  // y padded by the 3/2 rule
   ProfScope ps(c, KID_Y_INVERSE, s);
@@ -987,7 +989,7 @@ template <typename T>
 static int launch_z_inverse(Ctx* c, hipStream_t s) {
   Bufs<T> b = bufs_of<T>(c);
   const int ntile = (c->Nx1 + kTC - 1) / kTC;
-  const dim3 grid(c->Py * ntile);
+  const dim3 grid(c->Py * ntile, c->zinv_seq ? 6 : 4);  // one pass per workgroup
   ProfScope ps(c, KID_Z_INVERSE, s);
   switch (c->Pz) {
 #define X(P)                                                                                    \
@@ -1070,7 +1072,7 @@ template <typename T>
 int launch_z_forward_buf(Ctx* c, const cx<T>* in, cx<T>* out, int ny, int kid, hipStream_t s) {
   Bufs<T> b = bufs_of<T>(c);
   const int ntile = (c->Nx1 + kTC - 1) / kTC;
-  const dim3 grid(ny * ntile);
+  const dim3 grid(ny * ntile, 3);  // one component per workgroup
   ProfScope ps(c, kid, s);
   switch (c->Pz) {
 #define X(P)                                                                                   \
