@@ -372,44 +372,58 @@ __global__ void __launch_bounds__(R * Plan<P>::N1, 2)
   cx<T>* rl = lds + r * ROW;
   const T inv = static_cast<T>(1.0 / P);
 
-  // Inverse transform of one (u, w) pair into physical values re/im.  The
-  // row's 16 lanes first load the pair's modes coalesced (kx = j + N1 i) and
-  // stage the combined input z_m = X_m + i Y_m (X_{P-k} = conj X_k) in the
-  // row's LDS; the step-A lanes read theirs from there.  This keeps a lane's
-  // loads in flight to 2 Nx1 / N1 values instead of 2 N1.  Step A's output
-  // reuses the buffer: within a wavefront its reads complete before its
-  // writes, and rows that span wavefronts synchronise in between.
-  auto inverse_pair = [&](int pair, T* re, T* im) {
-    {
-      const int j = opaque(j_);
-      constexpr int PER = (Nx1 + N1 - 1) / N1;
+  // Inverse transform of one (u, w) pair into physical values re/im, in
+  // three parts so a pair's loads can be issued while the previous pair is
+  // transformed.  load_pair: the row's 16 lanes load the pair's modes
+  // coalesced (kx = j + N1 i), 2 Nx1 / N1 values per lane.  stage_pair: the
+  // combined input z_m = X_m + i Y_m (X_{P-k} = conj X_k) goes to the row's
+  // LDS.  inverse_pair: the step-A lanes read theirs from there.  Step A's
+  // output reuses the buffer: within a wavefront its reads complete before
+  // its writes, and rows that span wavefronts synchronise in between.
+  constexpr int PER = (Nx1 + N1 - 1) / N1;
+  struct PairIn {
+    cx<T> X[PER], G[PER];
+    T kk[PER];
+  };
+  auto load_pair = [&](int pair, PairIn& q) {
+    const int j = opaque(j_);
+    const int fx = pair == 0 ? 2 : (pair == 1 ? 1 : 0);
+    const int fg = pair == 0 ? 4 : (pair == 1 ? 5 : 3);
 #pragma unroll
-      for (int i = 0; i < PER; ++i) {
-        const int k = j + N1 * i;
-        if (Nx1 % N1 == 0 || k < Nx1) {
-          const T kk = static_cast<T>(kx_tab[k]);
-          cx<T> X, Y;
-          if (pair == 0) {  // (u_z, w_y = Z(i kz Yu_x) - i kx u_z)
-            X = in[2 * fstride + k];
-            const cx<T> g = in[4 * fstride + k];
-            Y = {g.x + kk * X.y, g.y - kk * X.x};
-          } else if (pair == 1) {  // (u_y, w_z = Z(-Y(i ky u_x)) + i kx u_y)
-            X = in[1 * fstride + k];
-            const cx<T> g = in[5 * fstride + k];
-            Y = {g.x - kk * X.y, g.y + kk * X.x};
-          } else {  // (u_x, w_x)
-            X = in[0 * fstride + k];
-            Y = in[3 * fstride + k];
-          }
-          if (k == 0) {  // irfft ignores the imaginary part of the DC bin
-            X.y = 0;
-            Y.y = 0;
-          }
-          rl[k] = {X.x - Y.y, X.y + Y.x};
-          if (k != 0) rl[P - k] = {X.x + Y.y, Y.x - X.y};
-        }
+    for (int i = 0; i < PER; ++i) {
+      const int k = j + N1 * i;
+      if (Nx1 % N1 == 0 || k < Nx1) {
+        q.kk[i] = static_cast<T>(kx_tab[k]);
+        q.X[i] = in[fx * fstride + k];
+        q.G[i] = in[fg * fstride + k];
       }
     }
+  };
+  auto stage_pair = [&](int pair, const PairIn& q) {
+    const int j = opaque(j_);
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int k = j + N1 * i;
+      if (Nx1 % N1 == 0 || k < Nx1) {
+        const T kk = q.kk[i];
+        cx<T> X = q.X[i], Y;
+        const cx<T> g = q.G[i];
+        if (pair == 0)  // (u_z, w_y = Z(i kz Yu_x) - i kx u_z)
+          Y = {g.x + kk * X.y, g.y - kk * X.x};
+        else if (pair == 1)  // (u_y, w_z = Z(-Y(i ky u_x)) + i kx u_y)
+          Y = {g.x - kk * X.y, g.y + kk * X.x};
+        else  // (u_x, w_x)
+          Y = g;
+        if (k == 0) {  // irfft ignores the imaginary part of the DC bin
+          X.y = 0;
+          Y.y = 0;
+        }
+        rl[k] = {X.x - Y.y, X.y + Y.x};
+        if (k != 0) rl[P - k] = {X.x + Y.y, Y.x - X.y};
+      }
+    }
+  };
+  auto inverse_pair = [&](int pair, T* re, T* im) {
     row_sync<kWaveRows>();
     const int j = opaque(j_);
     cx<T> z[N1];
@@ -475,7 +489,13 @@ __global__ void __launch_bounds__(R * Plan<P>::N1, 2)
   // so the product is the same sum of four terms with no per-row constants
   // held across the passes.
   T uz[N2], wy[N2], uy[N2], wz[N2];
+  PairIn q;
+  load_pair(0, q);
+  stage_pair(0, q);
+  load_pair(1, q);
   inverse_pair(0, uz, wy);
+  stage_pair(1, q);
+  load_pair(2, q);
   inverse_pair(1, uy, wz);
   {
     T nl0[N2];
@@ -485,6 +505,7 @@ __global__ void __launch_bounds__(R * Plan<P>::N1, 2)
     }
     forward_store(0, nl0);
   }
+  stage_pair(2, q);
   // Components 1 and 2 are real: one complex transform of nl1 + i nl2,
   // separated by Hermitian symmetry, X1[k] = (Z[k] + conj Z[P-k]) / 2 and
   // X2[k] = (Z[k] - conj Z[P-k]) / 2i.  Z[P-k] for k < Nx1 lives in another
