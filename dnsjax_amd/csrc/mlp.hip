@@ -35,6 +35,11 @@ __host__ __device__ constexpr int mlp_nparams(int H) {
 }
 
 __device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+// Inference form: the hardware reciprocal (1 ulp) instead of an IEEE divide;
+// its error is far below the bf16 rounding of the activations that follows.
+__device__ __forceinline__ float silu_fast(float z) {
+  return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z));
+}
 __device__ __forceinline__ float dsilu(float z) {
   const float s = 1.0f / (1.0f + __expf(-z));
   return s * (1.0f + z * (1.0f - s));
@@ -269,7 +274,7 @@ __device__ __forceinline__ v4f mlp_wave(v4s xb, const v4s* __restrict__ wt,
     acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wt[o * 64 + lane], xb, acc, 0, 0, 0);
     float a[4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) a[j] = silu(acc[j] + bias[o * 16 + 4 * g + j]);
+    for (int j = 0; j < 4; ++j) a[j] = silu_fast(acc[j] + bias[o * 16 + 4 * g + j]);
     h1[o] = pack_b(a[0], a[1], a[2], a[3]);
   }
   v4s h2[NH];
@@ -282,7 +287,7 @@ __device__ __forceinline__ v4f mlp_wave(v4s xb, const v4s* __restrict__ wt,
                                                       0, 0, 0);
     float a[4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) a[j] = silu(acc[j] + bias[H + o * 16 + 4 * g + j]);
+    for (int j = 0; j < 4; ++j) a[j] = silu_fast(acc[j] + bias[H + o * 16 + 4 * g + j]);
     h2[o] = pack_b(a[0], a[1], a[2], a[3]);
   }
   v4f acc = {0.f, 0.f, 0.f, 0.f};
@@ -350,6 +355,18 @@ __device__ __forceinline__ void mode_k(const FeatArgs& a, long long i, double& k
   const long long t = i / a.Nx1;
   const int iz = (int)(t % a.Nz1);
   const int iy = (int)(t / a.Nz1);
+  kx = a.kx[ix];
+  ky = a.ky[iy];
+  kz = a.kz[iz];
+}
+
+// The same with 32-bit index arithmetic (the per-component mode count of
+// every supported grid is far below 2^31; the launcher checks it).
+__device__ __forceinline__ void mode_k32(const FeatArgs& a, unsigned i, double& kx, double& ky,
+                                         double& kz) {
+  const unsigned nx1 = (unsigned)a.Nx1, nz1 = (unsigned)a.Nz1;
+  const unsigned t = i / nx1, ix = i - t * nx1;
+  const unsigned iy = t / nz1, iz = t - iy * nz1;
   kx = a.kx[ix];
   ky = a.ky[iy];
   kz = a.kz[iz];
@@ -458,7 +475,7 @@ __global__ void __launch_bounds__(256)
   const bool live = i < a.n;
   const long long ii = live ? i : 0;
   double kx, ky, kz;
-  mode_k(a, ii, kx, ky, kz);
+  mode_k32(a, (unsigned)ii, kx, ky, kz);
   cx<double> nc[3], d1[3], d2[3];
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
@@ -477,9 +494,13 @@ __global__ void __launch_bounds__(256)
       feats[wave][lane][6 + 2 * q] = (float)d2[q].x * a.s_2;
       feats[wave][lane][6 + 2 * q + 1] = (float)d2[q].y * a.s_2;
     }
-    for (int f = 12; f < kDin; ++f) feats[wave][lane][f] = k_feature(a, ii, f);
+    // k features as k_feature forms them, from this mode's wavenumbers
+    feats[wave][lane][12] = (float)fmin(a.dt * ((kx * kx + ky * ky) + kz * kz) / a.re, 8.0);
+    feats[wave][lane][13] = (float)(kx / a.kmax);
+    feats[wave][lane][14] = (float)(ky / a.kmax);
+    feats[wave][lane][15] = (float)(kz / a.kmax);
   }
-  __syncthreads();
+  __syncthreads();  // the weights are loaded by the whole workgroup
   if (use_net) {
     const int g = lane / 16, ml = lane % 16;
 #pragma unroll
@@ -491,11 +512,15 @@ __global__ void __launch_bounds__(256)
         if (4 * g + j < kDout) outs[wave][sb * 16 + ml][4 * g + j] = y[j];
     }
   }
-  __syncthreads();
+  // outs[wave] is written and read by this wavefront only, whose LDS
+  // operations complete in issue order
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
   if (!live) return;
   const double lapl = -((kx * kx + ky * ky) + kz * kz);
   const bool mean = (i == 0);
   const double w = mode_weight(a, lapl, mean);
+  const double inv_w = mean ? 0.0 : 1.0 - a.dt * a.c * lapl / a.re;  // 1 / w
   cx<double> dn[3];
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
@@ -506,8 +531,8 @@ __global__ void __launch_bounds__(256)
     else
       dn[q] = {d1[q].x + d2[q].x, d1[q].y + d2[q].y};
     if (use_net && !mean) {
-      dn[q].x += (double)a.s_out * outs[wave][lane][2 * q] / w;
-      dn[q].y += (double)a.s_out * outs[wave][lane][2 * q + 1] / w;
+      dn[q].x += (double)a.s_out * outs[wave][lane][2 * q] * inv_w;
+      dn[q].y += (double)a.s_out * outs[wave][lane][2 * q + 1] * inv_w;
     }
   }
   const double il = lapl < 0 ? 1.0 / lapl : 0.0;
@@ -640,6 +665,7 @@ int guess_periodic(Ctx* c, const double* state, double* pred, double* rhs_prev, 
   int rc = periodic_get_rhs(c, state, c->rhs0, s);
   if (rc) return rc;
   const FeatArgs a = feat_args(c, s_1, s_2, s_out);
+  if (a.n >= (1LL << 31)) return c->fail(DNSMI_EINVAL, "guess: too many modes for 32-bit indexing");
   const int nb = (int)((a.n + 255) / 256);
   const cx<double>* n0 = reinterpret_cast<const cx<double>*>(c->rhs0);
   if (mode != 2 || H == 32)
