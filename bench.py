@@ -46,6 +46,7 @@ def kernel_bytes(n: int, b: int) -> dict:
         "z_inverse": 5 * P * Nz * Nx * b + 6 * P * P * Nx * b,
         "x_nonlinear": 9 * P * P * Nx * b,
         "z_forward": 3 * P * P * Nx * b + 3 * P * Nz * Nx * b,
+        "y_forward:rhs": 3 * P * Nz * Nx * b + S,
         "y_forward:predict": 3 * P * Nz * Nx * b + 3 * S,
         "y_forward:correct": 3 * P * Nz * Nx * b + 4 * S,
         "S": S,
@@ -87,6 +88,15 @@ def gpu_form(precision: str, steps: int, warmup: int):
     state, counts2 = run_steps(flow, state, steps)
     torch.cuda.synchronize()
     prof = flow._native.profile_end()
+    # third region: the RHS alone (the y forward with its RHS-only epilogue),
+    # the "FFT/NL pass" the pass model describes
+    rhs = flow._native.get_rhs(state)
+    torch.cuda.synchronize()
+    flow._native.profile_begin()
+    for _ in range(20):
+        flow._native.get_rhs(state, out=rhs)
+    torch.cuda.synchronize()
+    prof_rhs = flow._native.profile_end()
     return {
         "steps_per_s": steps / elapsed,
         "ms_per_step": 1e3 * elapsed / steps,
@@ -94,6 +104,7 @@ def gpu_form(precision: str, steps: int, warmup: int):
         "counts": counts,
         "prof": prof,
         "prof_counts": counts2,
+        "prof_rhs": prof_rhs,
     }
 
 
@@ -130,7 +141,13 @@ def roofline(form: dict, b: int) -> dict:
     dom = max(per, key=lambda k: prof[k][0])
     d = per[dom]
     rhs_names = ("y_inverse", "z_inverse", "x_nonlinear", "z_forward", "y_forward")
-    t_rhs = sum(per[k]["avg_us"] for k in rhs_names if k in per) * 1e-6
+    # one pass of the stepping loop: the five kernels with the y forward's
+    # predictor / corrector epilogue, against their algorithmic bytes
+    t_pass = sum(per[k]["avg_us"] for k in rhs_names if k in per) * 1e-6
+    b_pass = sum(per[k]["bytes"] for k in rhs_names if k in per)
+    # the RHS alone against the pass model
+    pr = form["prof_rhs"]
+    t_rhs = sum(pr[k][0] / pr[k][1] for k in rhs_names if k in pr) * 1e-3
     rhs_gbs = rhs_model_bytes(N, b) / t_rhs / 1e9
     return {
         "dominant": {"bound": "hbm", "kernel": dom, "achieved": round(d["gbs"], 1), "peak": PEAK_GBS,
@@ -138,7 +155,10 @@ def roofline(form: dict, b: int) -> dict:
                      "traffic": measured_traffic(dom, b),
                      "avg_us": round(d["avg_us"], 2), "bytes_per_launch": int(d["bytes"])},
         "rhs": {"model_bytes": rhs_model_bytes(N, b), "t_us": round(t_rhs * 1e6, 2),
-                "achieved": round(rhs_gbs, 1), "frac": round(rhs_gbs / PEAK_GBS, 4)},
+                "achieved": round(rhs_gbs, 1), "frac": round(rhs_gbs / PEAK_GBS, 4),
+                "kernels_us": {k: round(pr[k][0] / pr[k][1] * 1e3, 2) for k in rhs_names if k in pr}},
+        "pass": {"bytes": int(b_pass), "t_us": round(t_pass * 1e6, 2),
+                 "achieved": round(b_pass / t_pass / 1e9, 1), "frac": round(b_pass / t_pass / 1e9 / PEAK_GBS, 4)},
         "kernels": {k: {kk: (round(vv, 2) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                     for k, v in per.items()},
     }
@@ -491,11 +511,13 @@ def main():
                    "form": "plain fp64", "corrector_iters_per_step": round(fp64["corrector_iters_per_step"], 3)},
         "roofline": rf64["dominant"],
         "rhs_roofline": rf64["rhs"],
+        "pass_roofline": rf64["pass"],
         "kernels": rf64["kernels"],
         "mixed": {"value": round(mixed["steps_per_s"], 3), "ms_per_step": round(mixed["ms_per_step"], 4),
                   "speedup_vs_fp64": round(mixed["steps_per_s"] / fp64["steps_per_s"], 3),
                   "corrector_iters_per_step": round(mixed["corrector_iters_per_step"], 3),
-                  "roofline": rmix["dominant"], "rhs_roofline": rmix["rhs"], "kernels": rmix["kernels"]},
+                  "roofline": rmix["dominant"], "rhs_roofline": rmix["rhs"], "pass_roofline": rmix["pass"],
+                  "kernels": rmix["kernels"]},
     }
     if a.walls_steps > 0:
         out["walls"] = walls_form(a.walls_steps)
