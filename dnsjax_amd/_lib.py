@@ -76,7 +76,7 @@ SIGNATURES = {
     "dnsmi_tuning": (ctypes.c_char_p, [_P]),
     "dnsmi_get_stats": (_I, [_P, _P, _P, _P]),
     "dnsmi_step_device": (_I, [_P, _P, _P, _P, _P, _I, _P, _I, ctypes.c_double, _I, ctypes.c_double,
-                               ctypes.c_double, _P]),
+                               ctypes.c_double, ctypes.c_double, _P]),
     "dnsmi_step_stats": (_I, [_P, ctypes.POINTER(ctypes.c_double)]),
     "dnsmi_step_reset": (_I, [_P]),
     "dnsmi_mlp_sizes": (_I, [_I, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong),
@@ -91,7 +91,7 @@ SIGNATURES = {
                          ctypes.c_float, _I, _P]),
     "dnsmi_step_device_guess": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _I, ctypes.c_float, ctypes.c_float,
                                      ctypes.c_float, _I, _I, _P, _I, ctypes.c_double, _I, ctypes.c_double,
-                                     ctypes.c_double, _P]),
+                                     ctypes.c_double, ctypes.c_double, _P]),
 }
 
 KERNEL_IDS = 12

@@ -67,13 +67,13 @@ int walls_table(Ctx* c, int which, double* out, long long n);
 int get_stats(Ctx* c, const double* state, double* out, hipStream_t s);
 int step_device(Ctx* c, const double* state, double* pred, double* rhs_a, double* rhs_b, int slots,
                 int* counts, int counts_cap, double tol, int max_iter, double floor_ratio,
-                double floor_cap, hipStream_t s);
+                double floor_cap, double floor_margin, hipStream_t s);
 int step_read_ctl(Ctx* c, double* out6);
 int step_device_guess(Ctx* c, const double* state, double* pred, double* rhs_a, double* rhs_b,
                       const float* h1, float* h2, const uint16_t* wb, const float* bias, int H,
                       float s_1, float s_2, float s_out, int mode, int slots, int* counts,
                       int counts_cap, double tol, int max_pass, double floor_ratio,
-                      double floor_cap, hipStream_t s);
+                      double floor_cap, double floor_margin, hipStream_t s);
 int step_alloc_ctl(Ctx* c);
 int mlp_grad(int H, int B, const float* X, const float* T, const float* prm, int bf16, float* part,
              float* grad, float* loss, hipStream_t s);
@@ -252,7 +252,8 @@ int dnsmi_get_stats(dnsmi_ctx* ctx, const double* state, double* stats_dev, void
 
 int dnsmi_step_device(dnsmi_ctx* ctx, const double* state, double* next, double* rhs_a,
                       double* rhs_b, int slots, int* counts, int counts_cap, double tol,
-                      int max_iter, double floor_ratio, double floor_cap, void* stream) {
+                      int max_iter, double floor_ratio, double floor_cap, double floor_margin,
+                      void* stream) {
   DNSMI_CHECK_CTX(ctx)
   DNSMI_REQUIRE(state && next && rhs_a && rhs_b && slots >= 0 && max_iter >= 0,
                 "step_device: bad arguments");
@@ -261,7 +262,7 @@ int dnsmi_step_device(dnsmi_ctx* ctx, const double* state, double* next, double*
                     (const void*)state != rhs_b,
                 "step_device: buffers must be distinct");
   return dnsmi::step_device(c, state, next, rhs_a, rhs_b, slots, counts, counts_cap, tol, max_iter,
-                            floor_ratio, floor_cap, as_stream(stream));
+                            floor_ratio, floor_cap, floor_margin, as_stream(stream));
 }
 
 int dnsmi_step_stats(dnsmi_ctx* ctx, double* out6) {
@@ -354,7 +355,7 @@ int dnsmi_step_device_guess(dnsmi_ctx* ctx, const double* state, double* next, d
                             const void* packed_weights, const float* bias, int H, float s_1,
                             float s_2, float s_out, int mode, int slots, int* counts,
                             int counts_cap, double tol, int max_pass, double floor_ratio,
-                            double floor_cap, void* stream) {
+                            double floor_cap, double floor_margin, void* stream) {
   DNSMI_CHECK_CTX(ctx)
   DNSMI_REQUIRE(state && next && rhs_a && rhs_b && hist1 && hist2, "step_device_guess: null pointer");
   DNSMI_REQUIRE(mode != 2 || (packed_weights && bias), "step_device_guess: network weights missing");
@@ -366,7 +367,7 @@ int dnsmi_step_device_guess(dnsmi_ctx* ctx, const double* state, double* next, d
   return dnsmi::step_device_guess(c, state, next, rhs_a, rhs_b, hist1, hist2,
                                   static_cast<const uint16_t*>(packed_weights), bias, H, s_1, s_2,
                                   s_out, mode, slots, counts, counts_cap, tol, max_pass,
-                                  floor_ratio, floor_cap, as_stream(stream));
+                                  floor_ratio, floor_cap, floor_margin, as_stream(stream));
 }
 
 const char* dnsmi_tuning(const dnsmi_ctx* ctx) { return ctx ? ctx->impl.tuning.c_str() : ""; }

@@ -8,7 +8,7 @@
 // iteration is allowed; each of `slots` corrector slots runs only if
 // `active` (every kernel of the slot reads it and returns at once when 0),
 // then updates the count and `active` exactly as the host rule would
-// (including the optional noise-floor stop of the mixed form).  The step
+// (including the optional noise-floor stops of the mixed form).  The step
 // ends with correct_velocity and records the step's iteration count.
 #include <hip/hip_runtime.h>
 
@@ -30,6 +30,7 @@ struct StepCtl {
   int unconverged;
   int nan;
   int pad;
+  double floor_est;  // noise-floor estimate: the error of the last stall stop
 };
 
 int periodic_predict_and_correct(Ctx* c, const double* state, double* pred, double* rhs_next,
@@ -52,14 +53,26 @@ __global__ void k_ctl_begin(StepCtl* ctl, double tol, int max_iter) {
   ctl->active = (e > tol) && (max_iter > 0);
 }
 
+// The mixed form's noise-floor stops, the same rule as the host loops:
+// for an error above `tol` and below `cap`, a pass that failed to shrink the error by `ratio` is at the
+// floor (a stall stop; its error becomes the floor estimate), and with
+// `margin` > 0 a pass whose error is within `margin` times the estimate is
+// taken as at the floor without the confirming pass (the estimate can then
+// only decrease).
 __global__ void k_ctl_after_iteration(StepCtl* ctl, double tol, int max_iter, double ratio,
-                                      double cap) {
+                                      double cap, double margin) {
   if (threadIdx.x != 0 || !ctl->active) return;
   const double e = ctl->err;
   ctl->c += 1;
-  if (ratio > 0 && e <= cap && e > ratio * ctl->prev) {
+  if (ratio > 0 && e > tol && e <= cap && e > ratio * ctl->prev) {
     ctl->floor = 1;
     ctl->active = 0;
+    ctl->floor_est = e;
+  } else if (ratio > 0 && margin > 0 && ctl->floor_est > 0 && e > tol && e <= cap &&
+             e <= margin * ctl->floor_est) {
+    ctl->floor = 1;
+    ctl->active = 0;
+    ctl->floor_est = fmin(ctl->floor_est, e);
   } else {
     ctl->active = (e > tol) && (ctl->c < max_iter);
   }
@@ -78,7 +91,7 @@ __global__ void k_ctl_end(StepCtl* ctl, int* counts, int counts_cap, double tol)
 
 int step_device(Ctx* c, const double* state, double* pred, double* rhs_a, double* rhs_b, int slots,
                 int* counts, int counts_cap, double tol, int max_iter, double floor_ratio,
-                double floor_cap, hipStream_t s) {
+                double floor_cap, double floor_margin, hipStream_t s) {
   if (!c->ctl) return c->fail(DNSMI_EINVAL, "control block missing");
   StepCtl* ctl = reinterpret_cast<StepCtl*>(c->ctl);
   double* err = &ctl->err;
@@ -97,7 +110,7 @@ int step_device(Ctx* c, const double* state, double* pred, double* rhs_a, double
                   : periodic_iterate_correction(c, pred, rp, rn, err, s);
     if (rc) break;
     hipLaunchKernelGGL(k_ctl_after_iteration, dim3(1), dim3(64), 0, s, ctl, tol, max_iter,
-                       floor_ratio, floor_cap);
+                       floor_ratio, floor_cap, floor_margin);
     rc = check_launch(c, "ctl_after_iteration");
   }
   c->act = nullptr;
@@ -131,7 +144,7 @@ int step_device_guess(Ctx* c, const double* state, double* pred, double* rhs_a, 
                       const float* h1, float* h2, const uint16_t* wb, const float* bias, int H,
                       float s_1, float s_2, float s_out, int mode, int slots, int* counts,
                       int counts_cap, double tol, int max_pass, double floor_ratio,
-                      double floor_cap, hipStream_t s) {
+                      double floor_cap, double floor_margin, hipStream_t s) {
   if (!c->ctl) return c->fail(DNSMI_EINVAL, "control block missing");
   StepCtl* ctl = reinterpret_cast<StepCtl*>(c->ctl);
   double* err = &ctl->err;
@@ -151,7 +164,7 @@ int step_device_guess(Ctx* c, const double* state, double* pred, double* rhs_a, 
                   : periodic_iterate_correction(c, pred, rp, rn, err, s);
     if (rc) break;
     hipLaunchKernelGGL(k_ctl_after_iteration, dim3(1), dim3(64), 0, s, ctl, tol, max_pass,
-                       floor_ratio, floor_cap);
+                       floor_ratio, floor_cap, floor_margin);
     rc = check_launch(c, "ctl_after_iteration");
   }
   c->act = nullptr;

@@ -13,7 +13,11 @@ corrector error.  With ``params.corrector_floor_ratio`` set (the mixed
 default), a pass whose error did not shrink by that ratio while already
 below ``params.floor_cap_value`` (min(100 tolerance, 1e-8)) ends the
 corrector and counts as converged (``RunResult.floor_stops`` counts such
-steps).
+steps), and its error becomes the floor estimate F.  With
+``params.corrector_floor_margin`` set (the mixed default, 2), a later pass
+with error below the cap and within that margin of F is taken as at the
+floor without the confirming pass, and F becomes min(F, error).  The
+device-decided step (``dnsmi_step_device``) applies the same rule.
 """
 
 from __future__ import annotations
@@ -41,6 +45,8 @@ def run(flow, state, n_steps: int, callback=None) -> RunResult:
     tol, max_it = p.corrector_tolerance, p.max_corrector_iterations
     ratio = p.corrector_floor_ratio
     cap = p.floor_cap_value
+    margin = p.corrector_floor_margin
+    floor_est = 0.0
     res = RunResult(state=state)
     last_error = 0.0
     for it in range(n_steps):
@@ -55,9 +61,13 @@ def run(flow, state, n_steps: int, callback=None) -> RunResult:
             state, rhs, error = flow.iterate_correction(state_prev, state, rhs)
             prev, err = err, float(error)
             c += 1
-            if ratio is not None and err <= cap and err > ratio * prev:
-                floor = True
-                break
+            if ratio is not None and tol < err <= cap:
+                if err > ratio * prev:
+                    floor, floor_est = True, err
+                    break
+                if margin is not None and floor_est > 0 and err <= margin * floor_est:
+                    floor, floor_est = True, min(floor_est, err)
+                    break
         state, _ = flow.correct_velocity(state)
         res.counts.append(c)
         res.errors.append(err)

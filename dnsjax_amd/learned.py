@@ -236,6 +236,8 @@ def run_guessed(flow, state, n_steps: int, mode: int = 2, net: Net | None = None
     tol, max_pass = p.corrector_tolerance, p.max_corrector_iterations + 1
     ratio = p.corrector_floor_ratio
     cap = p.floor_cap_value
+    margin = p.corrector_floor_margin
+    floor_est = 0.0
     sc = (1.0, 1.0, 1.0) if scales is None else scales.astuple()
     ops = None if net is None else net.operands
     hist = [torch.zeros(nat.shape + (2,), dtype=torch.float32, device=nat.device) for _ in range(2)]
@@ -257,9 +259,13 @@ def run_guessed(flow, state, n_steps: int, mode: int = 2, net: Net | None = None
             cur = 1 - cur
             prev, err = err, float(err_t)
             c += 1
-            if ratio is not None and err <= cap and err > ratio * prev:
-                floor = True
-                break
+            if ratio is not None and tol < err <= cap:
+                if err > ratio * prev:
+                    floor, floor_est = True, err
+                    break
+                if margin is not None and floor_est > 0 and err <= margin * floor_est:
+                    floor, floor_est = True, min(floor_est, err)
+                    break
         pred, state = state, pred
         state, _ = flow.correct_velocity(state)
         res.passes.append(c)

@@ -130,7 +130,7 @@ class NativeContext:
 
     # ---------------------------------------------------------------- device-side steps
     def step_device(self, state, nxt, rhs_a, rhs_b, slots, counts=None, tol=None, max_iter=None,
-                    floor_ratio=None, floor_cap=None):
+                    floor_ratio=None, floor_cap=None, floor_margin=None):
         """Enqueue one full time step with the corrector decided on the device
         (capturable; no host synchronisation).  The new state goes to `nxt`."""
         p = self.params
@@ -138,10 +138,12 @@ class NativeContext:
         max_iter = p.max_corrector_iterations if max_iter is None else max_iter
         ratio = p.corrector_floor_ratio if floor_ratio is None else floor_ratio
         cap = min(p.corrector_floor_cap * tol, p.corrector_floor_abs) if floor_cap is None else floor_cap
+        margin = p.corrector_floor_margin if floor_margin is None else floor_margin
         ncap = 0 if counts is None else counts.numel()
         self._chk(self.lib.dnsmi_step_device(self.handle, _ptr(state), _ptr(nxt), _ptr(rhs_a), _ptr(rhs_b),
                                              slots, _ptr(counts), ncap, tol, max_iter,
-                                             0.0 if ratio is None else ratio, cap, _stream()))
+                                             0.0 if ratio is None else ratio, cap,
+                                             0.0 if margin is None else margin, _stream()))
 
     def mlp_features(self, n_m2, n_m1, n_cur, n_next, modes, s_1, s_2, s_out, X, T=None):
         """Training features (and targets when n_next and T are given)."""
@@ -165,13 +167,13 @@ class NativeContext:
         p = self.params
         H, wb, bias = (32, None, None) if net is None else net
         s_1, s_2, s_out = scales
-        ratio = p.corrector_floor_ratio
+        ratio, margin = p.corrector_floor_ratio, p.corrector_floor_margin
         ncap = 0 if counts is None else counts.numel()
         self._chk(self.lib.dnsmi_step_device_guess(
             self.handle, _ptr(state), _ptr(nxt), _ptr(rhs_a), _ptr(rhs_b), _ptr(hist1), _ptr(hist2), _ptr(wb),
             _ptr(bias), H, s_1, s_2, s_out, mode, slots, _ptr(counts), ncap, p.corrector_tolerance,
             p.max_corrector_iterations + 1, 0.0 if ratio is None else ratio,
-            p.floor_cap_value, _stream()))
+            p.floor_cap_value, 0.0 if margin is None else margin, _stream()))
 
     def step_stats(self) -> dict:
         out = (ctypes.c_double * 6)()

@@ -51,6 +51,14 @@ class Params:
     # contraction (the wall-bounded corrector contracts only ~2x per pass),
     # not noise, and must not end the corrector.
     corrector_floor_abs: float = 1e-8
+    # Mixed form only: once a stall has shown the floor level F (the error
+    # of the last stall stop), a pass with tol < error <= floor_cap and error <=
+    # corrector_floor_margin * F is taken as at the floor, without the pass
+    # that would confirm the stall; F then becomes min(F, error).  The state
+    # is then converged to within the last correction over the corrector's
+    # contraction factor (about 70 for the periodic corrector).  None keeps
+    # the stall rule alone.
+    corrector_floor_margin: float | None = None
 
     def __post_init__(self) -> None:
         if self.system not in PERIODIC_SYSTEMS + WALLED_SYSTEMS:
@@ -61,6 +69,8 @@ class Params:
             raise ValueError("precision must be 'fp64' or 'mixed'")
         if self.corrector_floor_ratio is None and self.precision == "mixed":
             self.corrector_floor_ratio = 0.5
+        if self.corrector_floor_margin is None and self.precision == "mixed":
+            self.corrector_floor_margin = 2.0
         if not self.re > 0 or not self.dt > 0 or not 0 <= self.implicitness <= 1:
             raise ValueError("need re > 0, dt > 0, 0 <= implicitness <= 1")
 

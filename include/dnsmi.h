@@ -135,17 +135,21 @@ int dnsmi_walls_table(dnsmi_ctx* ctx, int which, double* out, long long n);
  * `state` into `next`, then up to `slots` iterate_correction passes, each
  * skipped on the device once error <= tol or max_iter extra passes ran
  * (and, if floor_ratio > 0, once a pass failed to shrink the error by that
- * ratio while error <= floor_cap: the mixed form's noise floor), then
+ * ratio while tol < error <= floor_cap: the mixed form's noise floor; that error
+ * becomes the context's floor estimate F, and with floor_margin > 0 a later
+ * pass with tol < error <= floor_cap and error <= floor_margin * F ends the
+ * corrector, setting F to min(F, error)), then
  * correct_velocity(next).  rhs_a / rhs_b are two RHS work arrays.  No host
  * synchronisation; the launch sequence is fixed, so it can be captured in
  * a hipGraph.  counts (device int[counts_cap], may be NULL) receives each
  * step's extra-iteration count in order.  `state` is not modified. */
 int dnsmi_step_device(dnsmi_ctx* ctx, const double* state, double* next, double* rhs_a,
                       double* rhs_b, int slots, int* counts, int counts_cap, double tol,
-                      int max_iter, double floor_ratio, double floor_cap, void* stream);
+                      int max_iter, double floor_ratio, double floor_cap, double floor_margin,
+                      void* stream);
 /* Synchronising read of the device step counters: out6 = {steps, total extra
  * iterations, steps not converged, NaN steps, last error, last step stopped
- * at the noise floor}; dnsmi_step_reset zeroes them. */
+ * at the noise floor}; dnsmi_step_reset zeroes them and the floor estimate. */
 int dnsmi_step_stats(dnsmi_ctx* ctx, double* out6);
 int dnsmi_step_reset(dnsmi_ctx* ctx);
 
@@ -225,7 +229,7 @@ int dnsmi_step_device_guess(dnsmi_ctx* ctx, const double* state, double* next, d
                             const void* packed_weights, const float* bias, int H, float s_1,
                             float s_2, float s_out, int mode, int slots, int* counts,
                             int counts_cap, double tol, int max_pass, double floor_ratio,
-                            double floor_cap, void* stream);
+                            double floor_cap, double floor_margin, void* stream);
 
 /* Per-kernel timing with HIP events on the launch stream (no counterpart in
  * the reference, whose bench.timer (bench.py:26-67) times whole calls).

@@ -135,6 +135,29 @@ def test_driver_noise_floor_rule_for_mixed():
     assert r.counts == [9] and r.floor_stops == 0
 
 
+def test_noise_floor_margin_after_a_stall():
+    """After a stall stop has shown the floor F, a pass within the margin
+    (2 F for the mixed form) of it ends the corrector without the confirming
+    pass; F then only decreases, until a new stall sets it again."""
+    f = _ScriptedFlow([1e-5, 1e-7, 2e-9, 1.6e-10, 1.3e-10,  # stall: F = 1.3e-10
+                       1e-5, 1e-7, 2e-9, 2.5e-10,  # <= 2 F: margin stop
+                       1e-5, 1e-7, 3e-10, 1.2e-10,  # 3e-10 > 2 F goes on; then F = 1.2e-10
+                       1e-5, 2.5e-10, 2.3e-10],  # > 2 F, then a stall: F = 2.3e-10
+                      tol=1e-10, max_it=10)
+    f.params = Params(precision="mixed", corrector_tolerance=1e-10, max_corrector_iterations=10)
+    assert f.params.corrector_floor_margin == 2.0
+    r = run(f, 0.0, 4)
+    assert r.counts == [4, 3, 3, 2] and r.floor_stops == 4 and r.converged
+    assert not f.errs
+    # without the margin the second step needs the confirming pass
+    g = _ScriptedFlow([1e-5, 1e-7, 2e-9, 1.6e-10, 1.3e-10, 1e-5, 1e-7, 2e-9, 2.5e-10, 1.3e-10],
+                      tol=1e-10, max_it=10)
+    g.params = Params(precision="mixed", corrector_tolerance=1e-10, max_corrector_iterations=10,
+                      corrector_floor_margin=0.0)
+    r = run(g, 0.0, 2)
+    assert r.counts == [4, 4] and r.floor_stops == 2
+
+
 def test_noise_floor_rule_ignores_slow_contraction():
     """A loose tolerance with a corrector contracting only ~2x per pass (the
     wall-bounded one) must iterate to the tolerance: a stall above the
