@@ -81,3 +81,64 @@ def run(flow, state, n_steps: int, callback=None) -> RunResult:
     res.state = state
     res.converged = not (last_error > tol) and not res.nan
     return res
+
+
+class StatsRecorder:
+    """The reference driver's energy record and its ``stats.npz``
+    (``__main__.py:107-120``, ``137-158``, ``227-242``): E' of the initial
+    state at t0; at the top of every loop iteration ``it`` with
+    ``it % it_stats == 0`` (so never for the state after the last step); and
+    of the final state once the loop ends, when more than one step ran.  The
+    file holds ``ts`` (time, accumulated by repeated ``t += dt``) and ``Eps``.
+
+    Pass the recorder as ``run``'s callback, then call ``finish``::
+
+        rec = StatsRecorder(flow, state, it_stats=10)
+        res = run(flow, state, n, callback=rec)
+        rec.finish(res.state, len(res.counts))
+        rec.save("stats.npz")
+    """
+
+    def __init__(self, flow, state, it_stats: int | None = None, t0: float = 0.0, it0: int = 0):
+        self.flow, self.it_stats, self.it0 = flow, it_stats, it0
+        self.dt = flow.params.dt
+        self._t = [t0]  # t after k steps, accumulated as the reference does
+        self.ts = [t0]
+        self.Eps = [self._energy(state)]
+        self._last_it = None  # iteration of the last in-loop record
+        self.complete = False
+
+    def _energy(self, state) -> float:
+        return float(self.flow.get_stats(state)["E'"])
+
+    def _time(self, k: int) -> float:
+        while len(self._t) <= k:
+            self._t.append(self._t[-1] + self.dt)
+        return self._t[k]
+
+    def __call__(self, k: int, state) -> None:
+        it = self.it0 + k
+        if self.it_stats is not None and it % self.it_stats == 0 and it > self.it0:
+            self.ts.append(self._time(k))
+            self.Eps.append(self._energy(state))
+            self._last_it = it
+
+    def finish(self, state, k: int) -> None:
+        """Record the final state after k steps (the loop's exit)."""
+        it = self.it0 + k
+        if it <= self.it0 + 1:
+            return  # the reference writes nothing for fewer than two steps
+        if self._last_it == it:  # the loop never reaches the top of iteration it
+            self.ts.pop()
+            self.Eps.pop()
+        self.ts.append(self._time(k))
+        self.Eps.append(self._energy(state))
+        self.complete = True
+
+    def save(self, path: str = "stats.npz") -> None:
+        import numpy as np
+
+        if not self.complete:
+            raise ValueError("finish() a run of at least two steps first")
+        np.savez(path, ts=np.array(self.ts), Eps=np.array(self.Eps))
+

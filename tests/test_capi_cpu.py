@@ -167,3 +167,41 @@ def test_noise_floor_rule_ignores_slow_contraction():
     assert f.params.floor_cap_value == 1e-8
     r = run(f, 0.0, 1)
     assert r.counts == [8] and r.floor_stops == 0 and r.converged
+
+
+class _EnergyFlow(_ScriptedFlow):
+    """Scripted errors; the 'state' is a step counter and E' = 1 / (1 + state)."""
+
+    def predict_and_correct(self, s):
+        return s + 1, None, self.errs.pop(0)
+
+    def get_stats(self, s):
+        return {"E'": 1.0 / (1.0 + s)}
+
+
+def test_stats_recorder_follows_the_reference_driver(tmp_path):
+    from dnsjax_amd.driver import StatsRecorder
+
+    f = _EnergyFlow([1e-9] * 7, tol=1e-6, max_it=3)
+    f.params = Params(corrector_tolerance=1e-6, dt=0.1)
+    rec = StatsRecorder(f, 0, it_stats=3)
+    r = run(f, 0, 6, callback=rec)
+    rec.finish(r.state, len(r.counts))
+    # t0, the top of iteration 3, the final state (iteration 6 is never a loop top)
+    t3 = 0.1 + 0.1 + 0.1
+    t6 = t3 + 0.1 + 0.1 + 0.1
+    assert rec.ts == [0.0, t3, t6] and rec.Eps == [1.0, 1 / 4, 1 / 7]
+    path = str(tmp_path / "stats.npz")
+    rec.save(path)
+    with np.load(path) as z:
+        assert sorted(z.files) == ["Eps", "ts"]
+        np.testing.assert_array_equal(z["ts"], [0.0, t3, t6])
+    # a one-step run writes nothing
+    g = _EnergyFlow([1e-9], tol=1e-6, max_it=3)
+    g.params = Params(corrector_tolerance=1e-6, dt=0.1)
+    rec1 = StatsRecorder(g, 0, it_stats=1)
+    r1 = run(g, 0, 1, callback=rec1)
+    rec1.finish(r1.state, len(r1.counts))
+    with pytest.raises(ValueError):
+        rec1.save(str(tmp_path / "none.npz"))
+
