@@ -89,7 +89,8 @@ int guess_periodic(Ctx* c, const double* state, double* pred, double* rhs_prev, 
                    float* h2, const uint16_t* wb, const float* bias, int H, float s_1, float s_2,
                    float s_out, int mode, hipStream_t s);
 int guess_walls(Ctx* c, const double* state, double* pred, double* rhs_prev, const float* h1,
-                float* h2, int mode, hipStream_t s);
+                float* h2, const uint16_t* wb, const float* bias, int H, float s_1, float s_2,
+                float s_out, int mode, hipStream_t s);
 int step_reset_ctl(Ctx* c);
 
 }  // namespace dnsmi
@@ -344,9 +345,11 @@ int dnsmi_guess(dnsmi_ctx* ctx, const double* state, double* pred, double* rhs_p
   DNSMI_REQUIRE((const void*)state != pred && (const void*)state != rhs_prev &&
                     (void*)pred != rhs_prev && (const void*)hist1 != hist2,
                 "guess: buffers must be distinct");
-  if (c->walls) return dnsmi::guess_walls(c, state, pred, rhs_prev, hist1, hist2, mode, as_stream(stream));
-  return dnsmi::guess_periodic(c, state, pred, rhs_prev, hist1, hist2,
-                               static_cast<const uint16_t*>(packed_weights), bias, H, s_1, s_2,
+  const uint16_t* wb = static_cast<const uint16_t*>(packed_weights);
+  if (c->walls)
+    return dnsmi::guess_walls(c, state, pred, rhs_prev, hist1, hist2, wb, bias, H, s_1, s_2, s_out,
+                              mode, as_stream(stream));
+  return dnsmi::guess_periodic(c, state, pred, rhs_prev, hist1, hist2, wb, bias, H, s_1, s_2,
                                s_out, mode, as_stream(stream));
 }
 

@@ -339,37 +339,50 @@ __global__ void __launch_bounds__(256)
 // the residual of the quadratic extrapolation, weighted by
 // w = ildt_2 / dt (in (0, 1]) so that the squared loss weights each mode as
 // the corrector's error norm does.
+// A wall-bounded point (kz, kx, y) has the same features with y (in [-1, 1])
+// in place of ky/kmax and |k|^2 = kx^2 + kz^2, and the same weight formula
+// (not zeroed at the mean mode, whose profile the wall scheme advances).
 struct FeatArgs {
   const double* kx;
-  const double* ky;
+  const double* ky;  // periodic: ky; wall-bounded: the y grid
   const double* kz;
-  int Nz1, Nx1;
-  long long n;
+  int Nz1, Nx1, Ny;
+  int walls;
+  long long n;  // points per component
   double kmax, dt, re, c;
   float s_1, s_2, s_out;
 };
 
-__device__ __forceinline__ void mode_k(const FeatArgs& a, long long i, double& kx, double& ky,
-                                       double& kz) {
-  const int ix = (int)(i % a.Nx1);
-  const long long t = i / a.Nx1;
-  const int iz = (int)(t % a.Nz1);
-  const int iy = (int)(t / a.Nz1);
-  kx = a.kx[ix];
-  ky = a.ky[iy];
-  kz = a.kz[iz];
+// Coordinates of flat index i: a periodic mode over (ky, kz, kx) gives
+// (kx, ky, kz); a wall-bounded point over (kz, kx, y) gives (kx, y, kz).
+// I = unsigned where every index fits in 32 bits (the launchers check it).
+template <typename I>
+__device__ __forceinline__ void coords(const FeatArgs& a, I i, double& kx, double& kb, double& kz) {
+  if (a.walls) {
+    const I t = i / (I)a.Ny, u = t / (I)a.Nx1;
+    kx = a.kx[t - u * (I)a.Nx1];
+    kb = a.ky[i - t * (I)a.Ny];
+    kz = a.kz[u];
+  } else {
+    const I t = i / (I)a.Nx1, u = t / (I)a.Nz1;
+    kx = a.kx[i - t * (I)a.Nx1];
+    kb = a.ky[u];
+    kz = a.kz[t - u * (I)a.Nz1];
+  }
 }
 
-// The same with 32-bit index arithmetic (the per-component mode count of
-// every supported grid is far below 2^31; the launcher checks it).
-__device__ __forceinline__ void mode_k32(const FeatArgs& a, unsigned i, double& kx, double& ky,
-                                         double& kz) {
-  const unsigned nx1 = (unsigned)a.Nx1, nz1 = (unsigned)a.Nz1;
-  const unsigned t = i / nx1, ix = i - t * nx1;
-  const unsigned iy = t / nz1, iz = t - iy * nz1;
-  kx = a.kx[ix];
-  ky = a.ky[iy];
-  kz = a.kz[iz];
+// |k|^2 of the features and the weight.
+__device__ __forceinline__ double ksq(const FeatArgs& a, double kx, double kb, double kz) {
+  return a.walls ? kx * kx + kz * kz : (kx * kx + kb * kb) + kz * kz;
+}
+
+// Features 12..15.
+__device__ __forceinline__ void k_features(const FeatArgs& a, double kx, double kb, double kz,
+                                           float* f) {
+  f[0] = (float)fmin(a.dt * ksq(a, kx, kb, kz) / a.re, 8.0);
+  f[1] = (float)(kx / a.kmax);
+  f[2] = a.walls ? (float)kb : (float)(kb / a.kmax);
+  f[3] = (float)(kz / a.kmax);
 }
 
 // ildt_2 / dt for mode i (0 for the mean mode, which the scheme holds fixed).
@@ -377,30 +390,11 @@ __device__ __forceinline__ double mode_weight(const FeatArgs& a, double lapl, bo
   return mean ? 0.0 : 1.0 / (1.0 - a.dt * a.c * lapl / a.re);
 }
 
-__device__ __forceinline__ float k_feature(const FeatArgs& a, long long i, int f) {
-  double kx, ky, kz;
-  mode_k(a, i, kx, ky, kz);
-  if (f == 12) return (float)fmin(a.dt * ((kx * kx + ky * ky) + kz * kz) / a.re, 8.0);
-  const double v = f == 13 ? kx : (f == 14 ? ky : kz);
-  return (float)(v / a.kmax);
-}
-
 // D1 and D2 of one real component from N^n (FP64) and the FP32 history.
 __device__ __forceinline__ void diffs(double cur, float h1, float h2, double& d1, double& d2) {
   const double p1 = (double)h1, p2 = (double)h2;
   d1 = cur - p1;
   d2 = (cur - 2.0 * p1) + p2;
-}
-
-__device__ __forceinline__ float feature(const FeatArgs& a, const cx<double>* ncur,
-                                         const float* h1, const float* h2, long long i, int f) {
-  if (f >= 12) return k_feature(a, i, f);
-  const int q = f < 6 ? f : f - 6;
-  const cx<double> v = ncur[(q / 2) * a.n + i];
-  const long long e = ((q / 2) * a.n + i) * 2 + (q & 1);
-  double d1, d2;
-  diffs((q & 1) ? v.y : v.x, h1[e], h2[e], d1, d2);
-  return f < 6 ? (float)d1 * a.s_1 : (float)d2 * a.s_2;
 }
 
 // Training samples from four successive nonlinear terms (FP64 spectral
@@ -422,11 +416,13 @@ __global__ void k_mlp_features(FeatArgs a, const cx<double>* __restrict__ nm2,
     X[(size_t)s * kDin + q] = (float)d1 * a.s_1;
     X[(size_t)s * kDin + 6 + q] = (float)d2 * a.s_2;
   }
-  for (int f = 12; f < kDin; ++f) X[(size_t)s * kDin + f] = k_feature(a, i, f);
+  double kx, kb, kz;
+  coords(a, i, kx, kb, kz);
+  float kf[4];
+  k_features(a, kx, kb, kz, kf);
+  for (int f = 12; f < kDin; ++f) X[(size_t)s * kDin + f] = kf[f - 12];
   if (T && nnext) {
-    double kx, ky, kz;
-    mode_k(a, i, kx, ky, kz);
-    const double w = mode_weight(a, -((kx * kx + ky * ky) + kz * kz), i == 0);
+    const double w = mode_weight(a, -ksq(a, kx, kb, kz), !a.walls && i == 0);
     for (int q = 0; q < 6; ++q) {
       const long long e = (q / 2) * a.n + i;
       const cx<double> p2 = nm2[e], p1 = nm1[e], c = ncur[e], x = nnext[e];
@@ -475,7 +471,7 @@ __global__ void __launch_bounds__(256)
   const bool live = i < a.n;
   const long long ii = live ? i : 0;
   double kx, ky, kz;
-  mode_k32(a, (unsigned)ii, kx, ky, kz);
+  coords(a, (unsigned)ii, kx, ky, kz);
   cx<double> nc[3], d1[3], d2[3];
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
@@ -494,11 +490,7 @@ __global__ void __launch_bounds__(256)
       feats[wave][lane][6 + 2 * q] = (float)d2[q].x * a.s_2;
       feats[wave][lane][6 + 2 * q + 1] = (float)d2[q].y * a.s_2;
     }
-    // k features as k_feature forms them, from this mode's wavenumbers
-    feats[wave][lane][12] = (float)fmin(a.dt * ((kx * kx + ky * ky) + kz * kz) / a.re, 8.0);
-    feats[wave][lane][13] = (float)(kx / a.kmax);
-    feats[wave][lane][14] = (float)(ky / a.kmax);
-    feats[wave][lane][15] = (float)(kz / a.kmax);
+    k_features(a, kx, ky, kz, &feats[wave][lane][12]);
   }
   __syncthreads();  // the weights are loaded by the whole workgroup
   if (use_net) {
@@ -559,6 +551,104 @@ __global__ void __launch_bounds__(256)
     rp[e] = mode == 0 ? nc[q] : cx<double>{nc[q].x + d.x, nc[q].y + d.y};
     reinterpret_cast<float2*>(h2)[e] = float2{(float)nc[q].x, (float)nc[q].y};
   }
+}
+
+// Wall-bounded guess: the nonlinear term fed to the predictor's solve,
+//   mode 0: N^n ; 1: N^n + D1 ; 3: N^n + D1 + D2 ;
+//   mode 2: N^n + D1 + D2 + s_out net(x) / w     (learned residual)
+// per point (kz, kx, y) of the (3, Nz1, Nx1, Ny) layout; h2 <- N^n.  One
+// lane per point; the network runs as in k_guess_periodic.
+template <int H>
+__global__ void __launch_bounds__(256)
+    k_extrap_walls(FeatArgs a, const cx<double>* __restrict__ n0, const float2* __restrict__ h1,
+                   float2* __restrict__ h2, cx<double>* __restrict__ next,
+                   const uint16_t* __restrict__ wb, const float* __restrict__ bias, int mode) {
+  constexpr int NT = 2 * (H / 16) + (H / 16) * (H / 16);
+  __shared__ v4s wt[NT * 64];
+  __shared__ float bs[2 * H + 16];
+  __shared__ float feats[4][64][kDin + 1];
+  __shared__ float outs[4][64][kDout + 1];
+  const bool use_net = mode == 2;
+  if (use_net) {
+    for (int i = threadIdx.x; i < NT * 64; i += blockDim.x) wt[i] = reinterpret_cast<const v4s*>(wb)[i];
+    for (int i = threadIdx.x; i < 2 * H + 16; i += blockDim.x) bs[i] = bias[i];
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = i < a.n;
+  const long long ii = live ? i : 0;
+  cx<double> nc[3];
+  double d1x[3], d1y[3], d2x[3], d2y[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const long long e = q * a.n + ii;
+    nc[q] = live ? n0[e] : cx<double>{0, 0};
+    const float2 q1 = live ? h1[e] : float2{0.f, 0.f};
+    const float2 q2 = live ? h2[e] : float2{0.f, 0.f};
+    diffs(nc[q].x, q1.x, q2.x, d1x[q], d2x[q]);
+    diffs(nc[q].y, q1.y, q2.y, d1y[q], d2y[q]);
+  }
+  double kx = 0, kb = 0, kz = 0;
+  if (use_net) {
+    coords(a, (unsigned)ii, kx, kb, kz);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      feats[wave][lane][2 * q] = (float)d1x[q] * a.s_1;
+      feats[wave][lane][2 * q + 1] = (float)d1y[q] * a.s_1;
+      feats[wave][lane][6 + 2 * q] = (float)d2x[q] * a.s_2;
+      feats[wave][lane][6 + 2 * q + 1] = (float)d2y[q] * a.s_2;
+    }
+    k_features(a, kx, kb, kz, &feats[wave][lane][12]);
+  }
+  __syncthreads();  // the weights are loaded by the whole workgroup
+  if (use_net) {
+    const int g = lane / 16, ml = lane % 16;
+#pragma unroll
+    for (int sb = 0; sb < 4; ++sb) {
+      const float* fr = feats[wave][sb * 16 + ml];
+      const v4f y = mlp_wave<H>(pack_b(fr[4 * g], fr[4 * g + 1], fr[4 * g + 2], fr[4 * g + 3]), wt, bs, lane);
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (4 * g + j < kDout) outs[wave][sb * 16 + ml][4 * g + j] = y[j];
+    }
+  }
+  // outs[wave] is written and read by this wavefront only
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  if (!live) return;
+  const double inv_w = 1.0 - a.dt * a.c * (-ksq(a, kx, kb, kz)) / a.re;  // 1 / w
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    cx<double> o = nc[q];
+    if (mode == 1) o = {nc[q].x + d1x[q], nc[q].y + d1y[q]};
+    else if (mode >= 2) o = {nc[q].x + (d1x[q] + d2x[q]), nc[q].y + (d1y[q] + d2y[q])};
+    if (use_net) {
+      o.x += (double)a.s_out * outs[wave][lane][2 * q] * inv_w;
+      o.y += (double)a.s_out * outs[wave][lane][2 * q + 1] * inv_w;
+    }
+    const long long e = q * a.n + i;
+    next[e] = o;
+    h2[e] = float2{(float)nc[q].x, (float)nc[q].y};
+  }
+}
+
+static FeatArgs feat_args(Ctx* c, float s_1, float s_2, float s_out);
+
+int extrap_walls(Ctx* c, const double* n0, const float* h1, float* h2, double* next,
+                 const uint16_t* wb, const float* bias, int H, float s_1, float s_2, float s_out,
+                 int mode, hipStream_t s) {
+  const FeatArgs a = feat_args(c, s_1, s_2, s_out);
+  if (a.n >= (1LL << 31)) return c->fail(DNSMI_EINVAL, "guess: too many points for 32-bit indexing");
+  const int nb = (int)((a.n + 255) / 256);
+  const cx<double>* n = reinterpret_cast<const cx<double>*>(n0);
+  const float2* p1 = reinterpret_cast<const float2*>(h1);
+  float2* p2 = reinterpret_cast<float2*>(h2);
+  cx<double>* o = reinterpret_cast<cx<double>*>(next);
+  if (mode != 2 || H == 32)
+    hipLaunchKernelGGL(k_extrap_walls<32>, dim3(nb), dim3(256), 0, s, a, n, p1, p2, o, wb, bias, mode);
+  else
+    hipLaunchKernelGGL(k_extrap_walls<64>, dim3(nb), dim3(256), 0, s, a, n, p1, p2, o, wb, bias, mode);
+  return check_launch(c, "walls_extrap");
 }
 
 // ------------------------------------------------------------------ launchers
@@ -622,14 +712,16 @@ int mlp_forward(int H, const float* X, int B, const uint16_t* wb, const float* b
 static FeatArgs feat_args(Ctx* c, float s_1, float s_2, float s_out) {
   FeatArgs a{};
   a.kx = c->kx;
-  a.ky = c->ky;
+  a.ky = c->ky;  // wall-bounded contexts keep the y grid here
   a.kz = c->kz;
   a.Nz1 = c->Nz1;
   a.Nx1 = c->Nx1;
-  a.n = (long long)c->Ny1 * c->Nz1 * c->Nx1;
+  a.Ny = c->ny;
+  a.walls = c->walls ? 1 : 0;
+  a.n = (long long)c->Ny1 * c->Nz1 * c->Nx1;  // walls: Ny1 = Ny points
   const double kxm = 2 * kPi / c->cfg.lx * (c->nx / 2), kym = 2 * kPi / c->cfg.ly * (c->ny / 2),
                kzm = 2 * kPi / c->cfg.lz * (c->nz / 2);
-  a.kmax = fmax(kxm, fmax(kym, kzm));
+  a.kmax = c->walls ? fmax(kxm, kzm) : fmax(kxm, fmax(kym, kzm));
   a.dt = c->cfg.dt;
   a.re = c->cfg.re;
   a.c = c->cfg.implicitness;
@@ -642,7 +734,6 @@ static FeatArgs feat_args(Ctx* c, float s_1, float s_2, float s_out) {
 int mlp_features(Ctx* c, const double* nm2, const double* nm1, const double* ncur,
                  const double* nnext, const long long* modes, int ns, float s_1, float s_2,
                  float s_out, float* X, float* T, hipStream_t s) {
-  if (c->walls) return c->fail(DNSMI_EINVAL, "features: periodic contexts only");
   const FeatArgs a = feat_args(c, s_1, s_2, s_out);
   if (ns == 0) return DNSMI_OK;
   hipLaunchKernelGGL(k_mlp_features, dim3((ns + 255) / 256), dim3(256), 0, s, a,

@@ -134,7 +134,8 @@ int guess_periodic(Ctx* c, const double* state, double* pred, double* rhs_prev, 
                    float* h2, const uint16_t* wb, const float* bias, int H, float s_1, float s_2,
                    float s_out, int mode, hipStream_t s);
 int guess_walls(Ctx* c, const double* state, double* pred, double* rhs_prev, const float* h1,
-                float* h2, int mode, hipStream_t s);
+                float* h2, const uint16_t* wb, const float* bias, int H, float s_1, float s_2,
+                float s_out, int mode, hipStream_t s);
 
 // A step started from the corrector guess (periodic): the guessed predictor,
 // then up to `slots` passes, each one RHS, with the same device rule as
@@ -150,7 +151,7 @@ int step_device_guess(Ctx* c, const double* state, double* pred, double* rhs_a, 
   double* err = &ctl->err;
   if (slots > max_pass) slots = max_pass;
   c->act = nullptr;
-  int rc = c->walls ? guess_walls(c, state, pred, rhs_a, h1, h2, mode, s)
+  int rc = c->walls ? guess_walls(c, state, pred, rhs_a, h1, h2, wb, bias, H, s_1, s_2, s_out, mode, s)
                     : guess_periodic(c, state, pred, rhs_a, h1, h2, wb, bias, H, s_1, s_2, s_out,
                                      mode, s);
   if (rc) return rc;

@@ -304,6 +304,8 @@ int walls_setup(Ctx* c) {
   std::vector<double> y(Ny);
   for (int j = 0; j < Ny; ++j) y[j] = -std::cos((double)j * kPi / (double)(Ny - 1));
   t->h_ys = y;
+  // the y grid in the ky slot: the learned guess's coordinate of a point
+  if ((rc = upload_doubles(c, &c->ky, y))) return rc;
   std::vector<double> base(3 * Ny, 0.0), cbase(3 * Ny, 0.0), nbase(3 * Ny, 0.0);
   for (int j = 0; j < Ny; ++j) {
     base[j] = y[j];
@@ -1046,44 +1048,29 @@ int walls_norm(Ctx* c, const double* corr, double* err, hipStream_t s) {
   return finalize_norm(c, nb, err, s);
 }
 
+int extrap_walls(Ctx* c, const double* n0, const float* h1, float* h2, double* next,
+                 const uint16_t* wb, const float* bias, int H, float s_1, float s_2, float s_out,
+                 int mode, hipStream_t s);
+
 // Wall-bounded corrector guess (no reference counterpart).  The walls
 // corrector's fixed point is u* = S(u^n; N(u*)) with S the influence-matrix
 // solve (the explicit term is the previous evaluation,
-// flows/plane_couette.py:534-539), so the guess is the predictor's solve fed
-// with an extrapolated nonlinear term N_ext = N^n + dN:
-//   mode 0: dN = 0 (the plain predictor), 1: D1, 3: D1 + D2,
-// D1, D2 the first and second differences of N over the FP32 history
-// (h1 = N^{n-1}, h2 = N^{n-2}); h2 <- N^n.  rhs_prev <- N_ext, and the
+// flows/plane_couette.py:534-539), so the guess is the predictor's solve
+// (flows/plane_couette.py:507-519) fed with an extrapolated nonlinear term
+// N_ext = N^n + dN (extrap_walls: dN = 0, D1, D1 + D2, or D1 + D2 plus the
+// network's residual, D1, D2 the differences of N over the FP32 history
+// h1 = N^{n-1}, h2 = N^{n-2}; h2 <- N^n).  rhs_prev <- N_ext, and the
 // reference's iterate_correction continues from (pred, rhs_prev).
-__global__ void __launch_bounds__(256)
-    k_walls_extrap(const cx<double>* __restrict__ n0, const float2* __restrict__ h1,
-                   float2* __restrict__ h2, cx<double>* __restrict__ next, long long n, int mode) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const cx<double> nc = n0[i];
-  const float2 q1 = h1[i], q2 = h2[i];
-  const double p1x = q1.x, p1y = q1.y, p2x = q2.x, p2y = q2.y;
-  const double d1x = nc.x - p1x, d1y = nc.y - p1y;
-  const double d2x = (nc.x - 2.0 * p1x) + p2x, d2y = (nc.y - 2.0 * p1y) + p2y;
-  cx<double> o = nc;
-  if (mode == 1) o = {nc.x + d1x, nc.y + d1y};
-  else if (mode == 3) o = {nc.x + (d1x + d2x), nc.y + (d1y + d2y)};
-  next[i] = o;
-  h2[i] = float2{(float)nc.x, (float)nc.y};
-}
-
 int guess_walls(Ctx* c, const double* state, double* pred, double* rhs_prev, const float* h1,
-                float* h2, int mode, hipStream_t s) {
-  if (mode != 0 && mode != 1 && mode != 3)
-    return c->fail(DNSMI_EINVAL, "guess: wall-bounded contexts take modes 0, 1 and 3");
+                float* h2, const uint16_t* wb, const float* bias, int H, float s_1, float s_2,
+                float s_out, int mode, hipStream_t s) {
+  if (mode < 0 || mode > 3) return c->fail(DNSMI_EINVAL, "guess: mode must be 0, 1, 2 or 3");
+  if (mode == 2 && H != 32 && H != 64)
+    return c->fail(DNSMI_EINVAL, "guess: hidden width must be 32 or 64");
   int rc = walls_get_rhs(c, state, c->rhs0, s);
   if (rc) return rc;
-  const long long n = c->spec_elems;
-  hipLaunchKernelGGL(k_walls_extrap, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
-                     reinterpret_cast<const cx<double>*>(c->rhs0),
-                     reinterpret_cast<const float2*>(h1), reinterpret_cast<float2*>(h2),
-                     reinterpret_cast<cx<double>*>(rhs_prev), n, mode);
-  if ((rc = check_launch(c, "walls_extrap"))) return rc;
+  if ((rc = extrap_walls(c, c->rhs0, h1, h2, rhs_prev, wb, bias, H, s_1, s_2, s_out, mode, s)))
+    return rc;
   return walls_predict(c, state, rhs_prev, pred, s);
 }
 

@@ -1,4 +1,4 @@
-"""Learned corrector guess for the triply periodic stepper.
+"""Learned corrector guess for the triply periodic and plane Couette steppers.
 
 No counterpart in the reference.  The reference starts every step's
 corrector from the Euler predictor ``u^1 = (u^n ldt_1 + N^n) ildt_2``
@@ -18,6 +18,13 @@ third difference.  The iteration map, the fixed point and the stopping
 rule are the reference's; only the starting point moves, so a good guess
 saves whole RHS evaluations (the plain scheme spends ``2 + c`` per step,
 the guessed one ``1 + passes``).
+
+Plane Couette (``flows/plane_couette.py:507-539``): the corrector's explicit
+term is the previous evaluation, so its fixed point is ``u* = S(u^n;
+N(u*))`` with ``S`` the influence-matrix solve.  The guess is the
+predictor's solve fed with ``N_ext = N^n + dN`` and ``rhs_prev = N_ext``,
+with the same ``dN`` per point (kz, kx, y) (y itself as the network's third
+coordinate).
 
 The network (``Net``) is trained on the GPU with the library's own FP32
 kernels on samples of the solver's nonlinear-term history (``collect``,
@@ -217,9 +224,9 @@ def run_guessed(flow, state, n_steps: int, mode: int = 2, net: Net | None = None
                 scales: Scales | None = None, callback=None) -> GuessRunResult:
     """The reference's time loop (``__main__.py:125-203``) with the corrector
     started from a guess: mode 1 linear, 3 quadratic extrapolation of N,
-    2 quadratic plus the network (periodic flows; plane Couette takes modes
-    0, 1 and 3, its guess being the predictor's solve fed with the
-    extrapolated N).  Mode 0 is the plain predictor: the same steps as the
+    2 quadratic plus the network (for plane Couette the guess is the
+    predictor's solve fed with the extrapolated N, and the network acts per
+    point (kz, kx, y)).  Mode 0 is the plain predictor: the same steps as the
     plain driver, counted as passes (= iterations + 1).  The first steps of a run lack history and
     start from the plain predictor (step 0) and linear extrapolation (step
     1).  Up to ``max_corrector_iterations + 1`` passes per step (the plain

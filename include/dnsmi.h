@@ -196,12 +196,15 @@ int dnsmi_mlp_forward(int H, const float* X, int B, const void* packed_weights, 
  * N^{n-2}) / s_out, w = 1 / (1 + c dt |k|^2 / Re): the residual of quadratic
  * extrapolation.  Features: D1 = N^n - N^{n-1} times s_1, D2 = N^n -
  * 2 N^{n-1} + N^{n-2} times s_2 (older terms rounded to FP32 as the step
- * history keeps them), min(dt |k|^2/Re, 8), k / k_max. */
+ * history keeps them), min(dt |k|^2/Re, 8), k / k_max.  Wall-bounded
+ * contexts: `modes` index points over (kz, kx, y); y itself replaces
+ * ky / k_max, |k|^2 = kx^2 + kz^2, and the weight is not zeroed at the mean
+ * mode. */
 int dnsmi_mlp_features(dnsmi_ctx* ctx, const double* n_m2, const double* n_m1,
                        const double* n_cur, const double* n_next, const long long* modes,
                        int nsamples, float s_1, float s_2, float s_out, float* X, float* T,
                        void* stream);
-/* Predictor with a guess, periodic contexts: N^n = rhs(state), u^1 =
+/* Predictor with a guess.  Periodic contexts: N^n = rhs(state), u^1 =
  * predict(state, N^n) (flows/monochromatic.py:320-333), then per mode
  * dN = 0 (mode 0), D1 (mode 1, linear extrapolation), D1 + D2 (mode 3,
  * quadratic) or D1 + D2 plus the network's residual (mode 2),
@@ -211,13 +214,18 @@ int dnsmi_mlp_features(dnsmi_ctx* ctx, const double* n_m2, const double* n_m1,
  * u <- ildt_2 (ldt_1 u^n + c N(u) + (1 - c) N^n), started from the guess.
  * hist1 = N^{n-1}, hist2 = N^{n-2}: 3 * spec_elems complex values each as
  * FP32 re/im pairs; hist2 is overwritten with N^n, so the caller swaps the
- * two for the next step.  Mode 0 (the plain predictor) needs no history. */
+ * two for the next step.  Mode 0 (the plain predictor) needs no history.
+ * Wall-bounded contexts: pred = the predictor's solve
+ * (flows/plane_couette.py:507-519) fed with N_ext = N^n + dN per point
+ * (same modes, with the network's residual divided by w = 1 / (1 + c dt
+ * (kx^2 + kz^2) / Re)) and rhs_prev = N_ext: the walls corrector's fixed
+ * point u* = S(u^n; N(u*)) is then reached from the guess. */
 int dnsmi_guess(dnsmi_ctx* ctx, const double* state, double* pred, double* rhs_prev,
                 const float* hist1, float* hist2, const void* packed_weights, const float* bias,
                 int H, float s_1, float s_2, float s_out, int mode, void* stream);
 
 /* One time step started from the corrector guess, decided on the device
- * (periodic contexts): dnsmi_guess from `state` into `next` (with rhs_a as
+ * (periodic and wall-bounded): dnsmi_guess from `state` into `next` (with rhs_a as
  * rhs_prev), then up to `slots` corrector passes, each skipped on the device
  * once error <= tol, max_pass passes ran, or the noise-floor rule of
  * dnsmi_step_device fires; then correct_velocity(next).  counts[i] receives

@@ -16,13 +16,18 @@ GPU inference kernel feeds its matrix cores.
 ``features`` and ``guess`` restate the corrector-guess step built on the
 network for the periodic stepper (no reference counterpart; it reuses the
 reference's predictor ``flows/monochromatic.py:320-333`` and the Leray
-projection ``flows/monochromatic.py:516-565``).
+projection ``flows/monochromatic.py:516-565``); ``features_walls`` and
+``guess_walls`` the same for plane Couette, per point (kz, kx, y) (it reuses
+the predictor's solve ``flows/plane_couette.py:507-519``).
 
 Pinned by: central finite differences of the loss (tests/test_oracle_mlp.py),
 which fix every gradient independently of the backward formulas; for
 ``guess``: mode 0 reduces to the reference predictor exactly, and the
 fixed-point identity (the corrector pass from the guess equals the plain
-iteration map, tests/test_oracle_mlp.py) holds for any dN.
+iteration map, tests/test_oracle_mlp.py) holds for any dN; for the wall
+features: exact targets on polynomial histories (the residual of quadratic
+extrapolation vanishes for quadratic and equals the third difference for
+cubic histories) and coordinates read back from a field built from them.
 """
 
 from __future__ import annotations
@@ -181,11 +186,54 @@ def guess(state, h1, h2, flow, mode, params=None, scales=(1.0, 1.0, 1.0)):
     return pred, n0 + d, n0.copy()
 
 
-def guess_walls(state, h1, h2, flow, mode):
+def mode_weight_walls(flow):
+    """w = 1 / (1 + c dt (kx^2 + kz^2) / Re) per point (kz, kx, y)."""
+    p = flow.p
+    w = 1.0 / (1.0 - p.dt * p.implicitness * (-(flow.kx**2 + flow.kz**2)) / p.re)
+    return np.broadcast_to(w, (p.nz - 1, p.nx // 2, p.ny)).copy()
+
+
+def k_features_walls(flow):
+    """Features 12..15 per point: min(dt (kx^2 + kz^2)/Re, 8), kx/kmax, y, kz/kmax."""
+    p = flow.p
+    kmax = max(2 * np.pi / p.lx * (p.nx // 2), 2 * np.pi / p.lz * (p.nz // 2))
+    shape = (p.nz - 1, p.nx // 2, p.ny)
+    k2 = flow.kx**2 + flow.kz**2
+    return [np.broadcast_to(np.minimum(p.dt * k2 / p.re, 8.0), shape),
+            np.broadcast_to(flow.kx / kmax, shape), np.broadcast_to(flow.ys.reshape(1, 1, -1), shape),
+            np.broadcast_to(flow.kz / kmax, shape)]
+
+
+def features_walls(n_m2, n_m1, n_cur, n_next, points, flow, s_1, s_2, s_out):
+    """X (S,16), T (S,6) for flat point indices over (kz, kx, y), as
+    ``features`` with the wall-bounded coordinates and weight."""
+    f32 = lambda a: a.astype(np.float32).astype(np.float64)  # noqa: E731
+    X = np.zeros((len(points), 16))
+    for q in range(6):
+        comp = (lambda a: a.real) if q % 2 == 0 else (lambda a: a.imag)
+        cur = comp(n_cur[q // 2].reshape(-1)[points])
+        p1 = f32(comp(n_m1[q // 2].reshape(-1)[points]))
+        p2 = f32(comp(n_m2[q // 2].reshape(-1)[points]))
+        X[:, q] = (cur - p1) * s_1
+        X[:, 6 + q] = ((cur - 2 * p1) + p2) * s_2
+    for j, kf in enumerate(k_features_walls(flow)):
+        X[:, 12 + j] = kf.reshape(-1)[points]
+    T = None
+    if n_next is not None:
+        w = mode_weight_walls(flow).reshape(-1)[points]
+        T = np.zeros((len(points), 6))
+        for q in range(6):
+            d3 = (n_next[q // 2] - 3 * n_cur[q // 2] + 3 * n_m1[q // 2] - n_m2[q // 2]).reshape(-1)[points]
+            T[:, q] = w * (d3.real if q % 2 == 0 else d3.imag) / s_out
+    return X, T
+
+
+def guess_walls(state, h1, h2, flow, mode, params=None, scales=(1.0, 1.0, 1.0)):
     """(pred, rhs_prev, N^n) of the wall-bounded guess: the predictor's solve
     (``flows/plane_couette.py:507-519``) fed with N_ext = N^n + dN, dN = 0
-    (mode 0), D1 (mode 1) or D1 + D2 (mode 3); rhs_prev = N_ext.  h1, h2
-    are N^{n-1}, N^{n-2} rounded to FP32 by the caller."""
+    (mode 0), D1 (mode 1), D1 + D2 (mode 3) or D1 + D2 + s_out net / w
+    (mode 2, bf16 inference); rhs_prev = N_ext.  h1, h2 are N^{n-1}, N^{n-2}
+    rounded to FP32 by the caller."""
     from . import walls as W
 
     n0 = W.get_rhs(state, flow)
@@ -194,4 +242,13 @@ def guess_walls(state, h1, h2, flow, mode):
     else:
         d1 = n0 - h1
         ext = n0 + (d1 if mode == 1 else d1 + ((n0 - 2 * h1) + h2))
+    if mode == 2:
+        s_1, s_2, s_out = scales
+        w = mode_weight_walls(flow)
+        pts = np.arange(w.size)
+        X, _ = features_walls(h2, h1, n0, None, pts, flow, s_1, s_2, s_out)
+        Y, _ = forward(params, X, bf16=True)
+        for c in range(3):
+            net = (Y[:, 2 * c] + 1j * Y[:, 2 * c + 1]).reshape(w.shape)
+            ext[c] = ext[c] + s_out * net / w
     return W.predict(state, ext, flow), ext, n0.copy()

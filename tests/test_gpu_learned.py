@@ -214,7 +214,41 @@ def _walls(nx, ny, nz, precision="fp64", dt=0.005):
                                    max_corrector_iterations=20, precision=precision, tile_cols=16))
 
 
-@pytest.mark.parametrize("mode", [0, 1, 3])
+def _features_on_device(flow, seq, idx, scales):
+    nat = flow._native
+    X = torch.empty((len(idx), 16), dtype=torch.float32, device=nat.device)
+    T = torch.empty((len(idx), 6), dtype=torch.float32, device=nat.device)
+    nat.mlp_features(*[flow.to_device(a) for a in seq], torch.from_numpy(idx).to(nat.device), *scales, X, T)
+    return X.cpu().numpy().astype(np.float64), T.cpu().numpy().astype(np.float64)
+
+
+def test_features_match_oracle_periodic_and_walls():
+    """Training samples (features and weighted third-difference targets) of
+    both layouts against the oracle, at every mode / point."""
+    from oracle import walls as W
+    from dnsjax_amd.fields import periodic_field, wall_field
+
+    n, L = 16, 2 * math.pi
+    flow = _periodic(n, 40.0)
+    of = P.Flow(P.PeriodicParams(nx=n, ny=n, nz=n, lx=L, ly=L, lz=L, re=40.0, dt=0.01, system="kolmogorov"))
+    seq = [P.get_rhs(periodic_field(n, n, n, L, L, L, 0.1, s), of) for s in (3, 4, 5, 6)]
+    idx = np.arange(seq[0][0].size)
+    sc = (2.0, 3.0, 0.25)
+    X, T = _features_on_device(flow, seq, idx, sc)
+    wX, wT = M.features(*seq, idx, of, *sc)
+    assert np.allclose(X, wX, rtol=2e-6, atol=2e-6 * np.max(np.abs(wX)))
+    assert np.allclose(T, wT, rtol=2e-6, atol=2e-6 * np.max(np.abs(wT)))
+    wflow = _walls(8, 17, 8, dt=0.01)
+    wf = W.WallFlow(W.WallParams(nx=8, ny=17, nz=8, fd_order=4, re=400.0, dt=0.01))
+    wseq = [W.get_rhs(wall_field(8, 17, 8, wf.p.lx, wf.p.lz, 0.05, seed=s), wf) for s in (3, 4, 5, 6)]
+    widx = np.arange(wseq[0][0].size)
+    X, T = _features_on_device(wflow, wseq, widx, sc)
+    wX, wT = M.features_walls(*wseq, widx, wf, *sc)
+    assert np.allclose(X, wX, rtol=2e-6, atol=2e-6 * np.max(np.abs(wX)))
+    assert np.allclose(T, wT, rtol=2e-6, atol=2e-6 * np.max(np.abs(wT)))
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
 def test_walls_guess_matches_oracle(mode):
     from oracle import walls as W
     from dnsjax_amd.fields import wall_field
@@ -224,15 +258,27 @@ def test_walls_guess_matches_oracle(mode):
     u = wall_field(8, 17, 8, of.p.lx, of.p.lz, 0.05, seed=3)
     h1 = W.get_rhs(wall_field(8, 17, 8, of.p.lx, of.p.lz, 0.05, seed=4), of).astype(np.complex64)
     h2 = W.get_rhs(wall_field(8, 17, 8, of.p.lx, of.p.lz, 0.05, seed=5), of).astype(np.complex64)
+    params = _random_params(32, 2)
+    scales = (1.0, 3.0, 0.05)
     want_pred, want_rhs, want_hist = M.guess_walls(u, h1.astype(np.complex128), h2.astype(np.complex128), of,
-                                                   mode)
+                                                   mode, params, scales)
     nat = flow._native
     dev = lambda h: torch.from_numpy(np.stack([h.real, h.imag], -1).astype(np.float32)).to(nat.device)  # noqa: E731
     d1, d2 = dev(h1).contiguous(), dev(h2).contiguous()
     pred, rhs = nat.empty(), nat.empty()
-    nat.guess(flow.to_device(u), pred, rhs, d1, d2, mode)
-    assert rel(rhs.cpu().numpy(), want_rhs) <= 1e-12
-    assert rel(pred.cpu().numpy(), want_pred) <= 1e-11  # IMM tolerance
+    net = _net_with(32, params)
+    net.pack()
+    nat.guess(flow.to_device(u), pred, rhs, d1, d2, mode, net.operands, scales)
+    if mode == 2:  # bf16 network: the residual it adds is held to bf16 accuracy
+        n0 = W.get_rhs(u, of)
+        _, ext3, _ = M.guess_walls(u, h1.astype(np.complex128), h2.astype(np.complex128), of, 3)
+        assert rel(rhs.cpu().numpy() - ext3, want_rhs - ext3) <= 5e-3
+        assert rel(rhs.cpu().numpy(), want_rhs) <= 1e-4
+        assert rel(pred.cpu().numpy(), want_pred) <= 1e-4
+        assert np.max(np.abs(want_rhs - ext3)) > 1e-3 * np.max(np.abs(n0))  # the net did contribute
+    else:
+        assert rel(rhs.cpu().numpy(), want_rhs) <= 1e-12
+        assert rel(pred.cpu().numpy(), want_pred) <= 1e-11  # IMM tolerance
     h = d2.cpu().numpy().astype(np.float64)
     w = np.stack([want_hist.real, want_hist.imag], -1)
     assert np.max(np.abs(h - w)) <= 1e-7 * np.max(np.abs(w))
@@ -266,3 +312,39 @@ def test_walls_guessed_runs_and_graph_steps():
         torch.cuda.synchronize()
         assert torch.equal(gs.state, ref[i]), f"step {i} differs"
     assert gs.passes() == g3.passes
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_walls_learned_runs_and_graph_steps(precision):
+    """Plane Couette 16 x 33 x 16 with a network trained on its own history:
+    the learned run stays on the plain trajectory to the tolerance and its
+    device-decided graph steps equal the eager loop bit for bit."""
+    from dnsjax_amd.driver import run
+    from dnsjax_amd.fields import wall_field
+    from dnsjax_amd.learned import (GuessedGraphStepper, Net, collect, fit_scales, make_dataset, mean_loss,
+                                    run_guessed, train)
+
+    flow = _walls(16, 33, 16, precision=precision)
+    u0 = flow.to_device(wall_field(16, 33, 16, 2 * math.pi, math.pi, 0.05, seed=2))
+    snaps, u_mid = collect(flow, u0.clone(), 10)
+    sc = fit_scales(snaps)
+    X, T = make_dataset(flow, snaps, sc, 8000, seed=1)
+    net = Net(32, seed=0)
+    train(net, X, T, iters=200, batch=2048, lr=3e-3)
+    loss, zero = mean_loss(net, X, T)
+    assert loss < zero  # the network explains part of the quadratic residual
+    n = 8
+    plain = run(flow, u_mid.clone(), n)
+    ref = []
+    g2 = run_guessed(flow, u_mid.clone(), n, mode=2, net=net, scales=sc, callback=lambda i, s: ref.append(s.clone()))
+    assert plain.converged and g2.converged and len(g2.passes) == n
+    assert rel(g2.state.cpu().numpy(), plain.state.cpu().numpy()) <= (1e-8 if precision == "fp64" else 1e-6)
+    gs = GuessedGraphStepper(flow, u_mid, mode=2, net=net, scales=sc)
+    torch.cuda.synchronize()
+    assert torch.equal(gs.state, ref[1])
+    for i in range(2, n):
+        gs.step(1)
+        torch.cuda.synchronize()
+        assert torch.equal(gs.state, ref[i]), f"step {i} differs"
+    assert gs.passes() == g2.passes
+

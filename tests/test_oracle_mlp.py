@@ -155,3 +155,71 @@ def test_walls_guess_mode0_and_corrector_map():
         want_ext = n0 + (d1 if mode == 1 else 2 * d1 - (h1 - h2))
         assert np.allclose(ext, want_ext, rtol=0, atol=1e-13 * np.max(np.abs(n0)))
         assert np.array_equal(g, W.predict(u, ext, flow))
+
+
+def _wall_flow():
+    from oracle import walls as W
+
+    return W.WallFlow(W.WallParams(nx=8, ny=17, nz=8, fd_order=4, re=400.0, dt=0.01))
+
+
+def test_wall_features_on_polynomial_histories():
+    """Quadratic histories leave no residual; a cubic one leaves exactly its
+    third difference, 6 e per unit step, weighted by w.  The coordinate
+    features read back the point's (kx, y, kz), found here by index
+    arithmetic over (kz, kx, y)."""
+    from dnsjax_amd.fields import wall_field
+    from oracle import walls as W
+
+    flow = _wall_flow()
+    p = flow.p
+    f = lambda s: _f32(W.get_rhs(wall_field(8, 17, 8, p.lx, p.lz, 0.05, seed=s), flow))  # noqa: E731
+    a, b, e = f(3), f(9) * 0.25, f(11) * 0.125
+    pts = np.arange(a[0].size)
+    quad = [a + k * b + k * k * e for k in range(4)]
+    X, T = M.features_walls(*quad, pts, flow, 1.0, 1.0, 1.0)
+    scale = np.max(np.abs(b))
+    assert np.max(np.abs(T)) <= 1e-12 * scale
+    assert np.allclose(X[:, 0], (b[0] + 3 * e[0]).real.reshape(-1), atol=1e-6 * scale)
+    assert np.allclose(X[:, 7], (2 * e[0]).imag.reshape(-1), atol=1e-6 * scale)
+    cub = [a + k * k * k * e for k in range(4)]
+    _, T3 = M.features_walls(*cub, pts, flow, 1.0, 1.0, 2.0)
+    iz, rest = np.divmod(pts, (p.nx // 2) * p.ny)
+    ix, iy = np.divmod(rest, p.ny)
+    kx = 2 * np.pi / p.lx * ix
+    kz = 2 * np.pi / p.lz * np.where(iz < p.nz // 2, iz, iz - (p.nz - 1))
+    w = 1.0 / (1.0 + p.implicitness * p.dt * (kx**2 + kz**2) / p.re)
+    want = w * (6 * e[1]).imag.reshape(-1) / 2.0
+    assert np.allclose(T3[:, 3], want, rtol=1e-12, atol=1e-12 * np.max(np.abs(want)))
+    kmax = max(2 * np.pi / p.lx * (p.nx // 2), 2 * np.pi / p.lz * (p.nz // 2))
+    ys = -np.cos(np.pi * iy / (p.ny - 1))
+    assert np.allclose(X[:, 13], kx / kmax) and np.allclose(X[:, 14], ys) and np.allclose(X[:, 15], kz / kmax)
+    assert np.allclose(X[:, 12], np.minimum(p.dt * (kx**2 + kz**2) / p.re, 8.0))
+
+
+def test_wall_guess_mode2_adds_the_net_residual():
+    """Mode 2 = mode 3 plus s_out * net / w, with the bf16 network on the
+    point features; a zero output layer reproduces mode 3 exactly."""
+    from dnsjax_amd.fields import wall_field
+    from oracle import walls as W
+
+    flow = _wall_flow()
+    p = flow.p
+    u = wall_field(8, 17, 8, p.lx, p.lz, 0.05, seed=3)
+    h1 = _f32(W.get_rhs(wall_field(8, 17, 8, p.lx, p.lz, 0.05, seed=4), flow))
+    h2 = _f32(W.get_rhs(wall_field(8, 17, 8, p.lx, p.lz, 0.05, seed=5), flow))
+    params = M.init(16, 32, 6, seed=1)
+    zero = [q.copy() for q in params]
+    zero[4][:] = 0.0
+    g3, ext3, _ = M.guess_walls(u, h1, h2, flow, 3)
+    g0, ext0, _ = M.guess_walls(u, h1, h2, flow, 2, zero, (1.0, 1.0, 1.0))
+    assert np.array_equal(ext0, ext3) and np.array_equal(g0, g3)
+    s = (2.0, 3.0, 0.5)
+    g2, ext2, _ = M.guess_walls(u, h1, h2, flow, 2, params, s)
+    n0 = W.get_rhs(u, flow)
+    X, _ = M.features_walls(h2, h1, n0, None, np.arange(n0[0].size), flow, *s)
+    Y, _ = M.forward(params, X, bf16=True)
+    w = M.mode_weight_walls(flow)
+    for c in range(3):
+        want = ext3[c] + s[2] * (Y[:, 2 * c] + 1j * Y[:, 2 * c + 1]).reshape(w.shape) / w
+        assert np.allclose(ext2[c], want, rtol=1e-14, atol=1e-14)
