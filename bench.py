@@ -312,29 +312,50 @@ def learned_form(steps: int, collect_steps: int = 30, hidden: int = 32, train_it
             "segment": f"{steps} steps after a {collect_steps}-step training run from the workload's initial state"}
 
 
-def walls_form(steps: int, warmup: int = 2) -> dict:
+def walls_form(steps: int, warmup: int = 2, collect_steps: int = 30, train_iters: int = 1500,
+               tol: float = 1e-5) -> dict:
     """Plane Couette 64 x 128 x 64 (CGL points in y), Lx = 2 pi, Lz = pi,
-    Re 400, fd_order 4, dt 0.01, seeded perturbation RMS 0.05 (seed 1), the
-    reference's default corrector tolerance (1e-5) and limit (10).
-    Time-steps/s of the driver loop for the plain FP64 method (with
-    per-kernel event times), the mixed form, and the mixed form started from
-    the extrapolated guess (mode 3), all over the same segment; energy of
-    each end state against the FP64 run's."""
+    Re 400, fd_order 4, dt 0.01, seeded perturbation RMS 0.05 (seed 1),
+    corrector tolerance `tol` (the reference's default is 1e-5) and limit 10.
+
+    After `warmup` FP64 steps, a `collect_steps` mixed run records the
+    nonlinear-term history on which the per-point network is trained (set-up,
+    timed separately).  From the state that run ends in, time-steps/s of the
+    driver loop are timed over the same segment for the plain FP64 method
+    (with per-kernel event times), the mixed form, and the mixed form started
+    from the extrapolated guess (mode 3) and from the learned guess (mode 2);
+    energy of each end state against the FP64 run's."""
     import torch
 
     from dnsjax_amd.driver import run
     from dnsjax_amd.flows import PlaneCouetteFlow
-    from dnsjax_amd.learned import run_guessed
+    from dnsjax_amd.learned import Net, collect, fit_scales, make_dataset, mean_loss, run_guessed, train
     from dnsjax_amd.params import Params
 
     def mk(prec):
         return PlaneCouetteFlow(Params(system="plane-couette", nx=64, ny=128, nz=64, lx=2 * math.pi, lz=math.pi,
-                                       re=400.0, dt=0.01, fd_order=4, corrector_tolerance=1e-5,
+                                       re=400.0, dt=0.01, fd_order=4, corrector_tolerance=tol,
                                        max_corrector_iterations=10, precision=prec))
 
     flow, mixed = mk("fp64"), mk("mixed")
     state = flow.init_state(seed=SEED, rms=0.05)
     state = run(flow, state, warmup).state
+    t0 = time.perf_counter()
+    snaps, state = collect(mixed, state, collect_steps)
+    sc = fit_scales(snaps)
+    X, T = make_dataset(mixed, snaps, sc, 100000, seed=1)
+    nv = 100000
+    Xv, Tv, X, T = X[-nv:], T[-nv:], X[:-nv], T[:-nv]
+    del snaps
+    torch.cuda.synchronize()
+    t_data = time.perf_counter() - t0
+    net = Net(32, seed=0)
+    t0 = time.perf_counter()
+    train(net, X, T, iters=train_iters, batch=16384, lr=3e-3)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    loss, base = mean_loss(net, Xv, Tv)
+    del X, T, Xv, Tv
 
     def timed(fn):
         torch.cuda.synchronize()
@@ -346,7 +367,8 @@ def walls_form(steps: int, warmup: int = 2) -> dict:
     # one untimed pass of each form first (allocator growth, module loads)
     run(flow, state.clone(), 2)
     run(mixed, state.clone(), 2)
-    run_guessed(mixed, state.clone(), 3, mode=3)
+    for mode in (3, 2):
+        run_guessed(mixed, state.clone(), 3, mode=mode, net=net, scales=sc)
     r, el = timed(lambda: run(flow, state.clone(), steps))
     flow._native.profile_begin()
     r2 = run(flow, state.clone(), steps)
@@ -355,11 +377,13 @@ def walls_form(steps: int, warmup: int = 2) -> dict:
     e_ref = float(flow.get_stats(r.state)["E'"])
     rm, tm = timed(lambda: run(mixed, state.clone(), steps))
     rg, tg = timed(lambda: run_guessed(mixed, state.clone(), steps, mode=3))
+    rl, tl = timed(lambda: run_guessed(mixed, state.clone(), steps, mode=2, net=net, scales=sc))
 
     def err(st):
         return abs(float(flow.get_stats(st)["E'"]) - e_ref) / abs(e_ref)
 
-    return {"workload": "plane Couette 64x128x64, p=4, Re 400, dt 0.01, RMS 0.05, tol 1e-5 (reference default)",
+    return {"workload": f"plane Couette 64x128x64, p=4, Re 400, dt 0.01, RMS 0.05, tol {tol:g}"
+                        + (" (reference default)" if tol == 1e-5 else ""),
             "steps_per_s": round(steps / el, 3), "ms_per_step": round(1e3 * el / steps, 4),
             "corrector_iters_per_step": round(sum(r.counts) / max(len(r.counts), 1), 3),
             "converged": r.converged and r2.converged,
@@ -370,7 +394,16 @@ def walls_form(steps: int, warmup: int = 2) -> dict:
                             "converged": rg.converged, "energy_rel_err_vs_fp64": err(rg.state),
                             "speedup_vs_fp64": round(el / tg, 3),
                             "guess": "quadratic extrapolation of N fed to the predictor's solve"},
-            "fp64_rhs_evals_per_step": round(r.rhs_evals / steps, 3)}
+            "mixed_learned": {"steps_per_s": round(steps / tl, 3),
+                              "rhs_evals_per_step": round(rl.rhs_evals / steps, 3),
+                              "converged": rl.converged, "energy_rel_err_vs_fp64": err(rl.state),
+                              "speedup_vs_fp64": round(el / tl, 3),
+                              "guess": "quadratic extrapolation plus the per-point network's residual",
+                              "net": {"hidden": 32, "train_iters": train_iters, "train_s": round(t_train, 3),
+                                      "data_s": round(t_data, 3), "heldout_loss": round(loss, 4),
+                                      "heldout_loss_without_net": round(base, 4)}},
+            "fp64_rhs_evals_per_step": round(r.rhs_evals / steps, 3),
+            "segment": f"{steps} steps after a {collect_steps}-step mixed data run from the workload's state"}
 
 
 def ensemble_form(steps: int, n_streams: int = 8) -> dict:
@@ -521,6 +554,17 @@ def main():
     }
     if a.walls_steps > 0:
         out["walls"] = walls_form(a.walls_steps)
+        out["walls_mixed_learned_vs_fp64"] = out["walls"]["mixed_learned"]["speedup_vs_fp64"]
+        # the same forms where the tolerance no longer sets the energy
+        # difference (the wall corrector contracts ~2x per pass, so at 1e-5
+        # a step stopping at the first pass differs by O(tol) from one that
+        # paid the next pair of passes)
+        wt = walls_form(a.walls_steps, tol=1e-7)
+        out["walls_tol_1e-7"] = {
+            "fp64_steps_per_s": wt["steps_per_s"], "fp64_rhs_evals_per_step": wt["fp64_rhs_evals_per_step"],
+            **{f: {k: wt[f][k] for k in ("steps_per_s", "rhs_evals_per_step", "energy_rel_err_vs_fp64",
+                                         "converged")} for f in ("mixed", "mixed_guess", "mixed_learned")},
+            "mixed_learned_vs_fp64": wt["mixed_learned"]["speedup_vs_fp64"]}
     if a.learned_steps > 0:
         lf = learned_form(a.learned_steps)
         out["mixed_learned"] = lf
