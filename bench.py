@@ -246,9 +246,6 @@ def learned_form(steps: int, collect_steps: int = 30, hidden: int = 32, train_it
         torch.cuda.synchronize()
         return r, time.perf_counter() - t
 
-    def energy(flow, state):
-        return float(flow.get_stats(state)["E'"])
-
     # one untimed pass of each form first (allocator growth, module loads)
     run(fp64, u_mid.clone(), 2)
     run(mixed, u_mid.clone(), 2)
@@ -256,7 +253,6 @@ def learned_form(steps: int, collect_steps: int = 30, hidden: int = 32, train_it
         run_guessed(mixed, u_mid.clone(), 3, mode=mode, net=net, scales=sc)
     rows = {}
     ref, t_ref = timed(lambda: run(fp64, u_mid.clone(), steps))
-    e_ref = energy(fp64, ref.state)
     rows["fp64_plain"] = (t_ref, ref.rhs_evals, ref.converged, ref.state, steps)
     r, t = timed(lambda: run(mixed, u_mid.clone(), steps))
     rows["mixed_plain"] = (t, r.rhs_evals, r.converged, r.state, steps)
@@ -286,19 +282,26 @@ def learned_form(steps: int, collect_steps: int = 30, hidden: int = 32, train_it
         rows["mixed_learned_graph" + tag] = (t, sum(c + 1 for c in gg.passes()[4:]),
                                              gg.step_stats()["unconverged"] == 0, gg.state, gg.steps)
     # The graph steppers' end states include their start-up and warm-up
-    # steps: each is compared with an eager FP64 run of the same length.
-    e_refs = {steps: e_ref}
+    # steps: each is compared with an eager FP64 run of the same length
+    # (energy E', dissipation D and the velocity field).
+    refs = {steps: ref.state}
 
-    def e_at(n):
-        if n not in e_refs:
-            e_refs[n] = energy(fp64, run(fp64, u_mid.clone(), n).state)
-        return e_refs[n]
+    def ref_at(n):
+        if n not in refs:
+            refs[n] = run(fp64, u_mid.clone(), n).state
+        return refs[n]
+
+    def errors(st, n):
+        r = ref_at(n)
+        a, b = fp64.get_stats(st), fp64.get_stats(r)
+        rel = lambda k: abs(float(a[k]) - float(b[k])) / abs(float(b[k]))  # noqa: E731
+        return {"energy_rel_err_vs_fp64": rel("E'"), "dissipation_rel_err_vs_fp64": rel("D"),
+                "velocity_rel_l2_vs_fp64": float(torch.linalg.vector_norm(st - r) / torch.linalg.vector_norm(r))}
 
     out = {}
     for name, (t, rhs, conv, st, n) in rows.items():
         out[name] = {"steps_per_s": round(steps / t, 3), "ms_per_step": round(1e3 * t / steps, 4),
-                     "rhs_evals_per_step": round(rhs / steps, 3), "converged": conv,
-                     "energy_rel_err_vs_fp64": abs(energy(fp64, st) - e_at(n)) / abs(e_at(n))}
+                     "rhs_evals_per_step": round(rhs / steps, 3), "converged": conv, **errors(st, n)}
     sp = out["mixed_learned"]["steps_per_s"] / out["fp64_plain"]["steps_per_s"]
     spg = out["mixed_learned_graph"]["steps_per_s"] / out["fp64_plain_graph"]["steps_per_s"]
     spg4 = out["mixed_learned_graph_4slots"]["steps_per_s"] / out["fp64_plain_graph_4slots"]["steps_per_s"]
