@@ -138,6 +138,7 @@ int dnsmi_create(const dnsmi_config* cfg, dnsmi_ctx** out) {
       c->yinv_seq = (cfg->variants & 1) != 0;
       c->zinv_seq = (cfg->variants & 2) != 0;
       c->xpass_split = (cfg->variants & 4) != 0;
+      c->sweep_seq = (cfg->variants & 8) != 0;
     }
     if (cfg->system == DNSMI_PLANE_COUETTE) {
       c->walls = true;

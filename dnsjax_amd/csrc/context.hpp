@@ -77,6 +77,7 @@ struct Ctx {
   bool xpass_split = false;
   bool yinv_seq = false;
   bool zinv_seq = false;
+  bool sweep_seq = false;  // wall-bounded: sequential band sweeps instead of chunk maps
   std::string tuning;
 
   // per-kernel timing (off unless dnsmi_profile_begin was called)

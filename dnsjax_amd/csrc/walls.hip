@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <cmath>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "context.hpp"
@@ -56,6 +57,9 @@ struct WallTables {
   double* mkx = nullptr;   // per mode
   double* mkz = nullptr;
   double* scratch = nullptr;  // IMM work vectors, complex [vec][mode][y]
+  // chunk maps of the band solves (chunked sweeps): forward [c][k][mode][t]
+  // (WF x WF per chunk and mode), back [c][k][mode][t] (WB x WB)
+  double *lk_fmap = nullptr, *lk_bmap = nullptr, *hk_fmap = nullptr, *hk_bmap = nullptr;
   // host copies in the reference layouts, for inspection by tests
   std::vector<double> h_lk_ab, h_hk_ab, h_v1, h_q1, h_minv, h_ys;
   std::vector<int> h_lk_piv, h_hk_piv;
@@ -193,6 +197,55 @@ static void band_solve(const Band& B, std::vector<double>& b) {
   }
 }
 
+// Rows per chunk of the chunked band solve.
+constexpr int kChunk = 16;
+
+// Chunk maps of one mode's band solve (the operation order of band_solve):
+// forward chunk c applies the steps j in [cB, min(cB + B, n - 1)) (row swap,
+// elimination) to the window of rows cB + t, t < WF = B + kl; its map F_c
+// (out_t = sum_k F_c[t][k] w_k) is those row operations applied to the
+// identity.  Back chunk c, top row T = n - 1 - cB, applies the steps
+// j = T, T - 1, ... (at most B, j >= 0: divide by the diagonal, eliminate
+// upwards) to the window of rows T - t, t < WB = B + kl + ku.  Stored at
+// [((c W + k) M + m) W + t] so that the lanes t of a mode read consecutive
+// words.  `raw(r, j)` reads the factored band, `piv(j)` the pivot row.
+template <typename Raw, typename Piv>
+static void chunk_maps(int n, int kl, int ku, int m, int M, Raw raw, Piv piv, double* F, double* G) {
+  const int B = kChunk, kuu = kl + ku, WF = B + kl, WB = B + kuu;
+  const int nC = (n + B - 1) / B;
+  std::vector<double> X((size_t)WB * WB);
+  for (int c = 0; c < nC; ++c) {
+    const int j0 = c * B;
+    std::fill(X.begin(), X.end(), 0.0);
+    for (int t = 0; t < WF; ++t) X[(size_t)t * WF + t] = 1.0;
+    for (int j = j0; j < std::min(j0 + B, n - 1); ++j) {
+      const int tj = j - j0, tp = piv(j) - j0;
+      if (tp != tj)
+        for (int k = 0; k < WF; ++k) std::swap(X[(size_t)tj * WF + k], X[(size_t)tp * WF + k]);
+      for (int i = 1; i <= kl && j + i < n; ++i) {
+        const double l = raw(kl + ku + i, j);
+        for (int k = 0; k < WF; ++k) X[(size_t)(tj + i) * WF + k] -= l * X[(size_t)tj * WF + k];
+      }
+    }
+    for (int k = 0; k < WF; ++k)
+      for (int t = 0; t < WF; ++t) F[(((size_t)c * WF + k) * M + m) * WF + t] = X[(size_t)t * WF + k];
+    const int T = n - 1 - j0;
+    std::fill(X.begin(), X.end(), 0.0);
+    for (int t = 0; t < WB; ++t) X[(size_t)t * WB + t] = 1.0;
+    for (int j = T; j > T - B && j >= 0; --j) {
+      const int tj = T - j;
+      const double dg = raw(kl + ku, j);
+      for (int k = 0; k < WB; ++k) X[(size_t)tj * WB + k] /= dg;
+      for (int i = 1; i <= kuu && j - i >= 0; ++i) {
+        const double u = raw(kuu - i, j);
+        for (int k = 0; k < WB; ++k) X[(size_t)(tj + i) * WB + k] -= u * X[(size_t)tj * WB + k];
+      }
+    }
+    for (int k = 0; k < WB; ++k)
+      for (int t = 0; t < WB; ++t) G[(((size_t)c * WB + k) * M + m) * WB + t] = X[(size_t)t * WB + k];
+  }
+}
+
 // Linear weights of the y quadrature, divided by Ly.  Odd Ny: the
 // non-uniform composite Simpson rule (geometries/cartesian.py:125-140).
 // Even Ny (the reference exits there): Simpson panels over the first Ny-4
@@ -247,6 +300,8 @@ static int upload_vec(Ctx* c, V** dst, const std::vector<V>& v) {
   if (e != hipSuccess) return c->fail(DNSMI_EHIP, hipGetErrorString(e));
   return DNSMI_OK;
 }
+
+static int walls_autotune(Ctx* c);
 
 int walls_setup(Ctx* c) {
   const dnsmi_config& g = c->cfg;
@@ -423,6 +478,31 @@ int walls_setup(Ctx* c) {
   }
   if ((rc = upload_vec(c, &t->lk_ab, lk_ab))) return rc;
   if ((rc = upload_vec(c, &t->hk_ab, hk_ab))) return rc;
+  {
+    // chunk maps of the L_k and H_k solves, built over modes in parallel
+    const int B = kChunk, WF = B + kl, WB = B + kl + ku, nC = (Ny + B - 1) / B;
+    std::vector<double> lf((size_t)nC * WF * M * WF), lb((size_t)nC * WB * M * WB);
+    std::vector<double> hf(lf.size()), hb(lb.size());
+    const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    auto work = [&](int w) {
+      for (int m = w; m < M; m += nth) {
+        auto lraw = [&](int r, int j) { return lk_ab[((size_t)j * rows + r) * M + m]; };
+        auto lpiv = [&](int j) { return lk_piv[(size_t)j * M + m]; };
+        auto hraw = [&](int r, int j) { return hk_ab[((size_t)j * rows + r) * M + m]; };
+        auto hpiv = [&](int j) { return hk_piv[(size_t)j * M + m]; };
+        chunk_maps(Ny, kl, ku, m, M, lraw, lpiv, lf.data(), lb.data());
+        chunk_maps(Ny, kl, ku, m, M, hraw, hpiv, hf.data(), hb.data());
+      }
+    };
+    std::vector<std::thread> pool;
+    for (int w = 1; w < nth; ++w) pool.emplace_back(work, w);
+    work(0);
+    for (auto& th : pool) th.join();
+    if ((rc = upload_vec(c, &t->lk_fmap, lf))) return rc;
+    if ((rc = upload_vec(c, &t->lk_bmap, lb))) return rc;
+    if ((rc = upload_vec(c, &t->hk_fmap, hf))) return rc;
+    if ((rc = upload_vec(c, &t->hk_bmap, hb))) return rc;
+  }
   if ((rc = upload_vec(c, &t->lk_piv, lk_piv))) return rc;
   if ((rc = upload_vec(c, &t->hk_piv, hk_piv))) return rc;
   if ((rc = upload_vec(c, &t->v1, v1))) return rc;
@@ -432,6 +512,7 @@ int walls_setup(Ctx* c) {
   if ((rc = upload_vec(c, &t->minv, minv))) return rc;
   if ((rc = upload_vec(c, &t->mkx, mkx))) return rc;
   if ((rc = upload_vec(c, &t->mkz, mkz))) return rc;
+  if (g.tile_cols == 0) return walls_autotune(c);
   return DNSMI_OK;
 }
 
@@ -791,6 +872,168 @@ __global__ void __launch_bounds__(kSweepLanes)
   }
 }
 
+// Phases 2 and 4, chunked form: the same solves through the precomputed
+// chunk maps (chunk_maps).  32 lanes per mode, two modes per wavefront;
+// lane t forms output t of each chunk's map from the window, which the
+// mode's lanes share through LDS, so the dependent chain is nC chunks
+// instead of Ny rows, and every load is coalesced and independent of the
+// recurrence: a chunk's map row is loaded two chunks ahead, through three
+// register slots (the chunk loop is unrolled by three so the slots are
+// static).  Loads retire in issue order, so each chunk's own loads are
+// issued before the prefetch: waiting for them leaves the prefetch in
+// flight.  The forward results stay in LDS for the back substitution.
+// NR right-hand sides per mode share the maps.
+template <int W>
+__device__ __forceinline__ void map_row(double (&f)[W], const double* __restrict__ base, size_t ks,
+                                        int c, int nC) {
+  const double* p = base + (size_t)(c < nC ? c : nC - 1) * W * ks;
+#pragma unroll
+  for (int k = 0; k < W; ++k) f[k] = p[(size_t)k * ks];
+}
+
+// The window is shared by the mode's lanes, all in this wavefront, whose
+// LDS operations complete in issue order.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// One forward chunk: window position t <-> row cB + t.  rin + r * vs is
+// right-hand side r (row stride 1).
+template <int PO, int NR>
+__device__ __forceinline__ void fwd_chunk(const double (&f)[kChunk + PO], double (&pf)[kChunk + PO],
+                                          const double* __restrict__ fbase, size_t ks,
+                                          const cx<double>* __restrict__ rin, size_t vs,
+                                          cx<double>* vec, cx<double>* win, int t, int c, int nC,
+                                          int Ny) {
+  constexpr int KL = PO, B = kChunk, WF = B + KL;
+  const int frow = (c + 1) * B + t;
+  const bool fr = t >= KL && t < WF && frow < Ny;
+  cx<double> fresh[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) fresh[r] = fr ? rin[r * vs + frow] : cx<double>{0, 0};
+  map_row<WF>(pf, fbase, ks, c + 2, nC);
+  cx<double> acc[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) acc[r] = cx<double>{0, 0};
+#pragma unroll
+  for (int k = 0; k < WF; ++k) {
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const cx<double> w = win[r * 32 + k];
+      acc[r].x += f[k] * w.x;
+      acc[r].y += f[k] * w.y;
+    }
+  }
+  wave_sync();
+  const int row = c * B + t;
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    if (t < B && row < Ny) vec[r * Ny + row] = acc[r];
+    if (t >= B && t < WF) win[r * 32 + t - B] = acc[r];
+    if (t >= KL && t < WF) win[r * 32 + t] = fresh[r];
+  }
+  wave_sync();
+}
+
+// One back chunk: window position t <-> row T - t, T = Ny - 1 - cB.
+template <int PO, int NR>
+__device__ __forceinline__ void back_chunk(const double (&g)[kChunk + 2 * PO],
+                                           double (&pg)[kChunk + 2 * PO],
+                                           const double* __restrict__ gbase, size_t ks,
+                                           cx<double>* __restrict__ out, size_t vs, bool live,
+                                           cx<double>* vec, cx<double>* win, int t, int c, int nC,
+                                           int Ny) {
+  constexpr int KUU = 2 * PO, B = kChunk, WB = B + KUU;
+  map_row<WB>(pg, gbase, ks, c + 2, nC);
+  const int T = Ny - 1 - c * B;
+  const int frow = T - B - t;
+  const bool fr = t >= KUU && t < WB && frow >= 0;
+  cx<double> fresh[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) fresh[r] = fr ? vec[r * Ny + frow] : cx<double>{0, 0};
+  cx<double> acc[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) acc[r] = cx<double>{0, 0};
+#pragma unroll
+  for (int k = 0; k < WB; ++k) {
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const cx<double> w = win[r * 32 + k];
+      acc[r].x += g[k] * w.x;
+      acc[r].y += g[k] * w.y;
+    }
+  }
+  wave_sync();
+  const int row = T - t;
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    if (t < B && row >= 0 && live) out[r * vs + row] = acc[r];
+    if (t >= B && t < WB) win[r * 32 + t - B] = acc[r];
+    if (t >= KUU && t < WB) win[r * 32 + t] = fresh[r];
+  }
+  wave_sync();
+}
+
+template <int PO, int NR>
+__global__ void __launch_bounds__(64)
+    k_band_chunks(ImmArgs a, const double* __restrict__ fmap, const double* __restrict__ bmap,
+                  int vin, int vout) {
+  if (a.act && *a.act == 0) return;
+  constexpr int KL = PO, KUU = 2 * PO, B = kChunk, WF = B + KL, WB = B + KUU;
+  static_assert(WB <= 32, "a window fits the 32 lanes of a mode");
+  extern __shared__ cx<double> sh[];  // [2][NR][Ny] vectors, then [2][NR][32] windows
+  const int Ny = a.Ny, M = a.M, nC = (Ny + B - 1) / B;
+  const int lane = threadIdx.x, half = lane >> 5, t = lane & 31;
+  const int m0 = blockIdx.x * 2 + half;
+  const bool live = m0 < M;
+  const int m = live ? m0 : M - 1;  // a spare half repeats the last mode and stores nothing
+  cx<double>* vec = sh + (size_t)half * NR * Ny;
+  cx<double>* win = sh + (size_t)2 * NR * Ny + (size_t)half * NR * 32;
+  const size_t vs = (size_t)M * Ny;  // between work vectors (VS = 1: rows contiguous)
+  const cx<double>* __restrict__ rin = a.scratch + ((size_t)vin * M + m) * Ny;
+  cx<double>* __restrict__ out = a.scratch + ((size_t)vout * M + m) * Ny;
+  {
+    const size_t ks = (size_t)M * WF;
+    const double* __restrict__ fbase = fmap + (size_t)m * WF + (t < WF ? t : 0);
+    double f0[WF], f1[WF], f2[WF];
+    map_row<WF>(f0, fbase, ks, 0, nC);
+    map_row<WF>(f1, fbase, ks, 1, nC);
+    if (t < WF) {
+#pragma unroll
+      for (int r = 0; r < NR; ++r) win[r * 32 + t] = t < Ny ? rin[r * vs + t] : cx<double>{0, 0};
+    }
+    wave_sync();
+    for (int c = 0; c < nC; c += 3) {
+      fwd_chunk<PO, NR>(f0, f2, fbase, ks, rin, vs, vec, win, t, c, nC, Ny);
+      if (c + 1 >= nC) break;
+      fwd_chunk<PO, NR>(f1, f0, fbase, ks, rin, vs, vec, win, t, c + 1, nC, Ny);
+      if (c + 2 >= nC) break;
+      fwd_chunk<PO, NR>(f2, f1, fbase, ks, rin, vs, vec, win, t, c + 2, nC, Ny);
+    }
+  }
+  {
+    const size_t ks = (size_t)M * WB;
+    const double* __restrict__ gbase = bmap + (size_t)m * WB + (t < WB ? t : 0);
+    double g0[WB], g1[WB], g2[WB];
+    map_row<WB>(g0, gbase, ks, 0, nC);
+    map_row<WB>(g1, gbase, ks, 1, nC);
+    if (t < WB) {
+#pragma unroll
+      for (int r = 0; r < NR; ++r)
+        win[r * 32 + t] = (Ny - 1 - t >= 0) ? vec[r * Ny + Ny - 1 - t] : cx<double>{0, 0};
+    }
+    wave_sync();
+    for (int c = 0; c < nC; c += 3) {
+      back_chunk<PO, NR>(g0, g2, gbase, ks, out, vs, live, vec, win, t, c, nC, Ny);
+      if (c + 1 >= nC) break;
+      back_chunk<PO, NR>(g1, g0, gbase, ks, out, vs, live, vec, win, t, c + 1, nC, Ny);
+      if (c + 2 >= nC) break;
+      back_chunk<PO, NR>(g2, g1, gbase, ks, out, vs, live, vec, win, t, c + 2, nC, Ny);
+    }
+  }
+}
+
 // Phase 3 (flows/plane_couette.py:449-472): R_c for the three components
 // (zero on the walls) -> V_R + c, with the pressure row in LDS.
 template <int PO>
@@ -998,18 +1241,27 @@ static int walls_imm(Ctx* c, const double* un, const double* uj, const double* n
     ProfScope ps(c, KID_W_IMM, s);
     const size_t lds = (size_t)t->Ny * sizeof(cx<double>);
     const dim3 gm(t->M), gs((t->M + kSweepLanes - 1) / kSweepLanes, 1),
-        gs3((t->M + kSweepLanes - 1) / kSweepLanes, 3);
+        gs3((t->M + kSweepLanes - 1) / kSweepLanes, 3), gc((t->M + 1) / 2);
+    // chunked solves: per workgroup two modes' vectors and windows
+    const size_t lds_c1 = (size_t)2 * 1 * (t->Ny + 32) * sizeof(cx<double>);
+    const size_t lds_c3 = (size_t)2 * 3 * (t->Ny + 32) * sizeof(cx<double>);
     switch (t->p) {
 #define IMM(PO)                                                                                   \
   case PO:                                                                                        \
     hipLaunchKernelGGL(k_imm_fhat<PO>, gm, dim3(kImmT), lds, s, a);                               \
-    hipLaunchKernelGGL(k_band_sweeps<PO>, gs, dim3(kSweepLanes), 0, s, a, t->lk_ab, t->lk_piv,     \
-                       (int)V_F,                                                                  \
-                       (int)V_T, (int)V_P);                                                       \
+    if (c->sweep_seq)                                                                             \
+      hipLaunchKernelGGL(k_band_sweeps<PO>, gs, dim3(kSweepLanes), 0, s, a, t->lk_ab, t->lk_piv,   \
+                         (int)V_F, (int)V_T, (int)V_P);                                           \
+    else                                                                                          \
+      hipLaunchKernelGGL((k_band_chunks<PO, 1>), gc, dim3(64), lds_c1, s, a, t->lk_fmap,          \
+                         t->lk_bmap, (int)V_F, (int)V_P);                                         \
     hipLaunchKernelGGL(k_imm_rc<PO>, gm, dim3(kImmT), lds, s, a);                                 \
-    hipLaunchKernelGGL(k_band_sweeps<PO>, gs3, dim3(kSweepLanes), 0, s, a, t->hk_ab, t->hk_piv,   \
-                       (int)V_R,                                                                  \
-                       (int)V_T, (int)V_R);                                                       \
+    if (c->sweep_seq)                                                                             \
+      hipLaunchKernelGGL(k_band_sweeps<PO>, gs3, dim3(kSweepLanes), 0, s, a, t->hk_ab, t->hk_piv, \
+                         (int)V_R, (int)V_T, (int)V_R);                                           \
+    else                                                                                          \
+      hipLaunchKernelGGL((k_band_chunks<PO, 3>), gc, dim3(64), lds_c3, s, a, t->hk_fmap,          \
+                         t->hk_bmap, (int)V_R, (int)V_R);                                         \
     hipLaunchKernelGGL(k_imm_final<PO>, gm, dim3(kImmT), 0, s, a);                                \
     break;
       IMM(2) IMM(4) IMM(6) IMM(8)
@@ -1020,6 +1272,38 @@ static int walls_imm(Ctx* c, const double* un, const double* uj, const double* n
   int rc = check_launch(c, "imm");
   if (rc || !err) return rc;
   return finalize_norm(c, nb, err, s);
+}
+
+// Chooses the band-solve form by timing one influence-matrix step of each
+// on zeroed buffers (both forms are exact solves of the same systems).
+static int walls_autotune(Ctx* c) {
+  if (hipMemset(c->rhs0, 0, c->spec_elems * 16) != hipSuccess)
+    return c->fail(DNSMI_EHIP, "autotune: memset");
+  double* z = c->rhs0;
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    return c->fail(DNSMI_EHIP, "autotune: events");
+  auto time_imm = [&](bool seq) {
+    c->sweep_seq = seq;
+    walls_imm(c, z, z, z, z, z, nullptr, nullptr, nullptr);  // warm-up
+    (void)hipEventRecord(e0, nullptr);
+    for (int i = 0; i < 3; ++i) walls_imm(c, z, z, z, z, z, nullptr, nullptr, nullptr);
+    (void)hipEventRecord(e1, nullptr);
+    (void)hipEventSynchronize(e1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    return ms / 3.0;
+  };
+  const double tc = time_imm(false), ts = time_imm(true);
+  c->sweep_seq = ts < tc;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  char buf[120];
+  snprintf(buf, sizeof buf, "band_solve=%s(chunked %.1f us, sequential %.1f us) ",
+           c->sweep_seq ? "sequential" : "chunked", 1e3 * tc, 1e3 * ts);
+  c->tuning = buf;
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? DNSMI_OK : c->fail(DNSMI_EHIP, std::string("autotune: ") + hipGetErrorString(e));
 }
 
 int walls_predict(Ctx* c, const double* state, const double* rhs, double* pred, hipStream_t s) {

@@ -74,7 +74,8 @@ typedef struct dnsmi_config {
   int tile_cols;           /* 0: autotune the kernel variants at create;
                               otherwise use `variants` as given */
   int variants;            /* bit mask when tile_cols != 0: 1 sequential
-                              y-inverse, 2 sequential z-inverse, 4 split x pass */
+                              y-inverse, 2 sequential z-inverse, 4 split x pass,
+                              8 sequential band sweeps (walls; else chunk maps) */
 } dnsmi_config;
 
 typedef struct dnsmi_ctx dnsmi_ctx;

@@ -75,9 +75,12 @@ def test_setup_tables_match_oracle(case):
 
 
 @pytest.mark.parametrize("case", list(CASES))
-def test_golden_rhs_and_imm_step(case):
+@pytest.mark.parametrize("variants", [None, 0, 8], ids=["autotuned", "chunked", "sequential"])
+def test_golden_rhs_and_imm_step(case, variants):
+    """Both band-solve forms (chunk maps; sequential sweeps) and the
+    autotuner's choice against the golden steps."""
     g = np.load(os.path.join(GOLDEN, case + ".npz"))
-    flow = make(case)
+    flow = make(case) if variants is None else make(case, tile_cols=16, variants=variants)
     u = flow.to_device(g["state"])
     rhs = flow.get_rhs(u)
     assert rel(rhs, g["rhs0"]) < 1e-12
@@ -114,7 +117,8 @@ def test_generic_callables_match_fused(case):
 
 
 @pytest.mark.parametrize("ny", [33, 32])
-def test_run_counts_and_per_step_error(ny):
+@pytest.mark.parametrize("variants", [0, 8], ids=["chunked", "sequential"])
+def test_run_counts_and_per_step_error(ny, variants):
     """Plane Couette 16 x ny x 16, p = 4, Re 400, dt 0.005, tol 1e-10: 10 steps.
 
     The wall corrector contracts by about 2 per pass, so dt 0.005 with up
@@ -124,7 +128,7 @@ def test_run_counts_and_per_step_error(ny):
 
     nx = nz = 16
     flow = make(nx=nx, ny=ny, nz=nz, fd_order=4, re=400.0, dt=0.005, corrector_tolerance=1e-10,
-                max_corrector_iterations=20)
+                max_corrector_iterations=20, tile_cols=16, variants=variants)
     op = W.WallParams(nx=nx, ny=ny, nz=nz, fd_order=4, re=400.0, dt=0.005)
     of = W.WallFlow(op)
     u0 = wall_field(nx, ny, nz, op.lx, op.lz, 0.05, seed=2)
