@@ -348,3 +348,36 @@ def test_walls_learned_runs_and_graph_steps(precision):
         assert torch.equal(gs.state, ref[i]), f"step {i} differs"
     assert gs.passes() == g2.passes
 
+
+
+def test_walls_network_gain_over_extrapolation():
+    """A/B on plane Couette 16 x 33 x 16 at the reference's default
+    tolerance (1e-5): the network trained on the solver's own history
+    explains most of the held-out residual of quadratic extrapolation, and
+    the run started from its guess needs no more RHS + IMM evaluations than
+    the run started from quadratic extrapolation over the same steps."""
+    from dnsjax_amd.fields import wall_field
+    from dnsjax_amd.flows import PlaneCouetteFlow
+    from dnsjax_amd.learned import Net, collect, fit_scales, make_dataset, mean_loss, run_guessed, train
+
+    flow = PlaneCouetteFlow(Params(system="plane-couette", nx=16, ny=33, nz=16, lx=2 * math.pi, lz=math.pi,
+                                   re=400.0, dt=0.01, fd_order=4, corrector_tolerance=1e-5,
+                                   max_corrector_iterations=10, precision="mixed", tile_cols=16))
+    u0 = flow.to_device(wall_field(16, 33, 16, 2 * math.pi, math.pi, 0.05, seed=2))
+    snaps, u_mid = collect(flow, u0, 24)
+    sc = fit_scales(snaps)
+    X, T = make_dataset(flow, snaps, sc, 20000, seed=1)
+    perm = torch.randperm(X.shape[0], generator=torch.Generator().manual_seed(3)).to(X.device)
+    nv = 40000
+    Xv, Tv, X, T = X[perm[:nv]], T[perm[:nv]], X[perm[nv:]], T[perm[nv:]]
+    net = Net(32, seed=0)
+    train(net, X, T, iters=600, batch=4096, lr=3e-3)
+    loss, base = mean_loss(net, Xv, Tv)
+    n = 16
+    quad = run_guessed(flow, u_mid.clone(), n, mode=3)
+    learned = run_guessed(flow, u_mid.clone(), n, mode=2, net=net, scales=sc)
+    print(f"held-out loss {loss:.4g} (without the net {base:.4g}); passes quadratic {quad.passes}, "
+          f"learned {learned.passes}")
+    assert loss < 0.5 * base, (loss, base)
+    assert quad.converged and learned.converged
+    assert learned.rhs_evals <= quad.rhs_evals, (learned.passes, quad.passes)
