@@ -1,0 +1,95 @@
+"""Plane Couette at its largest configured shape: 128 x 256 x 128 CGL points,
+fd_order 6, Re 1000, dt 0.005, corrector tolerance 1e-10 (the learned-guess
+configuration).
+
+Bar: the two band-solve forms (chunk maps, sequential sweeps) give the same
+predictor-corrector step to 1e-11 relative, with no-slip walls and zero wall
+divergence; a mixed run started from the per-point learned guess converges
+every step with no more RHS + IMM evaluations than the plain FP64 path and
+ends within 1e-6 relative of its energy.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+from dnsjax_amd.params import Params  # noqa: E402
+from oracle import walls as W  # noqa: E402
+
+NX, NY, NZ, P = 128, 256, 128, 6
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+
+
+def make(**kw):
+    from dnsjax_amd.flows import PlaneCouetteFlow
+
+    base = dict(system="plane-couette", nx=NX, ny=NY, nz=NZ, lx=2 * math.pi, lz=math.pi, re=1000.0,
+                dt=0.005, fd_order=P, corrector_tolerance=1e-10, max_corrector_iterations=20)
+    base.update(kw)
+    return PlaneCouetteFlow(Params(**base))
+
+
+def rel(a, b):
+    return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+
+
+def test_band_solve_forms_agree_with_wall_invariants():
+    from dnsjax_amd.fields import wall_field
+
+    u0 = wall_field(NX, NY, NZ, 2 * math.pi, math.pi, 0.05, seed=4)
+    out = {}
+    for name, variants in (("chunked", 0), ("sequential", 8)):
+        flow = make(tile_cols=16, variants=variants)
+        pred, rhs, err = flow.predict_and_correct(flow.to_device(u0))
+        out[name] = (pred, float(err), flow)
+    assert rel(out["chunked"][0], out["sequential"][0]) < 1e-11
+    assert abs(out["chunked"][1] - out["sequential"][1]) <= 1e-9 * out["sequential"][1]
+    pred, _, flow = out["chunked"]
+    v = pred.cpu().numpy()
+    scale = np.max(np.abs(v))
+    assert np.max(np.abs(v[..., 0])) < 1e-13 * scale and np.max(np.abs(v[..., -1])) < 1e-13 * scale
+    ys = flow._native.walls_table(5, (NY,))
+    D1, _ = W.build_diff_matrices(ys, P)
+    kx = (np.arange(NX // 2) * 2 * np.pi / (2 * np.pi)).reshape(1, -1)
+    hz = np.concatenate([np.arange(NZ // 2), np.arange(-NZ // 2 + 1, 0)])
+    kz = (hz * 2 * np.pi / np.pi).reshape(-1, 1)
+    div = 1j * kx[..., None] * v[0] + np.einsum("ij,zxj->zxi", D1, v[1]) + 1j * kz[..., None] * v[2]
+    assert np.max(np.abs(div[..., -1])) < 1e-9 * scale
+    bot = div[..., 0].copy()
+    bot[0, 0] = 0
+    assert np.max(np.abs(bot)) < 1e-9 * scale
+
+
+def test_learned_guess_against_the_plain_path():
+    from dnsjax_amd.driver import run
+    from dnsjax_amd.fields import wall_field
+    from dnsjax_amd.learned import Net, collect, fit_scales, make_dataset, run_guessed, train
+
+    fp64, mixed = make(), make(precision="mixed")
+    u0 = fp64.to_device(wall_field(NX, NY, NZ, 2 * math.pi, math.pi, 0.05, seed=4))
+    snaps, u_mid = collect(mixed, u0, 10)
+    sc = fit_scales(snaps)
+    X, T = make_dataset(mixed, snaps, sc, 50000, seed=1)
+    del snaps
+    net = Net(32, seed=0)
+    train(net, X, T, iters=400, batch=8192, lr=3e-3)
+    n = 6
+    plain = run(fp64, u_mid.clone(), n)
+    learned = run_guessed(mixed, u_mid.clone(), n, mode=2, net=net, scales=sc)
+    print(f"RHS + IMM evaluations per step: plain {plain.rhs_evals / n:.2f} ({plain.counts}), "
+          f"mixed + learned {learned.rhs_evals / n:.2f} ({learned.passes})")
+    assert plain.converged and learned.converged
+    assert learned.rhs_evals <= plain.rhs_evals
+    e_plain = float(fp64.get_stats(plain.state)["E'"])
+    e_learned = float(fp64.get_stats(learned.state)["E'"])
+    assert abs(e_learned - e_plain) <= 1e-6 * abs(e_plain), (e_learned, e_plain)
