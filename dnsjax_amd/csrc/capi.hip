@@ -21,6 +21,17 @@ void* Ctx::alloc(size_t bytes) {
   return p;
 }
 
+void Ctx::release(void* p) {
+  if (!p) return;
+  for (size_t i = 0; i < allocations.size(); ++i) {
+    if (allocations[i] == p) {
+      (void)hipFree(p);
+      allocations.erase(allocations.begin() + (long)i);
+      return;
+    }
+  }
+}
+
 hipEvent_t Ctx::ev_get() {
   if (ev_used == ev_pool.size()) {
     hipEvent_t e = nullptr;

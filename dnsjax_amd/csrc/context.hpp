@@ -95,6 +95,7 @@ struct Ctx {
     return code;
   }
   void* alloc(size_t bytes);
+  void release(void* p);  // frees one allocation made by alloc (null: no-op)
 };
 
 extern thread_local std::string g_create_error;

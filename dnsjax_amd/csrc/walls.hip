@@ -478,8 +478,9 @@ int walls_setup(Ctx* c) {
   }
   if ((rc = upload_vec(c, &t->lk_ab, lk_ab))) return rc;
   if ((rc = upload_vec(c, &t->hk_ab, hk_ab))) return rc;
-  {
-    // chunk maps of the L_k and H_k solves, built over modes in parallel
+  // chunk maps of the L_k and H_k solves, built over modes in parallel
+  // (not when the sequential sweeps are requested)
+  if (!(g.tile_cols != 0 && (g.variants & 8) != 0)) {
     const int B = kChunk, WF = B + kl, WB = B + kl + ku, nC = (Ny + B - 1) / B;
     std::vector<double> lf((size_t)nC * WF * M * WF), lb((size_t)nC * WB * M * WB);
     std::vector<double> hf(lf.size()), hb(lb.size());
@@ -1238,6 +1239,7 @@ static int walls_imm(Ctx* c, const double* un, const double* uj, const double* n
   a.c = c->cfg.implicitness;
   a.nu = 1.0 / c->cfg.re;
   {
+    if (!c->sweep_seq && !t->lk_fmap) return c->fail(DNSMI_EINVAL, "imm: chunk maps not built");
     ProfScope ps(c, KID_W_IMM, s);
     const size_t lds = (size_t)t->Ny * sizeof(cx<double>);
     const dim3 gm(t->M), gs((t->M + kSweepLanes - 1) / kSweepLanes, 1),
@@ -1296,6 +1298,13 @@ static int walls_autotune(Ctx* c) {
   };
   const double tc = time_imm(false), ts = time_imm(true);
   c->sweep_seq = ts < tc;
+  if (c->sweep_seq) {  // the chunk maps are not needed any more
+    WallTables* t = c->wt;
+    for (double** p : {&t->lk_fmap, &t->lk_bmap, &t->hk_fmap, &t->hk_bmap}) {
+      c->release(*p);
+      *p = nullptr;
+    }
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   char buf[120];
