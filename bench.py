@@ -414,7 +414,13 @@ def ensemble_form(steps: int, n_streams: int = 8) -> dict:
     uniformly in [20, 80] with seed 3; dt 0.01; seeds 300 + i, RMS 0.05;
     mixed form, tol 1e-10), advanced by one captured graph launch per step,
     against the same 64 trajectories run one after another with the eager
-    driver.  Aggregate time-steps/s over all members."""
+    driver.  Aggregate time-steps/s over all members.
+
+    Every captured step launches a fixed number of corrector slots (skipped
+    slots still launch their kernels, which exit at once).  The ensemble takes
+    one slot more than the largest iteration count of the sequential runs;
+    the device counters flag any step that would need more (`unconverged`),
+    and every member's counts must equal its sequential run's."""
     import numpy as np
     import torch
 
@@ -438,7 +444,8 @@ def ensemble_form(steps: int, n_streams: int = 8) -> dict:
     seq_counts = [run(f, s.clone(), steps).counts for f, s in zip(flows, states)]
     torch.cuda.synchronize()
     t_seq = time.perf_counter() - t0
-    ens = Ensemble(flows, states, n_streams=n_streams, capacity=steps)
+    slots = min(max(c for cs in seq_counts for c in cs) + 1, 10)
+    ens = Ensemble(flows, states, n_streams=n_streams, slots=slots, capacity=steps)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     ens.step(steps)
@@ -447,6 +454,7 @@ def ensemble_form(steps: int, n_streams: int = 8) -> dict:
     same = ens.counts() == seq_counts
     st = ens.step_stats()
     return {"members": m, "sizes": "16 x {32,48,64,96}^3", "steps": steps, "n_streams": n_streams,
+            "slots": slots,
             "value": round(m * steps / t_ens, 2), "unit": "member time-steps/s",
             "sequential_value": round(m * steps / t_seq, 2),
             "speedup_vs_sequential": round(t_seq / t_ens, 3),
