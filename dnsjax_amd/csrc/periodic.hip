@@ -338,12 +338,20 @@ struct XRow {
   static constexpr int value = a > b ? a : b;
 };
 
+// Waves per SIMD the x pass asks for: three for FP64 at P <= 96, whose rows
+// fit 168 registers (the plane Couette x pass at P = 96 has 2304 waves:
+// at two per SIMD the last 256 would run as a second, nearly empty round).
+template <typename T, int P>
+struct XWaves {
+  static constexpr int value = (sizeof(T) == 8 && P <= 96) ? 3 : 2;
+};
+
 // The nonlinear term is formed in two stages to bound the live registers:
 // after the (u_z, w_y) and (u_y, w_z) transforms, component 0 is complete
 // and is transformed and stored; the (u_x, w_x) transform then completes
 // components 1 and 2.  At most four physical fields are held at once.
 template <typename T, int P, int R>
-__global__ void __launch_bounds__(R * Plan<P>::N1, 2)
+__global__ void __launch_bounds__(R * Plan<P>::N1, (XWaves<T, P>::value))
     k_x_nonlinear(const cx<T>* __restrict__ w2, cx<T>* __restrict__ w3,
                   const cx<T>* __restrict__ tw, const double* __restrict__ kx_tab,
                   const double* __restrict__ base, const double* __restrict__ cbase,

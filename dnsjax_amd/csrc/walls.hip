@@ -607,7 +607,7 @@ __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
   const size_t out_f = (size_t)Ny * P * Nx1;
   const size_t out_off = (size_t)y * P * Nx1 + kx;
 #pragma unroll 1
-  for (int f = 0; f < NF; ++f) {
+  for (int f = blockIdx.y; f < NF; f += gridDim.y) {  // one field per workgroup row
     const int j = opaque(j_);
     if (j < N2) {
       cx<T> x[N1];
@@ -1174,7 +1174,7 @@ static int walls_rhs_t(Ctx* c, const double* state, double* rhs, hipStream_t s) 
     switch (c->Pz) {
 #define X(P)                                                                                    \
   case P:                                                                                       \
-    hipLaunchKernelGGL((k_z_inverse_plain<T, P, 16, 6>), dim3(Ny * ntile),                      \
+    hipLaunchKernelGGL((k_z_inverse_plain<T, P, 16, 6>), dim3(Ny * ntile, 6),                   \
                        dim3(16 * ColPlan<P>::threads), 0, s, A, B, twz, c->nz, Ny, c->Nx1,     \
                        c->act);                                                                 \
     break;
