@@ -22,8 +22,11 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
+#include <cstring>
 #include <string>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "context.hpp"
@@ -49,17 +52,18 @@ struct WallTables {
   int* lk_piv = nullptr;    // [j][mode]
   double* hk_ab = nullptr;
   int* hk_piv = nullptr;
-  double* v1 = nullptr;  // [y][mode]
-  double* v2 = nullptr;
-  double* q1 = nullptr;
-  double* q2 = nullptr;
+  double* vq = nullptr;   // [map][y][4]: v1, v2, q1, q2 of a k^2 class
+  int* umode = nullptr;   // per mode: its k^2 class (map)
   double* minv = nullptr;  // [4][mode]
   double* mkx = nullptr;   // per mode
   double* mkz = nullptr;
   double* scratch = nullptr;  // IMM work vectors, complex [vec][mode][y]
-  // chunk maps of the band solves (chunked sweeps): forward [c][k][mode][t]
-  // (WF x WF per chunk and mode), back [c][k][mode][t] (WB x WB)
+  // chunk maps of the band solves (chunked sweeps), one per distinct k^2,
+  // packed (map_offset); sched = the modes, with their maps, in the order
+  // the chunked solve takes them (two per wavefront)
   double *lk_fmap = nullptr, *lk_bmap = nullptr, *hk_fmap = nullptr, *hk_bmap = nullptr;
+  int2* sched = nullptr;  // (mode, map) per slot
+  int U = 0;              // distinct k^2 values
   // host copies in the reference layouts, for inspection by tests
   std::vector<double> h_lk_ab, h_hk_ab, h_v1, h_q1, h_minv, h_ys;
   std::vector<int> h_lk_piv, h_hk_piv;
@@ -197,22 +201,46 @@ static void band_solve(const Band& B, std::vector<double>& b) {
   }
 }
 
-// Rows per chunk of the chunked band solve.
+// Rows per chunk of the chunked band solve, and the register slots of its
+// map prefetch (a chunk's map is loaded kChunkSlots - 1 chunks ahead).
 constexpr int kChunk = 16;
+constexpr int kChunkSlots = 3;
+// two slots where three would spill (fd_order 8, three right-hand sides)
+constexpr int chunk_slots(int po, int nr) { return po == 8 && nr == 3 ? 2 : kChunkSlots; }
 
-// Chunk maps of one mode's band solve (the operation order of band_solve):
-// forward chunk c applies the steps j in [cB, min(cB + B, n - 1)) (row swap,
-// elimination) to the window of rows cB + t, t < WF = B + kl; its map F_c
-// (out_t = sum_k F_c[t][k] w_k) is those row operations applied to the
-// identity.  Back chunk c, top row T = n - 1 - cB, applies the steps
-// j = T, T - 1, ... (at most B, j >= 0: divide by the diagonal, eliminate
-// upwards) to the window of rows T - t, t < WB = B + kl + ku.  Stored at
-// [((c W + k) M + m) W + t] so that the lanes t of a mode read consecutive
-// words.  `raw(r, j)` reads the factored band, `piv(j)` the pivot row.
+// Packed chunk maps.  Entry (t, k) of a map (out_t = sum_k map[t][k] w_k)
+// is zero by construction for k > t + lag: lag = kl for a forward map (a
+// row swap or elimination at window row t reaches at most kl rows down),
+// lag = 0 for a back map (row t draws only on the rows solved before it).
+// Column k is therefore stored from t = map_lo(k) on, map_len(k) words per
+// map, the maps' columns side by side: chunk c, column k, map u, row t at
+// map_offset (lanes t of one map read consecutive words).
+__host__ __device__ constexpr int map_lo(int k, int lag) { return k > lag ? k - lag : 0; }
+__host__ __device__ constexpr int map_len(int k, int lag, int w) { return w - map_lo(k, lag); }
+// words of columns 0 .. k-1
+__host__ __device__ constexpr int map_pre(int k, int lag, int w) {
+  return k * w - (k > lag + 1 ? (k - lag - 1) * (k - lag) / 2 : 0);
+}
+static size_t map_offset(int c, int k, int u, int t, int lag, int w, int U) {
+  return (size_t)c * U * map_pre(w, lag, w) + (size_t)U * map_pre(k, lag, w) +
+         (size_t)u * map_len(k, lag, w) + (t - map_lo(k, lag));
+}
+
+// Chunk maps of one band solve (the operation order of band_solve): forward
+// chunk c applies the steps j in [cB, min(cB + B, n - 1)) (row swap,
+// elimination) to the window of rows cB + t, t < WF = B + kl; its map F_c is
+// those row operations applied to the identity.  Back chunk c, top row
+// T = n - 1 - cB, applies the steps j = T, T - 1, ... (at most B, j >= 0:
+// divide by the diagonal, eliminate upwards) to the window of rows T - t,
+// t < WB = B + kl + ku.  Written packed as map u of U; `raw(r, j)` reads
+// the factored band, `piv(j)` the pivot row.  Returns false if an entry
+// outside the packed region is not zero (it cannot be, by the argument
+// above; the check keeps the packing honest).
 template <typename Raw, typename Piv>
-static void chunk_maps(int n, int kl, int ku, int m, int M, Raw raw, Piv piv, double* F, double* G) {
+static bool chunk_maps(int n, int kl, int ku, int u, int U, Raw raw, Piv piv, double* F, double* G) {
   const int B = kChunk, kuu = kl + ku, WF = B + kl, WB = B + kuu;
   const int nC = (n + B - 1) / B;
+  bool ok = true;
   std::vector<double> X((size_t)WB * WB);
   for (int c = 0; c < nC; ++c) {
     const int j0 = c * B;
@@ -228,7 +256,11 @@ static void chunk_maps(int n, int kl, int ku, int m, int M, Raw raw, Piv piv, do
       }
     }
     for (int k = 0; k < WF; ++k)
-      for (int t = 0; t < WF; ++t) F[(((size_t)c * WF + k) * M + m) * WF + t] = X[(size_t)t * WF + k];
+      for (int t = 0; t < WF; ++t) {
+        const double x = X[(size_t)t * WF + k];
+        if (t >= map_lo(k, kl)) F[map_offset(c, k, u, t, kl, WF, U)] = x;
+        else ok = ok && x == 0.0;
+      }
     const int T = n - 1 - j0;
     std::fill(X.begin(), X.end(), 0.0);
     for (int t = 0; t < WB; ++t) X[(size_t)t * WB + t] = 1.0;
@@ -242,8 +274,13 @@ static void chunk_maps(int n, int kl, int ku, int m, int M, Raw raw, Piv piv, do
       }
     }
     for (int k = 0; k < WB; ++k)
-      for (int t = 0; t < WB; ++t) G[(((size_t)c * WB + k) * M + m) * WB + t] = X[(size_t)t * WB + k];
+      for (int t = 0; t < WB; ++t) {
+        const double x = X[(size_t)t * WB + k];
+        if (t >= map_lo(k, 0)) G[map_offset(c, k, u, t, 0, WB, U)] = x;
+        else ok = ok && x == 0.0;
+      }
   }
+  return ok;
 }
 
 // Linear weights of the y quadrature, divided by Ly.  Odd Ny: the
@@ -384,7 +421,6 @@ int walls_setup(Ctx* c) {
   const double dt = g.dt, cc = g.implicitness, nu = 1.0 / g.re;
   std::vector<double> lk_ab((size_t)Ny * rows * M), hk_ab((size_t)Ny * rows * M);
   std::vector<int> lk_piv((size_t)Ny * M), hk_piv((size_t)Ny * M);
-  std::vector<double> v1((size_t)Ny * M), v2((size_t)Ny * M), q1((size_t)Ny * M), q2((size_t)Ny * M);
   std::vector<double> minv(4ull * M), mkx(M), mkz(M);
   t->h_lk_ab.assign((size_t)M * rows * Ny, 0.0);
   t->h_hk_ab.assign((size_t)M * rows * Ny, 0.0);
@@ -393,12 +429,56 @@ int walls_setup(Ctx* c) {
   t->h_v1.assign((size_t)M * Ny, 0.0);
   t->h_q1.assign((size_t)M * Ny, 0.0);
   t->h_minv.assign(4ull * M, 0.0);
+  // L_k and H_k depend on the mode only through k^2: modes with the same
+  // k^2 (bit for bit) share every per-mode table, computed at the first of
+  // them (first[u]) and copied to the others
+  std::vector<int> umode(M), first;
+  {
+    std::unordered_map<uint64_t, int> key;
+    for (int iz = 0; iz < Nkz; ++iz)
+      for (int ix = 0; ix < Nkx; ++ix) {
+        const int m = iz * Nkx + ix;
+        const double k2 = kx[ix] * kx[ix] + kz[iz] * kz[iz];
+        uint64_t b;
+        std::memcpy(&b, &k2, sizeof b);
+        const auto it = key.emplace(b, (int)first.size());
+        if (it.second) first.push_back(m);
+        umode[m] = it.first->second;
+      }
+  }
+  const int U = (int)first.size();
+  t->U = U;
+  std::vector<double> vq((size_t)U * Ny * 4);
+  auto copy_mode = [&](int m, int r) {
+    for (int j = 0; j < Ny; ++j) {
+      for (int q = 0; q < rows; ++q) {
+        lk_ab[((size_t)j * rows + q) * M + m] = lk_ab[((size_t)j * rows + q) * M + r];
+        hk_ab[((size_t)j * rows + q) * M + m] = hk_ab[((size_t)j * rows + q) * M + r];
+        t->h_lk_ab[((size_t)m * rows + q) * Ny + j] = t->h_lk_ab[((size_t)r * rows + q) * Ny + j];
+        t->h_hk_ab[((size_t)m * rows + q) * Ny + j] = t->h_hk_ab[((size_t)r * rows + q) * Ny + j];
+      }
+      lk_piv[(size_t)j * M + m] = lk_piv[(size_t)j * M + r];
+      hk_piv[(size_t)j * M + m] = hk_piv[(size_t)j * M + r];
+      t->h_lk_piv[(size_t)m * Ny + j] = t->h_lk_piv[(size_t)r * Ny + j];
+      t->h_hk_piv[(size_t)m * Ny + j] = t->h_hk_piv[(size_t)r * Ny + j];
+      t->h_v1[(size_t)m * Ny + j] = t->h_v1[(size_t)r * Ny + j];
+      t->h_q1[(size_t)m * Ny + j] = t->h_q1[(size_t)r * Ny + j];
+    }
+    for (int q = 0; q < 4; ++q) {
+      minv[(size_t)q * M + m] = minv[(size_t)q * M + r];
+      t->h_minv[(size_t)m * 4 + q] = t->h_minv[(size_t)r * 4 + q];
+    }
+  };
   for (int iz = 0; iz < Nkz; ++iz) {
     for (int ix = 0; ix < Nkx; ++ix) {
       const int m = iz * Nkx + ix;
       const double k2 = kx[ix] * kx[ix] + kz[iz] * kz[iz];
       mkx[m] = kx[ix];
       mkz[m] = kz[iz];
+      if (first[umode[m]] != m) {
+        copy_mode(m, first[umode[m]]);
+        continue;
+      }
       Band L(Ny, kl, ku), H(Ny, kl, ku);
       for (int i = 0; i < Ny; ++i) {
         for (int j = std::max(0, i - kl); j <= std::min(Ny - 1, i + ku); ++j) {
@@ -467,10 +547,11 @@ int walls_setup(Ctx* c) {
         hk_piv[(size_t)j * M + m] = H.piv[j];
         t->h_lk_piv[(size_t)m * Ny + j] = L.piv[j];
         t->h_hk_piv[(size_t)m * Ny + j] = H.piv[j];
-        v1[(size_t)j * M + m] = r1[j];
-        v2[(size_t)j * M + m] = r2[j];
-        q1[(size_t)j * M + m] = s1[j];
-        q2[(size_t)j * M + m] = s2[j];
+        double* w = &vq[((size_t)umode[m] * Ny + j) * 4];
+        w[0] = r1[j];
+        w[1] = r2[j];
+        w[2] = s1[j];
+        w[3] = s2[j];
         t->h_v1[(size_t)m * Ny + j] = r1[j];
         t->h_q1[(size_t)m * Ny + j] = s1[j];
       }
@@ -478,27 +559,43 @@ int walls_setup(Ctx* c) {
   }
   if ((rc = upload_vec(c, &t->lk_ab, lk_ab))) return rc;
   if ((rc = upload_vec(c, &t->hk_ab, hk_ab))) return rc;
-  // chunk maps of the L_k and H_k solves, built over modes in parallel
-  // (not when the sequential sweeps are requested)
+  // chunk maps of the L_k and H_k solves, one per distinct k^2, built in
+  // parallel (not when the sequential sweeps are requested)
   if (!(g.tile_cols != 0 && (g.variants & 8) != 0)) {
     const int B = kChunk, WF = B + kl, WB = B + kl + ku, nC = (Ny + B - 1) / B;
-    std::vector<double> lf((size_t)nC * WF * M * WF), lb((size_t)nC * WB * M * WB);
-    std::vector<double> hf(lf.size()), hb(lb.size());
+    const size_t nf = (size_t)nC * U * map_pre(WF, kl, WF), nb = (size_t)nC * U * map_pre(WB, 0, WB);
+    if (nf >= (1ull << 28) || nb >= (1ull << 28))
+      return c->fail(DNSMI_EINVAL, "chunk maps exceed 2^28 words: use the sequential band solve");
+    std::vector<double> lf(nf), lb(nb), hf(nf), hb(nb);
     const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<char> ok(nth, 1);
     auto work = [&](int w) {
-      for (int m = w; m < M; m += nth) {
+      for (int u = w; u < U; u += nth) {
+        const int m = first[u];
         auto lraw = [&](int r, int j) { return lk_ab[((size_t)j * rows + r) * M + m]; };
         auto lpiv = [&](int j) { return lk_piv[(size_t)j * M + m]; };
         auto hraw = [&](int r, int j) { return hk_ab[((size_t)j * rows + r) * M + m]; };
         auto hpiv = [&](int j) { return hk_piv[(size_t)j * M + m]; };
-        chunk_maps(Ny, kl, ku, m, M, lraw, lpiv, lf.data(), lb.data());
-        chunk_maps(Ny, kl, ku, m, M, hraw, hpiv, hf.data(), hb.data());
+        const bool a = chunk_maps(Ny, kl, ku, u, U, lraw, lpiv, lf.data(), lb.data());
+        const bool b = chunk_maps(Ny, kl, ku, u, U, hraw, hpiv, hf.data(), hb.data());
+        if (!a || !b) ok[w] = 0;
       }
     };
     std::vector<std::thread> pool;
     for (int w = 1; w < nth; ++w) pool.emplace_back(work, w);
     work(0);
     for (auto& th : pool) th.join();
+    for (char o : ok)
+      if (!o) return c->fail(DNSMI_EINVAL, "chunk maps: nonzero entry outside the packed region");
+    // the chunked solve's mode order: by map, so that the two modes of a
+    // wavefront share theirs wherever two modes have the same k^2
+    std::vector<int> order(M);
+    for (int m = 0; m < M; ++m) order[m] = m;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return umode[a] < umode[b]; });
+    order.resize((size_t)2 * ((M + 1) / 2), order[M - 1]);  // a spare slot repeats the last mode
+    std::vector<int2> sched(order.size());
+    for (size_t i = 0; i < order.size(); ++i) sched[i] = make_int2(order[i], umode[order[i]]);
+    if ((rc = upload_vec(c, &t->sched, sched))) return rc;
     if ((rc = upload_vec(c, &t->lk_fmap, lf))) return rc;
     if ((rc = upload_vec(c, &t->lk_bmap, lb))) return rc;
     if ((rc = upload_vec(c, &t->hk_fmap, hf))) return rc;
@@ -506,10 +603,8 @@ int walls_setup(Ctx* c) {
   }
   if ((rc = upload_vec(c, &t->lk_piv, lk_piv))) return rc;
   if ((rc = upload_vec(c, &t->hk_piv, hk_piv))) return rc;
-  if ((rc = upload_vec(c, &t->v1, v1))) return rc;
-  if ((rc = upload_vec(c, &t->v2, v2))) return rc;
-  if ((rc = upload_vec(c, &t->q1, q1))) return rc;
-  if ((rc = upload_vec(c, &t->q2, q2))) return rc;
+  if ((rc = upload_vec(c, &t->vq, vq))) return rc;
+  if ((rc = upload_vec(c, &t->umode, umode))) return rc;
   if ((rc = upload_vec(c, &t->minv, minv))) return rc;
   if ((rc = upload_vec(c, &t->mkx, mkx))) return rc;
   if ((rc = upload_vec(c, &t->mkz, mkz))) return rc;
@@ -668,7 +763,9 @@ struct ImmArgs {
   const int *d1j0, *d2j0;
   const double *lk_ab, *hk_ab;
   const int *lk_piv, *hk_piv;
-  const double *v1, *v2, *q1, *q2, *minv, *mkx, *mkz;
+  const double* vq;    // [map][y][4]: v1, v2, q1, q2
+  const int* umode;    // per mode: its map
+  const double *minv, *mkx, *mkz;
   cx<double>* scratch;  // work vectors [vec][mode][y]
   const int* act;       // if set and 0: the launch is skipped
   int Ny, M, Nkx;
@@ -878,18 +975,40 @@ __global__ void __launch_bounds__(kSweepLanes)
 // lane t forms output t of each chunk's map from the window, which the
 // mode's lanes share through LDS, so the dependent chain is nC chunks
 // instead of Ny rows, and every load is coalesced and independent of the
-// recurrence: a chunk's map row is loaded two chunks ahead, through three
+// recurrence: a chunk's map column is loaded two chunks ahead, through three
 // register slots (the chunk loop is unrolled by three so the slots are
 // static).  Loads retire in issue order, so each chunk's own loads are
 // issued before the prefetch: waiting for them leaves the prefetch in
 // flight.  The forward results stay in LDS for the back substitution.
-// NR right-hand sides per mode share the maps.
-template <int W>
-__device__ __forceinline__ void map_row(double (&f)[W], const double* __restrict__ base, size_t ks,
-                                        int c, int nC) {
-  const double* p = base + (size_t)(c < nC ? c : nC - 1) * W * ks;
+// NR right-hand sides per mode share the maps; the two modes of a
+// wavefront mostly share one map (the modes come in the order `sched`,
+// grouped by k^2), whose words are then fetched once for both.  The
+// packed map's structural zeros are not loaded; their products are still
+// formed, as zeros, so the sums are those of the full maps.
+template <int W, int LAG>
+__device__ __forceinline__ void map_row(double (&f)[W], const double* __restrict__ map,
+                                        int U, int u, int t, int c, int nC) {
+  // Every lane loads (branch-free): lanes below the column's first stored
+  // row read that row's word, which the use replaces by 0 (map_word; a
+  // select here would wait for the load); lanes past the window read the
+  // last row's and are never stored.  Byte offsets from the chunk's base
+  // fit 32 bits (the maps are at most 2^28 words).  opaque keeps the
+  // compiler from holding all W offsets live across chunks.
+  const char* p = reinterpret_cast<const char*>(map + (size_t)(c < nC ? c : nC - 1) * U *
+                                                          map_pre(W, LAG, W));
+  const int tt = opaque(t < W ? t : W - 1), uu = opaque(u);
 #pragma unroll
-  for (int k = 0; k < W; ++k) f[k] = p[(size_t)k * ks];
+  for (int k = 0; k < W; ++k) {
+    const int lo = map_lo(k, LAG), r = tt - lo;
+    const unsigned off = (unsigned)(U * map_pre(k, LAG, W) + uu * map_len(k, LAG, W) + (r > 0 ? r : 0));
+    f[k] = *reinterpret_cast<const double*>(p + (size_t)(off * 8u));
+  }
+}
+
+// Word k of lane t's map row as loaded by map_row: 0 in the structural zeros.
+template <int LAG>
+__device__ __forceinline__ double map_word(const double* f, int k, int t) {
+  return t >= map_lo(k, LAG) ? f[k] : 0.0;
 }
 
 // The window is shared by the mode's lanes, all in this wavefront, whose
@@ -899,12 +1018,13 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_wave_barrier();
 }
 
-// One forward chunk: window position t <-> row cB + t.  rin + r * vs is
-// right-hand side r (row stride 1).
-template <int PO, int NR>
+// One forward chunk: window position t <-> row cB + t.  vec holds the
+// right-hand sides (row stride 1, Ny rows each); the chunk overwrites rows
+// below (c + 1) B with results and reads rows from (c + 1) B + KL on.  pf
+// receives the map of chunk c + S - 1.
+template <int PO, int NR, int S>
 __device__ __forceinline__ void fwd_chunk(const double (&f)[kChunk + PO], double (&pf)[kChunk + PO],
-                                          const double* __restrict__ fbase, size_t ks,
-                                          const cx<double>* __restrict__ rin, size_t vs,
+                                          const double* __restrict__ fmap, int U, int u,
                                           cx<double>* vec, cx<double>* win, int t, int c, int nC,
                                           int Ny) {
   constexpr int KL = PO, B = kChunk, WF = B + KL;
@@ -912,18 +1032,19 @@ __device__ __forceinline__ void fwd_chunk(const double (&f)[kChunk + PO], double
   const bool fr = t >= KL && t < WF && frow < Ny;
   cx<double> fresh[NR];
 #pragma unroll
-  for (int r = 0; r < NR; ++r) fresh[r] = fr ? rin[r * vs + frow] : cx<double>{0, 0};
-  map_row<WF>(pf, fbase, ks, c + 2, nC);
+  for (int r = 0; r < NR; ++r) fresh[r] = fr ? vec[r * Ny + frow] : cx<double>{0, 0};
+  map_row<WF, KL>(pf, fmap, U, u, t, c + S - 1, nC);
   cx<double> acc[NR];
 #pragma unroll
   for (int r = 0; r < NR; ++r) acc[r] = cx<double>{0, 0};
 #pragma unroll
   for (int k = 0; k < WF; ++k) {
+    const double fk = map_word<KL>(f, k, t);
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
       const cx<double> w = win[r * 32 + k];
-      acc[r].x += f[k] * w.x;
-      acc[r].y += f[k] * w.y;
+      acc[r].x += fk * w.x;
+      acc[r].y += fk * w.y;
     }
   }
   wave_sync();
@@ -937,16 +1058,17 @@ __device__ __forceinline__ void fwd_chunk(const double (&f)[kChunk + PO], double
   wave_sync();
 }
 
-// One back chunk: window position t <-> row T - t, T = Ny - 1 - cB.
-template <int PO, int NR>
+// One back chunk: window position t <-> row T - t, T = Ny - 1 - cB; pg
+// receives the map of chunk c + S - 1.
+template <int PO, int NR, int S>
 __device__ __forceinline__ void back_chunk(const double (&g)[kChunk + 2 * PO],
                                            double (&pg)[kChunk + 2 * PO],
-                                           const double* __restrict__ gbase, size_t ks,
+                                           const double* __restrict__ bmap, int U, int u,
                                            cx<double>* __restrict__ out, size_t vs, bool live,
                                            cx<double>* vec, cx<double>* win, int t, int c, int nC,
                                            int Ny) {
   constexpr int KUU = 2 * PO, B = kChunk, WB = B + KUU;
-  map_row<WB>(pg, gbase, ks, c + 2, nC);
+  map_row<WB, 0>(pg, bmap, U, u, t, c + S - 1, nC);
   const int T = Ny - 1 - c * B;
   const int frow = T - B - t;
   const bool fr = t >= KUU && t < WB && frow >= 0;
@@ -958,11 +1080,12 @@ __device__ __forceinline__ void back_chunk(const double (&g)[kChunk + 2 * PO],
   for (int r = 0; r < NR; ++r) acc[r] = cx<double>{0, 0};
 #pragma unroll
   for (int k = 0; k < WB; ++k) {
+    const double gk = map_word<0>(g, k, t);
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
       const cx<double> w = win[r * 32 + k];
-      acc[r].x += g[k] * w.x;
-      acc[r].y += g[k] * w.y;
+      acc[r].x += gk * w.x;
+      acc[r].y += gk * w.y;
     }
   }
   wave_sync();
@@ -976,61 +1099,80 @@ __device__ __forceinline__ void back_chunk(const double (&g)[kChunk + 2 * PO],
   wave_sync();
 }
 
-template <int PO, int NR>
+template <int PO, int NR, int S>
 __global__ void __launch_bounds__(64)
     k_band_chunks(ImmArgs a, const double* __restrict__ fmap, const double* __restrict__ bmap,
-                  int vin, int vout) {
+                  const int2* __restrict__ sched, int U, int vin, int vout) {
   if (a.act && *a.act == 0) return;
   constexpr int KL = PO, KUU = 2 * PO, B = kChunk, WF = B + KL, WB = B + KUU;
   static_assert(WB <= 32, "a window fits the 32 lanes of a mode");
+  static_assert(S >= 2, "a map slot in use and at least one being filled");
   extern __shared__ cx<double> sh[];  // [2][NR][Ny] vectors, then [2][NR][32] windows
   const int Ny = a.Ny, M = a.M, nC = (Ny + B - 1) / B;
   const int lane = threadIdx.x, half = lane >> 5, t = lane & 31;
-  const int m0 = blockIdx.x * 2 + half;
-  const bool live = m0 < M;
-  const int m = live ? m0 : M - 1;  // a spare half repeats the last mode and stores nothing
+  const int slot = blockIdx.x * 2 + half;
+  const bool live = slot < M;  // a spare half repeats the last mode and stores nothing
+  const int2 mu = sched[slot];
+  const int m = mu.x, u = mu.y;
   cx<double>* vec = sh + (size_t)half * NR * Ny;
   cx<double>* win = sh + (size_t)2 * NR * Ny + (size_t)half * NR * 32;
   const size_t vs = (size_t)M * Ny;  // between work vectors (VS = 1: rows contiguous)
   const cx<double>* __restrict__ rin = a.scratch + ((size_t)vin * M + m) * Ny;
   cx<double>* __restrict__ out = a.scratch + ((size_t)vout * M + m) * Ny;
-  {
-    const size_t ks = (size_t)M * WF;
-    const double* __restrict__ fbase = fmap + (size_t)m * WF + (t < WF ? t : 0);
-    double f0[WF], f1[WF], f2[WF];
-    map_row<WF>(f0, fbase, ks, 0, nC);
-    map_row<WF>(f1, fbase, ks, 1, nC);
-    if (t < WF) {
+  // the first maps' loads, then the right-hand sides' to LDS (no global
+  // load on the chunk chain), eight rows per lane in flight at a time,
+  // all under one wait; lanes past the last row load
+  // and store that row again (the same value), so nothing branches
+  double f[S][WF];
 #pragma unroll
-      for (int r = 0; r < NR; ++r) win[r * 32 + t] = t < Ny ? rin[r * vs + t] : cx<double>{0, 0};
+  for (int q = 0; q < S - 1; ++q) map_row<WF, KL>(f[q], fmap, U, u, t, q, nC);
+  for (int i0 = 0; i0 < Ny; i0 += 8 * 32) {
+    cx<double> v[8][NR];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int i = min(i0 + q * 32 + t, Ny - 1);
+#pragma unroll
+      for (int r = 0; r < NR; ++r) v[q][r] = rin[r * vs + i];
     }
-    wave_sync();
-    for (int c = 0; c < nC; c += 3) {
-      fwd_chunk<PO, NR>(f0, f2, fbase, ks, rin, vs, vec, win, t, c, nC, Ny);
-      if (c + 1 >= nC) break;
-      fwd_chunk<PO, NR>(f1, f0, fbase, ks, rin, vs, vec, win, t, c + 1, nC, Ny);
-      if (c + 2 >= nC) break;
-      fwd_chunk<PO, NR>(f2, f1, fbase, ks, rin, vs, vec, win, t, c + 2, nC, Ny);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int i = min(i0 + q * 32 + t, Ny - 1);
+#pragma unroll
+      for (int r = 0; r < NR; ++r) vec[r * Ny + i] = v[q][r];
     }
   }
   {
-    const size_t ks = (size_t)M * WB;
-    const double* __restrict__ gbase = bmap + (size_t)m * WB + (t < WB ? t : 0);
-    double g0[WB], g1[WB], g2[WB];
-    map_row<WB>(g0, gbase, ks, 0, nC);
-    map_row<WB>(g1, gbase, ks, 1, nC);
+    wave_sync();
+    if (t < WF) {
+#pragma unroll
+      for (int r = 0; r < NR; ++r) win[r * 32 + t] = t < Ny ? vec[r * Ny + t] : cx<double>{0, 0};
+    }
+    wave_sync();
+    for (int c = 0; c < nC; c += S) {
+#pragma unroll
+      for (int q = 0; q < S; ++q) {
+        if (c + q >= nC) break;
+        fwd_chunk<PO, NR, S>(f[q], f[(q + S - 1) % S], fmap, U, u, vec, win, t, c + q, nC, Ny);
+      }
+    }
+  }
+  {
+    double g[S][WB];
+#pragma unroll
+    for (int q = 0; q < S - 1; ++q) map_row<WB, 0>(g[q], bmap, U, u, t, q, nC);
     if (t < WB) {
 #pragma unroll
       for (int r = 0; r < NR; ++r)
         win[r * 32 + t] = (Ny - 1 - t >= 0) ? vec[r * Ny + Ny - 1 - t] : cx<double>{0, 0};
     }
     wave_sync();
-    for (int c = 0; c < nC; c += 3) {
-      back_chunk<PO, NR>(g0, g2, gbase, ks, out, vs, live, vec, win, t, c, nC, Ny);
-      if (c + 1 >= nC) break;
-      back_chunk<PO, NR>(g1, g0, gbase, ks, out, vs, live, vec, win, t, c + 1, nC, Ny);
-      if (c + 2 >= nC) break;
-      back_chunk<PO, NR>(g2, g1, gbase, ks, out, vs, live, vec, win, t, c + 2, nC, Ny);
+    for (int c = 0; c < nC; c += S) {
+#pragma unroll
+      for (int q = 0; q < S; ++q) {
+        if (c + q >= nC) break;
+        back_chunk<PO, NR, S>(g[q], g[(q + S - 1) % S], bmap, U, u, out, vs, live, vec, win, t,
+                              c + q, nC, Ny);
+      }
     }
   }
 }
@@ -1098,9 +1240,10 @@ __global__ void __launch_bounds__(kImmT) k_imm_final(ImmArgs a) {
   const cx<double> al2 = {-(m10 * dw0.x + m11 * dw1.x), -(m10 * dw0.y + m11 * dw1.y)};
   const double km = (m % a.Nkx == 0) ? 1.0 : 2.0;
   double acc = 0.0;
+  const double4* __restrict__ vq = reinterpret_cast<const double4*>(a.vq) + (size_t)a.umode[m] * Ny;
   for (int y = threadIdx.x; y < Ny; y += blockDim.x) {
-    const double v1 = a.v1[(size_t)y * M + m], v2 = a.v2[(size_t)y * M + m];
-    const double q1 = a.q1[(size_t)y * M + m], q2 = a.q2[(size_t)y * M + m];
+    const double4 w = vq[y];
+    const double v1 = w.x, v2 = w.y, q1 = w.z, q2 = w.w;
     const cx<double> vv = va[(size_t)y * VS];
     const cx<double> vn = {(vv.x + al1.x * v1) + al2.x * v2, (vv.y + al1.y * v1) + al2.y * v2};
     const cx<double> q = {al1.x * q1 + al2.x * q2, al1.y * q1 + al2.y * q2};
@@ -1223,10 +1366,8 @@ static int walls_imm(Ctx* c, const double* un, const double* uj, const double* n
   a.hk_ab = t->hk_ab;
   a.lk_piv = t->lk_piv;
   a.hk_piv = t->hk_piv;
-  a.v1 = t->v1;
-  a.v2 = t->v2;
-  a.q1 = t->q1;
-  a.q2 = t->q2;
+  a.vq = t->vq;
+  a.umode = t->umode;
   a.minv = t->minv;
   a.mkx = t->mkx;
   a.mkz = t->mkz;
@@ -1255,15 +1396,15 @@ static int walls_imm(Ctx* c, const double* un, const double* uj, const double* n
       hipLaunchKernelGGL(k_band_sweeps<PO>, gs, dim3(kSweepLanes), 0, s, a, t->lk_ab, t->lk_piv,   \
                          (int)V_F, (int)V_T, (int)V_P);                                           \
     else                                                                                          \
-      hipLaunchKernelGGL((k_band_chunks<PO, 1>), gc, dim3(64), lds_c1, s, a, t->lk_fmap,          \
-                         t->lk_bmap, (int)V_F, (int)V_P);                                         \
+      hipLaunchKernelGGL((k_band_chunks<PO, 1, chunk_slots(PO, 1)>), gc, dim3(64), lds_c1, s, a, t->lk_fmap,          \
+                         t->lk_bmap, t->sched, t->U, (int)V_F, (int)V_P);                         \
     hipLaunchKernelGGL(k_imm_rc<PO>, gm, dim3(kImmT), lds, s, a);                                 \
     if (c->sweep_seq)                                                                             \
       hipLaunchKernelGGL(k_band_sweeps<PO>, gs3, dim3(kSweepLanes), 0, s, a, t->hk_ab, t->hk_piv, \
                          (int)V_R, (int)V_T, (int)V_R);                                           \
     else                                                                                          \
-      hipLaunchKernelGGL((k_band_chunks<PO, 3>), gc, dim3(64), lds_c3, s, a, t->hk_fmap,          \
-                         t->hk_bmap, (int)V_R, (int)V_R);                                         \
+      hipLaunchKernelGGL((k_band_chunks<PO, 3, chunk_slots(PO, 3)>), gc, dim3(64), lds_c3, s, a, t->hk_fmap,          \
+                         t->hk_bmap, t->sched, t->U, (int)V_R, (int)V_R);                         \
     hipLaunchKernelGGL(k_imm_final<PO>, gm, dim3(kImmT), 0, s, a);                                \
     break;
       IMM(2) IMM(4) IMM(6) IMM(8)
@@ -1304,6 +1445,8 @@ static int walls_autotune(Ctx* c) {
       c->release(*p);
       *p = nullptr;
     }
+    c->release(t->sched);
+    t->sched = nullptr;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
