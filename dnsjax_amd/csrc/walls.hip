@@ -207,23 +207,33 @@ constexpr int kChunk = 16;
 constexpr int kChunkSlots = 3;
 // two slots where three would spill (fd_order 8, three right-hand sides)
 constexpr int chunk_slots(int po, int nr) { return po == 8 && nr == 3 ? 2 : kChunkSlots; }
+// window words read from LDS per batch: four where eight would not leave
+// two waves per SIMD (one right-hand side, fd_order >= 6)
+constexpr int window_batch(int po, int nr) { return nr == 1 && po >= 6 ? 4 : 8; }
+// waves per SIMD the chunked solve asks for: two with one right-hand side
+// (registers within 256) up to fd_order 6, else one
+constexpr int chunk_waves(int po, int nr) { return nr == 1 && po <= 6 ? 2 : 1; }
 
 // Packed chunk maps.  Entry (t, k) of a map (out_t = sum_k map[t][k] w_k)
 // is zero by construction for k > t + lag: lag = kl for a forward map (a
 // row swap or elimination at window row t reaches at most kl rows down),
 // lag = 0 for a back map (row t draws only on the rows solved before it).
-// Column k is therefore stored from t = map_lo(k) on, map_len(k) words per
-// map, the maps' columns side by side: chunk c, column k, map u, row t at
-// map_offset (lanes t of one map read consecutive words).
+// Column k of a map is therefore stored as one zero word (read by the rows
+// t < map_lo(k)) followed by rows map_lo(k) .. w-1, map_len(k) words in
+// all; the maps' columns lie side by side: chunk c, column k, map u, row t
+// at map_offset (lanes t of one map read consecutive words).
 __host__ __device__ constexpr int map_lo(int k, int lag) { return k > lag ? k - lag : 0; }
-__host__ __device__ constexpr int map_len(int k, int lag, int w) { return w - map_lo(k, lag); }
+__host__ __device__ constexpr int map_len(int k, int lag, int w) { return w - map_lo(k, lag) + 1; }
 // words of columns 0 .. k-1
 __host__ __device__ constexpr int map_pre(int k, int lag, int w) {
-  return k * w - (k > lag + 1 ? (k - lag - 1) * (k - lag) / 2 : 0);
+  return k * (w + 1) - (k > lag + 1 ? (k - lag - 1) * (k - lag) / 2 : 0);
+}
+// word of row t (t < w) of column k of map u within one chunk's maps
+__host__ __device__ constexpr int map_word_at(int k, int u, int t, int lag, int w, int U) {
+  return U * map_pre(k, lag, w) + u * map_len(k, lag, w) + (t >= map_lo(k, lag) ? 1 + t - map_lo(k, lag) : 0);
 }
 static size_t map_offset(int c, int k, int u, int t, int lag, int w, int U) {
-  return (size_t)c * U * map_pre(w, lag, w) + (size_t)U * map_pre(k, lag, w) +
-         (size_t)u * map_len(k, lag, w) + (t - map_lo(k, lag));
+  return (size_t)c * U * map_pre(w, lag, w) + (size_t)map_word_at(k, u, t, lag, w, U);
 }
 
 // Chunk maps of one band solve (the operation order of band_solve): forward
@@ -259,7 +269,7 @@ static bool chunk_maps(int n, int kl, int ku, int u, int U, Raw raw, Piv piv, do
       for (int t = 0; t < WF; ++t) {
         const double x = X[(size_t)t * WF + k];
         if (t >= map_lo(k, kl)) F[map_offset(c, k, u, t, kl, WF, U)] = x;
-        else ok = ok && x == 0.0;
+        else ok = ok && x == 0.0;  // read as the column's zero word
       }
     const int T = n - 1 - j0;
     std::fill(X.begin(), X.end(), 0.0);
@@ -985,30 +995,53 @@ __global__ void __launch_bounds__(kSweepLanes)
 // grouped by k^2), whose words are then fetched once for both.  The
 // packed map's structural zeros are not loaded; their products are still
 // formed, as zeros, so the sums are those of the full maps.
+// Byte offsets of lane t's words in one chunk's maps (the same for every
+// chunk): rows below a column's first stored row read its zero word; lanes
+// past the window read row w-1 and are never stored.  They fit 32 bits
+// (the maps are at most 2^28 words).
 template <int W, int LAG>
-__device__ __forceinline__ void map_row(double (&f)[W], const double* __restrict__ map,
-                                        int U, int u, int t, int c, int nC) {
-  // Every lane loads (branch-free): lanes below the column's first stored
-  // row read that row's word, which the use replaces by 0 (map_word; a
-  // select here would wait for the load); lanes past the window read the
-  // last row's and are never stored.  Byte offsets from the chunk's base
-  // fit 32 bits (the maps are at most 2^28 words).  opaque keeps the
-  // compiler from holding all W offsets live across chunks.
-  const char* p = reinterpret_cast<const char*>(map + (size_t)(c < nC ? c : nC - 1) * U *
-                                                          map_pre(W, LAG, W));
-  const int tt = opaque(t < W ? t : W - 1), uu = opaque(u);
+__device__ __forceinline__ void map_offsets(unsigned (&o)[W], int U, int u, int t) {
+  const int tt = t < W ? t : W - 1;
 #pragma unroll
-  for (int k = 0; k < W; ++k) {
-    const int lo = map_lo(k, LAG), r = tt - lo;
-    const unsigned off = (unsigned)(U * map_pre(k, LAG, W) + uu * map_len(k, LAG, W) + (r > 0 ? r : 0));
-    f[k] = *reinterpret_cast<const double*>(p + (size_t)(off * 8u));
-  }
+  for (int k = 0; k < W; ++k) o[k] = 8u * (unsigned)map_word_at(k, u, tt, LAG, W, U);
 }
 
-// Word k of lane t's map row as loaded by map_row: 0 in the structural zeros.
-template <int LAG>
-__device__ __forceinline__ double map_word(const double* f, int k, int t) {
-  return t >= map_lo(k, LAG) ? f[k] : 0.0;
+// Lane t's row of chunk c's map (chunks past the last repeat it): a
+// wave-uniform base and the lane's offsets, no per-load arithmetic.
+template <int W, int LAG>
+__device__ __forceinline__ void map_row(double (&f)[W], const double* __restrict__ map,
+                                        const unsigned (&o)[W], int U, int c, int nC) {
+  const char* p = reinterpret_cast<const char*>(map + (size_t)(c < nC ? c : nC - 1) * U *
+                                                          map_pre(W, LAG, W));
+#pragma unroll
+  for (int k = 0; k < W; ++k) f[k] = *reinterpret_cast<const double*>(p + o[k]);
+}
+
+// acc[r] += sum_k map[t][k] win[r][k] in order k = 0, 1, ...; the window
+// words are read from LDS in batches of KB before their products, so that
+// a batch's reads are in flight together.
+template <int W, int NR, int KB>
+__device__ __forceinline__ void window_dot(cx<double> (&acc)[NR], const double (&f)[W],
+                                           const cx<double>* win) {
+#pragma unroll
+  for (int k0 = 0; k0 < W; k0 += KB) {
+    cx<double> w[KB][NR];
+#pragma unroll
+    for (int kk = 0; kk < KB; ++kk)
+#pragma unroll
+      for (int r = 0; r < NR; ++r)
+        if (k0 + kk < W) w[kk][r] = win[r * 32 + k0 + kk];
+#pragma unroll
+    for (int kk = 0; kk < KB; ++kk) {
+      if (k0 + kk >= W) break;
+      const double fk = f[k0 + kk];
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        acc[r].x += fk * w[kk][r].x;
+        acc[r].y += fk * w[kk][r].y;
+      }
+    }
+  }
 }
 
 // The window is shared by the mode's lanes, all in this wavefront, whose
@@ -1024,8 +1057,8 @@ __device__ __forceinline__ void wave_sync() {
 // receives the map of chunk c + S - 1.
 template <int PO, int NR, int S>
 __device__ __forceinline__ void fwd_chunk(const double (&f)[kChunk + PO], double (&pf)[kChunk + PO],
-                                          const double* __restrict__ fmap, int U, int u,
-                                          cx<double>* vec, cx<double>* win, int t, int c, int nC,
+                                          const double* __restrict__ fmap,
+                                          const unsigned (&o)[kChunk + PO], int U, cx<double>* vec, cx<double>* win, int t, int c, int nC,
                                           int Ny) {
   constexpr int KL = PO, B = kChunk, WF = B + KL;
   const int frow = (c + 1) * B + t;
@@ -1033,20 +1066,11 @@ __device__ __forceinline__ void fwd_chunk(const double (&f)[kChunk + PO], double
   cx<double> fresh[NR];
 #pragma unroll
   for (int r = 0; r < NR; ++r) fresh[r] = fr ? vec[r * Ny + frow] : cx<double>{0, 0};
-  map_row<WF, KL>(pf, fmap, U, u, t, c + S - 1, nC);
+  map_row<WF, KL>(pf, fmap, o, U, c + S - 1, nC);
   cx<double> acc[NR];
 #pragma unroll
   for (int r = 0; r < NR; ++r) acc[r] = cx<double>{0, 0};
-#pragma unroll
-  for (int k = 0; k < WF; ++k) {
-    const double fk = map_word<KL>(f, k, t);
-#pragma unroll
-    for (int r = 0; r < NR; ++r) {
-      const cx<double> w = win[r * 32 + k];
-      acc[r].x += fk * w.x;
-      acc[r].y += fk * w.y;
-    }
-  }
+  window_dot<WF, NR, window_batch(PO, NR)>(acc, f, win);
   wave_sync();
   const int row = c * B + t;
 #pragma unroll
@@ -1063,12 +1087,13 @@ __device__ __forceinline__ void fwd_chunk(const double (&f)[kChunk + PO], double
 template <int PO, int NR, int S>
 __device__ __forceinline__ void back_chunk(const double (&g)[kChunk + 2 * PO],
                                            double (&pg)[kChunk + 2 * PO],
-                                           const double* __restrict__ bmap, int U, int u,
+                                           const double* __restrict__ bmap,
+                                           const unsigned (&o)[kChunk + 2 * PO], int U,
                                            cx<double>* __restrict__ out, size_t vs, bool live,
                                            cx<double>* vec, cx<double>* win, int t, int c, int nC,
                                            int Ny) {
   constexpr int KUU = 2 * PO, B = kChunk, WB = B + KUU;
-  map_row<WB, 0>(pg, bmap, U, u, t, c + S - 1, nC);
+  map_row<WB, 0>(pg, bmap, o, U, c + S - 1, nC);
   const int T = Ny - 1 - c * B;
   const int frow = T - B - t;
   const bool fr = t >= KUU && t < WB && frow >= 0;
@@ -1078,16 +1103,7 @@ __device__ __forceinline__ void back_chunk(const double (&g)[kChunk + 2 * PO],
   cx<double> acc[NR];
 #pragma unroll
   for (int r = 0; r < NR; ++r) acc[r] = cx<double>{0, 0};
-#pragma unroll
-  for (int k = 0; k < WB; ++k) {
-    const double gk = map_word<0>(g, k, t);
-#pragma unroll
-    for (int r = 0; r < NR; ++r) {
-      const cx<double> w = win[r * 32 + k];
-      acc[r].x += gk * w.x;
-      acc[r].y += gk * w.y;
-    }
-  }
+  window_dot<WB, NR, window_batch(PO, NR)>(acc, g, win);
   wave_sync();
   const int row = T - t;
 #pragma unroll
@@ -1100,7 +1116,7 @@ __device__ __forceinline__ void back_chunk(const double (&g)[kChunk + 2 * PO],
 }
 
 template <int PO, int NR, int S>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(64, chunk_waves(PO, NR))
     k_band_chunks(ImmArgs a, const double* __restrict__ fmap, const double* __restrict__ bmap,
                   const int2* __restrict__ sched, int U, int vin, int vout) {
   if (a.act && *a.act == 0) return;
@@ -1124,8 +1140,10 @@ __global__ void __launch_bounds__(64)
   // all under one wait; lanes past the last row load
   // and store that row again (the same value), so nothing branches
   double f[S][WF];
+  unsigned of[WF];
+  map_offsets<WF, KL>(of, U, u, t);
 #pragma unroll
-  for (int q = 0; q < S - 1; ++q) map_row<WF, KL>(f[q], fmap, U, u, t, q, nC);
+  for (int q = 0; q < S - 1; ++q) map_row<WF, KL>(f[q], fmap, of, U, q, nC);
   for (int i0 = 0; i0 < Ny; i0 += 8 * 32) {
     cx<double> v[8][NR];
 #pragma unroll
@@ -1152,14 +1170,16 @@ __global__ void __launch_bounds__(64)
 #pragma unroll
       for (int q = 0; q < S; ++q) {
         if (c + q >= nC) break;
-        fwd_chunk<PO, NR, S>(f[q], f[(q + S - 1) % S], fmap, U, u, vec, win, t, c + q, nC, Ny);
+        fwd_chunk<PO, NR, S>(f[q], f[(q + S - 1) % S], fmap, of, U, vec, win, t, c + q, nC, Ny);
       }
     }
   }
   {
     double g[S][WB];
+    unsigned og[WB];
+    map_offsets<WB, 0>(og, U, u, t);
 #pragma unroll
-    for (int q = 0; q < S - 1; ++q) map_row<WB, 0>(g[q], bmap, U, u, t, q, nC);
+    for (int q = 0; q < S - 1; ++q) map_row<WB, 0>(g[q], bmap, og, U, q, nC);
     if (t < WB) {
 #pragma unroll
       for (int r = 0; r < NR; ++r)
@@ -1170,7 +1190,7 @@ __global__ void __launch_bounds__(64)
 #pragma unroll
       for (int q = 0; q < S; ++q) {
         if (c + q >= nC) break;
-        back_chunk<PO, NR, S>(g[q], g[(q + S - 1) % S], bmap, U, u, out, vs, live, vec, win, t,
+        back_chunk<PO, NR, S>(g[q], g[(q + S - 1) % S], bmap, og, U, out, vs, live, vec, win, t,
                               c + q, nC, Ny);
       }
     }
