@@ -505,6 +505,45 @@ def profile_workload(steps: int) -> None:
     print(f"mixed + learned guess: {steps} steps, corrector passes {g.passes}", flush=True)
 
 
+def profile_imm(calls: int) -> None:
+    """The plane Couette influence-matrix step (predict: one IMM on the
+    state's RHS) in both band-solve forms, chunk maps and sequential sweeps,
+    at 64x128x64 (p=4) and 128x256x128 (p=6), `calls` timed calls each after
+    five warm-up calls; prints the time per call, a checksum of the result
+    (equal for both forms up to rounding) and the autotuner's choice."""
+    import math
+
+    import torch
+
+    from dnsjax_amd.flows import PlaneCouetteFlow
+    from dnsjax_amd.params import Params
+
+    for nx, ny, nz, po in ((64, 128, 64, 4), (128, 256, 128, 6)):
+        base = dict(system="plane-couette", nx=nx, ny=ny, nz=nz, lx=2 * math.pi, lz=math.pi, re=400.0,
+                    dt=0.01, fd_order=po, corrector_tolerance=1e-5)
+        for form, variants in (("chunked", 0), ("sequential", 8)):
+            flow = PlaneCouetteFlow(Params(tile_cols=16, variants=variants, **base))
+            nat = flow._native
+            state = flow.init_state(seed=1, rms=0.05)
+            rhs = nat.get_rhs(state)
+            out = nat.empty()
+            for _ in range(5):
+                nat.predict(state, rhs, out)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            e0.record()
+            for _ in range(calls):
+                nat.predict(state, rhs, out)
+            e1.record()
+            torch.cuda.synchronize()
+            print(f"{nx}x{ny}x{nz} p={po} {form}: {e0.elapsed_time(e1) * 1e3 / calls:.1f} us per IMM step, "
+                  f"checksum {float(out.abs().sum()):.15g}", flush=True)
+            del flow, nat
+        flow = PlaneCouetteFlow(Params(**base))
+        print(f"{nx}x{ny}x{nz} p={po} autotuner: {flow._native.tuning}", flush=True)
+        del flow
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -522,6 +561,8 @@ def main():
                     help="run a few steps of the plane Couette form for rocprofv3 and exit")
     ap.add_argument("--profile-workload", action="store_true",
                     help="run the rocprofv3 workload (calibration copy + a few steps) and exit")
+    ap.add_argument("--profile-imm", action="store_true",
+                    help="time the plane Couette influence-matrix step in both band-solve forms and exit")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); the multi-GPU slab "
@@ -532,6 +573,9 @@ def main():
         return
     if a.profile_walls:
         print(json.dumps(walls_form(a.steps, warmup=1)), flush=True)
+        return
+    if a.profile_imm:
+        profile_imm(a.steps)
         return
     os.environ.setdefault("OMP_NUM_THREADS", "1")
     fp64 = gpu_form("fp64", a.steps, a.warmup)
