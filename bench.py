@@ -315,6 +315,35 @@ def learned_form(steps: int, collect_steps: int = 30, hidden: int = 32, train_it
             "segment": f"{steps} steps after a {collect_steps}-step training run from the workload's initial state"}
 
 
+def walls_model_bytes(nx: int, ny: int, nz: int, p: int, b: int = 16) -> dict:
+    """Fixed algorithmic byte models of one plane Couette RHS (five-pass
+    pencil model: b * Nx1 * Ny * (6 Nz1 + 18 Pz)) and one FP64 influence-
+    matrix step (5 S + 2 ab + 2 piv + 4 vec + M_inv: the fields read and
+    written, the two band factors and pivots, the homogeneous solutions and
+    the 2x2 inverses), with Nx1 = nx/2, Nz1 = nz - 1, Pz = 3 nz / 2."""
+    nx1, nz1, pz = nx // 2, nz - 1, 3 * nz // 2
+    m = nx1 * nz1
+    spec = 3 * m * ny * 16
+    imm = 5 * spec + 2 * m * (3 * p + 1) * ny * 8 + 2 * m * ny * 4 + 4 * m * ny * 8 + m * 4 * 8
+    return {"rhs": b * nx1 * ny * (6 * nz1 + 18 * pz), "imm": imm}
+
+
+def walls_roofline(prof: dict, nx: int, ny: int, nz: int, p: int) -> dict:
+    """Time per call of the RHS (curl and z-inverse, x pass, z-forward and
+    transpose) and of the IMM step from the per-kernel event times, against
+    the byte models and the 8 TB/s HBM peak."""
+    us = {k: 1e3 * ms / n for k, (ms, n) in prof.items()}
+    mb = walls_model_bytes(nx, ny, nz, p)
+    t_rhs = us.get("walls_z_inverse", 0.0) + us.get("walls_x_nonlinear", 0.0) + us.get("walls_z_forward", 0.0)
+    t_imm = us.get("walls_imm", 0.0)
+    out = {}
+    for name, t, by in (("rhs", t_rhs, mb["rhs"]), ("imm", t_imm, mb["imm"])):
+        gbs = by / (t * 1e-6) / 1e9 if t > 0 else 0.0
+        out[name] = {"model_bytes": by, "t_us": round(t, 2), "achieved": round(gbs, 1), "peak": 8000.0,
+                     "unit": "GB/s", "frac": round(gbs / 8000.0, 4)}
+    return out
+
+
 def walls_form(steps: int, warmup: int = 2, collect_steps: int = 30, train_iters: int = 1500,
                tol: float = 1e-5) -> dict:
     """Plane Couette 64 x 128 x 64 (CGL points in y), Lx = 2 pi, Lz = pi,
@@ -391,6 +420,7 @@ def walls_form(steps: int, warmup: int = 2, collect_steps: int = 30, train_iters
             "corrector_iters_per_step": round(sum(r.counts) / max(len(r.counts), 1), 3),
             "converged": r.converged and r2.converged,
             "kernels_avg_us": {k: round(1e3 * ms / n, 2) for k, (ms, n) in prof.items()},
+            "roofline": walls_roofline(prof, 64, 128, 64, 4),
             "mixed": {"steps_per_s": round(steps / tm, 3), "rhs_evals_per_step": round(rm.rhs_evals / steps, 3),
                       "converged": rm.converged, "energy_rel_err_vs_fp64": err(rm.state)},
             "mixed_guess": {"steps_per_s": round(steps / tg, 3), "rhs_evals_per_step": round(rg.rhs_evals / steps, 3),
