@@ -571,11 +571,13 @@ int walls_setup(Ctx* c) {
   if ((rc = upload_vec(c, &t->hk_ab, hk_ab))) return rc;
   // chunk maps of the L_k and H_k solves, one per distinct k^2, built in
   // parallel (not when the sequential sweeps are requested)
-  if (!(g.tile_cols != 0 && (g.variants & 8) != 0)) {
-    const int B = kChunk, WF = B + kl, WB = B + kl + ku, nC = (Ny + B - 1) / B;
-    const size_t nf = (size_t)nC * U * map_pre(WF, kl, WF), nb = (size_t)nC * U * map_pre(WB, 0, WB);
-    if (nf >= (1ull << 28) || nb >= (1ull << 28))
-      return c->fail(DNSMI_EINVAL, "chunk maps exceed 2^28 words: use the sequential band solve");
+  const int B = kChunk, WF = B + kl, WB = B + kl + ku, nC = (Ny + B - 1) / B;
+  const size_t nf = (size_t)nC * U * map_pre(WF, kl, WF), nb = (size_t)nC * U * map_pre(WB, 0, WB);
+  const bool maps_fit = nf < (1ull << 28) && nb < (1ull << 28);
+  const bool forced = g.tile_cols != 0;  // the variant bits choose the form
+  if (forced && (g.variants & 8) == 0 && !maps_fit)
+    return c->fail(DNSMI_EINVAL, "chunk maps exceed 2^28 words: use the sequential band solve");
+  if (maps_fit && !(forced && (g.variants & 8) != 0)) {
     std::vector<double> lf(nf), lb(nb), hf(nf), hb(nb);
     const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     std::vector<char> ok(nth, 1);
@@ -618,7 +620,12 @@ int walls_setup(Ctx* c) {
   if ((rc = upload_vec(c, &t->minv, minv))) return rc;
   if ((rc = upload_vec(c, &t->mkx, mkx))) return rc;
   if ((rc = upload_vec(c, &t->mkz, mkz))) return rc;
-  if (g.tile_cols == 0) return walls_autotune(c);
+  if (!forced && !maps_fit) {  // the autotuner has one form to choose
+    c->sweep_seq = true;
+    c->tuning = "band_solve=sequential(chunk maps exceed 2^28 words) ";
+    return DNSMI_OK;
+  }
+  if (!forced) return walls_autotune(c);
   return DNSMI_OK;
 }
 
