@@ -830,14 +830,24 @@ template <int PO>
 __global__ void __launch_bounds__(kImmT) k_imm_fhat(ImmArgs a) {
   if (a.act && *a.act == 0) return;
   constexpr int S1 = PO + 1, S2 = PO + 2;
+  // LDS: d_hat, then the y rows of u^n_y, N_j,y and N_n,y (the D1 stencils'
+  // operands, read once from HBM)
   extern __shared__ cx<double> dh[];
   const int m = blockIdx.x, Ny = a.Ny;
   const size_t comp = (size_t)a.M * Ny, row = (size_t)m * Ny;
   const double kx = a.mkx[m], kz = a.mkz[m];
   const double k2 = kx * kx + kz * kz;
   const cx<double>* un0 = a.un + row;
-  const cx<double>* un1 = a.un + comp + row;
   const cx<double>* un2 = a.un + 2 * comp + row;
+  cx<double>* un1 = dh + Ny;
+  cx<double>* nj1 = dh + 2 * Ny;
+  cx<double>* nn1 = dh + 3 * Ny;
+  for (int y = threadIdx.x; y < Ny; y += blockDim.x) {
+    un1[y] = a.un[comp + row + y];
+    nj1[y] = a.nj[comp + row + y];
+    nn1[y] = a.nn[comp + row + y];
+  }
+  __syncthreads();
   for (int y = threadIdx.x; y < Ny; y += blockDim.x) {
     const cx<double> dv = stencil<S1>(a.d1w, a.d1j0, un1, y);
     const cx<double> t0 = ikm(kx, un0[y]);
@@ -854,8 +864,8 @@ __global__ void __launch_bounds__(kImmT) k_imm_fhat(ImmArgs a) {
       continue;
     }
     const cx<double> d = dh[y];
-    const cx<double> dvj = stencil<S1>(a.d1w, a.d1j0, a.nj + comp + row, y);
-    const cx<double> dvn = stencil<S1>(a.d1w, a.d1j0, a.nn + comp + row, y);
+    const cx<double> dvj = stencil<S1>(a.d1w, a.d1j0, nj1, y);
+    const cx<double> dvn = stencil<S1>(a.d1w, a.d1j0, nn1, y);
     const cx<double> aj = ikm(kx, a.nj[row + y]), cj = ikm(kz, a.nj[2 * comp + row + y]);
     const cx<double> an = ikm(kx, a.nn[row + y]), cn = ikm(kz, a.nn[2 * comp + row + y]);
     const cx<double> divj = {(aj.x + dvj.x) + cj.x, (aj.y + dvj.y) + cj.y};
@@ -1210,6 +1220,7 @@ template <int PO>
 __global__ void __launch_bounds__(kImmT) k_imm_rc(ImmArgs a) {
   if (a.act && *a.act == 0) return;
   constexpr int S1 = PO + 1, S2 = PO + 2;
+  // LDS: the p_P row, then the rows of u^n (the D2 stencils' operands)
   extern __shared__ cx<double> pp[];
   const int m = blockIdx.x, Ny = a.Ny;
   const size_t comp = (size_t)a.M * Ny, row = (size_t)m * Ny;
@@ -1217,7 +1228,12 @@ __global__ void __launch_bounds__(kImmT) k_imm_rc(ImmArgs a) {
   const double k2 = kx * kx + kz * kz;
   const cx<double>* P = svec(a, V_P, m);
   const size_t M = a.M;
-  for (int y = threadIdx.x; y < Ny; y += blockDim.x) pp[y] = P[y * VS];
+  cx<double>* ur = pp + Ny;  // [c][y]
+  for (int y = threadIdx.x; y < Ny; y += blockDim.x) {
+    pp[y] = P[y * VS];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) ur[c * Ny + y] = a.un[c * comp + row + y];
+  }
   __syncthreads();
   const double dt = a.dt, cc = a.c, nu = a.nu;
   for (int y = threadIdx.x; y < Ny; y += blockDim.x) {
@@ -1230,7 +1246,7 @@ __global__ void __launch_bounds__(kImmT) k_imm_rc(ImmArgs a) {
     const cx<double> dpp[3] = {ikm(kx, p), stencil<S1>(a.d1w, a.d1j0, pp, y), ikm(kz, p)};
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const cx<double>* unc = a.un + c * comp + row;
+      const cx<double>* unc = ur + c * Ny;
       const cx<double> u = unc[y];
       const cx<double> du = stencil<S2>(a.d2w, a.d2j0, unc, y);
       const cx<double> l = {du.x - k2 * u.x, du.y - k2 * u.y};
@@ -1415,17 +1431,19 @@ static int walls_imm(Ctx* c, const double* un, const double* uj, const double* n
     // chunked solves: per workgroup two modes' vectors and windows
     const size_t lds_c1 = (size_t)2 * 1 * (t->Ny + 32) * sizeof(cx<double>);
     const size_t lds_c3 = (size_t)2 * 3 * (t->Ny + 32) * sizeof(cx<double>);
+    if (4 * lds > 65536 || (!c->sweep_seq && lds_c3 > 65536))
+      return c->fail(DNSMI_EINVAL, "imm: ny too large for the per-mode LDS rows");
     switch (t->p) {
 #define IMM(PO)                                                                                   \
   case PO:                                                                                        \
-    hipLaunchKernelGGL(k_imm_fhat<PO>, gm, dim3(kImmT), lds, s, a);                               \
+    hipLaunchKernelGGL(k_imm_fhat<PO>, gm, dim3(kImmT), 4 * lds, s, a);                           \
     if (c->sweep_seq)                                                                             \
       hipLaunchKernelGGL(k_band_sweeps<PO>, gs, dim3(kSweepLanes), 0, s, a, t->lk_ab, t->lk_piv,   \
                          (int)V_F, (int)V_T, (int)V_P);                                           \
     else                                                                                          \
       hipLaunchKernelGGL((k_band_chunks<PO, 1, chunk_slots(PO, 1)>), gc, dim3(64), lds_c1, s, a, t->lk_fmap,          \
                          t->lk_bmap, t->sched, t->U, (int)V_F, (int)V_P);                         \
-    hipLaunchKernelGGL(k_imm_rc<PO>, gm, dim3(kImmT), lds, s, a);                                 \
+    hipLaunchKernelGGL(k_imm_rc<PO>, gm, dim3(kImmT), 4 * lds, s, a);                             \
     if (c->sweep_seq)                                                                             \
       hipLaunchKernelGGL(k_band_sweeps<PO>, gs3, dim3(kSweepLanes), 0, s, a, t->hk_ab, t->hk_piv, \
                          (int)V_R, (int)V_T, (int)V_R);                                           \
