@@ -164,21 +164,40 @@ class PeriodicParams:
 
 
 # ------------------------------------------------------------ transforms
+# The axis transforms are numpy.fft's (pocketfft, one thread).  For timing
+# the oracle on several host cores, fft_workers(n) routes them through
+# scipy.fft with n worker threads (the same algorithm family; the rounding
+# may differ in the last place, so the tests keep the default).
+_FFT = np.fft
+_FFT_KW: dict = {}
+
+
+def fft_workers(n: int | None) -> None:
+    """n > 1: scipy.fft with n workers; None or 1: numpy.fft."""
+    global _FFT, _FFT_KW
+    if n and n > 1:
+        import scipy.fft
+
+        _FFT, _FFT_KW = scipy.fft, {"workers": int(n)}
+    else:
+        _FFT, _FFT_KW = np.fft, {}
+
+
 def irfft3d(x: np.ndarray, p: PeriodicParams) -> np.ndarray:
     """``fft.py:351-408``: pad y, ifft y, pad z, ifft z, pad kx, irfft x."""
     y = zeropad_fft(x, p.ny_padded, 0)
-    y = np.fft.ifft(y, axis=0, norm="forward")
+    y = _FFT.ifft(y, axis=0, norm="forward", **_FFT_KW)
     y = zeropad_fft(y, p.nz_padded, 1)
-    y = np.fft.ifft(y, axis=1, norm="forward")
+    y = _FFT.ifft(y, axis=1, norm="forward", **_FFT_KW)
     y = zeropad_rfft(y, p.nx_padded // 2 + 1)
-    return np.fft.irfft(y, axis=2, norm="forward")
+    return _FFT.irfft(y, axis=2, norm="forward", **_FFT_KW)
 
 
 def rfft3d(x: np.ndarray, p: PeriodicParams) -> np.ndarray:
     """``fft.py:291-348``: rfft x, truncate, fft z, truncate, fft y, truncate."""
-    y = truncate_rfft(np.fft.rfft(x, axis=2, norm="forward"), p.nx // 2)
-    y = truncate_fft(np.fft.fft(y, axis=1, norm="forward"), p.nz, 1)
-    y = truncate_fft(np.fft.fft(y, axis=0, norm="forward"), p.ny, 0)
+    y = truncate_rfft(_FFT.rfft(x, axis=2, norm="forward", **_FFT_KW), p.nx // 2)
+    y = truncate_fft(_FFT.fft(y, axis=1, norm="forward", **_FFT_KW), p.nz, 1)
+    y = truncate_fft(_FFT.fft(y, axis=0, norm="forward", **_FFT_KW), p.ny, 0)
     return y
 
 

@@ -200,3 +200,22 @@ def test_stats_power_balance_closed_forms(system, L, tilt):
     st = P.get_stats(rest, flow)
     assert abs(st["E"]) < 1e-14 and abs(st["I"]) < 1e-14 * F and abs(st["D"]) < 1e-12 * F
     assert abs(st["E'"] - 0.25) < 1e-14
+
+
+def test_fft_workers_switch_keeps_the_transforms():
+    """The host-timing switch (scipy.fft with worker threads) gives the same
+    round trip and forward transform as the default numpy path, to rounding,
+    and switching back restores numpy.fft."""
+    p = P.PeriodicParams(nx=16, ny=16, nz=16, lx=2 * np.pi, ly=2 * np.pi, lz=2 * np.pi)
+    rng = np.random.default_rng(0)
+    phys = rng.standard_normal(p.phys_shape)
+    ref = P.rfft3d(phys, p)
+    P.fft_workers(4)
+    try:
+        got = P.rfft3d(phys, p)
+        back = P.irfft3d(got, p)
+    finally:
+        P.fft_workers(None)
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-15)
+    np.testing.assert_allclose(back, P.irfft3d(ref, p), rtol=0, atol=1e-13)
+    assert P._FFT is np.fft and P._FFT_KW == {}
