@@ -638,23 +638,24 @@ void walls_release(Ctx* c) {
 // WT1: FD curl + transpose.  In: u (3, M, Ny) with M = Nkz*Nkx modes, FP64.
 // Out: V (6, Ny, Nkz, Nkx) in T: u_x, u_y, u_z, w_x = D1 u_z - i kz u_y,
 // i kz u_x, -D1 u_x (the i kx parts of w_y, w_z are added by the x pass).
-// A block handles TY y-points x TM modes; loads run along y, stores along
-// the mode index.
-constexpr int kTY = 16, kTM = 16;
+// A block handles kCurlTY y-points x TM modes; loads run along y, stores
+// along the mode index.  The curl's tiles are taller than the transposes'
+// (kTY): each re-reads a halo of S1 rows on either side.
+constexpr int kTY = 16, kTM = 16, kCurlTY = 32;
 
 template <typename T, int S1>
-__global__ void __launch_bounds__(kTY* kTM)
+__global__ void __launch_bounds__(kCurlTY* kTM)
     k_walls_curl_t(const cx<double>* __restrict__ u, cx<T>* __restrict__ v,
                    const double* __restrict__ d1w, const int* __restrict__ d1j0,
                    const double* __restrict__ mkz, int Ny, int M, const int* __restrict__ act) {
   if (act && *act == 0) return;
   constexpr int HALO = S1;
-  __shared__ cx<double> tile[3][kTM][kTY + 2 * HALO + 1];
-  const int ntile_y = (Ny + kTY - 1) / kTY;
-  const int y0 = (blockIdx.x % ntile_y) * kTY;
+  __shared__ cx<double> tile[3][kTM][kCurlTY + 2 * HALO + 1];
+  const int ntile_y = (Ny + kCurlTY - 1) / kCurlTY;
+  const int y0 = (blockIdx.x % ntile_y) * kCurlTY;
   const int m0 = (blockIdx.x / ntile_y) * kTM;
   const int ylo = max(0, y0 - HALO);
-  const int yhi = min(Ny, y0 + kTY + HALO);
+  const int yhi = min(Ny, y0 + kCurlTY + HALO);
   const int span = yhi - ylo;
   const size_t comp = (size_t)M * Ny;
   // load: lanes along y
@@ -1342,13 +1343,13 @@ static int walls_rhs_t(Ctx* c, const double* state, double* rhs, hipStream_t s) 
   const int ntile_y = (Ny + kTY - 1) / kTY, ntile_m = (M + kTM - 1) / kTM;
   {
     ProfScope ps(c, KID_W_Z_INVERSE, s);
-    const dim3 grid(ntile_y * ntile_m);
+    const dim3 grid(((Ny + kCurlTY - 1) / kCurlTY) * ntile_m);
     const cx<double>* u = reinterpret_cast<const cx<double>*>(state);
     switch (t->p) {
 #define CURL(PO)                                                                                \
   case PO:                                                                                      \
-    hipLaunchKernelGGL((k_walls_curl_t<T, PO + 1>), grid, dim3(kTY * kTM), 0, s, u, A, t->d1w,  \
-                       t->d1j0, t->mkz, Ny, M, c->act);                                         \
+    hipLaunchKernelGGL((k_walls_curl_t<T, PO + 1>), grid, dim3(kCurlTY * kTM), 0, s, u, A,      \
+                       t->d1w, t->d1j0, t->mkz, Ny, M, c->act);                                 \
     break;
       CURL(2) CURL(4) CURL(6) CURL(8)
 #undef CURL
