@@ -1003,16 +1003,15 @@ __global__ void __launch_bounds__(kSweepLanes)
 // lane t forms output t of each chunk's map from the window, which the
 // mode's lanes share through LDS, so the dependent chain is nC chunks
 // instead of Ny rows, and every load is coalesced and independent of the
-// recurrence: a chunk's map column is loaded two chunks ahead, through three
-// register slots (the chunk loop is unrolled by three so the slots are
-// static).  Loads retire in issue order, so each chunk's own loads are
-// issued before the prefetch: waiting for them leaves the prefetch in
-// flight.  The forward results stay in LDS for the back substitution.
-// NR right-hand sides per mode share the maps; the two modes of a
-// wavefront mostly share one map (the modes come in the order `sched`,
-// grouped by k^2), whose words are then fetched once for both.  The
-// packed map's structural zeros are not loaded; their products are still
-// formed, as zeros, so the sums are those of the full maps.
+// recurrence.  The right-hand sides are staged in LDS at the start; a
+// chunk's map is loaded S - 1 chunks ahead through S register slots (the
+// chunk loop is unrolled by S so the slots are static).  The forward
+// results stay in LDS for the back substitution.  NR right-hand sides per
+// mode share the maps; the two modes of a wavefront mostly share one map
+// (the modes come in the order `sched`, grouped by k^2), whose words are
+// then fetched once for both.  Lanes in a packed column's structural zeros
+// read its zero word, so the sums are those of the full maps.
+
 // Byte offsets of lane t's words in one chunk's maps (the same for every
 // chunk): rows below a column's first stored row read its zero word; lanes
 // past the window read row w-1 and are never stored.  They fit 32 bits
@@ -1154,9 +1153,9 @@ __global__ void __launch_bounds__(64, chunk_waves(PO, NR))
   const cx<double>* __restrict__ rin = a.scratch + ((size_t)vin * M + m) * Ny;
   cx<double>* __restrict__ out = a.scratch + ((size_t)vout * M + m) * Ny;
   // the first maps' loads, then the right-hand sides' to LDS (no global
-  // load on the chunk chain), eight rows per lane in flight at a time,
-  // all under one wait; lanes past the last row load
-  // and store that row again (the same value), so nothing branches
+  // load on the chunk chain), eight rows per lane in flight at a time, all
+  // under one wait; lanes past the last row load and store that row again
+  // (the same value), so nothing branches
   double f[S][WF];
   unsigned of[WF];
   map_offsets<WF, KL>(of, U, u, t);
