@@ -415,9 +415,31 @@ def walls_form(steps: int, warmup: int = 2, collect_steps: int = 30, train_iters
     rm, tm = timed(lambda: run(mixed, state.clone(), steps))
     rg, tg = timed(lambda: run_guessed(mixed, state.clone(), steps, mode=3))
     rl, tl = timed(lambda: run_guessed(mixed, state.clone(), steps, mode=2, net=net, scales=sc))
+    # The plain FP64 and the mixed + learned forms again with the corrector
+    # decided on the device, one captured graph launch per step.  Each graph
+    # holds one corrector slot more than the largest count of its eager run
+    # (a skipped slot still costs its launches); a step needing more would
+    # be flagged unconverged by the device counters.
+    from dnsjax_amd.graph import GraphStepper
+    from dnsjax_amd.learned import GuessedGraphStepper
+
+    gp = GraphStepper(flow, state, slots=max(r.counts) + 1, capacity=steps + 4)
+    gp.step(2)
+    _, tgp = timed(lambda: gp.step(steps))
+    gg = GuessedGraphStepper(mixed, state, mode=2, net=net, scales=sc, slots=max(rl.passes) + 1,
+                             capacity=steps + 4)
+    gg.step(2)
+    _, tgg = timed(lambda: gg.step(steps))
+    # their end states include their start-up and warm-up steps (gg.steps of
+    # them for the guessed stepper): compare with an eager FP64 run of the
+    # same length
+    e_graph_ref = float(flow.get_stats(run(flow, state.clone(), gg.steps).state)["E'"])
 
     def err(st):
         return abs(float(flow.get_stats(st)["E'"]) - e_ref) / abs(e_ref)
+
+    def err_graph(st):
+        return abs(float(flow.get_stats(st)["E'"]) - e_graph_ref) / abs(e_graph_ref)
 
     return {"workload": f"plane Couette 64x128x64, p=4, Re 400, dt 0.01, RMS 0.05, tol {tol:g}"
                         + (" (reference default)" if tol == 1e-5 else ""),
@@ -440,6 +462,12 @@ def walls_form(steps: int, warmup: int = 2, collect_steps: int = 30, train_iters
                               "net": {"hidden": 32, "train_iters": train_iters, "train_s": round(t_train, 3),
                                       "data_s": round(t_data, 3), "heldout_loss": round(loss, 4),
                                       "heldout_loss_without_net": round(base, 4)}},
+            "graph": {"fp64": {"steps_per_s": round(steps / tgp, 3), "slots": gp.members[0].slots,
+                               "converged": gp.step_stats()[0]["unconverged"] == 0},
+                      "mixed_learned": {"steps_per_s": round(steps / tgg, 3), "slots": gg.slots,
+                                        "converged": gg.step_stats()["unconverged"] == 0,
+                                        "energy_rel_err_vs_fp64": err_graph(gg.state)},
+                      "speedup_vs_fp64": round(tgp / tgg, 3)},
             "fp64_rhs_evals_per_step": round(r.rhs_evals / steps, 3),
             "segment": f"{steps} steps after a {collect_steps}-step mixed data run from the workload's state"}
 
