@@ -480,10 +480,11 @@ def ensemble_form(steps: int, n_streams: int = 8) -> dict:
     driver.  Aggregate time-steps/s over all members.
 
     Every captured step launches a fixed number of corrector slots (skipped
-    slots still launch their kernels, which exit at once).  The ensemble takes
-    one slot more than the largest iteration count of the sequential runs;
-    the device counters flag any step that would need more (`unconverged`),
-    and every member's counts must equal its sequential run's."""
+    slots still launch their kernels, which exit at once).  Each member takes
+    one slot more than the largest iteration count of its own sequential
+    run; the device counters flag any step that would need more
+    (`unconverged`), and every member's counts must equal its sequential
+    run's."""
     import numpy as np
     import torch
 
@@ -507,7 +508,7 @@ def ensemble_form(steps: int, n_streams: int = 8) -> dict:
     seq_counts = [run(f, s.clone(), steps).counts for f, s in zip(flows, states)]
     torch.cuda.synchronize()
     t_seq = time.perf_counter() - t0
-    slots = min(max(c for cs in seq_counts for c in cs) + 1, 10)
+    slots = [min(max(cs) + 1, 10) for cs in seq_counts]
     ens = Ensemble(flows, states, n_streams=n_streams, slots=slots, capacity=steps)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -517,7 +518,7 @@ def ensemble_form(steps: int, n_streams: int = 8) -> dict:
     same = ens.counts() == seq_counts
     st = ens.step_stats()
     return {"members": m, "sizes": "16 x {32,48,64,96}^3", "steps": steps, "n_streams": n_streams,
-            "slots": slots,
+            "slots": {"min": min(slots), "max": max(slots), "mean": round(sum(slots) / m, 2)},
             "value": round(m * steps / t_ens, 2), "unit": "member time-steps/s",
             "sequential_value": round(m * steps / t_seq, 2),
             "speedup_vs_sequential": round(t_seq / t_ens, 3),

@@ -40,13 +40,19 @@ class _Member:
 
 
 class Ensemble:
-    """Trajectories advanced together by one captured graph launch per step."""
+    """Trajectories advanced together by one captured graph launch per step.
 
-    def __init__(self, flows, states, n_streams: int = 8, slots: int | None = None,
-                 capacity: int = 4096):
+    `slots` is the corrector slots per captured step: one value for every
+    member, or one per member (a member's skipped slots still launch their
+    kernels, so each can hold just what its own trajectory needs)."""
+
+    def __init__(self, flows, states, n_streams: int = 8, slots=None, capacity: int = 4096):
         if len(flows) != len(states) or not flows:
             raise ValueError("need one state per flow")
-        self.members = [_Member(f, s, slots, capacity) for f, s in zip(flows, states)]
+        per = list(slots) if isinstance(slots, (list, tuple)) else [slots] * len(flows)
+        if len(per) != len(flows):
+            raise ValueError("need one slot count per flow")
+        self.members = [_Member(f, s, k, capacity) for f, s, k in zip(flows, states, per)]
         self.capacity = capacity
         self.parity = 0
         self.steps = 0

@@ -93,3 +93,27 @@ def test_ensemble_members_match_single_runs():
             assert torch.equal(ens.states()[m], states[i]), (m, i)
     for m, (_, counts) in enumerate(refs):
         assert ens.counts()[m] == counts
+
+
+def test_ensemble_with_per_member_slots():
+    """One slot count per member, each one more than its own eager run's
+    largest count: every member still equals its eager run, with no step
+    flagged unconverged."""
+    from dnsjax_amd.graph import Ensemble
+
+    specs = [(16, 25.0), (32, 40.0), (16, 70.0)]
+    n = 4
+    flows = [periodic(sz, re, "mixed") for sz, re in specs]
+    states0 = [f.init_state(seed=310 + i, rms=0.05) for i, f in enumerate(flows)]
+    refs = [eager_trajectory(periodic(sz, re, "mixed"), u0, n) for (sz, re), u0 in zip(specs, states0)]
+    slots = [max(counts) + 1 for _, counts in refs]
+    ens = Ensemble(flows, states0, n_streams=2, slots=slots)
+    assert [mb.slots for mb in ens.members] == slots
+    ens.step(n)
+    torch.cuda.synchronize()
+    for m, (states, counts) in enumerate(refs):
+        assert torch.equal(ens.states()[m], states[-1]), m
+        assert ens.counts()[m] == counts
+    assert all(x["unconverged"] == 0 for x in ens.step_stats())
+    with pytest.raises(ValueError):
+        Ensemble(flows, states0, slots=[3, 3])
