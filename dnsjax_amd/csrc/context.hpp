@@ -34,6 +34,14 @@ struct ProfRec {
   hipEvent_t a, b;
 };
 
+// The corrector rule of a device step (tolerance, iteration limit and the
+// mixed form's noise-floor stops: ratio, cap, margin; 0 disables them).
+struct CtlRule {
+  double tol;
+  int max_iter;
+  double ratio, cap, margin;
+};
+
 struct Ctx {
   dnsmi_config cfg{};
   bool walls = false;
@@ -70,7 +78,11 @@ struct Ctx {
   // Device-side corrector control (graph-capturable steps): when `act` is
   // set, every step kernel first reads *act and skips its launch if 0.
   const int* act = nullptr;
-  int* ctl = nullptr;  // device control block (see step.hip)
+  int* ctl = nullptr;  // device control block (StepCtl, ctl.hpp)
+  // While set (the corrector slots of a device step), the final norm
+  // reduction of each pass also applies `rule` to the control block.
+  bool fuse_ctl = false;
+  CtlRule rule{};
 
   int tile_cols = 16;
   // kernel variants (all exact; the autotuner keeps the fastest)

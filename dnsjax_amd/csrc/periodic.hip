@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "context.hpp"
+#include "ctl.hpp"
 #include "dft.hpp"
 #include "passes.hpp"
 
@@ -942,20 +943,31 @@ __global__ void __launch_bounds__(64) k_zero_mean(double* __restrict__ u, long l
   if (threadIdx.x < 6) u[2 * (threadIdx.x / 2) * n + (threadIdx.x & 1)] = 0.0;
 }
 
+// The norm from its per-block partials, summed in a fixed order.  With a
+// control block (the corrector slots of a device step, whose err it writes)
+// it then applies the corrector rule, one launch instead of two.  Every
+// thread reads *act before the reduction's barriers, so the rule's update of
+// `active` (the same word) cannot race with that read.
 __global__ void __launch_bounds__(256) k_finalize_norm(const double* __restrict__ partials, int n,
-                                                       double* __restrict__ err,
-                                                       const int* __restrict__ act) {
+                                                       double* err, const int* act, StepCtl* ctl,
+                                                       CtlRule rule) {
   if (act && *act == 0) return;
   __shared__ double red[16];
   double v = 0.0;
   for (int i = threadIdx.x; i < n; i += blockDim.x) v += partials[i];
   const double s = block_sum(v, red);
-  if (threadIdx.x == 0) *err = sqrt(s);
+  if (threadIdx.x == 0) {
+    const double e = sqrt(s);
+    *err = e;
+    if (ctl) ctl_after_iteration(ctl, rule, e);
+  }
 }
 
 int finalize_norm(Ctx* c, int nblocks, double* err_dev, hipStream_t s) {
   ProfScope ps(c, KID_FINALIZE, s);
-  hipLaunchKernelGGL(k_finalize_norm, dim3(1), dim3(256), 0, s, c->partials, nblocks, err_dev, c->act);
+  StepCtl* ctl = c->fuse_ctl ? reinterpret_cast<StepCtl*>(c->ctl) : nullptr;
+  hipLaunchKernelGGL(k_finalize_norm, dim3(1), dim3(256), 0, s, c->partials, nblocks, err_dev, c->act,
+                     ctl, c->rule);
   return check_launch(c, "finalize_norm");
 }
 

@@ -7,31 +7,19 @@
 // after predict_and_correct, `active` = (err > tol) and at least one extra
 // iteration is allowed; each of `slots` corrector slots runs only if
 // `active` (every kernel of the slot reads it and returns at once when 0),
-// then updates the count and `active` exactly as the host rule would
-// (including the optional noise-floor stops of the mixed form).  The step
-// ends with correct_velocity and records the step's iteration count.
+// and the pass's final norm reduction then updates the count and `active`
+// exactly as the host rule would (ctl_after_iteration, including the
+// optional noise-floor stops of the mixed form).  The step ends with
+// correct_velocity and records the step's iteration count.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
 #include <cstdint>
 
 #include "context.hpp"
+#include "ctl.hpp"
 
 namespace dnsmi {
-
-struct StepCtl {
-  double err;
-  double prev;
-  int active;
-  int c;
-  int floor;
-  int steps;
-  int total_c;
-  int unconverged;
-  int nan;
-  int pad;
-  double floor_est;  // noise-floor estimate: the error of the last stall stop
-};
 
 int periodic_predict_and_correct(Ctx* c, const double* state, double* pred, double* rhs_next,
                                  double* err, hipStream_t s);
@@ -51,32 +39,6 @@ __global__ void k_ctl_begin(StepCtl* ctl, double tol, int max_iter) {
   ctl->floor = 0;
   ctl->prev = e;
   ctl->active = (e > tol) && (max_iter > 0);
-}
-
-// The mixed form's noise-floor stops, the same rule as the host loops:
-// for an error above `tol` and below `cap`, a pass that failed to shrink the error by `ratio` is at the
-// floor (a stall stop; its error becomes the floor estimate), and with
-// `margin` > 0 a pass whose error is within `margin` times the estimate is
-// taken as at the floor without the confirming pass (the estimate can then
-// only decrease).
-__global__ void k_ctl_after_iteration(StepCtl* ctl, double tol, int max_iter, double ratio,
-                                      double cap, double margin) {
-  if (threadIdx.x != 0 || !ctl->active) return;
-  const double e = ctl->err;
-  ctl->c += 1;
-  if (ratio > 0 && e > tol && e <= cap && e > ratio * ctl->prev) {
-    ctl->floor = 1;
-    ctl->active = 0;
-    ctl->floor_est = e;
-  } else if (ratio > 0 && margin > 0 && ctl->floor_est > 0 && e > tol && e <= cap &&
-             e <= margin * ctl->floor_est) {
-    ctl->floor = 1;
-    ctl->active = 0;
-    ctl->floor_est = fmin(ctl->floor_est, e);
-  } else {
-    ctl->active = (e > tol) && (ctl->c < max_iter);
-  }
-  ctl->prev = e;
 }
 
 __global__ void k_ctl_end(StepCtl* ctl, int* counts, int counts_cap, double tol) {
@@ -103,16 +65,15 @@ int step_device(Ctx* c, const double* state, double* pred, double* rhs_a, double
   hipLaunchKernelGGL(k_ctl_begin, dim3(1), dim3(64), 0, s, ctl, tol, max_iter);
   if ((rc = check_launch(c, "ctl_begin"))) return rc;
   c->act = &ctl->active;
+  c->rule = CtlRule{tol, max_iter, floor_ratio, floor_cap, floor_margin};
+  c->fuse_ctl = true;
   for (int i = 1; i <= slots && rc == DNSMI_OK; ++i) {
     const double* rp = (i % 2) ? rhs_a : rhs_b;
     double* rn = (i % 2) ? rhs_b : rhs_a;
     rc = c->walls ? walls_iterate_correction(c, state, pred, rp, rn, err, s)
                   : periodic_iterate_correction(c, pred, rp, rn, err, s);
-    if (rc) break;
-    hipLaunchKernelGGL(k_ctl_after_iteration, dim3(1), dim3(64), 0, s, ctl, tol, max_iter,
-                       floor_ratio, floor_cap, floor_margin);
-    rc = check_launch(c, "ctl_after_iteration");
   }
+  c->fuse_ctl = false;
   c->act = nullptr;
   if (rc) return rc;
   if (!c->walls && (rc = periodic_correct_velocity(c, pred, nullptr, s))) return rc;
@@ -158,16 +119,15 @@ int step_device_guess(Ctx* c, const double* state, double* pred, double* rhs_a, 
   hipLaunchKernelGGL(k_ctl_begin_guess, dim3(1), dim3(64), 0, s, ctl, max_pass);
   if ((rc = check_launch(c, "ctl_begin_guess"))) return rc;
   c->act = &ctl->active;
+  c->rule = CtlRule{tol, max_pass, floor_ratio, floor_cap, floor_margin};
+  c->fuse_ctl = true;
   for (int i = 1; i <= slots && rc == DNSMI_OK; ++i) {
     const double* rp = (i % 2) ? rhs_a : rhs_b;
     double* rn = (i % 2) ? rhs_b : rhs_a;
     rc = c->walls ? walls_iterate_correction(c, state, pred, rp, rn, err, s)
                   : periodic_iterate_correction(c, pred, rp, rn, err, s);
-    if (rc) break;
-    hipLaunchKernelGGL(k_ctl_after_iteration, dim3(1), dim3(64), 0, s, ctl, tol, max_pass,
-                       floor_ratio, floor_cap, floor_margin);
-    rc = check_launch(c, "ctl_after_iteration");
   }
+  c->fuse_ctl = false;
   c->act = nullptr;
   if (rc) return rc;
   if (!c->walls && (rc = periodic_correct_velocity(c, pred, nullptr, s))) return rc;
