@@ -569,6 +569,72 @@ def profile_workload(steps: int) -> None:
     print(f"mixed + learned guess: {steps} steps, corrector passes {g.passes}", flush=True)
 
 
+def walls_large_form(steps: int, collect_steps: int = 10, train_iters: int = 400) -> dict:
+    """Plane Couette at the learned-guess configuration's shape: 128 x 256 x
+    128 CGL points, fd_order 6, Re 1000, dt 0.005, corrector tolerance 1e-10,
+    seeded perturbation RMS 0.05 (seed 4).  A `collect_steps` mixed run
+    records the nonlinear-term history; the per-point network is trained on
+    it (set-up, timed separately).  From the state that run ends in, the
+    plain FP64 method and the mixed form started from the learned guess are
+    timed over the same `steps`: corrector RHS + IMM evaluations per step,
+    steps/s and the end-state energy against the FP64 run's (a bounded
+    segment of the configuration's 1000 steps)."""
+    import torch
+
+    from dnsjax_amd.driver import run
+    from dnsjax_amd.fields import wall_field
+    from dnsjax_amd.flows import PlaneCouetteFlow
+    from dnsjax_amd.learned import Net, collect, fit_scales, make_dataset, run_guessed, train
+    from dnsjax_amd.params import Params
+
+    nx, ny, nz = 128, 256, 128
+
+    def mk(prec):
+        return PlaneCouetteFlow(Params(system="plane-couette", nx=nx, ny=ny, nz=nz, lx=2 * math.pi,
+                                       lz=math.pi, re=1000.0, dt=0.005, fd_order=6,
+                                       corrector_tolerance=1e-10, max_corrector_iterations=20,
+                                       precision=prec))
+
+    t0 = time.perf_counter()
+    fp64, mixed = mk("fp64"), mk("mixed")
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    u0 = fp64.to_device(wall_field(nx, ny, nz, 2 * math.pi, math.pi, 0.05, seed=4))
+    t0 = time.perf_counter()
+    snaps, u_mid = collect(mixed, u0, collect_steps)
+    sc = fit_scales(snaps)
+    X, T = make_dataset(mixed, snaps, sc, 50000, seed=1)
+    del snaps
+    net = Net(32, seed=0)
+    train(net, X, T, iters=train_iters, batch=8192, lr=3e-3)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    del X, T
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        return r, time.perf_counter() - t
+
+    plain, tp = timed(lambda: run(fp64, u_mid.clone(), steps))
+    learned, tl = timed(lambda: run_guessed(mixed, u_mid.clone(), steps, mode=2, net=net, scales=sc))
+    e_p = float(fp64.get_stats(plain.state)["E'"])
+    e_l = float(fp64.get_stats(learned.state)["E'"])
+    return {"workload": "plane Couette 128x256x128, p=6, Re 1000, dt 0.005, RMS 0.05 (seed 4), tol 1e-10",
+            "steps": steps, "setup_s": round(t_setup, 2), "data_and_train_s": round(t_train, 2),
+            "fp64": {"steps_per_s": round(steps / tp, 3), "rhs_evals_per_step": round(plain.rhs_evals / steps, 3),
+                     "converged": plain.converged, "tuning": fp64._native.tuning.strip()},
+            "mixed_learned": {"steps_per_s": round(steps / tl, 3),
+                              "rhs_evals_per_step": round(learned.rhs_evals / steps, 3),
+                              "converged": learned.converged,
+                              "energy_rel_err_vs_fp64": abs(e_l - e_p) / abs(e_p)},
+            "speedup_vs_fp64": round(tp / tl, 3),
+            "segment": f"{steps} steps after a {collect_steps}-step mixed data run (the first guessed step "
+                       "has no history and runs the plain predictor)"}
+
+
 def profile_imm(calls: int) -> None:
     """The plane Couette influence-matrix step (predict: one IMM on the
     state's RHS) in both band-solve forms, chunk maps and sequential sweeps,
@@ -621,6 +687,8 @@ def main():
                     help="steps of each form in the mixed+learned A/B (0 skips it)")
     ap.add_argument("--ensemble-steps", type=int, default=50,
                     help="steps of the 64-member ensemble form (0 skips it)")
+    ap.add_argument("--walls-large-steps", type=int, default=8,
+                    help="steps of the plane Couette 128x256x128 learned-guess form (0 skips it)")
     ap.add_argument("--profile-walls", action="store_true",
                     help="run a few steps of the plane Couette form for rocprofv3 and exit")
     ap.add_argument("--profile-workload", action="store_true",
@@ -690,6 +758,8 @@ def main():
         out["mixed_learned_vs_fp64"] = lf["speedup_vs_fp64"]
     if a.ensemble_steps > 0:
         out["ensemble"] = ensemble_form(a.ensemble_steps)
+    if a.walls_large_steps > 0:
+        out["walls_large"] = walls_large_form(a.walls_large_steps)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(a.cpu_seconds)
         ncores = min(16, len(os.sched_getaffinity(0)))
