@@ -179,3 +179,30 @@ def test_walls_stats_match_oracle(case):
     of = W.WallFlow(W.WallParams(nx=c["nx"], ny=c["ny"], nz=c["nz"], fd_order=c["fd_order"], re=c["re"], dt=c["dt"]))
     ref = W.get_stats(g["pred2"], of)["E'"]
     assert abs(float(st["E'"]) - ref) <= 1e-12 * ref
+
+
+@pytest.mark.parametrize("p", [2, 8])
+@pytest.mark.parametrize("variants", [0, 8], ids=["chunked", "sequential"])
+def test_fd_orders_2_and_8_against_oracle(p, variants):
+    """The remaining stencil orders (the goldens hold 4 and 6): 8 x 33 x 8,
+    Re 400, dt 0.01.  The RHS to 1e-12, one predictor-corrector step and
+    one corrector iteration to 1e-11 relative, their errors to 1e-9, for
+    both band-solve forms (the chunked one runs with two map slots and one
+    wave per SIMD at order 8)."""
+    from dnsjax_amd.fields import wall_field
+
+    nx, ny, nz = 8, 33, 8
+    flow = make(nx=nx, ny=ny, nz=nz, fd_order=p, re=400.0, dt=0.01, tile_cols=16, variants=variants)
+    of = W.WallFlow(W.WallParams(nx=nx, ny=ny, nz=nz, fd_order=p, re=400.0, dt=0.01))
+    u0 = wall_field(nx, ny, nz, 2 * math.pi, math.pi, 0.05, seed=7)
+    u = flow.to_device(u0)
+    assert rel(flow.get_rhs(u), W.get_rhs(u0, of)) < 1e-12
+    pc, ic = S.make_stepper(W.get_rhs, W.predict, W.correct, W.norm)
+    want, want_rhs, want_err = pc(u0, of)
+    got, got_rhs, got_err = flow.predict_and_correct(u)
+    assert rel(got, want) < 1e-11
+    assert abs(float(got_err) - want_err) <= 1e-9 * want_err
+    want2, _, want_err2 = ic(u0, want, want_rhs, of)
+    got2, _, got_err2 = flow.iterate_correction(u, got, got_rhs)
+    assert rel(got2, want2) < 1e-11
+    assert abs(float(got_err2) - want_err2) <= 1e-9 * want_err2
