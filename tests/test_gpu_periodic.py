@@ -226,3 +226,40 @@ def test_unpadded_y_matches_oracle(precision):
     want, _, _ = pc(u, of)
     got, _, _ = flow.predict_and_correct(flow.to_device(u))
     assert rel(got, want) < tol
+
+
+def test_decaying_box_against_oracle_and_beltrami_decay():
+    """The decaying box (no base flow, no forcing) on the GPU: a seeded
+    field's RHS, step and corrector iteration against the oracle (1e-12);
+    and the closed form: the Beltrami field (sin kz, cos kz, 0) has a zero
+    nonlinear term, so a step multiplies it by exactly
+    g = (1/dt - (1-c) k^2/Re) / (1/dt + c k^2/Re) with no corrector pass."""
+    from dnsjax_amd.fields import periodic_field
+    from dnsjax_amd.flows import TriplyPeriodicFlow
+    from dnsjax_amd.driver import run
+
+    n, L = 16, 2 * math.pi
+    kw = dict(system="decaying-box", nx=n, ny=n, nz=n, lx=L, ly=L, lz=L, re=10.0, dt=0.05)
+    flow = TriplyPeriodicFlow(Params(**kw))
+    of = P.Flow(P.PeriodicParams(implicitness=0.5, **kw))
+    u0 = periodic_field(n, n, n, L, L, L, 0.2, 13)
+    u = flow.to_device(u0)
+    assert rel(flow.get_rhs(u), P.get_rhs(u0, of)) < 1e-12
+    pc, ic = S.make_stepper(P.get_rhs, P.predict, P.correct, P.norm)
+    want, want_rhs, want_err = pc(u0, of)
+    got, got_rhs, got_err = flow.predict_and_correct(u)
+    assert rel(got, want) < 1e-12
+    assert abs(float(got_err) - want_err) <= 1e-9 * want_err
+    want2, _, _ = ic(u0, want, want_rhs, of)
+    got2, _, _ = flow.iterate_correction(u, got, got_rhs)
+    assert rel(got2, want2) < 1e-12
+
+    p = of.p
+    z = np.arange(p.nz_padded) * p.lz / p.nz_padded
+    Z = np.broadcast_to(z[None, :, None], (p.ny_padded, p.nz_padded, p.nx_padded))
+    k = 2 * 2 * np.pi / p.lz
+    b = P.phys_to_spec(np.array([np.sin(k * Z), np.cos(k * Z), 0 * Z]), p)
+    g = (1 / p.dt - 0.5 * k**2 / p.re) / (1 / p.dt + 0.5 * k**2 / p.re)
+    r = run(flow, flow.to_device(b), 3)
+    assert r.counts == [0, 0, 0]
+    np.testing.assert_allclose(r.state.cpu().numpy(), g**3 * b, rtol=0, atol=1e-14)
