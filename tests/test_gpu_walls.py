@@ -206,3 +206,25 @@ def test_fd_orders_2_and_8_against_oracle(p, variants):
     got2, _, got_err2 = flow.iterate_correction(u, got, got_rhs)
     assert rel(got2, want2) < 1e-11
     assert abs(float(got_err2) - want_err2) <= 1e-9 * want_err2
+
+
+@pytest.mark.parametrize("c", [0.3, 1.0])
+def test_implicitness_against_oracle(c):
+    """The Crank-Nicolson weight c enters H_k, the homogeneous solutions and
+    the IMM right-hand sides: a step and a corrector iteration at c != 1/2
+    against the oracle (8 x 17 x 8, p = 4, 1e-11)."""
+    from dnsjax_amd.fields import wall_field
+
+    nx, ny, nz = 8, 17, 8
+    flow = make(nx=nx, ny=ny, nz=nz, fd_order=4, re=400.0, dt=0.01, implicitness=c)
+    of = W.WallFlow(W.WallParams(nx=nx, ny=ny, nz=nz, fd_order=4, re=400.0, dt=0.01, implicitness=c))
+    u0 = wall_field(nx, ny, nz, 2 * math.pi, math.pi, 0.05, seed=11)
+    u = flow.to_device(u0)
+    pc, ic = S.make_stepper(W.get_rhs, W.predict, W.correct, W.norm)
+    want, want_rhs, want_err = pc(u0, of)
+    got, got_rhs, got_err = flow.predict_and_correct(u)
+    assert rel(got, want) < 1e-11
+    assert abs(float(got_err) - want_err) <= 1e-9 * want_err
+    want2, _, _ = ic(u0, want, want_rhs, of)
+    got2, _, _ = flow.iterate_correction(u, got, got_rhs)
+    assert rel(got2, want2) < 1e-11
