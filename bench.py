@@ -54,7 +54,9 @@ def kernel_bytes(n: int, b: int) -> dict:
 
 
 def rhs_model_bytes(n: int, b: int) -> int:
-    """The fixed 5-pass pencil model of one RHS (stated with the build's design)."""
+    """The fixed 5-pass pencil model of one RHS: each pass (y inverse, z
+    inverse, x pass with the product, z forward, y forward) reads its input
+    and writes its output once; zero pads are not read."""
     Nx, Ny, Nz, P = n // 2, n - 1, n - 1, 3 * n // 2
     return b * Nx * (6 * Ny * Nz + 18 * P * Nz + 18 * P * P)
 
@@ -698,7 +700,7 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); the multi-GPU slab "
-              "decomposition is a design note, not part of this build.", flush=True)
+              "decomposition is described, not implemented.", flush=True)
         sys.exit(2)
     if a.profile_workload:
         profile_workload(a.steps)

@@ -1,8 +1,8 @@
 """Oracle: the per-mode corrector-guess network, FP64 NumPy -- TEST INFRASTRUCTURE ONLY.
 
-The reference has no learned component (SURVEY a'); this restates the
-network this build trains and runs, so that the GPU kernels' forward
-values and parameter gradients can be checked:
+The reference has no learned component; this restates the network the
+library trains and runs, so that the GPU kernels' forward values and
+parameter gradients can be checked:
 
     x (D_IN) -> z1 = W1^T x + b1 -> h1 = silu(z1)
              -> z2 = W2^T h1 + b2 -> h2 = silu(z2)
