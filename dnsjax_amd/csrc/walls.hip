@@ -207,9 +207,9 @@ constexpr int kChunk = 16;
 constexpr int kChunkSlots = 3;
 // two slots where three would spill (fd_order 8, three right-hand sides)
 constexpr int chunk_slots(int po, int nr) { return po == 8 && nr == 3 ? 2 : kChunkSlots; }
-// window words read from LDS per batch: four where eight would not leave
-// two waves per SIMD (one right-hand side, fd_order >= 6)
-constexpr int window_batch(int po, int nr) { return nr == 1 && po >= 6 ? 4 : 8; }
+// window words read from LDS per batch: two where eight would not leave
+// two waves per SIMD without spilling (one right-hand side, fd_order >= 6)
+constexpr int window_batch(int po, int nr) { return nr == 1 && po >= 6 ? 2 : 8; }
 // waves per SIMD the chunked solve asks for: two with one right-hand side
 // (registers within 256) up to fd_order 6, else one
 constexpr int chunk_waves(int po, int nr) { return nr == 1 && po <= 6 ? 2 : 1; }
