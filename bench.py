@@ -309,10 +309,22 @@ def learned_form(steps: int, collect_steps: int = 30, hidden: int = 32, train_it
     for name, (t, rhs, conv, st, n) in rows.items():
         out[name] = {"steps_per_s": round(steps / t, 3), "ms_per_step": round(1e3 * t / steps, 4),
                      "rhs_evals_per_step": round(rhs / steps, 3), "converged": conv, **errors(st, n)}
+    # The configuration's full length: 500 steps of each form from the same
+    # state, end-state errors of the mixed + learned run against the FP64 one.
+    long_n = 500
+    long_ref = run(fp64, u_mid.clone(), long_n)
+    long_ml = run_guessed(mixed, u_mid.clone(), long_n, mode=2, net=net, scales=sc)
+    refs[long_n] = long_ref.state
+    long_run = {"steps": long_n, "fp64_converged": long_ref.converged,
+                "mixed_learned_converged": long_ml.converged,
+                "fp64_rhs_evals_per_step": round(long_ref.rhs_evals / long_n, 3),
+                "mixed_learned_rhs_evals_per_step": round(long_ml.rhs_evals / long_n, 3),
+                **errors(long_ml.state, long_n)}
     sp = out["mixed_learned"]["steps_per_s"] / out["fp64_plain"]["steps_per_s"]
     spg = out["mixed_learned_graph"]["steps_per_s"] / out["fp64_plain_graph"]["steps_per_s"]
     spg4 = out["mixed_learned_graph_4slots"]["steps_per_s"] / out["fp64_plain_graph_4slots"]["steps_per_s"]
-    return {"forms": out, "speedup_vs_fp64": round(sp, 3), "speedup_vs_fp64_graph": round(spg, 3),
+    return {"forms": out, "long_run": long_run, "speedup_vs_fp64": round(sp, 3),
+            "speedup_vs_fp64_graph": round(spg, 3),
             "speedup_vs_fp64_graph_4slots": round(spg4, 3),
             "speedup_vs_mixed_quadratic": round(out["mixed_learned"]["steps_per_s"]
                                                 / out["mixed_quadratic"]["steps_per_s"], 3),
