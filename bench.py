@@ -312,10 +312,12 @@ def learned_form(steps: int, collect_steps: int = 30, hidden: int = 32, train_it
     # The configuration's full length: 500 steps of each form from the same
     # state, end-state errors of the mixed + learned run against the FP64 one.
     long_n = 500
-    long_ref = run(fp64, u_mid.clone(), long_n)
-    long_ml = run_guessed(mixed, u_mid.clone(), long_n, mode=2, net=net, scales=sc)
+    long_ref, t_lr = timed(lambda: run(fp64, u_mid.clone(), long_n))
+    long_ml, t_lm = timed(lambda: run_guessed(mixed, u_mid.clone(), long_n, mode=2, net=net, scales=sc))
     refs[long_n] = long_ref.state
-    long_run = {"steps": long_n, "fp64_converged": long_ref.converged,
+    long_run = {"steps": long_n, "fp64_steps_per_s": round(long_n / t_lr, 3),
+                "mixed_learned_steps_per_s": round(long_n / t_lm, 3),
+                "speedup_vs_fp64": round(t_lr / t_lm, 3), "fp64_converged": long_ref.converged,
                 "mixed_learned_converged": long_ml.converged,
                 "fp64_rhs_evals_per_step": round(long_ref.rhs_evals / long_n, 3),
                 "mixed_learned_rhs_evals_per_step": round(long_ml.rhs_evals / long_n, 3),
@@ -455,6 +457,19 @@ def walls_form(steps: int, warmup: int = 2, collect_steps: int = 30, train_iters
     def err_graph(st):
         return abs(float(flow.get_stats(st)["E'"]) - e_graph_ref) / abs(e_graph_ref)
 
+    # the configuration's full length: 200 steps of each form from the same state
+    long_n = 200
+    long_ref, t_lr = timed(lambda: run(flow, state.clone(), long_n))
+    long_ml, t_lm = timed(lambda: run_guessed(mixed, state.clone(), long_n, mode=2, net=net, scales=sc))
+    e_long = float(flow.get_stats(long_ref.state)["E'"])
+    long_run = {"steps": long_n, "fp64_steps_per_s": round(long_n / t_lr, 3),
+                "mixed_learned_steps_per_s": round(long_n / t_lm, 3),
+                "speedup_vs_fp64": round(t_lr / t_lm, 3), "fp64_converged": long_ref.converged,
+                "mixed_learned_converged": long_ml.converged,
+                "fp64_rhs_evals_per_step": round(long_ref.rhs_evals / long_n, 3),
+                "mixed_learned_rhs_evals_per_step": round(long_ml.rhs_evals / long_n, 3),
+                "energy_rel_err_vs_fp64": abs(float(flow.get_stats(long_ml.state)["E'"]) - e_long) / abs(e_long)}
+
     return {"workload": f"plane Couette 64x128x64, p=4, Re 400, dt 0.01, RMS 0.05, tol {tol:g}"
                         + (" (reference default)" if tol == 1e-5 else ""),
             "steps_per_s": round(steps / el, 3), "ms_per_step": round(1e3 * el / steps, 4),
@@ -482,6 +497,7 @@ def walls_form(steps: int, warmup: int = 2, collect_steps: int = 30, train_iters
                                         "converged": gg.step_stats()["unconverged"] == 0,
                                         "energy_rel_err_vs_fp64": err_graph(gg.state)},
                       "speedup_vs_fp64": round(tgp / tgg, 3)},
+            "long_run": long_run,
             "fp64_rhs_evals_per_step": round(r.rhs_evals / steps, 3),
             "segment": f"{steps} steps after a {collect_steps}-step mixed data run from the workload's state"}
 
@@ -765,7 +781,7 @@ def main():
             "fp64_steps_per_s": wt["steps_per_s"], "fp64_rhs_evals_per_step": wt["fp64_rhs_evals_per_step"],
             **{f: {k: wt[f][k] for k in ("steps_per_s", "rhs_evals_per_step", "energy_rel_err_vs_fp64",
                                          "converged")} for f in ("mixed", "mixed_guess", "mixed_learned")},
-            "mixed_learned_vs_fp64": wt["mixed_learned"]["speedup_vs_fp64"]}
+            "mixed_learned_vs_fp64": wt["mixed_learned"]["speedup_vs_fp64"], "long_run": wt["long_run"]}
     if a.learned_steps > 0:
         lf = learned_form(a.learned_steps)
         out["mixed_learned"] = lf
