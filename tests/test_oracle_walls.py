@@ -201,3 +201,38 @@ def test_wall_generator_no_slip_and_rms():
     # Hermitian in (kz) at kx = 0: the 2D round trip is the identity
     phys = W.spec_to_phys_2d(u, flow.p)
     assert np.max(np.abs(W.phys_to_spec_2d(phys, flow.p) - u)) < 1e-15
+
+
+def test_batched_banded_solve_is_bitwise_the_scalar_solve():
+    """The oracle's batched solve does the scalar dgbtrs-order solve's
+    operations per right-hand side: identical bits."""
+    rng = np.random.default_rng(3)
+    for Ny, p in ((17, 4), (33, 6), (20, 2)):
+        y = W.cgl_grid(Ny)
+        D1, D2 = W.build_diff_matrices(y, p)
+        # Neumann L_k (its wall rows pivot) and random diagonally dominant bands
+        mats = [W.build_Lk_neumann(k2, D1, D2) for k2 in (0.0, 1.0, 7.5)]
+        mats += [_make_random_banded(Ny, p, p, seed=s) for s in range(2)]
+        facs = [W.banded_lu(A, p, p) for A in mats]
+        ab = np.stack([f[0] for f in facs])
+        piv = np.stack([f[1] for f in facs])
+        assert any(np.any(pv != np.arange(Ny)) for pv in piv)  # some rows swap
+        b = rng.standard_normal((5, Ny))
+        got = W.banded_solve_batch(ab, piv, b, p, p)
+        for i in range(5):
+            assert np.array_equal(got[i], W.banded_solve(ab[i], piv[i], b[i], p, p))
+
+
+def test_setup_tables_shared_per_k2_equal_the_per_mode_loop():
+    """precompute_imm computes each table once per distinct k^2: at a box
+    where +-kz share k^2, the copies equal a fresh per-mode computation."""
+    flow = _small_flow(ny=17, nx=8, nz=8, lx=3.0, lz=1.7)
+    imm = flow.imm
+    for iz, ix in ((1, 2), (6, 2), (3, 1)):
+        k2 = float(flow.k2[iz, ix, 0])
+        Lk = W.build_Lk_neumann(k2, flow.D1, flow.D2)
+        e1 = np.zeros(17)
+        e1[0] = 1.0
+        assert np.array_equal(imm["p1"][iz, ix], np.linalg.solve(Lk, e1))
+        ab, piv = W.banded_lu(Lk, 4, 4)
+        assert np.array_equal(imm["Lk_ab"][iz, ix], ab) and np.array_equal(imm["Lk_piv"][iz, ix], piv)
