@@ -354,6 +354,8 @@ int walls_setup(Ctx* c) {
   const dnsmi_config& g = c->cfg;
   if (g.nx % 2 || g.nz % 2 || g.nx < 4 || g.nz < 4 || g.ny < 5)
     return c->fail(DNSMI_EINVAL, "walls: nx, nz even >= 4 and ny >= 5");
+  // the y grid is fixed on [-1, 1] (parameters.py:190-191)
+  if (g.ly != 2.0) return c->fail(DNSMI_EINVAL, "walls: ly must be 2 (the channel is y in [-1, 1])");
   if (g.oversampling_factor != 3) return c->fail(DNSMI_EINVAL, "oversampling_factor must be 3");
   const int p = g.fd_order;
   if (p != 2 && p != 4 && p != 6 && p != 8) return c->fail(DNSMI_EINVAL, "fd_order must be 2, 4, 6 or 8");
@@ -424,7 +426,7 @@ int walls_setup(Ctx* c) {
   if ((rc = upload_vec(c, &t->d1j0, d1.j0))) return rc;
   if ((rc = upload_vec(c, &t->d2w, d2.w))) return rc;
   if ((rc = upload_vec(c, &t->d2j0, d2.j0))) return rc;
-  if ((rc = upload_vec(c, &t->qw, quadrature_weights(y, g.ly > 0 ? g.ly : 2.0)))) return rc;
+  if ((rc = upload_vec(c, &t->qw, quadrature_weights(y, g.ly)))) return rc;
 
   // per-mode IMM tables (geometries/cartesian.py:704-779)
   const int kl = p, ku = p, rows = 3 * p + 1;
@@ -573,10 +575,14 @@ int walls_setup(Ctx* c) {
   // parallel (not when the sequential sweeps are requested)
   const int B = kChunk, WF = B + kl, WB = B + kl + ku, nC = (Ny + B - 1) / B;
   const size_t nf = (size_t)nC * U * map_pre(WF, kl, WF), nb = (size_t)nC * U * map_pre(WB, 0, WB);
-  const bool maps_fit = nf < (1ull << 28) && nb < (1ull << 28);
+  // the chunked solve also stages three right-hand sides per mode pair in
+  // LDS (2 * 3 * (Ny + 32) complex values), which must fit in 64 KiB
+  const bool maps_fit = nf < (1ull << 28) && nb < (1ull << 28) &&
+                        (size_t)2 * 3 * (Ny + 32) * sizeof(cx<double>) <= 65536;
   const bool forced = g.tile_cols != 0;  // the variant bits choose the form
   if (forced && (g.variants & 8) == 0 && !maps_fit)
-    return c->fail(DNSMI_EINVAL, "chunk maps exceed 2^28 words: use the sequential band solve");
+    return c->fail(DNSMI_EINVAL,
+                   "chunk maps exceed 2^28 words or their LDS staging 64 KiB: use the sequential band solve");
   if (maps_fit && !(forced && (g.variants & 8) != 0)) {
     std::vector<double> lf(nf), lb(nb), hf(nf), hb(nb);
     const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
@@ -622,7 +628,7 @@ int walls_setup(Ctx* c) {
   if ((rc = upload_vec(c, &t->mkz, mkz))) return rc;
   if (!forced && !maps_fit) {  // the autotuner has one form to choose
     c->sweep_seq = true;
-    c->tuning = "band_solve=sequential(chunk maps exceed 2^28 words) ";
+    c->tuning = "band_solve=sequential(chunk maps exceed 2^28 words or 64 KiB LDS) ";
     return DNSMI_OK;
   }
   if (!forced) return walls_autotune(c);
@@ -1471,11 +1477,14 @@ static int walls_autotune(Ctx* c) {
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
     return c->fail(DNSMI_EHIP, "autotune: events");
+  // a form that cannot run here (nonzero return) never wins
   auto time_imm = [&](bool seq) {
     c->sweep_seq = seq;
-    walls_imm(c, z, z, z, z, z, nullptr, nullptr, nullptr);  // warm-up
+    if (walls_imm(c, z, z, z, z, z, nullptr, nullptr, nullptr) != DNSMI_OK)  // warm-up
+      return (double)INFINITY;
     (void)hipEventRecord(e0, nullptr);
-    for (int i = 0; i < 3; ++i) walls_imm(c, z, z, z, z, z, nullptr, nullptr, nullptr);
+    for (int i = 0; i < 3; ++i)
+      if (walls_imm(c, z, z, z, z, z, nullptr, nullptr, nullptr) != DNSMI_OK) return (double)INFINITY;
     (void)hipEventRecord(e1, nullptr);
     (void)hipEventSynchronize(e1);
     float ms = 0.f;
@@ -1483,6 +1492,8 @@ static int walls_autotune(Ctx* c) {
     return ms / 3.0;
   };
   const double tc = time_imm(false), ts = time_imm(true);
+  if (!(ts < INFINITY)) return c->fail(DNSMI_EINVAL, "autotune: the sequential band solve failed: " + c->err);
+  c->err.clear();
   c->sweep_seq = ts < tc;
   if (c->sweep_seq) {  // the chunk maps are not needed any more
     WallTables* t = c->wt;

@@ -49,6 +49,10 @@ class Ensemble:
     def __init__(self, flows, states, n_streams: int = 8, slots=None, capacity: int = 4096):
         if len(flows) != len(states) or not flows:
             raise ValueError("need one state per flow")
+        if len({id(f._native) for f in flows}) != len(flows):
+            # members run concurrently and each uses its context's work buffers
+            # and control block: two members may not share one flow
+            raise ValueError("every member needs its own flow (native context)")
         per = list(slots) if isinstance(slots, (list, tuple)) else [slots] * len(flows)
         if len(per) != len(flows):
             raise ValueError("need one slot count per flow")

@@ -65,6 +65,9 @@ class Params:
             raise ValueError(f"unknown system {self.system!r}")
         if self.ly is None:
             self.ly = 4.0 if self.system in PERIODIC_SYSTEMS else 2.0
+        if self.system in WALLED_SYSTEMS and self.ly != 2.0:
+            # parameters.py:190-191: the walls sit at y = -1 and y = 1
+            raise ValueError("walled systems have ly = 2 (y in [-1, 1])")
         if self.precision not in ("fp64", "mixed"):
             raise ValueError("precision must be 'fp64' or 'mixed'")
         if self.corrector_floor_ratio is None and self.precision == "mixed":

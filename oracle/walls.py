@@ -181,40 +181,46 @@ def banded_solve(ab: np.ndarray, ipiv: np.ndarray, b: np.ndarray, kl: int, ku: i
 def banded_solve_batch(ab: np.ndarray, ipiv: np.ndarray, b: np.ndarray, kl: int, ku: int):
     """``banded_solve`` for a batch of real right-hand sides at once.
 
-    ab (..., 2kl+ku+1, Ny), ipiv (..., Ny), b (..., Ny): the same operations
-    per batch entry, in the same order, as ``banded_solve`` (so the results
-    are bit-identical to it); vectorised over the batch for speed."""
+    ab (..., 2kl+ku+1, Ny), ipiv (..., Ny), b (..., Ny) or (..., Ny, K) for K
+    right-hand sides per matrix: the same operations per right-hand side, in
+    the same order, as ``banded_solve`` (so the results are bit-identical to
+    it); vectorised over the batch for speed."""
     Ny = ab.shape[-1]
-    lead = b.shape[:-1]
+    lead = ab.shape[:-2]
     M = int(np.prod(lead, dtype=np.int64))
     A = np.ascontiguousarray(ab, dtype=np.float64).reshape(M, ab.shape[-2], Ny)
     piv = np.asarray(ipiv).reshape(M, Ny)
+    bb = np.asarray(b, dtype=np.float64)
+    multi = bb.ndim == len(lead) + 2
+    K = bb.shape[-1] if multi else 1
     kuu = kl + ku
-    w = np.zeros((M, Ny + kl))
-    w[:, :Ny] = np.asarray(b, dtype=np.float64).reshape(M, Ny)
+    w = np.zeros((M, Ny + kl, K))
+    w[:, :Ny] = bb.reshape(M, Ny, K)
     rows = np.arange(M)
     for j in range(Ny - 1):
         pj = piv[:, j]
         bj = w[:, j].copy()
         bpj = w[rows, pj].copy()
         w[:, j] = bpj
-        w[rows, pj] = np.where(pj == j, bpj, bj)
-        w[:, j + 1 : j + 1 + kl] = w[:, j + 1 : j + 1 + kl] - A[:, kl + ku + 1 : kl + ku + 1 + kl, j] * bpj[:, None]
-    v = np.zeros((M, kuu + Ny))
+        w[rows, pj] = bj
+        w[:, j + 1 : j + 1 + kl] = w[:, j + 1 : j + 1 + kl] - A[:, kl + ku + 1 : kl + ku + 1 + kl, j, None] * bpj[:, None]
+    v = np.zeros((M, kuu + Ny, K))
     v[:, kuu:] = w[:, :Ny]
     for j in range(Ny - 1, -1, -1):
         jp = j + kuu
-        xj = v[:, jp] / A[:, kl + ku, j]
+        xj = v[:, jp] / A[:, kl + ku, j, None]
         v[:, jp] = xj
-        v[:, jp - kuu : jp] = v[:, jp - kuu : jp] - A[:, 0:kuu, j] * xj[:, None]
-    return v[:, kuu:].reshape(lead + (Ny,))
+        v[:, jp - kuu : jp] = v[:, jp - kuu : jp] - A[:, 0:kuu, j, None] * xj[:, None]
+    out = v[:, kuu:]
+    return out.reshape(lead + (Ny, K)) if multi else out.reshape(lead + (Ny,))
 
 
 def banded_solve_modes(ab: np.ndarray, ipiv: np.ndarray, rhs: np.ndarray) -> np.ndarray:
     """``geometries/cartesian.py:463-488``: per-mode solve, complex split re/im."""
     p = (ab.shape[-2] - 1) // 3
     if np.iscomplexobj(rhs):
-        return banded_solve_batch(ab, ipiv, rhs.real, p, p) + 1j * banded_solve_batch(ab, ipiv, rhs.imag, p, p)
+        x = banded_solve_batch(ab, ipiv, np.stack([rhs.real, rhs.imag], axis=-1), p, p)
+        return x[..., 0] + 1j * x[..., 1]
     return banded_solve_batch(ab, ipiv, rhs, p, p)
 
 
@@ -408,8 +414,16 @@ class WallFlow:
 
 
 def dy(D: np.ndarray, u: np.ndarray) -> np.ndarray:
-    """``einsum("ij, zxj -> zxi", D, u)``."""
-    return np.einsum("ij,zxj->zxi", D, u)
+    """``einsum("ij, zxj -> zxi", D, u)`` (``flows/plane_couette.py:259-260``),
+    as a matrix product over the real and imaginary parts (the reference's
+    summation order is XLA's and is not reproduced by either form)."""
+    DT = np.ascontiguousarray(D.T)
+    n = u.shape[-1]
+    if np.iscomplexobj(u):
+        re = np.ascontiguousarray(u.real).reshape(-1, n) @ DT
+        im = np.ascontiguousarray(u.imag).reshape(-1, n) @ DT
+        return (re + 1j * im).reshape(u.shape)
+    return (np.ascontiguousarray(u).reshape(-1, n) @ DT).reshape(u.shape)
 
 
 def curl_walls(state: np.ndarray, flow: WallFlow) -> np.ndarray:

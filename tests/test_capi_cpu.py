@@ -205,3 +205,24 @@ def test_stats_recorder_follows_the_reference_driver(tmp_path):
     with pytest.raises(ValueError):
         rec1.save(str(tmp_path / "none.npz"))
 
+
+
+def test_walled_systems_fix_ly_to_two():
+    """parameters.py:190-191: the channel is y in [-1, 1]; any other ly would
+    only rescale the y-quadrature norm, so it is rejected."""
+    assert Params(system="plane-couette").ly == 2.0
+    with pytest.raises(ValueError):
+        Params(system="plane-couette", ly=4.0)
+
+
+def test_ensemble_rejects_members_sharing_a_context():
+    """Members run concurrently on their contexts' work buffers and control
+    blocks, so one flow may not appear twice (checked before any GPU work)."""
+    from dnsjax_amd.graph import Ensemble
+
+    class _Flow:
+        _native = object()
+
+    f = _Flow()
+    with pytest.raises(ValueError, match="own flow"):
+        Ensemble([f, f], [None, None])
