@@ -82,6 +82,7 @@ SIGNATURES = {
     "dnsmi_mlp_sizes": (_I, [_I, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong),
                              ctypes.POINTER(ctypes.c_longlong)]),
     "dnsmi_mlp_grad": (_I, [_I, _I, _P, _P, _P, _I, _P, _P, _P, _P]),
+    "dnsmi_mlp_grad_static": (_I, [_I, _I, _P, _P, _P, _I, _P, _P, _P]),
     "dnsmi_mlp_adam": (_I, [_I, _P, _P, _P, _P, _I, ctypes.c_float, _P]),
     "dnsmi_mlp_pack": (_I, [_I, _P, _P, _P, _P]),
     "dnsmi_mlp_forward": (_I, [_I, _P, _I, _P, _P, _P, _P]),

@@ -88,6 +88,9 @@ int step_device_guess(Ctx* c, const double* state, double* pred, double* rhs_a, 
 int step_alloc_ctl(Ctx* c);
 int mlp_grad(int H, int B, const float* X, const float* T, const float* prm, int bf16, float* part,
              float* grad, float* loss, hipStream_t s);
+int mlp_grad_static(int H, int B, const float* X, const float* T, const float* prm, int bf16,
+                    float* grad, float* loss, hipStream_t s, std::string* why);
+long long mlp_grad_scratch(int H, int B);
 int mlp_adam(int H, float* prm, const float* g, float* m, float* v, int step, float lr,
              hipStream_t s);
 int mlp_pack(int H, const float* prm, uint16_t* wb, float* bias, hipStream_t s);
@@ -313,6 +316,19 @@ int dnsmi_mlp_grad(int H, int B, const float* X, const float* T, const float* pa
   if (!X || !T || !params || !part || !grad || B <= 0) return ctxfree(DNSMI_EINVAL, "mlp_grad");
   return ctxfree(dnsmi::mlp_grad(H, B, X, T, params, bf16, part, grad, loss_dev, as_stream(stream)),
                  "mlp_grad");
+}
+
+int dnsmi_mlp_grad_static(int H, int B, const float* X, const float* T, const float* params,
+                          int bf16, float* grad, float* loss_dev, void* stream) {
+  if (!X || !T || !params || !grad || B <= 0) return ctxfree(DNSMI_EINVAL, "mlp_grad_static");
+  if (H != 32 && H != 64) return ctxfree(DNSMI_EINVAL, "mlp_grad_static");
+  std::string why;
+  const int rc = dnsmi::mlp_grad_static(H, B, X, T, params, bf16, grad, loss_dev, as_stream(stream), &why);
+  if (rc && !why.empty()) {
+    dnsmi::g_create_error = why;
+    return rc;
+  }
+  return ctxfree(rc, "mlp_grad_static");
 }
 
 int dnsmi_mlp_adam(int H, float* params, const float* grad, float* m, float* v, int step, float lr,

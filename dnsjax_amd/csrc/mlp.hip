@@ -22,6 +22,8 @@
 
 #include <cmath>
 #include <cstdint>
+#include <mutex>
+#include <string>
 
 #include "context.hpp"
 #include "dft.hpp"
@@ -54,13 +56,13 @@ __device__ __forceinline__ float bf16_round(float f) {
 
 // ------------------------------------------------------------------ training
 // One block = S samples (one per thread).  Inputs X (B, 16), targets T (B, 6).
-// grad_part[block][p] receives the block's sum of dL/dp, loss_part[block]
-// its sum of squared errors.  L = (1/B) sum (y - t)^2.
+// Row `block` of grad_part (P + 1 floats) receives the block's sum of dL/dp
+// in its first P entries and its sum of squared errors in entry P.
+// L = (1/B) sum (y - t)^2.
 template <int H, int S>
 __global__ void __launch_bounds__(S)
     k_mlp_grad_block(const float* __restrict__ X, const float* __restrict__ T, int B,
-                     const float* __restrict__ prm, int bf16, float* __restrict__ grad_part,
-                     float* __restrict__ loss_part) {
+                     const float* __restrict__ prm, int bf16, float* __restrict__ grad_part) {
   constexpr int P = mlp_nparams(H);
   __shared__ float w[P];
   __shared__ float sx[S][kDin], sh1[S][H], sh2[S][H], sdz1[S][H], sdz2[S][H], sdy[S][kDout];
@@ -125,7 +127,7 @@ __global__ void __launch_bounds__(S)
   }
   __syncthreads();
   // parameter gradients: each thread owns a strided set, sums samples in order
-  float* gp = grad_part + (size_t)blockIdx.x * P;
+  float* gp = grad_part + (size_t)blockIdx.x * (P + 1);
   for (int p = t; p < P; p += S) {
     float g = 0.f;
     if (p < kDin * H) {
@@ -156,40 +158,36 @@ __global__ void __launch_bounds__(S)
   if (t == 0) {
     float s = 0.f;
     for (int i = 0; i < S; ++i) s += red[i];
-    loss_part[blockIdx.x] = s;
+    gp[P] = s;
   }
 }
 
-// Deterministic two-level sum of the per-block partials: chunk q sums blocks
-// [32 q, 32 q + 32) in order into part2[q][p]; the final pass sums the
-// chunks in order.
+// Deterministic two-level sum of the per-block rows (P parameters and the
+// loss, Q = P + 1 columns): chunk q sums blocks [32 q, 32 q + 32) in order
+// into part2[q][p]; the final pass sums the chunks in order.
 constexpr int kReduceChunk = 32;
 
-__global__ void k_mlp_reduce(const float* __restrict__ part, int nblocks, int P,
+__global__ void k_mlp_reduce(const float* __restrict__ part, int nblocks, int Q,
                              float* __restrict__ part2) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= P) return;
+  if (p >= Q) return;
   const int b0 = blockIdx.y * kReduceChunk;
   const int b1 = b0 + kReduceChunk < nblocks ? b0 + kReduceChunk : nblocks;
   float g = 0.f;
-  for (int b = b0; b < b1; ++b) g += part[(size_t)b * P + p];
-  part2[(size_t)blockIdx.y * P + p] = g;
+  for (int b = b0; b < b1; ++b) g += part[(size_t)b * Q + p];
+  part2[(size_t)blockIdx.y * Q + p] = g;
 }
 
 __global__ void k_mlp_reduce_final(const float* __restrict__ part2, int nchunks, int P,
-                                   float* __restrict__ grad, const float* __restrict__ loss_part,
-                                   int nblocks, float* __restrict__ loss, int B) {
+                                   float* __restrict__ grad, float* __restrict__ loss, int B) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p < P) {
-    float g = 0.f;
-    for (int q = 0; q < nchunks; ++q) g += part2[(size_t)q * P + p];
+  if (p > P) return;
+  float g = 0.f;
+  for (int q = 0; q < nchunks; ++q) g += part2[(size_t)q * (P + 1) + p];
+  if (p < P)
     grad[p] = g;
-  }
-  if (p == 0 && loss) {
-    float s = 0.f;
-    for (int b = 0; b < nblocks; ++b) s += loss_part[b];
-    *loss = s / (float)B;
-  }
+  else if (loss)
+    *loss = g / (float)B;
 }
 
 // Adam (Kingma & Ba 2015) on the FP32 parameters.
@@ -652,27 +650,80 @@ int extrap_walls(Ctx* c, const double* n0, const float* h1, float* h2, double* n
 }
 
 // ------------------------------------------------------------------ launchers
+// Floats of scratch mlp_grad needs for B samples: one row of P + 1 per
+// block of 32 samples, then one per chunk of 32 blocks.
+long long mlp_grad_scratch(int H, int B) {
+  constexpr int S = 32;
+  const long long nb = (B + S - 1) / S, nq = (nb + kReduceChunk - 1) / kReduceChunk;
+  return (nb + nq) * (mlp_nparams(H) + 1);
+}
+
 int mlp_grad(int H, int B, const float* X, const float* T, const float* prm, int bf16, float* part,
              float* grad, float* loss, hipStream_t s) {
   constexpr int S = 32;
   const int nb = (B + S - 1) / S;
-  float* loss_part = part + (size_t)nb * mlp_nparams(H);
   if (H == 32) {
-    hipLaunchKernelGGL((k_mlp_grad_block<32, S>), dim3(nb), dim3(S), 0, s, X, T, B, prm, bf16, part,
-                       loss_part);
+    hipLaunchKernelGGL((k_mlp_grad_block<32, S>), dim3(nb), dim3(S), 0, s, X, T, B, prm, bf16, part);
   } else if (H == 64) {
-    hipLaunchKernelGGL((k_mlp_grad_block<64, S>), dim3(nb), dim3(S), 0, s, X, T, B, prm, bf16, part,
-                       loss_part);
+    hipLaunchKernelGGL((k_mlp_grad_block<64, S>), dim3(nb), dim3(S), 0, s, X, T, B, prm, bf16, part);
   } else {
     return DNSMI_EINVAL;
   }
-  const int P = mlp_nparams(H);
+  const int P = mlp_nparams(H), Q = P + 1;
   const int nq = (nb + kReduceChunk - 1) / kReduceChunk;
-  float* part2 = loss_part + nb;
-  hipLaunchKernelGGL(k_mlp_reduce, dim3((P + 255) / 256, nq), dim3(256), 0, s, part, nb, P, part2);
-  hipLaunchKernelGGL(k_mlp_reduce_final, dim3((P + 255) / 256), dim3(256), 0, s, part2, nq, P, grad,
-                     loss_part, nb, loss, B);
+  float* part2 = part + (size_t)nb * Q;
+  hipLaunchKernelGGL(k_mlp_reduce, dim3((Q + 255) / 256, nq), dim3(256), 0, s, part, nb, Q, part2);
+  hipLaunchKernelGGL(k_mlp_reduce_final, dim3((Q + 255) / 256), dim3(256), 0, s, part2, nq, P, grad,
+                     loss, B);
   return hipGetLastError() == hipSuccess ? DNSMI_OK : DNSMI_EHIP;
+}
+
+// ------------------------------------------------------------------ no-allocation gradient
+// mlp_grad with its scratch in a file-scope device array instead of a
+// workspace argument, for callers that cannot allocate device memory.  Every
+// call on one GPU uses that GPU's copy of the array (the kernels overwrite
+// every word they read, so nothing needs clearing); calls that overlap on
+// one GPU race.  The array's device address is looked up once per device
+// (hipGetSymbolAddress) and only that address reaches the kernels.
+constexpr int kStaticMaxSamples = 8192;
+constexpr long long kStaticScratchFloats =
+    ((kStaticMaxSamples + 31) / 32 + ((kStaticMaxSamples + 31) / 32 + kReduceChunk - 1) / kReduceChunk) *
+    (mlp_nparams(64) + 1);
+static __device__ __align__(256) float scratch[kStaticScratchFloats];
+
+constexpr int kMaxDevices = 64;
+static float* g_scratch_dev[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+int mlp_grad_static(int H, int B, const float* X, const float* T, const float* prm, int bf16,
+                    float* grad, float* loss, hipStream_t s, std::string* why) {
+  if (B > kStaticMaxSamples) {
+    *why = "mlp_grad_static: at most " + std::to_string(kStaticMaxSamples) + " samples per call";
+    return DNSMI_EINVAL;
+  }
+  if (mlp_grad_scratch(H, B) > kStaticScratchFloats) {
+    *why = "mlp_grad_static: scratch too small";
+    return DNSMI_EINVAL;
+  }
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    *why = "mlp_grad_static: no current device";
+    return DNSMI_EHIP;
+  }
+  float* ptr = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+      void* p = nullptr;
+      if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+        *why = "mlp_grad_static: hipGetSymbolAddress failed";
+        return DNSMI_EHIP;
+      }
+      g_scratch_dev[dev] = static_cast<float*>(p);
+    }
+    ptr = g_scratch_dev[dev];
+  }
+  return mlp_grad(H, B, X, T, prm, bf16, ptr, grad, loss, s);
 }
 
 int mlp_adam(int H, float* prm, const float* g, float* m, float* v, int step, float lr,

@@ -94,7 +94,7 @@ class Net:
     def compute_grad(self, X: torch.Tensor, T: torch.Tensor, bf16: bool = True) -> torch.Tensor:
         """FP32 gradient of the mean squared error into ``self.grad``; returns the loss (device)."""
         B = X.shape[0]
-        need = ((B + 31) // 32) * (self.nparams + 1) + ((B + 1023) // 1024) * self.nparams
+        need = ((B + 31) // 32 + (B + 1023) // 1024) * (self.nparams + 1)
         if self._part.numel() < need:
             self._part = torch.empty(need, dtype=torch.float32, device=self.device)
         _lib.check(self.lib.dnsmi_mlp_grad(self.H, B, _ptr(X), _ptr(T), _ptr(self.params), int(bf16),

@@ -175,13 +175,23 @@ int dnsmi_get_stats(dnsmi_ctx* ctx, const double* state, double* stats_dev, void
 int dnsmi_mlp_sizes(int H, long long* nparams, long long* packed_weight_bytes,
                     long long* bias_floats);
 /* Gradient of L = (1/B) sum (y - t)^2 over X (B,16), T (B,6), FP32, samples
- * summed in a fixed order (deterministic).  part: ceil(B/32) * (nparams + 1)
- * + ceil(B/1024) * nparams floats of scratch; grad: nparams floats;
+ * summed in a fixed order (deterministic).  part: (ceil(B/32) +
+ * ceil(B/1024)) * (nparams + 1) floats of scratch; grad: nparams floats;
  * loss_dev: 1 float or NULL.
  * bf16 != 0 rounds weights and layer inputs to bfloat16 as inference does
  * (straight-through gradients). */
 int dnsmi_mlp_grad(int H, int B, const float* X, const float* T, const float* params, int bf16,
                    float* part, float* grad, float* loss_dev, void* stream);
+/* dnsmi_mlp_grad without the `part` workspace and without any device
+ * allocation.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under graph capture, serving loops).
+ * The scratch is a file-scope device array of 6.0 MB per GPU (held until
+ * the process ends), which limits a call to B <= 8192 samples (larger B:
+ * DNSMI_EINVAL).  Every call on one GPU shares that array: calls that overlap
+ * on one GPU (other streams, other threads) race.  Same kernels and sums as
+ * dnsmi_mlp_grad, so the results are identical. */
+int dnsmi_mlp_grad_static(int H, int B, const float* X, const float* T, const float* params,
+                          int bf16, float* grad, float* loss_dev, void* stream);
 /* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8; step counts from 1). */
 int dnsmi_mlp_adam(int H, float* params, const float* grad, float* m, float* v, int step, float lr,
                    void* stream);
