@@ -30,6 +30,12 @@ METRIC = "time-steps/s + HBM roofline % at 128³ Kolmogorov; mixed+learned vs FP
 PEAK_GBS = 8000.0
 N = 128
 RE, DT, RMS, SEED, TOL = 100.0, 0.005, 0.1, 1, 1e-10
+WALLS_SEED = 2  # the plane Couette 64x128x64 workload's seeded perturbation
+
+
+def progress(what: str) -> None:
+    """One line on stderr per stage (stdout carries only the result line)."""
+    print(f"[bench] {what}", file=sys.stderr, flush=True)
 
 
 def kernel_bytes(n: int, b: int) -> dict:
@@ -49,6 +55,12 @@ def kernel_bytes(n: int, b: int) -> dict:
         "y_forward:rhs": 3 * P * Nz * Nx * b + S,
         "y_forward:predict": 3 * P * Nz * Nx * b + 3 * S,
         "y_forward:correct": 3 * P * Nz * Nx * b + 4 * S,
+        # the mixed form's split evaluations: the recording evaluation writes
+        # u_ref and the six physical fields (FP32), each increment reads them
+        # and the y forward adds N(u_ref)
+        "y_inverse:split": S + 5 * P * Nz * Nx * b + S,
+        "x_nonlinear:split": 9 * P * P * Nx * b + 6 * P * P * P * 4,
+        "y_forward:correct_split": 3 * P * Nz * Nx * b + 5 * S,
         "S": S,
     }
 
@@ -117,22 +129,28 @@ def measured_traffic(kernel: str, b: int, mode: str = "2"):
     if not os.path.exists(path):
         return None
     t = "double" if b == 16 else "float"
-    key = f"k_{kernel}<{t}, 192, 16>" if kernel != "y_forward" else f"k_y_forward<{t}, 192, 16, {mode}, true>"
-    ent = json.load(open(path))["kernels"].get(key)
+    args = {"y_inverse": f"{t}, 192, 16, true, 0", "x_nonlinear": f"{t}, 192, 16, 0",
+            "y_forward": f"{t}, 192, 16, {mode}, true"}.get(kernel, f"{t}, 192, 16")
+    ent = json.load(open(path))["kernels"].get(f"k_{kernel}<{args}>")
     return int(ent["traffic_bytes"]) if ent else None
 
 
-def roofline(form: dict, b: int) -> dict:
+def roofline(form: dict, b: int, split: bool = False) -> dict:
     kb = kernel_bytes(N, b)
     prof = form["prof"]
-    # y_forward launches: one predict per step, the rest correct
+    # y_forward launches: one predict per step, the rest correct; in the
+    # split mixed form every y inverse and x pass moves the reference
+    # fields too (recorded once per step, read by every increment)
     steps = len(form["prof_counts"])
     per = {}
     for name, (ms, cnt) in prof.items():
         if name == "y_forward":
             n_pred = steps
             n_corr = cnt - n_pred
-            byts = (n_pred * kb["y_forward:predict"] + n_corr * kb["y_forward:correct"]) / cnt
+            corr = kb["y_forward:correct_split" if split else "y_forward:correct"]
+            byts = (n_pred * kb["y_forward:predict"] + n_corr * corr) / cnt
+        elif split and name in ("y_inverse", "x_nonlinear"):
+            byts = kb[name + ":split"]
         elif name in kb:
             byts = kb[name]
         else:
@@ -309,19 +327,29 @@ def learned_form(steps: int, collect_steps: int = 30, hidden: int = 32, train_it
     for name, (t, rhs, conv, st, n) in rows.items():
         out[name] = {"steps_per_s": round(steps / t, 3), "ms_per_step": round(1e3 * t / steps, 4),
                      "rhs_evals_per_step": round(rhs / steps, 3), "converged": conv, **errors(st, n)}
-    # The configuration's full length: 500 steps of each form from the same
-    # state, end-state errors of the mixed + learned run against the FP64 one.
+    # The configuration itself: 500 steps of each form from the seeded
+    # initial state (the network was trained on the first 30 steps' history),
+    # every step's corrector iterated to the tolerance; end-state errors of
+    # each mixed form against the FP64 run.
     long_n = 500
-    long_ref, t_lr = timed(lambda: run(fp64, u_mid.clone(), long_n))
-    long_ml, t_lm = timed(lambda: run_guessed(mixed, u_mid.clone(), long_n, mode=2, net=net, scales=sc))
-    refs[long_n] = long_ref.state
-    long_run = {"steps": long_n, "fp64_steps_per_s": round(long_n / t_lr, 3),
-                "mixed_learned_steps_per_s": round(long_n / t_lm, 3),
-                "speedup_vs_fp64": round(t_lr / t_lm, 3), "fp64_converged": long_ref.converged,
-                "mixed_learned_converged": long_ml.converged,
+    long_ref, t_lr = timed(lambda: run(fp64, u0.clone(), long_n))
+    long_forms = {}
+    for name, fn in (("mixed", lambda: run(mixed, u0.clone(), long_n)),
+                     ("mixed_quadratic", lambda: run_guessed(mixed, u0.clone(), long_n, mode=3)),
+                     ("mixed_learned", lambda: run_guessed(mixed, u0.clone(), long_n, mode=2, net=net, scales=sc))):
+        rr, tt = timed(fn)
+        a, b = fp64.get_stats(rr.state), fp64.get_stats(long_ref.state)
+        rel_ = lambda k: abs(float(a[k]) - float(b[k])) / abs(float(b[k]))  # noqa: E731
+        long_forms[name] = {
+            "steps_per_s": round(long_n / tt, 3), "speedup_vs_fp64": round(t_lr / tt, 3),
+            "rhs_evals_per_step": round(rr.rhs_evals / long_n, 3), "converged": rr.converged,
+            "max_error": max(rr.errors), "escalations": rr.escalations,
+            "energy_rel_err_vs_fp64": rel_("E'"), "dissipation_rel_err_vs_fp64": rel_("D"),
+            "velocity_rel_l2_vs_fp64": float(torch.linalg.vector_norm(rr.state - long_ref.state)
+                                             / torch.linalg.vector_norm(long_ref.state))}
+    long_run = {"steps": long_n, "from": "the seeded initial state", "fp64_steps_per_s": round(long_n / t_lr, 3),
                 "fp64_rhs_evals_per_step": round(long_ref.rhs_evals / long_n, 3),
-                "mixed_learned_rhs_evals_per_step": round(long_ml.rhs_evals / long_n, 3),
-                **errors(long_ml.state, long_n)}
+                "fp64_converged": long_ref.converged, **long_forms}
     sp = out["mixed_learned"]["steps_per_s"] / out["fp64_plain"]["steps_per_s"]
     spg = out["mixed_learned_graph"]["steps_per_s"] / out["fp64_plain_graph"]["steps_per_s"]
     spg4 = out["mixed_learned_graph_4slots"]["steps_per_s"] / out["fp64_plain_graph_4slots"]["steps_per_s"]
@@ -368,7 +396,7 @@ def walls_roofline(prof: dict, nx: int, ny: int, nz: int, p: int) -> dict:
 def walls_form(steps: int, warmup: int = 2, collect_steps: int = 30, train_iters: int = 1500,
                tol: float = 1e-5) -> dict:
     """Plane Couette 64 x 128 x 64 (CGL points in y), Lx = 2 pi, Lz = pi,
-    Re 400, fd_order 4, dt 0.01, seeded perturbation RMS 0.05 (seed 1),
+    Re 400, fd_order 4, dt 0.01, seeded perturbation RMS 0.05 (seed 2),
     corrector tolerance `tol` (the reference's default is 1e-5) and limit 10.
 
     After `warmup` FP64 steps, a `collect_steps` mixed run records the
@@ -391,8 +419,8 @@ def walls_form(steps: int, warmup: int = 2, collect_steps: int = 30, train_iters
                                        max_corrector_iterations=10, precision=prec))
 
     flow, mixed = mk("fp64"), mk("mixed")
-    state = flow.init_state(seed=SEED, rms=0.05)
-    state = run(flow, state, warmup).state
+    u0 = flow.init_state(seed=WALLS_SEED, rms=0.05)
+    state = run(flow, u0.clone(), warmup).state
     t0 = time.perf_counter()
     snaps, state = collect(mixed, state, collect_steps)
     sc = fit_scales(snaps)
@@ -457,20 +485,22 @@ def walls_form(steps: int, warmup: int = 2, collect_steps: int = 30, train_iters
     def err_graph(st):
         return abs(float(flow.get_stats(st)["E'"]) - e_graph_ref) / abs(e_graph_ref)
 
-    # the configuration's full length: 200 steps of each form from the same state
+    # the configuration itself: 200 steps of each form from the seeded state
     long_n = 200
-    long_ref, t_lr = timed(lambda: run(flow, state.clone(), long_n))
-    long_ml, t_lm = timed(lambda: run_guessed(mixed, state.clone(), long_n, mode=2, net=net, scales=sc))
+    long_ref, t_lr = timed(lambda: run(flow, u0.clone(), long_n))
     e_long = float(flow.get_stats(long_ref.state)["E'"])
-    long_run = {"steps": long_n, "fp64_steps_per_s": round(long_n / t_lr, 3),
-                "mixed_learned_steps_per_s": round(long_n / t_lm, 3),
-                "speedup_vs_fp64": round(t_lr / t_lm, 3), "fp64_converged": long_ref.converged,
-                "mixed_learned_converged": long_ml.converged,
-                "fp64_rhs_evals_per_step": round(long_ref.rhs_evals / long_n, 3),
-                "mixed_learned_rhs_evals_per_step": round(long_ml.rhs_evals / long_n, 3),
-                "energy_rel_err_vs_fp64": abs(float(flow.get_stats(long_ml.state)["E'"]) - e_long) / abs(e_long)}
+    long_run = {"steps": long_n, "from": "the seeded initial state", "fp64_steps_per_s": round(long_n / t_lr, 3),
+                "fp64_converged": long_ref.converged,
+                "fp64_rhs_evals_per_step": round(long_ref.rhs_evals / long_n, 3)}
+    for name, fn in (("mixed", lambda: run(mixed, u0.clone(), long_n)),
+                     ("mixed_learned", lambda: run_guessed(mixed, u0.clone(), long_n, mode=2, net=net, scales=sc))):
+        rr, tt = timed(fn)
+        long_run[name] = {"steps_per_s": round(long_n / tt, 3), "speedup_vs_fp64": round(t_lr / tt, 3),
+                          "converged": rr.converged, "rhs_evals_per_step": round(rr.rhs_evals / long_n, 3),
+                          "escalations": rr.escalations,
+                          "energy_rel_err_vs_fp64": abs(float(flow.get_stats(rr.state)["E'"]) - e_long) / abs(e_long)}
 
-    return {"workload": f"plane Couette 64x128x64, p=4, Re 400, dt 0.01, RMS 0.05, tol {tol:g}"
+    return {"workload": f"plane Couette 64x128x64, p=4, Re 400, dt 0.01, RMS 0.05 (seed {WALLS_SEED}), tol {tol:g}"
                         + (" (reference default)" if tol == 1e-5 else ""),
             "steps_per_s": round(steps / el, 3), "ms_per_step": round(1e3 * el / steps, 4),
             "corrector_iters_per_step": round(sum(r.counts) / max(len(r.counts), 1), 3),
@@ -564,7 +594,8 @@ def profile_workload(steps: int) -> None:
     dispatch for the FETCH_SIZE / WRITE_SIZE counters: it must read and
     write exactly CAL_BYTES), then `steps` steps of the FP64 and of the
     mixed form at 128^3, then a short training run of the corrector-guess
-    network and `steps` steps of the mixed form with the learned guess.
+    network and `steps` steps of the mixed form with the learned guess, then
+    `steps` steps of plane Couette 64 x 128 x 64 in both forms.
     """
     import torch
 
@@ -597,25 +628,43 @@ def profile_workload(steps: int) -> None:
     g = run_guessed(flow, state, steps, mode=2, net=net, scales=sc)
     torch.cuda.synchronize()
     print(f"mixed + learned guess: {steps} steps, corrector passes {g.passes}", flush=True)
+    del flow, state, snaps, X, T, g
+    # plane Couette 64 x 128 x 64 (p = 4), both forms
+    from dnsjax_amd.flows import PlaneCouetteFlow
+    from dnsjax_amd.params import Params
+
+    for prec in ("fp64", "mixed"):
+        wf = PlaneCouetteFlow(Params(system="plane-couette", nx=64, ny=128, nz=64, lx=2 * math.pi, lz=math.pi,
+                                     re=400.0, dt=0.01, fd_order=4, corrector_tolerance=1e-5,
+                                     max_corrector_iterations=10, precision=prec))
+        ws = wf.init_state(seed=WALLS_SEED, rms=0.05)
+        ws, counts = run_steps(wf, ws, steps)
+        torch.cuda.synchronize()
+        print(f"plane Couette {prec}: {steps} steps, corrector iterations {counts}", flush=True)
 
 
-def walls_large_form(steps: int, collect_steps: int = 10, train_iters: int = 400) -> dict:
-    """Plane Couette at the learned-guess configuration's shape: 128 x 256 x
-    128 CGL points, fd_order 6, Re 1000, dt 0.005, corrector tolerance 1e-10,
-    seeded perturbation RMS 0.05 (seed 4).  A `collect_steps` mixed run
-    records the nonlinear-term history; the per-point network is trained on
-    it (set-up, timed separately).  From the state that run ends in, the
-    plain FP64 method and the mixed form started from the learned guess are
-    timed over the same `steps`: corrector RHS + IMM evaluations per step,
-    steps/s and the end-state energy against the FP64 run's (a bounded
-    segment of the configuration's 1000 steps)."""
+def walls_large_form(steps: int, collect_steps: int = 8, train_iters: int = 300, pairs: int = 20000) -> dict:
+    """The learned-guess configuration: plane Couette 128 x 256 x 128 CGL
+    points, fd_order 6, Re 1000, dt 0.005, corrector tolerance 1e-10,
+    seeded perturbation RMS 0.05 (seed 4), `steps` steps (1000).
+
+    A `collect_steps` mixed run from the seeded state records the nonlinear-
+    term history; `pairs` (predictor history -> converged corrector N) pairs
+    drawn from it train the per-(kz, kx) profile network (two hidden layers
+    of 256, bf16 matrix-core GEMMs in both passes; set-up, timed
+    separately).  Then each form runs the configuration from the seeded
+    state: plain FP64 (the reference's method), mixed, mixed started from
+    quadratic extrapolation and mixed started from the network's guess;
+    corrector evaluations (RHS + IMM) per step, steps/s and end-state
+    energy against the FP64 run."""
     import torch
 
     from dnsjax_amd.driver import run
     from dnsjax_amd.fields import wall_field
     from dnsjax_amd.flows import PlaneCouetteFlow
-    from dnsjax_amd.learned import Net, collect, fit_scales, make_dataset, run_guessed, train
+    from dnsjax_amd.learned import collect, run_guessed
     from dnsjax_amd.params import Params
+    from dnsjax_amd.profile_net import ProfileNet, fit_scales, make_dataset, mean_loss, train
 
     nx, ny, nz = 128, 256, 128
 
@@ -631,15 +680,20 @@ def walls_large_form(steps: int, collect_steps: int = 10, train_iters: int = 400
     t_setup = time.perf_counter() - t0
     u0 = fp64.to_device(wall_field(nx, ny, nz, 2 * math.pi, math.pi, 0.05, seed=4))
     t0 = time.perf_counter()
-    snaps, u_mid = collect(mixed, u0, collect_steps)
+    snaps, _ = collect(mixed, u0.clone(), collect_steps)
     sc = fit_scales(snaps)
-    X, T = make_dataset(mixed, snaps, sc, 50000, seed=1)
+    net = ProfileNet(ny, hidden=256, seed=0)
+    X, T = make_dataset(mixed, snaps, sc, pairs + 2000, net.din, seed=1)
     del snaps
-    net = Net(32, seed=0)
-    train(net, X, T, iters=train_iters, batch=8192, lr=3e-3)
+    Xv, Tv, X, T = X[pairs:], T[pairs:], X[:pairs], T[:pairs]
+    torch.cuda.synchronize()
+    t_data = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    train(net, X, T, iters=train_iters, batch=1024, lr=1e-3)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
-    del X, T
+    loss, base = mean_loss(net, Xv, Tv)
+    del X, T, Xv, Tv
 
     def timed(fn):
         torch.cuda.synchronize()
@@ -648,21 +702,29 @@ def walls_large_form(steps: int, collect_steps: int = 10, train_iters: int = 400
         torch.cuda.synchronize()
         return r, time.perf_counter() - t
 
-    plain, tp = timed(lambda: run(fp64, u_mid.clone(), steps))
-    learned, tl = timed(lambda: run_guessed(mixed, u_mid.clone(), steps, mode=2, net=net, scales=sc))
+    plain, tp = timed(lambda: run(fp64, u0.clone(), steps))
     e_p = float(fp64.get_stats(plain.state)["E'"])
-    e_l = float(fp64.get_stats(learned.state)["E'"])
+    forms = {}
+    for name, fn in (("mixed", lambda: run(mixed, u0.clone(), steps)),
+                     ("mixed_quadratic", lambda: run_guessed(mixed, u0.clone(), steps, mode=3)),
+                     ("mixed_learned", lambda: run_guessed(mixed, u0.clone(), steps, mode=2, net=net, scales=sc))):
+        r, t = timed(fn)
+        e = float(fp64.get_stats(r.state)["E'"])
+        forms[name] = {"steps_per_s": round(steps / t, 3), "speedup_vs_fp64": round(tp / t, 3),
+                       "rhs_evals_per_step": round(r.rhs_evals / len(r.errors), 3), "converged": r.converged,
+                       "steps_run": len(r.errors), "escalations": r.escalations,
+                       "energy_rel_err_vs_fp64": abs(e - e_p) / abs(e_p)}
     return {"workload": "plane Couette 128x256x128, p=6, Re 1000, dt 0.005, RMS 0.05 (seed 4), tol 1e-10",
-            "steps": steps, "setup_s": round(t_setup, 2), "data_and_train_s": round(t_train, 2),
+            "steps": steps, "from": "the seeded initial state", "setup_s": round(t_setup, 2),
             "fp64": {"steps_per_s": round(steps / tp, 3), "rhs_evals_per_step": round(plain.rhs_evals / steps, 3),
+                     "corrector_iters_per_step": round(sum(plain.counts) / steps, 3),
                      "converged": plain.converged, "tuning": fp64._native.tuning.strip()},
-            "mixed_learned": {"steps_per_s": round(steps / tl, 3),
-                              "rhs_evals_per_step": round(learned.rhs_evals / steps, 3),
-                              "converged": learned.converged,
-                              "energy_rel_err_vs_fp64": abs(e_l - e_p) / abs(e_p)},
-            "speedup_vs_fp64": round(tp / tl, 3),
-            "segment": f"{steps} steps after a {collect_steps}-step mixed data run (the first guessed step "
-                       "has no history and runs the plain predictor)"}
+            **forms,
+            "speedup_vs_fp64": forms["mixed_learned"]["speedup_vs_fp64"],
+            "net": {"kind": "per-(kz, kx) profile MLP", "din": net.din, "hidden": [net.H, net.H], "dout": net.dout,
+                    "weights": "bf16 on the matrix cores, FP32 master copy", "pairs": pairs,
+                    "train_iters": train_iters, "data_s": round(t_data, 2), "train_s": round(t_train, 2),
+                    "heldout_loss": round(loss, 5), "heldout_loss_without_net": round(base, 5)}}
 
 
 def profile_imm(calls: int) -> None:
@@ -717,8 +779,8 @@ def main():
                     help="steps of each form in the mixed+learned A/B (0 skips it)")
     ap.add_argument("--ensemble-steps", type=int, default=50,
                     help="steps of the 64-member ensemble form (0 skips it)")
-    ap.add_argument("--walls-large-steps", type=int, default=8,
-                    help="steps of the plane Couette 128x256x128 learned-guess form (0 skips it)")
+    ap.add_argument("--walls-large-steps", type=int, default=1000,
+                    help="steps of the plane Couette 128x256x128 learned-guess configuration (0 skips it)")
     ap.add_argument("--profile-walls", action="store_true",
                     help="run a few steps of the plane Couette form for rocprofv3 and exit")
     ap.add_argument("--profile-workload", action="store_true",
@@ -740,10 +802,11 @@ def main():
         profile_imm(a.steps)
         return
     os.environ.setdefault("OMP_NUM_THREADS", "1")
+    progress("periodic 128^3: fp64 and mixed")
     fp64 = gpu_form("fp64", a.steps, a.warmup)
     mixed = gpu_form("mixed", a.steps, a.warmup)
     rf64 = roofline(fp64, 16)
-    rmix = roofline(mixed, 8)
+    rmix = roofline(mixed, 8, split=True)
     out = {
         "metric": METRIC,
         "value": round(fp64["steps_per_s"], 3),
@@ -770,32 +833,65 @@ def main():
                   "kernels": rmix["kernels"]},
     }
     if a.walls_steps > 0:
+        progress("plane Couette 64x128x64")
         out["walls"] = walls_form(a.walls_steps)
         out["walls_mixed_learned_vs_fp64"] = out["walls"]["mixed_learned"]["speedup_vs_fp64"]
         # the same forms where the tolerance no longer sets the energy
-        # difference (the wall corrector contracts ~2x per pass, so at 1e-5
-        # a step stopping at the first pass differs by O(tol) from one that
-        # paid the next pair of passes)
-        wt = walls_form(a.walls_steps, tol=1e-7)
-        out["walls_tol_1e-7"] = {
+        # difference (the wall corrector contracts ~30x per pair of passes,
+        # so at 1e-5 a step stopping at the first pass differs by O(tol) from
+        # one that paid the next pair)
+        wt = walls_form(a.walls_steps, tol=1e-8)
+        out["walls_tol_1e-8"] = {
             "fp64_steps_per_s": wt["steps_per_s"], "fp64_rhs_evals_per_step": wt["fp64_rhs_evals_per_step"],
             **{f: {k: wt[f][k] for k in ("steps_per_s", "rhs_evals_per_step", "energy_rel_err_vs_fp64",
                                          "converged")} for f in ("mixed", "mixed_guess", "mixed_learned")},
             "mixed_learned_vs_fp64": wt["mixed_learned"]["speedup_vs_fp64"], "long_run": wt["long_run"]}
     if a.learned_steps > 0:
+        progress("periodic 128^3: learned guess")
         lf = learned_form(a.learned_steps)
         out["mixed_learned"] = lf
         out["mixed_learned_vs_fp64"] = lf["speedup_vs_fp64"]
     if a.ensemble_steps > 0:
+        progress("64-member ensemble")
         out["ensemble"] = ensemble_form(a.ensemble_steps)
     if a.walls_large_steps > 0:
+        progress("plane Couette 128x256x128 with the profile network")
         out["walls_large"] = walls_large_form(a.walls_large_steps)
     if not a.no_cpu:
+        progress("CPU baseline")
         out["cpu_baseline"] = cpu_baseline(a.cpu_seconds)
         ncores = min(16, len(os.sched_getaffinity(0)))
         out["cpu_baseline_all_cores"] = cpu_baseline(a.cpu_seconds, workers=ncores) if ncores > 1 else None
     else:
         out["cpu_baseline"] = None
+    # the headline figures once more, last on the line (a reader of the
+    # line's tail sees them)
+    summ = {"fp64_steps_per_s": out["value"], "mixed_steps_per_s": out["mixed"]["value"],
+            "rhs_roofline_frac_fp64": out["rhs_roofline"]["frac"],
+            "rhs_roofline_frac_mixed": out["mixed"]["rhs_roofline"]["frac"]}
+    if "mixed_learned" in out:
+        lr = out["mixed_learned"]["long_run"]
+        summ["periodic_500_steps"] = {
+            f: {k: lr[f][k] for k in ("speedup_vs_fp64", "rhs_evals_per_step", "converged", "escalations",
+                                      "energy_rel_err_vs_fp64", "dissipation_rel_err_vs_fp64",
+                                      "velocity_rel_l2_vs_fp64")}
+            for f in ("mixed", "mixed_quadratic", "mixed_learned")}
+        summ["periodic_500_steps"]["fp64_rhs_evals_per_step"] = lr["fp64_rhs_evals_per_step"]
+    if "walls" in out:
+        summ["walls_200_steps"] = {t: {f: {k: w["long_run"][f][k] for k in ("speedup_vs_fp64", "converged",
+                                                                               "energy_rel_err_vs_fp64")}
+                                       for f in ("mixed", "mixed_learned")}
+                                   for t, w in (("tol_1e-5", out["walls"]), ("tol_1e-8", out["walls_tol_1e-8"]))}
+    if "walls_large" in out:
+        wl = out["walls_large"]
+        summ["walls_128x256x128_1000_steps"] = {
+            f: {k: wl[f][k] for k in ("speedup_vs_fp64", "rhs_evals_per_step", "converged",
+                                      "energy_rel_err_vs_fp64")} for f in ("mixed", "mixed_quadratic", "mixed_learned")}
+        summ["walls_128x256x128_1000_steps"]["fp64_rhs_evals_per_step"] = wl["fp64"]["rhs_evals_per_step"]
+    if "ensemble" in out:
+        summ["ensemble_member_steps_per_s"] = out["ensemble"]["value"]
+        summ["ensemble_speedup_vs_sequential"] = out["ensemble"]["speedup_vs_sequential"]
+    out["summary"] = summ
     print(json.dumps(out), flush=True)
 
 

@@ -50,6 +50,7 @@ class DnsmiConfig(ctypes.Structure):
         ("correct_divergence", ctypes.c_int),
         ("tile_cols", ctypes.c_int),
         ("variants", ctypes.c_int),
+        ("plain_mixed", ctypes.c_int),
     ]
 
 
@@ -75,15 +76,24 @@ SIGNATURES = {
     "dnsmi_walls_table": (_I, [_P, _I, _P, ctypes.c_longlong]),
     "dnsmi_tuning": (ctypes.c_char_p, [_P]),
     "dnsmi_get_stats": (_I, [_P, _P, _P, _P]),
-    "dnsmi_step_device": (_I, [_P, _P, _P, _P, _P, _I, _P, _I, ctypes.c_double, _I, ctypes.c_double,
-                               ctypes.c_double, ctypes.c_double, _P]),
+    "dnsmi_step_device": (_I, [_P, _P, _P, _P, _P, _I, _I, _P, _I, ctypes.c_double, _I, ctypes.c_double,
+                               ctypes.c_double, _P]),
     "dnsmi_step_stats": (_I, [_P, ctypes.POINTER(ctypes.c_double)]),
     "dnsmi_step_reset": (_I, [_P]),
+    "dnsmi_escalate": (_I, [_P, _I]),
     "dnsmi_mlp_sizes": (_I, [_I, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong),
                              ctypes.POINTER(ctypes.c_longlong)]),
     "dnsmi_mlp_grad": (_I, [_I, _I, _P, _P, _P, _I, _P, _P, _P, _P]),
     "dnsmi_mlp_grad_static": (_I, [_I, _I, _P, _P, _P, _I, _P, _P, _P]),
     "dnsmi_mlp_adam": (_I, [_I, _P, _P, _P, _P, _I, ctypes.c_float, _P]),
+    "dnsmi_pnet_params": (ctypes.c_longlong, [_I, _I, _I]),
+    "dnsmi_pnet_grad": (_I, [_I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "dnsmi_pnet_forward": (_I, [_I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P]),
+    "dnsmi_pnet_adam": (_I, [ctypes.c_longlong, _P, _P, _P, _P, _I, ctypes.c_float, _P]),
+    "dnsmi_pnet_features": (_I, [_P, _P, _P, _P, _P, _P, _I, _I, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                 _P, _P, _P]),
+    "dnsmi_guess_profile": (_I, [_P, _P, _P, _P, _P, _P, _I, _I, _I, _P, _P, _P, _P, _P, _P, ctypes.c_float,
+                                 ctypes.c_float, ctypes.c_float, _P]),
     "dnsmi_mlp_pack": (_I, [_I, _P, _P, _P, _P]),
     "dnsmi_mlp_forward": (_I, [_I, _P, _I, _P, _P, _P, _P]),
     "dnsmi_mlp_features": (_I, [_P, _P, _P, _P, _P, _P, _I, ctypes.c_float, ctypes.c_float,
@@ -91,8 +101,8 @@ SIGNATURES = {
     "dnsmi_guess": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _I, ctypes.c_float, ctypes.c_float,
                          ctypes.c_float, _I, _P]),
     "dnsmi_step_device_guess": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _I, ctypes.c_float, ctypes.c_float,
-                                     ctypes.c_float, _I, _I, _P, _I, ctypes.c_double, _I, ctypes.c_double,
-                                     ctypes.c_double, ctypes.c_double, _P]),
+                                     ctypes.c_float, _I, _I, _I, _P, _I, ctypes.c_double, _I, ctypes.c_double,
+                                     ctypes.c_double, _P]),
 }
 
 KERNEL_IDS = 12

@@ -77,14 +77,14 @@ int walls_iterate_correction(Ctx* c, const double* state_prev, double* pred,
 int walls_table(Ctx* c, int which, double* out, long long n);
 int get_stats(Ctx* c, const double* state, double* out, hipStream_t s);
 int step_device(Ctx* c, const double* state, double* pred, double* rhs_a, double* rhs_b, int slots,
-                int* counts, int counts_cap, double tol, int max_iter, double floor_ratio,
-                double floor_cap, double floor_margin, hipStream_t s);
+                int esc_slots, int* counts, int counts_cap, double tol, int max_iter,
+                double esc_ratio, double esc_cap, hipStream_t s);
 int step_read_ctl(Ctx* c, double* out6);
 int step_device_guess(Ctx* c, const double* state, double* pred, double* rhs_a, double* rhs_b,
                       const float* h1, float* h2, const uint16_t* wb, const float* bias, int H,
-                      float s_1, float s_2, float s_out, int mode, int slots, int* counts,
-                      int counts_cap, double tol, int max_pass, double floor_ratio,
-                      double floor_cap, double floor_margin, hipStream_t s);
+                      float s_1, float s_2, float s_out, int mode, int slots, int esc_slots,
+                      int* counts, int counts_cap, double tol, int max_pass, double esc_ratio,
+                      double esc_cap, hipStream_t s);
 int step_alloc_ctl(Ctx* c);
 int mlp_grad(int H, int B, const float* X, const float* T, const float* prm, int bf16, float* part,
              float* grad, float* loss, hipStream_t s);
@@ -93,6 +93,19 @@ int mlp_grad_static(int H, int B, const float* X, const float* T, const float* p
 long long mlp_grad_scratch(int H, int B);
 int mlp_adam(int H, float* prm, const float* g, float* m, float* v, int step, float lr,
              hipStream_t s);
+int pnet_forward(int din, int H, int dout, int B, const uint16_t* X, const float* prm, uint16_t* wb,
+                 float* wf, uint16_t* wh, float* Y, hipStream_t s);
+int pnet_grad(int din, int H, int dout, int B, const uint16_t* X, const float* T, const float* prm,
+              uint16_t* wb, float* wf, uint16_t* wh, float* part, float* grad, float* loss,
+              hipStream_t s);
+int pnet_adam(long long P, float* prm, const float* g, float* m, float* v, int step, float lr,
+              hipStream_t s);
+int pnet_features(Ctx* c, const double* nm2, const double* nm1, const double* ncur, const double* nnext,
+                  const long long* modes, int ns, int din, float s_1, float s_2, float s_out, uint16_t* X,
+                  float* T, hipStream_t s);
+int guess_walls_profile(Ctx* c, const double* state, double* pred, double* rhs_prev, const float* h1, float* h2,
+                        int din, int H, int dout, const float* prm, uint16_t* wb, uint16_t* X, float* wf,
+                        uint16_t* wh, float* Y, float s_1, float s_2, float s_out, hipStream_t s);
 int mlp_pack(int H, const float* prm, uint16_t* wb, float* bias, hipStream_t s);
 int mlp_forward(int H, const float* X, int B, const uint16_t* wb, const float* bias, float* Y,
                 hipStream_t s);
@@ -143,10 +156,14 @@ int dnsmi_create(const dnsmi_config* cfg, dnsmi_ctx** out) {
     rc = c->fail(DNSMI_EHIP, "no HIP device");
   } else if (cfg->precision != DNSMI_FP64 && cfg->precision != DNSMI_MIXED) {
     rc = c->fail(DNSMI_EINVAL, "precision must be DNSMI_FP64 or DNSMI_MIXED");
+  } else if (cfg->tile_cols != 0 && cfg->tile_cols != 8 && cfg->tile_cols != 16) {
+    rc = c->fail(DNSMI_EINVAL, "tile_cols must be 0 (autotune), 8 or 16");
   } else if (!(cfg->re > 0) || !(cfg->dt > 0) || cfg->implicitness < 0 || cfg->implicitness > 1) {
     rc = c->fail(DNSMI_EINVAL, "need re > 0, dt > 0, 0 <= implicitness <= 1");
   } else {
     c->fp32 = cfg->precision == DNSMI_MIXED;
+    c->fp32_primary = c->fp32;
+    c->split = c->fp32 && !cfg->plain_mixed;
     if (cfg->tile_cols > 0) {
       c->tile_cols = cfg->tile_cols;
       c->yinv_seq = (cfg->variants & 1) != 0;
@@ -267,18 +284,17 @@ int dnsmi_get_stats(dnsmi_ctx* ctx, const double* state, double* stats_dev, void
 }
 
 int dnsmi_step_device(dnsmi_ctx* ctx, const double* state, double* next, double* rhs_a,
-                      double* rhs_b, int slots, int* counts, int counts_cap, double tol,
-                      int max_iter, double floor_ratio, double floor_cap, double floor_margin,
-                      void* stream) {
+                      double* rhs_b, int slots, int esc_slots, int* counts, int counts_cap,
+                      double tol, int max_iter, double esc_ratio, double esc_cap, void* stream) {
   DNSMI_CHECK_CTX(ctx)
-  DNSMI_REQUIRE(state && next && rhs_a && rhs_b && slots >= 0 && max_iter >= 0,
+  DNSMI_REQUIRE(state && next && rhs_a && rhs_b && slots >= 0 && esc_slots >= 0 && max_iter >= 0,
                 "step_device: bad arguments");
   DNSMI_REQUIRE((const void*)state != next && rhs_a != rhs_b && (void*)next != rhs_a &&
                     (void*)next != rhs_b && (const void*)state != rhs_a &&
                     (const void*)state != rhs_b,
                 "step_device: buffers must be distinct");
-  return dnsmi::step_device(c, state, next, rhs_a, rhs_b, slots, counts, counts_cap, tol, max_iter,
-                            floor_ratio, floor_cap, floor_margin, as_stream(stream));
+  return dnsmi::step_device(c, state, next, rhs_a, rhs_b, slots, esc_slots, counts, counts_cap, tol,
+                            max_iter, esc_ratio, esc_cap, as_stream(stream));
 }
 
 int dnsmi_step_stats(dnsmi_ctx* ctx, double* out6) {
@@ -290,6 +306,13 @@ int dnsmi_step_stats(dnsmi_ctx* ctx, double* out6) {
 int dnsmi_step_reset(dnsmi_ctx* ctx) {
   DNSMI_CHECK_CTX(ctx)
   return dnsmi::step_reset_ctl(c);
+}
+
+int dnsmi_escalate(dnsmi_ctx* ctx, int fp64) {
+  DNSMI_CHECK_CTX(ctx)
+  DNSMI_REQUIRE(!fp64 || c->fp32_primary, "escalate: FP64 passes need the mixed form");
+  c->fp32 = fp64 ? false : c->fp32_primary;
+  return DNSMI_OK;
 }
 
 // Context-free entry points report through dnsmi_last_error(NULL).
@@ -337,6 +360,37 @@ int dnsmi_mlp_adam(int H, float* params, const float* grad, float* m, float* v, 
   return ctxfree(dnsmi::mlp_adam(H, params, grad, m, v, step, lr, as_stream(stream)), "mlp_adam");
 }
 
+long long dnsmi_pnet_params(int din, int H, int dout) {
+  if (din <= 0 || din % 32 || H <= 0 || H % 64 || dout <= 0) return -1;
+  return (long long)din * H + H + (long long)H * H + H + (long long)H * dout + dout;
+}
+
+int dnsmi_pnet_grad(int din, int H, int dout, int B, const void* X, const float* T, const float* params,
+                    void* wb, float* wf, void* wh, float* part, float* grad, float* loss_dev, void* stream) {
+  if (!X || !T || !params || !wb || !wf || !wh || !part || !grad || B <= 0 || dnsmi_pnet_params(din, H, dout) < 0)
+    return ctxfree(DNSMI_EINVAL, "pnet_grad");
+  return ctxfree(dnsmi::pnet_grad(din, H, dout, B, static_cast<const uint16_t*>(X), T, params,
+                                  static_cast<uint16_t*>(wb), wf, static_cast<uint16_t*>(wh), part, grad,
+                                  loss_dev, as_stream(stream)),
+                 "pnet_grad");
+}
+
+int dnsmi_pnet_forward(int din, int H, int dout, int B, const void* X, const float* params, void* wb,
+                       float* wf, void* wh, float* Y, void* stream) {
+  if (!X || !params || !wb || !wf || !wh || !Y || B <= 0 || dnsmi_pnet_params(din, H, dout) < 0)
+    return ctxfree(DNSMI_EINVAL, "pnet_forward");
+  return ctxfree(dnsmi::pnet_forward(din, H, dout, B, static_cast<const uint16_t*>(X), params,
+                                     static_cast<uint16_t*>(wb), wf, static_cast<uint16_t*>(wh), Y,
+                                     as_stream(stream)),
+                 "pnet_forward");
+}
+
+int dnsmi_pnet_adam(long long P, float* params, const float* grad, float* m, float* v, int step, float lr,
+                    void* stream) {
+  if (!params || !grad || !m || !v || step < 1 || P <= 0) return ctxfree(DNSMI_EINVAL, "pnet_adam");
+  return ctxfree(dnsmi::pnet_adam(P, params, grad, m, v, step, lr, as_stream(stream)), "pnet_adam");
+}
+
 int dnsmi_mlp_pack(int H, const float* params, void* packed_weights, float* bias, void* stream) {
   if (!params || !packed_weights || !bias) return ctxfree(DNSMI_EINVAL, "mlp_pack");
   return ctxfree(dnsmi::mlp_pack(H, params, static_cast<uint16_t*>(packed_weights), bias,
@@ -364,6 +418,30 @@ int dnsmi_mlp_features(dnsmi_ctx* ctx, const double* n_m2, const double* n_m1,
                              as_stream(stream));
 }
 
+int dnsmi_pnet_features(dnsmi_ctx* ctx, const double* n_m2, const double* n_m1, const double* n_cur,
+                        const double* n_next, const long long* modes, int nsamples, int din, float s_1,
+                        float s_2, float s_out, void* X, float* T, void* stream) {
+  DNSMI_CHECK_CTX(ctx)
+  DNSMI_REQUIRE(n_m2 && n_m1 && n_cur && modes && X && nsamples >= 0, "pnet_features: null pointer");
+  return dnsmi::pnet_features(c, n_m2, n_m1, n_cur, n_next, modes, nsamples, din, s_1, s_2, s_out,
+                              static_cast<uint16_t*>(X), T, as_stream(stream));
+}
+
+int dnsmi_guess_profile(dnsmi_ctx* ctx, const double* state, double* pred, double* rhs_prev,
+                        const float* hist1, float* hist2, int din, int H, int dout, const float* params,
+                        void* wb, void* X, float* wf, void* wh, float* Y, float s_1, float s_2, float s_out,
+                        void* stream) {
+  DNSMI_CHECK_CTX(ctx)
+  DNSMI_REQUIRE(state && pred && rhs_prev && hist1 && hist2 && params && wb && X && wf && wh && Y,
+                "guess_profile: null pointer");
+  DNSMI_REQUIRE(dnsmi_pnet_params(din, H, dout) > 0, "guess_profile: din a multiple of 32, H of 64");
+  DNSMI_REQUIRE((const void*)state != pred && (void*)pred != rhs_prev && (const void*)hist1 != hist2,
+                "guess_profile: buffers must be distinct");
+  return dnsmi::guess_walls_profile(c, state, pred, rhs_prev, hist1, hist2, din, H, dout, params,
+                                    static_cast<uint16_t*>(wb), static_cast<uint16_t*>(X), wf,
+                                    static_cast<uint16_t*>(wh), Y, s_1, s_2, s_out, as_stream(stream));
+}
+
 int dnsmi_guess(dnsmi_ctx* ctx, const double* state, double* pred, double* rhs_prev,
                 const float* hist1, float* hist2, const void* packed_weights, const float* bias,
                 int H, float s_1, float s_2, float s_out, int mode, void* stream) {
@@ -384,21 +462,21 @@ int dnsmi_guess(dnsmi_ctx* ctx, const double* state, double* pred, double* rhs_p
 int dnsmi_step_device_guess(dnsmi_ctx* ctx, const double* state, double* next, double* rhs_a,
                             double* rhs_b, const float* hist1, float* hist2,
                             const void* packed_weights, const float* bias, int H, float s_1,
-                            float s_2, float s_out, int mode, int slots, int* counts,
-                            int counts_cap, double tol, int max_pass, double floor_ratio,
-                            double floor_cap, double floor_margin, void* stream) {
+                            float s_2, float s_out, int mode, int slots, int esc_slots,
+                            int* counts, int counts_cap, double tol, int max_pass,
+                            double esc_ratio, double esc_cap, void* stream) {
   DNSMI_CHECK_CTX(ctx)
   DNSMI_REQUIRE(state && next && rhs_a && rhs_b && hist1 && hist2, "step_device_guess: null pointer");
   DNSMI_REQUIRE(mode != 2 || (packed_weights && bias), "step_device_guess: network weights missing");
-  DNSMI_REQUIRE(slots >= 0 && max_pass >= 1 && (counts || counts_cap == 0),
+  DNSMI_REQUIRE(slots >= 0 && esc_slots >= 0 && max_pass >= 1 && (counts || counts_cap == 0),
                 "step_device_guess: bad slots / max_pass / counts");
   DNSMI_REQUIRE((const void*)state != next && (void*)rhs_a != rhs_b && (void*)next != rhs_a &&
                     (void*)next != rhs_b && (const void*)hist1 != hist2,
                 "step_device_guess: buffers must be distinct");
   return dnsmi::step_device_guess(c, state, next, rhs_a, rhs_b, hist1, hist2,
                                   static_cast<const uint16_t*>(packed_weights), bias, H, s_1, s_2,
-                                  s_out, mode, slots, counts, counts_cap, tol, max_pass,
-                                  floor_ratio, floor_cap, floor_margin, as_stream(stream));
+                                  s_out, mode, slots, esc_slots, counts, counts_cap, tol, max_pass,
+                                  esc_ratio, esc_cap, as_stream(stream));
 }
 
 const char* dnsmi_tuning(const dnsmi_ctx* ctx) { return ctx ? ctx->impl.tuning.c_str() : ""; }

@@ -34,12 +34,27 @@ struct ProfRec {
   hipEvent_t a, b;
 };
 
-// The corrector rule of a device step (tolerance, iteration limit and the
-// mixed form's noise-floor stops: ratio, cap, margin; 0 disables them).
+// The corrector rule of a device step: the reference's tolerance and
+// iteration limit, and the mixed form's escalation to FP64 passes (a pass
+// whose error, at or below `cap`, is not below `ratio` times the error two
+// passes earlier has reached the FP32 evaluations' resolution; `ratio` 0
+// disables it) after at most `primary` passes in the context's precision.
 struct CtlRule {
   double tol;
   int max_iter;
-  double ratio, cap, margin;
+  double ratio, cap;
+  int primary;
+};
+
+// How one evaluation of the nonlinear term treats the mixed form's
+// reference state (split-precision evaluations, periodic.hip / walls.hip).
+enum EvalMode {
+  EV_PLAIN = 0,   // N(u) evaluated in full in the context's precision
+  EV_RECORD = 1,  // the same, and u, its physical velocity and vorticity and
+                  // N(u) become the reference (u_ref, x_ref, n_ref)
+  EV_INC = 2      // N(u) = n_ref + P[dN]: only the increment
+                  // dN = (U + u_ref) x w_d + d x (W_ref + w_d), d = u - u_ref,
+                  // goes through the FP32 transforms
 };
 
 struct Ctx {
@@ -70,6 +85,17 @@ struct Ctx {
   double* nbase = nullptr;  // (3, Py)
 
   double* rhs0 = nullptr;      // N^0 of the predictor (spectral vector field)
+
+  // Split-precision evaluations of the mixed form (DNSMI_MIXED, unless
+  // cfg.plain_mixed): every evaluation of a step after its first computes
+  // only the increment over the step's first state u_ref in FP32, so the
+  // FP32 rounding scales with the increment, not with N.
+  bool split = false;
+  bool ref_valid = false;         // u_ref / x_ref / n_ref hold one evaluation
+  double* uref = nullptr;         // u_ref, spectral FP64 (spec_elems complex)
+  float* xref = nullptr;          // (6, rows, Px): U + u_ref (3), W + w_ref (3)
+  double* nref = nullptr;         // N(u_ref), spectral FP64
+  bool fp32_primary = false;      // the context's own precision (cfg)
   double* partials = nullptr;  // per-block partial sums of the norm
   int max_partials = 0;
 

@@ -10,41 +10,40 @@ namespace dnsmi {
 
 struct StepCtl {
   double err;
-  double prev;
-  int active;
-  int c;
-  int floor;
+  double prev;   // error of the previous pass
+  double prev2;  // error two passes back
+  int active;    // the corrector continues (the reference's rule)
+  int c;         // passes of this step so far
+  int esc;       // this step's remaining passes run in FP64 (mixed form)
+  int run_p;     // active && !esc: the next slot in the context's precision runs
+  int run_e;     // active && esc: the next FP64 escalation slot runs
   int steps;
   int total_c;
   int unconverged;
   int nan;
-  int pad;
-  double floor_est;  // noise-floor estimate: the error of the last stall stop
+  int escalated;  // steps that escalated
 };
 
-// After a pass with error e (the host rule of __main__.py:161-191): count
-// the iteration and keep going while e > tol and the limit allows.  The
-// mixed form's noise-floor stops, the same rule as the host loops: for an
-// error above `tol` and below `cap`, a pass that failed to shrink the error
-// by `ratio` is at the floor (a stall stop; its error becomes the floor
-// estimate), and with `margin` > 0 a pass whose error is within `margin`
-// times the estimate is taken as at the floor without the confirming pass
-// (the estimate can then only decrease).  One thread, active passes only.
+// After a pass with error e: count it and continue while e > tol and the
+// limit allows (__main__.py:161-191).  Mixed form: a pass that did not
+// bring the error below `ratio` times the error two passes back, at an error
+// of at most `cap`, has reached the resolution of its FP32 evaluations, and
+// so has a step whose `primary` passes in the context's precision all ran
+// without converging; the step's remaining passes then run in FP64, so it
+// still ends at error <= tol.  One thread, active passes only.
 __device__ inline void ctl_after_iteration(StepCtl* ctl, const CtlRule& r, double e) {
   ctl->c += 1;
-  if (r.ratio > 0 && e > r.tol && e <= r.cap && e > r.ratio * ctl->prev) {
-    ctl->floor = 1;
-    ctl->active = 0;
-    ctl->floor_est = e;
-  } else if (r.ratio > 0 && r.margin > 0 && ctl->floor_est > 0 && e > r.tol && e <= r.cap &&
-             e <= r.margin * ctl->floor_est) {
-    ctl->floor = 1;
-    ctl->active = 0;
-    ctl->floor_est = fmin(ctl->floor_est, e);
-  } else {
-    ctl->active = (e > r.tol) && (ctl->c < r.max_iter);
+  const bool more = (e > r.tol) && (ctl->c < r.max_iter);
+  if (more && !ctl->esc && r.ratio > 0 &&
+      ((e <= r.cap && e > r.ratio * ctl->prev2) || ctl->c >= r.primary)) {
+    ctl->esc = 1;
+    ctl->escalated += 1;
   }
+  ctl->active = more;
+  ctl->prev2 = ctl->prev;
   ctl->prev = e;
+  ctl->run_p = more && !ctl->esc;
+  ctl->run_e = more && ctl->esc;
 }
 
 }  // namespace dnsmi

@@ -795,7 +795,7 @@ int mlp_features(Ctx* c, const double* nm2, const double* nm1, const double* ncu
   return check_launch(c, "mlp_features");
 }
 
-int periodic_get_rhs(Ctx* c, const double* state, double* rhs, hipStream_t s);
+int periodic_ref_rhs(Ctx* c, const double* state, double* rhs, hipStream_t s);
 
 int guess_periodic(Ctx* c, const double* state, double* pred, double* rhs_prev, const float* h1,
                    float* h2, const uint16_t* wb, const float* bias, int H, float s_1, float s_2,
@@ -804,7 +804,8 @@ int guess_periodic(Ctx* c, const double* state, double* pred, double* rhs_prev, 
   if (mode < 0 || mode > 3) return c->fail(DNSMI_EINVAL, "guess: mode must be 0, 1, 2 or 3");
   if (mode == 2 && H != 32 && H != 64)
     return c->fail(DNSMI_EINVAL, "guess: hidden width must be 32 or 64");
-  int rc = periodic_get_rhs(c, state, c->rhs0, s);
+  // N^n is the reference of the step's split evaluations (mixed form)
+  int rc = periodic_ref_rhs(c, state, c->rhs0, s);
   if (rc) return rc;
   const FeatArgs a = feat_args(c, s_1, s_2, s_out);
   if (a.n >= (1LL << 31)) return c->fail(DNSMI_EINVAL, "guess: too many modes for 32-bit indexing");

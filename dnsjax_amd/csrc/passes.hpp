@@ -9,9 +9,10 @@
 namespace dnsmi {
 
 // x pass (c2r, nonlinear term, r2c) over rows (y, z): periodic.hip
+// em: EvalMode (EV_RECORD / EV_INC need T = float and the reference buffers)
 template <typename T>
 int launch_x_pass(Ctx* c, const cx<T>* xin, cx<T>* xout, const double* base, const double* cbase,
-                  const double* nbase, int ny, int kid, hipStream_t s);
+                  const double* nbase, int ny, int kid, hipStream_t s, int em);
 // z forward over columns (y, kx): periodic.hip
 template <typename T>
 int launch_z_forward_buf(Ctx* c, const cx<T>* in, cx<T>* out, int ny, int kid, hipStream_t s);

@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdio>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "context.hpp"
@@ -39,6 +40,7 @@ struct StepArgs {
   double* rhs_out;         // N of this evaluation
   double* partials;        // MODE_CORRECT: per-block sum k_metric |delta|^2
   const int* act;          // if set and 0: the whole launch is skipped
+  const double* nref;      // EV_INC: N(u_ref), added to the projected increment
   double dt, c, re;
 };
 
@@ -49,11 +51,14 @@ __device__ __forceinline__ cx<T> to_t(cx<double> d) {
 }
 
 // ------------------------------------------------------------------ K1
-template <typename T, int P, int TC, bool PADY>
+// EM (EvalMode): EV_RECORD also copies u to ucopy (u_ref); EV_INC transforms
+// d = u - uref, formed in FP64 before the rounding to T.
+template <typename T, int P, int TC, bool PADY, int EM>
 __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
     k_y_inverse(const cx<double>* __restrict__ u, cx<T>* __restrict__ w1,
                 const cx<T>* __restrict__ tw, const double* __restrict__ ky, int ny, int Nz1,
-                int Nx1, const int* __restrict__ act) {
+                int Nx1, const int* __restrict__ act, const cx<double>* __restrict__ uref,
+                cx<double>* __restrict__ ucopy) {
   if (act && *act == 0) return;
   constexpr int N1 = ColPlan<P>::N1, N2 = ColPlan<P>::N2;
   __shared__ cx<T> lds[TC * P];
@@ -84,7 +89,13 @@ __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
         const int s = PadN<P, PADY ? 2 * P / 3 : P>::stored_in(N2 * a, N2, j);
         cx<T> v{0, 0}, vk{0, 0};
         if (ok && s >= 0) {
-          const cx<double> d = u[c * in_comp + (size_t)s * plane + off];
+          const size_t e = c * in_comp + (size_t)s * plane + off;
+          cx<double> d = u[e];
+          if constexpr (EM == EV_RECORD) ucopy[e] = d;
+          if constexpr (EM == EV_INC) {
+            const cx<double> r = uref[e];
+            d = {d.x - r.x, d.y - r.y};
+          }
           const double k = ky[s];
           v = to_t<T>(d);
           vk = {static_cast<T>(-k * d.y), static_cast<T>(k * d.x)};
@@ -201,11 +212,12 @@ __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
 // K1 / K2, sequential form: one transform in flight per pass (fewer live
 // registers, higher occupancy); an input used by two outputs is read twice,
 // the second time from L2.  Same outputs as the paired forms above.
-template <typename T, int P, int TC, bool PADY>
+template <typename T, int P, int TC, bool PADY, int EM>
 __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
     k_y_inverse_seq(const cx<double>* __restrict__ u, cx<T>* __restrict__ w1,
                     const cx<T>* __restrict__ tw, const double* __restrict__ ky, int ny, int Nz1,
-                    int Nx1, const int* __restrict__ act) {
+                    int Nx1, const int* __restrict__ act, const cx<double>* __restrict__ uref,
+                    cx<double>* __restrict__ ucopy) {
   if (act && *act == 0) return;
   constexpr int N1 = ColPlan<P>::N1, N2 = ColPlan<P>::N2;
   __shared__ cx<T> lds[TC * P];
@@ -235,7 +247,15 @@ __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
         const int s = PadN<P, PADY ? 2 * P / 3 : P>::stored_in(N2 * a, N2, j);
         cx<T> v{0, 0};
         if (ok && s >= 0) {
-          const cx<double> d = u[c * in_comp + (size_t)s * plane + off];
+          const size_t e = c * in_comp + (size_t)s * plane + off;
+          cx<double> d = u[e];
+          if constexpr (EM == EV_RECORD) {
+            if (!iky) ucopy[e] = d;  // each component is read by one such pass
+          }
+          if constexpr (EM == EV_INC) {
+            const cx<double> r = uref[e];
+            d = {d.x - r.x, d.y - r.y};
+          }
           if (iky) {
             const double k = ky[s];
             v = {static_cast<T>(-k * d.y), static_cast<T>(k * d.x)};
@@ -351,13 +371,22 @@ struct XWaves {
 // after the (u_z, w_y) and (u_y, w_z) transforms, component 0 is complete
 // and is transformed and stored; the (u_x, w_x) transform then completes
 // components 1 and 2.  At most four physical fields are held at once.
-template <typename T, int P, int R>
+//
+// EM (EvalMode, FP32 only): EV_RECORD also stores each row's total velocity
+// and vorticity U + u, W + w (the reference's physical fields) to xref,
+// lane-major: field f, row r, lane j, point k2 at ((f rows + r) P + j N2 + k2).
+// EV_INC takes the transformed fields as the increment d, w_d over u_ref
+// (no base flow added) and forms dN = U x w_d + d x (W + w_d) with U, W
+// read back from xref: every FP32 product is then of increment size.
+template <typename T, int P, int R, int EM>
 __global__ void __launch_bounds__(R * Plan<P>::N1, (XWaves<T, P>::value))
     k_x_nonlinear(const cx<T>* __restrict__ w2, cx<T>* __restrict__ w3,
                   const cx<T>* __restrict__ tw, const double* __restrict__ kx_tab,
                   const double* __restrict__ base, const double* __restrict__ cbase,
                   const double* __restrict__ /* nbase: U x curl U, implied */, int nx, int Py,
-                  int Pz, int ybase_stride, const int* __restrict__ act) {
+                  int Pz, int ybase_stride, const int* __restrict__ act,
+                  float* __restrict__ xref) {
+  static_assert(EM == EV_PLAIN || sizeof(T) == 4, "split evaluations are FP32 only");
   if (act && *act == 0) return;
   constexpr int N1 = Plan<P>::N1, N2 = Plan<P>::N2;
   constexpr int ROW = XRow<N1, N2>::value;
@@ -454,16 +483,35 @@ __global__ void __launch_bounds__(R * Plan<P>::N1, (XWaves<T, P>::value))
     cx<T> o[N2];
     row_step_b_load<P, N1, N2, +1>(o, j, rl);
     // total velocity and vorticity: u + U and w + curl U, component
-    // (2, 1), (1, 2), (0, 0) for pairs 0, 1, 2
-    const int yr = opaque(yrow);
+    // (2, 1), (1, 2), (0, 0) for pairs 0, 1, 2 (increments: d and w_d alone)
     const int cu = pair == 0 ? 2 : (pair == 1 ? 1 : 0);
     const int cw = pair == 0 ? 1 : (pair == 1 ? 2 : 0);
-    const T Uc = static_cast<T>(base[cu * ybase_stride + yr]);
-    const T Cc = static_cast<T>(cbase[cw * ybase_stride + yr]);
+    if constexpr (EM == EV_INC) {
 #pragma unroll
-    for (int k2 = 0; k2 < N2; ++k2) {
-      re[k2] = o[k2].x + Uc;
-      im[k2] = o[k2].y + Cc;
+      for (int k2 = 0; k2 < N2; ++k2) {
+        re[k2] = o[k2].x;
+        im[k2] = o[k2].y;
+      }
+    } else {
+      const int yr = opaque(yrow);
+      const T Uc = static_cast<T>(base[cu * ybase_stride + yr]);
+      const T Cc = static_cast<T>(cbase[cw * ybase_stride + yr]);
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) {
+        re[k2] = o[k2].x + Uc;
+        im[k2] = o[k2].y + Cc;
+      }
+    }
+    if constexpr (EM == EV_RECORD) {
+      if (ok) {
+        const size_t fs = (size_t)nrows * P;
+        float* xr = xref + (size_t)row * P + opaque(j_) * N2;
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) {
+          xr[cu * fs + k2] = re[k2];
+          xr[(3 + cw) * fs + k2] = im[k2];
+        }
+      }
     }
     row_sync<kWaveRows>();
   };
@@ -506,11 +554,24 @@ __global__ void __launch_bounds__(R * Plan<P>::N1, (XWaves<T, P>::value))
   stage_pair(1, q);
   load_pair(2, q);
   inverse_pair(1, uy, wz);
+  // EV_INC: the reference fields of this lane's points (rows past the end
+  // read row 0's, like the inputs)
+  const size_t xfs = (size_t)nrows * P;
+  const float* xr = EM == EV_INC ? xref + (size_t)(ok ? row : 0) * P + j_ * N2 : nullptr;
   {
     T nl0[N2];
+    if constexpr (EM == EV_INC) {
 #pragma unroll
-    for (int k2 = 0; k2 < N2; ++k2) {
-      nl0[k2] = uy[k2] * wz[k2] - uz[k2] * wy[k2];
+      for (int k2 = 0; k2 < N2; ++k2) {
+        const T Uy = xr[1 * xfs + k2], Uz = xr[2 * xfs + k2];
+        const T Wy = xr[4 * xfs + k2], Wz = xr[5 * xfs + k2];
+        nl0[k2] = (Uy * wz[k2] - Uz * wy[k2]) + (uy[k2] * (Wz + wz[k2]) - uz[k2] * (Wy + wy[k2]));
+      }
+    } else {
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) {
+        nl0[k2] = uy[k2] * wz[k2] - uz[k2] * wy[k2];
+      }
     }
     forward_store(0, nl0);
   }
@@ -525,10 +586,20 @@ __global__ void __launch_bounds__(R * Plan<P>::N1, (XWaves<T, P>::value))
     {
       T ux[N2], wx[N2];
       inverse_pair(2, ux, wx);
+      if constexpr (EM == EV_INC) {
 #pragma unroll
-      for (int k2 = 0; k2 < N2; ++k2) {
-        v[k2].x = uz[k2] * wx[k2] - ux[k2] * wz[k2];
-        v[k2].y = ux[k2] * wy[k2] - uy[k2] * wx[k2];
+        for (int k2 = 0; k2 < N2; ++k2) {
+          const T Ux = xr[0 * xfs + k2], Uy = xr[1 * xfs + k2], Uz = xr[2 * xfs + k2];
+          const T Wx = xr[3 * xfs + k2], Wy = xr[4 * xfs + k2], Wz = xr[5 * xfs + k2];
+          v[k2].x = (Uz * wx[k2] - Ux * wz[k2]) + (uz[k2] * (Wx + wx[k2]) - ux[k2] * (Wz + wz[k2]));
+          v[k2].y = (Ux * wy[k2] - Uy * wx[k2]) + (ux[k2] * (Wy + wy[k2]) - uy[k2] * (Wx + wx[k2]));
+        }
+      } else {
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) {
+          v[k2].x = uz[k2] * wx[k2] - ux[k2] * wz[k2];
+          v[k2].y = ux[k2] * wy[k2] - uy[k2] * wx[k2];
+        }
       }
     }
     const int j = opaque(j_);
@@ -830,9 +901,13 @@ __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
       const size_t pos = (size_t)s * plane + off;
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        // rhs = N - i k p
-        const cx<double> rh = {nn[c].x + kk[c] * pr.y, nn[c].y - kk[c] * pr.x};
+        // rhs = N - i k p (increments: N(u_ref) + P[dN])
+        cx<double> rh = {nn[c].x + kk[c] * pr.y, nn[c].y - kk[c] * pr.x};
         const size_t idx = c * comp + pos;
+        if (args.nref) {
+          const cx<double> r0 = reinterpret_cast<const cx<double>*>(args.nref)[idx];
+          rh = {r0.x + rh.x, r0.y + rh.y};
+        }
         cx<double>* rout = reinterpret_cast<cx<double>*>(args.rhs_out);
         if (MODE == MODE_RHS) {
           rout[idx] = rh;
@@ -990,34 +1065,67 @@ static Bufs<T> bufs_of(Ctx* c) {
           static_cast<const cx<T>*>(tw[2])};
 }
 
-constexpr int kTC = 16;  // columns per workgroup of the y/z passes
+// Columns per workgroup of the y/z passes: 16 (the default) or 8, chosen
+// per context by the autotuner (c->tile_cols); kTCMin sizes the buffers
+// that scale with the number of column tiles.
+constexpr int kTC = 16, kTCMin = 8;
+
+// Calls f(std::integral_constant<int, TC>) for the context's column tile.
+template <typename F>
+static void with_tc(const Ctx* c, F&& f) {
+  if (c->tile_cols == 8)
+    f(std::integral_constant<int, 8>{});
+  else
+    f(std::integral_constant<int, kTC>{});
+}
 constexpr int kRows = 16;  // rows per workgroup of the x pass
 
+template <typename T, int P, int EM, int TC>
+static void launch_y_inverse_p(Ctx* c, dim3 grid, const cx<double>* u, cx<T>* w1, const cx<T>* tw,
+                               hipStream_t s) {
+  const dim3 blk(TC * ColPlan<P>::threads);
+  const bool pady = c->Py != c->ny;  // y padded by the 3/2 rule
+  const cx<double>* uref = reinterpret_cast<const cx<double>*>(c->uref);
+  cx<double>* ucopy = reinterpret_cast<cx<double>*>(c->uref);
+  if (c->yinv_seq && pady)
+    hipLaunchKernelGGL((k_y_inverse_seq<T, P, TC, true, EM>), grid, blk, 0, s, u, w1, tw, c->ky,
+                       c->ny, c->Nz1, c->Nx1, c->act, uref, ucopy);
+  else if (c->yinv_seq)
+    hipLaunchKernelGGL((k_y_inverse_seq<T, P, TC, false, EM>), grid, blk, 0, s, u, w1, tw, c->ky,
+                       c->ny, c->Nz1, c->Nx1, c->act, uref, ucopy);
+  else if (pady)
+    hipLaunchKernelGGL((k_y_inverse<T, P, TC, true, EM>), grid, blk, 0, s, u, w1, tw, c->ky, c->ny,
+                       c->Nz1, c->Nx1, c->act, uref, ucopy);
+  else
+    hipLaunchKernelGGL((k_y_inverse<T, P, TC, false, EM>), grid, blk, 0, s, u, w1, tw, c->ky,
+                       c->ny, c->Nz1, c->Nx1, c->act, uref, ucopy);
+}
+
 template <typename T>
-static int launch_y_inverse(Ctx* c, const double* state, hipStream_t s) {
+static int launch_y_inverse(Ctx* c, const double* state, hipStream_t s, int em = EV_PLAIN) {
   Bufs<T> b = bufs_of<T>(c);
-  const int ntile = (c->Nx1 + kTC - 1) / kTC;
+  const int ntile = (c->Nx1 + c->tile_cols - 1) / c->tile_cols;
   // one pass per workgroup (grid.y): the passes read different inputs and
   // write different outputs, and Nz1 * ntile workgroups alone underfill the GPU
   const dim3 grid(c->Nz1 * ntile, c->yinv_seq ? 5 : 3);
   const cx<double>* u = reinterpret_cast<const cx<double>*>(state);
-  const bool pady = c->Py != c->ny;  // y padded by the 3/2 rule
+  if (em != EV_PLAIN && (!std::is_same<T, float>::value || !c->uref))
+    return c->fail(DNSMI_EINVAL, "split evaluation without FP32 reference buffers");
   ProfScope ps(c, KID_Y_INVERSE, s);
   switch (c->Py) {
-#define X(P)                                                                                    \
-  case P:                                                                                       \
-    if (c->yinv_seq && pady)                                                                    \
-      hipLaunchKernelGGL((k_y_inverse_seq<T, P, kTC, true>), grid, dim3(kTC * ColPlan<P>::threads), \
-                         0, s, u, b.A, b.twy, c->ky, c->ny, c->Nz1, c->Nx1, c->act);            \
-    else if (c->yinv_seq)                                                                       \
-      hipLaunchKernelGGL((k_y_inverse_seq<T, P, kTC, false>), grid, dim3(kTC * ColPlan<P>::threads), \
-                         0, s, u, b.A, b.twy, c->ky, c->ny, c->Nz1, c->Nx1, c->act);            \
-    else if (pady)                                                                              \
-      hipLaunchKernelGGL((k_y_inverse<T, P, kTC, true>), grid, dim3(kTC * ColPlan<P>::threads), 0, \
-                         s, u, b.A, b.twy, c->ky, c->ny, c->Nz1, c->Nx1, c->act);               \
-    else                                                                                        \
-      hipLaunchKernelGGL((k_y_inverse<T, P, kTC, false>), grid, dim3(kTC * ColPlan<P>::threads), 0, \
-                         s, u, b.A, b.twy, c->ky, c->ny, c->Nz1, c->Nx1, c->act);               \
+#define X(P)                                                                    \
+  case P:                                                                       \
+    with_tc(c, [&](auto tcv) {                                                  \
+      constexpr int TC = decltype(tcv)::value;                                  \
+      if (em == EV_PLAIN)                                                       \
+        launch_y_inverse_p<T, P, EV_PLAIN, TC>(c, grid, u, b.A, b.twy, s);      \
+      else if constexpr (std::is_same<T, float>::value) {                       \
+        if (em == EV_RECORD)                                                    \
+          launch_y_inverse_p<T, P, EV_RECORD, TC>(c, grid, u, b.A, b.twy, s);   \
+        else                                                                    \
+          launch_y_inverse_p<T, P, EV_INC, TC>(c, grid, u, b.A, b.twy, s);      \
+      }                                                                         \
+    });                                                                         \
     break;
     DNSMI_FOR_EACH_P(X)
 #undef X
@@ -1029,18 +1137,21 @@ static int launch_y_inverse(Ctx* c, const double* state, hipStream_t s) {
 template <typename T>
 static int launch_z_inverse(Ctx* c, hipStream_t s) {
   Bufs<T> b = bufs_of<T>(c);
-  const int ntile = (c->Nx1 + kTC - 1) / kTC;
+  const int ntile = (c->Nx1 + c->tile_cols - 1) / c->tile_cols;
   const dim3 grid(c->Py * ntile, c->zinv_seq ? 6 : 4);  // one pass per workgroup
   ProfScope ps(c, KID_Z_INVERSE, s);
   switch (c->Pz) {
 #define X(P)                                                                                    \
   case P:                                                                                       \
-    if (c->zinv_seq)                                                                            \
-      hipLaunchKernelGGL((k_z_inverse_seq<T, P, kTC>), grid, dim3(kTC * ColPlan<P>::threads), 0, \
-                         s, b.A, b.B, b.twz, c->kz, c->nz, c->Py, c->Nx1, c->act);                      \
-    else                                                                                        \
-      hipLaunchKernelGGL((k_z_inverse<T, P, kTC>), grid, dim3(kTC * ColPlan<P>::threads), 0, s,  \
-                         b.A, b.B, b.twz, c->kz, c->nz, c->Py, c->Nx1, c->act);                         \
+    with_tc(c, [&](auto tcv) {                                                                  \
+      constexpr int TC = decltype(tcv)::value;                                                  \
+      if (c->zinv_seq)                                                                          \
+        hipLaunchKernelGGL((k_z_inverse_seq<T, P, TC>), grid, dim3(TC * ColPlan<P>::threads), 0, \
+                           s, b.A, b.B, b.twz, c->kz, c->nz, c->Py, c->Nx1, c->act);            \
+      else                                                                                      \
+        hipLaunchKernelGGL((k_z_inverse<T, P, TC>), grid, dim3(TC * ColPlan<P>::threads), 0, s,  \
+                           b.A, b.B, b.twz, c->kz, c->nz, c->Py, c->Nx1, c->act);               \
+    });                                                                                         \
     break;
     DNSMI_FOR_EACH_P(X)
 #undef X
@@ -1059,36 +1170,47 @@ struct XSplit {
 template <typename T, int P>
 static void launch_x_len(Ctx* c, const Bufs<T>& b, const cx<T>* xin, cx<T>* xout,
                          const double* base, const double* cbase, const double* nbase, int ny,
-                         hipStream_t s) {
+                         int em, hipStream_t s) {
   constexpr int R = XSplit<P>::rows;
   const int nrows = ny * c->Pz;
   const dim3 grid((nrows + R - 1) / R);
   bool split = false;
-  if constexpr (XSplit<P>::value) split = c->xpass_split;
+  if constexpr (XSplit<P>::value) split = c->xpass_split && em == EV_PLAIN;
   if (split) {
     if constexpr (XSplit<P>::value)
       hipLaunchKernelGGL((k_x_nonlinear_split<T, P, R>), grid, dim3(R * XSplit<P>::tpr), 0, s,
                          xin, xout, b.twx, c->kx, base, cbase, nbase, c->nx, ny, c->Pz, ny, c->act);
   } else {
     constexpr int R1 = kRows;
-    const dim3 grid1((nrows + R1 - 1) / R1);
-    hipLaunchKernelGGL((k_x_nonlinear<T, P, R1>), grid1, dim3(R1 * Plan<P>::N1), 0, s, xin, xout,
-                       b.twx, c->kx, base, cbase, nbase, c->nx, ny, c->Pz, ny, c->act);
+    const dim3 grid1((nrows + R1 - 1) / R1), blk(R1 * Plan<P>::N1);
+    if (em == EV_PLAIN) {
+      hipLaunchKernelGGL((k_x_nonlinear<T, P, R1, EV_PLAIN>), grid1, blk, 0, s, xin, xout, b.twx,
+                         c->kx, base, cbase, nbase, c->nx, ny, c->Pz, ny, c->act, c->xref);
+    } else if constexpr (std::is_same<T, float>::value) {
+      if (em == EV_RECORD)
+        hipLaunchKernelGGL((k_x_nonlinear<T, P, R1, EV_RECORD>), grid1, blk, 0, s, xin, xout, b.twx,
+                           c->kx, base, cbase, nbase, c->nx, ny, c->Pz, ny, c->act, c->xref);
+      else
+        hipLaunchKernelGGL((k_x_nonlinear<T, P, R1, EV_INC>), grid1, blk, 0, s, xin, xout, b.twx,
+                           c->kx, base, cbase, nbase, c->nx, ny, c->Pz, ny, c->act, c->xref);
+    }
   }
 }
 
 // x pass over rows (y, z) of W[f][y][z][kx]: periodic (y = padded grid) and
 // wall-bounded (y = the grid points) share it; the base flow profiles are
-// given per y.
+// given per y.  em: EvalMode (FP32 contexts with reference buffers only).
 template <typename T>
 int launch_x_pass(Ctx* c, const cx<T>* xin, cx<T>* xout, const double* base, const double* cbase,
-                  const double* nbase, int ny, int kid, hipStream_t s) {
+                  const double* nbase, int ny, int kid, hipStream_t s, int em) {
+  if (em != EV_PLAIN && (!std::is_same<T, float>::value || !c->xref))
+    return c->fail(DNSMI_EINVAL, "split evaluation without FP32 reference buffers");
   Bufs<T> b = bufs_of<T>(c);
   ProfScope ps(c, kid, s);
   switch (c->Px) {
 #define X(P)                                                                \
   case P:                                                                   \
-    launch_x_len<T, P>(c, b, xin, xout, base, cbase, nbase, ny, s);        \
+    launch_x_len<T, P>(c, b, xin, xout, base, cbase, nbase, ny, em, s);    \
     break;
     DNSMI_FOR_EACH_P(X)
 #undef X
@@ -1097,14 +1219,14 @@ int launch_x_pass(Ctx* c, const cx<T>* xin, cx<T>* xout, const double* base, con
   return check_launch(c, "x_nonlinear");
 }
 template int launch_x_pass<double>(Ctx*, const cx<double>*, cx<double>*, const double*,
-                                   const double*, const double*, int, int, hipStream_t);
+                                   const double*, const double*, int, int, hipStream_t, int);
 template int launch_x_pass<float>(Ctx*, const cx<float>*, cx<float>*, const double*,
-                                  const double*, const double*, int, int, hipStream_t);
+                                  const double*, const double*, int, int, hipStream_t, int);
 
 template <typename T>
-static int launch_x_nonlinear(Ctx* c, hipStream_t s) {
+static int launch_x_nonlinear(Ctx* c, hipStream_t s, int em = EV_PLAIN) {
   Bufs<T> b = bufs_of<T>(c);
-  return launch_x_pass<T>(c, b.B, b.A, c->base, c->cbase, c->nbase, c->Py, KID_X_NONLINEAR, s);
+  return launch_x_pass<T>(c, b.B, b.A, c->base, c->cbase, c->nbase, c->Py, KID_X_NONLINEAR, s, em);
 }
 
 // z forward over columns (y, kx) of W[c][y][z][kx] -> W[c][y][kz][kx]; ny is
@@ -1112,14 +1234,17 @@ static int launch_x_nonlinear(Ctx* c, hipStream_t s) {
 template <typename T>
 int launch_z_forward_buf(Ctx* c, const cx<T>* in, cx<T>* out, int ny, int kid, hipStream_t s) {
   Bufs<T> b = bufs_of<T>(c);
-  const int ntile = (c->Nx1 + kTC - 1) / kTC;
+  const int ntile = (c->Nx1 + c->tile_cols - 1) / c->tile_cols;
   const dim3 grid(ny * ntile, 3);  // one component per workgroup
   ProfScope ps(c, kid, s);
   switch (c->Pz) {
 #define X(P)                                                                                   \
   case P:                                                                                      \
-    hipLaunchKernelGGL((k_z_forward<T, P, kTC>), grid, dim3(kTC * ColPlan<P>::threads), 0, s,  \
-                       in, out, b.twz, c->nz, ny, c->Nx1, c->act);                                     \
+    with_tc(c, [&](auto tcv) {                                                                 \
+      constexpr int TC = decltype(tcv)::value;                                                 \
+      hipLaunchKernelGGL((k_z_forward<T, P, TC>), grid, dim3(TC * ColPlan<P>::threads), 0, s,  \
+                         in, out, b.twz, c->nz, ny, c->Nx1, c->act);                           \
+    });                                                                                        \
     break;
     DNSMI_FOR_EACH_P(X)
 #undef X
@@ -1141,21 +1266,25 @@ static int launch_z_forward(Ctx* c, hipStream_t s) {
 template <typename T, int MODE>
 static int launch_y_forward(Ctx* c, const StepArgs& a, hipStream_t s, int* nblocks) {
   Bufs<T> b = bufs_of<T>(c);
-  const int ntile = (c->Nx1 + kTC - 1) / kTC;
+  const int ntile = (c->Nx1 + c->tile_cols - 1) / c->tile_cols;
   const int nb = c->Nz1 * ntile;
   if (nblocks) *nblocks = nb;
+  if (MODE == MODE_CORRECT && nb > c->max_partials) return c->fail(DNSMI_EINVAL, "y_forward: too many blocks");
   ProfScope ps(c, KID_Y_FORWARD, s);
   switch (c->Py) {
 #define X(P)                                                                                   \
   case P:                                                                                      \
-    if (c->Py != c->ny)                                                                        \
-      hipLaunchKernelGGL((k_y_forward<T, P, kTC, MODE, true>), dim3(nb),                      \
-                         dim3(kTC * ColPlan<P>::threads), 0, s, b.B, b.twy, c->kx, c->ky, c->kz, \
-                         c->ny, c->Nz1, c->Nx1, a);                                            \
-    else                                                                                       \
-      hipLaunchKernelGGL((k_y_forward<T, P, kTC, MODE, false>), dim3(nb),                     \
-                         dim3(kTC * ColPlan<P>::threads), 0, s, b.B, b.twy, c->kx, c->ky, c->kz, \
-                         c->ny, c->Nz1, c->Nx1, a);                                            \
+    with_tc(c, [&](auto tcv) {                                                                 \
+      constexpr int TC = decltype(tcv)::value;                                                 \
+      if (c->Py != c->ny)                                                                      \
+        hipLaunchKernelGGL((k_y_forward<T, P, TC, MODE, true>), dim3(nb),                     \
+                           dim3(TC * ColPlan<P>::threads), 0, s, b.B, b.twy, c->kx, c->ky,     \
+                           c->kz, c->ny, c->Nz1, c->Nx1, a);                                   \
+      else                                                                                     \
+        hipLaunchKernelGGL((k_y_forward<T, P, TC, MODE, false>), dim3(nb),                    \
+                           dim3(TC * ColPlan<P>::threads), 0, s, b.B, b.twy, c->kx, c->ky,     \
+                           c->kz, c->ny, c->Nz1, c->Nx1, a);                                   \
+    });                                                                                        \
     break;
     DNSMI_FOR_EACH_P(X)
 #undef X
@@ -1166,20 +1295,27 @@ static int launch_y_forward(Ctx* c, const StepArgs& a, hipStream_t s, int* nbloc
 
 template <typename T, int MODE>
 static int periodic_rhs_t(Ctx* c, const double* state, const StepArgs& a, hipStream_t s,
-                          int* nblocks) {
+                          int* nblocks, int em) {
   int rc;
-  if ((rc = launch_y_inverse<T>(c, state, s))) return rc;
+  if ((rc = launch_y_inverse<T>(c, state, s, em))) return rc;
   if ((rc = launch_z_inverse<T>(c, s))) return rc;
-  if ((rc = launch_x_nonlinear<T>(c, s))) return rc;
+  if ((rc = launch_x_nonlinear<T>(c, s, em))) return rc;
   if ((rc = launch_z_forward<T>(c, s))) return rc;
   return launch_y_forward<T, MODE>(c, a, s, nblocks);
 }
 
+// One evaluation of N(state) in the context's current precision.  em: see
+// EvalMode; EV_INC needs a recorded reference (falls back to EV_PLAIN
+// otherwise, and in FP64 passes) and adds n_ref through a.nref.
 template <int MODE>
-static int periodic_rhs_mode(Ctx* c, const double* state, const StepArgs& a, hipStream_t s,
-                             int* nblocks) {
-  return c->fp32 ? periodic_rhs_t<float, MODE>(c, state, a, s, nblocks)
-                 : periodic_rhs_t<double, MODE>(c, state, a, s, nblocks);
+static int periodic_rhs_mode(Ctx* c, const double* state, StepArgs a, hipStream_t s,
+                             int* nblocks, int em = EV_PLAIN) {
+  if (!c->fp32 || !c->split || (em == EV_INC && !c->ref_valid)) em = EV_PLAIN;
+  a.nref = em == EV_INC ? c->nref : nullptr;
+  if (!c->fp32) return periodic_rhs_t<double, MODE>(c, state, a, s, nblocks, EV_PLAIN);
+  const int rc = periodic_rhs_t<float, MODE>(c, state, a, s, nblocks, em);
+  if (rc == DNSMI_OK && em == EV_RECORD) c->ref_valid = true;
+  return rc;
 }
 
 static StepArgs step_args(Ctx* c) {
@@ -1198,20 +1334,30 @@ int periodic_get_rhs(Ctx* c, const double* state, double* rhs, hipStream_t s) {
   return periodic_rhs_mode<MODE_RHS>(c, state, a, s, nullptr);
 }
 
+// N(state) into rhs, recorded as the reference of the split evaluations
+// that follow (the first evaluation of a guessed step).
+int periodic_ref_rhs(Ctx* c, const double* state, double* rhs, hipStream_t s) {
+  StepArgs a = step_args(c);
+  a.rhs_out = rhs;
+  return periodic_rhs_mode<MODE_RHS>(c, state, a, s, nullptr, EV_RECORD);
+}
+
+// The predictor's evaluation N(u^n) (into rhs0 = n_ref) is the step's
+// reference; the corrector's evaluations are increments over it.
 int periodic_predict_and_correct(Ctx* c, const double* state, double* pred, double* rhs_next,
                                  double* err, hipStream_t s) {
   StepArgs a = step_args(c);
   a.state = state;
   a.pred = pred;
   a.rhs_out = c->rhs0;
-  int rc = periodic_rhs_mode<MODE_PREDICT>(c, state, a, s, nullptr);
+  int rc = periodic_rhs_mode<MODE_PREDICT>(c, state, a, s, nullptr, EV_RECORD);
   if (rc) return rc;
   StepArgs b = step_args(c);
   b.rhs_prev = c->rhs0;
   b.pred = pred;
   b.rhs_out = rhs_next;
   int nb = 0;
-  rc = periodic_rhs_mode<MODE_CORRECT>(c, pred, b, s, &nb);
+  rc = periodic_rhs_mode<MODE_CORRECT>(c, pred, b, s, &nb, EV_INC);
   if (rc) return rc;
   return finalize_norm(c, nb, err, s);
 }
@@ -1223,7 +1369,7 @@ int periodic_iterate_correction(Ctx* c, double* pred, const double* rhs_prev, do
   b.pred = pred;
   b.rhs_out = rhs_next;
   int nb = 0;
-  int rc = periodic_rhs_mode<MODE_CORRECT>(c, pred, b, s, &nb);
+  int rc = periodic_rhs_mode<MODE_CORRECT>(c, pred, b, s, &nb, EV_INC);
   if (rc) return rc;
   return finalize_norm(c, nb, err, s);
 }
@@ -1417,6 +1563,24 @@ static int periodic_autotune_t(Ctx* c) {
     default: break;
   }
   if (split_ok) pick("x_split", c->xpass_split, [&] { launch_x_nonlinear<T>(c, nullptr); });
+  // the column tile of the y and z passes: one whole RHS with 16 and with 8
+  {
+    double* out = static_cast<double*>(c->alloc(c->spec_elems * 16));
+    if (!out) return c->fail(DNSMI_ENOMEM, "autotune: device allocation failed");
+    StepArgs a = step_args(c);
+    a.rhs_out = out;
+    auto rhs = [&] { periodic_rhs_t<T, MODE_RHS>(c, state, a, nullptr, nullptr, EV_PLAIN); };
+    c->tile_cols = kTC;
+    const double t16 = time_launches(rhs, reps);
+    c->tile_cols = kTCMin;
+    const double t8 = time_launches(rhs, reps);
+    c->tile_cols = t8 < t16 ? kTCMin : kTC;
+    c->release(out);
+    char buf[160];
+    snprintf(buf, sizeof buf, "tile_cols=%d(16: %.1f / 8: %.1f us) ", c->tile_cols, 1e3 * t16 / reps,
+             1e3 * t8 / reps);
+    log += buf;
+  }
   c->tuning = log;
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? DNSMI_OK : c->fail(DNSMI_EHIP, std::string("autotune: ") + hipGetErrorString(e));
@@ -1447,7 +1611,8 @@ int periodic_setup(Ctx* c) {
   if (!supported_length(c->Px) || !supported_length(c->Py) || !supported_length(c->Pz))
     return c->fail(DNSMI_EINVAL, "padded lengths must be one of 12,16,24,32,48,64,72,96,128,144,192");
   c->spec_elems = 3LL * c->Ny1 * c->Nz1 * c->Nx1;
-  const size_t cxs = c->fp32 ? 8 : 16;
+  // FP64-sized in the mixed form too: its escalated passes run in FP64
+  const size_t cxs = 16;
   const size_t w1 = 5ull * c->Py * c->Nz1 * c->Nx1, w3 = 3ull * c->Py * c->Pz * c->Nx1;
   const size_t w2 = 6ull * c->Py * c->Pz * c->Nx1, w4 = 3ull * c->Py * c->Nz1 * c->Nx1;
   c->bufA_bytes = (w1 > w3 ? w1 : w3) * cxs;
@@ -1455,11 +1620,18 @@ int periodic_setup(Ctx* c) {
   c->bufA = c->alloc(c->bufA_bytes);
   c->bufB = c->alloc(c->bufB_bytes);
   c->rhs0 = static_cast<double*>(c->alloc(c->spec_elems * 16));
+  if (c->split) {
+    // n_ref is rhs0: the predictor's (or the guess's) evaluation writes it
+    c->nref = c->rhs0;
+    c->uref = static_cast<double*>(c->alloc(c->spec_elems * 16));
+    c->xref = static_cast<float*>(c->alloc(6ull * c->Py * c->Pz * c->Px * sizeof(float)));
+    if (!c->uref || !c->xref) return c->fail(DNSMI_ENOMEM, "device allocation failed");
+  }
   const long long per_comp = (long long)c->Ny1 * c->Nz1 * c->Nx1;
   // norm partials (one per y-forward block) and the statistics' two per
   // 256-mode block, whichever is larger
   const long long stats_partials = 2 * ((per_comp + 255) / 256);
-  const long long norm_partials = (per_comp + 255) / 256 + (long long)c->Nz1 * ((c->Nx1 + kTC - 1) / kTC);
+  const long long norm_partials = (per_comp + 255) / 256 + (long long)c->Nz1 * ((c->Nx1 + kTCMin - 1) / kTCMin);
   c->max_partials = (int)((stats_partials > norm_partials ? stats_partials : norm_partials) + 1024);
   c->partials = static_cast<double*>(c->alloc(sizeof(double) * c->max_partials));
   if (!c->bufA || !c->bufB || !c->rhs0 || !c->partials)

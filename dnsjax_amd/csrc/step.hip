@@ -8,9 +8,11 @@
 // iteration is allowed; each of `slots` corrector slots runs only if
 // `active` (every kernel of the slot reads it and returns at once when 0),
 // and the pass's final norm reduction then updates the count and `active`
-// exactly as the host rule would (ctl_after_iteration, including the
-// optional noise-floor stops of the mixed form).  The step ends with
-// correct_velocity and records the step's iteration count.
+// exactly as the host rule would (ctl_after_iteration).  In the mixed form,
+// optional FP64 slots follow, run only by a step whose FP32 passes stopped
+// converging (the escalation of ctl_after_iteration), so every step still
+// ends at error <= tol.  The step ends with correct_velocity and records the
+// step's iteration count.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -36,9 +38,12 @@ __global__ void k_ctl_begin(StepCtl* ctl, double tol, int max_iter) {
   if (threadIdx.x != 0) return;
   const double e = ctl->err;
   ctl->c = 0;
-  ctl->floor = 0;
+  ctl->esc = 0;
   ctl->prev = e;
+  ctl->prev2 = INFINITY;
   ctl->active = (e > tol) && (max_iter > 0);
+  ctl->run_p = ctl->active;
+  ctl->run_e = 0;
 }
 
 __global__ void k_ctl_end(StepCtl* ctl, int* counts, int counts_cap, double tol) {
@@ -46,26 +51,31 @@ __global__ void k_ctl_end(StepCtl* ctl, int* counts, int counts_cap, double tol)
   const double e = ctl->err;
   if (counts && ctl->steps < counts_cap) counts[ctl->steps] = ctl->c;
   ctl->total_c += ctl->c;
-  if (!ctl->floor && e > tol) ctl->unconverged += 1;
+  if (!(e <= tol)) ctl->unconverged += 1;
   if (e != e) ctl->nan += 1;
   ctl->steps += 1;
 }
 
-int step_device(Ctx* c, const double* state, double* pred, double* rhs_a, double* rhs_b, int slots,
-                int* counts, int counts_cap, double tol, int max_iter, double floor_ratio,
-                double floor_cap, double floor_margin, hipStream_t s) {
-  if (!c->ctl) return c->fail(DNSMI_EINVAL, "control block missing");
+// Before the FP64 escalation slots: the passes alternate between the two
+// RHS arrays (pass i reads rhs_a when i is odd), so after an odd number of
+// passes the latest evaluation is in rhs_b; it is copied to rhs_a, where the
+// escalation slots start.  Skipped unless the step escalated.
+__global__ void __launch_bounds__(256) k_ctl_settle(const StepCtl* ctl, double* __restrict__ rhs_a,
+                                                    const double* __restrict__ rhs_b, long long n) {
+  if (!ctl->run_e || (ctl->c & 1) == 0) return;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) rhs_a[i] = rhs_b[i];
+}
+
+// The corrector slots of a device step: `slots` passes in the context's
+// precision, then (mixed form) `esc_slots` FP64 passes that run only once
+// the step has escalated.
+static int step_slots(Ctx* c, const double* state, double* pred, double* rhs_a, double* rhs_b,
+                      int slots, int esc_slots, hipStream_t s) {
   StepCtl* ctl = reinterpret_cast<StepCtl*>(c->ctl);
   double* err = &ctl->err;
-  if (slots > max_iter) slots = max_iter;
-  c->act = nullptr;
-  int rc = c->walls ? walls_predict_and_correct(c, state, pred, rhs_a, err, s)
-                    : periodic_predict_and_correct(c, state, pred, rhs_a, err, s);
-  if (rc) return rc;
-  hipLaunchKernelGGL(k_ctl_begin, dim3(1), dim3(64), 0, s, ctl, tol, max_iter);
-  if ((rc = check_launch(c, "ctl_begin"))) return rc;
-  c->act = &ctl->active;
-  c->rule = CtlRule{tol, max_iter, floor_ratio, floor_cap, floor_margin};
+  int rc = DNSMI_OK;
+  c->act = &ctl->run_p;
   c->fuse_ctl = true;
   for (int i = 1; i <= slots && rc == DNSMI_OK; ++i) {
     const double* rp = (i % 2) ? rhs_a : rhs_b;
@@ -73,9 +83,46 @@ int step_device(Ctx* c, const double* state, double* pred, double* rhs_a, double
     rc = c->walls ? walls_iterate_correction(c, state, pred, rp, rn, err, s)
                   : periodic_iterate_correction(c, pred, rp, rn, err, s);
   }
+  if (rc == DNSMI_OK && esc_slots > 0) {
+    const long long n = 2 * c->spec_elems;
+    hipLaunchKernelGGL(k_ctl_settle, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ctl, rhs_a,
+                       rhs_b, n);
+    rc = check_launch(c, "ctl_settle");
+    c->act = &ctl->run_e;
+    c->fp32 = false;
+    for (int i = 1; i <= esc_slots && rc == DNSMI_OK; ++i) {
+      const double* rp = (i % 2) ? rhs_a : rhs_b;
+      double* rn = (i % 2) ? rhs_b : rhs_a;
+      rc = c->walls ? walls_iterate_correction(c, state, pred, rp, rn, err, s)
+                    : periodic_iterate_correction(c, pred, rp, rn, err, s);
+    }
+    c->fp32 = c->fp32_primary;
+  }
   c->fuse_ctl = false;
   c->act = nullptr;
+  return rc;
+}
+
+int step_device(Ctx* c, const double* state, double* pred, double* rhs_a, double* rhs_b, int slots,
+                int esc_slots, int* counts, int counts_cap, double tol, int max_iter,
+                double esc_ratio, double esc_cap, hipStream_t s) {
+  if (!c->ctl) return c->fail(DNSMI_EINVAL, "control block missing");
+  if (esc_slots > 0 && !c->fp32_primary)
+    return c->fail(DNSMI_EINVAL, "escalation slots need the mixed form");
+  StepCtl* ctl = reinterpret_cast<StepCtl*>(c->ctl);
+  double* err = &ctl->err;
+  if (slots > max_iter) slots = max_iter;
+  if (esc_slots > max_iter) esc_slots = max_iter;
+  c->act = nullptr;
+  c->fp32 = c->fp32_primary;
+  int rc = c->walls ? walls_predict_and_correct(c, state, pred, rhs_a, err, s)
+                    : periodic_predict_and_correct(c, state, pred, rhs_a, err, s);
   if (rc) return rc;
+  hipLaunchKernelGGL(k_ctl_begin, dim3(1), dim3(64), 0, s, ctl, tol, max_iter);
+  if ((rc = check_launch(c, "ctl_begin"))) return rc;
+  c->rule = CtlRule{tol, max_iter, esc_slots > 0 ? esc_ratio : 0.0, esc_cap,
+                    esc_slots > 0 ? slots : max_iter};
+  if ((rc = step_slots(c, state, pred, rhs_a, rhs_b, slots, esc_slots, s))) return rc;
   if (!c->walls && (rc = periodic_correct_velocity(c, pred, nullptr, s))) return rc;
   hipLaunchKernelGGL(k_ctl_end, dim3(1), dim3(64), 0, s, ctl, counts, counts_cap, tol);
   return check_launch(c, "ctl_end");
@@ -85,10 +132,13 @@ int step_device(Ctx* c, const double* state, double* pred, double* rhs_a, double
 __global__ void k_ctl_begin_guess(StepCtl* ctl, int max_pass) {
   if (threadIdx.x != 0) return;
   ctl->c = 0;
-  ctl->floor = 0;
+  ctl->esc = 0;
   ctl->err = INFINITY;
   ctl->prev = INFINITY;
+  ctl->prev2 = INFINITY;
   ctl->active = max_pass > 0;
+  ctl->run_p = ctl->active;
+  ctl->run_e = 0;
 }
 
 int guess_periodic(Ctx* c, const double* state, double* pred, double* rhs_prev, const float* h1,
@@ -98,38 +148,32 @@ int guess_walls(Ctx* c, const double* state, double* pred, double* rhs_prev, con
                 float* h2, const uint16_t* wb, const float* bias, int H, float s_1, float s_2,
                 float s_out, int mode, hipStream_t s);
 
-// A step started from the corrector guess (periodic): the guessed predictor,
-// then up to `slots` passes, each one RHS, with the same device rule as
-// step_device (max_pass passes at most; the count recorded is the number of
-// passes).
+// A step started from the corrector guess: the guessed predictor, then up
+// to `slots` passes (and the escalation slots), each one RHS, with the same
+// device rule as step_device (max_pass passes at most; the count recorded is
+// the number of passes).
 int step_device_guess(Ctx* c, const double* state, double* pred, double* rhs_a, double* rhs_b,
                       const float* h1, float* h2, const uint16_t* wb, const float* bias, int H,
-                      float s_1, float s_2, float s_out, int mode, int slots, int* counts,
-                      int counts_cap, double tol, int max_pass, double floor_ratio,
-                      double floor_cap, double floor_margin, hipStream_t s) {
+                      float s_1, float s_2, float s_out, int mode, int slots, int esc_slots,
+                      int* counts, int counts_cap, double tol, int max_pass, double esc_ratio,
+                      double esc_cap, hipStream_t s) {
   if (!c->ctl) return c->fail(DNSMI_EINVAL, "control block missing");
+  if (esc_slots > 0 && !c->fp32_primary)
+    return c->fail(DNSMI_EINVAL, "escalation slots need the mixed form");
   StepCtl* ctl = reinterpret_cast<StepCtl*>(c->ctl);
-  double* err = &ctl->err;
   if (slots > max_pass) slots = max_pass;
+  if (esc_slots > max_pass) esc_slots = max_pass;
   c->act = nullptr;
+  c->fp32 = c->fp32_primary;
   int rc = c->walls ? guess_walls(c, state, pred, rhs_a, h1, h2, wb, bias, H, s_1, s_2, s_out, mode, s)
                     : guess_periodic(c, state, pred, rhs_a, h1, h2, wb, bias, H, s_1, s_2, s_out,
                                      mode, s);
   if (rc) return rc;
   hipLaunchKernelGGL(k_ctl_begin_guess, dim3(1), dim3(64), 0, s, ctl, max_pass);
   if ((rc = check_launch(c, "ctl_begin_guess"))) return rc;
-  c->act = &ctl->active;
-  c->rule = CtlRule{tol, max_pass, floor_ratio, floor_cap, floor_margin};
-  c->fuse_ctl = true;
-  for (int i = 1; i <= slots && rc == DNSMI_OK; ++i) {
-    const double* rp = (i % 2) ? rhs_a : rhs_b;
-    double* rn = (i % 2) ? rhs_b : rhs_a;
-    rc = c->walls ? walls_iterate_correction(c, state, pred, rp, rn, err, s)
-                  : periodic_iterate_correction(c, pred, rp, rn, err, s);
-  }
-  c->fuse_ctl = false;
-  c->act = nullptr;
-  if (rc) return rc;
+  c->rule = CtlRule{tol, max_pass, esc_slots > 0 ? esc_ratio : 0.0, esc_cap,
+                    esc_slots > 0 ? slots : max_pass};
+  if ((rc = step_slots(c, state, pred, rhs_a, rhs_b, slots, esc_slots, s))) return rc;
   if (!c->walls && (rc = periodic_correct_velocity(c, pred, nullptr, s))) return rc;
   hipLaunchKernelGGL(k_ctl_end, dim3(1), dim3(64), 0, s, ctl, counts, counts_cap, tol);
   return check_launch(c, "ctl_end");
@@ -154,7 +198,7 @@ int step_read_ctl(Ctx* c, double* out6) {
   out6[2] = h.unconverged;
   out6[3] = h.nan;
   out6[4] = h.err;
-  out6[5] = h.floor;
+  out6[5] = h.escalated;
   return DNSMI_OK;
 }
 

@@ -383,7 +383,7 @@ int walls_setup(Ctx* c) {
 
   // buffers: WT1 out (6,Ny,Nkz,Nkx) | z-inverse out (6,Ny,Pz,Nkx) | x-pass out
   // (3,Ny,Pz,Nkx) | z-forward out (3,Ny,Nkz,Nkx)
-  const size_t cxs = c->fp32 ? 8 : 16;
+  const size_t cxs = 16;  // FP64-sized in the mixed form too: escalated passes run in FP64
   const size_t a1 = 6ull * Ny * Nkz * Nkx, a3 = 3ull * Ny * c->Pz * Nkx;
   const size_t b2 = 6ull * Ny * c->Pz * Nkx, b4 = 3ull * Ny * Nkz * Nkx;
   c->bufA_bytes = std::max(a1, a3) * cxs;
@@ -391,6 +391,14 @@ int walls_setup(Ctx* c) {
   c->bufA = c->alloc(c->bufA_bytes);
   c->bufB = c->alloc(c->bufB_bytes);
   c->rhs0 = static_cast<double*>(c->alloc(c->spec_elems * 16));
+  c->nref = c->rhs0;
+  if (c->split) {
+    // n_ref apart from rhs0, which the corrector uses as scratch
+    c->nref = static_cast<double*>(c->alloc(c->spec_elems * 16));
+    c->uref = static_cast<double*>(c->alloc(c->spec_elems * 16));
+    c->xref = static_cast<float*>(c->alloc(6ull * Ny * c->Pz * c->Px * sizeof(float)));
+    if (!c->nref || !c->uref || !c->xref) return c->fail(DNSMI_ENOMEM, "device allocation failed");
+  }
   c->max_partials = M + 1024;  // the IMM writes one norm partial per mode
   c->partials = static_cast<double*>(c->alloc(sizeof(double) * c->max_partials));
   t->scratch = static_cast<double*>(c->alloc((size_t)V_COUNT * Ny * M * 16));
@@ -649,11 +657,14 @@ void walls_release(Ctx* c) {
 // (kTY): each re-reads a halo of S1 rows on either side.
 constexpr int kTY = 16, kTM = 16, kCurlTY = 32;
 
-template <typename T, int S1>
+// EM (EvalMode): EV_RECORD also copies u to ucopy (u_ref); EV_INC takes the
+// curl of d = u - uref, formed in FP64.
+template <typename T, int S1, int EM>
 __global__ void __launch_bounds__(kCurlTY* kTM)
     k_walls_curl_t(const cx<double>* __restrict__ u, cx<T>* __restrict__ v,
                    const double* __restrict__ d1w, const int* __restrict__ d1j0,
-                   const double* __restrict__ mkz, int Ny, int M, const int* __restrict__ act) {
+                   const double* __restrict__ mkz, int Ny, int M, const int* __restrict__ act,
+                   const cx<double>* __restrict__ uref, cx<double>* __restrict__ ucopy) {
   if (act && *act == 0) return;
   constexpr int HALO = S1;
   __shared__ cx<double> tile[3][kTM][kCurlTY + 2 * HALO + 1];
@@ -670,7 +681,20 @@ __global__ void __launch_bounds__(kCurlTY* kTM)
     const int r = e - c * kTM * span;
     const int mi = r / span, yi = r - mi * span;
     const int m = m0 + mi;
-    tile[c][mi][yi] = (m < M) ? u[c * comp + (size_t)m * Ny + ylo + yi] : cx<double>{0, 0};
+    cx<double> val{0, 0};
+    if (m < M) {
+      const size_t e = c * comp + (size_t)m * Ny + ylo + yi;
+      val = u[e];
+      if constexpr (EM == EV_RECORD) {
+        const int yy = ylo + yi;  // the tile's own rows (not its halo)
+        if (yy >= y0 && yy < y0 + kCurlTY) ucopy[e] = val;
+      }
+      if constexpr (EM == EV_INC) {
+        const cx<double> r = uref[e];
+        val = {val.x - r.x, val.y - r.y};
+      }
+    }
+    tile[c][mi][yi] = val;
   }
   __syncthreads();
   const int mi = threadIdx.x % kTM, yi = threadIdx.x / kTM;
@@ -748,11 +772,12 @@ __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
   }
 }
 
-// WT5: (3, Ny, Nkz, Nkx) T -> rhs (3, M, Ny) FP64, tiled transpose.
+// WT5: (3, Ny, Nkz, Nkx) T -> rhs (3, M, Ny) FP64, tiled transpose; with
+// nref (split evaluations) rhs = nref + the transformed increment.
 template <typename T>
 __global__ void __launch_bounds__(kTY* kTM)
     k_walls_to_spec(const cx<T>* __restrict__ w, cx<double>* __restrict__ rhs, int Ny, int M,
-                    const int* __restrict__ act) {
+                    const int* __restrict__ act, const cx<double>* __restrict__ nref) {
   if (act && *act == 0) return;
   __shared__ cx<double> tile[kTY][kTM + 1];
   const int ntile_y = (Ny + kTY - 1) / kTY;
@@ -771,7 +796,12 @@ __global__ void __launch_bounds__(kTY* kTM)
   __syncthreads();
   const int yi = threadIdx.x % kTY, mi = threadIdx.x / kTY;
   const int m = m0 + mi, y = y0 + yi;
-  if (m < M && y < Ny) rhs[c * f + (size_t)m * Ny + y] = tile[yi][mi];
+  if (m < M && y < Ny) {
+    const size_t e = c * f + (size_t)m * Ny + y;
+    cx<double> r = tile[yi][mi];
+    if (nref) r = {nref[e].x + r.x, nref[e].y + r.y};
+    rhs[e] = r;
+  }
 }
 
 // ------------------------------------------------------------------ IMM kernel
@@ -1338,8 +1368,26 @@ __global__ void __launch_bounds__(256)
 }
 
 // ------------------------------------------------------------------ launchers
+template <typename T, int EM>
+static void launch_walls_curl(Ctx* c, dim3 grid, const cx<double>* u, cx<T>* A, hipStream_t s) {
+  WallTables* t = c->wt;
+  const cx<double>* uref = reinterpret_cast<const cx<double>*>(c->uref);
+  cx<double>* ucopy = reinterpret_cast<cx<double>*>(c->uref);
+  switch (t->p) {
+#define CURL(PO)                                                                                \
+  case PO:                                                                                      \
+    hipLaunchKernelGGL((k_walls_curl_t<T, PO + 1, EM>), grid, dim3(kCurlTY * kTM), 0, s, u, A,  \
+                       t->d1w, t->d1j0, t->mkz, t->Ny, t->M, c->act, uref, ucopy);              \
+    break;
+    CURL(2) CURL(4) CURL(6) CURL(8)
+#undef CURL
+    default: break;
+  }
+}
+
+// em: EvalMode; EV_RECORD / EV_INC only in FP32 with the reference buffers.
 template <typename T>
-static int walls_rhs_t(Ctx* c, const double* state, double* rhs, hipStream_t s) {
+static int walls_rhs_t(Ctx* c, const double* state, double* rhs, hipStream_t s, int em) {
   WallTables* t = c->wt;
   const int Ny = t->Ny, M = t->M;
   cx<T>* A = static_cast<cx<T>*>(c->bufA);
@@ -1350,15 +1398,14 @@ static int walls_rhs_t(Ctx* c, const double* state, double* rhs, hipStream_t s) 
     ProfScope ps(c, KID_W_Z_INVERSE, s);
     const dim3 grid(((Ny + kCurlTY - 1) / kCurlTY) * ntile_m);
     const cx<double>* u = reinterpret_cast<const cx<double>*>(state);
-    switch (t->p) {
-#define CURL(PO)                                                                                \
-  case PO:                                                                                      \
-    hipLaunchKernelGGL((k_walls_curl_t<T, PO + 1>), grid, dim3(kCurlTY * kTM), 0, s, u, A,      \
-                       t->d1w, t->d1j0, t->mkz, Ny, M, c->act);                                 \
-    break;
-      CURL(2) CURL(4) CURL(6) CURL(8)
-#undef CURL
-      default: return c->fail(DNSMI_EINVAL, "fd_order");
+    if (t->p != 2 && t->p != 4 && t->p != 6 && t->p != 8) return c->fail(DNSMI_EINVAL, "fd_order");
+    if (em == EV_PLAIN)
+      launch_walls_curl<T, EV_PLAIN>(c, grid, u, A, s);
+    else if constexpr (std::is_same<T, float>::value) {
+      if (em == EV_RECORD)
+        launch_walls_curl<T, EV_RECORD>(c, grid, u, A, s);
+      else
+        launch_walls_curl<T, EV_INC>(c, grid, u, A, s);
     }
     int rc = check_launch(c, "walls_curl");
     if (rc) return rc;
@@ -1377,20 +1424,36 @@ static int walls_rhs_t(Ctx* c, const double* state, double* rhs, hipStream_t s) 
     rc = check_launch(c, "walls_z_inverse");
     if (rc) return rc;
   }
-  int rc = launch_x_pass<T>(c, B, A, c->base, c->cbase, c->nbase, Ny, KID_W_X_NONLINEAR, s);
+  int rc = launch_x_pass<T>(c, B, A, c->base, c->cbase, c->nbase, Ny, KID_W_X_NONLINEAR, s, em);
   if (rc) return rc;
   rc = launch_z_forward_buf<T>(c, A, B, Ny, KID_W_Z_FORWARD, s);
   if (rc) return rc;
   {
     ProfScope ps(c, KID_W_Z_FORWARD, s);
     hipLaunchKernelGGL((k_walls_to_spec<T>), dim3(ntile_y * ntile_m, 3), dim3(kTY * kTM), 0, s, B,
-                       reinterpret_cast<cx<double>*>(rhs), Ny, M, c->act);
+                       reinterpret_cast<cx<double>*>(rhs), Ny, M, c->act,
+                       em == EV_INC ? reinterpret_cast<const cx<double>*>(c->nref) : nullptr);
   }
   return check_launch(c, "walls_to_spec");
 }
 
+// One evaluation in the context's current precision (see EvalMode; EV_INC
+// without a recorded reference, and every FP64 evaluation, is EV_PLAIN).
+static int walls_rhs_mode(Ctx* c, const double* state, double* rhs, hipStream_t s, int em) {
+  if (!c->fp32 || !c->split || (em == EV_INC && !c->ref_valid)) em = EV_PLAIN;
+  if (!c->fp32) return walls_rhs_t<double>(c, state, rhs, s, EV_PLAIN);
+  const int rc = walls_rhs_t<float>(c, state, rhs, s, em);
+  if (rc == DNSMI_OK && em == EV_RECORD) c->ref_valid = true;
+  return rc;
+}
+
 int walls_get_rhs(Ctx* c, const double* state, double* rhs, hipStream_t s) {
-  return c->fp32 ? walls_rhs_t<float>(c, state, rhs, s) : walls_rhs_t<double>(c, state, rhs, s);
+  return walls_rhs_mode(c, state, rhs, s, EV_PLAIN);
+}
+
+// N(state), recorded as the reference of the step's split evaluations.
+int walls_rhs_record(Ctx* c, const double* state, double* rhs, hipStream_t s) {
+  return walls_rhs_mode(c, state, rhs, s, EV_RECORD);
 }
 
 static int walls_imm(Ctx* c, const double* un, const double* uj, const double* nn,
@@ -1559,20 +1622,24 @@ int guess_walls(Ctx* c, const double* state, double* pred, double* rhs_prev, con
   if (mode < 0 || mode > 3) return c->fail(DNSMI_EINVAL, "guess: mode must be 0, 1, 2 or 3");
   if (mode == 2 && H != 32 && H != 64)
     return c->fail(DNSMI_EINVAL, "guess: hidden width must be 32 or 64");
-  int rc = walls_get_rhs(c, state, c->rhs0, s);
+  // N^n (into n_ref) is the reference of the step's split evaluations
+  int rc = walls_rhs_mode(c, state, c->nref, s, EV_RECORD);
   if (rc) return rc;
-  if ((rc = extrap_walls(c, c->rhs0, h1, h2, rhs_prev, wb, bias, H, s_1, s_2, s_out, mode, s)))
+  if ((rc = extrap_walls(c, c->nref, h1, h2, rhs_prev, wb, bias, H, s_1, s_2, s_out, mode, s)))
     return rc;
   return walls_predict(c, state, rhs_prev, pred, s);
 }
 
+// The predictor's evaluation N(u^n) (into n_ref) is the step's reference;
+// the corrector's evaluations are increments over it (mixed form).
 int walls_predict_and_correct(Ctx* c, const double* state, double* pred, double* rhs_next,
                               double* err, hipStream_t s) {
-  int rc = walls_get_rhs(c, state, c->rhs0, s);
+  double* n0 = c->nref;
+  int rc = walls_rhs_mode(c, state, n0, s, EV_RECORD);
   if (rc) return rc;
-  if ((rc = walls_imm(c, state, state, c->rhs0, c->rhs0, pred, nullptr, nullptr, s))) return rc;
-  if ((rc = walls_get_rhs(c, pred, rhs_next, s))) return rc;
-  return walls_imm(c, state, pred, c->rhs0, rhs_next, pred, nullptr, err, s);
+  if ((rc = walls_imm(c, state, state, n0, n0, pred, nullptr, nullptr, s))) return rc;
+  if ((rc = walls_rhs_mode(c, pred, rhs_next, s, EV_INC))) return rc;
+  return walls_imm(c, state, pred, n0, rhs_next, pred, nullptr, err, s);
 }
 
 int walls_iterate_correction(Ctx* c, const double* state_prev, double* pred,
@@ -1580,14 +1647,14 @@ int walls_iterate_correction(Ctx* c, const double* state_prev, double* pred,
                              hipStream_t s) {
   // N_n := rhs_prev, the previous evaluation (flows/plane_couette.py:534-539)
   if ((const void*)rhs_next == (const void*)rhs_prev) {
-    int rc = walls_get_rhs(c, pred, c->rhs0, s);
+    int rc = walls_rhs_mode(c, pred, c->rhs0, s, EV_INC);
     if (rc) return rc;
     if ((rc = walls_imm(c, state_prev, pred, rhs_prev, c->rhs0, pred, nullptr, err, s))) return rc;
     const hipError_t e = hipMemcpyAsync(rhs_next, c->rhs0, c->spec_elems * 16,
                                         hipMemcpyDeviceToDevice, s);
     return e == hipSuccess ? DNSMI_OK : c->fail(DNSMI_EHIP, hipGetErrorString(e));
   }
-  int rc = walls_get_rhs(c, pred, rhs_next, s);
+  int rc = walls_rhs_mode(c, pred, rhs_next, s, EV_INC);
   if (rc) return rc;
   return walls_imm(c, state_prev, pred, rhs_prev, rhs_next, pred, nullptr, err, s);
 }

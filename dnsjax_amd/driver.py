@@ -8,16 +8,14 @@ step whose corrector did not converge.  A NaN error is reported (the
 reference stops silently on it).  The error is read on the host once per
 corrector pass, as in the reference.
 
-The mixed form (FP32 transforms) has a noise floor near 1e-10 in the
-corrector error.  With ``params.corrector_floor_ratio`` set (the mixed
-default), a pass whose error did not shrink by that ratio while already
-below ``params.floor_cap_value`` (min(100 tolerance, 1e-8)) ends the
-corrector and counts as converged (``RunResult.floor_stops`` counts such
-steps), and its error becomes the floor estimate F.  With
-``params.corrector_floor_margin`` set (the mixed default, 2), a later pass
-with error below the cap and within that margin of F is taken as at the
-floor without the confirming pass, and F becomes min(F, error).  The
-device-decided step (``dnsmi_step_device``) applies the same rule.
+The mixed form evaluates each corrector pass as an FP32 increment over the
+step's first evaluation, so its passes converge as FP64 ones do.  If a pass
+still stops converging (its error, at or below
+``params.escalate_cap_value``, is not below ``params.escalate_ratio``
+times the error two passes earlier), the step's remaining passes run in
+FP64 (``RunResult.escalations`` counts such steps): every step ends at
+error <= ``corrector_tolerance``, as in the reference.  The device-decided
+step (``dnsmi_step_device``) applies the same rule.
 """
 
 from __future__ import annotations
@@ -33,20 +31,50 @@ class RunResult:
     errors: list = field(default_factory=list)
     converged: bool = True
     nan: bool = False
-    floor_stops: int = 0
+    escalations: int = 0
 
     @property
     def rhs_evals(self) -> int:  # __main__.py:199-201
         return sum(c + 2 for c in self.counts)
 
 
+class Escalation:
+    """The mixed form's switch to FP64 passes within one step (see the
+    module docstring).  ``after(c, err)`` is called after pass c with its
+    error and switches the flow to FP64 for the rest of the step when the
+    passes stopped converging; ``end()`` switches back."""
+
+    def __init__(self, flow):
+        p = flow.params
+        self.flow = flow
+        self.ratio = p.escalate_ratio if p.precision == "mixed" else None
+        self.cap = p.escalate_cap_value
+        self.tol = p.corrector_tolerance
+        self.on = False
+        self.prev = self.prev2 = math.inf
+
+    def begin(self, err0: float = math.inf) -> None:
+        self.prev, self.prev2 = err0, math.inf
+
+    def after(self, err: float) -> None:
+        if (self.ratio is not None and not self.on and self.tol < err <= self.cap
+                and err > self.ratio * self.prev2):
+            self.on = True
+            self.flow.escalate(True)
+        self.prev2, self.prev = self.prev, err
+
+    def end(self) -> bool:
+        was = self.on
+        if self.on:
+            self.flow.escalate(False)
+            self.on = False
+        return was
+
+
 def run(flow, state, n_steps: int, callback=None) -> RunResult:
     p = flow.params
     tol, max_it = p.corrector_tolerance, p.max_corrector_iterations
-    ratio = p.corrector_floor_ratio
-    cap = p.floor_cap_value
-    margin = p.corrector_floor_margin
-    floor_est = 0.0
+    esc = Escalation(flow)
     res = RunResult(state=state)
     last_error = 0.0
     for it in range(n_steps):
@@ -55,24 +83,19 @@ def run(flow, state, n_steps: int, callback=None) -> RunResult:
         state_prev = state
         state, rhs, error = flow.predict_and_correct(state_prev)
         err = float(error)
+        esc.begin(err)
         c = 0
-        floor = False
         while err > tol and c < max_it:
             state, rhs, error = flow.iterate_correction(state_prev, state, rhs)
-            prev, err = err, float(error)
+            err = float(error)
             c += 1
-            if ratio is not None and tol < err <= cap:
-                if err > ratio * prev:
-                    floor, floor_est = True, err
-                    break
-                if margin is not None and floor_est > 0 and err <= margin * floor_est:
-                    floor, floor_est = True, min(floor_est, err)
-                    break
+            if c < max_it:
+                esc.after(err)
+        res.escalations += int(esc.end())
         state, _ = flow.correct_velocity(state)
         res.counts.append(c)
         res.errors.append(err)
-        res.floor_stops += int(floor)
-        last_error = 0.0 if floor else err
+        last_error = err
         if callback is not None:
             callback(it + 1, state)
         if math.isnan(err):

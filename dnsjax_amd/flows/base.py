@@ -66,6 +66,10 @@ class NativeFlow:
         idx = {"E": 0, "I": 1, "D": 2, "E'": 3}
         return {k: v[idx[k]] for k in keys}
 
+    def escalate(self, on: bool) -> None:
+        """Mixed form: the following corrector passes run in FP64 (or back)."""
+        self._native.escalate(on)
+
     def correct_velocity(self, state):
         """In place (the reference donates its input); returns
         ``(state, norm_corrections)``."""

@@ -23,10 +23,11 @@ import torch
 
 
 class _Member:
-    def __init__(self, flow, state, slots, capacity):
+    def __init__(self, flow, state, slots, capacity, esc_slots=None):
         nat = flow._native
         self.flow = flow
         self.nat = nat
+        self.esc_slots = esc_slots
         p = flow.params
         self.slots = p.max_corrector_iterations if slots is None else min(slots, p.max_corrector_iterations)
         self.bufs = [state.detach().clone().contiguous(), nat.empty()]
@@ -36,7 +37,7 @@ class _Member:
 
     def enqueue(self, parity: int) -> None:
         self.nat.step_device(self.bufs[parity], self.bufs[1 - parity], self.rhs[0], self.rhs[1], self.slots,
-                             self.counts)
+                             self.counts, esc_slots=self.esc_slots)
 
 
 class Ensemble:
@@ -44,9 +45,11 @@ class Ensemble:
 
     `slots` is the corrector slots per captured step: one value for every
     member, or one per member (a member's skipped slots still launch their
-    kernels, so each can hold just what its own trajectory needs)."""
+    kernels, so each can hold just what its own trajectory needs).
+    `esc_slots`: the mixed form's FP64 escalation slots per step (None: the
+    library default)."""
 
-    def __init__(self, flows, states, n_streams: int = 8, slots=None, capacity: int = 4096):
+    def __init__(self, flows, states, n_streams: int = 8, slots=None, capacity: int = 4096, esc_slots=None):
         if len(flows) != len(states) or not flows:
             raise ValueError("need one state per flow")
         if len({id(f._native) for f in flows}) != len(flows):
@@ -56,7 +59,7 @@ class Ensemble:
         per = list(slots) if isinstance(slots, (list, tuple)) else [slots] * len(flows)
         if len(per) != len(flows):
             raise ValueError("need one slot count per flow")
-        self.members = [_Member(f, s, k, capacity) for f, s, k in zip(flows, states, per)]
+        self.members = [_Member(f, s, k, capacity, esc_slots) for f, s, k in zip(flows, states, per)]
         self.capacity = capacity
         self.parity = 0
         self.steps = 0
@@ -98,8 +101,8 @@ class Ensemble:
 class GraphStepper(Ensemble):
     """A single trajectory stepped by captured graphs (one launch per step)."""
 
-    def __init__(self, flow, state, slots: int | None = None, capacity: int = 4096):
-        super().__init__([flow], [state], n_streams=1, slots=slots, capacity=capacity)
+    def __init__(self, flow, state, slots: int | None = None, capacity: int = 4096, esc_slots=None):
+        super().__init__([flow], [state], n_streams=1, slots=slots, capacity=capacity, esc_slots=esc_slots)
 
     @property
     def state(self):

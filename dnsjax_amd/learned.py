@@ -41,6 +41,7 @@ from dataclasses import dataclass, field
 import torch
 
 from . import _lib
+from .driver import Escalation
 from .native import _ptr, _stream
 
 
@@ -210,7 +211,7 @@ class GuessRunResult:
     errors: list = field(default_factory=list)
     converged: bool = True
     nan: bool = False
-    floor_stops: int = 0
+    escalations: int = 0
 
     @property
     def rhs_evals(self) -> int:
@@ -226,12 +227,14 @@ def run_guessed(flow, state, n_steps: int, mode: int = 2, net: Net | None = None
     started from a guess: mode 1 linear, 3 quadratic extrapolation of N,
     2 quadratic plus the network (for plane Couette the guess is the
     predictor's solve fed with the extrapolated N, and the network acts per
-    point (kz, kx, y)).  Mode 0 is the plain predictor: the same steps as the
+    point (kz, kx, y), or per mode (kz, kx) on whole wall-normal profiles
+    when ``net`` is a ``profile_net.ProfileNet``).  Mode 0 is the plain
+    predictor: the same steps as the
     plain driver, counted as passes (= iterations + 1).  The first steps of a run lack history and
     start from the plain predictor (step 0) and linear extrapolation (step
     1).  Up to ``max_corrector_iterations + 1`` passes per step (the plain
     scheme's RHS budget); the loop stops after a step that did not
-    converge, and the mixed form's noise-floor rule applies as in
+    converge, and the mixed form's escalation to FP64 passes applies as in
     ``driver.run``.  The state handed to ``callback`` is a buffer that
     later steps reuse: copy it to keep it."""
     if mode not in (0, 1, 2, 3):
@@ -241,12 +244,9 @@ def run_guessed(flow, state, n_steps: int, mode: int = 2, net: Net | None = None
     p = flow.params
     nat = flow._native
     tol, max_pass = p.corrector_tolerance, p.max_corrector_iterations + 1
-    ratio = p.corrector_floor_ratio
-    cap = p.floor_cap_value
-    margin = p.corrector_floor_margin
-    floor_est = 0.0
+    esc = Escalation(flow)
     sc = (1.0, 1.0, 1.0) if scales is None else scales.astuple()
-    ops = None if net is None else net.operands
+    ops = None if net is None or hasattr(net, "guess") else net.operands
     hist = [torch.zeros(nat.shape + (2,), dtype=torch.float32, device=nat.device) for _ in range(2)]
     res = GuessRunResult(state=state)
     state = state.clone()
@@ -258,27 +258,26 @@ def run_guessed(flow, state, n_steps: int, mode: int = 2, net: Net | None = None
         if not last_error < tol:
             break
         m = 0 if it == 0 else (min(mode, 1) if it == 1 else mode)
-        nat.guess(state, pred, rhs[0], hist[0], hist[1], m, ops, sc)
+        if m == 2 and hasattr(net, "guess"):  # the plane Couette profile network
+            net.guess(nat, state, pred, rhs[0], hist[0], hist[1], scales)
+        else:
+            nat.guess(state, pred, rhs[0], hist[0], hist[1], m, ops, sc)
         hist.reverse()  # hist[0] = N^n, hist[1] = N^{n-1}
-        c, err, floor, cur = 0, math.inf, False, 0
+        c, err, cur = 0, math.inf, 0
+        esc.begin()
         while err > tol and c < max_pass:
             nat.iterate_correction(state, pred, rhs[cur], rhs[1 - cur], err_t)
             cur = 1 - cur
-            prev, err = err, float(err_t)
+            err = float(err_t)
             c += 1
-            if ratio is not None and tol < err <= cap:
-                if err > ratio * prev:
-                    floor, floor_est = True, err
-                    break
-                if margin is not None and floor_est > 0 and err <= margin * floor_est:
-                    floor, floor_est = True, min(floor_est, err)
-                    break
+            if c < max_pass:
+                esc.after(err)
+        res.escalations += int(esc.end())
         pred, state = state, pred
         state, _ = flow.correct_velocity(state)
         res.passes.append(c)
         res.errors.append(err)
-        res.floor_stops += int(floor)
-        last_error = 0.0 if floor else err
+        last_error = err
         if callback is not None:
             callback(it + 1, state)
         if math.isnan(err):

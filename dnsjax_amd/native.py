@@ -38,7 +38,7 @@ class NativeContext:
             tilt_degree=p.tilt_degree, fd_order=p.fd_order,
             precision=_lib.DNSMI_MIXED if p.precision == "mixed" else _lib.DNSMI_FP64,
             correct_divergence=int(p.correct_divergence), tile_cols=p.tile_cols,
-            variants=p.variants,
+            variants=p.variants, plain_mixed=int(p.plain_mixed),
         )
         handle = ctypes.c_void_p()
         torch.cuda.current_device()  # make the device current before any HIP call
@@ -129,21 +129,34 @@ class NativeContext:
         return out
 
     # ---------------------------------------------------------------- device-side steps
+    def _esc_args(self, esc_slots):
+        """(esc_slots, ratio, cap) of the mixed form's FP64 escalation slots."""
+        p = self.params
+        if p.precision != "mixed" or p.escalate_ratio is None:
+            return 0, 0.0, 0.0
+        # two: the first FP64 pass still differences against the last FP32
+        # evaluation, so its error sits at the FP32 resolution
+        n = 2 if esc_slots is None else esc_slots
+        return n, p.escalate_ratio, p.escalate_cap_value
+
     def step_device(self, state, nxt, rhs_a, rhs_b, slots, counts=None, tol=None, max_iter=None,
-                    floor_ratio=None, floor_cap=None, floor_margin=None):
+                    esc_slots=None):
         """Enqueue one full time step with the corrector decided on the device
-        (capturable; no host synchronisation).  The new state goes to `nxt`."""
+        (capturable; no host synchronisation).  The new state goes to `nxt`.
+        Mixed form: `esc_slots` FP64 slots (default 2) follow the `slots`
+        passes, for a step whose FP32 passes stop converging."""
         p = self.params
         tol = p.corrector_tolerance if tol is None else tol
         max_iter = p.max_corrector_iterations if max_iter is None else max_iter
-        ratio = p.corrector_floor_ratio if floor_ratio is None else floor_ratio
-        cap = min(p.corrector_floor_cap * tol, p.corrector_floor_abs) if floor_cap is None else floor_cap
-        margin = p.corrector_floor_margin if floor_margin is None else floor_margin
+        n_esc, ratio, cap = self._esc_args(esc_slots)
         ncap = 0 if counts is None else counts.numel()
         self._chk(self.lib.dnsmi_step_device(self.handle, _ptr(state), _ptr(nxt), _ptr(rhs_a), _ptr(rhs_b),
-                                             slots, _ptr(counts), ncap, tol, max_iter,
-                                             0.0 if ratio is None else ratio, cap,
-                                             0.0 if margin is None else margin, _stream()))
+                                             slots, n_esc, _ptr(counts), ncap, tol, max_iter, ratio, cap,
+                                             _stream()))
+
+    def escalate(self, on: bool) -> None:
+        """Mixed form: run the following corrector passes in FP64 (or back)."""
+        self._chk(self.lib.dnsmi_escalate(self.handle, int(bool(on))))
 
     def mlp_features(self, n_m2, n_m1, n_cur, n_next, modes, s_1, s_2, s_out, X, T=None):
         """Training features (and targets when n_next and T are given)."""
@@ -161,25 +174,25 @@ class NativeContext:
                                        _ptr(hist2), _ptr(wb), _ptr(bias), H, s_1, s_2, s_out, mode, _stream()))
 
     def step_device_guess(self, state, nxt, rhs_a, rhs_b, hist1, hist2, mode, slots, counts=None, net=None,
-                          scales=(1.0, 1.0, 1.0)):
+                          scales=(1.0, 1.0, 1.0), esc_slots=None):
         """Enqueue one time step started from the corrector guess, decided on
-        the device (capturable).  Up to max_corrector_iterations + 1 passes."""
+        the device (capturable).  Up to max_corrector_iterations + 1 passes;
+        mixed form: `esc_slots` FP64 slots (default 2) as in step_device."""
         p = self.params
         H, wb, bias = (32, None, None) if net is None else net
         s_1, s_2, s_out = scales
-        ratio, margin = p.corrector_floor_ratio, p.corrector_floor_margin
+        n_esc, ratio, cap = self._esc_args(esc_slots)
         ncap = 0 if counts is None else counts.numel()
         self._chk(self.lib.dnsmi_step_device_guess(
             self.handle, _ptr(state), _ptr(nxt), _ptr(rhs_a), _ptr(rhs_b), _ptr(hist1), _ptr(hist2), _ptr(wb),
-            _ptr(bias), H, s_1, s_2, s_out, mode, slots, _ptr(counts), ncap, p.corrector_tolerance,
-            p.max_corrector_iterations + 1, 0.0 if ratio is None else ratio,
-            p.floor_cap_value, 0.0 if margin is None else margin, _stream()))
+            _ptr(bias), H, s_1, s_2, s_out, mode, slots, n_esc, _ptr(counts), ncap, p.corrector_tolerance,
+            p.max_corrector_iterations + 1, ratio, cap, _stream()))
 
     def step_stats(self) -> dict:
         out = (ctypes.c_double * 6)()
         self._chk(self.lib.dnsmi_step_stats(self.handle, out))
         return {"steps": int(out[0]), "total_iterations": int(out[1]), "unconverged": int(out[2]),
-                "nan": int(out[3]), "last_error": out[4], "last_floor_stop": bool(out[5])}
+                "nan": int(out[3]), "last_error": out[4], "escalated": int(out[5])}
 
     def step_reset(self) -> None:
         self._chk(self.lib.dnsmi_step_reset(self.handle))

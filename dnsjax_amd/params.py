@@ -40,25 +40,23 @@ class Params:
     precision: str = "fp64"
     tile_cols: int = 0  # 0: autotune kernel variants; else use `variants`
     variants: int = 0
-    # Mixed form only: stop the corrector at the FP32 noise floor, i.e. when
-    # an iteration fails to shrink the error by this factor while the error
-    # is already below floor_cap (see `floor_cap_value`).  None keeps the
-    # reference's rule (iterate until error <= tolerance) unchanged.
-    corrector_floor_ratio: float | None = None
-    corrector_floor_cap: float = 100.0
-    # Absolute ceiling of that window: the FP32 floor of the corrector error
-    # for O(1) fields is near 1e-10 (128^3); a stall above 1e-8 is slow
-    # contraction (the wall-bounded corrector contracts only ~2x per pass),
-    # not noise, and must not end the corrector.
-    corrector_floor_abs: float = 1e-8
-    # Mixed form only: once a stall has shown the floor level F (the error
-    # of the last stall stop), a pass with tol < error <= floor_cap and error <=
-    # corrector_floor_margin * F is taken as at the floor, without the pass
-    # that would confirm the stall; F then becomes min(F, error).  The state
-    # is then converged to within the last correction over the corrector's
-    # contraction factor (about 70 for the periodic corrector).  None keeps
-    # the stall rule alone.
-    corrector_floor_margin: float | None = None
+    # Mixed form only.  Every corrector pass of a step is evaluated as an
+    # FP32 increment over the step's first evaluation (split precision), so
+    # its rounding scales with the increment and the corrector converges to
+    # corrector_tolerance as in FP64.  plain_mixed evaluates each pass in
+    # full in FP32 instead (its corrector error stalls near 1e-10 at 128^3).
+    plain_mixed: bool = False
+    # Mixed form only: escalation to FP64 passes.  A pass whose error, at or
+    # below `escalate_cap_value`, is not below escalate_ratio times the error
+    # two passes earlier has reached the resolution of its FP32 evaluations;
+    # the step's remaining passes then run in FP64, so every step still ends
+    # at error <= corrector_tolerance (__main__.py:167-169).  None disables it.
+    escalate_ratio: float | None = None
+    escalate_cap: float = 1e4  # times corrector_tolerance
+    # Absolute ceiling of that window: a stall above 1e-8 is slow contraction
+    # (the wall-bounded corrector repeats its error on the pass after a
+    # start), not FP32 resolution.
+    escalate_abs: float = 1e-8
 
     def __post_init__(self) -> None:
         if self.system not in PERIODIC_SYSTEMS + WALLED_SYSTEMS:
@@ -70,17 +68,15 @@ class Params:
             raise ValueError("walled systems have ly = 2 (y in [-1, 1])")
         if self.precision not in ("fp64", "mixed"):
             raise ValueError("precision must be 'fp64' or 'mixed'")
-        if self.corrector_floor_ratio is None and self.precision == "mixed":
-            self.corrector_floor_ratio = 0.5
-        if self.corrector_floor_margin is None and self.precision == "mixed":
-            self.corrector_floor_margin = 2.0
+        if self.escalate_ratio is None and self.precision == "mixed":
+            self.escalate_ratio = 0.5
         if not self.re > 0 or not self.dt > 0 or not 0 <= self.implicitness <= 1:
             raise ValueError("need re > 0, dt > 0, 0 <= implicitness <= 1")
 
     @property
-    def floor_cap_value(self) -> float:
-        """Error below which a stalled pass counts as the noise floor."""
-        return min(self.corrector_floor_cap * self.corrector_tolerance, self.corrector_floor_abs)
+    def escalate_cap_value(self) -> float:
+        """Error at or below which a stalled pass escalates to FP64."""
+        return min(self.escalate_cap * self.corrector_tolerance, self.escalate_abs)
 
     @property
     def walls(self) -> bool:

@@ -71,11 +71,16 @@ typedef struct dnsmi_config {
   int fd_order;            /* walls: stencil order p; parameters.py:62 */
   int precision;           /* DNSMI_FP64 or DNSMI_MIXED */
   int correct_divergence;  /* parameters.py:108 */
-  int tile_cols;           /* 0: autotune the kernel variants at create;
-                              otherwise use `variants` as given */
+  int tile_cols;           /* 0: autotune the kernel variants and the column
+                              tile of the y/z passes at create; 8 or 16: use
+                              that tile and `variants` as given */
   int variants;            /* bit mask when tile_cols != 0: 1 sequential
                               y-inverse, 2 sequential z-inverse, 4 split x pass,
                               8 sequential band sweeps (walls; else chunk maps) */
+  int plain_mixed;         /* DNSMI_MIXED only: 0 (default) evaluates every
+                              corrector pass of a step as an FP32 increment over
+                              the step's first evaluation (split precision);
+                              1 evaluates each pass in full in FP32 */
 } dnsmi_config;
 
 typedef struct dnsmi_ctx dnsmi_ctx;
@@ -134,25 +139,32 @@ int dnsmi_walls_table(dnsmi_ctx* ctx, int which, double* out, long long n);
 /* One full time step with the corrector loop decided on the device
  * (__main__.py:161-191 without host reads): predict_and_correct from
  * `state` into `next`, then up to `slots` iterate_correction passes, each
- * skipped on the device once error <= tol or max_iter extra passes ran
- * (and, if floor_ratio > 0, once a pass failed to shrink the error by that
- * ratio while tol < error <= floor_cap: the mixed form's noise floor; that error
- * becomes the context's floor estimate F, and with floor_margin > 0 a later
- * pass with tol < error <= floor_cap and error <= floor_margin * F ends the
- * corrector, setting F to min(F, error)), then
- * correct_velocity(next).  rhs_a / rhs_b are two RHS work arrays.  No host
- * synchronisation; the launch sequence is fixed, so it can be captured in
- * a hipGraph.  counts (device int[counts_cap], may be NULL) receives each
- * step's extra-iteration count in order.  `state` is not modified. */
+ * skipped on the device once error <= tol or max_iter extra passes ran, then
+ * correct_velocity(next).  Mixed form: `esc_slots` FP64 passes follow, run
+ * only by a step that escalated: a pass whose error (tol < error <= esc_cap)
+ * is not below esc_ratio times the error two passes earlier has reached the
+ * resolution of the FP32 evaluations, and so has a step whose `slots` passes
+ * all ran without converging; its remaining passes run in FP64 and it still
+ * ends at error <= tol (esc_slots 0: such a step ends unconverged).
+ * rhs_a / rhs_b are two RHS work arrays.  No host synchronisation; the
+ * launch sequence is fixed, so it can be captured in a hipGraph.  counts
+ * (device int[counts_cap], may be NULL) receives each step's
+ * extra-iteration count in order.  `state` is not modified. */
 int dnsmi_step_device(dnsmi_ctx* ctx, const double* state, double* next, double* rhs_a,
-                      double* rhs_b, int slots, int* counts, int counts_cap, double tol,
-                      int max_iter, double floor_ratio, double floor_cap, double floor_margin,
-                      void* stream);
+                      double* rhs_b, int slots, int esc_slots, int* counts, int counts_cap,
+                      double tol, int max_iter, double esc_ratio, double esc_cap, void* stream);
 /* Synchronising read of the device step counters: out6 = {steps, total extra
- * iterations, steps not converged, NaN steps, last error, last step stopped
- * at the noise floor}; dnsmi_step_reset zeroes them and the floor estimate. */
+ * iterations, steps not converged (error > tol at the end), NaN steps, last
+ * error, steps that escalated to FP64 passes}; dnsmi_step_reset zeroes them. */
 int dnsmi_step_stats(dnsmi_ctx* ctx, double* out6);
 int dnsmi_step_reset(dnsmi_ctx* ctx);
+
+/* Mixed form only (no counterpart in the reference): while fp64 != 0 the
+ * corrector passes (dnsmi_iterate_correction and the passes of the device
+ * steps) evaluate in FP64, the host loop's escalation after a pass that no
+ * longer converges in FP32 (see dnsmi_step_device); 0 returns to the mixed
+ * form. */
+int dnsmi_escalate(dnsmi_ctx* ctx, int fp64);
 
 /* Diagnostics of a state into stats_dev[4] = {E, I, D, E'} (device doubles).
  * Replaces get_stats: flows/monochromatic.py:433-510 (E' = |u|^2/2, input,
@@ -238,17 +250,62 @@ int dnsmi_guess(dnsmi_ctx* ctx, const double* state, double* pred, double* rhs_p
 /* One time step started from the corrector guess, decided on the device
  * (periodic and wall-bounded): dnsmi_guess from `state` into `next` (with rhs_a as
  * rhs_prev), then up to `slots` corrector passes, each skipped on the device
- * once error <= tol, max_pass passes ran, or the noise-floor rule of
- * dnsmi_step_device fires; then correct_velocity(next).  counts[i] receives
+ * once error <= tol or max_pass passes ran, and the escalation slots of
+ * dnsmi_step_device; then correct_velocity(next).  counts[i] receives
  * step i's number of passes (one RHS each).  hist1/hist2 as in dnsmi_guess
  * (hist2 receives N^n; swap them, and state/next, for the next step).  A
  * fixed launch sequence with no host synchronisation: capturable. */
 int dnsmi_step_device_guess(dnsmi_ctx* ctx, const double* state, double* next, double* rhs_a,
                             double* rhs_b, const float* hist1, float* hist2,
                             const void* packed_weights, const float* bias, int H, float s_1,
-                            float s_2, float s_out, int mode, int slots, int* counts,
-                            int counts_cap, double tol, int max_pass, double floor_ratio,
-                            double floor_cap, double floor_margin, void* stream);
+                            float s_2, float s_out, int mode, int slots, int esc_slots,
+                            int* counts, int counts_cap, double tol, int max_pass,
+                            double esc_ratio, double esc_cap, void* stream);
+
+/* ---- Profile network for the plane Couette guess (no counterpart in the
+ * reference) -------------------------------------------------------------
+ * One network per context: din -> H -> H -> dout, SiLU, FP32 parameters W1
+ * (din,H) | b1 (H) | W2 (H,H) | b2 (H) | W3 (H,dout) | b3 (dout), row-major.
+ * Training and inference run every layer as a bf16 matrix-core GEMM with
+ * FP32 accumulation (backward included).  One sample is one (kz, kx) mode:
+ * input the wall-normal profiles of D1 = N^n - N^{n-1} and D2 = N^n -
+ * 2 N^{n-1} + N^{n-2} (3 components, real and imaginary: 12 Ny values
+ * scaled by s_1, s_2), dt k^2/Re, kx/k_max, kz/k_max, zero padded to din (a
+ * multiple of 32, >= 12 Ny + 3); output (dout = 6 Ny) the profile of the
+ * residual of quadratic extrapolation, N^{n+1} - (3 N^n - 3 N^{n-1} +
+ * N^{n-2}), times w = 1/(1 + c dt k^2/Re) / s_out.  H a multiple of 64.
+ * X and the bf16 buffers are 16-bit words; wb holds one word per parameter. */
+long long dnsmi_pnet_params(int din, int H, int dout); /* -1 if unsupported */
+/* Gradient of L = (1/(B dout)) sum (Y - T)^2 over X (B,din) bf16, T (B,dout)
+ * FP32 into grad (FP32), loss into loss_dev (may be NULL).  Work: wf
+ * B (2H + dout) floats, wh B (3H + dout) words, part ceil(B dout/256)
+ * floats. */
+int dnsmi_pnet_grad(int din, int H, int dout, int B, const void* X, const float* T, const float* params,
+                    void* wb, float* wf, void* wh, float* part, float* grad, float* loss_dev, void* stream);
+/* Y (B,dout) = net(X); work wf 2 H B floats, wh 2 H B words. */
+int dnsmi_pnet_forward(int din, int H, int dout, int B, const void* X, const float* params, void* wb,
+                       float* wf, void* wh, float* Y, void* stream);
+/* One Adam step over P parameters (beta1 0.9, beta2 0.999, eps 1e-8). */
+int dnsmi_pnet_adam(long long P, float* params, const float* grad, float* m, float* v, int step, float lr,
+                    void* stream);
+/* Training rows X (nsamples, din) and, with n_next, targets T (nsamples,
+ * dout) from four successive nonlinear terms of a plane Couette context
+ * (walls layout, complex128), for the mode indices `modes` (over (kz, kx));
+ * the two older terms are rounded to FP32 as the guess's history keeps
+ * them. */
+int dnsmi_pnet_features(dnsmi_ctx* ctx, const double* n_m2, const double* n_m1, const double* n_cur,
+                        const double* n_next, const long long* modes, int nsamples, int din, float s_1,
+                        float s_2, float s_out, void* X, float* T, void* stream);
+/* dnsmi_guess (plane Couette) with the profile network: N^n (recorded as
+ * the step's reference) into the context, rhs_prev = quadratic
+ * extrapolation plus s_out Y / w per point, pred = the predictor's solve fed
+ * with rhs_prev, hist2 <- N^n.  Work, M = (nz-1) nx/2 modes: X M din words,
+ * wf 2 H M floats, wh 2 H M words, Y M dout floats, wb one word per
+ * parameter. */
+int dnsmi_guess_profile(dnsmi_ctx* ctx, const double* state, double* pred, double* rhs_prev,
+                        const float* hist1, float* hist2, int din, int H, int dout, const float* params,
+                        void* wb, void* X, float* wf, void* wh, float* Y, float s_1, float s_2, float s_out,
+                        void* stream);
 
 /* Per-kernel timing with HIP events on the launch stream (no counterpart in
  * the reference, whose bench.timer (bench.py:26-67) times whole calls).

@@ -252,3 +252,82 @@ def guess_walls(state, h1, h2, flow, mode, params=None, scales=(1.0, 1.0, 1.0)):
             net = (Y[:, 2 * c] + 1j * Y[:, 2 * c + 1]).reshape(w.shape)
             ext[c] = ext[c] + s_out * net / w
     return W.predict(state, ext, flow), ext, n0.copy()
+
+
+# ------------------------------------------------------------ profile network
+def profile_din(ny: int) -> int:
+    """Input width of the per-(kz, kx) profile network: 12 Ny profile values
+    and three mode features, padded to a multiple of 32."""
+    return -(-(12 * ny + 3) // 32) * 32
+
+
+def features_walls_profile(n_m2, n_m1, n_cur, n_next, modes, flow, s_1, s_2, s_out, din):
+    """X (S, din), T (S, 6 Ny) for flat mode indices over (kz, kx): the wall-
+    normal profiles of D1 and D2 (older terms rounded to FP32) ordered
+    [component][real, imaginary][y], D1 block then D2 block, then
+    min(dt k^2/Re, 8), kx/k_max, kz/k_max, zeros; T the profile of the
+    weighted residual of quadratic extrapolation in the D1 block's order."""
+    p = flow.p
+    ny = p.ny
+    f32 = lambda a: a.astype(np.float32).astype(np.float64)  # noqa: E731
+    nkx = p.nx // 2
+    iz, ix = np.asarray(modes) // nkx, np.asarray(modes) % nkx
+    X = np.zeros((len(modes), din))
+    for c in range(3):
+        cur = n_cur[c][iz, ix]        # (S, Ny)
+        p1 = f32(n_m1[c][iz, ix].real) + 1j * f32(n_m1[c][iz, ix].imag)
+        p2 = f32(n_m2[c][iz, ix].real) + 1j * f32(n_m2[c][iz, ix].imag)
+        d1 = cur - p1
+        d2 = (cur - 2 * p1) + p2
+        o = 2 * c * ny
+        X[:, o:o + ny] = d1.real * s_1
+        X[:, o + ny:o + 2 * ny] = d1.imag * s_1
+        X[:, 6 * ny + o:6 * ny + o + ny] = d2.real * s_2
+        X[:, 6 * ny + o + ny:6 * ny + o + 2 * ny] = d2.imag * s_2
+    kx, kz = flow.kx_global[ix], flow.kz_global[iz]
+    kmax = max(2 * np.pi / p.lx * (p.nx // 2), 2 * np.pi / p.lz * (p.nz // 2))
+    k2 = kx**2 + kz**2
+    X[:, 12 * ny] = np.minimum(p.dt * k2 / p.re, 8.0)
+    X[:, 12 * ny + 1] = kx / kmax
+    X[:, 12 * ny + 2] = kz / kmax
+    T = None
+    if n_next is not None:
+        w = 1.0 / (1.0 + p.dt * p.implicitness * k2 / p.re)
+        T = np.zeros((len(modes), 6 * ny))
+        for c in range(3):
+            d3 = (n_next[c] - 3 * n_cur[c] + 3 * n_m1[c] - n_m2[c])[iz, ix]
+            o = 2 * c * ny
+            T[:, o:o + ny] = w[:, None] * d3.real / s_out
+            T[:, o + ny:o + 2 * ny] = w[:, None] * d3.imag / s_out
+    return X, T
+
+
+def profile_loss_and_grads(params, x, t, bf16=True):
+    """The profile network's loss (1/(B dout)) sum (y - t)^2 and gradients:
+    ``loss_and_grads`` divided by dout."""
+    loss, g = loss_and_grads(params, x, t, bf16)
+    d = t.shape[1]
+    return loss / d, tuple(gi / d for gi in g)
+
+
+def guess_walls_profile(state, h1, h2, flow, params, scales, din):
+    """(pred, rhs_prev, N^n) of the wall-bounded guess with the profile
+    network: N_ext = N^n + D1 + D2 + s_out net / w per point (bf16
+    inference), the predictor's solve fed with it (h1, h2 rounded to FP32 by
+    the caller)."""
+    from . import walls as W
+
+    p = flow.p
+    s_1, s_2, s_out = scales
+    n0 = W.get_rhs(state, flow)
+    ext = n0 + (n0 - h1) + ((n0 - 2 * h1) + h2)
+    M = (p.nz - 1) * (p.nx // 2)
+    X, _ = features_walls_profile(h2, h1, n0, None, np.arange(M), flow, s_1, s_2, s_out, din)
+    Y, _ = forward(params, X, bf16=True)
+    ny = p.ny
+    w = mode_weight_walls(flow)
+    for c in range(3):
+        o = 2 * c * ny
+        net = (Y[:, o:o + ny] + 1j * Y[:, o + ny:o + 2 * ny]).reshape(p.nz - 1, p.nx // 2, ny)
+        ext[c] = ext[c] + s_out * net / w
+    return W.predict(state, ext, flow), ext, n0.copy()

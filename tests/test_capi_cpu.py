@@ -123,50 +123,67 @@ def test_driver_detects_nan():
     assert r.nan and not r.converged and r.counts == [0]
 
 
-def test_driver_noise_floor_rule_for_mixed():
-    f = _ScriptedFlow([1e-5, 1e-7, 2e-9, 1.6e-10, 1.3e-10, 1e-5, 1e-11], tol=1e-10, max_it=10)
-    f.params = Params(precision="mixed", corrector_tolerance=1e-10, max_corrector_iterations=10)
+class _EscalatingFlow(_ScriptedFlow):
+    """Scripted errors that change once the flow escalates to FP64 passes:
+    `fp32` before the switch, `fp64` after it (popped per pass)."""
+
+    def __init__(self, fp32, fp64, tol=1e-10, max_it=10, **kw):
+        super().__init__(fp32, tol=tol, max_it=max_it)
+        self.params = Params(precision="mixed", corrector_tolerance=tol, max_corrector_iterations=max_it, **kw)
+        self.fp64 = list(fp64)
+        self.on = False
+        self.switches = []
+
+    def iterate_correction(self, sp, s, r):
+        return s, None, (self.fp64 if self.on else self.errs).pop(0)
+
+    def escalate(self, on):
+        self.on = on
+        self.switches.append(on)
+
+
+def test_mixed_form_escalates_a_stalled_corrector_to_fp64():
+    """A pass that is not below half the error two passes back, at an error
+    below the window's cap, switches the step's remaining passes to FP64; the
+    step still iterates to the tolerance, and the next step starts in FP32."""
+    f = _EscalatingFlow([1e-5, 1e-7, 2e-9, 1.6e-10, 1.3e-10, 1.2e-10,
+                         1e-5, 1e-7, 1e-9, 5e-11],
+                        [2e-12])
+    # step 1: each of 1e-7, 2e-9, 1.6e-10 and 1.3e-10 is below half the error
+    # two passes back; 1.2e-10 > 0.5 * 1.6e-10 -> the next pass is FP64 and
+    # ends the step at 2e-12 <= tol.  Step 2 converges in FP32.
     r = run(f, 0.0, 2)
-    assert r.counts == [4, 1] and r.floor_stops == 1 and r.converged
-    # the plain form keeps the reference rule
-    g = _ScriptedFlow([1e-5, 1e-7, 2e-9, 1.6e-10, 1.3e-10, 1.2e-10, 1.1e-10, 1.05e-10, 1.01e-10, 1.0e-10], tol=1e-10, max_it=10)
-    g.params = Params(corrector_tolerance=1e-10, max_corrector_iterations=10)
-    r = run(g, 0.0, 1)
-    assert r.counts == [9] and r.floor_stops == 0
+    assert r.counts == [6, 3] and r.escalations == 1 and r.converged
+    assert f.switches == [True, False]
+    assert r.errors == [2e-12, 5e-11]
+    assert not f.errs and not f.fp64
 
 
-def test_noise_floor_margin_after_a_stall():
-    """After a stall stop has shown the floor F, a pass within the margin
-    (2 F for the mixed form) of it ends the corrector without the confirming
-    pass; F then only decreases, until a new stall sets it again."""
-    f = _ScriptedFlow([1e-5, 1e-7, 2e-9, 1.6e-10, 1.3e-10,  # stall: F = 1.3e-10
-                       1e-5, 1e-7, 2e-9, 2.5e-10,  # <= 2 F: margin stop
-                       1e-5, 1e-7, 3e-10, 1.2e-10,  # 3e-10 > 2 F goes on; then F = 1.2e-10
-                       1e-5, 2.5e-10, 2.3e-10],  # > 2 F, then a stall: F = 2.3e-10
-                      tol=1e-10, max_it=10)
-    f.params = Params(precision="mixed", corrector_tolerance=1e-10, max_corrector_iterations=10)
-    assert f.params.corrector_floor_margin == 2.0
-    r = run(f, 0.0, 4)
-    assert r.counts == [4, 3, 3, 2] and r.floor_stops == 4 and r.converged
-    assert not f.errs
-    # without the margin the second step needs the confirming pass
-    g = _ScriptedFlow([1e-5, 1e-7, 2e-9, 1.6e-10, 1.3e-10, 1e-5, 1e-7, 2e-9, 2.5e-10, 1.3e-10],
-                      tol=1e-10, max_it=10)
-    g.params = Params(precision="mixed", corrector_tolerance=1e-10, max_corrector_iterations=10,
-                      corrector_floor_margin=0.0)
-    r = run(g, 0.0, 2)
-    assert r.counts == [4, 4] and r.floor_stops == 2
-
-
-def test_noise_floor_rule_ignores_slow_contraction():
-    """A loose tolerance with a corrector contracting only ~2x per pass (the
-    wall-bounded one) must iterate to the tolerance: a stall above the
-    absolute ceiling (1e-8) is not the noise floor."""
-    f = _ScriptedFlow([1e-3, 6e-4, 3.5e-4, 2e-4, 1.1e-4, 6e-5, 3.2e-5, 1.7e-5, 9e-6], tol=1e-5, max_it=10)
-    f.params = Params(precision="mixed", corrector_tolerance=1e-5, max_corrector_iterations=10)
-    assert f.params.floor_cap_value == 1e-8
+def test_escalation_needs_an_error_inside_the_window():
+    """Above the cap (min(1e4 tol, 1e-8)) a slow pass is slow contraction,
+    not FP32 resolution: no escalation, the reference rule alone."""
+    f = _EscalatingFlow([1e-3, 6e-4, 3.5e-4, 2e-4, 1.1e-4, 6e-5, 3.2e-5, 1.7e-5, 9e-6], [], tol=1e-5)
+    assert f.params.escalate_cap_value == 1e-8
     r = run(f, 0.0, 1)
-    assert r.counts == [8] and r.floor_stops == 0 and r.converged
+    assert r.counts == [8] and r.escalations == 0 and r.converged and f.switches == []
+
+
+def test_plain_form_never_escalates():
+    g = _ScriptedFlow([1e-5, 1e-7, 2e-9, 1.6e-10, 1.3e-10, 1.2e-10, 1.1e-10, 1.05e-10, 1.01e-10, 1.0e-10],
+                      tol=1e-10, max_it=10)
+    g.params = Params(corrector_tolerance=1e-10, max_corrector_iterations=10)
+    assert g.params.escalate_ratio is None
+    r = run(g, 0.0, 1)
+    assert r.counts == [9] and r.escalations == 0
+
+
+def test_escalated_step_that_still_stalls_ends_unconverged():
+    """Escalation does not relax the tolerance: a step whose FP64 passes do
+    not reach it either stops the run after max_corrector_iterations, as the
+    reference does."""
+    f = _EscalatingFlow([1e-7, 2e-9, 1.6e-10, 1.5e-10, 1.45e-10], [1.4e-10] * 4, max_it=8)
+    r = run(f, 0.0, 3)
+    assert r.counts == [8] and not r.converged and r.escalations == 1
 
 
 class _EnergyFlow(_ScriptedFlow):

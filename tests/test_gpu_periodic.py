@@ -5,6 +5,7 @@ relative, identical corrector iteration counts over a run; the mixed form
 to the tolerances stated in each test.
 """
 
+import functools
 import math
 import os
 
@@ -109,26 +110,45 @@ def test_norm_and_measure_corrections():
     assert abs(float(meas["div"]) - P.get_norm(corr, o.fourier.k_metric)) < 1e-10 * P.get_norm(corr, o.fourier.k_metric)
 
 
-def test_run_32cubed_counts_and_per_step_error():
-    """Config 0 (32^3, Re 40, dt 0.01, tol 1e-10, seed 0, RMS 1e-2): its full
-    100 steps against the oracle, identical counts, every step within 1e-12."""
-    from dnsjax_amd.driver import run as gpu_run
-    from dnsjax_amd.flows import TriplyPeriodicFlow
+_C0 = dict(n=32, L=2 * math.pi, re=40.0, dt=0.01, tol=1e-10, steps=100, rms=1e-2, seed=0)
+
+
+@functools.lru_cache(maxsize=1)
+def _c0_oracle():
+    """Config 0 (32^3, Re 40, dt 0.01, tol 1e-10, seed 0, RMS 1e-2) on the
+    oracle: its 100 states, counts and errors."""
     from dnsjax_amd.fields import periodic_field
 
-    L = 2 * math.pi
-    p = Params(system="kolmogorov", nx=32, ny=32, nz=32, lx=L, ly=L, lz=L, re=40.0, dt=0.01,
-               corrector_tolerance=1e-10, max_corrector_iterations=10)
-    flow = TriplyPeriodicFlow(p)
-    u0 = periodic_field(32, 32, 32, L, L, L, 1e-2, seed=0)
-    op = P.PeriodicParams(nx=32, ny=32, nz=32, lx=L, ly=L, lz=L, re=40.0, dt=0.01)
+    n, L = _C0["n"], _C0["L"]
+    u0 = periodic_field(n, n, n, L, L, L, _C0["rms"], seed=_C0["seed"])
+    op = P.PeriodicParams(nx=n, ny=n, nz=n, lx=L, ly=L, lz=L, re=_C0["re"], dt=_C0["dt"])
     of = P.Flow(op)
     pc, ic = S.make_stepper(P.get_rhs, P.predict, P.correct, P.norm)
     ref_states = []
-    _, ref_counts, ref_errs = S.run(u0, 100, 1e-10, 10, lambda s: pc(s, of), lambda a, b, c: ic(a, b, c, of),
-                                    lambda s: P.correct_velocity(s, of), callback=lambda i, s: ref_states.append(s))
+    _, ref_counts, ref_errs = S.run(u0, _C0["steps"], _C0["tol"], 10, lambda s: pc(s, of),
+                                    lambda a, b, c: ic(a, b, c, of), lambda s: P.correct_velocity(s, of),
+                                    callback=lambda i, s: ref_states.append(s))
+    return u0, of, ref_states, ref_counts, ref_errs
+
+
+def _c0_flow(precision):
+    from dnsjax_amd.flows import TriplyPeriodicFlow
+
+    n, L = _C0["n"], _C0["L"]
+    return TriplyPeriodicFlow(Params(system="kolmogorov", nx=n, ny=n, nz=n, lx=L, ly=L, lz=L, re=_C0["re"],
+                                     dt=_C0["dt"], corrector_tolerance=_C0["tol"], max_corrector_iterations=10,
+                                     precision=precision))
+
+
+def test_run_32cubed_counts_and_per_step_error():
+    """Config 0: its full 100 steps against the oracle, identical counts,
+    every step within 1e-12."""
+    from dnsjax_amd.driver import run as gpu_run
+
+    u0, _, ref_states, ref_counts, ref_errs = _c0_oracle()
+    flow = _c0_flow("fp64")
     gpu_states = []
-    r = gpu_run(flow, flow.to_device(u0), 100, callback=lambda i, s: gpu_states.append(s.cpu().numpy()))
+    r = gpu_run(flow, flow.to_device(u0), _C0["steps"], callback=lambda i, s: gpu_states.append(s.cpu().numpy()))
     assert len(r.counts) == 100 and r.converged
     assert r.counts == ref_counts
     worst = max(rel(a, b) for a, b in zip(gpu_states, ref_states))
@@ -136,6 +156,30 @@ def test_run_32cubed_counts_and_per_step_error():
     assert worst < 1e-12, worst
     for a, b in zip(r.errors, ref_errs):
         assert abs(a - b) <= 1e-6 * b + 1e-300
+
+
+def test_mixed_run_32cubed_against_the_oracle():
+    """Config 0 in the mixed form (split-precision FP32 evaluations) over its
+    100 steps: every step's corrector reaches the tolerance without
+    escalating to FP64, and the end state's energy, dissipation and
+    velocity stay within 1e-6 relative of the FP64 oracle's."""
+    from dnsjax_amd.driver import run as gpu_run
+
+    u0, of, ref_states, ref_counts, _ = _c0_oracle()
+    flow = _c0_flow("mixed")
+    r = gpu_run(flow, flow.to_device(u0), _C0["steps"])
+    assert len(r.counts) == 100 and r.converged and r.escalations == 0
+    assert max(r.errors) <= _C0["tol"]
+    end = r.state.cpu().numpy()
+    st, ref = flow.get_stats(r.state), P.get_stats(ref_states[-1], of)
+    e_err = abs(float(st["E'"]) - ref["E'"]) / abs(ref["E'"])
+    d_err = abs(float(st["D"]) - ref["D"]) / abs(ref["D"])
+    v_err = rel(end, ref_states[-1])
+    diff = sum(abs(a - b) for a, b in zip(r.counts, ref_counts))
+    print(f"MEASURED c0_mixed energy {e_err:.2e} dissipation {d_err:.2e} velocity {v_err:.2e} "
+          f"extra iterations {sum(r.counts)} (oracle {sum(ref_counts)}, per-step differences {diff})")
+    assert e_err < 1e-6 and d_err < 1e-6 and v_err < 1e-6
+    assert abs(sum(r.counts) - sum(ref_counts)) <= 5
 
 
 def test_mixed_precision_close_to_fp64():

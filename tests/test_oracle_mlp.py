@@ -223,3 +223,62 @@ def test_wall_guess_mode2_adds_the_net_residual():
     for c in range(3):
         want = ext3[c] + s[2] * (Y[:, 2 * c] + 1j * Y[:, 2 * c + 1]).reshape(w.shape) / w
         assert np.allclose(ext2[c], want, rtol=1e-14, atol=1e-14)
+
+
+def test_profile_features_on_polynomial_histories():
+    """The per-(kz, kx) profile network's rows: a quadratic history leaves no
+    target; a cubic one leaves its third difference (6 e per unit step)
+    weighted by w, in the [component][real, imaginary][y] order; the D1 and
+    D2 blocks and the mode features read back what index arithmetic over
+    (kz, kx) and y gives."""
+    from dnsjax_amd.fields import wall_field
+    from oracle import walls as W
+
+    flow = _wall_flow()
+    p = flow.p
+    ny, nkx = p.ny, p.nx // 2
+    din = M.profile_din(ny)
+    assert din % 32 == 0 and din >= 12 * ny + 3
+    f = lambda s: _f32(W.get_rhs(wall_field(8, 17, 8, p.lx, p.lz, 0.05, seed=s), flow))  # noqa: E731
+    a, b, e = f(3), f(9) * 0.25, f(11) * 0.125
+    modes = np.arange((p.nz - 1) * nkx)
+    quad = [a + k * b + k * k * e for k in range(4)]
+    X, T = M.features_walls_profile(*quad, modes, flow, 1.0, 1.0, 1.0, din)
+    scale = np.max(np.abs(b))
+    assert T.shape == (len(modes), 6 * ny) and X.shape == (len(modes), din)
+    assert np.max(np.abs(T)) <= 1e-12 * scale
+    iz, ix = np.divmod(modes, nkx)
+    # D1 = b + 3 e at k = 3 (component 1, imaginary part: block 2 * 1 + 1)
+    assert np.allclose(X[:, 3 * ny:4 * ny], (b[1] + 3 * e[1])[iz, ix].imag, atol=1e-6 * scale)
+    # D2 = 2 e (component 2, real part, in the D2 block)
+    assert np.allclose(X[:, 6 * ny + 4 * ny:6 * ny + 5 * ny], (2 * e[2])[iz, ix].real, atol=1e-6 * scale)
+    cub = [a + k * k * k * e for k in range(4)]
+    _, T3 = M.features_walls_profile(*cub, modes, flow, 1.0, 1.0, 2.0, din)
+    kx = 2 * np.pi / p.lx * ix
+    kz = 2 * np.pi / p.lz * np.where(iz < p.nz // 2, iz, iz - (p.nz - 1))
+    w = 1.0 / (1.0 + p.implicitness * p.dt * (kx**2 + kz**2) / p.re)
+    want = w[:, None] * (6 * e[0])[iz, ix].real / 2.0
+    assert np.allclose(T3[:, 0:ny], want, rtol=1e-12, atol=1e-12 * np.max(np.abs(want)))
+    kmax = max(2 * np.pi / p.lx * nkx, 2 * np.pi / p.lz * (p.nz // 2))
+    assert np.allclose(X[:, 12 * ny], np.minimum(p.dt * (kx**2 + kz**2) / p.re, 8.0))
+    assert np.allclose(X[:, 12 * ny + 1], kx / kmax) and np.allclose(X[:, 12 * ny + 2], kz / kmax)
+    assert not X[:, 12 * ny + 3:].any()
+
+
+def test_profile_loss_scales_the_pinned_gradients():
+    """The profile network's loss is the mean over samples and outputs: its
+    gradients are the finite-difference-pinned ``loss_and_grads`` over dout,
+    checked here by a central difference on one weight of each layer."""
+    rng = np.random.default_rng(4)
+    params = M.init(64, 32, 24, seed=5)
+    x, t = rng.standard_normal((10, 64)), rng.standard_normal((10, 24))
+    loss, g = M.profile_loss_and_grads(params, x, t, bf16=False)
+    for li, idx in ((0, (3, 7)), (2, (5, 1)), (4, (30, 20)), (5, (2,))):
+        h = 1e-6
+        pp = [q.copy() for q in params]
+        pp[li][idx] += h
+        lp, _ = M.profile_loss_and_grads(pp, x, t, bf16=False)
+        pp[li][idx] -= 2 * h
+        lm, _ = M.profile_loss_and_grads(pp, x, t, bf16=False)
+        assert abs((lp - lm) / (2 * h) - g[li][idx]) <= 1e-6 * max(1.0, abs(g[li][idx]))
+    assert abs(loss - np.sum((M.forward(params, x)[0] - t) ** 2) / (10 * 24)) < 1e-12
