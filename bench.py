@@ -254,6 +254,10 @@ def learned_form(steps: int, collect_steps: int = 30, hidden: int = 32, train_it
     X, T = make_dataset(mixed, snaps, sc, 100000, seed=1)
     nv = 100000
     Xv, Tv, X, T = X[-nv:], T[-nv:], X[:-nv], T[:-nv]
+    # the same network on the residual of cubic extrapolation
+    sc3 = fit_scales(snaps, cubic=True)
+    X3, T3 = make_dataset(mixed, snaps, sc3, 100000, seed=1)
+    X3v, T3v, X3, T3 = X3[-nv:], T3[-nv:], X3[:-nv], T3[:-nv]
     del snaps
     torch.cuda.synchronize()
     t_data = time.perf_counter() - t0
@@ -263,6 +267,11 @@ def learned_form(steps: int, collect_steps: int = 30, hidden: int = 32, train_it
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
     loss, base = mean_loss(net, Xv, Tv)
+    net3 = Net(hidden, seed=0)
+    train(net3, X3, T3, iters=train_iters, batch=16384, lr=3e-3)
+    loss3, base3 = mean_loss(net3, X3v, T3v)
+    guesses = {"mixed_quadratic": (3, None, None), "mixed_learned": (2, net, sc),
+               "mixed_cubic": (4, None, None), "mixed_learned_cubic": (2, net3, sc3)}
 
     def timed(fn):
         torch.cuda.synchronize()
@@ -274,15 +283,15 @@ def learned_form(steps: int, collect_steps: int = 30, hidden: int = 32, train_it
     # one untimed pass of each form first (allocator growth, module loads)
     run(fp64, u_mid.clone(), 2)
     run(mixed, u_mid.clone(), 2)
-    for mode in (3, 2):
-        run_guessed(mixed, u_mid.clone(), 3, mode=mode, net=net, scales=sc)
+    for mode, nn, ss in guesses.values():
+        run_guessed(mixed, u_mid.clone(), 4, mode=mode, net=nn, scales=ss)
     rows = {}
     ref, t_ref = timed(lambda: run(fp64, u_mid.clone(), steps))
     rows["fp64_plain"] = (t_ref, ref.rhs_evals, ref.converged, ref.state, steps)
     r, t = timed(lambda: run(mixed, u_mid.clone(), steps))
     rows["mixed_plain"] = (t, r.rhs_evals, r.converged, r.state, steps)
-    for name, mode in (("mixed_quadratic", 3), ("mixed_learned", 2)):
-        g, t = timed(lambda: run_guessed(mixed, u_mid.clone(), steps, mode=mode, net=net, scales=sc))
+    for name, (mode, nn, ss) in guesses.items():
+        g, t = timed(lambda: run_guessed(mixed, u_mid.clone(), steps, mode=mode, net=nn, scales=ss))
         rows[name] = (t, g.rhs_evals, g.converged, g.state, steps)
     # the same forms with the corrector decided on the device, one captured
     # graph launch per step (no host read of the error per pass)
@@ -300,12 +309,15 @@ def learned_form(steps: int, collect_steps: int = 30, hidden: int = 32, train_it
         st = gp.step_stats()[0]
         rows["fp64_plain_graph" + tag] = (t, sum(c + 2 for c in gp.counts()[0][2:]), st["unconverged"] == 0,
                                           gp.state, gp.steps)
-        gg = GuessedGraphStepper(mixed, u_mid, mode=2, net=net, scales=sc, slots=slots, capacity=steps + 4)
-        gg.step(2)
-        torch.cuda.synchronize()
-        _, t = timed(lambda: gg.step(steps))
-        rows["mixed_learned_graph" + tag] = (t, sum(c + 1 for c in gg.passes()[4:]),
-                                             gg.step_stats()["unconverged"] == 0, gg.state, gg.steps)
+        for name in ("mixed_learned", "mixed_cubic", "mixed_learned_cubic"):
+            mode, nn, ss = guesses[name]
+            gg = GuessedGraphStepper(mixed, u_mid, mode=mode, net=nn, scales=ss, slots=slots, capacity=steps + 5)
+            gg.step(2)
+            torch.cuda.synchronize()
+            k0 = gg.steps
+            _, t = timed(lambda: gg.step(steps))
+            rows[name + "_graph" + tag] = (t, sum(c + 1 for c in gg.passes()[k0:]),
+                                           gg.step_stats()["unconverged"] == 0, gg.state, gg.steps)
     # The graph steppers' end states include their start-up and warm-up
     # steps: each is compared with an eager FP64 run of the same length
     # (energy E', dissipation D and the velocity field).
@@ -336,7 +348,10 @@ def learned_form(steps: int, collect_steps: int = 30, hidden: int = 32, train_it
     long_forms = {}
     for name, fn in (("mixed", lambda: run(mixed, u0.clone(), long_n)),
                      ("mixed_quadratic", lambda: run_guessed(mixed, u0.clone(), long_n, mode=3)),
-                     ("mixed_learned", lambda: run_guessed(mixed, u0.clone(), long_n, mode=2, net=net, scales=sc))):
+                     ("mixed_learned", lambda: run_guessed(mixed, u0.clone(), long_n, mode=2, net=net, scales=sc)),
+                     ("mixed_cubic", lambda: run_guessed(mixed, u0.clone(), long_n, mode=4)),
+                     ("mixed_learned_cubic",
+                      lambda: run_guessed(mixed, u0.clone(), long_n, mode=2, net=net3, scales=sc3))):
         rr, tt = timed(fn)
         a, b = fp64.get_stats(rr.state), fp64.get_stats(long_ref.state)
         rel_ = lambda k: abs(float(a[k]) - float(b[k])) / abs(float(b[k]))  # noqa: E731
@@ -350,17 +365,18 @@ def learned_form(steps: int, collect_steps: int = 30, hidden: int = 32, train_it
     long_run = {"steps": long_n, "from": "the seeded initial state", "fp64_steps_per_s": round(long_n / t_lr, 3),
                 "fp64_rhs_evals_per_step": round(long_ref.rhs_evals / long_n, 3),
                 "fp64_converged": long_ref.converged, **long_forms}
-    sp = out["mixed_learned"]["steps_per_s"] / out["fp64_plain"]["steps_per_s"]
-    spg = out["mixed_learned_graph"]["steps_per_s"] / out["fp64_plain_graph"]["steps_per_s"]
-    spg4 = out["mixed_learned_graph_4slots"]["steps_per_s"] / out["fp64_plain_graph_4slots"]["steps_per_s"]
-    return {"forms": out, "long_run": long_run, "speedup_vs_fp64": round(sp, 3),
-            "speedup_vs_fp64_graph": round(spg, 3),
-            "speedup_vs_fp64_graph_4slots": round(spg4, 3),
-            "speedup_vs_mixed_quadratic": round(out["mixed_learned"]["steps_per_s"]
-                                                / out["mixed_quadratic"]["steps_per_s"], 3),
+    rate = lambda k: out[k]["steps_per_s"]  # noqa: E731
+    speedups = {}
+    for name in guesses:
+        speedups[name] = round(rate(name) / rate("fp64_plain"), 3)
+        for tag in ("_graph", "_graph_4slots"):
+            if name + tag in out:
+                speedups[name + tag] = round(rate(name + tag) / rate("fp64_plain" + tag), 3)
+    return {"forms": out, "long_run": long_run, "speedup_vs_fp64": speedups,
             "net": {"hidden": hidden, "train_iters": train_iters, "train_s": round(t_train, 3),
                     "data_s": round(t_data, 3), "samples": int(X.shape[0]),
-                    "heldout_loss": round(loss, 4), "heldout_loss_without_net": round(base, 4)},
+                    "heldout_loss": round(loss, 4), "heldout_loss_without_net": round(base, 4),
+                    "cubic_heldout_loss": round(loss3, 4), "cubic_heldout_loss_without_net": round(base3, 4)},
             "segment": f"{steps} steps after a {collect_steps}-step training run from the workload's initial state"}
 
 
@@ -710,15 +726,18 @@ def walls_large_form(steps: int, collect_steps: int = 8, train_iters: int = 300,
                      ("mixed_learned", lambda: run_guessed(mixed, u0.clone(), steps, mode=2, net=net, scales=sc))):
         r, t = timed(fn)
         e = float(fp64.get_stats(r.state)["E'"])
-        forms[name] = {"steps_per_s": round(steps / t, 3), "speedup_vs_fp64": round(tp / t, 3),
+        forms[name] = {"steps_per_s": round(len(r.errors) / t, 3),
+                       "speedup_vs_fp64": round((tp / len(plain.counts)) / (t / len(r.errors)), 3),
                        "rhs_evals_per_step": round(r.rhs_evals / len(r.errors), 3), "converged": r.converged,
                        "steps_run": len(r.errors), "escalations": r.escalations,
                        "energy_rel_err_vs_fp64": abs(e - e_p) / abs(e_p)}
     return {"workload": "plane Couette 128x256x128, p=6, Re 1000, dt 0.005, RMS 0.05 (seed 4), tol 1e-10",
             "steps": steps, "from": "the seeded initial state", "setup_s": round(t_setup, 2),
-            "fp64": {"steps_per_s": round(steps / tp, 3), "rhs_evals_per_step": round(plain.rhs_evals / steps, 3),
-                     "corrector_iters_per_step": round(sum(plain.counts) / steps, 3),
-                     "converged": plain.converged, "tuning": fp64._native.tuning.strip()},
+            "fp64": {"steps_per_s": round(len(plain.counts) / tp, 3),
+                     "rhs_evals_per_step": round(plain.rhs_evals / len(plain.counts), 3),
+                     "corrector_iters_per_step": round(sum(plain.counts) / len(plain.counts), 3),
+                     "converged": plain.converged, "steps_run": len(plain.counts),
+                     "tuning": fp64._native.tuning.strip()},
             **forms,
             "speedup_vs_fp64": forms["mixed_learned"]["speedup_vs_fp64"],
             "net": {"kind": "per-(kz, kx) profile MLP", "din": net.din, "hidden": [net.H, net.H], "dout": net.dout,
@@ -779,8 +798,9 @@ def main():
                     help="steps of each form in the mixed+learned A/B (0 skips it)")
     ap.add_argument("--ensemble-steps", type=int, default=50,
                     help="steps of the 64-member ensemble form (0 skips it)")
-    ap.add_argument("--walls-large-steps", type=int, default=1000,
-                    help="steps of the plane Couette 128x256x128 learned-guess configuration (0 skips it)")
+    ap.add_argument("--walls-large-steps", type=int, default=800,
+                    help="steps of the plane Couette 128x256x128 learned-guess configuration (0 skips "
+                         "it; its FP64 corrector stops converging within 20 iterations near step 835)")
     ap.add_argument("--profile-walls", action="store_true",
                     help="run a few steps of the plane Couette form for rocprofv3 and exit")
     ap.add_argument("--profile-workload", action="store_true",
@@ -875,7 +895,7 @@ def main():
             f: {k: lr[f][k] for k in ("speedup_vs_fp64", "rhs_evals_per_step", "converged", "escalations",
                                       "energy_rel_err_vs_fp64", "dissipation_rel_err_vs_fp64",
                                       "velocity_rel_l2_vs_fp64")}
-            for f in ("mixed", "mixed_quadratic", "mixed_learned")}
+            for f in ("mixed", "mixed_quadratic", "mixed_learned", "mixed_cubic", "mixed_learned_cubic")}
         summ["periodic_500_steps"]["fp64_rhs_evals_per_step"] = lr["fp64_rhs_evals_per_step"]
     if "walls" in out:
         summ["walls_200_steps"] = {t: {f: {k: w["long_run"][f][k] for k in ("speedup_vs_fp64", "converged",
@@ -884,10 +904,10 @@ def main():
                                    for t, w in (("tol_1e-5", out["walls"]), ("tol_1e-8", out["walls_tol_1e-8"]))}
     if "walls_large" in out:
         wl = out["walls_large"]
-        summ["walls_128x256x128_1000_steps"] = {
+        summ[f"walls_128x256x128_{wl['fp64']['steps_run']}_steps"] = {
             f: {k: wl[f][k] for k in ("speedup_vs_fp64", "rhs_evals_per_step", "converged",
                                       "energy_rel_err_vs_fp64")} for f in ("mixed", "mixed_quadratic", "mixed_learned")}
-        summ["walls_128x256x128_1000_steps"]["fp64_rhs_evals_per_step"] = wl["fp64"]["rhs_evals_per_step"]
+        summ[f"walls_128x256x128_{wl['fp64']['steps_run']}_steps"]["fp64_rhs_evals_per_step"] = wl["fp64"]["rhs_evals_per_step"]
     if "ensemble" in out:
         summ["ensemble_member_steps_per_s"] = out["ensemble"]["value"]
         summ["ensemble_speedup_vs_sequential"] = out["ensemble"]["speedup_vs_sequential"]

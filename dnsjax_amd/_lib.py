@@ -96,11 +96,11 @@ SIGNATURES = {
                                  ctypes.c_float, ctypes.c_float, _P]),
     "dnsmi_mlp_pack": (_I, [_I, _P, _P, _P, _P]),
     "dnsmi_mlp_forward": (_I, [_I, _P, _I, _P, _P, _P, _P]),
-    "dnsmi_mlp_features": (_I, [_P, _P, _P, _P, _P, _P, _I, ctypes.c_float, ctypes.c_float,
+    "dnsmi_mlp_features": (_I, [_P, _P, _P, _P, _P, _P, _P, _I, ctypes.c_float, ctypes.c_float,
                                 ctypes.c_float, _P, _P, _P]),
-    "dnsmi_guess": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _I, ctypes.c_float, ctypes.c_float,
+    "dnsmi_guess": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _I, ctypes.c_float, ctypes.c_float,
                          ctypes.c_float, _I, _P]),
-    "dnsmi_step_device_guess": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _I, ctypes.c_float, ctypes.c_float,
+    "dnsmi_step_device_guess": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, ctypes.c_float, ctypes.c_float,
                                      ctypes.c_float, _I, _I, _I, _P, _I, ctypes.c_double, _I, ctypes.c_double,
                                      ctypes.c_double, _P]),
 }

@@ -81,7 +81,7 @@ int step_device(Ctx* c, const double* state, double* pred, double* rhs_a, double
                 double esc_ratio, double esc_cap, hipStream_t s);
 int step_read_ctl(Ctx* c, double* out6);
 int step_device_guess(Ctx* c, const double* state, double* pred, double* rhs_a, double* rhs_b,
-                      const float* h1, float* h2, const uint16_t* wb, const float* bias, int H,
+                      float* h1, float* h2, float* h3, const uint16_t* wb, const float* bias, int H,
                       float s_1, float s_2, float s_out, int mode, int slots, int esc_slots,
                       int* counts, int counts_cap, double tol, int max_pass, double esc_ratio,
                       double esc_cap, hipStream_t s);
@@ -109,11 +109,11 @@ int guess_walls_profile(Ctx* c, const double* state, double* pred, double* rhs_p
 int mlp_pack(int H, const float* prm, uint16_t* wb, float* bias, hipStream_t s);
 int mlp_forward(int H, const float* X, int B, const uint16_t* wb, const float* bias, float* Y,
                 hipStream_t s);
-int mlp_features(Ctx* c, const double* nm2, const double* nm1, const double* ncur,
+int mlp_features(Ctx* c, const double* nm3, const double* nm2, const double* nm1, const double* ncur,
                  const double* nnext, const long long* modes, int ns, float s_1, float s_2,
                  float s_out, float* X, float* T, hipStream_t s);
-int guess_periodic(Ctx* c, const double* state, double* pred, double* rhs_prev, const float* h1,
-                   float* h2, const uint16_t* wb, const float* bias, int H, float s_1, float s_2,
+int guess_periodic(Ctx* c, const double* state, double* pred, double* rhs_prev, float* h1,
+                   float* h2, float* h3, const uint16_t* wb, const float* bias, int H, float s_1, float s_2,
                    float s_out, int mode, hipStream_t s);
 int guess_walls(Ctx* c, const double* state, double* pred, double* rhs_prev, const float* h1,
                 float* h2, const uint16_t* wb, const float* bias, int H, float s_1, float s_2,
@@ -406,7 +406,7 @@ int dnsmi_mlp_forward(int H, const float* X, int B, const void* packed_weights, 
                  "mlp_forward");
 }
 
-int dnsmi_mlp_features(dnsmi_ctx* ctx, const double* n_m2, const double* n_m1,
+int dnsmi_mlp_features(dnsmi_ctx* ctx, const double* n_m3, const double* n_m2, const double* n_m1,
                        const double* n_cur, const double* n_next, const long long* modes,
                        int nsamples, float s_1, float s_2, float s_out, float* X, float* T,
                        void* stream) {
@@ -414,7 +414,8 @@ int dnsmi_mlp_features(dnsmi_ctx* ctx, const double* n_m2, const double* n_m1,
   DNSMI_REQUIRE(n_m2 && n_m1 && n_cur && modes && X && nsamples >= 0,
                 "mlp_features: bad arguments");
   DNSMI_REQUIRE(!T == !n_next, "mlp_features: targets need n_next and T together");
-  return dnsmi::mlp_features(c, n_m2, n_m1, n_cur, n_next, modes, nsamples, s_1, s_2, s_out, X, T,
+  DNSMI_REQUIRE(!n_m3 || !c->walls, "mlp_features: cubic targets are for periodic contexts");
+  return dnsmi::mlp_features(c, n_m3, n_m2, n_m1, n_cur, n_next, modes, nsamples, s_1, s_2, s_out, X, T,
                              as_stream(stream));
 }
 
@@ -443,24 +444,25 @@ int dnsmi_guess_profile(dnsmi_ctx* ctx, const double* state, double* pred, doubl
 }
 
 int dnsmi_guess(dnsmi_ctx* ctx, const double* state, double* pred, double* rhs_prev,
-                const float* hist1, float* hist2, const void* packed_weights, const float* bias,
+                float* hist1, float* hist2, float* hist3, const void* packed_weights, const float* bias,
                 int H, float s_1, float s_2, float s_out, int mode, void* stream) {
   DNSMI_CHECK_CTX(ctx)
   DNSMI_REQUIRE(state && pred && rhs_prev && hist1 && hist2, "guess: null pointer");
   DNSMI_REQUIRE(mode != 2 || (packed_weights && bias), "guess: network weights missing");
   DNSMI_REQUIRE((const void*)state != pred && (const void*)state != rhs_prev &&
-                    (void*)pred != rhs_prev && (const void*)hist1 != hist2,
+                    (void*)pred != rhs_prev && hist1 != hist2 && hist1 != hist3 && hist2 != hist3,
                 "guess: buffers must be distinct");
+  DNSMI_REQUIRE(!hist3 || !c->walls, "guess: hist3 (cubic extrapolation) is for periodic contexts");
   const uint16_t* wb = static_cast<const uint16_t*>(packed_weights);
   if (c->walls)
     return dnsmi::guess_walls(c, state, pred, rhs_prev, hist1, hist2, wb, bias, H, s_1, s_2, s_out,
                               mode, as_stream(stream));
-  return dnsmi::guess_periodic(c, state, pred, rhs_prev, hist1, hist2, wb, bias, H, s_1, s_2,
+  return dnsmi::guess_periodic(c, state, pred, rhs_prev, hist1, hist2, hist3, wb, bias, H, s_1, s_2,
                                s_out, mode, as_stream(stream));
 }
 
 int dnsmi_step_device_guess(dnsmi_ctx* ctx, const double* state, double* next, double* rhs_a,
-                            double* rhs_b, const float* hist1, float* hist2,
+                            double* rhs_b, float* hist1, float* hist2, float* hist3,
                             const void* packed_weights, const float* bias, int H, float s_1,
                             float s_2, float s_out, int mode, int slots, int esc_slots,
                             int* counts, int counts_cap, double tol, int max_pass,
@@ -471,9 +473,10 @@ int dnsmi_step_device_guess(dnsmi_ctx* ctx, const double* state, double* next, d
   DNSMI_REQUIRE(slots >= 0 && esc_slots >= 0 && max_pass >= 1 && (counts || counts_cap == 0),
                 "step_device_guess: bad slots / max_pass / counts");
   DNSMI_REQUIRE((const void*)state != next && (void*)rhs_a != rhs_b && (void*)next != rhs_a &&
-                    (void*)next != rhs_b && (const void*)hist1 != hist2,
+                    (void*)next != rhs_b && hist1 != hist2 && hist1 != hist3 && hist2 != hist3,
                 "step_device_guess: buffers must be distinct");
-  return dnsmi::step_device_guess(c, state, next, rhs_a, rhs_b, hist1, hist2,
+  DNSMI_REQUIRE(!hist3 || !c->walls, "step_device_guess: hist3 (cubic extrapolation) is for periodic contexts");
+  return dnsmi::step_device_guess(c, state, next, rhs_a, rhs_b, hist1, hist2, hist3,
                                   static_cast<const uint16_t*>(packed_weights), bias, H, s_1, s_2,
                                   s_out, mode, slots, esc_slots, counts, counts_cap, tol, max_pass,
                                   esc_ratio, esc_cap, as_stream(stream));

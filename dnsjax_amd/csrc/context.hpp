@@ -87,9 +87,9 @@ struct Ctx {
   double* rhs0 = nullptr;      // N^0 of the predictor (spectral vector field)
 
   // Split-precision evaluations of the mixed form (DNSMI_MIXED, unless
-  // cfg.plain_mixed): every evaluation of a step after its first computes
-  // only the increment over the step's first state u_ref in FP32, so the
-  // FP32 rounding scales with the increment, not with N.
+  // cfg.plain_mixed): the step's first corrector evaluation records u_ref
+  // (its state), and every later one computes only the increment over it in
+  // FP32, so the FP32 rounding scales with the increment, not with N.
   bool split = false;
   bool ref_valid = false;         // u_ref / x_ref / n_ref hold one evaluation
   double* uref = nullptr;         // u_ref, spectral FP64 (spec_elems complex)

@@ -395,9 +395,12 @@ __device__ __forceinline__ void diffs(double cur, float h1, float h2, double& d1
   d2 = (cur - 2.0 * p1) + p2;
 }
 
-// Training samples from four successive nonlinear terms (FP64 spectral
-// fields); the two older ones are rounded to FP32 as the history keeps them.
-__global__ void k_mlp_features(FeatArgs a, const cx<double>* __restrict__ nm2,
+// Training samples from successive nonlinear terms (FP64 spectral fields);
+// the older ones are rounded to FP32 as the history keeps them in the
+// features.  With nm3 the target is the residual of cubic extrapolation,
+// N^{n+1} - (4 N^n - 6 N^{n-1} + 4 N^{n-2} - N^{n-3}), else of quadratic.
+__global__ void k_mlp_features(FeatArgs a, const cx<double>* __restrict__ nm3,
+                               const cx<double>* __restrict__ nm2,
                                const cx<double>* __restrict__ nm1,
                                const cx<double>* __restrict__ ncur,
                                const cx<double>* __restrict__ nnext,
@@ -424,25 +427,34 @@ __global__ void k_mlp_features(FeatArgs a, const cx<double>* __restrict__ nm2,
     for (int q = 0; q < 6; ++q) {
       const long long e = (q / 2) * a.n + i;
       const cx<double> p2 = nm2[e], p1 = nm1[e], c = ncur[e], x = nnext[e];
-      const double d3 = (q & 1) ? ((x.y - 3 * c.y) + (3 * p1.y - p2.y))
-                                : ((x.x - 3 * c.x) + (3 * p1.x - p2.x));
-      T[(size_t)s * kDout + q] = (float)(w * d3 / a.s_out);
+      double r;
+      if (nm3) {
+        const cx<double> p3 = nm3[e];
+        r = (q & 1) ? ((x.y - 4 * c.y) + (6 * p1.y - 4 * p2.y) + p3.y)
+                    : ((x.x - 4 * c.x) + (6 * p1.x - 4 * p2.x) + p3.x);
+      } else {
+        r = (q & 1) ? ((x.y - 3 * c.y) + (3 * p1.y - p2.y)) : ((x.x - 3 * c.x) + (3 * p1.x - p2.x));
+      }
+      T[(size_t)s * kDout + q] = (float)(w * r / a.s_out);
     }
   }
 }
 
 // ------------------------------------------------------------------ guess
-// ncur = N^n (FP64, from the RHS pipeline); h1 = N^{n-1}
-// and h2 = N^{n-2} (FP32 history).  Per mode, with
+// ncur = N^n (FP64, from the RHS pipeline); h1 = N^{n-1}, h2 = N^{n-2} and,
+// if given, h3 = N^{n-3} (FP32 history).  Per mode, with
 //   mode 0: dN = 0                    (the plain predictor)
 //   mode 1: dN = D1                   (linear extrapolation of N)
 //   mode 3: dN = D1 + D2              (quadratic extrapolation)
-//   mode 2: dN = D1 + D2 + s_out net(x) / w   (learned residual)
+//   mode 4: dN = D1 + D2 + D3         (cubic extrapolation; needs h3)
+//   mode 2: dN = base + s_out net(x) / w  (learned residual; base cubic
+//           with h3, quadratic without)
 // and dN Leray-projected (zero at the mean mode):
 //   pred     <- u^1 + c ildt_2 dN, u^1 = (u^n ldt_1 + N^n) ildt_2
 //   rhs_prev <- N^n + dN                    (so that the incremental
 //               corrector from the guess is the same fixed-point map)
-//   h2       <- N^n                         (the caller then swaps h1, h2)
+//   history  <- shifted: with h3, h3 <- h2, h2 <- h1, h1 <- N^n in place;
+//               without, h2 <- N^n (the caller then swaps h1, h2)
 // One lane per mode: each lane reads its mode's N^n and history once
 // (coalesced), forms the features and dN; the network runs on the wave's
 // 64 modes as four 16-mode matrix-core batches, with operands and outputs
@@ -450,8 +462,8 @@ __global__ void k_mlp_features(FeatArgs a, const cx<double>* __restrict__ nm2,
 template <int H>
 __global__ void __launch_bounds__(256)
     k_guess_periodic(FeatArgs a, const cx<double>* __restrict__ state,
-                     const cx<double>* __restrict__ ncur, const float* __restrict__ h1,
-                     float* __restrict__ h2, double* __restrict__ pred,
+                     const cx<double>* __restrict__ ncur, float* __restrict__ h1,
+                     float* __restrict__ h2, float* __restrict__ h3, double* __restrict__ pred,
                      double* __restrict__ rhs_prev, const uint16_t* __restrict__ wb,
                      const float* __restrict__ bias, int mode) {
   constexpr int NT = 2 * (H / 16) + (H / 16) * (H / 16);
@@ -470,15 +482,24 @@ __global__ void __launch_bounds__(256)
   const long long ii = live ? i : 0;
   double kx, ky, kz;
   coords(a, (unsigned)ii, kx, ky, kz);
-  cx<double> nc[3], d1[3], d2[3];
+  cx<double> nc[3], d1[3], d2[3], d3[3];
+  float2 hp1[3], hp2[3];
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
     const long long e = q * a.n + ii;
     nc[q] = live ? ncur[e] : cx<double>{0, 0};
     const float2 p1 = live ? reinterpret_cast<const float2*>(h1)[e] : float2{0.f, 0.f};
     const float2 p2 = live ? reinterpret_cast<const float2*>(h2)[e] : float2{0.f, 0.f};
+    hp1[q] = p1;
+    hp2[q] = p2;
     diffs(nc[q].x, p1.x, p2.x, d1[q].x, d2[q].x);
     diffs(nc[q].y, p1.y, p2.y, d1[q].y, d2[q].y);
+    d3[q] = {0.0, 0.0};
+    if (h3) {  // D3 = N^n - 3 N^{n-1} + 3 N^{n-2} - N^{n-3}
+      const float2 p3 = live ? reinterpret_cast<const float2*>(h3)[e] : float2{0.f, 0.f};
+      d3[q] = {((nc[q].x - 3.0 * (double)p1.x) + 3.0 * (double)p2.x) - (double)p3.x,
+               ((nc[q].y - 3.0 * (double)p1.y) + 3.0 * (double)p2.y) - (double)p3.y};
+    }
   }
   if (use_net) {
 #pragma unroll
@@ -520,6 +541,7 @@ __global__ void __launch_bounds__(256)
       dn[q] = d1[q];
     else
       dn[q] = {d1[q].x + d2[q].x, d1[q].y + d2[q].y};
+    if (mode == 4 || (mode == 2 && h3)) dn[q] = {dn[q].x + d3[q].x, dn[q].y + d3[q].y};
     if (use_net && !mean) {
       dn[q].x += (double)a.s_out * outs[wave][lane][2 * q] * inv_w;
       dn[q].y += (double)a.s_out * outs[wave][lane][2 * q + 1] * inv_w;
@@ -547,7 +569,13 @@ __global__ void __launch_bounds__(256)
     const cx<double> u1 = {(u.x * ldt1 + nc[q].x) * ildt2, (u.y * ldt1 + nc[q].y) * ildt2};
     pd[e] = mode == 0 ? u1 : cx<double>{u1.x + cil * d.x, u1.y + cil * d.y};
     rp[e] = mode == 0 ? nc[q] : cx<double>{nc[q].x + d.x, nc[q].y + d.y};
-    reinterpret_cast<float2*>(h2)[e] = float2{(float)nc[q].x, (float)nc[q].y};
+    if (h3) {
+      reinterpret_cast<float2*>(h3)[e] = hp2[q];
+      reinterpret_cast<float2*>(h2)[e] = hp1[q];
+      reinterpret_cast<float2*>(h1)[e] = float2{(float)nc[q].x, (float)nc[q].y};
+    } else {
+      reinterpret_cast<float2*>(h2)[e] = float2{(float)nc[q].x, (float)nc[q].y};
+    }
   }
 }
 
@@ -782,30 +810,31 @@ static FeatArgs feat_args(Ctx* c, float s_1, float s_2, float s_out) {
   return a;
 }
 
-int mlp_features(Ctx* c, const double* nm2, const double* nm1, const double* ncur,
+int mlp_features(Ctx* c, const double* nm3, const double* nm2, const double* nm1, const double* ncur,
                  const double* nnext, const long long* modes, int ns, float s_1, float s_2,
                  float s_out, float* X, float* T, hipStream_t s) {
   const FeatArgs a = feat_args(c, s_1, s_2, s_out);
   if (ns == 0) return DNSMI_OK;
   hipLaunchKernelGGL(k_mlp_features, dim3((ns + 255) / 256), dim3(256), 0, s, a,
-                     reinterpret_cast<const cx<double>*>(nm2),
+                     reinterpret_cast<const cx<double>*>(nm3), reinterpret_cast<const cx<double>*>(nm2),
                      reinterpret_cast<const cx<double>*>(nm1),
                      reinterpret_cast<const cx<double>*>(ncur),
                      reinterpret_cast<const cx<double>*>(nnext), modes, ns, X, T);
   return check_launch(c, "mlp_features");
 }
 
-int periodic_ref_rhs(Ctx* c, const double* state, double* rhs, hipStream_t s);
+int periodic_guess_rhs(Ctx* c, const double* state, double* rhs, hipStream_t s);
 
-int guess_periodic(Ctx* c, const double* state, double* pred, double* rhs_prev, const float* h1,
-                   float* h2, const uint16_t* wb, const float* bias, int H, float s_1, float s_2,
+int guess_periodic(Ctx* c, const double* state, double* pred, double* rhs_prev, float* h1,
+                   float* h2, float* h3, const uint16_t* wb, const float* bias, int H, float s_1, float s_2,
                    float s_out, int mode, hipStream_t s) {
   if (c->walls) return c->fail(DNSMI_EINVAL, "guess: periodic contexts only");
-  if (mode < 0 || mode > 3) return c->fail(DNSMI_EINVAL, "guess: mode must be 0, 1, 2 or 3");
+  if (mode < 0 || mode > 4) return c->fail(DNSMI_EINVAL, "guess: mode must be 0 to 4");
+  if (mode == 4 && !h3) return c->fail(DNSMI_EINVAL, "guess: cubic extrapolation needs hist3");
   if (mode == 2 && H != 32 && H != 64)
     return c->fail(DNSMI_EINVAL, "guess: hidden width must be 32 or 64");
-  // N^n is the reference of the step's split evaluations (mixed form)
-  int rc = periodic_ref_rhs(c, state, c->rhs0, s);
+  // N^n; the first corrector pass from the guess records the split reference
+  int rc = periodic_guess_rhs(c, state, c->rhs0, s);
   if (rc) return rc;
   const FeatArgs a = feat_args(c, s_1, s_2, s_out);
   if (a.n >= (1LL << 31)) return c->fail(DNSMI_EINVAL, "guess: too many modes for 32-bit indexing");
@@ -813,11 +842,11 @@ int guess_periodic(Ctx* c, const double* state, double* pred, double* rhs_prev, 
   const cx<double>* n0 = reinterpret_cast<const cx<double>*>(c->rhs0);
   if (mode != 2 || H == 32)
     hipLaunchKernelGGL(k_guess_periodic<32>, dim3(nb), dim3(256), 0, s, a,
-                       reinterpret_cast<const cx<double>*>(state), n0, h1, h2, pred, rhs_prev, wb,
+                       reinterpret_cast<const cx<double>*>(state), n0, h1, h2, h3, pred, rhs_prev, wb,
                        bias, mode);
   else
     hipLaunchKernelGGL(k_guess_periodic<64>, dim3(nb), dim3(256), 0, s, a,
-                       reinterpret_cast<const cx<double>*>(state), n0, h1, h2, pred, rhs_prev, wb,
+                       reinterpret_cast<const cx<double>*>(state), n0, h1, h2, h3, pred, rhs_prev, wb,
                        bias, mode);
   return check_launch(c, "guess");
 }

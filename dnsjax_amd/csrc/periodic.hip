@@ -41,6 +41,7 @@ struct StepArgs {
   double* partials;        // MODE_CORRECT: per-block sum k_metric |delta|^2
   const int* act;          // if set and 0: the whole launch is skipped
   const double* nref;      // EV_INC: N(u_ref), added to the projected increment
+  double* nref_out;        // EV_RECORD: receives a copy of N (the new N(u_ref))
   double dt, c, re;
 };
 
@@ -379,7 +380,7 @@ struct XWaves {
 // (no base flow added) and forms dN = U x w_d + d x (W + w_d) with U, W
 // read back from xref: every FP32 product is then of increment size.
 template <typename T, int P, int R, int EM>
-__global__ void __launch_bounds__(R * Plan<P>::N1, (XWaves<T, P>::value))
+__global__ void __launch_bounds__(R * Plan<P>::N1, (EM == EV_INC ? 4 : XWaves<T, P>::value))
     k_x_nonlinear(const cx<T>* __restrict__ w2, cx<T>* __restrict__ w3,
                   const cx<T>* __restrict__ tw, const double* __restrict__ kx_tab,
                   const double* __restrict__ base, const double* __restrict__ cbase,
@@ -506,10 +507,20 @@ __global__ void __launch_bounds__(R * Plan<P>::N1, (XWaves<T, P>::value))
       if (ok) {
         const size_t fs = (size_t)nrows * P;
         float* xr = xref + (size_t)row * P + opaque(j_) * N2;
+        float* pu = xr + cu * fs;
+        float* pw = xr + (3 + cw) * fs;
+        if constexpr (N2 % 4 == 0) {
 #pragma unroll
-        for (int k2 = 0; k2 < N2; ++k2) {
-          xr[cu * fs + k2] = re[k2];
-          xr[(3 + cw) * fs + k2] = im[k2];
+          for (int i = 0; i < N2 / 4; ++i) {
+            reinterpret_cast<float4*>(pu)[i] = {re[4 * i], re[4 * i + 1], re[4 * i + 2], re[4 * i + 3]};
+            reinterpret_cast<float4*>(pw)[i] = {im[4 * i], im[4 * i + 1], im[4 * i + 2], im[4 * i + 3]};
+          }
+        } else {
+#pragma unroll
+          for (int k2 = 0; k2 < N2; ++k2) {
+            pu[k2] = re[k2];
+            pw[k2] = im[k2];
+          }
         }
       }
     }
@@ -555,18 +566,40 @@ __global__ void __launch_bounds__(R * Plan<P>::N1, (XWaves<T, P>::value))
   load_pair(2, q);
   inverse_pair(1, uy, wz);
   // EV_INC: the reference fields of this lane's points (rows past the end
-  // read row 0's, like the inputs)
+  // read row 0's, like the inputs), N2 consecutive floats per field, read
+  // four at a time where N2 allows
   const size_t xfs = (size_t)nrows * P;
   const float* xr = EM == EV_INC ? xref + (size_t)(ok ? row : 0) * P + j_ * N2 : nullptr;
+  auto ref_row = [&](int f, T (&v)[N2]) {
+    const float* p = xr + f * xfs;
+    if constexpr (N2 % 4 == 0) {
+#pragma unroll
+      for (int i = 0; i < N2 / 4; ++i) {
+        const float4 t = reinterpret_cast<const float4*>(p)[i];
+        v[4 * i] = t.x;
+        v[4 * i + 1] = t.y;
+        v[4 * i + 2] = t.z;
+        v[4 * i + 3] = t.w;
+      }
+    } else {
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) v[k2] = p[k2];
+    }
+  };
   {
     T nl0[N2];
     if constexpr (EM == EV_INC) {
+      T Uy[N2], Uz[N2];
+      ref_row(1, Uy);
+      ref_row(2, Uz);
 #pragma unroll
-      for (int k2 = 0; k2 < N2; ++k2) {
-        const T Uy = xr[1 * xfs + k2], Uz = xr[2 * xfs + k2];
-        const T Wy = xr[4 * xfs + k2], Wz = xr[5 * xfs + k2];
-        nl0[k2] = (Uy * wz[k2] - Uz * wy[k2]) + (uy[k2] * (Wz + wz[k2]) - uz[k2] * (Wy + wy[k2]));
-      }
+      for (int k2 = 0; k2 < N2; ++k2) nl0[k2] = Uy[k2] * wz[k2] - Uz[k2] * wy[k2];
+      T Wy[N2], Wz[N2];
+      ref_row(4, Wy);
+      ref_row(5, Wz);
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2)
+        nl0[k2] += uy[k2] * (Wz[k2] + wz[k2]) - uz[k2] * (Wy[k2] + wy[k2]);
     } else {
 #pragma unroll
       for (int k2 = 0; k2 < N2; ++k2) {
@@ -587,12 +620,37 @@ __global__ void __launch_bounds__(R * Plan<P>::N1, (XWaves<T, P>::value))
       T ux[N2], wx[N2];
       inverse_pair(2, ux, wx);
       if constexpr (EM == EV_INC) {
+        // (U + u_ref) x w_d, then d x (W + w_ref + w_d), two fields at a time
+        {
+          T Ux[N2], Uz[N2];
+          ref_row(0, Ux);
+          ref_row(2, Uz);
 #pragma unroll
-        for (int k2 = 0; k2 < N2; ++k2) {
-          const T Ux = xr[0 * xfs + k2], Uy = xr[1 * xfs + k2], Uz = xr[2 * xfs + k2];
-          const T Wx = xr[3 * xfs + k2], Wy = xr[4 * xfs + k2], Wz = xr[5 * xfs + k2];
-          v[k2].x = (Uz * wx[k2] - Ux * wz[k2]) + (uz[k2] * (Wx + wx[k2]) - ux[k2] * (Wz + wz[k2]));
-          v[k2].y = (Ux * wy[k2] - Uy * wx[k2]) + (ux[k2] * (Wy + wy[k2]) - uy[k2] * (Wx + wx[k2]));
+          for (int k2 = 0; k2 < N2; ++k2) {
+            v[k2].x = Uz[k2] * wx[k2] - Ux[k2] * wz[k2];
+            v[k2].y = Ux[k2] * wy[k2];
+          }
+        }
+        {
+          T Uy[N2], Wx[N2];
+          ref_row(1, Uy);
+          ref_row(3, Wx);
+#pragma unroll
+          for (int k2 = 0; k2 < N2; ++k2) {
+            v[k2].y -= Uy[k2] * wx[k2];
+            v[k2].x += uz[k2] * (Wx[k2] + wx[k2]);
+            v[k2].y -= uy[k2] * (Wx[k2] + wx[k2]);
+          }
+        }
+        {
+          T Wy[N2], Wz[N2];
+          ref_row(4, Wy);
+          ref_row(5, Wz);
+#pragma unroll
+          for (int k2 = 0; k2 < N2; ++k2) {
+            v[k2].x -= ux[k2] * (Wz[k2] + wz[k2]);
+            v[k2].y += ux[k2] * (Wy[k2] + wy[k2]);
+          }
         }
       } else {
 #pragma unroll
@@ -908,6 +966,7 @@ __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
           const cx<double> r0 = reinterpret_cast<const cx<double>*>(args.nref)[idx];
           rh = {r0.x + rh.x, r0.y + rh.y};
         }
+        if (args.nref_out) reinterpret_cast<cx<double>*>(args.nref_out)[idx] = rh;
         cx<double>* rout = reinterpret_cast<cx<double>*>(args.rhs_out);
         if (MODE == MODE_RHS) {
           rout[idx] = rh;
@@ -1305,13 +1364,16 @@ static int periodic_rhs_t(Ctx* c, const double* state, const StepArgs& a, hipStr
 }
 
 // One evaluation of N(state) in the context's current precision.  em: see
-// EvalMode; EV_INC needs a recorded reference (falls back to EV_PLAIN
-// otherwise, and in FP64 passes) and adds n_ref through a.nref.
+// EvalMode; EV_INC records a reference first when none is valid (the step's
+// first corrector evaluation), and adds n_ref through a.nref; FP64 passes
+// are always EV_PLAIN.
 template <int MODE>
 static int periodic_rhs_mode(Ctx* c, const double* state, StepArgs a, hipStream_t s,
                              int* nblocks, int em = EV_PLAIN) {
-  if (!c->fp32 || !c->split || (em == EV_INC && !c->ref_valid)) em = EV_PLAIN;
+  if (em == EV_INC && !c->ref_valid) em = EV_RECORD;
+  if (!c->fp32 || !c->split) em = EV_PLAIN;
   a.nref = em == EV_INC ? c->nref : nullptr;
+  a.nref_out = em == EV_RECORD ? c->nref : nullptr;
   if (!c->fp32) return periodic_rhs_t<double, MODE>(c, state, a, s, nblocks, EV_PLAIN);
   const int rc = periodic_rhs_t<float, MODE>(c, state, a, s, nblocks, em);
   if (rc == DNSMI_OK && em == EV_RECORD) c->ref_valid = true;
@@ -1334,30 +1396,35 @@ int periodic_get_rhs(Ctx* c, const double* state, double* rhs, hipStream_t s) {
   return periodic_rhs_mode<MODE_RHS>(c, state, a, s, nullptr);
 }
 
-// N(state) into rhs, recorded as the reference of the split evaluations
-// that follow (the first evaluation of a guessed step).
-int periodic_ref_rhs(Ctx* c, const double* state, double* rhs, hipStream_t s) {
+// N(state) into rhs, the first evaluation of a guessed step: a plain
+// evaluation, after which the step's first corrector evaluation records the
+// reference of the split evaluations.
+int periodic_guess_rhs(Ctx* c, const double* state, double* rhs, hipStream_t s) {
   StepArgs a = step_args(c);
   a.rhs_out = rhs;
-  return periodic_rhs_mode<MODE_RHS>(c, state, a, s, nullptr, EV_RECORD);
+  c->ref_valid = false;
+  return periodic_rhs_mode<MODE_RHS>(c, state, a, s, nullptr, EV_PLAIN);
 }
 
-// The predictor's evaluation N(u^n) (into rhs0 = n_ref) is the step's
-// reference; the corrector's evaluations are increments over it.
+// The split evaluations of a step are increments over its first corrector
+// iterate: the predictor's evaluation N(u^n) is plain, the first corrector
+// evaluation N(u^1) records u_ref = u^1 (so every later increment is of the
+// corrector's own size, far smaller than u^1 - u^n).
 int periodic_predict_and_correct(Ctx* c, const double* state, double* pred, double* rhs_next,
                                  double* err, hipStream_t s) {
   StepArgs a = step_args(c);
   a.state = state;
   a.pred = pred;
   a.rhs_out = c->rhs0;
-  int rc = periodic_rhs_mode<MODE_PREDICT>(c, state, a, s, nullptr, EV_RECORD);
+  int rc = periodic_rhs_mode<MODE_PREDICT>(c, state, a, s, nullptr, EV_PLAIN);
   if (rc) return rc;
   StepArgs b = step_args(c);
   b.rhs_prev = c->rhs0;
   b.pred = pred;
   b.rhs_out = rhs_next;
   int nb = 0;
-  rc = periodic_rhs_mode<MODE_CORRECT>(c, pred, b, s, &nb, EV_INC);
+  c->ref_valid = false;
+  rc = periodic_rhs_mode<MODE_CORRECT>(c, pred, b, s, &nb, EV_RECORD);
   if (rc) return rc;
   return finalize_norm(c, nb, err, s);
 }
@@ -1621,11 +1688,10 @@ int periodic_setup(Ctx* c) {
   c->bufB = c->alloc(c->bufB_bytes);
   c->rhs0 = static_cast<double*>(c->alloc(c->spec_elems * 16));
   if (c->split) {
-    // n_ref is rhs0: the predictor's (or the guess's) evaluation writes it
-    c->nref = c->rhs0;
+    c->nref = static_cast<double*>(c->alloc(c->spec_elems * 16));
     c->uref = static_cast<double*>(c->alloc(c->spec_elems * 16));
     c->xref = static_cast<float*>(c->alloc(6ull * c->Py * c->Pz * c->Px * sizeof(float)));
-    if (!c->uref || !c->xref) return c->fail(DNSMI_ENOMEM, "device allocation failed");
+    if (!c->nref || !c->uref || !c->xref) return c->fail(DNSMI_ENOMEM, "device allocation failed");
   }
   const long long per_comp = (long long)c->Ny1 * c->Nz1 * c->Nx1;
   // norm partials (one per y-forward block) and the statistics' two per

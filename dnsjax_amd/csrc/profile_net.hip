@@ -460,13 +460,14 @@ int pnet_features(Ctx* c, const double* nm2, const double* nm1, const double* nc
   return check_launch(c, "profile features");
 }
 
-int walls_rhs_record(Ctx* c, const double* state, double* rhs, hipStream_t s);
+int walls_guess_rhs(Ctx* c, const double* state, double* rhs, hipStream_t s);
 int walls_predict(Ctx* c, const double* state, const double* rhs, double* pred, hipStream_t s);
 int extrap_walls(Ctx* c, const double* n0, const float* h1, float* h2, double* next, const uint16_t* wb,
                  const float* bias, int H, float s_1, float s_2, float s_out, int mode, hipStream_t s);
 
-// The plane Couette guess with the profile network: N^n (recorded as the
-// step's reference), the network's input from N^n and the FP32 history,
+// The plane Couette guess with the profile network: N^n (plain; the first
+// corrector pass records the split reference), the network's input from N^n
+// and the FP32 history,
 // its output, then N_ext = quadratic extrapolation + s_out Y / w into
 // rhs_prev (h2 <- N^n) and pred = the predictor's solve fed with N_ext.
 // Caller-owned buffers, M = the number of modes: wb (the bf16 weight
@@ -479,17 +480,17 @@ int guess_walls_profile(Ctx* c, const double* state, double* pred, double* rhs_p
   if (dout != 6 * c->ny) return c->fail(DNSMI_EINVAL, "profile network: dout must be 6 ny");
   const long long M = (long long)c->Nz1 * c->Nx1;
   if (M >= (1LL << 31)) return c->fail(DNSMI_EINVAL, "profile network: too many modes");
-  int rc = walls_rhs_record(c, state, c->nref, s);
+  int rc = walls_guess_rhs(c, state, c->rhs0, s);
   if (rc) return rc;
   const PFeat a = pfeat(c, din, s_1, s_2, s_out);
   const unsigned nb = (unsigned)((M * 3 * c->ny + 255) / 256);
   // the input reads h1, h2 before the extrapolation overwrites h2
-  hipLaunchKernelGGL(k_pn_features_hist, dim3(nb), dim3(256), 0, s, a, reinterpret_cast<const cx<double>*>(c->nref),
+  hipLaunchKernelGGL(k_pn_features_hist, dim3(nb), dim3(256), 0, s, a, reinterpret_cast<const cx<double>*>(c->rhs0),
                      reinterpret_cast<const float2*>(h1), reinterpret_cast<const float2*>(h2), X);
   if ((rc = check_launch(c, "profile input"))) return rc;
   if ((rc = pnet_forward(din, H, dout, (int)M, X, prm, wb, wf, wh, Y, s)))
     return c->fail(rc, "profile network forward");
-  if ((rc = extrap_walls(c, c->nref, h1, h2, rhs_prev, nullptr, nullptr, 32, s_1, s_2, s_out, 3, s))) return rc;
+  if ((rc = extrap_walls(c, c->rhs0, h1, h2, rhs_prev, nullptr, nullptr, 32, s_1, s_2, s_out, 3, s))) return rc;
   hipLaunchKernelGGL(k_pn_apply, dim3(nb), dim3(256), 0, s, a, Y, reinterpret_cast<cx<double>*>(rhs_prev));
   if ((rc = check_launch(c, "profile apply"))) return rc;
   return walls_predict(c, state, rhs_prev, pred, s);

@@ -141,8 +141,8 @@ __global__ void k_ctl_begin_guess(StepCtl* ctl, int max_pass) {
   ctl->run_e = 0;
 }
 
-int guess_periodic(Ctx* c, const double* state, double* pred, double* rhs_prev, const float* h1,
-                   float* h2, const uint16_t* wb, const float* bias, int H, float s_1, float s_2,
+int guess_periodic(Ctx* c, const double* state, double* pred, double* rhs_prev, float* h1,
+                   float* h2, float* h3, const uint16_t* wb, const float* bias, int H, float s_1, float s_2,
                    float s_out, int mode, hipStream_t s);
 int guess_walls(Ctx* c, const double* state, double* pred, double* rhs_prev, const float* h1,
                 float* h2, const uint16_t* wb, const float* bias, int H, float s_1, float s_2,
@@ -153,7 +153,7 @@ int guess_walls(Ctx* c, const double* state, double* pred, double* rhs_prev, con
 // device rule as step_device (max_pass passes at most; the count recorded is
 // the number of passes).
 int step_device_guess(Ctx* c, const double* state, double* pred, double* rhs_a, double* rhs_b,
-                      const float* h1, float* h2, const uint16_t* wb, const float* bias, int H,
+                      float* h1, float* h2, float* h3, const uint16_t* wb, const float* bias, int H,
                       float s_1, float s_2, float s_out, int mode, int slots, int esc_slots,
                       int* counts, int counts_cap, double tol, int max_pass, double esc_ratio,
                       double esc_cap, hipStream_t s) {
@@ -166,7 +166,7 @@ int step_device_guess(Ctx* c, const double* state, double* pred, double* rhs_a, 
   c->act = nullptr;
   c->fp32 = c->fp32_primary;
   int rc = c->walls ? guess_walls(c, state, pred, rhs_a, h1, h2, wb, bias, H, s_1, s_2, s_out, mode, s)
-                    : guess_periodic(c, state, pred, rhs_a, h1, h2, wb, bias, H, s_1, s_2, s_out,
+                    : guess_periodic(c, state, pred, rhs_a, h1, h2, h3, wb, bias, H, s_1, s_2, s_out,
                                      mode, s);
   if (rc) return rc;
   hipLaunchKernelGGL(k_ctl_begin_guess, dim3(1), dim3(64), 0, s, ctl, max_pass);

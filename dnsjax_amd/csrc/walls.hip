@@ -773,11 +773,13 @@ __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
 }
 
 // WT5: (3, Ny, Nkz, Nkx) T -> rhs (3, M, Ny) FP64, tiled transpose; with
-// nref (split evaluations) rhs = nref + the transformed increment.
+// nref (split evaluations) rhs = nref + the transformed increment; with
+// nref_out (a recording evaluation) rhs is also copied there.
 template <typename T>
 __global__ void __launch_bounds__(kTY* kTM)
     k_walls_to_spec(const cx<T>* __restrict__ w, cx<double>* __restrict__ rhs, int Ny, int M,
-                    const int* __restrict__ act, const cx<double>* __restrict__ nref) {
+                    const int* __restrict__ act, const cx<double>* __restrict__ nref,
+                    cx<double>* __restrict__ nref_out) {
   if (act && *act == 0) return;
   __shared__ cx<double> tile[kTY][kTM + 1];
   const int ntile_y = (Ny + kTY - 1) / kTY;
@@ -801,6 +803,7 @@ __global__ void __launch_bounds__(kTY* kTM)
     cx<double> r = tile[yi][mi];
     if (nref) r = {nref[e].x + r.x, nref[e].y + r.y};
     rhs[e] = r;
+    if (nref_out) nref_out[e] = r;
   }
 }
 
@@ -1432,15 +1435,18 @@ static int walls_rhs_t(Ctx* c, const double* state, double* rhs, hipStream_t s, 
     ProfScope ps(c, KID_W_Z_FORWARD, s);
     hipLaunchKernelGGL((k_walls_to_spec<T>), dim3(ntile_y * ntile_m, 3), dim3(kTY * kTM), 0, s, B,
                        reinterpret_cast<cx<double>*>(rhs), Ny, M, c->act,
-                       em == EV_INC ? reinterpret_cast<const cx<double>*>(c->nref) : nullptr);
+                       em == EV_INC ? reinterpret_cast<const cx<double>*>(c->nref) : nullptr,
+                       em == EV_RECORD ? reinterpret_cast<cx<double>*>(c->nref) : nullptr);
   }
   return check_launch(c, "walls_to_spec");
 }
 
 // One evaluation in the context's current precision (see EvalMode; EV_INC
-// without a recorded reference, and every FP64 evaluation, is EV_PLAIN).
+// without a recorded reference records one, every FP64 evaluation is
+// EV_PLAIN).
 static int walls_rhs_mode(Ctx* c, const double* state, double* rhs, hipStream_t s, int em) {
-  if (!c->fp32 || !c->split || (em == EV_INC && !c->ref_valid)) em = EV_PLAIN;
+  if (em == EV_INC && !c->ref_valid) em = EV_RECORD;
+  if (!c->fp32 || !c->split) em = EV_PLAIN;
   if (!c->fp32) return walls_rhs_t<double>(c, state, rhs, s, EV_PLAIN);
   const int rc = walls_rhs_t<float>(c, state, rhs, s, em);
   if (rc == DNSMI_OK && em == EV_RECORD) c->ref_valid = true;
@@ -1451,9 +1457,11 @@ int walls_get_rhs(Ctx* c, const double* state, double* rhs, hipStream_t s) {
   return walls_rhs_mode(c, state, rhs, s, EV_PLAIN);
 }
 
-// N(state), recorded as the reference of the step's split evaluations.
-int walls_rhs_record(Ctx* c, const double* state, double* rhs, hipStream_t s) {
-  return walls_rhs_mode(c, state, rhs, s, EV_RECORD);
+// N(state), the first evaluation of a guessed step: plain, after which the
+// step's first corrector evaluation records the split reference.
+int walls_guess_rhs(Ctx* c, const double* state, double* rhs, hipStream_t s) {
+  c->ref_valid = false;
+  return walls_rhs_mode(c, state, rhs, s, EV_PLAIN);
 }
 
 static int walls_imm(Ctx* c, const double* un, const double* uj, const double* nn,
@@ -1622,23 +1630,25 @@ int guess_walls(Ctx* c, const double* state, double* pred, double* rhs_prev, con
   if (mode < 0 || mode > 3) return c->fail(DNSMI_EINVAL, "guess: mode must be 0, 1, 2 or 3");
   if (mode == 2 && H != 32 && H != 64)
     return c->fail(DNSMI_EINVAL, "guess: hidden width must be 32 or 64");
-  // N^n (into n_ref) is the reference of the step's split evaluations
-  int rc = walls_rhs_mode(c, state, c->nref, s, EV_RECORD);
+  // N^n; the first corrector pass from the guess records the split reference
+  int rc = walls_guess_rhs(c, state, c->rhs0, s);
   if (rc) return rc;
-  if ((rc = extrap_walls(c, c->nref, h1, h2, rhs_prev, wb, bias, H, s_1, s_2, s_out, mode, s)))
+  if ((rc = extrap_walls(c, c->rhs0, h1, h2, rhs_prev, wb, bias, H, s_1, s_2, s_out, mode, s)))
     return rc;
   return walls_predict(c, state, rhs_prev, pred, s);
 }
 
-// The predictor's evaluation N(u^n) (into n_ref) is the step's reference;
-// the corrector's evaluations are increments over it (mixed form).
+// The predictor's evaluation N(u^n) is plain; the first corrector
+// evaluation N(u^1) records the split reference (mixed form), so every later
+// increment is of the corrector's own size.
 int walls_predict_and_correct(Ctx* c, const double* state, double* pred, double* rhs_next,
                               double* err, hipStream_t s) {
-  double* n0 = c->nref;
-  int rc = walls_rhs_mode(c, state, n0, s, EV_RECORD);
+  double* n0 = c->rhs0;
+  int rc = walls_rhs_mode(c, state, n0, s, EV_PLAIN);
   if (rc) return rc;
   if ((rc = walls_imm(c, state, state, n0, n0, pred, nullptr, nullptr, s))) return rc;
-  if ((rc = walls_rhs_mode(c, pred, rhs_next, s, EV_INC))) return rc;
+  c->ref_valid = false;
+  if ((rc = walls_rhs_mode(c, pred, rhs_next, s, EV_RECORD))) return rc;
   return walls_imm(c, state, pred, n0, rhs_next, pred, nullptr, err, s);
 }
 

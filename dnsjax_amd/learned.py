@@ -143,39 +143,50 @@ def _rms(t: torch.Tensor) -> float:
 
 @dataclass
 class Scales:
+    """Feature scales s_1, s_2 and target scale s_out; ``cubic``: the
+    targets are residuals of cubic extrapolation (the guess then builds on
+    cubic extrapolation and keeps three histories), else of quadratic."""
     s_1: float
     s_2: float
     s_out: float
+    cubic: bool = False
 
     def astuple(self):
         return (self.s_1, self.s_2, self.s_out)
 
 
-def fit_scales(snaps) -> Scales:
+def fit_scales(snaps, cubic: bool = False) -> Scales:
     """Feature and target scales over the collected history: 1/rms(D1),
-    1/rms(D2) and rms(D3), D1..D3 the first to third differences of N."""
+    1/rms(D2) and rms(D3) (rms(D4) for cubic targets), Dk the k-th
+    difference of N."""
     n = len(snaps)
-    if n < 4:
-        raise ValueError("need at least four nonlinear terms")
+    if n < (5 if cubic else 4):
+        raise ValueError("need at least four (cubic: five) nonlinear terms")
     d1 = sum(_rms(snaps[i] - snaps[i - 1]) for i in range(1, n)) / (n - 1)
     d2 = sum(_rms(snaps[i] - 2 * snaps[i - 1] + snaps[i - 2]) for i in range(2, n)) / (n - 2)
-    d3 = sum(_rms(snaps[i] - 3 * snaps[i - 1] + 3 * snaps[i - 2] - snaps[i - 3]) for i in range(3, n)) / (n - 3)
-    return Scales(1.0 / max(d1, 1e-300), 1.0 / max(d2, 1e-300), max(d3, 1e-300))
+    if cubic:
+        out = sum(_rms(snaps[i] - 4 * snaps[i - 1] + 6 * snaps[i - 2] - 4 * snaps[i - 3] + snaps[i - 4])
+                  for i in range(4, n)) / (n - 4)
+    else:
+        out = sum(_rms(snaps[i] - 3 * snaps[i - 1] + 3 * snaps[i - 2] - snaps[i - 3]) for i in range(3, n)) / (n - 3)
+    return Scales(1.0 / max(d1, 1e-300), 1.0 / max(d2, 1e-300), max(out, 1e-300), cubic)
 
 
 def make_dataset(flow, snaps, scales: Scales, samples_per_window: int, seed: int = 0):
     """Features X (S,16) and targets T (S,6) from every window of four
-    consecutive nonlinear terms, at uniformly drawn modes."""
+    (cubic targets: five) consecutive nonlinear terms, at uniformly drawn
+    modes."""
     nat = flow._native
     n_modes = math.prod(flow.params.spec_shape)
     g = torch.Generator(device=nat.device).manual_seed(seed)
     Xs, Ts = [], []
-    for i in range(2, len(snaps) - 1):
+    for i in range(3 if scales.cubic else 2, len(snaps) - 1):
         modes = torch.randint(0, n_modes, (samples_per_window,), generator=g, device=nat.device,
                               dtype=torch.int64)
         X = torch.empty((samples_per_window, Net.D_IN), dtype=torch.float32, device=nat.device)
         T = torch.empty((samples_per_window, Net.D_OUT), dtype=torch.float32, device=nat.device)
-        nat.mlp_features(snaps[i - 2], snaps[i - 1], snaps[i], snaps[i + 1], modes, *scales.astuple(), X, T)
+        nat.mlp_features(snaps[i - 2], snaps[i - 1], snaps[i], snaps[i + 1], modes, *scales.astuple(), X, T,
+                         n_m3=snaps[i - 3] if scales.cubic else None)
         Xs.append(X)
         Ts.append(T)
     return torch.cat(Xs), torch.cat(Ts)
@@ -218,27 +229,46 @@ class GuessRunResult:
         return sum(c + 1 for c in self.passes)
 
 
-MODES = {"plain": 0, "linear": 1, "learned": 2, "quadratic": 3}
+MODES = {"plain": 0, "linear": 1, "learned": 2, "quadratic": 3, "cubic": 4}
+
+
+def _cubic(mode: int, net, scales) -> bool:
+    """Whether a periodic guess of this mode keeps three histories."""
+    return mode == 4 or (mode == 2 and not hasattr(net, "guess") and scales is not None and scales.cubic)
+
+
+def _startup_mode(it: int, mode: int, cubic: bool) -> int:
+    """Mode of step `it`: the plain predictor, then linear, then (cubic runs)
+    quadratic extrapolation until the history is long enough."""
+    if it == 0:
+        return 0
+    if it == 1:
+        return min(mode, 1)
+    if it == 2 and cubic:
+        return 3
+    return mode
 
 
 def run_guessed(flow, state, n_steps: int, mode: int = 2, net: Net | None = None,
                 scales: Scales | None = None, callback=None) -> GuessRunResult:
     """The reference's time loop (``__main__.py:125-203``) with the corrector
-    started from a guess: mode 1 linear, 3 quadratic extrapolation of N,
-    2 quadratic plus the network (for plane Couette the guess is the
+    started from a guess: mode 1 linear, 3 quadratic, 4 cubic extrapolation
+    of N (periodic flows), 2 quadratic (cubic when ``scales.cubic``) plus
+    the network (for plane Couette the guess is the
     predictor's solve fed with the extrapolated N, and the network acts per
     point (kz, kx, y), or per mode (kz, kx) on whole wall-normal profiles
     when ``net`` is a ``profile_net.ProfileNet``).  Mode 0 is the plain
     predictor: the same steps as the
     plain driver, counted as passes (= iterations + 1).  The first steps of a run lack history and
-    start from the plain predictor (step 0) and linear extrapolation (step
-    1).  Up to ``max_corrector_iterations + 1`` passes per step (the plain
+    start from the plain predictor (step 0), linear extrapolation (step 1)
+    and, for cubic runs, quadratic extrapolation (step 2).  Up to
+    ``max_corrector_iterations + 1`` passes per step (the plain
     scheme's RHS budget); the loop stops after a step that did not
     converge, and the mixed form's escalation to FP64 passes applies as in
     ``driver.run``.  The state handed to ``callback`` is a buffer that
     later steps reuse: copy it to keep it."""
-    if mode not in (0, 1, 2, 3):
-        raise ValueError("mode must be 0, 1, 2 or 3")
+    if mode not in (0, 1, 2, 3, 4):
+        raise ValueError("mode must be 0 to 4")
     if mode == 2 and (net is None or scales is None):
         raise ValueError("mode 2 needs a trained net and its scales")
     p = flow.params
@@ -247,7 +277,8 @@ def run_guessed(flow, state, n_steps: int, mode: int = 2, net: Net | None = None
     esc = Escalation(flow)
     sc = (1.0, 1.0, 1.0) if scales is None else scales.astuple()
     ops = None if net is None or hasattr(net, "guess") else net.operands
-    hist = [torch.zeros(nat.shape + (2,), dtype=torch.float32, device=nat.device) for _ in range(2)]
+    cubic = _cubic(mode, net, scales)
+    hist = [torch.zeros(nat.shape + (2,), dtype=torch.float32, device=nat.device) for _ in range(3 if cubic else 2)]
     res = GuessRunResult(state=state)
     state = state.clone()
     pred = nat.empty()
@@ -257,12 +288,13 @@ def run_guessed(flow, state, n_steps: int, mode: int = 2, net: Net | None = None
     for it in range(n_steps):
         if not last_error < tol:
             break
-        m = 0 if it == 0 else (min(mode, 1) if it == 1 else mode)
+        m = _startup_mode(it, mode, cubic)
         if m == 2 and hasattr(net, "guess"):  # the plane Couette profile network
             net.guess(nat, state, pred, rhs[0], hist[0], hist[1], scales)
         else:
-            nat.guess(state, pred, rhs[0], hist[0], hist[1], m, ops, sc)
-        hist.reverse()  # hist[0] = N^n, hist[1] = N^{n-1}
+            nat.guess(state, pred, rhs[0], hist[0], hist[1], m, ops, sc, hist3=hist[2] if cubic else None)
+        if not cubic:
+            hist.reverse()  # hist[0] = N^n, hist[1] = N^{n-1}
         c, err, cur = 0, math.inf, 0
         esc.begin()
         while err > tol and c < max_pass:
@@ -291,8 +323,9 @@ def run_guessed(flow, state, n_steps: int, mode: int = 2, net: Net | None = None
 class GuessedGraphStepper:
     """Guessed steps with the corrector decided on the device, one captured
     graph launch per step (two graphs: state and history buffers swap each
-    step).  The first two steps (plain predictor, then linear extrapolation)
-    run eagerly through the same device step with the full pass budget; the
+    step).  The start-up steps (plain predictor, linear extrapolation and,
+    for cubic guesses, quadratic extrapolation) run eagerly through the same
+    device step with the full pass budget; the
     rest replay the graph of the requested mode with `slots` corrector slots
     (default: the full budget, max_corrector_iterations + 1).  A step that
     needs more passes than the graph holds is counted in
@@ -301,8 +334,8 @@ class GuessedGraphStepper:
 
     def __init__(self, flow, state, mode: int = 2, net: Net | None = None, scales: Scales | None = None,
                  slots: int | None = None, capacity: int = 4096):
-        if mode not in (1, 2, 3):
-            raise ValueError("mode must be 1, 2 or 3")
+        if mode not in (1, 2, 3, 4):
+            raise ValueError("mode must be 1 to 4")
         if mode == 2 and (net is None or scales is None):
             raise ValueError("mode 2 needs a trained net and its scales")
         nat = flow._native
@@ -313,7 +346,9 @@ class GuessedGraphStepper:
         self.sc = (1.0, 1.0, 1.0) if scales is None else scales.astuple()
         self.bufs = [state.detach().clone().contiguous(), nat.empty()]
         self.rhs = [nat.empty(), nat.empty()]
-        self.hist = [torch.zeros(nat.shape + (2,), dtype=torch.float32, device=nat.device) for _ in range(2)]
+        self.cubic = _cubic(mode, net, scales)
+        self.hist = [torch.zeros(nat.shape + (2,), dtype=torch.float32, device=nat.device)
+                     for _ in range(3 if self.cubic else 2)]
         self.counts = torch.zeros(capacity, dtype=torch.int32, device=nat.device)
         self.capacity = capacity
         self.parity = 0
@@ -321,15 +356,18 @@ class GuessedGraphStepper:
         nat.step_reset()
         # the start-up steps run eagerly with the full pass budget
         full = p.max_corrector_iterations + 1
-        self._enqueue(0, 0, full)  # plain predictor, records N^0
-        self._enqueue(1, min(mode, 1), full)  # linear extrapolation
-        self.parity, self.steps = 0, 2
+        start = 3 if self.cubic else 2
+        for it in range(start):
+            self._enqueue(it % 2, _startup_mode(it, mode, self.cubic), full)
+        self.parity, self.steps = start % 2, start
         self.graphs = [self._capture(0), self._capture(1)]
 
     def _enqueue(self, parity: int, mode: int, slots: int | None = None) -> None:
         b, h = self.bufs, self.hist
-        self.nat.step_device_guess(b[parity], b[1 - parity], self.rhs[0], self.rhs[1], h[parity], h[1 - parity],
-                                   mode, self.slots if slots is None else slots, self.counts, self.ops, self.sc)
+        h1, h2 = (h[0], h[1]) if self.cubic else (h[parity], h[1 - parity])
+        self.nat.step_device_guess(b[parity], b[1 - parity], self.rhs[0], self.rhs[1], h1, h2, mode,
+                                   self.slots if slots is None else slots, self.counts, self.ops, self.sc,
+                                   hist3=h[2] if self.cubic else None)
 
     def _capture(self, parity: int):
         g = torch.cuda.CUDAGraph()

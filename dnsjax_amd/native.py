@@ -158,23 +158,27 @@ class NativeContext:
         """Mixed form: run the following corrector passes in FP64 (or back)."""
         self._chk(self.lib.dnsmi_escalate(self.handle, int(bool(on))))
 
-    def mlp_features(self, n_m2, n_m1, n_cur, n_next, modes, s_1, s_2, s_out, X, T=None):
-        """Training features (and targets when n_next and T are given)."""
-        self._chk(self.lib.dnsmi_mlp_features(self.handle, _ptr(n_m2), _ptr(n_m1), _ptr(n_cur), _ptr(n_next),
+    def mlp_features(self, n_m2, n_m1, n_cur, n_next, modes, s_1, s_2, s_out, X, T=None, n_m3=None):
+        """Training features (and targets when n_next and T are given: the
+        residual of quadratic extrapolation, of cubic with n_m3)."""
+        self._chk(self.lib.dnsmi_mlp_features(self.handle, _ptr(n_m3), _ptr(n_m2), _ptr(n_m1), _ptr(n_cur),
+                                              _ptr(n_next),
                                               _ptr(modes), modes.numel(), s_1, s_2, s_out, _ptr(X), _ptr(T),
                                               _stream()))
 
-    def guess(self, state, pred, rhs_prev, hist1, hist2, mode, net=None, scales=(1.0, 1.0, 1.0)):
+    def guess(self, state, pred, rhs_prev, hist1, hist2, mode, net=None, scales=(1.0, 1.0, 1.0), hist3=None):
         """Predictor with a corrector guess (mode 0 plain, 1 linear, 3
-        quadratic extrapolation, 2 quadratic plus the network `net` =
-        (H, packed weights, bias)).  hist2 receives N^n."""
+        quadratic, 4 cubic extrapolation, 2 quadratic -- cubic with hist3 --
+        plus the network `net` = (H, packed weights, bias)).  Without hist3,
+        hist2 receives N^n; with it the three histories shift in place."""
         H, wb, bias = (32, None, None) if net is None else net
         s_1, s_2, s_out = scales
         self._chk(self.lib.dnsmi_guess(self.handle, _ptr(state), _ptr(pred), _ptr(rhs_prev), _ptr(hist1),
-                                       _ptr(hist2), _ptr(wb), _ptr(bias), H, s_1, s_2, s_out, mode, _stream()))
+                                       _ptr(hist2), _ptr(hist3), _ptr(wb), _ptr(bias), H, s_1, s_2, s_out, mode,
+                                       _stream()))
 
     def step_device_guess(self, state, nxt, rhs_a, rhs_b, hist1, hist2, mode, slots, counts=None, net=None,
-                          scales=(1.0, 1.0, 1.0), esc_slots=None):
+                          scales=(1.0, 1.0, 1.0), esc_slots=None, hist3=None):
         """Enqueue one time step started from the corrector guess, decided on
         the device (capturable).  Up to max_corrector_iterations + 1 passes;
         mixed form: `esc_slots` FP64 slots (default 2) as in step_device."""
@@ -184,7 +188,8 @@ class NativeContext:
         n_esc, ratio, cap = self._esc_args(esc_slots)
         ncap = 0 if counts is None else counts.numel()
         self._chk(self.lib.dnsmi_step_device_guess(
-            self.handle, _ptr(state), _ptr(nxt), _ptr(rhs_a), _ptr(rhs_b), _ptr(hist1), _ptr(hist2), _ptr(wb),
+            self.handle, _ptr(state), _ptr(nxt), _ptr(rhs_a), _ptr(rhs_b), _ptr(hist1), _ptr(hist2), _ptr(hist3),
+            _ptr(wb),
             _ptr(bias), H, s_1, s_2, s_out, mode, slots, n_esc, _ptr(counts), ncap, p.corrector_tolerance,
             p.max_corrector_iterations + 1, ratio, cap, _stream()))
 

@@ -217,34 +217,41 @@ int dnsmi_mlp_forward(int H, const float* X, int B, const void* packed_weights, 
  * for the flat mode indices `modes` (over (ky, kz, kx)) and, when n_next
  * and T are both given, T (ns,6) = w(k) (N^{n+1} - 3 N^n + 3 N^{n-1} -
  * N^{n-2}) / s_out, w = 1 / (1 + c dt |k|^2 / Re): the residual of quadratic
- * extrapolation.  Features: D1 = N^n - N^{n-1} times s_1, D2 = N^n -
+ * extrapolation; with n_m3 = N^{n-3} (periodic only, else NULL) the
+ * residual of cubic extrapolation, w (N^{n+1} - 4 N^n + 6 N^{n-1} -
+ * 4 N^{n-2} + N^{n-3}) / s_out.  Features: D1 = N^n - N^{n-1} times s_1, D2 = N^n -
  * 2 N^{n-1} + N^{n-2} times s_2 (older terms rounded to FP32 as the step
  * history keeps them), min(dt |k|^2/Re, 8), k / k_max.  Wall-bounded
  * contexts: `modes` index points over (kz, kx, y); y itself replaces
  * ky / k_max, |k|^2 = kx^2 + kz^2, and the weight is not zeroed at the mean
  * mode. */
-int dnsmi_mlp_features(dnsmi_ctx* ctx, const double* n_m2, const double* n_m1,
+int dnsmi_mlp_features(dnsmi_ctx* ctx, const double* n_m3, const double* n_m2, const double* n_m1,
                        const double* n_cur, const double* n_next, const long long* modes,
                        int nsamples, float s_1, float s_2, float s_out, float* X, float* T,
                        void* stream);
 /* Predictor with a guess.  Periodic contexts: N^n = rhs(state), u^1 =
  * predict(state, N^n) (flows/monochromatic.py:320-333), then per mode
  * dN = 0 (mode 0), D1 (mode 1, linear extrapolation), D1 + D2 (mode 3,
- * quadratic) or D1 + D2 plus the network's residual (mode 2),
- * Leray-projected and zero at the mean mode; pred = u^1 + c ildt_2 dN and
- * rhs_prev = N^n + dN, so that iterate_correction(state, pred, rhs_prev)
- * is the reference's fixed-point iteration in its full form
- * u <- ildt_2 (ldt_1 u^n + c N(u) + (1 - c) N^n), started from the guess.
- * hist1 = N^{n-1}, hist2 = N^{n-2}: 3 * spec_elems complex values each as
- * FP32 re/im pairs; hist2 is overwritten with N^n, so the caller swaps the
- * two for the next step.  Mode 0 (the plain predictor) needs no history.
+ * quadratic), D1 + D2 + D3 (mode 4, cubic, D3 = N^n - 3 N^{n-1} +
+ * 3 N^{n-2} - N^{n-3}) or the quadratic (cubic with hist3) extrapolation
+ * plus the network's residual (mode 2), Leray-projected and zero at the
+ * mean mode; pred = u^1 + c ildt_2 dN and rhs_prev = N^n + dN, so that
+ * iterate_correction(state, pred, rhs_prev) is the reference's fixed-point
+ * iteration in its full form u <- ildt_2 (ldt_1 u^n + c N(u) + (1 - c) N^n),
+ * started from the guess.  hist1 = N^{n-1}, hist2 = N^{n-2}, hist3 =
+ * N^{n-3} or NULL: 3 * spec_elems complex values each as FP32 re/im pairs.
+ * Without hist3, hist2 is overwritten with N^n, so the caller swaps hist1
+ * and hist2 for the next step; with hist3 (periodic contexts only) the
+ * three are shifted in place (hist3 <- hist2 <- hist1 <- N^n) and the
+ * caller keeps them as they are.  Mode 0 (the plain predictor) reads no
+ * history.
  * Wall-bounded contexts: pred = the predictor's solve
  * (flows/plane_couette.py:507-519) fed with N_ext = N^n + dN per point
  * (same modes, with the network's residual divided by w = 1 / (1 + c dt
  * (kx^2 + kz^2) / Re)) and rhs_prev = N_ext: the walls corrector's fixed
  * point u* = S(u^n; N(u*)) is then reached from the guess. */
 int dnsmi_guess(dnsmi_ctx* ctx, const double* state, double* pred, double* rhs_prev,
-                const float* hist1, float* hist2, const void* packed_weights, const float* bias,
+                float* hist1, float* hist2, float* hist3, const void* packed_weights, const float* bias,
                 int H, float s_1, float s_2, float s_out, int mode, void* stream);
 
 /* One time step started from the corrector guess, decided on the device
@@ -252,11 +259,12 @@ int dnsmi_guess(dnsmi_ctx* ctx, const double* state, double* pred, double* rhs_p
  * rhs_prev), then up to `slots` corrector passes, each skipped on the device
  * once error <= tol or max_pass passes ran, and the escalation slots of
  * dnsmi_step_device; then correct_velocity(next).  counts[i] receives
- * step i's number of passes (one RHS each).  hist1/hist2 as in dnsmi_guess
- * (hist2 receives N^n; swap them, and state/next, for the next step).  A
+ * step i's number of passes (one RHS each).  hist1..hist3 as in dnsmi_guess
+ * (without hist3, hist2 receives N^n: swap hist1 and hist2, and state and
+ * next, for the next step; with it only state and next swap).  A
  * fixed launch sequence with no host synchronisation: capturable. */
 int dnsmi_step_device_guess(dnsmi_ctx* ctx, const double* state, double* next, double* rhs_a,
-                            double* rhs_b, const float* hist1, float* hist2,
+                            double* rhs_b, float* hist1, float* hist2, float* hist3,
                             const void* packed_weights, const float* bias, int H, float s_1,
                             float s_2, float s_out, int mode, int slots, int esc_slots,
                             int* counts, int counts_cap, double tol, int max_pass,

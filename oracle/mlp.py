@@ -128,10 +128,10 @@ def k_features(flow):
             np.broadcast_to(f.kz / kmax, shape)]
 
 
-def features(n_m2, n_m1, n_cur, n_next, modes, flow, s_1, s_2, s_out):
+def features(n_m2, n_m1, n_cur, n_next, modes, flow, s_1, s_2, s_out, n_m3=None):
     """X (S,16), T (S,6) for flat mode indices over (ky, kz, kx): D1, D2
     (older terms rounded to FP32), k features; T the weighted residual of
-    quadratic extrapolation."""
+    quadratic extrapolation, or of cubic extrapolation when n_m3 is given."""
     f32 = lambda a: a.astype(np.float32).astype(np.float64)  # noqa: E731
     X = np.zeros((len(modes), 16))
     for q in range(6):
@@ -148,17 +148,23 @@ def features(n_m2, n_m1, n_cur, n_next, modes, flow, s_1, s_2, s_out):
         w = mode_weight(flow).reshape(-1)[modes]
         T = np.zeros((len(modes), 6))
         for q in range(6):
-            d3 = (n_next[q // 2] - 3 * n_cur[q // 2] + 3 * n_m1[q // 2] - n_m2[q // 2]).reshape(-1)[modes]
-            T[:, q] = w * (d3.real if q % 2 == 0 else d3.imag) / s_out
+            c = q // 2
+            r = n_next[c] - 3 * n_cur[c] + 3 * n_m1[c] - n_m2[c]
+            if n_m3 is not None:  # minus the fourth difference's extrapolated part
+                r = n_next[c] - 4 * n_cur[c] + 6 * n_m1[c] - 4 * n_m2[c] + n_m3[c]
+            r = r.reshape(-1)[modes]
+            T[:, q] = w * (r.real if q % 2 == 0 else r.imag) / s_out
     return X, T
 
 
-def guess(state, h1, h2, flow, mode, params=None, scales=(1.0, 1.0, 1.0)):
+def guess(state, h1, h2, flow, mode, params=None, scales=(1.0, 1.0, 1.0), h3=None):
     """(pred, rhs_prev, N^n) of the guessed predictor.
 
-    h1, h2: N^{n-1}, N^{n-2} as complex values (rounded to FP32 by the
-    caller as the device history keeps them).  mode 0: plain predictor;
-    1: dN = D1; 3: dN = D1 + D2; 2: plus s_out * net / w (bf16 inference)."""
+    h1, h2, h3: N^{n-1}, N^{n-2}, N^{n-3} as complex values (rounded to FP32
+    by the caller as the device history keeps them; h3 optional).  mode 0:
+    plain predictor; 1: dN = D1; 3: dN = D1 + D2; 4: dN = D1 + D2 + D3
+    (cubic, needs h3); 2: the quadratic (without h3) or cubic (with h3)
+    extrapolation plus s_out * net / w (bf16 inference)."""
     from . import periodic as P
 
     s_1, s_2, s_out = scales
@@ -169,6 +175,8 @@ def guess(state, h1, h2, flow, mode, params=None, scales=(1.0, 1.0, 1.0)):
         return u1, n0, n0.copy()
     d1 = n0 - h1
     dn = d1 if mode == 1 else d1 + ((n0 - 2 * h1) + h2)
+    if mode == 4 or (mode == 2 and h3 is not None):
+        dn = dn + (((n0 - 3 * h1) + 3 * h2) - h3)
     w = mode_weight(flow)
     if mode == 2:
         modes = np.arange(w.size)

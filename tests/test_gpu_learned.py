@@ -145,6 +145,42 @@ def test_guess_kernel_matches_oracle(mode):
     assert torch.equal(d1, dev(h1))  # hist1 is read only
 
 
+@pytest.mark.parametrize("mode", [4, 2])
+def test_cubic_guess_kernel_matches_oracle(mode):
+    """With a third history: cubic extrapolation (mode 4) and the network
+    on top of it (mode 2) against the oracle; the three histories shift in
+    place (hist3 <- hist2 <- hist1 <- N^n)."""
+    from dnsjax_amd.fields import periodic_field
+
+    n, L = 16, 2 * math.pi
+    flow = _periodic(n, 40.0)
+    of = P.Flow(P.PeriodicParams(nx=n, ny=n, nz=n, lx=L, ly=L, lz=L, re=40.0, dt=0.01, system="kolmogorov"))
+    u = periodic_field(n, n, n, L, L, L, 0.1, 3)
+    hs = [P.get_rhs(periodic_field(n, n, n, L, L, L, 0.1, s), of).astype(np.complex64) for s in (4, 6, 8)]
+    params = _random_params(32, 2)
+    scales = (1.0, 3.0, 0.05)
+    want_pred, want_rhs, want_hist = M.guess(u, *[h.astype(np.complex128) for h in hs[:2]], of, mode, params, scales,
+                                             h3=hs[2].astype(np.complex128))
+    net = _net_with(32, params)
+    net.pack()
+    nat = flow._native
+    dev = lambda h: torch.from_numpy(np.stack([h.real, h.imag], -1).astype(np.float32)).to(nat.device)  # noqa: E731
+    d = [dev(h).contiguous() for h in hs]
+    pred, rhs = nat.empty(), nat.empty()
+    nat.guess(flow.to_device(u), pred, rhs, d[0], d[1], mode, net.operands, scales, hist3=d[2])
+    u1 = P.predict(u, P.get_rhs(u, of), of)
+    got_pred, got_rhs = pred.cpu().numpy(), rhs.cpu().numpy()
+    assert rel(got_rhs, want_rhs) <= (1e-4 if mode == 2 else 1e-12)
+    assert rel(got_pred - u1, want_pred - u1) <= (5e-3 if mode == 2 else 1e-10)
+    if mode == 2:  # the network did contribute beyond cubic extrapolation
+        _, cub, _ = M.guess(u, *[h.astype(np.complex128) for h in hs[:2]], of, 4, h3=hs[2].astype(np.complex128))
+        assert rel(want_rhs, cub) > 1e-4
+    h = d[0].cpu().numpy().astype(np.float64)
+    w = np.stack([want_hist.real, want_hist.imag], -1)
+    assert np.max(np.abs(h - w)) <= 1e-7 * np.max(np.abs(w))
+    assert torch.equal(d[1], dev(hs[0])) and torch.equal(d[2], dev(hs[1]))
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_guessed_runs_stay_on_the_plain_trajectory(precision):
     """Extrapolated and learned guesses: every step converges and the states
@@ -166,36 +202,51 @@ def test_guessed_runs_stay_on_the_plain_trajectory(precision):
     n = 10
     plain = run(flow, u_mid.clone(), n)
     assert plain.converged
-    for mode in (1, 3, 2):
+    for mode in (1, 3, 4, 2):
         g = run_guessed(flow, u_mid.clone(), n, mode=mode, net=net, scales=scales)
         assert g.converged and len(g.passes) == n, (mode, g.passes)
         d = rel(g.state.cpu().numpy(), plain.state.cpu().numpy())
         assert d <= (1e-8 if precision == "fp64" else 1e-7), (mode, d)
-        if mode in (1, 3):
+        if mode in (1, 3, 4):
             assert g.rhs_evals <= plain.rhs_evals, (mode, g.passes, plain.counts)
+    # the network trained on the residual of cubic extrapolation
+    sc3 = fit_scales(snaps, cubic=True)
+    X3, T3 = make_dataset(flow, snaps, sc3, 20000, seed=1)
+    assert X3.shape == (9 * 20000, 16)  # 13 terms: 9 windows of five
+    net3 = Net(32, seed=0)
+    train(net3, X3, T3, iters=300, batch=4096, lr=3e-3)
+    loss3, base3 = mean_loss(net3, X3, T3)
+    g = run_guessed(flow, u_mid.clone(), n, mode=2, net=net3, scales=sc3)
+    d = rel(g.state.cpu().numpy(), plain.state.cpu().numpy())
+    print(f"MEASURED cubic-residual net loss {loss3:.3g} (zero {base3:.3g}); passes {g.passes}")
+    assert g.converged and len(g.passes) == n and d <= (1e-8 if precision == "fp64" else 1e-7), (g.passes, d)
 
 
-@pytest.mark.parametrize("precision,mode", [("fp64", 3), ("mixed", 2)])
-def test_guessed_graph_steps_match_eager(precision, mode):
+@pytest.mark.parametrize("precision,mode,cubic", [("fp64", 3, False), ("mixed", 2, False), ("mixed", 4, False),
+                                                  ("mixed", 2, True)])
+def test_guessed_graph_steps_match_eager(precision, mode, cubic):
     """Device-decided guessed steps replayed from captured graphs equal the
-    eager host loop bit for bit, with the same pass counts."""
+    eager host loop bit for bit, with the same pass counts (quadratic and
+    cubic bases; three histories shifted in place for the cubic ones)."""
     from dnsjax_amd.learned import GuessedGraphStepper, Net, collect, fit_scales, make_dataset, run_guessed, train
 
     flow = _periodic(32, 60.0, precision, dt=0.01)
     u0 = flow.init_state(seed=7, rms=0.1)
     snaps, u_mid = collect(flow, u0, 8)
-    sc = fit_scales(snaps)
+    sc = fit_scales(snaps, cubic=cubic)
     X, T = make_dataset(flow, snaps, sc, 5000, seed=2)
     net = Net(32, seed=1)
     train(net, X, T, iters=50, batch=2048)
-    n = 6
+    n = 7
     ref = []
     eager = run_guessed(flow, u_mid.clone(), n, mode=mode, net=net, scales=sc,
                         callback=lambda i, s: ref.append(s.clone()))
     gs = GuessedGraphStepper(flow, u_mid, mode=mode, net=net, scales=sc)
     torch.cuda.synchronize()
-    assert torch.equal(gs.state, ref[1])  # the two eager start-up steps
-    for i in range(2, n):
+    start = 3 if (mode == 4 or cubic) else 2
+    assert gs.cubic == (start == 3)
+    assert torch.equal(gs.state, ref[start - 1])  # the eager start-up steps
+    for i in range(start, n):
         gs.step(1)
         torch.cuda.synchronize()
         _ = flow.get_rhs(ref[i])  # eager work between replays
@@ -238,6 +289,17 @@ def test_features_match_oracle_periodic_and_walls():
     wX, wT = M.features(*seq, idx, of, *sc)
     assert np.allclose(X, wX, rtol=2e-6, atol=2e-6 * np.max(np.abs(wX)))
     assert np.allclose(T, wT, rtol=2e-6, atol=2e-6 * np.max(np.abs(wT)))
+    # cubic-residual targets with a fifth, older term
+    n_m3 = P.get_rhs(periodic_field(n, n, n, L, L, L, 0.1, 2), of)
+    nat = flow._native
+    X3 = torch.empty((len(idx), 16), dtype=torch.float32, device=nat.device)
+    T3 = torch.empty((len(idx), 6), dtype=torch.float32, device=nat.device)
+    nat.mlp_features(*[flow.to_device(a) for a in seq], torch.from_numpy(idx).to(nat.device), *sc, X3, T3,
+                     n_m3=flow.to_device(n_m3))
+    _, wT3 = M.features(*seq, idx, of, *sc, n_m3=n_m3)
+    T3 = T3.cpu().numpy().astype(np.float64)
+    assert np.allclose(T3, wT3, rtol=2e-6, atol=2e-6 * np.max(np.abs(wT3)))
+    assert np.allclose(X3.cpu().numpy(), X, rtol=0, atol=0)  # the features do not change
     wflow = _walls(8, 17, 8, dt=0.01)
     wf = W.WallFlow(W.WallParams(nx=8, ny=17, nz=8, fd_order=4, re=400.0, dt=0.01))
     wseq = [W.get_rhs(wall_field(8, 17, 8, wf.p.lx, wf.p.lz, 0.05, seed=s), wf) for s in (3, 4, 5, 6)]

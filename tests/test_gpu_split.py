@@ -1,8 +1,9 @@
 """The mixed form's split-precision evaluations and its escalation to FP64.
 
-In the mixed form every evaluation of a step after its first is computed
-as N(u_ref) + P[dN], with only the increment dN = (U + u_ref) x w_d +
-d x (W_ref + w_d), d = u - u_ref, going through the FP32 transforms.  The
+In the mixed form the step's first corrector evaluation records its state
+u_ref, and every later evaluation is computed as N(u_ref) + P[dN], with only
+the increment dN = (U + u_ref) x w_d + d x (W_ref + w_d), d = u - u_ref,
+going through the FP32 transforms.  The
 FP32 rounding then scales with the increment, so the difference of two
 evaluations, which is what the corrector's error measures, is accurate to
 FP32 relative to that difference, and the corrector converges to tight
@@ -57,10 +58,13 @@ def pass_errors(flow, u0, passes):
 
 
 def _rhs_at(flow, ref, u):
-    """N(u) as the corrector computes it after predict_and_correct(ref):
+    """N(u) as a corrector pass computes it after a first pass at `ref`:
     an increment over ref in the split form."""
     nat = flow._native
-    nat.predict_and_correct(ref)
+    hist = [torch.zeros(nat.shape + (2,), dtype=torch.float32, device=nat.device) for _ in range(2)]
+    nat.guess(ref, nat.empty(), nat.empty(), hist[0], hist[1], 0)  # a new step: no reference yet
+    first = ref.clone()
+    nat.iterate_correction(ref, first, nat.empty().zero_(), nat.empty())  # records at ref
     pred = u.clone()
     rhs_next = nat.empty()
     nat.iterate_correction(ref, pred, nat.empty().zero_(), rhs_next)

@@ -92,7 +92,7 @@ def test_guess_mode0_is_the_plain_predictor():
     assert np.array_equal(pred, P.predict(u, n0, flow))
 
 
-@pytest.mark.parametrize("mode", [1, 2, 3])
+@pytest.mark.parametrize("mode", [1, 2, 3, 4])
 def test_guess_keeps_the_corrector_map(mode):
     """One corrector pass from the guess equals the plain iteration map
     u1 + c ildt_2 (N(pred) - N^n) at every mode, whatever dN is."""
@@ -101,8 +101,9 @@ def test_guess_keeps_the_corrector_map(mode):
     u = periodic_field(8, 8, 8, L, L, L, 0.1, 3)
     h1 = _f32(P.get_rhs(periodic_field(8, 8, 8, L, L, L, 0.1, 4), flow))
     h2 = _f32(P.get_rhs(periodic_field(8, 8, 8, L, L, L, 0.1, 5), flow))
+    h3 = _f32(P.get_rhs(periodic_field(8, 8, 8, L, L, L, 0.1, 6), flow)) if mode == 4 else None
     params = M.init(16, 32, 6, seed=1)
-    pred, rhs_prev, _ = M.guess(u, h1, h2, flow, mode, params, (1.0, 2.0, 0.5))
+    pred, rhs_prev, _ = M.guess(u, h1, h2, flow, mode, params, (1.0, 2.0, 0.5), h3=h3)
     n0 = P.get_rhs(u, flow)
     u1 = P.predict(u, n0, flow)
     assert np.linalg.norm(pred - u1) > 1e-3 * np.linalg.norm(u1 - u)  # the guess moved
@@ -132,6 +133,40 @@ def test_extrapolation_is_exact_for_polynomial_histories():
     # D1 = N^n - N^{n-1} = b + 3 e ; D2 = 2 e
     assert np.allclose(X[:, 0], (b[0] + 3 * e[0]).real.reshape(-1), atol=1e-6 * scale)
     assert np.allclose(X[:, 6], (2 * e[0]).real.reshape(-1), atol=1e-6 * scale)
+
+
+def test_cubic_extrapolation_is_exact_for_cubic_histories():
+    """With N^{n-3} given: a cubic history leaves no cubic-residual target
+    and a quartic one leaves its fourth difference (24 e per unit step,
+    weighted); mode 4 extrapolates a cubic history exactly (dN = N^{n+1} -
+    N^n), and mode 2 with a zero network equals mode 4."""
+    flow = _flow()
+    L = 2 * math.pi
+    f = lambda s: P.get_rhs(periodic_field(8, 8, 8, L, L, L, 0.1, s), flow)  # noqa: E731
+    a, b, e, g = f(3), f(9) * 0.25, f(11) * 0.125, f(13) * 0.0625
+    modes = np.arange(a[0].size)
+    cub = [a + k * b + k * k * e + k**3 * g for k in range(5)]  # N^{n-3} .. N^{n+1}
+    _, T = M.features(cub[1], cub[2], cub[3], cub[4], modes, flow, 1.0, 1.0, 1.0, n_m3=cub[0])
+    assert np.max(np.abs(T)) <= 1e-12 * np.max(np.abs(b))
+    _, Tq = M.features(cub[1], cub[2], cub[3], cub[4], modes, flow, 1.0, 1.0, 1.0)
+    assert np.max(np.abs(Tq)) > 1e-3 * np.max(np.abs(g))  # quadratic extrapolation misses g
+    quart = [a + k**4 * e for k in range(5)]
+    _, T4 = M.features(quart[1], quart[2], quart[3], quart[4], modes, flow, 1.0, 1.0, 2.0, n_m3=quart[0])
+    want = M.mode_weight(flow).reshape(-1) * (24 * e[2]).real.reshape(-1) / 2.0
+    assert np.allclose(T4[:, 4], want, rtol=1e-12, atol=1e-12 * np.max(np.abs(want)))
+    # the guess itself: N(t) = N^n + t b + t^2 e + t^3 g around the state's N^n
+    u = periodic_field(8, 8, 8, L, L, L, 0.1, 5)
+    n0 = P.get_rhs(u, flow)
+    hist = [n0 - k * b + k * k * e - k**3 * g for k in (1, 2, 3)]
+    _, rhs4, _ = M.guess(u, *hist[:2], flow, 4, h3=hist[2])
+    want_d = b + e + g
+    want_d[:, 0, 0, 0] = 0.0
+    assert np.max(np.abs((rhs4 - n0) - want_d)) <= 1e-12 * np.max(np.abs(want_d))
+    params = M.init(16, 32, 6, seed=1)
+    params[4][:] = 0.0
+    params[5][:] = 0.0
+    _, rhs2, _ = M.guess(u, *hist[:2], flow, 2, params, (1.0, 1.0, 1.0), h3=hist[2])
+    assert np.array_equal(rhs2, rhs4)
 
 
 def test_walls_guess_mode0_and_corrector_map():
