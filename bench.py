@@ -748,7 +748,8 @@ def walls_large_form(steps: int, collect_steps: int = 8, train_iters: int = 300,
 
 def profile_imm(calls: int) -> None:
     """The plane Couette influence-matrix step (predict: one IMM on the
-    state's RHS) in both band-solve forms, chunk maps and sequential sweeps,
+    state's RHS) in the three band-solve forms (chunk maps in five launches
+    or fused into two, sequential sweeps),
     at 64x128x64 (p=4) and 128x256x128 (p=6), `calls` timed calls each after
     five warm-up calls; prints the time per call, a checksum of the result
     (equal for both forms up to rounding) and the autotuner's choice."""
@@ -759,10 +760,18 @@ def profile_imm(calls: int) -> None:
     from dnsjax_amd.flows import PlaneCouetteFlow
     from dnsjax_amd.params import Params
 
+    # the PMC passes' calibration dispatch, as in profile_workload
+    cal = 256 * 1024 * 1024
+    a = torch.ones(cal // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(f"calibration copy: {cal} bytes read, {cal} bytes written", flush=True)
+    del a, b
     for nx, ny, nz, po in ((64, 128, 64, 4), (128, 256, 128, 6)):
         base = dict(system="plane-couette", nx=nx, ny=ny, nz=nz, lx=2 * math.pi, lz=math.pi, re=400.0,
                     dt=0.01, fd_order=po, corrector_tolerance=1e-5)
-        for form, variants in (("chunked", 0), ("sequential", 8)):
+        for form, variants in (("chunked", 0), ("fused", 16), ("sequential", 8)):
             flow = PlaneCouetteFlow(Params(tile_cols=16, variants=variants, **base))
             nat = flow._native
             state = flow.init_state(seed=1, rms=0.05)

@@ -170,6 +170,7 @@ int dnsmi_create(const dnsmi_config* cfg, dnsmi_ctx** out) {
       c->zinv_seq = (cfg->variants & 2) != 0;
       c->xpass_split = (cfg->variants & 4) != 0;
       c->sweep_seq = (cfg->variants & 8) != 0;
+      c->imm_fused = (cfg->variants & 16) != 0 && !c->sweep_seq;
     }
     if (cfg->system == DNSMI_PLANE_COUETTE) {
       c->walls = true;

@@ -116,6 +116,7 @@ struct Ctx {
   bool yinv_seq = false;
   bool zinv_seq = false;
   bool sweep_seq = false;  // wall-bounded: sequential band sweeps instead of chunk maps
+  bool imm_fused = false;  // wall-bounded, chunk maps: the two-launch fused IMM
   std::string tuning;
 
   // per-kernel timing (off unless dnsmi_profile_begin was called)

@@ -864,12 +864,125 @@ __device__ __forceinline__ cx<double> stencil(const double* w, const int* j0t, c
   return s;
 }
 
+// Per-row arithmetic of the parallel phases, shared by the five-launch
+// form and the fused chunked form (so both compute the same values, bit for
+// bit).  `row` = m Ny, `comp` = M Ny; the rows the stencils read are in LDS.
+
+// d_hat^n at row y (phase 1): i kx u_x + D1 u_y + i kz u_z; un1 = u^n_y's row.
+template <int PO>
+__device__ __forceinline__ cx<double> imm_dhat_at(const ImmArgs& a, const cx<double>* un1, size_t row,
+                                                  size_t comp, double kx, double kz, int y) {
+  const cx<double> dv = stencil<PO + 1>(a.d1w, a.d1j0, un1, y);
+  const cx<double> t0 = ikm(kx, a.un[row + y]);
+  const cx<double> t2 = ikm(kz, a.un[2 * comp + row + y]);
+  return {(t0.x + dv.x) + t2.x, (t0.y + dv.y) + t2.y};
+}
+
+// f_hat at row y (phase 1), zero on the walls; dh = d_hat^n's row, nj1 and
+// nn1 the y rows of N_j and N_n.
+template <int PO>
+__device__ __forceinline__ cx<double> imm_fhat_at(const ImmArgs& a, const cx<double>* dh, const cx<double>* nj1,
+                                                  const cx<double>* nn1, size_t row, size_t comp, double kx,
+                                                  double kz, double k2, int y) {
+  if (y <= 0 || y >= a.Ny - 1) return {0, 0};
+  constexpr int S1 = PO + 1, S2 = PO + 2;
+  const double dt = a.dt, cc = a.c, nu = a.nu;
+  const cx<double> d = dh[y];
+  const cx<double> dvj = stencil<S1>(a.d1w, a.d1j0, nj1, y);
+  const cx<double> dvn = stencil<S1>(a.d1w, a.d1j0, nn1, y);
+  const cx<double> aj = ikm(kx, a.nj[row + y]), cj = ikm(kz, a.nj[2 * comp + row + y]);
+  const cx<double> an = ikm(kx, a.nn[row + y]), cn = ikm(kz, a.nn[2 * comp + row + y]);
+  const cx<double> divj = {(aj.x + dvj.x) + cj.x, (aj.y + dvj.y) + cj.y};
+  const cx<double> divn = {(an.x + dvn.x) + cn.x, (an.y + dvn.y) + cn.y};
+  const cx<double> d2d = stencil<S2>(a.d2w, a.d2j0, dh, y);
+  const cx<double> lkd = {d2d.x - k2 * d.x, d2d.y - k2 * d.y};
+  const double g = (1 - cc) * nu;
+  return {((d.x / dt + cc * divj.x) + (1 - cc) * divn.x) + g * lkd.x,
+          ((d.y / dt + cc * divj.y) + (1 - cc) * divn.y) + g * lkd.y};
+}
+
+// R_c at row y for the three components (phase 3), zero on the walls; pp =
+// the p_P row, ur = the rows of u^n ([c][Ny]).
+template <int PO>
+__device__ __forceinline__ void imm_r_at(const ImmArgs& a, const cx<double>* pp, const cx<double>* ur, size_t row,
+                                         size_t comp, double kx, double kz, double k2, int y,
+                                         cx<double> (&r)[3]) {
+  const int Ny = a.Ny;
+  if (y <= 0 || y >= Ny - 1) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) r[c] = {0, 0};
+    return;
+  }
+  constexpr int S1 = PO + 1, S2 = PO + 2;
+  const double dt = a.dt, cc = a.c, nu = a.nu;
+  const cx<double> p = pp[y];
+  const cx<double> dpp[3] = {ikm(kx, p), stencil<S1>(a.d1w, a.d1j0, pp, y), ikm(kz, p)};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const cx<double>* unc = ur + c * Ny;
+    const cx<double> u = unc[y];
+    const cx<double> du = stencil<S2>(a.d2w, a.d2j0, unc, y);
+    const cx<double> l = {du.x - k2 * u.x, du.y - k2 * u.y};
+    const double g = (1.0 - cc) * nu;
+    const cx<double> hm = {(1.0 / dt) * u.x + g * l.x, (1.0 / dt) * u.y + g * l.y};
+    const cx<double> Nj = a.nj[c * comp + row + y], Nn = a.nn[c * comp + row + y];
+    r[c] = {((hm.x - dpp[c].x) + cc * Nj.x) + (1 - cc) * Nn.x, ((hm.y - dpp[c].y) + cc * Nj.y) + (1 - cc) * Nn.y};
+  }
+}
+
+// alpha = -M^-1 (wall divergence of v_arb) with the k^2 = 0 gauge (phase 5);
+// va = v_arb's row (element y at va[y * vs]).
+template <int PO>
+__device__ __forceinline__ void imm_alpha(const ImmArgs& a, int m, const cx<double>* va, size_t vs,
+                                          cx<double>& al1, cx<double>& al2) {
+  constexpr int S1 = PO + 1;
+  const int Ny = a.Ny;
+  const size_t M = a.M;
+  const double kx = a.mkx[m], kz = a.mkz[m];
+  cx<double> dw0 = stencil<S1>(a.d1w, a.d1j0, va, 0, vs), dw1 = stencil<S1>(a.d1w, a.d1j0, va, Ny - 1, vs);
+  if (kx * kx + kz * kz == 0.0) dw0 = {0, 0};
+  const double m00 = a.minv[0 * M + m], m01 = a.minv[1 * M + m];
+  const double m10 = a.minv[2 * M + m], m11 = a.minv[3 * M + m];
+  al1 = {-(m00 * dw0.x + m01 * dw1.x), -(m00 * dw0.y + m01 * dw1.y)};
+  al2 = {-(m10 * dw0.x + m11 * dw1.x), -(m10 * dw0.y + m11 * dw1.y)};
+}
+
+// Row y of the assembly (phase 5): the corrected velocity into out (and the
+// correction into corr) when `store`; returns k_metric |delta|^2 of the row
+// (before its quadrature weight).  ua, va, wa: v_arb's rows (stride vs).
+__device__ __forceinline__ double imm_assemble_at(const ImmArgs& a, int m, size_t row, size_t comp, double kx,
+                                                  double kz, const cx<double>* ua, const cx<double>* va,
+                                                  const cx<double>* wa, size_t vs, cx<double> al1,
+                                                  cx<double> al2, int y, bool store) {
+  const double km = (m % a.Nkx == 0) ? 1.0 : 2.0;
+  const double4 w = reinterpret_cast<const double4*>(a.vq)[(size_t)a.umode[m] * a.Ny + y];
+  const double v1 = w.x, v2 = w.y, q1 = w.z, q2 = w.w;
+  const cx<double> vv = va[(size_t)y * vs];
+  const cx<double> vn = {(vv.x + al1.x * v1) + al2.x * v2, (vv.y + al1.y * v1) + al2.y * v2};
+  const cx<double> q = {al1.x * q1 + al2.x * q2, al1.y * q1 + al2.y * q2};
+  const cx<double> uu = ua[(size_t)y * vs], ww = wa[(size_t)y * vs];
+  const cx<double> iq_x = ikm(kx, q), iq_z = ikm(kz, q);
+  const cx<double> nw[3] = {{uu.x - iq_x.x, uu.y - iq_x.y}, vn, {ww.x - iq_z.x, ww.y - iq_z.y}};
+  double s = 0.0;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const size_t idx = c * comp + row + y;
+    const cx<double> old = a.uj[idx];
+    const cx<double> d = {nw[c].x - old.x, nw[c].y - old.y};
+    if (store) {
+      a.out[idx] = nw[c];
+      if (a.corr) a.corr[idx] = d;
+    }
+    s += km * (d.x * d.x) + km * (d.y * d.y);
+  }
+  return s;
+}
+
 // Phase 1 (flows/plane_couette.py:427-445): d_hat^n into LDS, then f_hat
 // (zero on the walls) -> V_F.
 template <int PO>
 __global__ void __launch_bounds__(kImmT) k_imm_fhat(ImmArgs a) {
   if (a.act && *a.act == 0) return;
-  constexpr int S1 = PO + 1, S2 = PO + 2;
   // LDS: d_hat, then the y rows of u^n_y, N_j,y and N_n,y (the D1 stencils'
   // operands, read once from HBM)
   extern __shared__ cx<double> dh[];
@@ -877,8 +990,6 @@ __global__ void __launch_bounds__(kImmT) k_imm_fhat(ImmArgs a) {
   const size_t comp = (size_t)a.M * Ny, row = (size_t)m * Ny;
   const double kx = a.mkx[m], kz = a.mkz[m];
   const double k2 = kx * kx + kz * kz;
-  const cx<double>* un0 = a.un + row;
-  const cx<double>* un2 = a.un + 2 * comp + row;
   cx<double>* un1 = dh + Ny;
   cx<double>* nj1 = dh + 2 * Ny;
   cx<double>* nn1 = dh + 3 * Ny;
@@ -888,34 +999,11 @@ __global__ void __launch_bounds__(kImmT) k_imm_fhat(ImmArgs a) {
     nn1[y] = a.nn[comp + row + y];
   }
   __syncthreads();
-  for (int y = threadIdx.x; y < Ny; y += blockDim.x) {
-    const cx<double> dv = stencil<S1>(a.d1w, a.d1j0, un1, y);
-    const cx<double> t0 = ikm(kx, un0[y]);
-    const cx<double> t2 = ikm(kz, un2[y]);
-    dh[y] = {(t0.x + dv.x) + t2.x, (t0.y + dv.y) + t2.y};
-  }
+  for (int y = threadIdx.x; y < Ny; y += blockDim.x) dh[y] = imm_dhat_at<PO>(a, un1, row, comp, kx, kz, y);
   __syncthreads();
-  const double dt = a.dt, cc = a.c, nu = a.nu;
   cx<double>* f = svec(a, V_F, m);
-  const size_t M = a.M;
-  for (int y = threadIdx.x; y < Ny; y += blockDim.x) {
-    if (y <= 0 || y >= Ny - 1) {
-      f[y * VS] = {0, 0};
-      continue;
-    }
-    const cx<double> d = dh[y];
-    const cx<double> dvj = stencil<S1>(a.d1w, a.d1j0, nj1, y);
-    const cx<double> dvn = stencil<S1>(a.d1w, a.d1j0, nn1, y);
-    const cx<double> aj = ikm(kx, a.nj[row + y]), cj = ikm(kz, a.nj[2 * comp + row + y]);
-    const cx<double> an = ikm(kx, a.nn[row + y]), cn = ikm(kz, a.nn[2 * comp + row + y]);
-    const cx<double> divj = {(aj.x + dvj.x) + cj.x, (aj.y + dvj.y) + cj.y};
-    const cx<double> divn = {(an.x + dvn.x) + cn.x, (an.y + dvn.y) + cn.y};
-    const cx<double> d2d = stencil<S2>(a.d2w, a.d2j0, dh, y);
-    const cx<double> lkd = {d2d.x - k2 * d.x, d2d.y - k2 * d.y};
-    const double g = (1 - cc) * nu;
-    f[y * VS] = {((d.x / dt + cc * divj.x) + (1 - cc) * divn.x) + g * lkd.x,
-                ((d.y / dt + cc * divj.y) + (1 - cc) * divn.y) + g * lkd.y};
-  }
+  for (int y = threadIdx.x; y < Ny; y += blockDim.x)
+    f[y * VS] = imm_fhat_at<PO>(a, dh, nj1, nn1, row, comp, kx, kz, k2, y);
 }
 
 // Phases 2 and 4: solve ab x = rhs per (mode, right-hand side r =
@@ -1171,6 +1259,15 @@ __device__ __forceinline__ void back_chunk(const double (&g)[kChunk + 2 * PO],
   wave_sync();
 }
 
+// A launch's workgroups go to the 8 XCDs (each with its own L2) in turn:
+// renumbered so that each XCD takes one contiguous range of slots, where
+// neighbouring modes (grouped by k^2) share chunk-map cache lines.
+__device__ __forceinline__ int xcd_block(int b, int G) {
+  constexpr int X = 8;
+  const int q = G / X, r = G % X, x = b % X, i = b / X;
+  return x * q + (x < r ? x : r) + i;
+}
+
 template <int PO, int NR, int S>
 __global__ void __launch_bounds__(64, chunk_waves(PO, NR))
     k_band_chunks(ImmArgs a, const double* __restrict__ fmap, const double* __restrict__ bmap,
@@ -1182,7 +1279,7 @@ __global__ void __launch_bounds__(64, chunk_waves(PO, NR))
   extern __shared__ cx<double> sh[];  // [2][NR][Ny] vectors, then [2][NR][32] windows
   const int Ny = a.Ny, M = a.M, nC = (Ny + B - 1) / B;
   const int lane = threadIdx.x, half = lane >> 5, t = lane & 31;
-  const int slot = blockIdx.x * 2 + half;
+  const int slot = xcd_block(blockIdx.x, gridDim.x) * 2 + half;
   const bool live = slot < M;  // a spare half repeats the last mode and stores nothing
   const int2 mu = sched[slot];
   const int m = mu.x, u = mu.y;
@@ -1253,12 +1350,161 @@ __global__ void __launch_bounds__(64, chunk_waves(PO, NR))
   }
 }
 
+// Fused chunked form: two launches per influence-matrix step instead of
+// five.  Phases 1 and 2 (k_imm_lsolve): each mode's f_hat is formed by its
+// 32 lanes straight into the LDS row the chunked L_k solve starts from; the
+// solution p_P goes to V_P.  Phases 3 to 5 (k_imm_hsolve): R_c for the three
+// components straight into LDS, the H_k solve with its results kept in
+// LDS, then alpha and the assembly.  The per-row arithmetic and the chunk
+// chains are those of the five-launch form (same values, bit for bit); only
+// the work vectors' round trips through memory and three launches go.  The
+// forward and back chunk loops are those of k_band_chunks.
+template <int PO, int NR, int S>
+__device__ __forceinline__ void chunk_solve(const double* __restrict__ fmap, const double* __restrict__ bmap,
+                                            int U, int u, int t, int nC, int Ny, cx<double>* vec, cx<double>* win,
+                                            double (&f)[S][kChunk + PO], const unsigned (&of)[kChunk + PO],
+                                            cx<double>* __restrict__ out, size_t vs, bool live) {
+  constexpr int KL = PO, KUU = 2 * PO, B = kChunk, WF = B + KL, WB = B + KUU;
+  wave_sync();
+  if (t < WF) {
+#pragma unroll
+    for (int r = 0; r < NR; ++r) win[r * 32 + t] = t < Ny ? vec[r * Ny + t] : cx<double>{0, 0};
+  }
+  wave_sync();
+  for (int c = 0; c < nC; c += S) {
+#pragma unroll
+    for (int q = 0; q < S; ++q) {
+      if (c + q >= nC) break;
+      fwd_chunk<PO, NR, S>(f[q], f[(q + S - 1) % S], fmap, of, U, vec, win, t, c + q, nC, Ny);
+    }
+  }
+  double g[S][WB];
+  unsigned og[WB];
+  map_offsets<WB, 0>(og, U, u, t);
+#pragma unroll
+  for (int q = 0; q < S - 1; ++q) map_row<WB, 0>(g[q], bmap, og, U, q, nC);
+  if (t < WB) {
+#pragma unroll
+    for (int r = 0; r < NR; ++r) win[r * 32 + t] = (Ny - 1 - t >= 0) ? vec[r * Ny + Ny - 1 - t] : cx<double>{0, 0};
+  }
+  wave_sync();
+  for (int c = 0; c < nC; c += S) {
+#pragma unroll
+    for (int q = 0; q < S; ++q) {
+      if (c + q >= nC) break;
+      back_chunk<PO, NR, S>(g[q], g[(q + S - 1) % S], bmap, og, U, out, vs, live, vec, win, t, c + q, nC, Ny);
+    }
+  }
+}
+
+// LDS words (complex) per workgroup of the fused kernels
+__host__ __device__ constexpr size_t imm_lsolve_lds(int Ny) { return 2 * ((size_t)5 * Ny + 32); }
+__host__ __device__ constexpr size_t imm_hsolve_lds(int Ny) { return 2 * ((size_t)7 * Ny + 96); }
+
+template <int PO, int S>
+__global__ void __launch_bounds__(64, chunk_waves(PO, 1))
+    k_imm_lsolve(ImmArgs a, const double* __restrict__ fmap, const double* __restrict__ bmap,
+                 const int2* __restrict__ sched, int U) {
+  if (a.act && *a.act == 0) return;
+  constexpr int KL = PO, WF = kChunk + KL;
+  // per half: the solve's vector [Ny], its window [32], then d_hat and the
+  // y rows of u^n_y, N_j,y, N_n,y [4][Ny]
+  extern __shared__ cx<double> sh[];
+  const int Ny = a.Ny, M = a.M, nC = (Ny + kChunk - 1) / kChunk;
+  const int lane = threadIdx.x, half = lane >> 5, t = lane & 31;
+  const int slot = xcd_block(blockIdx.x, gridDim.x) * 2 + half;
+  const bool live = slot < M;  // a spare half repeats the last mode and stores nothing
+  const int2 mu = sched[slot];
+  const int m = mu.x, u = mu.y;
+  cx<double>* vec = sh + (size_t)half * Ny;
+  cx<double>* win = sh + (size_t)2 * Ny + (size_t)half * 32;
+  cx<double>* dh = sh + (size_t)2 * Ny + 64 + (size_t)half * 4 * Ny;
+  cx<double>* un1 = dh + Ny;
+  cx<double>* nj1 = dh + 2 * Ny;
+  cx<double>* nn1 = dh + 3 * Ny;
+  // the first maps' loads, then the right-hand side
+  double f[S][WF];
+  unsigned of[WF];
+  map_offsets<WF, KL>(of, U, u, t);
+#pragma unroll
+  for (int q = 0; q < S - 1; ++q) map_row<WF, KL>(f[q], fmap, of, U, q, nC);
+  const size_t comp = (size_t)M * Ny, row = (size_t)m * Ny;
+  const double kx = a.mkx[m], kz = a.mkz[m];
+  const double k2 = kx * kx + kz * kz;
+  for (int y = t; y < Ny; y += 32) {
+    un1[y] = a.un[comp + row + y];
+    nj1[y] = a.nj[comp + row + y];
+    nn1[y] = a.nn[comp + row + y];
+  }
+  wave_sync();
+  for (int y = t; y < Ny; y += 32) dh[y] = imm_dhat_at<PO>(a, un1, row, comp, kx, kz, y);
+  wave_sync();
+  for (int y = t; y < Ny; y += 32) vec[y] = imm_fhat_at<PO>(a, dh, nj1, nn1, row, comp, kx, kz, k2, y);
+  chunk_solve<PO, 1, S>(fmap, bmap, U, u, t, nC, Ny, vec, win, f, of, a.scratch + ((size_t)V_P * M + m) * Ny,
+                        (size_t)M * Ny, live);
+}
+
+template <int PO, int S>
+__global__ void __launch_bounds__(64, chunk_waves(PO, 3))
+    k_imm_hsolve(ImmArgs a, const double* __restrict__ fmap, const double* __restrict__ bmap,
+                 const int2* __restrict__ sched, int U) {
+  if (a.act && *a.act == 0) return;
+  constexpr int KL = PO, WF = kChunk + KL;
+  // per half: the solve's vectors [3][Ny], windows [3][32], then the p_P
+  // row and the rows of u^n [4][Ny], whose place the solutions take
+  extern __shared__ cx<double> sh[];
+  const int Ny = a.Ny, M = a.M, nC = (Ny + kChunk - 1) / kChunk;
+  const int lane = threadIdx.x, half = lane >> 5, t = lane & 31;
+  const int slot = xcd_block(blockIdx.x, gridDim.x) * 2 + half;
+  const bool live = slot < M;
+  const int2 mu = sched[slot];
+  const int m = mu.x, u = mu.y;
+  cx<double>* vec = sh + (size_t)half * 3 * Ny;
+  cx<double>* win = sh + (size_t)6 * Ny + (size_t)half * 96;
+  cx<double>* pp = sh + (size_t)6 * Ny + 192 + (size_t)half * 4 * Ny;
+  cx<double>* ur = pp + Ny;
+  double f[S][WF];
+  unsigned of[WF];
+  map_offsets<WF, KL>(of, U, u, t);
+#pragma unroll
+  for (int q = 0; q < S - 1; ++q) map_row<WF, KL>(f[q], fmap, of, U, q, nC);
+  const size_t comp = (size_t)M * Ny, row = (size_t)m * Ny;
+  const double kx = a.mkx[m], kz = a.mkz[m];
+  const double k2 = kx * kx + kz * kz;
+  const cx<double>* P = a.scratch + ((size_t)V_P * M + m) * Ny;
+  for (int y = t; y < Ny; y += 32) {
+    pp[y] = P[y];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) ur[c * Ny + y] = a.un[c * comp + row + y];
+  }
+  wave_sync();
+  for (int y = t; y < Ny; y += 32) {
+    cx<double> r[3];
+    imm_r_at<PO>(a, pp, ur, row, comp, kx, kz, k2, y, r);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) vec[c * Ny + y] = r[c];
+  }
+  // the solutions overwrite the p_P row and the u^n rows, read above
+  // before the solve's first wave_sync
+  cx<double>* sol = pp;
+  chunk_solve<PO, 3, S>(fmap, bmap, U, u, t, nC, Ny, vec, win, f, of, sol, (size_t)Ny, true);
+  wave_sync();
+  cx<double> al1, al2;
+  imm_alpha<PO>(a, m, sol + Ny, 1, al1, al2);
+  double acc = 0.0;
+  for (int y = t; y < Ny; y += 32)
+    acc += a.qw[y] * imm_assemble_at(a, m, row, comp, kx, kz, sol, sol + Ny, sol + 2 * Ny, 1, al1, al2, y, live);
+  // the mode's 32 lanes: a fixed pairwise order
+#pragma unroll
+  for (int o = 16; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 32);
+  if (t == 0 && live && a.partials) a.partials[m] = acc;
+}
+
 // Phase 3 (flows/plane_couette.py:449-472): R_c for the three components
 // (zero on the walls) -> V_R + c, with the pressure row in LDS.
 template <int PO>
 __global__ void __launch_bounds__(kImmT) k_imm_rc(ImmArgs a) {
   if (a.act && *a.act == 0) return;
-  constexpr int S1 = PO + 1, S2 = PO + 2;
   // LDS: the p_P row, then the rows of u^n (the D2 stencils' operands)
   extern __shared__ cx<double> pp[];
   const int m = blockIdx.x, Ny = a.Ny;
@@ -1266,7 +1512,6 @@ __global__ void __launch_bounds__(kImmT) k_imm_rc(ImmArgs a) {
   const double kx = a.mkx[m], kz = a.mkz[m];
   const double k2 = kx * kx + kz * kz;
   const cx<double>* P = svec(a, V_P, m);
-  const size_t M = a.M;
   cx<double>* ur = pp + Ny;  // [c][y]
   for (int y = threadIdx.x; y < Ny; y += blockDim.x) {
     pp[y] = P[y * VS];
@@ -1274,27 +1519,11 @@ __global__ void __launch_bounds__(kImmT) k_imm_rc(ImmArgs a) {
     for (int c = 0; c < 3; ++c) ur[c * Ny + y] = a.un[c * comp + row + y];
   }
   __syncthreads();
-  const double dt = a.dt, cc = a.c, nu = a.nu;
   for (int y = threadIdx.x; y < Ny; y += blockDim.x) {
-    if (y <= 0 || y >= Ny - 1) {
+    cx<double> r[3];
+    imm_r_at<PO>(a, pp, ur, row, comp, kx, kz, k2, y, r);
 #pragma unroll
-      for (int c = 0; c < 3; ++c) svec(a, V_R + c, m)[y * VS] = {0, 0};
-      continue;
-    }
-    const cx<double> p = pp[y];
-    const cx<double> dpp[3] = {ikm(kx, p), stencil<S1>(a.d1w, a.d1j0, pp, y), ikm(kz, p)};
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const cx<double>* unc = ur + c * Ny;
-      const cx<double> u = unc[y];
-      const cx<double> du = stencil<S2>(a.d2w, a.d2j0, unc, y);
-      const cx<double> l = {du.x - k2 * u.x, du.y - k2 * u.y};
-      const double g = (1.0 - cc) * nu;
-      const cx<double> hm = {(1.0 / dt) * u.x + g * l.x, (1.0 / dt) * u.y + g * l.y};
-      const cx<double> Nj = a.nj[c * comp + row + y], Nn = a.nn[c * comp + row + y];
-      svec(a, V_R + c, m)[y * VS] = {((hm.x - dpp[c].x) + cc * Nj.x) + (1 - cc) * Nn.x,
-                                   ((hm.y - dpp[c].y) + cc * Nj.y) + (1 - cc) * Nn.y};
-    }
+    for (int c = 0; c < 3; ++c) svec(a, V_R + c, m)[y * VS] = r[c];
   }
 }
 
@@ -1304,46 +1533,18 @@ __global__ void __launch_bounds__(kImmT) k_imm_rc(ImmArgs a) {
 template <int PO>
 __global__ void __launch_bounds__(kImmT) k_imm_final(ImmArgs a) {
   if (a.act && *a.act == 0) return;
-  constexpr int S1 = PO + 1;
   __shared__ double red[kImmT / 64];
   const int m = blockIdx.x, Ny = a.Ny, M = a.M;
   const size_t comp = (size_t)M * Ny, row = (size_t)m * Ny;
   const double kx = a.mkx[m], kz = a.mkz[m];
-  const bool k2zero = (kx * kx + kz * kz == 0.0);
   const cx<double>* ua = svec(a, V_R + 0, m);
   const cx<double>* va = svec(a, V_R + 1, m);
   const cx<double>* wa = svec(a, V_R + 2, m);
-  cx<double> dw0 = stencil<S1>(a.d1w, a.d1j0, va, 0, VS),
-             dw1 = stencil<S1>(a.d1w, a.d1j0, va, Ny - 1, VS);
-  if (k2zero) dw0 = {0, 0};
-  const double m00 = a.minv[0 * (size_t)M + m], m01 = a.minv[1 * (size_t)M + m];
-  const double m10 = a.minv[2 * (size_t)M + m], m11 = a.minv[3 * (size_t)M + m];
-  const cx<double> al1 = {-(m00 * dw0.x + m01 * dw1.x), -(m00 * dw0.y + m01 * dw1.y)};
-  const cx<double> al2 = {-(m10 * dw0.x + m11 * dw1.x), -(m10 * dw0.y + m11 * dw1.y)};
-  const double km = (m % a.Nkx == 0) ? 1.0 : 2.0;
+  cx<double> al1, al2;
+  imm_alpha<PO>(a, m, va, VS, al1, al2);
   double acc = 0.0;
-  const double4* __restrict__ vq = reinterpret_cast<const double4*>(a.vq) + (size_t)a.umode[m] * Ny;
-  for (int y = threadIdx.x; y < Ny; y += blockDim.x) {
-    const double4 w = vq[y];
-    const double v1 = w.x, v2 = w.y, q1 = w.z, q2 = w.w;
-    const cx<double> vv = va[(size_t)y * VS];
-    const cx<double> vn = {(vv.x + al1.x * v1) + al2.x * v2, (vv.y + al1.y * v1) + al2.y * v2};
-    const cx<double> q = {al1.x * q1 + al2.x * q2, al1.y * q1 + al2.y * q2};
-    const cx<double> uu = ua[(size_t)y * VS], ww = wa[(size_t)y * VS];
-    const cx<double> iq_x = ikm(kx, q), iq_z = ikm(kz, q);
-    const cx<double> nw[3] = {{uu.x - iq_x.x, uu.y - iq_x.y}, vn, {ww.x - iq_z.x, ww.y - iq_z.y}};
-    double s = 0.0;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const size_t idx = c * comp + row + y;
-      const cx<double> old = a.uj[idx];
-      const cx<double> d = {nw[c].x - old.x, nw[c].y - old.y};
-      a.out[idx] = nw[c];
-      if (a.corr) a.corr[idx] = d;
-      s += km * (d.x * d.x) + km * (d.y * d.y);
-    }
-    acc += a.qw[y] * s;
-  }
+  for (int y = threadIdx.x; y < Ny; y += blockDim.x)
+    acc += a.qw[y] * imm_assemble_at(a, m, row, comp, kx, kz, ua, va, wa, VS, al1, al2, y, true);
   const double s = block_sum(acc, red);
   if (threadIdx.x == 0 && a.partials) a.partials[m] = s;
 }
@@ -1510,6 +1711,25 @@ static int walls_imm(Ctx* c, const double* un, const double* uj, const double* n
     const size_t lds_c3 = (size_t)2 * 3 * (t->Ny + 32) * sizeof(cx<double>);
     if (4 * lds > 65536 || (!c->sweep_seq && lds_c3 > 65536))
       return c->fail(DNSMI_EINVAL, "imm: ny too large for the per-mode LDS rows");
+    // the fused form reads u^n while other modes already store the new
+    // velocity: it needs them apart
+    if (c->imm_fused && !c->sweep_seq && out != un) {
+      const size_t lds_l = imm_lsolve_lds(t->Ny) * sizeof(cx<double>);
+      const size_t lds_h = imm_hsolve_lds(t->Ny) * sizeof(cx<double>);
+      if (lds_h > 65536) return c->fail(DNSMI_EINVAL, "imm: ny too large for the fused form's LDS rows");
+      switch (t->p) {
+#define IMMF(PO)                                                                                   \
+  case PO:                                                                                         \
+    hipLaunchKernelGGL((k_imm_lsolve<PO, chunk_slots(PO, 1)>), gc, dim3(64), lds_l, s, a, t->lk_fmap, \
+                       t->lk_bmap, t->sched, t->U);                                                \
+    hipLaunchKernelGGL((k_imm_hsolve<PO, chunk_slots(PO, 3)>), gc, dim3(64), lds_h, s, a, t->hk_fmap, \
+                       t->hk_bmap, t->sched, t->U);                                                \
+    break;
+        IMMF(2) IMMF(4) IMMF(6) IMMF(8)
+#undef IMMF
+        default: return c->fail(DNSMI_EINVAL, "fd_order");
+      }
+    } else
     switch (t->p) {
 #define IMM(PO)                                                                                   \
   case PO:                                                                                        \
@@ -1542,30 +1762,36 @@ static int walls_imm(Ctx* c, const double* un, const double* uj, const double* n
 // Chooses the band-solve form by timing one influence-matrix step of each
 // on zeroed buffers (both forms are exact solves of the same systems).
 static int walls_autotune(Ctx* c) {
-  if (hipMemset(c->rhs0, 0, c->spec_elems * 16) != hipSuccess)
+  if (hipMemset(c->rhs0, 0, c->spec_elems * 16) != hipSuccess ||
+      hipMemset(c->bufA, 0, c->spec_elems * 16) != hipSuccess)
     return c->fail(DNSMI_EHIP, "autotune: memset");
+  // zero fields in, the new velocity into a buffer of its own (the fused
+  // form needs it apart from u^n); bufA is transform scratch, unused here
   double* z = c->rhs0;
+  double* o = static_cast<double*>(c->bufA);
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
     return c->fail(DNSMI_EHIP, "autotune: events");
   // a form that cannot run here (nonzero return) never wins
-  auto time_imm = [&](bool seq) {
+  auto time_imm = [&](bool seq, bool fused) {
     c->sweep_seq = seq;
-    if (walls_imm(c, z, z, z, z, z, nullptr, nullptr, nullptr) != DNSMI_OK)  // warm-up
+    c->imm_fused = fused;
+    if (walls_imm(c, z, z, z, z, o, nullptr, nullptr, nullptr) != DNSMI_OK)  // warm-up
       return (double)INFINITY;
     (void)hipEventRecord(e0, nullptr);
     for (int i = 0; i < 3; ++i)
-      if (walls_imm(c, z, z, z, z, z, nullptr, nullptr, nullptr) != DNSMI_OK) return (double)INFINITY;
+      if (walls_imm(c, z, z, z, z, o, nullptr, nullptr, nullptr) != DNSMI_OK) return (double)INFINITY;
     (void)hipEventRecord(e1, nullptr);
     (void)hipEventSynchronize(e1);
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, e0, e1);
     return ms / 3.0;
   };
-  const double tc = time_imm(false), ts = time_imm(true);
+  const double tc = time_imm(false, false), tf = time_imm(false, true), ts = time_imm(true, false);
   if (!(ts < INFINITY)) return c->fail(DNSMI_EINVAL, "autotune: the sequential band solve failed: " + c->err);
   c->err.clear();
-  c->sweep_seq = ts < tc;
+  c->sweep_seq = ts < tc && ts < tf;
+  c->imm_fused = !c->sweep_seq && tf < tc;
   if (c->sweep_seq) {  // the chunk maps are not needed any more
     WallTables* t = c->wt;
     for (double** p : {&t->lk_fmap, &t->lk_bmap, &t->hk_fmap, &t->hk_bmap}) {
@@ -1577,9 +1803,9 @@ static int walls_autotune(Ctx* c) {
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  char buf[120];
-  snprintf(buf, sizeof buf, "band_solve=%s(chunked %.1f us, sequential %.1f us) ",
-           c->sweep_seq ? "sequential" : "chunked", 1e3 * tc, 1e3 * ts);
+  char buf[160];
+  snprintf(buf, sizeof buf, "band_solve=%s(chunked %.1f us, fused %.1f us, sequential %.1f us) ",
+           c->sweep_seq ? "sequential" : (c->imm_fused ? "fused" : "chunked"), 1e3 * tc, 1e3 * tf, 1e3 * ts);
   c->tuning = buf;
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? DNSMI_OK : c->fail(DNSMI_EHIP, std::string("autotune: ") + hipGetErrorString(e));

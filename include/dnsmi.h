@@ -76,7 +76,9 @@ typedef struct dnsmi_config {
                               that tile and `variants` as given */
   int variants;            /* bit mask when tile_cols != 0: 1 sequential
                               y-inverse, 2 sequential z-inverse, 4 split x pass,
-                              8 sequential band sweeps (walls; else chunk maps) */
+                              8 sequential band sweeps (walls; else chunk maps),
+                              16 the chunk maps' fused two-launch influence-
+                              matrix step (walls, without 8) */
   int plain_mixed;         /* DNSMI_MIXED only: 0 (default) evaluates every
                               corrector pass of a step as an FP32 increment over
                               the step's first evaluation (split precision);

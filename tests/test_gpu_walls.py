@@ -78,10 +78,10 @@ def test_setup_tables_match_oracle(case):
 
 
 @pytest.mark.parametrize("case", list(CASES))
-@pytest.mark.parametrize("variants", [None, 0, 8], ids=["autotuned", "chunked", "sequential"])
+@pytest.mark.parametrize("variants", [None, 0, 16, 8], ids=["autotuned", "chunked", "fused", "sequential"])
 def test_golden_rhs_and_imm_step(case, variants):
-    """Both band-solve forms (chunk maps; sequential sweeps) and the
-    autotuner's choice against the golden steps."""
+    """The band-solve forms (chunk maps in five launches or fused into two;
+    sequential sweeps) and the autotuner's choice against the golden steps."""
     g = np.load(os.path.join(GOLDEN, case + ".npz"))
     flow = make(case) if variants is None else make(case, tile_cols=16, variants=variants)
     u = flow.to_device(g["state"])
@@ -120,7 +120,7 @@ def test_generic_callables_match_fused(case):
 
 
 @pytest.mark.parametrize("ny", [33, 32])
-@pytest.mark.parametrize("variants", [0, 8], ids=["chunked", "sequential"])
+@pytest.mark.parametrize("variants", [0, 16, 8], ids=["chunked", "fused", "sequential"])
 def test_run_counts_and_per_step_error(ny, variants):
     """Plane Couette 16 x ny x 16, p = 4, Re 400, dt 0.005, tol 1e-10: 10 steps.
 
@@ -186,7 +186,7 @@ def test_walls_stats_match_oracle(case):
 
 
 @pytest.mark.parametrize("p", [2, 8])
-@pytest.mark.parametrize("variants", [0, 8], ids=["chunked", "sequential"])
+@pytest.mark.parametrize("variants", [0, 16, 8], ids=["chunked", "fused", "sequential"])
 def test_fd_orders_2_and_8_against_oracle(p, variants):
     """The remaining stencil orders (the goldens hold 4 and 6): 8 x 33 x 8,
     Re 400, dt 0.01.  The RHS to 1e-12, one predictor-corrector step and
@@ -234,7 +234,7 @@ def test_implicitness_against_oracle(c):
     assert rel(got2, want2) < 1e-12
 
 
-@pytest.mark.parametrize("variants", [0, 8], ids=["chunked", "sequential"])
+@pytest.mark.parametrize("variants", [0, 16, 8], ids=["chunked", "fused", "sequential"])
 def test_incommensurate_box_against_oracle(variants):
     """Lx = 3, Lz = 1.7: only the +-kz pairs share k^2 (the set-up computes
     the per-mode tables and chunk maps once per distinct k^2, bit for bit),
@@ -259,7 +259,7 @@ def test_incommensurate_box_against_oracle(variants):
     assert rel(got2, want2) < 1e-12
 
 
-@pytest.mark.parametrize("variants", [0, 8], ids=["chunked", "sequential"])
+@pytest.mark.parametrize("variants", [0, 16, 8], ids=["chunked", "fused", "sequential"])
 def test_config2_shape_odd_ny_against_oracle(variants):
     """Config 2's shape with the odd ny where the reference's quadrature is
     defined (64 x 129 x 64, p = 4, Re 400, dt 0.01, seed 2, RMS 0.05) against
@@ -299,3 +299,25 @@ def test_config2_shape_odd_ny_against_oracle(variants):
     print(f"MEASURED c2_odd variants={variants} counts {r.counts} worst per-step rel {worst:.3e}")
     assert r.counts == ref_counts == [4, 4, 4, 4, 4]
     assert worst < 1e-12, worst
+
+
+@pytest.mark.parametrize("shape", [(16, 33, 16, 4), (64, 129, 64, 4), (8, 33, 8, 8), (16, 34, 8, 6)])
+def test_fused_imm_equals_five_launch_form(shape):
+    """The fused two-launch influence-matrix step computes the five-launch
+    chunked form's values bit for bit (same per-row arithmetic and chunk
+    chains): predictor, corrector and correction equal, the error norms
+    (summed in another order) to 1e-14."""
+    from dnsjax_amd.fields import wall_field
+
+    nx, ny, nz, p = shape
+    u0 = wall_field(nx, ny, nz, 2 * math.pi, math.pi, 0.05, seed=3)
+    out = {}
+    for v in (0, 16):
+        flow = make(nx=nx, ny=ny, nz=nz, fd_order=p, re=400.0, dt=0.01, tile_cols=16, variants=v)
+        u = flow.to_device(u0)
+        pred, rhs1, err1 = flow.predict_and_correct(u)
+        pred2, rhs2, err2 = flow.iterate_correction(u, pred, rhs1)
+        out[v] = (pred, pred2, rhs2, float(err1), float(err2))
+    a, b = out[0], out[16]
+    assert torch.equal(a[0], b[0]) and torch.equal(a[1], b[1]) and torch.equal(a[2], b[2])
+    assert abs(a[3] - b[3]) <= 1e-14 * a[3] and abs(a[4] - b[4]) <= 1e-14 * a[4]

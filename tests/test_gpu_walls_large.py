@@ -48,10 +48,12 @@ def test_band_solve_forms_agree_with_wall_invariants():
 
     u0 = wall_field(NX, NY, NZ, 2 * math.pi, math.pi, 0.05, seed=4)
     out = {}
-    for name, variants in (("chunked", 0), ("sequential", 8)):
+    for name, variants in (("chunked", 0), ("fused", 16), ("sequential", 8)):
         flow = make(tile_cols=16, variants=variants)
         pred, rhs, err = flow.predict_and_correct(flow.to_device(u0))
         out[name] = (pred, float(err), flow)
+    assert torch.equal(out["chunked"][0], out["fused"][0])
+    assert abs(out["chunked"][1] - out["fused"][1]) <= 1e-14 * out["chunked"][1]
     assert rel(out["chunked"][0], out["sequential"][0]) < 1e-11
     assert abs(out["chunked"][1] - out["sequential"][1]) <= 1e-9 * out["sequential"][1]
     pred, _, flow = out["chunked"]
