@@ -51,6 +51,8 @@ class DnsmiConfig(ctypes.Structure):
         ("tile_cols", ctypes.c_int),
         ("variants", ctypes.c_int),
         ("plain_mixed", ctypes.c_int),
+        ("recenter_far", ctypes.c_double),
+        ("recenter_near", ctypes.c_double),
     ]
 
 
@@ -79,6 +81,7 @@ SIGNATURES = {
     "dnsmi_step_device": (_I, [_P, _P, _P, _P, _P, _I, _I, _P, _I, ctypes.c_double, _I, ctypes.c_double,
                                ctypes.c_double, _P]),
     "dnsmi_step_stats": (_I, [_P, ctypes.POINTER(ctypes.c_double)]),
+    "dnsmi_split_rerecord": (_I, [_P]),
     "dnsmi_step_reset": (_I, [_P]),
     "dnsmi_escalate": (_I, [_P, _I]),
     "dnsmi_mlp_sizes": (_I, [_I, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong),

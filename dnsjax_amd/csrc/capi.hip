@@ -79,7 +79,7 @@ int get_stats(Ctx* c, const double* state, double* out, hipStream_t s);
 int step_device(Ctx* c, const double* state, double* pred, double* rhs_a, double* rhs_b, int slots,
                 int esc_slots, int* counts, int counts_cap, double tol, int max_iter,
                 double esc_ratio, double esc_cap, hipStream_t s);
-int step_read_ctl(Ctx* c, double* out6);
+int step_read_ctl(Ctx* c, double* out7);
 int step_device_guess(Ctx* c, const double* state, double* pred, double* rhs_a, double* rhs_b,
                       float* h1, float* h2, float* h3, const uint16_t* wb, const float* bias, int H,
                       float s_1, float s_2, float s_out, int mode, int slots, int esc_slots,
@@ -160,6 +160,9 @@ int dnsmi_create(const dnsmi_config* cfg, dnsmi_ctx** out) {
     rc = c->fail(DNSMI_EINVAL, "tile_cols must be 0 (autotune), 8 or 16");
   } else if (!(cfg->re > 0) || !(cfg->dt > 0) || cfg->implicitness < 0 || cfg->implicitness > 1) {
     rc = c->fail(DNSMI_EINVAL, "need re > 0, dt > 0, 0 <= implicitness <= 1");
+  } else if (cfg->recenter_far < 0 || cfg->recenter_near < 0 ||
+             (cfg->recenter_far > 0 && !(cfg->recenter_near < cfg->recenter_far))) {
+    rc = c->fail(DNSMI_EINVAL, "need 0 <= recenter_near < recenter_far (or recenter_far = 0)");
   } else {
     c->fp32 = cfg->precision == DNSMI_MIXED;
     c->fp32_primary = c->fp32;
@@ -296,6 +299,13 @@ int dnsmi_step_device(dnsmi_ctx* ctx, const double* state, double* next, double*
                 "step_device: buffers must be distinct");
   return dnsmi::step_device(c, state, next, rhs_a, rhs_b, slots, esc_slots, counts, counts_cap, tol,
                             max_iter, esc_ratio, esc_cap, as_stream(stream));
+}
+
+int dnsmi_split_rerecord(dnsmi_ctx* ctx) {
+  DNSMI_CHECK_CTX(ctx)
+  DNSMI_REQUIRE(c->split, "split_rerecord: the context has no split evaluations");
+  c->ref_valid = false;
+  return DNSMI_OK;
 }
 
 int dnsmi_step_stats(dnsmi_ctx* ctx, double* out6) {

@@ -44,6 +44,10 @@ struct CtlRule {
   int max_iter;
   double ratio, cap;
   int primary;
+  // split evaluations: a step whose predict-and-correct error exceeds `far`
+  // records a new reference at its first pass that starts from an error of
+  // at most `near` (far = infinity: never)
+  double far, near;
 };
 
 // How one evaluation of the nonlinear term treats the mixed form's

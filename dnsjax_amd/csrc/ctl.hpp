@@ -8,6 +8,9 @@
 
 namespace dnsmi {
 
+// Corrector slots of one device step at most (slot i is pass i).
+constexpr int kMaxSlots = 64;
+
 struct StepCtl {
   double err;
   double prev;   // error of the previous pass
@@ -22,7 +25,23 @@ struct StepCtl {
   int unconverged;
   int nan;
   int escalated;  // steps that escalated
+  int rec;        // split reference: 0 as recorded, 1 to re-record (far), 2 re-recorded,
+                  // 3 recorded by the first pass (a guessed step): decided after it
+  int recorded;   // steps that re-recorded their reference
+  // run flags of primary slot i (set by pass i - 1): rp the slot's
+  // increment evaluation, rr its re-recording evaluation (at most one of
+  // the two runs)
+  int rp[kMaxSlots + 1];
+  int rr[kMaxSlots + 1];
 };
+
+// Start of a step's corrector: no slot runs unless set below.
+__device__ inline void ctl_clear_slots(StepCtl* ctl) {
+  for (int i = 0; i <= kMaxSlots; ++i) {
+    ctl->rp[i] = 0;
+    ctl->rr[i] = 0;
+  }
+}
 
 // After a pass with error e: count it and continue while e > tol and the
 // limit allows (__main__.py:161-191).  Mixed form: a pass that did not
@@ -30,7 +49,10 @@ struct StepCtl {
 // of at most `cap`, has reached the resolution of its FP32 evaluations, and
 // so has a step whose `primary` passes in the context's precision all ran
 // without converging; the step's remaining passes then run in FP64, so it
-// still ends at error <= tol.  One thread, active passes only.
+// still ends at error <= tol.  Split evaluations whose reference lies far
+// from the fixed point (the evaluation that recorded it ended with an error
+// above `far`) are re-centred: the first pass after the error reached
+// `near` records a new reference.  One thread, active passes only.
 __device__ inline void ctl_after_iteration(StepCtl* ctl, const CtlRule& r, double e) {
   ctl->c += 1;
   const bool more = (e > r.tol) && (ctl->c < r.max_iter);
@@ -39,11 +61,24 @@ __device__ inline void ctl_after_iteration(StepCtl* ctl, const CtlRule& r, doubl
     ctl->esc = 1;
     ctl->escalated += 1;
   }
+  // split evaluations far from the step's fixed point: once the passes have
+  // closed in (e <= near), the next pass records a new reference there
+  if (ctl->rec == 3) ctl->rec = e > r.far ? 1 : 0;
+  const bool rec = more && !ctl->esc && ctl->rec == 1 && e <= r.near;
+  if (rec) {
+    ctl->rec = 2;
+    ctl->recorded += 1;
+  }
   ctl->active = more;
   ctl->prev2 = ctl->prev;
   ctl->prev = e;
   ctl->run_p = more && !ctl->esc;
   ctl->run_e = more && ctl->esc;
+  const int next = ctl->c + 1;
+  if (next <= kMaxSlots) {
+    ctl->rp[next] = more && !ctl->esc && !rec;
+    ctl->rr[next] = rec;
+  }
 }
 
 }  // namespace dnsmi

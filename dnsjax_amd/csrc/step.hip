@@ -34,7 +34,7 @@ int walls_iterate_correction(Ctx* c, const double* state_prev, double* pred,
                              const double* rhs_prev, double* rhs_next, double* err,
                              hipStream_t s);
 
-__global__ void k_ctl_begin(StepCtl* ctl, double tol, int max_iter) {
+__global__ void k_ctl_begin(StepCtl* ctl, double tol, int max_iter, double far) {
   if (threadIdx.x != 0) return;
   const double e = ctl->err;
   ctl->c = 0;
@@ -44,6 +44,15 @@ __global__ void k_ctl_begin(StepCtl* ctl, double tol, int max_iter) {
   ctl->active = (e > tol) && (max_iter > 0);
   ctl->run_p = ctl->active;
   ctl->run_e = 0;
+  ctl->rec = e > far ? 1 : 0;
+  ctl_clear_slots(ctl);
+  ctl->rp[1] = ctl->active;
+}
+
+// The re-centring threshold of a step (split evaluations only).
+static double recenter_far(const Ctx* c, double tol) {
+  return (c->split && c->fp32_primary && c->cfg.recenter_far > 0) ? c->cfg.recenter_far * tol
+                                                                  : (double)INFINITY;
 }
 
 __global__ void k_ctl_end(StepCtl* ctl, int* counts, int counts_cap, double tol) {
@@ -75,11 +84,23 @@ static int step_slots(Ctx* c, const double* state, double* pred, double* rhs_a, 
   StepCtl* ctl = reinterpret_cast<StepCtl*>(c->ctl);
   double* err = &ctl->err;
   int rc = DNSMI_OK;
-  c->act = &ctl->run_p;
   c->fuse_ctl = true;
+  // with re-centring, each slot after the first also holds the pass as a
+  // re-recording evaluation, run instead of the increment when the control
+  // block says so (never in slot 1, whose evaluation is an increment after
+  // predict-and-correct and the recording one after a guess)
+  const bool recenter = c->split && c->fp32_primary && c->rule.far < INFINITY;
   for (int i = 1; i <= slots && rc == DNSMI_OK; ++i) {
     const double* rp = (i % 2) ? rhs_a : rhs_b;
     double* rn = (i % 2) ? rhs_b : rhs_a;
+    if (recenter && i > 1) {
+      c->act = &ctl->rr[i];
+      c->ref_valid = false;  // the evaluation records
+      rc = c->walls ? walls_iterate_correction(c, state, pred, rp, rn, err, s)
+                    : periodic_iterate_correction(c, pred, rp, rn, err, s);
+      if (rc) break;  // the recording launch left ref_valid set: the next is an increment
+    }
+    c->act = &ctl->rp[i];
     rc = c->walls ? walls_iterate_correction(c, state, pred, rp, rn, err, s)
                   : periodic_iterate_correction(c, pred, rp, rn, err, s);
   }
@@ -113,15 +134,16 @@ int step_device(Ctx* c, const double* state, double* pred, double* rhs_a, double
   double* err = &ctl->err;
   if (slots > max_iter) slots = max_iter;
   if (esc_slots > max_iter) esc_slots = max_iter;
+  if (slots > kMaxSlots) return c->fail(DNSMI_EINVAL, "step_device: more than 64 corrector slots");
   c->act = nullptr;
   c->fp32 = c->fp32_primary;
   int rc = c->walls ? walls_predict_and_correct(c, state, pred, rhs_a, err, s)
                     : periodic_predict_and_correct(c, state, pred, rhs_a, err, s);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_ctl_begin, dim3(1), dim3(64), 0, s, ctl, tol, max_iter);
-  if ((rc = check_launch(c, "ctl_begin"))) return rc;
   c->rule = CtlRule{tol, max_iter, esc_slots > 0 ? esc_ratio : 0.0, esc_cap,
-                    esc_slots > 0 ? slots : max_iter};
+                    esc_slots > 0 ? slots : max_iter, recenter_far(c, tol), c->cfg.recenter_near * tol};
+  hipLaunchKernelGGL(k_ctl_begin, dim3(1), dim3(64), 0, s, ctl, tol, max_iter, c->rule.far);
+  if ((rc = check_launch(c, "ctl_begin"))) return rc;
   if ((rc = step_slots(c, state, pred, rhs_a, rhs_b, slots, esc_slots, s))) return rc;
   if (!c->walls && (rc = periodic_correct_velocity(c, pred, nullptr, s))) return rc;
   hipLaunchKernelGGL(k_ctl_end, dim3(1), dim3(64), 0, s, ctl, counts, counts_cap, tol);
@@ -139,6 +161,9 @@ __global__ void k_ctl_begin_guess(StepCtl* ctl, int max_pass) {
   ctl->active = max_pass > 0;
   ctl->run_p = ctl->active;
   ctl->run_e = 0;
+  ctl->rec = 3;  // the guess's reference is recorded by its first pass
+  ctl_clear_slots(ctl);
+  ctl->rp[1] = ctl->active;
 }
 
 int guess_periodic(Ctx* c, const double* state, double* pred, double* rhs_prev, float* h1,
@@ -163,6 +188,7 @@ int step_device_guess(Ctx* c, const double* state, double* pred, double* rhs_a, 
   StepCtl* ctl = reinterpret_cast<StepCtl*>(c->ctl);
   if (slots > max_pass) slots = max_pass;
   if (esc_slots > max_pass) esc_slots = max_pass;
+  if (slots > kMaxSlots) return c->fail(DNSMI_EINVAL, "step_device_guess: more than 64 corrector slots");
   c->act = nullptr;
   c->fp32 = c->fp32_primary;
   int rc = c->walls ? guess_walls(c, state, pred, rhs_a, h1, h2, wb, bias, H, s_1, s_2, s_out, mode, s)
@@ -172,7 +198,7 @@ int step_device_guess(Ctx* c, const double* state, double* pred, double* rhs_a, 
   hipLaunchKernelGGL(k_ctl_begin_guess, dim3(1), dim3(64), 0, s, ctl, max_pass);
   if ((rc = check_launch(c, "ctl_begin_guess"))) return rc;
   c->rule = CtlRule{tol, max_pass, esc_slots > 0 ? esc_ratio : 0.0, esc_cap,
-                    esc_slots > 0 ? slots : max_pass};
+                    esc_slots > 0 ? slots : max_pass, recenter_far(c, tol), c->cfg.recenter_near * tol};
   if ((rc = step_slots(c, state, pred, rhs_a, rhs_b, slots, esc_slots, s))) return rc;
   if (!c->walls && (rc = periodic_correct_velocity(c, pred, nullptr, s))) return rc;
   hipLaunchKernelGGL(k_ctl_end, dim3(1), dim3(64), 0, s, ctl, counts, counts_cap, tol);
@@ -186,7 +212,8 @@ int step_alloc_ctl(Ctx* c) {
   return DNSMI_OK;
 }
 
-int step_read_ctl(Ctx* c, double* out6) {
+int step_read_ctl(Ctx* c, double* out7) {
+  double* out6 = out7;
   StepCtl h{};
   if (c->ctl) {
     if (hipDeviceSynchronize() != hipSuccess ||
@@ -199,6 +226,7 @@ int step_read_ctl(Ctx* c, double* out6) {
   out6[3] = h.nan;
   out6[4] = h.err;
   out6[5] = h.escalated;
+  out7[6] = h.recorded;
   return DNSMI_OK;
 }
 

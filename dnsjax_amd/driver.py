@@ -14,8 +14,13 @@ still stops converging (its error, at or below
 ``params.escalate_cap_value``, is not below ``params.escalate_ratio``
 times the error two passes earlier), the step's remaining passes run in
 FP64 (``RunResult.escalations`` counts such steps): every step ends at
-error <= ``corrector_tolerance``, as in the reference.  The device-decided
-step (``dnsmi_step_device``) applies the same rule.
+error <= ``corrector_tolerance``, as in the reference.  The rounding of an
+increment scales with its size; a step whose predict-and-correct error
+exceeds ``params.recenter_far_value`` (large first corrections) records a
+new reference once its passes have closed in to ``recenter_near_value``,
+so its later increments are small again (``RunResult.rerecorded`` counts
+such steps).  The device-decided step (``dnsmi_step_device``) applies the
+same rules.
 """
 
 from __future__ import annotations
@@ -32,6 +37,7 @@ class RunResult:
     converged: bool = True
     nan: bool = False
     escalations: int = 0
+    rerecorded: int = 0
 
     @property
     def rhs_evals(self) -> int:  # __main__.py:199-201
@@ -50,17 +56,32 @@ class Escalation:
         self.ratio = p.escalate_ratio if p.precision == "mixed" else None
         self.cap = p.escalate_cap_value
         self.tol = p.corrector_tolerance
+        split = p.precision == "mixed" and not p.plain_mixed
+        self.far = p.recenter_far_value if split else math.inf
+        self.near = p.recenter_near_value
         self.on = False
         self.prev = self.prev2 = math.inf
+        self.pending = False
+        self.recorded = 0
 
-    def begin(self, err0: float = math.inf) -> None:
-        self.prev, self.prev2 = err0, math.inf
+    def begin(self, err0: float | None = None) -> None:
+        """Start of a step's passes; err0 = its predict-and-correct error,
+        whose evaluation recorded the split reference (None: a step started
+        from a guess, whose first pass records it)."""
+        self.prev, self.prev2 = (math.inf if err0 is None else err0), math.inf
+        self.pending = None if err0 is None else err0 > self.far
 
     def after(self, err: float) -> None:
+        if self.pending is None:  # the pass that recorded the reference
+            self.pending = err > self.far
         if (self.ratio is not None and not self.on and self.tol < err <= self.cap
                 and err > self.ratio * self.prev2):
             self.on = True
             self.flow.escalate(True)
+        if self.pending and not self.on and self.tol < err <= self.near:
+            self.pending = False
+            self.recorded += 1
+            self.flow._native.split_rerecord()  # the next pass records
         self.prev2, self.prev = self.prev, err
 
     def end(self) -> bool:
@@ -92,6 +113,7 @@ def run(flow, state, n_steps: int, callback=None) -> RunResult:
             if c < max_it:
                 esc.after(err)
         res.escalations += int(esc.end())
+        res.rerecorded = esc.recorded
         state, _ = flow.correct_velocity(state)
         res.counts.append(c)
         res.errors.append(err)

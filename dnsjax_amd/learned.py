@@ -223,6 +223,7 @@ class GuessRunResult:
     converged: bool = True
     nan: bool = False
     escalations: int = 0
+    rerecorded: int = 0
 
     @property
     def rhs_evals(self) -> int:
@@ -305,6 +306,7 @@ def run_guessed(flow, state, n_steps: int, mode: int = 2, net: Net | None = None
             if c < max_pass:
                 esc.after(err)
         res.escalations += int(esc.end())
+        res.rerecorded = esc.recorded
         pred, state = state, pred
         state, _ = flow.correct_velocity(state)
         res.passes.append(c)

@@ -39,6 +39,7 @@ class NativeContext:
             precision=_lib.DNSMI_MIXED if p.precision == "mixed" else _lib.DNSMI_FP64,
             correct_divergence=int(p.correct_divergence), tile_cols=p.tile_cols,
             variants=p.variants, plain_mixed=int(p.plain_mixed),
+            recenter_far=p.recenter_far or 0.0, recenter_near=p.recenter_near,
         )
         handle = ctypes.c_void_p()
         torch.cuda.current_device()  # make the device current before any HIP call
@@ -194,10 +195,15 @@ class NativeContext:
             p.max_corrector_iterations + 1, ratio, cap, _stream()))
 
     def step_stats(self) -> dict:
-        out = (ctypes.c_double * 6)()
+        out = (ctypes.c_double * 7)()
         self._chk(self.lib.dnsmi_step_stats(self.handle, out))
         return {"steps": int(out[0]), "total_iterations": int(out[1]), "unconverged": int(out[2]),
-                "nan": int(out[3]), "last_error": out[4], "escalated": int(out[5])}
+                "nan": int(out[3]), "last_error": out[4], "escalated": int(out[5]),
+                "rerecorded": int(out[6])}
+
+    def split_rerecord(self) -> None:
+        """Split evaluations: the next corrector evaluation records a new reference."""
+        self._chk(self.lib.dnsmi_split_rerecord(self.handle))
 
     def step_reset(self) -> None:
         self._chk(self.lib.dnsmi_step_reset(self.handle))

@@ -57,6 +57,19 @@ class Params:
     # (the wall-bounded corrector repeats its error on the pass after a
     # start), not FP32 resolution.
     escalate_abs: float = 1e-8
+    # Split evaluations only: re-centring of the reference.  The rounding of
+    # an increment scales with its size, and the increments are measured
+    # from the step's first corrector iterate; a step whose predict-and-
+    # correct error exceeds recenter_far times corrector_tolerance records a
+    # new reference at its first pass that starts from an error of at most
+    # recenter_near times the tolerance, so its later increments are small
+    # again (early: the new reference's own FP32 rounding then shows in a
+    # pass whose error is far above it).  None disables it; the default is on for wall-bounded flows,
+    # whose first corrections grow with the perturbation (at 128 x 256 x 128
+    # to 1e-3 and beyond, which put the split floor at the 1e-10 tolerance),
+    # and off for periodic ones (whose floor stays orders below it).
+    recenter_far: float | None = -1.0
+    recenter_near: float = 5e5
 
     def __post_init__(self) -> None:
         if self.system not in PERIODIC_SYSTEMS + WALLED_SYSTEMS:
@@ -70,6 +83,10 @@ class Params:
             raise ValueError("precision must be 'fp64' or 'mixed'")
         if self.escalate_ratio is None and self.precision == "mixed":
             self.escalate_ratio = 0.5
+        if self.recenter_far is not None and self.recenter_far < 0:  # the default
+            self.recenter_far = 1e6 if self.system in WALLED_SYSTEMS else None
+        if self.recenter_far is not None and not self.recenter_near < self.recenter_far:
+            raise ValueError("need recenter_near < recenter_far")
         if not self.re > 0 or not self.dt > 0 or not 0 <= self.implicitness <= 1:
             raise ValueError("need re > 0, dt > 0, 0 <= implicitness <= 1")
 
@@ -77,6 +94,15 @@ class Params:
     def escalate_cap_value(self) -> float:
         """Error at or below which a stalled pass escalates to FP64."""
         return min(self.escalate_cap * self.corrector_tolerance, self.escalate_abs)
+
+    @property
+    def recenter_far_value(self) -> float:
+        """Predict-and-correct error above which a step re-centres its split reference."""
+        return math.inf if self.recenter_far is None else self.recenter_far * self.corrector_tolerance
+
+    @property
+    def recenter_near_value(self) -> float:
+        return self.recenter_near * self.corrector_tolerance
 
     @property
     def walls(self) -> bool:

@@ -83,6 +83,13 @@ typedef struct dnsmi_config {
                               corrector pass of a step as an FP32 increment over
                               the step's first evaluation (split precision);
                               1 evaluates each pass in full in FP32 */
+  double recenter_far;     /* split evaluations only, device steps (host
+                              loops: dnsmi_split_rerecord): a step whose
+                              predict-and-correct error exceeds recenter_far
+                              times the tolerance records a new reference at
+                              its first pass that starts from an error of at
+                              most recenter_near times the tolerance; 0: never */
+  double recenter_near;
 } dnsmi_config;
 
 typedef struct dnsmi_ctx dnsmi_ctx;
@@ -155,11 +162,18 @@ int dnsmi_walls_table(dnsmi_ctx* ctx, int which, double* out, long long n);
 int dnsmi_step_device(dnsmi_ctx* ctx, const double* state, double* next, double* rhs_a,
                       double* rhs_b, int slots, int esc_slots, int* counts, int counts_cap,
                       double tol, int max_iter, double esc_ratio, double esc_cap, void* stream);
-/* Synchronising read of the device step counters: out6 = {steps, total extra
+/* Synchronising read of the device step counters: out7 = {steps, total extra
  * iterations, steps not converged (error > tol at the end), NaN steps, last
- * error, steps that escalated to FP64 passes}; dnsmi_step_reset zeroes them. */
-int dnsmi_step_stats(dnsmi_ctx* ctx, double* out6);
+ * error, steps that escalated to FP64 passes, steps that re-recorded their
+ * split reference}; dnsmi_step_reset zeroes them. */
+int dnsmi_step_stats(dnsmi_ctx* ctx, double* out7);
 int dnsmi_step_reset(dnsmi_ctx* ctx);
+
+/* Split evaluations only (no counterpart in the reference): the next
+ * corrector evaluation (dnsmi_iterate_correction) records a new reference
+ * at its iterate, and the later ones are increments over it.  A host loop
+ * re-centres as the device steps do with recenter_far / recenter_near. */
+int dnsmi_split_rerecord(dnsmi_ctx* ctx);
 
 /* Mixed form only (no counterpart in the reference): while fp64 != 0 the
  * corrector passes (dnsmi_iterate_correction and the passes of the device

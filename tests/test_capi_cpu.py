@@ -7,6 +7,8 @@ import os
 import re
 
 import numpy as np
+import math
+
 import pytest
 
 from dnsjax_amd import _lib
@@ -184,6 +186,45 @@ def test_escalated_step_that_still_stalls_ends_unconverged():
     f = _EscalatingFlow([1e-7, 2e-9, 1.6e-10, 1.5e-10, 1.45e-10], [1.4e-10] * 4, max_it=8)
     r = run(f, 0.0, 3)
     assert r.counts == [8] and not r.converged and r.escalations == 1
+
+
+class _RecenterFlow(_EscalatingFlow):
+    """Scripted errors of a mixed plane Couette flow; records before which
+    pass (counted over the run) the split reference was re-recorded."""
+
+    def __init__(self, errs, **kw):
+        super().__init__(errs, [], system="plane-couette", **kw)
+        self.passes = 0
+        self.rerecords = []
+        flow = self
+
+        class _Native:
+            def split_rerecord(self):
+                flow.rerecords.append(flow.passes + 1)
+
+        self._native = _Native()
+
+    def iterate_correction(self, sp, s, r):
+        self.passes += 1
+        return super().iterate_correction(sp, s, r)
+
+
+def test_recentering_rule_of_the_host_loop():
+    """Wall-bounded mixed flows re-centre by default (recenter_far 1e6,
+    recenter_near 5e5 times the tolerance): a step whose predict-and-correct
+    error is above 1e-4 records a new reference at the first pass that
+    starts from an error of at most 5e-5; a step starting below 1e-4 keeps
+    its reference; periodic flows never re-centre by default."""
+    f = _RecenterFlow([3.8e-4, 3.8e-4, 8.6e-7, 4.3e-7, 1.2e-8, 3.8e-9, 1.3e-10, 3.3e-11,
+                       5e-5, 4e-5, 1e-7, 1e-9, 5e-11])
+    assert f.params.recenter_far_value == pytest.approx(1e-4) and f.params.recenter_near_value == pytest.approx(5e-5)
+    r = run(f, 0.0, 2)
+    assert r.counts == [7, 4] and r.converged and r.escalations == 0
+    assert f.rerecords == [3] and r.rerecorded == 1  # pass 3 starts from 8.6e-7
+    assert Params(precision="mixed").recenter_far is None
+    assert Params(system="plane-couette", precision="mixed", recenter_far=None).recenter_far_value == math.inf
+    with pytest.raises(ValueError):
+        Params(system="plane-couette", recenter_far=10.0, recenter_near=20.0)
 
 
 class _EnergyFlow(_ScriptedFlow):

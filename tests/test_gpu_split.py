@@ -157,3 +157,78 @@ def test_escalation_on_the_host_and_on_the_device_agree():
     short.step()
     torch.cuda.synchronize()
     assert short.step_stats()[0]["unconverged"] == 1 and short.counts()[0] == [r.counts[0] - 1]
+
+
+@pytest.mark.parametrize("walls", [True, False], ids=["walls", "periodic"])
+def test_recentering_on_the_host_and_on_the_device_agree(walls):
+    """Re-centring of the split reference: a step whose predict-and-correct
+    error exceeds recenter_far (here 2e3 tol) records a new reference at its
+    first pass that starts from an error of at most recenter_near (1e3 tol),
+    and its later passes are increments over that one.  The host loop and
+    the device-decided steps re-centre at the same passes, giving the same
+    states and counts bit for bit, and the run stays on the FP64 trajectory.
+    Plane Couette keeps the FP64 run's counts.  The periodic corrector
+    contracts about 70x per pass, so the rounding of the new reference's
+    full FP32 evaluation (the plain FP32 floor) still shows near the
+    tolerance: re-centring there costs up to one pass per step, which is
+    why it is off by default for periodic flows."""
+    from dnsjax_amd.driver import run
+    from dnsjax_amd.fields import wall_field
+    from dnsjax_amd.graph import GraphStepper
+
+    kw = dict(recenter_far=2e3, recenter_near=1e3)
+    host, dev = flow_of(16, "mixed", walls=walls, **kw), flow_of(16, "mixed", walls=walls, **kw)
+    ref = flow_of(16, "fp64", walls=walls)
+    if walls:
+        u0 = host.to_device(wall_field(16, 33, 16, 2 * math.pi, math.pi, 0.05, seed=2))
+    else:
+        u0 = host.init_state(seed=1, rms=0.1)
+    n = 5
+    states = []
+    r = run(host, u0.clone(), n, callback=lambda i, s: states.append(s.clone()))
+    f = run(ref, u0.clone(), n)
+    print(f"MEASURED recentering walls={walls}: counts {r.counts} (fp64 {f.counts}), re-recorded {r.rerecorded}, "
+          f"escalations {r.escalations}, errors {['%.1e' % e for e in r.errors]}")
+    assert r.converged and r.rerecorded >= 1 and r.escalations == 0
+    if walls:
+        assert sum(abs(a - b) for a, b in zip(r.counts, f.counts)) <= 1
+    else:
+        assert all(0 <= a - b <= 1 for a, b in zip(r.counts, f.counts))
+    d = float(torch.linalg.vector_norm(r.state - f.state) / torch.linalg.vector_norm(f.state))
+    assert d < 1e-6, d
+    gs = GraphStepper(dev, u0, slots=host.params.max_corrector_iterations)
+    for i in range(n):
+        gs.step()
+        torch.cuda.synchronize()
+        assert torch.equal(gs.state, states[i]), i
+    st = gs.step_stats()[0]
+    assert gs.counts()[0] == r.counts
+    assert st["unconverged"] == 0 and st["rerecorded"] == r.rerecorded and st["escalated"] == 0
+
+
+def test_recentering_of_guessed_steps_on_the_host_and_on_the_device_agree():
+    """Plane Couette steps started from quadratic extrapolation record their
+    reference at the first pass; when that pass's error exceeds
+    recenter_far, the reference is re-recorded as after predict-and-correct.
+    The host loop and the device-decided guessed steps (captured graphs)
+    agree bit for bit, with the same re-recordings."""
+    from dnsjax_amd.fields import wall_field
+    from dnsjax_amd.learned import GuessedGraphStepper, run_guessed
+
+    kw = dict(recenter_far=2e3, recenter_near=1e3)
+    host, dev = flow_of(16, "mixed", walls=True, **kw), flow_of(16, "mixed", walls=True, **kw)
+    u0 = host.to_device(wall_field(16, 33, 16, 2 * math.pi, math.pi, 0.05, seed=2))
+    n = 6
+    ref = []
+    g = run_guessed(host, u0.clone(), n, mode=3, callback=lambda i, s: ref.append(s.clone()))
+    print(f"MEASURED guessed recentering: passes {g.passes}, re-recorded {g.rerecorded}, escalations {g.escalations}")
+    assert g.converged and g.rerecorded >= 1 and g.escalations == 0
+    gs = GuessedGraphStepper(dev, u0, mode=3)
+    torch.cuda.synchronize()
+    assert torch.equal(gs.state, ref[1])
+    for i in range(2, n):
+        gs.step(1)
+        torch.cuda.synchronize()
+        assert torch.equal(gs.state, ref[i]), i
+    st = gs.step_stats()
+    assert gs.passes() == g.passes and st["unconverged"] == 0 and st["rerecorded"] == g.rerecorded
