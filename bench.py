@@ -659,6 +659,37 @@ def profile_workload(steps: int) -> None:
         print(f"plane Couette {prec}: {steps} steps, corrector iterations {counts}", flush=True)
 
 
+def profile_cubic(steps: int) -> None:
+    """The rocprofv3 workload of the fastest periodic form: config 1's
+    128^3 workload, mixed steps started from cubic extrapolation, then as
+    many plain FP64 steps, each run after a short untimed warm-up."""
+    import torch
+
+    from dnsjax_amd.driver import run
+    from dnsjax_amd.flows import TriplyPeriodicFlow
+    from dnsjax_amd.learned import run_guessed
+    from dnsjax_amd.params import kolmogorov_2pi
+
+    def mk(prec):
+        return TriplyPeriodicFlow(kolmogorov_2pi(N, RE, DT, corrector_tolerance=TOL, max_corrector_iterations=10,
+                                                 precision=prec))
+
+    mixed, fp64 = mk("mixed"), mk("fp64")
+    u0 = mixed.init_state(seed=SEED, rms=RMS)
+    run_guessed(mixed, u0.clone(), 6, mode=4)
+    run(fp64, u0.clone(), 3)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    g = run_guessed(mixed, u0.clone(), steps, mode=4)
+    torch.cuda.synchronize()
+    print(f"mixed + cubic extrapolation: {steps} steps in {time.perf_counter() - t:.4f} s, passes {g.passes}",
+          flush=True)
+    t = time.perf_counter()
+    r = run(fp64, u0.clone(), steps)
+    torch.cuda.synchronize()
+    print(f"fp64: {steps} steps in {time.perf_counter() - t:.4f} s, corrector iterations {r.counts}", flush=True)
+
+
 def walls_large_form(steps: int, collect_steps: int = 8, train_iters: int = 300, pairs: int = 20000) -> dict:
     """The learned-guess configuration: plane Couette 128 x 256 x 128 CGL
     points, fd_order 6, Re 1000, dt 0.005, corrector tolerance 1e-10,
@@ -815,7 +846,9 @@ def main():
     ap.add_argument("--profile-workload", action="store_true",
                     help="run the rocprofv3 workload (calibration copy + a few steps) and exit")
     ap.add_argument("--profile-imm", action="store_true",
-                    help="time the plane Couette influence-matrix step in both band-solve forms and exit")
+                    help="time the plane Couette influence-matrix step in its three forms and exit")
+    ap.add_argument("--profile-cubic", action="store_true",
+                    help="run --steps mixed steps from cubic extrapolation and FP64 steps at 128^3 and exit")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); the multi-GPU slab "
@@ -829,6 +862,9 @@ def main():
         return
     if a.profile_imm:
         profile_imm(a.steps)
+        return
+    if a.profile_cubic:
+        profile_cubic(a.steps)
         return
     os.environ.setdefault("OMP_NUM_THREADS", "1")
     progress("periodic 128^3: fp64 and mixed")
