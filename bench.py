@@ -53,14 +53,22 @@ def kernel_bytes(n: int, b: int) -> dict:
         "x_nonlinear": 9 * P * P * Nx * b,
         "z_forward": 3 * P * P * Nx * b + 3 * P * Nz * Nx * b,
         "y_forward:rhs": 3 * P * Nz * Nx * b + S,
+        # the stepping loop's epilogues (carry form): the predictor reads u^n
+        # and writes u^1 and B; a corrector pass reads B and the iterate and
+        # writes the iterate
         "y_forward:predict": 3 * P * Nz * Nx * b + 3 * S,
-        "y_forward:correct": 3 * P * Nz * Nx * b + 4 * S,
+        "y_forward:correct": 3 * P * Nz * Nx * b + 3 * S,
+        # the incremental form of the public iterate_correction (also reads
+        # N_prev and writes N)
+        "y_forward:correct_incremental": 3 * P * Nz * Nx * b + 4 * S,
         # the mixed form's split evaluations: the recording evaluation writes
-        # u_ref and the six physical fields (FP32), each increment reads them
-        # and the y forward adds N(u_ref)
+        # u_ref and the six physical fields (FP32), each increment reads
+        # them; the recording corrector epilogue also writes B + c ildt_2
+        # N(u_ref), which each increment's epilogue reads in place of B
         "y_inverse:split": S + 5 * P * Nz * Nx * b + S,
         "x_nonlinear:split": 9 * P * P * Nx * b + 6 * P * P * P * 4,
-        "y_forward:correct_split": 3 * P * Nz * Nx * b + 5 * S,
+        "y_forward:correct_record": 3 * P * Nz * Nx * b + 4 * S,
+        "y_forward:correct_split": 3 * P * Nz * Nx * b + 3 * S,
         "S": S,
     }
 
@@ -145,10 +153,13 @@ def roofline(form: dict, b: int, split: bool = False) -> dict:
     per = {}
     for name, (ms, cnt) in prof.items():
         if name == "y_forward":
+            # per step one predictor and one recording corrector epilogue
+            # (predict_and_correct), then the increments
             n_pred = steps
-            n_corr = cnt - n_pred
+            n_rec = steps if split else 0
+            n_corr = cnt - n_pred - n_rec
             corr = kb["y_forward:correct_split" if split else "y_forward:correct"]
-            byts = (n_pred * kb["y_forward:predict"] + n_corr * corr) / cnt
+            byts = (n_pred * kb["y_forward:predict"] + n_rec * kb["y_forward:correct_record"] + n_corr * corr) / cnt
         elif split and name in ("y_inverse", "x_nonlinear"):
             byts = kb[name + ":split"]
         elif name in kb:

@@ -102,7 +102,9 @@ SIGNATURES = {
     "dnsmi_mlp_features": (_I, [_P, _P, _P, _P, _P, _P, _P, _I, ctypes.c_float, ctypes.c_float,
                                 ctypes.c_float, _P, _P, _P]),
     "dnsmi_guess": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _I, ctypes.c_float, ctypes.c_float,
-                         ctypes.c_float, _I, _P]),
+                         ctypes.c_float, _I, _I, _P]),
+    "dnsmi_predict_and_correct_carry": (_I, [_P, _P, _P, _P, _P, _P]),
+    "dnsmi_iterate_correction_carry": (_I, [_P, _P, _P, _P, _P]),
     "dnsmi_step_device_guess": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, ctypes.c_float, ctypes.c_float,
                                      ctypes.c_float, _I, _I, _I, _P, _I, ctypes.c_double, _I, ctypes.c_double,
                                      ctypes.c_double, _P]),

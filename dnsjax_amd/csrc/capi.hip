@@ -56,6 +56,9 @@ int periodic_correct(Ctx* c, const double* pred_in, const double* rhs_prev,
 int periodic_norm(Ctx* c, const double* corr, double* err, hipStream_t s);
 int periodic_predict_and_correct(Ctx* c, const double* state, double* pred, double* rhs_next,
                                  double* err, hipStream_t s);
+int periodic_predict_and_correct_carry(Ctx* c, const double* state, double* pred, double* carry,
+                                       double* err, hipStream_t s);
+int periodic_iterate_correction_carry(Ctx* c, double* pred, const double* carry, double* err, hipStream_t s);
 int periodic_iterate_correction(Ctx* c, double* pred, const double* rhs_prev, double* rhs_next,
                                 double* err, hipStream_t s);
 int periodic_correct_velocity(Ctx* c, double* state, double* err, hipStream_t s);
@@ -114,7 +117,7 @@ int mlp_features(Ctx* c, const double* nm3, const double* nm2, const double* nm1
                  float s_out, float* X, float* T, hipStream_t s);
 int guess_periodic(Ctx* c, const double* state, double* pred, double* rhs_prev, float* h1,
                    float* h2, float* h3, const uint16_t* wb, const float* bias, int H, float s_1, float s_2,
-                   float s_out, int mode, hipStream_t s);
+                   float s_out, int mode, hipStream_t s, int carry);
 int guess_walls(Ctx* c, const double* state, double* pred, double* rhs_prev, const float* h1,
                 float* h2, const uint16_t* wb, const float* bias, int H, float s_1, float s_2,
                 float s_out, int mode, hipStream_t s);
@@ -269,6 +272,25 @@ int dnsmi_iterate_correction(dnsmi_ctx* ctx, const double* state_prev, double* p
                                                     err_dev, as_stream(stream))
                   : dnsmi::periodic_iterate_correction(c, pred, rhs_prev, rhs_next_out, err_dev,
                                                        as_stream(stream));
+}
+
+int dnsmi_predict_and_correct_carry(dnsmi_ctx* ctx, const double* state, double* pred_out,
+                                    double* carry_out, double* err_dev, void* stream) {
+  DNSMI_CHECK_CTX(ctx)
+  DNSMI_REQUIRE(!c->walls, "predict_and_correct_carry: periodic contexts only");
+  DNSMI_REQUIRE(state && pred_out && carry_out && err_dev, "predict_and_correct_carry: null pointer");
+  DNSMI_REQUIRE((const void*)state != pred_out && (const void*)state != carry_out && pred_out != carry_out,
+                "predict_and_correct_carry: buffers must be distinct");
+  return dnsmi::periodic_predict_and_correct_carry(c, state, pred_out, carry_out, err_dev, as_stream(stream));
+}
+
+int dnsmi_iterate_correction_carry(dnsmi_ctx* ctx, double* pred, const double* carry, double* err_dev,
+                                   void* stream) {
+  DNSMI_CHECK_CTX(ctx)
+  DNSMI_REQUIRE(!c->walls, "iterate_correction_carry: periodic contexts only");
+  DNSMI_REQUIRE(pred && carry && err_dev && (const void*)pred != (const void*)carry,
+                "iterate_correction_carry: null or aliased pointer");
+  return dnsmi::periodic_iterate_correction_carry(c, pred, carry, err_dev, as_stream(stream));
 }
 
 int dnsmi_correct_velocity(dnsmi_ctx* ctx, double* state, double* err_dev, void* stream) {
@@ -456,7 +478,7 @@ int dnsmi_guess_profile(dnsmi_ctx* ctx, const double* state, double* pred, doubl
 
 int dnsmi_guess(dnsmi_ctx* ctx, const double* state, double* pred, double* rhs_prev,
                 float* hist1, float* hist2, float* hist3, const void* packed_weights, const float* bias,
-                int H, float s_1, float s_2, float s_out, int mode, void* stream) {
+                int H, float s_1, float s_2, float s_out, int mode, int carry, void* stream) {
   DNSMI_CHECK_CTX(ctx)
   DNSMI_REQUIRE(state && pred && rhs_prev && hist1 && hist2, "guess: null pointer");
   DNSMI_REQUIRE(mode != 2 || (packed_weights && bias), "guess: network weights missing");
@@ -464,12 +486,13 @@ int dnsmi_guess(dnsmi_ctx* ctx, const double* state, double* pred, double* rhs_p
                     (void*)pred != rhs_prev && hist1 != hist2 && hist1 != hist3 && hist2 != hist3,
                 "guess: buffers must be distinct");
   DNSMI_REQUIRE(!hist3 || !c->walls, "guess: hist3 (cubic extrapolation) is for periodic contexts");
+  DNSMI_REQUIRE(!carry || !c->walls, "guess: the carry output is for periodic contexts");
   const uint16_t* wb = static_cast<const uint16_t*>(packed_weights);
   if (c->walls)
     return dnsmi::guess_walls(c, state, pred, rhs_prev, hist1, hist2, wb, bias, H, s_1, s_2, s_out,
                               mode, as_stream(stream));
   return dnsmi::guess_periodic(c, state, pred, rhs_prev, hist1, hist2, hist3, wb, bias, H, s_1, s_2,
-                               s_out, mode, as_stream(stream));
+                               s_out, mode, as_stream(stream), carry);
 }
 
 int dnsmi_step_device_guess(dnsmi_ctx* ctx, const double* state, double* next, double* rhs_a,

@@ -452,7 +452,9 @@ __global__ void k_mlp_features(FeatArgs a, const cx<double>* __restrict__ nm3,
 // and dN Leray-projected (zero at the mean mode):
 //   pred     <- u^1 + c ildt_2 dN, u^1 = (u^n ldt_1 + N^n) ildt_2
 //   rhs_prev <- N^n + dN                    (so that the incremental
-//               corrector from the guess is the same fixed-point map)
+//               corrector from the guess is the same fixed-point map), or
+//               with `carry` B = ildt_2 (ldt_1 u^n + (1 - c) N^n), the
+//               carry form's corrector input
 //   history  <- shifted: with h3, h3 <- h2, h2 <- h1, h1 <- N^n in place;
 //               without, h2 <- N^n (the caller then swaps h1, h2)
 // One lane per mode: each lane reads its mode's N^n and history once
@@ -465,7 +467,7 @@ __global__ void __launch_bounds__(256)
                      const cx<double>* __restrict__ ncur, float* __restrict__ h1,
                      float* __restrict__ h2, float* __restrict__ h3, double* __restrict__ pred,
                      double* __restrict__ rhs_prev, const uint16_t* __restrict__ wb,
-                     const float* __restrict__ bias, int mode) {
+                     const float* __restrict__ bias, int mode, int carry) {
   constexpr int NT = 2 * (H / 16) + (H / 16) * (H / 16);
   __shared__ v4s wt[NT * 64];
   __shared__ float bs[2 * H + 16];
@@ -568,7 +570,10 @@ __global__ void __launch_bounds__(256)
     const cx<double> u = state[e];
     const cx<double> u1 = {(u.x * ldt1 + nc[q].x) * ildt2, (u.y * ldt1 + nc[q].y) * ildt2};
     pd[e] = mode == 0 ? u1 : cx<double>{u1.x + cil * d.x, u1.y + cil * d.y};
-    rp[e] = mode == 0 ? nc[q] : cx<double>{nc[q].x + d.x, nc[q].y + d.y};
+    if (carry)  // B = ildt_2 (ldt_1 u^n + (1 - c) N^n), the carry form's corrector input
+      rp[e] = {(u.x * ldt1 + (1.0 - a.c) * nc[q].x) * ildt2, (u.y * ldt1 + (1.0 - a.c) * nc[q].y) * ildt2};
+    else
+      rp[e] = mode == 0 ? nc[q] : cx<double>{nc[q].x + d.x, nc[q].y + d.y};
     if (h3) {
       reinterpret_cast<float2*>(h3)[e] = hp2[q];
       reinterpret_cast<float2*>(h2)[e] = hp1[q];
@@ -827,7 +832,7 @@ int periodic_guess_rhs(Ctx* c, const double* state, double* rhs, hipStream_t s);
 
 int guess_periodic(Ctx* c, const double* state, double* pred, double* rhs_prev, float* h1,
                    float* h2, float* h3, const uint16_t* wb, const float* bias, int H, float s_1, float s_2,
-                   float s_out, int mode, hipStream_t s) {
+                   float s_out, int mode, hipStream_t s, int carry) {
   if (c->walls) return c->fail(DNSMI_EINVAL, "guess: periodic contexts only");
   if (mode < 0 || mode > 4) return c->fail(DNSMI_EINVAL, "guess: mode must be 0 to 4");
   if (mode == 4 && !h3) return c->fail(DNSMI_EINVAL, "guess: cubic extrapolation needs hist3");
@@ -843,11 +848,11 @@ int guess_periodic(Ctx* c, const double* state, double* pred, double* rhs_prev, 
   if (mode != 2 || H == 32)
     hipLaunchKernelGGL(k_guess_periodic<32>, dim3(nb), dim3(256), 0, s, a,
                        reinterpret_cast<const cx<double>*>(state), n0, h1, h2, h3, pred, rhs_prev, wb,
-                       bias, mode);
+                       bias, mode, carry);
   else
     hipLaunchKernelGGL(k_guess_periodic<64>, dim3(nb), dim3(256), 0, s, a,
                        reinterpret_cast<const cx<double>*>(state), n0, h1, h2, h3, pred, rhs_prev, wb,
-                       bias, mode);
+                       bias, mode, carry);
   return check_launch(c, "guess");
 }
 

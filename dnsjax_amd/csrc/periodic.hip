@@ -31,17 +31,25 @@
 
 namespace dnsmi {
 
-enum { MODE_RHS = 0, MODE_PREDICT = 1, MODE_CORRECT = 2 };
+// K5 epilogues.  The carry forms keep B = ildt_2 (ldt_1 u^n + (1 - c) N^n)
+// in the RHS buffer instead of N: the predictor writes it beside u^1, and a
+// corrector pass is then u <- B + c ildt_2 N(u), the reference's fixed-point
+// iteration in its full form (the incremental form u + c ildt_2 (N(u) -
+// N_prev) telescopes to it), reading B and the iterate and writing the
+// iterate only.
+enum { MODE_RHS = 0, MODE_PREDICT = 1, MODE_CORRECT = 2, MODE_PREDICT_CARRY = 3, MODE_CORRECT_CARRY = 4 };
 
 struct StepArgs {
   const double* state;     // MODE_PREDICT: u^n
-  const double* rhs_prev;  // MODE_CORRECT: N_prev
+  const double* rhs_prev;  // MODE_CORRECT: N_prev; MODE_CORRECT_CARRY: B
   double* pred;            // MODE_PREDICT: out; MODE_CORRECT: in/out
   double* rhs_out;         // N of this evaluation
   double* partials;        // MODE_CORRECT: per-block sum k_metric |delta|^2
   const int* act;          // if set and 0: the whole launch is skipped
   const double* nref;      // EV_INC: N(u_ref), added to the projected increment
+                           // (carry form: B + c ildt_2 N(u_ref), in place of B)
   double* nref_out;        // EV_RECORD: receives a copy of N (the new N(u_ref))
+                           // (carry form: B + c ildt_2 N(u_ref), the new iterate)
   double dt, c, re;
 };
 
@@ -962,6 +970,28 @@ __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
         // rhs = N - i k p (increments: N(u_ref) + P[dN])
         cx<double> rh = {nn[c].x + kk[c] * pr.y, nn[c].y - kk[c] * pr.x};
         const size_t idx = c * comp + pos;
+        if constexpr (MODE == MODE_CORRECT_CARRY) {
+          // u <- B + c ildt_2 N(u); increments: (B + c ildt_2 N(u_ref)) + c ildt_2 P[dN]
+          const cx<double> base = reinterpret_cast<const cx<double>*>(args.nref ? args.nref : args.rhs_prev)[idx];
+          const double ci = args.c * ildt2;
+          const cx<double> un = {base.x + ci * rh.x, base.y + ci * rh.y};
+          cx<double>* pd = reinterpret_cast<cx<double>*>(args.pred);
+          const cx<double> p0 = pd[idx];
+          const cx<double> d = {un.x - p0.x, un.y - p0.y};
+          pd[idx] = un;
+          if (args.nref_out) reinterpret_cast<cx<double>*>(args.nref_out)[idx] = un;
+          acc += km * (d.x * d.x) + km * (d.y * d.y);
+          continue;
+        }
+        if constexpr (MODE == MODE_PREDICT_CARRY) {
+          const cx<double> u = reinterpret_cast<const cx<double>*>(args.state)[idx];
+          cx<double>* pd = reinterpret_cast<cx<double>*>(args.pred);
+          cx<double>* rout = reinterpret_cast<cx<double>*>(args.rhs_out);
+          pd[idx] = {(u.x * ldt1 + rh.x) * ildt2, (u.y * ldt1 + rh.y) * ildt2};
+          const double cm = 1.0 - args.c;
+          rout[idx] = {(u.x * ldt1 + cm * rh.x) * ildt2, (u.y * ldt1 + cm * rh.y) * ildt2};
+          continue;
+        }
         if (args.nref) {
           const cx<double> r0 = reinterpret_cast<const cx<double>*>(args.nref)[idx];
           rh = {r0.x + rh.x, r0.y + rh.y};
@@ -987,7 +1017,7 @@ __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
       }
     }
   }
-  if (MODE == MODE_CORRECT) {
+  if (MODE == MODE_CORRECT || MODE == MODE_CORRECT_CARRY) {
     const double s = block_sum(acc, red);
     if (threadIdx.x == 0) args.partials[blockIdx.x] = s;
   }
@@ -1328,7 +1358,8 @@ static int launch_y_forward(Ctx* c, const StepArgs& a, hipStream_t s, int* nbloc
   const int ntile = (c->Nx1 + c->tile_cols - 1) / c->tile_cols;
   const int nb = c->Nz1 * ntile;
   if (nblocks) *nblocks = nb;
-  if (MODE == MODE_CORRECT && nb > c->max_partials) return c->fail(DNSMI_EINVAL, "y_forward: too many blocks");
+  if ((MODE == MODE_CORRECT || MODE == MODE_CORRECT_CARRY) && nb > c->max_partials)
+    return c->fail(DNSMI_EINVAL, "y_forward: too many blocks");
   ProfScope ps(c, KID_Y_FORWARD, s);
   switch (c->Py) {
 #define X(P)                                                                                   \
@@ -1425,6 +1456,39 @@ int periodic_predict_and_correct(Ctx* c, const double* state, double* pred, doub
   int nb = 0;
   c->ref_valid = false;
   rc = periodic_rhs_mode<MODE_CORRECT>(c, pred, b, s, &nb, EV_RECORD);
+  if (rc) return rc;
+  return finalize_norm(c, nb, err, s);
+}
+
+// The carry forms of predict_and_correct and iterate_correction (see the K5
+// epilogue modes): `carry` receives B from the predictor and is read by
+// every corrector pass; split evaluations keep B + c ildt_2 N(u_ref) in
+// n_ref.  A pass reads B and the iterate and writes the iterate (3 S, where
+// the incremental form also reads N_prev and writes N: 4 S).
+int periodic_predict_and_correct_carry(Ctx* c, const double* state, double* pred, double* carry,
+                                       double* err, hipStream_t s) {
+  StepArgs a = step_args(c);
+  a.state = state;
+  a.pred = pred;
+  a.rhs_out = carry;
+  int rc = periodic_rhs_mode<MODE_PREDICT_CARRY>(c, state, a, s, nullptr, EV_PLAIN);
+  if (rc) return rc;
+  StepArgs b = step_args(c);
+  b.rhs_prev = carry;
+  b.pred = pred;
+  int nb = 0;
+  c->ref_valid = false;
+  rc = periodic_rhs_mode<MODE_CORRECT_CARRY>(c, pred, b, s, &nb, EV_RECORD);
+  if (rc) return rc;
+  return finalize_norm(c, nb, err, s);
+}
+
+int periodic_iterate_correction_carry(Ctx* c, double* pred, const double* carry, double* err, hipStream_t s) {
+  StepArgs b = step_args(c);
+  b.rhs_prev = carry;
+  b.pred = pred;
+  int nb = 0;
+  int rc = periodic_rhs_mode<MODE_CORRECT_CARRY>(c, pred, b, s, &nb, EV_INC);
   if (rc) return rc;
   return finalize_norm(c, nb, err, s);
 }

@@ -27,6 +27,9 @@ int periodic_predict_and_correct(Ctx* c, const double* state, double* pred, doub
                                  double* err, hipStream_t s);
 int periodic_iterate_correction(Ctx* c, double* pred, const double* rhs_prev, double* rhs_next,
                                 double* err, hipStream_t s);
+int periodic_predict_and_correct_carry(Ctx* c, const double* state, double* pred, double* carry,
+                                       double* err, hipStream_t s);
+int periodic_iterate_correction_carry(Ctx* c, double* pred, const double* carry, double* err, hipStream_t s);
 int periodic_correct_velocity(Ctx* c, double* state, double* err, hipStream_t s);
 int walls_predict_and_correct(Ctx* c, const double* state, double* pred, double* rhs_next,
                               double* err, hipStream_t s);
@@ -76,6 +79,15 @@ __global__ void __launch_bounds__(256) k_ctl_settle(const StepCtl* ctl, double* 
   if (i < n) rhs_a[i] = rhs_b[i];
 }
 
+// One corrector pass of a device step.  Periodic steps use the carry form:
+// rhs_a holds B for every pass (rhs_b is unused); wall-bounded passes
+// alternate between the two RHS arrays.
+static int step_pass(Ctx* c, const double* state, double* pred, const double* rp, double* rn,
+                     const double* carry, double* err, hipStream_t s) {
+  return c->walls ? walls_iterate_correction(c, state, pred, rp, rn, err, s)
+                  : periodic_iterate_correction_carry(c, pred, carry, err, s);
+}
+
 // The corrector slots of a device step: `slots` passes in the context's
 // precision, then (mixed form) `esc_slots` FP64 passes that run only once
 // the step has escalated.
@@ -96,26 +108,25 @@ static int step_slots(Ctx* c, const double* state, double* pred, double* rhs_a, 
     if (recenter && i > 1) {
       c->act = &ctl->rr[i];
       c->ref_valid = false;  // the evaluation records
-      rc = c->walls ? walls_iterate_correction(c, state, pred, rp, rn, err, s)
-                    : periodic_iterate_correction(c, pred, rp, rn, err, s);
+      rc = step_pass(c, state, pred, rp, rn, rhs_a, err, s);
       if (rc) break;  // the recording launch left ref_valid set: the next is an increment
     }
     c->act = &ctl->rp[i];
-    rc = c->walls ? walls_iterate_correction(c, state, pred, rp, rn, err, s)
-                  : periodic_iterate_correction(c, pred, rp, rn, err, s);
+    rc = step_pass(c, state, pred, rp, rn, rhs_a, err, s);
   }
   if (rc == DNSMI_OK && esc_slots > 0) {
-    const long long n = 2 * c->spec_elems;
-    hipLaunchKernelGGL(k_ctl_settle, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ctl, rhs_a,
-                       rhs_b, n);
-    rc = check_launch(c, "ctl_settle");
+    if (c->walls) {
+      const long long n = 2 * c->spec_elems;
+      hipLaunchKernelGGL(k_ctl_settle, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, ctl, rhs_a,
+                         rhs_b, n);
+      rc = check_launch(c, "ctl_settle");
+    }
     c->act = &ctl->run_e;
     c->fp32 = false;
     for (int i = 1; i <= esc_slots && rc == DNSMI_OK; ++i) {
       const double* rp = (i % 2) ? rhs_a : rhs_b;
       double* rn = (i % 2) ? rhs_b : rhs_a;
-      rc = c->walls ? walls_iterate_correction(c, state, pred, rp, rn, err, s)
-                    : periodic_iterate_correction(c, pred, rp, rn, err, s);
+      rc = step_pass(c, state, pred, rp, rn, rhs_a, err, s);
     }
     c->fp32 = c->fp32_primary;
   }
@@ -138,7 +149,7 @@ int step_device(Ctx* c, const double* state, double* pred, double* rhs_a, double
   c->act = nullptr;
   c->fp32 = c->fp32_primary;
   int rc = c->walls ? walls_predict_and_correct(c, state, pred, rhs_a, err, s)
-                    : periodic_predict_and_correct(c, state, pred, rhs_a, err, s);
+                    : periodic_predict_and_correct_carry(c, state, pred, rhs_a, err, s);
   if (rc) return rc;
   c->rule = CtlRule{tol, max_iter, esc_slots > 0 ? esc_ratio : 0.0, esc_cap,
                     esc_slots > 0 ? slots : max_iter, recenter_far(c, tol), c->cfg.recenter_near * tol};
@@ -168,7 +179,7 @@ __global__ void k_ctl_begin_guess(StepCtl* ctl, int max_pass) {
 
 int guess_periodic(Ctx* c, const double* state, double* pred, double* rhs_prev, float* h1,
                    float* h2, float* h3, const uint16_t* wb, const float* bias, int H, float s_1, float s_2,
-                   float s_out, int mode, hipStream_t s);
+                   float s_out, int mode, hipStream_t s, int carry);
 int guess_walls(Ctx* c, const double* state, double* pred, double* rhs_prev, const float* h1,
                 float* h2, const uint16_t* wb, const float* bias, int H, float s_1, float s_2,
                 float s_out, int mode, hipStream_t s);
@@ -193,7 +204,7 @@ int step_device_guess(Ctx* c, const double* state, double* pred, double* rhs_a, 
   c->fp32 = c->fp32_primary;
   int rc = c->walls ? guess_walls(c, state, pred, rhs_a, h1, h2, wb, bias, H, s_1, s_2, s_out, mode, s)
                     : guess_periodic(c, state, pred, rhs_a, h1, h2, h3, wb, bias, H, s_1, s_2, s_out,
-                                     mode, s);
+                                     mode, s, 1);
   if (rc) return rc;
   hipLaunchKernelGGL(k_ctl_begin_guess, dim3(1), dim3(64), 0, s, ctl, max_pass);
   if ((rc = check_launch(c, "ctl_begin_guess"))) return rc;

@@ -1,7 +1,8 @@
 """The time loop around the hot path, with the reference driver's semantics.
 
 ``run`` follows ``__main__.py:125-203``: per step one
-``predict_and_correct``, then ``iterate_correction`` while
+``predict_and_correct``, then ``iterate_correction`` (periodic native flows:
+their carry form, the same iteration written in full) while
 ``error > corrector_tolerance`` and fewer than ``max_corrector_iterations``
 extra iterations ran, then ``correct_velocity``; the loop stops after a
 step whose corrector did not converge.  A NaN error is reported (the
@@ -92,22 +93,46 @@ class Escalation:
         return was
 
 
+def carry_context(flow):
+    """The native context of a periodic flow, whose passes run in the carry
+    form (``dnsmi_iterate_correction_carry``: each pass reads B =
+    ildt_2 (ldt_1 u^n + (1 - c) N^n) and the iterate, and writes the
+    iterate), or None (plane Couette, whose corrector reads the previous
+    evaluation by design, and flows without a native context)."""
+    nat = getattr(flow, "_native", None)
+    if nat is None or flow.params.walls or not hasattr(nat, "iterate_correction_carry"):
+        return None
+    return nat
+
+
 def run(flow, state, n_steps: int, callback=None) -> RunResult:
     p = flow.params
     tol, max_it = p.corrector_tolerance, p.max_corrector_iterations
     esc = Escalation(flow)
     res = RunResult(state=state)
     last_error = 0.0
+    nat = carry_context(flow)
+    if nat is not None:
+        carry, err_t = nat.empty(), nat.scalar()
     for it in range(n_steps):
         if not last_error < tol:
             break
         state_prev = state
-        state, rhs, error = flow.predict_and_correct(state_prev)
+        if nat is not None:
+            state = nat.empty()
+            nat.predict_and_correct_carry(state_prev, state, carry, err_t)
+            error = err_t
+        else:
+            state, rhs, error = flow.predict_and_correct(state_prev)
         err = float(error)
         esc.begin(err)
         c = 0
         while err > tol and c < max_it:
-            state, rhs, error = flow.iterate_correction(state_prev, state, rhs)
+            if nat is not None:
+                nat.iterate_correction_carry(state, carry, err_t)
+                error = err_t
+            else:
+                state, rhs, error = flow.iterate_correction(state_prev, state, rhs)
             err = float(error)
             c += 1
             if c < max_it:

@@ -279,6 +279,7 @@ def run_guessed(flow, state, n_steps: int, mode: int = 2, net: Net | None = None
     sc = (1.0, 1.0, 1.0) if scales is None else scales.astuple()
     ops = None if net is None or hasattr(net, "guess") else net.operands
     cubic = _cubic(mode, net, scales)
+    carry = not p.walls  # periodic passes run in the carry form (driver.carry_context)
     hist = [torch.zeros(nat.shape + (2,), dtype=torch.float32, device=nat.device) for _ in range(3 if cubic else 2)]
     res = GuessRunResult(state=state)
     state = state.clone()
@@ -293,14 +294,18 @@ def run_guessed(flow, state, n_steps: int, mode: int = 2, net: Net | None = None
         if m == 2 and hasattr(net, "guess"):  # the plane Couette profile network
             net.guess(nat, state, pred, rhs[0], hist[0], hist[1], scales)
         else:
-            nat.guess(state, pred, rhs[0], hist[0], hist[1], m, ops, sc, hist3=hist[2] if cubic else None)
+            nat.guess(state, pred, rhs[0], hist[0], hist[1], m, ops, sc, hist3=hist[2] if cubic else None,
+                      carry=carry)
         if not cubic:
             hist.reverse()  # hist[0] = N^n, hist[1] = N^{n-1}
         c, err, cur = 0, math.inf, 0
         esc.begin()
         while err > tol and c < max_pass:
-            nat.iterate_correction(state, pred, rhs[cur], rhs[1 - cur], err_t)
-            cur = 1 - cur
+            if carry:
+                nat.iterate_correction_carry(pred, rhs[0], err_t)
+            else:
+                nat.iterate_correction(state, pred, rhs[cur], rhs[1 - cur], err_t)
+                cur = 1 - cur
             err = float(err_t)
             c += 1
             if c < max_pass:

@@ -167,16 +167,30 @@ class NativeContext:
                                               _ptr(modes), modes.numel(), s_1, s_2, s_out, _ptr(X), _ptr(T),
                                               _stream()))
 
-    def guess(self, state, pred, rhs_prev, hist1, hist2, mode, net=None, scales=(1.0, 1.0, 1.0), hist3=None):
+    def guess(self, state, pred, rhs_prev, hist1, hist2, mode, net=None, scales=(1.0, 1.0, 1.0), hist3=None,
+              carry=False):
         """Predictor with a corrector guess (mode 0 plain, 1 linear, 3
         quadratic, 4 cubic extrapolation, 2 quadratic -- cubic with hist3 --
         plus the network `net` = (H, packed weights, bias)).  Without hist3,
-        hist2 receives N^n; with it the three histories shift in place."""
+        hist2 receives N^n; with it the three histories shift in place.
+        carry: rhs_prev receives the carry form's B (periodic)."""
         H, wb, bias = (32, None, None) if net is None else net
         s_1, s_2, s_out = scales
         self._chk(self.lib.dnsmi_guess(self.handle, _ptr(state), _ptr(pred), _ptr(rhs_prev), _ptr(hist1),
                                        _ptr(hist2), _ptr(hist3), _ptr(wb), _ptr(bias), H, s_1, s_2, s_out, mode,
-                                       _stream()))
+                                       int(bool(carry)), _stream()))
+
+    def predict_and_correct_carry(self, state, pred, carry, err):
+        """Carry form (periodic): pred <- the first corrector iterate,
+        carry <- B = ildt_2 (ldt_1 u^n + (1 - c) N^n)."""
+        self.check_field(state, "state")
+        self._chk(self.lib.dnsmi_predict_and_correct_carry(self.handle, _ptr(state), _ptr(pred), _ptr(carry),
+                                                           _ptr(err), _stream()))
+
+    def iterate_correction_carry(self, pred, carry, err):
+        """Carry form (periodic): pred <- B + c ildt_2 N(pred) in place."""
+        self._chk(self.lib.dnsmi_iterate_correction_carry(self.handle, _ptr(pred), _ptr(carry), _ptr(err),
+                                                          _stream()))
 
     def step_device_guess(self, state, nxt, rhs_a, rhs_b, hist1, hist2, mode, slots, counts=None, net=None,
                           scales=(1.0, 1.0, 1.0), esc_slots=None, hist3=None):

@@ -128,6 +128,18 @@ int dnsmi_iterate_correction(dnsmi_ctx* ctx, const double* state_prev, double* p
                              const double* rhs_prev, double* rhs_next_out, double* err_dev,
                              void* stream);
 
+/* Carry form of the two calls above (periodic contexts only; no
+ * counterpart in the reference, whose iteration it reproduces in its full
+ * form): carry_out receives B = ildt_2 (ldt_1 u^n + (1 - c) N^n) instead of
+ * N, and each corrector pass is pred <- B + c ildt_2 N(pred), reading carry
+ * and pred and writing pred only (the incremental form u + c ildt_2 (N(u) -
+ * N_prev), which the reference writes, telescopes to it; results agree to
+ * rounding).  err_dev as above.  The device steps use this form. */
+int dnsmi_predict_and_correct_carry(dnsmi_ctx* ctx, const double* state, double* pred_out,
+                                    double* carry_out, double* err_dev, void* stream);
+int dnsmi_iterate_correction_carry(dnsmi_ctx* ctx, double* pred, const double* carry, double* err_dev,
+                                   void* stream);
+
 /* Divergence correction + mean-mode zeroing in place (periodic); a
  * pass-through for walls.  If err_dev is not NULL it receives the norm of
  * the divergence correction (debug.measure_corrections). */
@@ -260,7 +272,9 @@ int dnsmi_mlp_features(dnsmi_ctx* ctx, const double* n_m3, const double* n_m2, c
  * and hist2 for the next step; with hist3 (periodic contexts only) the
  * three are shifted in place (hist3 <- hist2 <- hist1 <- N^n) and the
  * caller keeps them as they are.  Mode 0 (the plain predictor) reads no
- * history.
+ * history.  carry != 0 (periodic contexts): rhs_prev receives the carry
+ * form's B = ildt_2 (ldt_1 u^n + (1 - c) N^n) instead, for
+ * dnsmi_iterate_correction_carry.
  * Wall-bounded contexts: pred = the predictor's solve
  * (flows/plane_couette.py:507-519) fed with N_ext = N^n + dN per point
  * (same modes, with the network's residual divided by w = 1 / (1 + c dt
@@ -268,7 +282,7 @@ int dnsmi_mlp_features(dnsmi_ctx* ctx, const double* n_m3, const double* n_m2, c
  * point u* = S(u^n; N(u*)) is then reached from the guess. */
 int dnsmi_guess(dnsmi_ctx* ctx, const double* state, double* pred, double* rhs_prev,
                 float* hist1, float* hist2, float* hist3, const void* packed_weights, const float* bias,
-                int H, float s_1, float s_2, float s_out, int mode, void* stream);
+                int H, float s_1, float s_2, float s_out, int mode, int carry, void* stream);
 
 /* One time step started from the corrector guess, decided on the device
  * (periodic and wall-bounded): dnsmi_guess from `state` into `next` (with rhs_a as

@@ -44,9 +44,16 @@ def test_periodic_kernel_bytes_parts():
     assert kb["z_inverse"] == 5 * plane + 6 * cube
     assert kb["x_nonlinear"] == 9 * cube
     assert kb["z_forward"] == 3 * cube + 3 * plane
-    # the epilogues add the state, the predictor's inputs, the corrector's
+    # the epilogues: the RHS writes N (S); the carry predictor reads u^n and
+    # writes u^1 and B; a carry corrector pass reads B and the iterate and
+    # writes the iterate; the incremental one also reads N_prev and writes N;
+    # the split recording pass also writes B + c ildt_2 N(u_ref), which each
+    # increment reads in place of B
     assert kb["y_forward:predict"] - kb["y_forward:rhs"] == 2 * S
-    assert kb["y_forward:correct"] - kb["y_forward:rhs"] == 3 * S
+    assert kb["y_forward:correct"] - kb["y_forward:rhs"] == 2 * S
+    assert kb["y_forward:correct_incremental"] - kb["y_forward:rhs"] == 3 * S
+    kb8 = bench.kernel_bytes(128, 8)
+    assert kb8["y_forward:correct_record"] - kb8["y_forward:correct_split"] == S
 
 
 @pytest.mark.parametrize("shape,rhs_mb,imm_mb", [((64, 128, 64, 4), 138.02, 125.99),

@@ -71,6 +71,42 @@ def test_golden_rhs_step_and_projection(case):
     assert rel(cv, g["corrected"]) < 1e-12
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("case", list(CASES))
+def test_carry_form_matches_golden_steps(case, precision):
+    """The carry form (the device steps' and host loops' passes: B =
+    ildt_2 (ldt_1 u^n + (1 - c) N^n) kept instead of N, each pass
+    pred <- B + c ildt_2 N(pred)) against the golden predict-and-correct
+    and corrector iteration: FP64 to 1e-12 with errors to 1e-9; mixed to
+    the mixed form's bound against the incremental form's own mixed result;
+    B itself against its definition from the golden state and N^n."""
+    g = np.load(os.path.join(GOLDEN, case + ".npz"))
+    flow = make(case, precision=precision)
+    nat = flow._native
+    u = flow.to_device(g["state"])
+    pred, carry, err = nat.empty(), nat.empty(), nat.scalar()
+    nat.predict_and_correct_carry(u, pred, carry, err)
+    e1 = float(err)
+    p1 = pred.clone()
+    nat.iterate_correction_carry(pred, carry, err)
+    e2 = float(err)
+    if precision == "fp64":
+        assert rel(p1, g["pred1"]) < 1e-12 and rel(pred, g["pred2"]) < 1e-12
+        assert abs(e1 - float(g["err1"])) <= 1e-9 * float(g["err1"])
+        assert abs(e2 - float(g["err2"])) <= 1e-9 * float(g["err2"])
+        c = CASES[case]
+        of = P.Flow(P.PeriodicParams(nx=c["nx"], ny=c["ny"], nz=c["nz"], lx=c["L"], ly=c["L"], lz=c["L"],
+                                     re=c["re"], dt=c["dt"], system=c["system"], tilt_degree=c["tilt"]))
+        want_b = of.ildt_2 * (of.ldt_1 * g["state"] + (1 - of.p.implicitness) * g["rhs0"])
+        want_b[:, 0, 0, 0] = 0.0
+        assert rel(carry, want_b) < 1e-12
+    else:
+        q1, _, f1 = flow.predict_and_correct(u)
+        q2, _, f2 = flow.iterate_correction(u, q1.clone(), flow.predict_and_correct(u)[1])
+        assert rel(p1, q1.cpu().numpy()) < 1e-9 and rel(pred, q2.cpu().numpy()) < 1e-9
+        assert abs(e1 - float(f1)) <= 1e-6 * float(f1) and abs(e2 - float(f2)) <= 1e-3 * float(f2)
+
+
 @pytest.mark.parametrize("case", list(CASES))
 def test_generic_composition_matches_fused(case):
     from dnsjax_amd.timestep import make_stepper
