@@ -455,8 +455,9 @@ __global__ void k_mlp_features(FeatArgs a, const cx<double>* __restrict__ nm3,
 //               corrector from the guess is the same fixed-point map), or
 //               with `carry` B = ildt_2 (ldt_1 u^n + (1 - c) N^n), the
 //               carry form's corrector input
-//   history  <- shifted: with h3, h3 <- h2, h2 <- h1, h1 <- N^n in place;
-//               without, h2 <- N^n (the caller then swaps h1, h2)
+//   history  <- N^n into the oldest field: h3 when given (the caller then
+//               rotates (h1, h2, h3) <- (h3, h1, h2)), else h2 (the caller
+//               swaps h1 and h2)
 // One lane per mode: each lane reads its mode's N^n and history once
 // (coalesced), forms the features and dN; the network runs on the wave's
 // 64 modes as four 16-mode matrix-core batches, with operands and outputs
@@ -485,15 +486,12 @@ __global__ void __launch_bounds__(256)
   double kx, ky, kz;
   coords(a, (unsigned)ii, kx, ky, kz);
   cx<double> nc[3], d1[3], d2[3], d3[3];
-  float2 hp1[3], hp2[3];
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
     const long long e = q * a.n + ii;
     nc[q] = live ? ncur[e] : cx<double>{0, 0};
     const float2 p1 = live ? reinterpret_cast<const float2*>(h1)[e] : float2{0.f, 0.f};
     const float2 p2 = live ? reinterpret_cast<const float2*>(h2)[e] : float2{0.f, 0.f};
-    hp1[q] = p1;
-    hp2[q] = p2;
     diffs(nc[q].x, p1.x, p2.x, d1[q].x, d2[q].x);
     diffs(nc[q].y, p1.y, p2.y, d1[q].y, d2[q].y);
     d3[q] = {0.0, 0.0};
@@ -574,13 +572,8 @@ __global__ void __launch_bounds__(256)
       rp[e] = {(u.x * ldt1 + (1.0 - a.c) * nc[q].x) * ildt2, (u.y * ldt1 + (1.0 - a.c) * nc[q].y) * ildt2};
     else
       rp[e] = mode == 0 ? nc[q] : cx<double>{nc[q].x + d.x, nc[q].y + d.y};
-    if (h3) {
-      reinterpret_cast<float2*>(h3)[e] = hp2[q];
-      reinterpret_cast<float2*>(h2)[e] = hp1[q];
-      reinterpret_cast<float2*>(h1)[e] = float2{(float)nc[q].x, (float)nc[q].y};
-    } else {
-      reinterpret_cast<float2*>(h2)[e] = float2{(float)nc[q].x, (float)nc[q].y};
-    }
+    // the newest N replaces the oldest in the history (a ring with h3)
+    reinterpret_cast<float2*>(h3 ? h3 : h2)[e] = float2{(float)nc[q].x, (float)nc[q].y};
   }
 }
 

@@ -296,7 +296,9 @@ def run_guessed(flow, state, n_steps: int, mode: int = 2, net: Net | None = None
         else:
             nat.guess(state, pred, rhs[0], hist[0], hist[1], m, ops, sc, hist3=hist[2] if cubic else None,
                       carry=carry)
-        if not cubic:
+        if cubic:
+            hist = [hist[2], hist[0], hist[1]]  # N^n went to the oldest field
+        else:
             hist.reverse()  # hist[0] = N^n, hist[1] = N^{n-1}
         c, err, cur = 0, math.inf, 0
         esc.begin()
@@ -330,7 +332,8 @@ def run_guessed(flow, state, n_steps: int, mode: int = 2, net: Net | None = None
 class GuessedGraphStepper:
     """Guessed steps with the corrector decided on the device, one captured
     graph launch per step (two graphs: state and history buffers swap each
-    step).  The start-up steps (plain predictor, linear extrapolation and,
+    step; six for cubic guesses, whose three histories rotate with period
+    three).  The start-up steps (plain predictor, linear extrapolation and,
     for cubic guesses, quadratic extrapolation) run eagerly through the same
     device step with the full pass budget; the
     rest replay the graph of the requested mode with `slots` corrector slots
@@ -363,33 +366,43 @@ class GuessedGraphStepper:
         nat.step_reset()
         # the start-up steps run eagerly with the full pass budget
         full = p.max_corrector_iterations + 1
+        self.period = 6 if self.cubic else 2
         start = 3 if self.cubic else 2
         for it in range(start):
-            self._enqueue(it % 2, _startup_mode(it, mode, self.cubic), full)
-        self.parity, self.steps = start % 2, start
-        self.graphs = [self._capture(0), self._capture(1)]
+            self._enqueue(it, _startup_mode(it, mode, self.cubic), full)
+        self.steps = start
+        self.parity = start % 2
+        self.graphs = [self._capture(k) for k in range(self.period)]
 
-    def _enqueue(self, parity: int, mode: int, slots: int | None = None) -> None:
+    def _enqueue(self, k: int, mode: int, slots: int | None = None) -> None:
+        """Step k's launches: state buffers by k mod 2; histories by k mod 2
+        (two, swapped) or k mod 3 (three, rotated: step k writes N into the
+        oldest, h[(2 - k) mod 3])."""
         b, h = self.bufs, self.hist
-        h1, h2 = (h[0], h[1]) if self.cubic else (h[parity], h[1 - parity])
+        parity = k % 2
+        if self.cubic:
+            r = k % 3
+            h1, h2, h3 = h[(3 - r) % 3], h[(4 - r) % 3], h[(5 - r) % 3]
+        else:
+            h1, h2, h3 = h[parity], h[1 - parity], None
         self.nat.step_device_guess(b[parity], b[1 - parity], self.rhs[0], self.rhs[1], h1, h2, mode,
                                    self.slots if slots is None else slots, self.counts, self.ops, self.sc,
-                                   hist3=h[2] if self.cubic else None)
+                                   hist3=h3)
 
-    def _capture(self, parity: int):
+    def _capture(self, k: int):
         g = torch.cuda.CUDAGraph()
         torch.cuda.synchronize()
         with torch.cuda.graph(g):
-            self._enqueue(parity, self.mode)
+            self._enqueue(k, self.mode)
         return g
 
     def step(self, n: int = 1) -> None:
         if self.steps + n > self.capacity:
             raise ValueError("pass-count capacity exceeded")
         for _ in range(n):
-            self.graphs[self.parity].replay()
-            self.parity ^= 1
+            self.graphs[self.steps % self.period].replay()
             self.steps += 1
+            self.parity = self.steps % 2
 
     @property
     def state(self):

@@ -171,8 +171,9 @@ class NativeContext:
               carry=False):
         """Predictor with a corrector guess (mode 0 plain, 1 linear, 3
         quadratic, 4 cubic extrapolation, 2 quadratic -- cubic with hist3 --
-        plus the network `net` = (H, packed weights, bias)).  Without hist3,
-        hist2 receives N^n; with it the three histories shift in place.
+        plus the network `net` = (H, packed weights, bias)).  N^n overwrites
+        the oldest history: hist2 (swap hist1, hist2 next) or hist3 (rotate
+        (hist1, hist2, hist3) <- (hist3, hist1, hist2) next).
         carry: rhs_prev receives the carry form's B (periodic)."""
         H, wb, bias = (32, None, None) if net is None else net
         s_1, s_2, s_out = scales

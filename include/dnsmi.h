@@ -268,10 +268,10 @@ int dnsmi_mlp_features(dnsmi_ctx* ctx, const double* n_m3, const double* n_m2, c
  * iteration in its full form u <- ildt_2 (ldt_1 u^n + c N(u) + (1 - c) N^n),
  * started from the guess.  hist1 = N^{n-1}, hist2 = N^{n-2}, hist3 =
  * N^{n-3} or NULL: 3 * spec_elems complex values each as FP32 re/im pairs.
- * Without hist3, hist2 is overwritten with N^n, so the caller swaps hist1
- * and hist2 for the next step; with hist3 (periodic contexts only) the
- * three are shifted in place (hist3 <- hist2 <- hist1 <- N^n) and the
- * caller keeps them as they are.  Mode 0 (the plain predictor) reads no
+ * N^n overwrites the oldest history: without hist3, hist2, so the caller
+ * swaps hist1 and hist2 for the next step; with hist3 (periodic contexts
+ * only), hist3, so the caller rotates (hist1, hist2, hist3) <- (hist3,
+ * hist1, hist2).  Mode 0 (the plain predictor) reads no
  * history.  carry != 0 (periodic contexts): rhs_prev receives the carry
  * form's B = ildt_2 (ldt_1 u^n + (1 - c) N^n) instead, for
  * dnsmi_iterate_correction_carry.
@@ -290,8 +290,7 @@ int dnsmi_guess(dnsmi_ctx* ctx, const double* state, double* pred, double* rhs_p
  * once error <= tol or max_pass passes ran, and the escalation slots of
  * dnsmi_step_device; then correct_velocity(next).  counts[i] receives
  * step i's number of passes (one RHS each).  hist1..hist3 as in dnsmi_guess
- * (without hist3, hist2 receives N^n: swap hist1 and hist2, and state and
- * next, for the next step; with it only state and next swap).  A
+ * (swap or rotate them, and swap state and next, for the next step).  A
  * fixed launch sequence with no host synchronisation: capturable. */
 int dnsmi_step_device_guess(dnsmi_ctx* ctx, const double* state, double* next, double* rhs_a,
                             double* rhs_b, float* hist1, float* hist2, float* hist3,

@@ -145,11 +145,47 @@ def test_guess_kernel_matches_oracle(mode):
     assert torch.equal(d1, dev(h1))  # hist1 is read only
 
 
+@pytest.mark.parametrize("mode", [0, 3, 2])
+def test_guess_carry_output(mode):
+    """With carry, the guess writes B = ildt_2 (ldt_1 u^n + (1 - c) N^n) (the
+    carry form's corrector input) in place of N^n + dN, and the same
+    prediction; one carry-form pass from it equals the incremental pass
+    from N^n + dN."""
+    from dnsjax_amd.fields import periodic_field
+
+    n, L = 16, 2 * math.pi
+    flow = _periodic(n, 40.0)
+    of = P.Flow(P.PeriodicParams(nx=n, ny=n, nz=n, lx=L, ly=L, lz=L, re=40.0, dt=0.01, system="kolmogorov"))
+    u = periodic_field(n, n, n, L, L, L, 0.1, 3)
+    hs = [P.get_rhs(periodic_field(n, n, n, L, L, L, 0.1, s), of).astype(np.complex64) for s in (4, 6)]
+    net = _net_with(32, _random_params(32, 2))
+    net.pack()
+    nat = flow._native
+    dev = lambda h: torch.from_numpy(np.stack([h.real, h.imag], -1).astype(np.float32)).to(nat.device)  # noqa: E731
+    ud = flow.to_device(u)
+    outs = []
+    for carry in (False, True):
+        d = [dev(h).contiguous() for h in hs]
+        pred, rhs = nat.empty(), nat.empty()
+        nat.guess(ud, pred, rhs, d[0], d[1], mode, net.operands, (1.0, 3.0, 0.05), carry=carry)
+        outs.append((pred, rhs))
+    assert torch.equal(outs[0][0], outs[1][0])
+    n0 = P.get_rhs(u, of)
+    want_b = of.ildt_2 * (of.ldt_1 * u + (1 - of.p.implicitness) * n0)
+    assert rel(outs[1][1].cpu().numpy(), want_b) <= 1e-12
+    # one pass each way from the same guess
+    p_inc, p_car = outs[0][0].clone(), outs[1][0].clone()
+    err = nat.scalar()
+    nat.iterate_correction(ud, p_inc, outs[0][1], nat.empty(), err)
+    nat.iterate_correction_carry(p_car, outs[1][1], err)
+    assert rel(p_car.cpu().numpy(), p_inc.cpu().numpy()) <= 1e-13
+
+
 @pytest.mark.parametrize("mode", [4, 2])
 def test_cubic_guess_kernel_matches_oracle(mode):
-    """With a third history: cubic extrapolation (mode 4) and the network
-    on top of it (mode 2) against the oracle; the three histories shift in
-    place (hist3 <- hist2 <- hist1 <- N^n)."""
+    """With a third history: cubic extrapolation (mode 4) and the network on
+    top of it (mode 2) against the oracle; N^n overwrites the oldest history, hist3, and the other two are
+    read only."""
     from dnsjax_amd.fields import periodic_field
 
     n, L = 16, 2 * math.pi
@@ -175,10 +211,10 @@ def test_cubic_guess_kernel_matches_oracle(mode):
     if mode == 2:  # the network did contribute beyond cubic extrapolation
         _, cub, _ = M.guess(u, *[h.astype(np.complex128) for h in hs[:2]], of, 4, h3=hs[2].astype(np.complex128))
         assert rel(want_rhs, cub) > 1e-4
-    h = d[0].cpu().numpy().astype(np.float64)
+    h = d[2].cpu().numpy().astype(np.float64)
     w = np.stack([want_hist.real, want_hist.imag], -1)
     assert np.max(np.abs(h - w)) <= 1e-7 * np.max(np.abs(w))
-    assert torch.equal(d[1], dev(hs[0])) and torch.equal(d[2], dev(hs[1]))
+    assert torch.equal(d[0], dev(hs[0])) and torch.equal(d[1], dev(hs[1]))
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
