@@ -53,6 +53,7 @@ class DnsmiConfig(ctypes.Structure):
         ("plain_mixed", ctypes.c_int),
         ("recenter_far", ctypes.c_double),
         ("recenter_near", ctypes.c_double),
+        ("setup_on_host", ctypes.c_int),
     ]
 
 

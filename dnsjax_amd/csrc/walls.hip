@@ -21,7 +21,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstdint>
 #include <cstring>
 #include <string>
@@ -64,9 +66,10 @@ struct WallTables {
   double *lk_fmap = nullptr, *lk_bmap = nullptr, *hk_fmap = nullptr, *hk_bmap = nullptr;
   int2* sched = nullptr;  // (mode, map) per slot
   int U = 0;              // distinct k^2 values
-  // host copies in the reference layouts, for inspection by tests
-  std::vector<double> h_lk_ab, h_hk_ab, h_v1, h_q1, h_minv, h_ys;
-  std::vector<int> h_lk_piv, h_hk_piv;
+  // host copies: the y grid, and each mode's class (walls_table reads the
+  // device tables back in the reference layouts through it)
+  std::vector<double> h_ys;
+  std::vector<int> h_umode;
 };
 
 // ------------------------------------------------------------------ host set-up
@@ -232,7 +235,7 @@ __host__ __device__ constexpr int map_pre(int k, int lag, int w) {
 __host__ __device__ constexpr int map_word_at(int k, int u, int t, int lag, int w, int U) {
   return U * map_pre(k, lag, w) + u * map_len(k, lag, w) + (t >= map_lo(k, lag) ? 1 + t - map_lo(k, lag) : 0);
 }
-static size_t map_offset(int c, int k, int u, int t, int lag, int w, int U) {
+__host__ __device__ inline size_t map_offset(int c, int k, int u, int t, int lag, int w, int U) {
   return (size_t)c * U * map_pre(w, lag, w) + (size_t)map_word_at(k, u, t, lag, w, U);
 }
 
@@ -350,6 +353,313 @@ static int upload_vec(Ctx* c, V** dst, const std::vector<V>& v) {
 
 static int walls_autotune(Ctx* c);
 
+// ------------------------------------------------------------------ set-up on the GPU
+// The per-class tables of geometries/cartesian.py:575-801 built on the GPU:
+// the band LU of L_k and H_k (band_lu), the homogeneous solutions and M^-1
+// (the band_solve / D1 sequence of walls_setup's class loop) and the chunk
+// maps (chunk_maps), with the host code's operations in its order and every
+// product, sum and quotient rounded on its own as the host computes them
+// (contraction into FMA is switched off in these kernels), so the tables
+// equal the host set-up's bit for bit.  One workgroup of one wavefront per
+// k^2 class (and matrix, and chunk); a class's band lives in LDS while it is
+// factored and used.
+struct GpuSetup {
+  int n, kl, ku, rows, U, W1, W2;  // W1, W2: widths of the D1 and D2 stencils
+  const double* d1w;
+  const int* d1j0;
+  const double* d2w;
+  const int* d2j0;
+  const double* k2;  // [U]
+  double dt, cc, nu;
+  double* band;  // [2][U][rows][n], raw(r, j) at r * n + j; L_k then H_k
+  int* piv;      // [2][U][n]
+  int* flag;     // [0]: a zero pivot, [1]: a chunk-map entry left out of the packing is not zero
+  double* vq;    // [U][n][4]
+  double* minv;  // [U][4]
+};
+
+__device__ inline double setup_stencil(const double* w, const int* j0, int W, int i, int j) {
+  const int k = j - j0[i];
+  return (k >= 0 && k < W) ? w[i * W + k] : 0.0;
+}
+
+// d1_apply of walls_setup: s += w_k v_(j0 + k), k ascending, from 0.
+__device__ inline double setup_d1(const GpuSetup& g, const double* v, int i) {
+#pragma clang fp contract(off)
+  double s = 0.0;
+  for (int k = 0; k < g.W1; ++k) {
+    const double t = g.d1w[i * g.W1 + k] * v[g.d1j0[i] + k];
+    s = s + t;
+  }
+  return s;
+}
+
+// Build and factor L_k (blockIdx.y = 0) or H_k (1) of class blockIdx.x.
+__global__ void __launch_bounds__(64) k_setup_lu(GpuSetup g) {
+#pragma clang fp contract(off)
+  extern __shared__ double ab[];  // rows x n
+  const int u = blockIdx.x, which = blockIdx.y, lane = threadIdx.x;
+  const int n = g.n, kl = g.kl, ku = g.ku, kv = kl + ku, rows = g.rows;
+  const double k2 = g.k2[u];
+  for (int e = lane; e < rows * n; e += 64) ab[e] = 0.0;
+  __syncthreads();
+  for (int i = lane; i < n; i += 64) {
+    const int jlo = i - kl > 0 ? i - kl : 0, jhi = i + ku < n - 1 ? i + ku : n - 1;
+    for (int j = jlo; j <= jhi; ++j) {
+      const double lraw = setup_stencil(g.d2w, g.d2j0, g.W2, i, j) - (i == j ? k2 : 0.0);
+      double v;
+      if (which == 0) {
+        v = lraw;
+        if (i == 0) v = (k2 == 0.0) ? (j == 0 ? 1.0 : 0.0) : setup_stencil(g.d1w, g.d1j0, g.W1, 0, j);
+        if (i == n - 1) v = setup_stencil(g.d1w, g.d1j0, g.W1, n - 1, j);
+      } else {
+        const double cn = g.cc * g.nu;
+        const double t = cn * lraw;
+        v = (i == j ? 1.0 / g.dt : 0.0) - t;
+        if (i == 0 || i == n - 1) v = (i == j) ? 1.0 : 0.0;
+      }
+      ab[(kv + i - j) * n + j] = v;
+    }
+  }
+  __syncthreads();
+  int ju = 0;
+  bool zero = false;
+  int* piv = g.piv + (size_t)(which * g.U + u) * n;
+  for (int j = 0; j < n; ++j) {
+    const int km = kl < n - 1 - j ? kl : n - 1 - j;
+    int jp = 0;
+    double best = fabs(ab[kv * n + j]);
+    for (int i = 1; i <= km; ++i) {
+      const double a = fabs(ab[(kv + i) * n + j]);
+      if (a > best) {
+        best = a;
+        jp = i;
+      }
+    }
+    const double pv = ab[(kv + jp) * n + j];
+    if (lane == 0) piv[j] = j + jp;
+    __syncthreads();
+    if (pv == 0.0) {
+      zero = true;
+      continue;
+    }
+    const int jr = j + ku + jp < n - 1 ? j + ku + jp : n - 1;
+    ju = ju > jr ? ju : jr;
+    if (jp != 0) {
+      for (int c = j + lane; c <= ju; c += 64) {
+        const double a0 = ab[(kv + j - c) * n + c];
+        ab[(kv + j - c) * n + c] = ab[(kv + j + jp - c) * n + c];
+        ab[(kv + j + jp - c) * n + c] = a0;
+      }
+      __syncthreads();
+    }
+    if (km > 0) {
+      const double r = 1.0 / ab[kv * n + j];
+      if (lane >= 1 && lane <= km) ab[(kv + lane) * n + j] = ab[(kv + lane) * n + j] * r;
+      __syncthreads();
+      const int nc = ju - j;
+      for (int e = lane; e < nc * km; e += 64) {
+        const int q = e / km, i = 1 + e - q * km, c = j + 1 + q;
+        const double t = -ab[(kv + j - c) * n + c];
+        if (t != 0.0) {
+          const double prod = ab[(kv + i) * n + j] * t;
+          ab[(kv + j + i - c) * n + c] = ab[(kv + j + i - c) * n + c] + prod;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (zero && lane == 0) g.flag[0] = 1;
+  double* out = g.band + (size_t)(which * g.U + u) * rows * n;
+  for (int e = lane; e < rows * n; e += 64) out[e] = ab[e];
+}
+
+// band_solve on the vectors v + q n (q < nvec), one per 16 lanes, with the
+// factored band ab and pivots pl in LDS.
+__device__ inline void setup_solve(const GpuSetup& g, const double* ab, const int* pl, double* v, int nvec) {
+#pragma clang fp contract(off)
+  const int n = g.n, kl = g.kl, kv = g.kl + g.ku, lane = threadIdx.x;
+  const int q = lane >> 4, t = lane & 15;
+  const bool act = q < nvec;
+  double* b = v + (act ? q : 0) * n;
+  for (int j = 0; j < n - 1; ++j) {
+    if (act && t == 0) {
+      const int pj = pl[j];
+      const double x = b[j];
+      b[j] = b[pj];
+      b[pj] = x;
+    }
+    __syncthreads();
+    if (act && t >= 1 && t <= kl && j + t < n) {
+      const double prod = ab[(kv + t) * n + j] * b[j];
+      b[j + t] = b[j + t] - prod;
+    }
+    __syncthreads();
+  }
+  for (int j = n - 1; j >= 0; --j) {
+    if (act && t == 0) b[j] = b[j] / ab[kv * n + j];
+    __syncthreads();
+    const int i = t + 1;
+    if (act && i <= kv && j - i >= 0) {
+      const double prod = ab[(kv - i) * n + j] * b[j];
+      b[j - i] = b[j - i] - prod;
+    }
+    __syncthreads();
+  }
+}
+
+// Homogeneous solutions and M^-1 of class blockIdx.x: p_1,2 = L^-1 e_(0, n-1);
+// v_1,2 = H^-1 (-D1 p_1,2 with zero wall rows); q_1,2 = H^-1 (p_1,2 with
+// zero wall rows); M = D1 v at the walls.
+__global__ void __launch_bounds__(64) k_setup_homog(GpuSetup g) {
+#pragma clang fp contract(off)
+  extern __shared__ double sm[];
+  const int u = blockIdx.x, lane = threadIdx.x, n = g.n, rows = g.rows;
+  double* ab = sm;               // rows x n
+  double* v = sm + rows * n;     // four vectors of n
+  int* pl = reinterpret_cast<int*>(v + 4 * n);
+  for (int which = 0; which < 2; ++which) {
+    const double* src = g.band + (size_t)(which * g.U + u) * rows * n;
+    const int* ps = g.piv + (size_t)(which * g.U + u) * n;
+    for (int e = lane; e < rows * n; e += 64) ab[e] = src[e];
+    for (int e = lane; e < n; e += 64) pl[e] = ps[e];
+    if (which == 0)
+      for (int e = lane; e < 2 * n; e += 64) v[e] = (e == 0 || e == 2 * n - 1) ? 1.0 : 0.0;
+    __syncthreads();
+    setup_solve(g, ab, pl, v, which == 0 ? 2 : 4);
+    if (which == 0) {
+      // r_1,2 = -D1 p_1,2 (zero on the walls) into vectors 2, 3; then the
+      // walls of p_1,2 zeroed in place: the four H_k right-hand sides
+      for (int i = lane; i < n; i += 64) {
+        const bool wall = i == 0 || i == n - 1;
+        v[2 * n + i] = wall ? 0.0 : -setup_d1(g, v, i);
+        v[3 * n + i] = wall ? 0.0 : -setup_d1(g, v + n, i);
+      }
+      __syncthreads();
+      if (lane == 0) v[0] = v[n - 1] = v[n] = v[2 * n - 1] = 0.0;
+      __syncthreads();
+    }
+  }
+  if (lane == 0) {
+    const double m00 = setup_d1(g, v + 2 * n, 0), m01 = setup_d1(g, v + 3 * n, 0);
+    const double m10 = setup_d1(g, v + 2 * n, n - 1), m11 = setup_d1(g, v + 3 * n, n - 1);
+    double mi[4];
+    if (g.k2[u] == 0.0) {
+      mi[0] = mi[1] = mi[2] = 0.0;
+      mi[3] = 1.0 / m11;
+    } else {
+      const double a = m00 * m11, b = m01 * m10;
+      const double det = a - b;
+      mi[0] = m11 / det;
+      mi[1] = -m01 / det;
+      mi[2] = -m10 / det;
+      mi[3] = m00 / det;
+    }
+    for (int q = 0; q < 4; ++q) g.minv[u * 4 + q] = mi[q];
+  }
+  for (int j = lane; j < n; j += 64) {
+    double* w = g.vq + ((size_t)u * n + j) * 4;
+    w[0] = v[2 * n + j];
+    w[1] = v[3 * n + j];
+    w[2] = v[j];
+    w[3] = v[n + j];
+  }
+}
+
+// Chunk map of chunk blockIdx.x, class blockIdx.y; blockIdx.z: 0 L_k
+// forward, 1 L_k back, 2 H_k forward, 3 H_k back (chunk_maps' steps).
+__global__ void __launch_bounds__(64) k_setup_maps(GpuSetup g, double* lf, double* lb, double* hf, double* hb) {
+#pragma clang fp contract(off)
+  constexpr int WMAX = kChunk + 16;  // B + 2p for p <= 8
+  __shared__ double X[WMAX * WMAX];
+  const int c = blockIdx.x, u = blockIdx.y, which = blockIdx.z >> 1, back = blockIdx.z & 1;
+  const int lane = threadIdx.x, n = g.n, kl = g.kl, kuu = g.kl + g.ku, B = kChunk;
+  const double* ab = g.band + (size_t)(which * g.U + u) * g.rows * n;
+  const int* pv = g.piv + (size_t)(which * g.U + u) * n;
+  const int W = back ? B + kuu : B + kl;
+  for (int e = lane; e < W * W; e += 64) X[e] = (e / W == e % W) ? 1.0 : 0.0;
+  __syncthreads();
+  const int j0 = c * B;
+  if (!back) {
+    const int jend = j0 + B < n - 1 ? j0 + B : n - 1;
+    for (int j = j0; j < jend; ++j) {
+      const int tj = j - j0, tp = pv[j] - j0;
+      if (tp != tj) {
+        for (int k = lane; k < W; k += 64) {
+          const double x = X[tj * W + k];
+          X[tj * W + k] = X[tp * W + k];
+          X[tp * W + k] = x;
+        }
+        __syncthreads();
+      }
+      for (int e = lane; e < kl * W; e += 64) {
+        const int i = 1 + e / W, k = e % W;
+        if (j + i < n) {
+          const double prod = ab[(kuu + i) * n + j] * X[tj * W + k];
+          X[(tj + i) * W + k] = X[(tj + i) * W + k] - prod;
+        }
+      }
+      __syncthreads();
+    }
+  } else {
+    const int T = n - 1 - j0;
+    for (int j = T; j > T - B && j >= 0; --j) {
+      const int tj = T - j;
+      const double dg = ab[kuu * n + j];
+      for (int k = lane; k < W; k += 64) X[tj * W + k] = X[tj * W + k] / dg;
+      __syncthreads();
+      for (int e = lane; e < kuu * W; e += 64) {
+        const int i = 1 + e / W, k = e % W;
+        if (j - i >= 0) {
+          const double prod = ab[(kuu - i) * n + j] * X[tj * W + k];
+          X[(tj + i) * W + k] = X[(tj + i) * W + k] - prod;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  double* F = which == 0 ? (back ? lb : lf) : (back ? hb : hf);
+  const int lag = back ? 0 : kl;
+  bool bad = false;
+  for (int e = lane; e < W * W; e += 64) {
+    const int t = e / W, k = e % W;
+    const double x = X[t * W + k];
+    if (t >= map_lo(k, lag)) F[map_offset(c, k, u, t, lag, W, g.U)] = x;
+    else bad = bad || x != 0.0;
+  }
+  if (bad) g.flag[1] = 1;
+}
+
+// Per-mode tables from the per-class ones: band factors [j][row][mode],
+// pivots [j][mode], M^-1 [4][mode].
+__global__ void __launch_bounds__(256) k_setup_scatter(GpuSetup g, const int* __restrict__ umode, int M,
+                                                       double* lk_ab, double* hk_ab, int* lk_piv, int* hk_piv,
+                                                       double* minv) {
+  const long long nab = (long long)g.n * g.rows * M, npv = (long long)g.n * M;
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < nab; e += (long long)gridDim.x * blockDim.x) {
+    const int m = (int)(e % M);
+    const long long jr = e / M;
+    const int r = (int)(jr % g.rows), j = (int)(jr / g.rows), u = umode[m];
+    lk_ab[e] = g.band[((size_t)u * g.rows + r) * g.n + j];
+    hk_ab[e] = g.band[((size_t)(g.U + u) * g.rows + r) * g.n + j];
+    if (e < npv) {
+      const int mm = (int)(e % M), jj = (int)(e / M), uu = umode[mm];
+      lk_piv[e] = g.piv[(size_t)uu * g.n + jj];
+      hk_piv[e] = g.piv[(size_t)(g.U + uu) * g.n + jj];
+    }
+    if (e < 4LL * M) {
+      const int mm = (int)(e % M), q = (int)(e / M);
+      minv[e] = g.minv[umode[mm] * 4 + q];
+    }
+  }
+}
+
+// LDS bytes of the set-up kernels (the GPU set-up runs where both fit 64 KiB).
+static size_t setup_lds_lu(int n, int rows) { return (size_t)rows * n * sizeof(double); }
+static size_t setup_lds_homog(int n, int rows) {
+  return ((size_t)rows * n + 4ull * n) * sizeof(double) + (size_t)n * sizeof(int);
+}
+
 int walls_setup(Ctx* c) {
   const dnsmi_config& g = c->cfg;
   if (g.nx % 2 || g.nz % 2 || g.nx < 4 || g.nz < 4 || g.ny < 5)
@@ -439,66 +749,138 @@ int walls_setup(Ctx* c) {
   // per-mode IMM tables (geometries/cartesian.py:704-779)
   const int kl = p, ku = p, rows = 3 * p + 1;
   const double dt = g.dt, cc = g.implicitness, nu = 1.0 / g.re;
-  std::vector<double> lk_ab((size_t)Ny * rows * M), hk_ab((size_t)Ny * rows * M);
-  std::vector<int> lk_piv((size_t)Ny * M), hk_piv((size_t)Ny * M);
-  std::vector<double> minv(4ull * M), mkx(M), mkz(M);
-  t->h_lk_ab.assign((size_t)M * rows * Ny, 0.0);
-  t->h_hk_ab.assign((size_t)M * rows * Ny, 0.0);
-  t->h_lk_piv.assign((size_t)M * Ny, 0);
-  t->h_hk_piv.assign((size_t)M * Ny, 0);
-  t->h_v1.assign((size_t)M * Ny, 0.0);
-  t->h_q1.assign((size_t)M * Ny, 0.0);
-  t->h_minv.assign(4ull * M, 0.0);
   // L_k and H_k depend on the mode only through k^2: modes with the same
-  // k^2 (bit for bit) share every per-mode table, computed at the first of
-  // them (first[u]) and copied to the others
+  // k^2 (bit for bit) share every per-mode table, computed once per class
+  // (at the class's first mode, first[u]) and copied to the others
   std::vector<int> umode(M), first;
+  std::vector<double> mkx(M), mkz(M), k2u;
   {
     std::unordered_map<uint64_t, int> key;
     for (int iz = 0; iz < Nkz; ++iz)
       for (int ix = 0; ix < Nkx; ++ix) {
         const int m = iz * Nkx + ix;
         const double k2 = kx[ix] * kx[ix] + kz[iz] * kz[iz];
+        mkx[m] = kx[ix];
+        mkz[m] = kz[iz];
         uint64_t b;
         std::memcpy(&b, &k2, sizeof b);
         const auto it = key.emplace(b, (int)first.size());
-        if (it.second) first.push_back(m);
+        if (it.second) {
+          first.push_back(m);
+          k2u.push_back(k2);
+        }
         umode[m] = it.first->second;
       }
   }
   const int U = (int)first.size();
   t->U = U;
-  std::vector<double> vq((size_t)U * Ny * 4);
-  auto copy_mode = [&](int m, int r) {
-    for (int j = 0; j < Ny; ++j) {
-      for (int q = 0; q < rows; ++q) {
-        lk_ab[((size_t)j * rows + q) * M + m] = lk_ab[((size_t)j * rows + q) * M + r];
-        hk_ab[((size_t)j * rows + q) * M + m] = hk_ab[((size_t)j * rows + q) * M + r];
-        t->h_lk_ab[((size_t)m * rows + q) * Ny + j] = t->h_lk_ab[((size_t)r * rows + q) * Ny + j];
-        t->h_hk_ab[((size_t)m * rows + q) * Ny + j] = t->h_hk_ab[((size_t)r * rows + q) * Ny + j];
+  t->h_umode = umode;
+  if ((rc = upload_vec(c, &t->umode, umode))) return rc;
+  // the chunk maps of the L_k and H_k solves, one per class (not when the
+  // sequential sweeps are requested)
+  const int B = kChunk, WF = B + kl, WB = B + kl + ku, nC = (Ny + B - 1) / B;
+  const size_t nf = (size_t)nC * U * map_pre(WF, kl, WF), nb = (size_t)nC * U * map_pre(WB, 0, WB);
+  // the chunked solve also stages three right-hand sides per mode pair in
+  // LDS (2 * 3 * (Ny + 32) complex values), which must fit in 64 KiB
+  const bool maps_fit = nf < (1ull << 28) && nb < (1ull << 28) &&
+                        (size_t)2 * 3 * (Ny + 32) * sizeof(cx<double>) <= 65536;
+  const bool forced = g.tile_cols != 0;  // the variant bits choose the form
+  if (forced && (g.variants & 8) == 0 && !maps_fit)
+    return c->fail(DNSMI_EINVAL,
+                   "chunk maps exceed 2^28 words or their LDS staging 64 KiB: use the sequential band solve");
+  const bool want_maps = maps_fit && !(forced && (g.variants & 8) != 0);
+  // on the GPU unless asked for on the host, or a class's band and vectors
+  // do not fit the set-up kernels' 64 KiB of LDS (or the classes a grid's
+  // y dimension)
+  const bool on_gpu = !g.setup_on_host && setup_lds_homog(Ny, rows) <= 65536 && U <= 65535;
+  const auto t0 = std::chrono::steady_clock::now();
+  if (on_gpu) {
+    GpuSetup gs{};
+    gs.n = Ny;
+    gs.kl = kl;
+    gs.ku = ku;
+    gs.rows = rows;
+    gs.U = U;
+    gs.W1 = d1.width;
+    gs.W2 = d2.width;
+    gs.d1w = t->d1w;
+    gs.d1j0 = t->d1j0;
+    gs.d2w = t->d2w;
+    gs.d2j0 = t->d2j0;
+    gs.dt = dt;
+    gs.cc = cc;
+    gs.nu = nu;
+    double* k2d = nullptr;
+    if ((rc = upload_vec(c, &k2d, k2u))) return rc;
+    gs.k2 = k2d;
+    gs.band = static_cast<double*>(c->alloc(2ull * U * rows * Ny * sizeof(double)));
+    gs.piv = static_cast<int*>(c->alloc(2ull * U * Ny * sizeof(int)));
+    gs.flag = static_cast<int*>(c->alloc(2 * sizeof(int)));
+    gs.minv = static_cast<double*>(c->alloc(4ull * U * sizeof(double)));
+    t->vq = static_cast<double*>(c->alloc(4ull * U * Ny * sizeof(double)));
+    t->lk_ab = static_cast<double*>(c->alloc((size_t)Ny * rows * M * sizeof(double)));
+    t->hk_ab = static_cast<double*>(c->alloc((size_t)Ny * rows * M * sizeof(double)));
+    t->lk_piv = static_cast<int*>(c->alloc((size_t)Ny * M * sizeof(int)));
+    t->hk_piv = static_cast<int*>(c->alloc((size_t)Ny * M * sizeof(int)));
+    t->minv = static_cast<double*>(c->alloc(4ull * M * sizeof(double)));
+    if (!gs.band || !gs.piv || !gs.flag || !gs.minv || !t->vq || !t->lk_ab || !t->hk_ab || !t->lk_piv ||
+        !t->hk_piv || !t->minv)
+      return c->fail(DNSMI_ENOMEM, "device allocation failed");
+    gs.vq = t->vq;
+    if (hipMemset(gs.flag, 0, 2 * sizeof(int)) != hipSuccess) return c->fail(DNSMI_EHIP, "memset");
+    hipLaunchKernelGGL(k_setup_lu, dim3(U, 2), dim3(64), setup_lds_lu(Ny, rows), nullptr, gs);
+    if ((rc = check_launch(c, "setup_lu"))) return rc;
+    hipLaunchKernelGGL(k_setup_homog, dim3(U), dim3(64), setup_lds_homog(Ny, rows), nullptr, gs);
+    if ((rc = check_launch(c, "setup_homog"))) return rc;
+    if (want_maps) {
+      for (double** m : {&t->lk_fmap, &t->hk_fmap})
+        if (!(*m = static_cast<double*>(c->alloc(nf * sizeof(double)))))
+          return c->fail(DNSMI_ENOMEM, "device allocation failed");
+      for (double** m : {&t->lk_bmap, &t->hk_bmap})
+        if (!(*m = static_cast<double*>(c->alloc(nb * sizeof(double)))))
+          return c->fail(DNSMI_ENOMEM, "device allocation failed");
+      // each packed column's leading zero word is read, never written
+      if (hipMemset(t->lk_fmap, 0, nf * sizeof(double)) != hipSuccess ||
+          hipMemset(t->hk_fmap, 0, nf * sizeof(double)) != hipSuccess ||
+          hipMemset(t->lk_bmap, 0, nb * sizeof(double)) != hipSuccess ||
+          hipMemset(t->hk_bmap, 0, nb * sizeof(double)) != hipSuccess)
+        return c->fail(DNSMI_EHIP, "memset");
+      hipLaunchKernelGGL(k_setup_maps, dim3(nC, U, 4), dim3(64), 0, nullptr, gs, t->lk_fmap, t->lk_bmap,
+                         t->hk_fmap, t->hk_bmap);
+      if ((rc = check_launch(c, "setup_maps"))) return rc;
+    }
+    const long long nab = (long long)Ny * rows * M;
+    const int sb = (int)std::min<long long>((nab + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_setup_scatter, dim3(sb), dim3(256), 0, nullptr, gs, t->umode, M, t->lk_ab,
+                       t->hk_ab, t->lk_piv, t->hk_piv, t->minv);
+    if ((rc = check_launch(c, "setup_scatter"))) return rc;
+    int flag[2] = {0, 0};
+    if (hipMemcpy(flag, gs.flag, sizeof flag, hipMemcpyDeviceToHost) != hipSuccess)
+      return c->fail(DNSMI_EHIP, "set-up: read flags");
+    for (void* q : {(void*)k2d, (void*)gs.band, (void*)gs.piv, (void*)gs.flag, (void*)gs.minv}) c->release(q);
+    if (flag[0]) return c->fail(DNSMI_ESINGULAR, "banded LU hit a zero pivot");
+    if (flag[1]) return c->fail(DNSMI_EINVAL, "chunk maps: nonzero entry outside the packed region");
+  } else {
+    std::vector<double> lk_ab((size_t)Ny * rows * M), hk_ab((size_t)Ny * rows * M);
+    std::vector<int> lk_piv((size_t)Ny * M), hk_piv((size_t)Ny * M);
+    std::vector<double> minv(4ull * M), vq((size_t)U * Ny * 4);
+    auto copy_mode = [&](int m, int r) {
+      for (int j = 0; j < Ny; ++j) {
+        for (int q = 0; q < rows; ++q) {
+          lk_ab[((size_t)j * rows + q) * M + m] = lk_ab[((size_t)j * rows + q) * M + r];
+          hk_ab[((size_t)j * rows + q) * M + m] = hk_ab[((size_t)j * rows + q) * M + r];
+        }
+        lk_piv[(size_t)j * M + m] = lk_piv[(size_t)j * M + r];
+        hk_piv[(size_t)j * M + m] = hk_piv[(size_t)j * M + r];
       }
-      lk_piv[(size_t)j * M + m] = lk_piv[(size_t)j * M + r];
-      hk_piv[(size_t)j * M + m] = hk_piv[(size_t)j * M + r];
-      t->h_lk_piv[(size_t)m * Ny + j] = t->h_lk_piv[(size_t)r * Ny + j];
-      t->h_hk_piv[(size_t)m * Ny + j] = t->h_hk_piv[(size_t)r * Ny + j];
-      t->h_v1[(size_t)m * Ny + j] = t->h_v1[(size_t)r * Ny + j];
-      t->h_q1[(size_t)m * Ny + j] = t->h_q1[(size_t)r * Ny + j];
-    }
-    for (int q = 0; q < 4; ++q) {
-      minv[(size_t)q * M + m] = minv[(size_t)q * M + r];
-      t->h_minv[(size_t)m * 4 + q] = t->h_minv[(size_t)r * 4 + q];
-    }
-  };
-  for (int iz = 0; iz < Nkz; ++iz) {
-    for (int ix = 0; ix < Nkx; ++ix) {
-      const int m = iz * Nkx + ix;
-      const double k2 = kx[ix] * kx[ix] + kz[iz] * kz[iz];
-      mkx[m] = kx[ix];
-      mkz[m] = kz[iz];
+      for (int q = 0; q < 4; ++q) minv[(size_t)q * M + m] = minv[(size_t)q * M + r];
+    };
+    for (int m = 0; m < M; ++m) {
       if (first[umode[m]] != m) {
         copy_mode(m, first[umode[m]]);
         continue;
       }
+      const double k2 = k2u[umode[m]];
       Band L(Ny, kl, ku), H(Ny, kl, ku);
       for (int i = 0; i < Ny; ++i) {
         for (int j = std::max(0, i - kl); j <= std::min(Ny - 1, i + ku); ++j) {
@@ -552,67 +934,61 @@ int walls_setup(Ctx* c) {
         Mi[2] = -M10 / det;
         Mi[3] = M00 / det;
       }
-      for (int q = 0; q < 4; ++q) {
-        minv[(size_t)q * M + m] = Mi[q];
-        t->h_minv[(size_t)m * 4 + q] = Mi[q];
-      }
+      for (int q = 0; q < 4; ++q) minv[(size_t)q * M + m] = Mi[q];
       for (int j = 0; j < Ny; ++j) {
         for (int r = 0; r < rows; ++r) {
           lk_ab[((size_t)j * rows + r) * M + m] = L.raw(r, j);
           hk_ab[((size_t)j * rows + r) * M + m] = H.raw(r, j);
-          t->h_lk_ab[((size_t)m * rows + r) * Ny + j] = L.raw(r, j);
-          t->h_hk_ab[((size_t)m * rows + r) * Ny + j] = H.raw(r, j);
         }
         lk_piv[(size_t)j * M + m] = L.piv[j];
         hk_piv[(size_t)j * M + m] = H.piv[j];
-        t->h_lk_piv[(size_t)m * Ny + j] = L.piv[j];
-        t->h_hk_piv[(size_t)m * Ny + j] = H.piv[j];
         double* w = &vq[((size_t)umode[m] * Ny + j) * 4];
         w[0] = r1[j];
         w[1] = r2[j];
         w[2] = s1[j];
         w[3] = s2[j];
-        t->h_v1[(size_t)m * Ny + j] = r1[j];
-        t->h_q1[(size_t)m * Ny + j] = s1[j];
       }
     }
+    if ((rc = upload_vec(c, &t->lk_ab, lk_ab))) return rc;
+    if ((rc = upload_vec(c, &t->hk_ab, hk_ab))) return rc;
+    if (want_maps) {  // built in parallel on the host threads
+      std::vector<double> lf(nf), lb(nb), hf(nf), hb(nb);
+      const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+      std::vector<char> ok(nth, 1);
+      auto work = [&](int w) {
+        for (int u = w; u < U; u += nth) {
+          const int m = first[u];
+          auto lraw = [&](int r, int j) { return lk_ab[((size_t)j * rows + r) * M + m]; };
+          auto lpiv = [&](int j) { return lk_piv[(size_t)j * M + m]; };
+          auto hraw = [&](int r, int j) { return hk_ab[((size_t)j * rows + r) * M + m]; };
+          auto hpiv = [&](int j) { return hk_piv[(size_t)j * M + m]; };
+          const bool a = chunk_maps(Ny, kl, ku, u, U, lraw, lpiv, lf.data(), lb.data());
+          const bool b = chunk_maps(Ny, kl, ku, u, U, hraw, hpiv, hf.data(), hb.data());
+          if (!a || !b) ok[w] = 0;
+        }
+      };
+      std::vector<std::thread> pool;
+      for (int w = 1; w < nth; ++w) pool.emplace_back(work, w);
+      work(0);
+      for (auto& th : pool) th.join();
+      for (char o : ok)
+        if (!o) return c->fail(DNSMI_EINVAL, "chunk maps: nonzero entry outside the packed region");
+      if ((rc = upload_vec(c, &t->lk_fmap, lf))) return rc;
+      if ((rc = upload_vec(c, &t->lk_bmap, lb))) return rc;
+      if ((rc = upload_vec(c, &t->hk_fmap, hf))) return rc;
+      if ((rc = upload_vec(c, &t->hk_bmap, hb))) return rc;
+    }
+    if ((rc = upload_vec(c, &t->lk_piv, lk_piv))) return rc;
+    if ((rc = upload_vec(c, &t->hk_piv, hk_piv))) return rc;
+    if ((rc = upload_vec(c, &t->vq, vq))) return rc;
+    if ((rc = upload_vec(c, &t->minv, minv))) return rc;
   }
-  if ((rc = upload_vec(c, &t->lk_ab, lk_ab))) return rc;
-  if ((rc = upload_vec(c, &t->hk_ab, hk_ab))) return rc;
-  // chunk maps of the L_k and H_k solves, one per distinct k^2, built in
-  // parallel (not when the sequential sweeps are requested)
-  const int B = kChunk, WF = B + kl, WB = B + kl + ku, nC = (Ny + B - 1) / B;
-  const size_t nf = (size_t)nC * U * map_pre(WF, kl, WF), nb = (size_t)nC * U * map_pre(WB, 0, WB);
-  // the chunked solve also stages three right-hand sides per mode pair in
-  // LDS (2 * 3 * (Ny + 32) complex values), which must fit in 64 KiB
-  const bool maps_fit = nf < (1ull << 28) && nb < (1ull << 28) &&
-                        (size_t)2 * 3 * (Ny + 32) * sizeof(cx<double>) <= 65536;
-  const bool forced = g.tile_cols != 0;  // the variant bits choose the form
-  if (forced && (g.variants & 8) == 0 && !maps_fit)
-    return c->fail(DNSMI_EINVAL,
-                   "chunk maps exceed 2^28 words or their LDS staging 64 KiB: use the sequential band solve");
-  if (maps_fit && !(forced && (g.variants & 8) != 0)) {
-    std::vector<double> lf(nf), lb(nb), hf(nf), hb(nb);
-    const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    std::vector<char> ok(nth, 1);
-    auto work = [&](int w) {
-      for (int u = w; u < U; u += nth) {
-        const int m = first[u];
-        auto lraw = [&](int r, int j) { return lk_ab[((size_t)j * rows + r) * M + m]; };
-        auto lpiv = [&](int j) { return lk_piv[(size_t)j * M + m]; };
-        auto hraw = [&](int r, int j) { return hk_ab[((size_t)j * rows + r) * M + m]; };
-        auto hpiv = [&](int j) { return hk_piv[(size_t)j * M + m]; };
-        const bool a = chunk_maps(Ny, kl, ku, u, U, lraw, lpiv, lf.data(), lb.data());
-        const bool b = chunk_maps(Ny, kl, ku, u, U, hraw, hpiv, hf.data(), hb.data());
-        if (!a || !b) ok[w] = 0;
-      }
-    };
-    std::vector<std::thread> pool;
-    for (int w = 1; w < nth; ++w) pool.emplace_back(work, w);
-    work(0);
-    for (auto& th : pool) th.join();
-    for (char o : ok)
-      if (!o) return c->fail(DNSMI_EINVAL, "chunk maps: nonzero entry outside the packed region");
+  if (hipDeviceSynchronize() != hipSuccess) return c->fail(DNSMI_EHIP, "set-up: synchronise");
+  const double setup_ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  char note[96];
+  std::snprintf(note, sizeof note, "imm_setup=%s(%.1f ms) ", on_gpu ? "gpu" : "host", setup_ms);
+  if (want_maps) {
     // the chunked solve's mode order: by map, so that the two modes of a
     // wavefront share theirs wherever two modes have the same k^2
     std::vector<int> order(M);
@@ -622,24 +998,20 @@ int walls_setup(Ctx* c) {
     std::vector<int2> sched(order.size());
     for (size_t i = 0; i < order.size(); ++i) sched[i] = make_int2(order[i], umode[order[i]]);
     if ((rc = upload_vec(c, &t->sched, sched))) return rc;
-    if ((rc = upload_vec(c, &t->lk_fmap, lf))) return rc;
-    if ((rc = upload_vec(c, &t->lk_bmap, lb))) return rc;
-    if ((rc = upload_vec(c, &t->hk_fmap, hf))) return rc;
-    if ((rc = upload_vec(c, &t->hk_bmap, hb))) return rc;
   }
-  if ((rc = upload_vec(c, &t->lk_piv, lk_piv))) return rc;
-  if ((rc = upload_vec(c, &t->hk_piv, hk_piv))) return rc;
-  if ((rc = upload_vec(c, &t->vq, vq))) return rc;
-  if ((rc = upload_vec(c, &t->umode, umode))) return rc;
-  if ((rc = upload_vec(c, &t->minv, minv))) return rc;
   if ((rc = upload_vec(c, &t->mkx, mkx))) return rc;
   if ((rc = upload_vec(c, &t->mkz, mkz))) return rc;
   if (!forced && !maps_fit) {  // the autotuner has one form to choose
     c->sweep_seq = true;
-    c->tuning = "band_solve=sequential(chunk maps exceed 2^28 words or 64 KiB LDS) ";
+    c->tuning = std::string(note) + "band_solve=sequential(chunk maps exceed 2^28 words or 64 KiB LDS) ";
     return DNSMI_OK;
   }
-  if (!forced) return walls_autotune(c);
+  if (!forced) {
+    rc = walls_autotune(c);
+    c->tuning = std::string(note) + c->tuning;
+    return rc;
+  }
+  c->tuning = note;
   return DNSMI_OK;
 }
 
@@ -1895,32 +2267,96 @@ int walls_iterate_correction(Ctx* c, const double* state_prev, double* pred,
   return walls_imm(c, state_prev, pred, rhs_prev, rhs_next, pred, nullptr, err, s);
 }
 
-// Host copies of the set-up tables in the reference layouts (for tests):
-// which = 0 Lk_ab, 1 Hk_ab, 2 v1, 3 q1, 4 M_inv, 5 ys (doubles) ; 6 Lk_piv,
-// 7 Hk_piv (ints written as doubles).
+// The set-up tables in the reference layouts (for tests), read back from the
+// device: which = 0 Lk_ab, 1 Hk_ab (mode, row, y), 2 v1, 3 q1 (mode, y),
+// 4 M_inv (mode, 2, 2), 5 ys; 6 Lk_piv, 7 Hk_piv (mode, y; ints written as
+// doubles); 8 .. 11 the packed chunk maps as stored (L_k forward, L_k back,
+// H_k forward, H_k back; empty without them); 12 the per-class homogeneous
+// solutions as stored ([class][y][4]); 13 the word counts of 8 .. 11 and
+// the number of k^2 classes.
 int walls_table(Ctx* c, int which, double* out, long long n) {
   WallTables* t = c->wt;
   if (!t) return c->fail(DNSMI_EINVAL, "not a wall-bounded context");
-  const std::vector<double>* v = nullptr;
+  const int Ny = t->Ny, M = t->M, rows = 3 * t->p + 1, U = t->U;
+  auto fetch = [&](const void* src, size_t bytes, void* dst) {
+    return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess;
+  };
+  auto want = [&](long long k) { return n == k ? DNSMI_OK : c->fail(DNSMI_EINVAL, "table size mismatch"); };
+  int rc;
   switch (which) {
-    case 0: v = &t->h_lk_ab; break;
-    case 1: v = &t->h_hk_ab; break;
-    case 2: v = &t->h_v1; break;
-    case 3: v = &t->h_q1; break;
-    case 4: v = &t->h_minv; break;
-    case 5: v = &t->h_ys; break;
+    case 0:
+    case 1: {
+      if ((rc = want((long long)M * rows * Ny))) return rc;
+      std::vector<double> d((size_t)Ny * rows * M);
+      if (!fetch(which == 0 ? t->lk_ab : t->hk_ab, d.size() * sizeof(double), d.data()))
+        return c->fail(DNSMI_EHIP, "read table");
+      for (int m = 0; m < M; ++m)
+        for (int r = 0; r < rows; ++r)
+          for (int j = 0; j < Ny; ++j) out[((size_t)m * rows + r) * Ny + j] = d[((size_t)j * rows + r) * M + m];
+      return DNSMI_OK;
+    }
+    case 2:
+    case 3:
+    case 12: {
+      std::vector<double> d(4ull * U * Ny);
+      if (!fetch(t->vq, d.size() * sizeof(double), d.data())) return c->fail(DNSMI_EHIP, "read table");
+      if (which == 12) {
+        if ((rc = want((long long)d.size()))) return rc;
+        std::copy(d.begin(), d.end(), out);
+        return DNSMI_OK;
+      }
+      if ((rc = want((long long)M * Ny))) return rc;
+      const int q = which == 2 ? 0 : 2;  // v1 = H^-1(-D1 p1), q1 = H^-1(p1)
+      for (int m = 0; m < M; ++m)
+        for (int j = 0; j < Ny; ++j) out[(size_t)m * Ny + j] = d[((size_t)t->h_umode[m] * Ny + j) * 4 + q];
+      return DNSMI_OK;
+    }
+    case 4: {
+      if ((rc = want(4LL * M))) return rc;
+      std::vector<double> d(4ull * M);
+      if (!fetch(t->minv, d.size() * sizeof(double), d.data())) return c->fail(DNSMI_EHIP, "read table");
+      for (int m = 0; m < M; ++m)
+        for (int q = 0; q < 4; ++q) out[(size_t)m * 4 + q] = d[(size_t)q * M + m];
+      return DNSMI_OK;
+    }
+    case 5:
+      if ((rc = want((long long)t->h_ys.size()))) return rc;
+      std::copy(t->h_ys.begin(), t->h_ys.end(), out);
+      return DNSMI_OK;
     case 6:
     case 7: {
-      const std::vector<int>& pv = which == 6 ? t->h_lk_piv : t->h_hk_piv;
-      if (n != (long long)pv.size()) return c->fail(DNSMI_EINVAL, "table size mismatch");
-      for (size_t i = 0; i < pv.size(); ++i) out[i] = pv[i];
+      if ((rc = want((long long)M * Ny))) return rc;
+      std::vector<int> d((size_t)Ny * M);
+      if (!fetch(which == 6 ? t->lk_piv : t->hk_piv, d.size() * sizeof(int), d.data()))
+        return c->fail(DNSMI_EHIP, "read table");
+      for (int m = 0; m < M; ++m)
+        for (int j = 0; j < Ny; ++j) out[(size_t)m * Ny + j] = d[(size_t)j * M + m];
+      return DNSMI_OK;
+    }
+    case 8:
+    case 9:
+    case 10:
+    case 11: {
+      const int kl = t->p, B = kChunk, WF = B + kl, WB = B + 2 * kl, nC = (Ny + B - 1) / B;
+      const bool fwd = which == 8 || which == 10;
+      const size_t words = (size_t)nC * U * (fwd ? map_pre(WF, kl, WF) : map_pre(WB, 0, WB));
+      const double* src = which == 8 ? t->lk_fmap : which == 9 ? t->lk_bmap : which == 10 ? t->hk_fmap : t->hk_bmap;
+      if (!src) return want(0);
+      if ((rc = want((long long)words))) return rc;
+      if (!fetch(src, words * sizeof(double), out)) return c->fail(DNSMI_EHIP, "read table");
+      return DNSMI_OK;
+    }
+    case 13: {  // the words of tables 8 .. 11 (0 where absent), then the class count
+      if ((rc = want(5))) return rc;
+      out[4] = U;
+      const int kl = t->p, B = kChunk, WF = B + kl, WB = B + 2 * kl, nC = (Ny + B - 1) / B;
+      const double* maps[4] = {t->lk_fmap, t->lk_bmap, t->hk_fmap, t->hk_bmap};
+      for (int i = 0; i < 4; ++i)
+        out[i] = maps[i] ? (double)((size_t)nC * U * (i % 2 == 0 ? map_pre(WF, kl, WF) : map_pre(WB, 0, WB))) : 0.0;
       return DNSMI_OK;
     }
     default: return c->fail(DNSMI_EINVAL, "unknown table");
   }
-  if (n != (long long)v->size()) return c->fail(DNSMI_EINVAL, "table size mismatch");
-  std::copy(v->begin(), v->end(), out);
-  return DNSMI_OK;
 }
 
 }  // namespace dnsmi

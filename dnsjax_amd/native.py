@@ -40,6 +40,7 @@ class NativeContext:
             correct_divergence=int(p.correct_divergence), tile_cols=p.tile_cols,
             variants=p.variants, plain_mixed=int(p.plain_mixed),
             recenter_far=p.recenter_far or 0.0, recenter_near=p.recenter_near,
+            setup_on_host=int(p.setup_on_host),
         )
         handle = ctypes.c_void_p()
         torch.cuda.current_device()  # make the device current before any HIP call

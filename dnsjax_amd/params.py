@@ -70,6 +70,11 @@ class Params:
     # and off for periodic ones (whose floor stays orders below it).
     recenter_far: float | None = -1.0
     recenter_near: float = 5e5
+    # Wall-bounded flows: build the influence-matrix tables (band LU,
+    # homogeneous solutions, M^-1, chunk maps) on the host instead of the
+    # GPU (the two agree bit for bit; the GPU set-up falls back to the host
+    # where a band does not fit its LDS).
+    setup_on_host: bool = False
 
     def __post_init__(self) -> None:
         if self.system not in PERIODIC_SYSTEMS + WALLED_SYSTEMS:

@@ -90,6 +90,13 @@ typedef struct dnsmi_config {
                               its first pass that starts from an error of at
                               most recenter_near times the tolerance; 0: never */
   double recenter_near;
+  int setup_on_host;       /* walls: 1 builds the influence-matrix tables
+                              (band LU, homogeneous solutions, M^-1, chunk
+                              maps) on the host; 0 (default) on the GPU, with
+                              the same operations in the same order (the two
+                              agree bit for bit), falling back to the host
+                              where a class's band does not fit 64 KiB of LDS
+                              (geometries/cartesian.py:575-801) */
 } dnsmi_config;
 
 typedef struct dnsmi_ctx dnsmi_ctx;
@@ -154,7 +161,12 @@ const char* dnsmi_tuning(const dnsmi_ctx* ctx);
  * keeps these as module attributes, flows/plane_couette.py:139-219):
  * which = 0 Lk_ab (Nkz,Nkx,3p+1,Ny), 1 Hk_ab, 2 v1 (Nkz,Nkx,Ny), 3 q1,
  * 4 M_inv (Nkz,Nkx,2,2), 5 y grid (Ny), 6 Lk_piv, 7 Hk_piv (0-based pivots
- * as doubles).  Synchronous, host memory only. */
+ * as doubles); as stored on the device: 8 .. 11 the packed chunk maps of the
+ * L_k forward, L_k back, H_k forward and H_k back solves, 12 the per-class
+ * homogeneous solutions ([class][y][v1, v2, q1, q2]), 13 the word counts of
+ * tables 8 .. 11 (0 where a table is absent) and the number of k^2 classes
+ * (5 values).  Read back from the device tables; synchronous, host memory
+ * only. */
 int dnsmi_walls_table(dnsmi_ctx* ctx, int which, double* out, long long n);
 
 /* One full time step with the corrector loop decided on the device

@@ -77,6 +77,37 @@ def test_setup_tables_match_oracle(case):
     assert rel(minv, of.imm["M_inv"]) < 1e-12
 
 
+SETUP_SHAPES = [
+    dict(nx=8, ny=17, nz=8, fd_order=4, re=400.0, dt=0.01),
+    dict(nx=16, ny=33, nz=8, fd_order=6, re=400.0, dt=0.01),
+    dict(nx=16, ny=24, nz=16, fd_order=2, re=100.0, dt=0.02),
+    dict(nx=16, ny=40, nz=8, fd_order=8, re=1000.0, dt=0.005, implicitness=0.3),
+    dict(nx=64, ny=129, nz=64, fd_order=4, re=400.0, dt=0.01),
+]
+
+
+@pytest.mark.parametrize("shape", SETUP_SHAPES, ids=lambda s: f"{s['nx']}x{s['ny']}x{s['nz']}_p{s['fd_order']}")
+def test_gpu_setup_equals_host_setup(shape):
+    """The influence-matrix tables built on the GPU (band LU, homogeneous
+    solutions, M^-1, chunk maps) equal the host set-up's bit for bit."""
+    g, h = make(None, **shape), make(None, setup_on_host=True, **shape)
+    assert "imm_setup=gpu" in g._native.tuning and "imm_setup=host" in h._native.tuning
+    G, H = g._native, h._native
+    ny, p = shape["ny"], shape["fd_order"]
+    M, rows = (shape["nz"] - 1) * (shape["nx"] // 2), 3 * p + 1
+    for which, shp in ((0, (M, rows, ny)), (1, (M, rows, ny)), (2, (M, ny)), (3, (M, ny)), (4, (M, 4)),
+                       (6, (M, ny)), (7, (M, ny))):
+        assert np.array_equal(G.walls_table(which, shp), H.walls_table(which, shp)), which
+    sizes = G.walls_table(13, (5,))
+    assert np.array_equal(sizes, H.walls_table(13, (5,)))
+    U = int(sizes[4])
+    assert np.array_equal(G.walls_table(12, (U * ny * 4,)), H.walls_table(12, (U * ny * 4,)))
+    for i, which in enumerate((8, 9, 10, 11)):
+        n = int(sizes[i])
+        assert n > 0
+        assert np.array_equal(G.walls_table(which, (n,)), H.walls_table(which, (n,))), which
+
+
 @pytest.mark.parametrize("case", list(CASES))
 @pytest.mark.parametrize("variants", [None, 0, 16, 8], ids=["autotuned", "chunked", "fused", "sequential"])
 def test_golden_rhs_and_imm_step(case, variants):

@@ -43,6 +43,28 @@ def rel(a, b):
     return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
 
 
+def test_gpu_setup_equals_host_setup_at_config_shape():
+    """At 128 x 256 x 128, p = 6 (2765 k^2 classes), the influence-matrix
+    tables built on the GPU equal the host set-up's bit for bit, and the GPU
+    builds them faster."""
+    g, h = make(tile_cols=16, variants=0), make(tile_cols=16, variants=0, setup_on_host=True)
+    G, H = g._native, h._native
+    assert "imm_setup=gpu" in G.tuning and "imm_setup=host" in H.tuning
+    M, rows = (NZ - 1) * (NX // 2), 3 * P + 1
+    for which, shp in ((0, (M, rows, NY)), (1, (M, rows, NY)), (4, (M, 4)), (6, (M, NY)), (7, (M, NY))):
+        assert np.array_equal(G.walls_table(which, shp), H.walls_table(which, shp)), which
+    sizes = G.walls_table(13, (5,))
+    assert np.array_equal(sizes, H.walls_table(13, (5,)))
+    U = int(sizes[4])
+    assert np.array_equal(G.walls_table(12, (U * NY * 4,)), H.walls_table(12, (U * NY * 4,)))
+    for i, which in enumerate((8, 9, 10, 11)):
+        n = int(sizes[i])
+        assert n > 0 and np.array_equal(G.walls_table(which, (n,)), H.walls_table(which, (n,))), which
+    ms = lambda t: float(t.split("imm_setup=")[1].split("(")[1].split(" ms")[0])  # noqa: E731
+    print(f"set-up: gpu {ms(G.tuning):.1f} ms, host {ms(H.tuning):.1f} ms", flush=True)
+    assert ms(G.tuning) < ms(H.tuning)
+
+
 def test_band_solve_forms_agree_with_wall_invariants():
     from dnsjax_amd.fields import wall_field
 
