@@ -566,12 +566,16 @@ def ensemble_form(steps: int, n_streams: int = 8) -> dict:
     against the same 64 trajectories run one after another with the eager
     driver.  Aggregate time-steps/s over all members.
 
-    Every captured step launches a fixed number of corrector slots (skipped
-    slots still launch their kernels, which exit at once).  Each member takes
-    one slot more than the largest iteration count of its own sequential
-    run; the device counters flag any step that would need more
-    (`unconverged`), and every member's counts must equal its sequential
-    run's."""
+    Two captured forms: every member's step a branch of its own (one launch
+    per kernel and member), and batched (the 16 members of a size advance by
+    one launch per kernel over all of them).  The members of a size run the
+    kernel variants the autotuner chose for its first member.  Every
+    captured step launches a fixed number of corrector slots (skipped slots
+    still launch their kernels, which exit at once): each member takes one
+    slot more than the largest iteration count of its own sequential run
+    (batched: the largest of its size); the device counters flag any step
+    that would need more (`unconverged`), and every member's counts must
+    equal its sequential run's."""
     import numpy as np
     import torch
 
@@ -583,9 +587,13 @@ def ensemble_form(steps: int, n_streams: int = 8) -> dict:
     sizes = [32] * 16 + [48] * 16 + [64] * 16 + [96] * 16
     res = np.random.default_rng(3).uniform(20.0, 80.0, len(sizes))
     t0 = time.perf_counter()
-    flows = [TriplyPeriodicFlow(kolmogorov_2pi(n, float(r), 0.01, corrector_tolerance=TOL,
-                                               max_corrector_iterations=10, precision="mixed"))
-             for n, r in zip(sizes, res)]
+    flows, pinned = [], {}
+    for n, r in zip(sizes, res):
+        kw = {} if n not in pinned else dict(zip(("tile_cols", "variants"), pinned[n]))
+        f = TriplyPeriodicFlow(kolmogorov_2pi(n, float(r), 0.01, corrector_tolerance=TOL,
+                                              max_corrector_iterations=10, precision="mixed", **kw))
+        pinned.setdefault(n, f._native.variants())
+        flows.append(f)
     states = [f.init_state(seed=300 + i, rms=0.05) for i, f in enumerate(flows)]
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t0
@@ -596,22 +604,33 @@ def ensemble_form(steps: int, n_streams: int = 8) -> dict:
     torch.cuda.synchronize()
     t_seq = time.perf_counter() - t0
     slots = [min(max(cs) + 1, 10) for cs in seq_counts]
-    ens = Ensemble(flows, states, n_streams=n_streams, slots=slots, capacity=steps)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ens.step(steps)
-    torch.cuda.synchronize()
-    t_ens = time.perf_counter() - t0
-    same = ens.counts() == seq_counts
-    st = ens.step_stats()
+    out = {}
+    for label, batched in (("branches", False), ("batched", True)):
+        ens = Ensemble(flows, states, n_streams=n_streams, slots=slots, capacity=steps, batched=batched)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ens.step(steps)
+        torch.cuda.synchronize()
+        t_ens = time.perf_counter() - t0
+        st = ens.step_stats()
+        out[label] = {"value": round(m * steps / t_ens, 2), "speedup_vs_sequential": round(t_seq / t_ens, 3),
+                      "counts_match_sequential": ens.counts() == seq_counts,
+                      "unconverged": sum(x["unconverged"] for x in st),
+                      "slots_mean": round(sum(mb.slots for mb in ens.members) / m, 2)}
+        if batched:
+            out[label]["classes"] = len(ens.plans[0])
+            out[label]["launches_per_class_step"] = max(getattr(pl, "launches", 0) for pl in ens.plans[0])
+        del ens
+    b = out["batched"]
     return {"members": m, "sizes": "16 x {32,48,64,96}^3", "steps": steps, "n_streams": n_streams,
             "slots": {"min": min(slots), "max": max(slots), "mean": round(sum(slots) / m, 2)},
-            "value": round(m * steps / t_ens, 2), "unit": "member time-steps/s",
+            "value": b["value"], "unit": "member time-steps/s", "form": "batched",
             "sequential_value": round(m * steps / t_seq, 2),
-            "speedup_vs_sequential": round(t_seq / t_ens, 3),
-            "counts_match_sequential": same,
-            "unconverged": sum(x["unconverged"] for x in st), "setup_s": round(t_setup, 2),
-            "note": "the graph timing includes the first launch of each of its two graphs"}
+            "speedup_vs_sequential": b["speedup_vs_sequential"],
+            "counts_match_sequential": b["counts_match_sequential"] and out["branches"]["counts_match_sequential"],
+            "unconverged": b["unconverged"] + out["branches"]["unconverged"],
+            "forms": out, "setup_s": round(t_setup, 2),
+            "note": "each graph timing includes the first launch of each of its two graphs"}
 
 
 def profile_workload(steps: int) -> None:

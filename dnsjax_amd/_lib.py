@@ -109,6 +109,12 @@ SIGNATURES = {
     "dnsmi_step_device_guess": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I, ctypes.c_float, ctypes.c_float,
                                      ctypes.c_float, _I, _I, _I, _P, _I, ctypes.c_double, _I, ctypes.c_double,
                                      ctypes.c_double, _P]),
+    "dnsmi_variants": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    "dnsmi_batch_plan": (_I, [_P, _I, _P, _P, _P, _P, _I, _I, _P, _I, ctypes.c_double, _I, ctypes.c_double,
+                              ctypes.c_double, ctypes.POINTER(_P)]),
+    "dnsmi_batch_enqueue": (_I, [_P, _P]),
+    "dnsmi_batch_launches": (_I, [_P]),
+    "dnsmi_batch_destroy": (None, [_P]),
 }
 
 KERNEL_IDS = 12

@@ -518,6 +518,18 @@ int dnsmi_step_device_guess(dnsmi_ctx* ctx, const double* state, double* next, d
 
 const char* dnsmi_tuning(const dnsmi_ctx* ctx) { return ctx ? ctx->impl.tuning.c_str() : ""; }
 
+int dnsmi_variants(const dnsmi_ctx* ctx, int* tile_cols, int* variants) {
+  if (!ctx || !tile_cols || !variants) {
+    dnsmi::g_create_error = "variants: bad arguments";
+    return DNSMI_EINVAL;
+  }
+  const Ctx* c = &ctx->impl;
+  *tile_cols = c->tile_cols;
+  *variants = (c->yinv_seq ? 1 : 0) | (c->zinv_seq ? 2 : 0) | (c->xpass_split ? 4 : 0) | (c->sweep_seq ? 8 : 0) |
+              (c->imm_fused ? 16 : 0);
+  return DNSMI_OK;
+}
+
 int dnsmi_walls_table(dnsmi_ctx* ctx, int which, double* out, long long n) {
   DNSMI_CHECK_CTX(ctx)
   DNSMI_REQUIRE(out && n >= 0, "walls_table: bad output");

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -59,6 +60,18 @@ enum EvalMode {
   EV_INC = 2      // N(u) = n_ref + P[dN]: only the increment
                   // dN = (U + u_ref) x w_d + d x (W_ref + w_d), d = u - u_ref,
                   // goes through the FP32 transforms
+};
+
+// Launches recorded instead of issued (batched ensembles): the batched twin
+// of each kernel (one member per grid.z, its argument struct read from a
+// device table) with the member's grid, block and argument struct.
+struct LaunchRec {
+  const void* kb;
+  dim3 grid, block;
+  std::vector<unsigned char> args;
+};
+struct Recorder {
+  std::vector<LaunchRec> list;
 };
 
 struct Ctx {
@@ -123,6 +136,9 @@ struct Ctx {
   bool imm_fused = false;  // wall-bounded, chunk maps: the two-launch fused IMM
   std::string tuning;
 
+  // while set, launches are recorded there instead of issued
+  Recorder* rec = nullptr;
+
   // per-kernel timing (off unless dnsmi_profile_begin was called)
   bool prof = false;
   std::vector<ProfRec> prof_recs;
@@ -165,6 +181,20 @@ struct ProfScope {
     }
   }
 };
+
+// One launch of kernel k with argument struct a, or, while the context
+// records, an entry naming its batched twin kb.
+template <typename A>
+inline void launch(Ctx* c, void (*k)(A), void (*kb)(const A*), dim3 grid, dim3 block, hipStream_t s,
+                   const A& a) {
+  if (c->rec) {
+    LaunchRec r{reinterpret_cast<const void*>(kb), grid, block, std::vector<unsigned char>(sizeof(A))};
+    std::memcpy(r.args.data(), &a, sizeof(A));
+    c->rec->list.push_back(std::move(r));
+    return;
+  }
+  hipLaunchKernelGGL(k, grid, block, 0, s, a);
+}
 
 // Shared small kernels (periodic.hip).
 int finalize_norm(Ctx* c, int nblocks, double* err_dev, hipStream_t s);

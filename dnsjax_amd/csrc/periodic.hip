@@ -53,6 +53,53 @@ struct StepArgs {
   double dt, c, re;
 };
 
+// Argument structs of the FFT-pass kernels: each kernel is launched with
+// one (K(args)) or, batched over the members of an ensemble, reads member
+// blockIdx.z's from a device table (K_b(table)); complex arrays are passed
+// untyped and cast to the kernel's precision.
+struct YInvArgs {
+  const void* u;
+  void* w1;
+  const void* tw;
+  const double* ky;
+  int ny, Nz1, Nx1;
+  const int* act;
+  const void* uref;
+  void* ucopy;
+};
+struct ZInvArgs {
+  const void* w1;
+  void* w2;
+  const void* tw;
+  const double* kz;
+  int nz, Py, Nx1;
+  const int* act;
+};
+struct XArgs {
+  const void* w2;
+  void* w3;
+  const void* tw;
+  const double* kx;
+  const double *base, *cbase, *nbase;
+  int nx, Py, Pz, ybs;
+  const int* act;
+  float* xref;
+};
+struct ZFwdArgs {
+  const void* w3;
+  void* w4;
+  const void* tw;
+  int nz, Py, Nx1;
+  const int* act;
+};
+struct YFwdArgs {
+  const void* w4;
+  const void* tw;
+  const double *kx, *ky, *kz;
+  int ny, Nz1, Nx1;
+  StepArgs s;
+};
+
 
 template <typename T>
 __device__ __forceinline__ cx<T> to_t(cx<double> d) {
@@ -63,8 +110,7 @@ __device__ __forceinline__ cx<T> to_t(cx<double> d) {
 // EM (EvalMode): EV_RECORD also copies u to ucopy (u_ref); EV_INC transforms
 // d = u - uref, formed in FP64 before the rounding to T.
 template <typename T, int P, int TC, bool PADY, int EM>
-__global__ void __launch_bounds__(TC * ColPlan<P>::threads)
-    k_y_inverse(const cx<double>* __restrict__ u, cx<T>* __restrict__ w1,
+__device__ __forceinline__ void y_inverse_body(const cx<double>* __restrict__ u, cx<T>* __restrict__ w1,
                 const cx<T>* __restrict__ tw, const double* __restrict__ ky, int ny, int Nz1,
                 int Nx1, const int* __restrict__ act, const cx<double>* __restrict__ uref,
                 cx<double>* __restrict__ ucopy) {
@@ -136,11 +182,20 @@ __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
     }
   }
 }
+template <typename T, int P, int TC, bool PADY, int EM>
+__global__ void __launch_bounds__(TC * ColPlan<P>::threads) k_y_inverse(YInvArgs a) {
+  y_inverse_body<T, P, TC, PADY, EM>(static_cast<const cx<double>*>(a.u), static_cast<cx<T>*>(a.w1), static_cast<const cx<T>*>(a.tw), a.ky, a.ny, a.Nz1, a.Nx1, a.act, static_cast<const cx<double>*>(a.uref), static_cast<cx<double>*>(a.ucopy));
+}
+template <typename T, int P, int TC, bool PADY, int EM>
+__global__ void __launch_bounds__(TC * ColPlan<P>::threads) k_y_inverse_b(const YInvArgs* __restrict__ t) {
+  const YInvArgs a = t[blockIdx.z];
+  y_inverse_body<T, P, TC, PADY, EM>(static_cast<const cx<double>*>(a.u), static_cast<cx<T>*>(a.w1), static_cast<const cx<T>*>(a.tw), a.ky, a.ny, a.Nz1, a.Nx1, a.act, static_cast<const cx<double>*>(a.uref), static_cast<cx<double>*>(a.ucopy));
+}
+
 
 // ------------------------------------------------------------------ K2
 template <typename T, int P, int TC>
-__global__ void __launch_bounds__(TC * ColPlan<P>::threads)
-    k_z_inverse(const cx<T>* __restrict__ w1, cx<T>* __restrict__ w2, const cx<T>* __restrict__ tw,
+__device__ __forceinline__ void z_inverse_body(const cx<T>* __restrict__ w1, cx<T>* __restrict__ w2, const cx<T>* __restrict__ tw,
                 const double* __restrict__ kz_tab, int nz, int Py, int Nx1, const int* __restrict__ act) {
   if (act && *act == 0) return;
   constexpr int N1 = ColPlan<P>::N1, N2 = ColPlan<P>::N2;
@@ -217,13 +272,22 @@ __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
     }
   }
 }
+template <typename T, int P, int TC>
+__global__ void __launch_bounds__(TC * ColPlan<P>::threads) k_z_inverse(ZInvArgs a) {
+  z_inverse_body<T, P, TC>(static_cast<const cx<T>*>(a.w1), static_cast<cx<T>*>(a.w2), static_cast<const cx<T>*>(a.tw), a.kz, a.nz, a.Py, a.Nx1, a.act);
+}
+template <typename T, int P, int TC>
+__global__ void __launch_bounds__(TC * ColPlan<P>::threads) k_z_inverse_b(const ZInvArgs* __restrict__ t) {
+  const ZInvArgs a = t[blockIdx.z];
+  z_inverse_body<T, P, TC>(static_cast<const cx<T>*>(a.w1), static_cast<cx<T>*>(a.w2), static_cast<const cx<T>*>(a.tw), a.kz, a.nz, a.Py, a.Nx1, a.act);
+}
+
 
 // K1 / K2, sequential form: one transform in flight per pass (fewer live
 // registers, higher occupancy); an input used by two outputs is read twice,
 // the second time from L2.  Same outputs as the paired forms above.
 template <typename T, int P, int TC, bool PADY, int EM>
-__global__ void __launch_bounds__(TC * ColPlan<P>::threads)
-    k_y_inverse_seq(const cx<double>* __restrict__ u, cx<T>* __restrict__ w1,
+__device__ __forceinline__ void y_inverse_seq_body(const cx<double>* __restrict__ u, cx<T>* __restrict__ w1,
                     const cx<T>* __restrict__ tw, const double* __restrict__ ky, int ny, int Nz1,
                     int Nx1, const int* __restrict__ act, const cx<double>* __restrict__ uref,
                     cx<double>* __restrict__ ucopy) {
@@ -286,10 +350,19 @@ __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
     __syncthreads();
   }
 }
+template <typename T, int P, int TC, bool PADY, int EM>
+__global__ void __launch_bounds__(TC * ColPlan<P>::threads) k_y_inverse_seq(YInvArgs a) {
+  y_inverse_seq_body<T, P, TC, PADY, EM>(static_cast<const cx<double>*>(a.u), static_cast<cx<T>*>(a.w1), static_cast<const cx<T>*>(a.tw), a.ky, a.ny, a.Nz1, a.Nx1, a.act, static_cast<const cx<double>*>(a.uref), static_cast<cx<double>*>(a.ucopy));
+}
+template <typename T, int P, int TC, bool PADY, int EM>
+__global__ void __launch_bounds__(TC * ColPlan<P>::threads) k_y_inverse_seq_b(const YInvArgs* __restrict__ t) {
+  const YInvArgs a = t[blockIdx.z];
+  y_inverse_seq_body<T, P, TC, PADY, EM>(static_cast<const cx<double>*>(a.u), static_cast<cx<T>*>(a.w1), static_cast<const cx<T>*>(a.tw), a.ky, a.ny, a.Nz1, a.Nx1, a.act, static_cast<const cx<double>*>(a.uref), static_cast<cx<double>*>(a.ucopy));
+}
+
 
 template <typename T, int P, int TC>
-__global__ void __launch_bounds__(TC * ColPlan<P>::threads)
-    k_z_inverse_seq(const cx<T>* __restrict__ w1, cx<T>* __restrict__ w2,
+__device__ __forceinline__ void z_inverse_seq_body(const cx<T>* __restrict__ w1, cx<T>* __restrict__ w2,
                     const cx<T>* __restrict__ tw, const double* __restrict__ kz_tab, int nz, int Py,
                     int Nx1, const int* __restrict__ act) {
   if (act && *act == 0) return;
@@ -356,6 +429,16 @@ __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
     __syncthreads();
   }
 }
+template <typename T, int P, int TC>
+__global__ void __launch_bounds__(TC * ColPlan<P>::threads) k_z_inverse_seq(ZInvArgs a) {
+  z_inverse_seq_body<T, P, TC>(static_cast<const cx<T>*>(a.w1), static_cast<cx<T>*>(a.w2), static_cast<const cx<T>*>(a.tw), a.kz, a.nz, a.Py, a.Nx1, a.act);
+}
+template <typename T, int P, int TC>
+__global__ void __launch_bounds__(TC * ColPlan<P>::threads) k_z_inverse_seq_b(const ZInvArgs* __restrict__ t) {
+  const ZInvArgs a = t[blockIdx.z];
+  z_inverse_seq_body<T, P, TC>(static_cast<const cx<T>*>(a.w1), static_cast<cx<T>*>(a.w2), static_cast<const cx<T>*>(a.tw), a.kz, a.nz, a.Py, a.Nx1, a.act);
+}
+
 
 // ------------------------------------------------------------------ K3
 // One row (fixed y, z) per N1 threads; R rows per workgroup.  Inverse plan
@@ -388,8 +471,7 @@ struct XWaves {
 // (no base flow added) and forms dN = U x w_d + d x (W + w_d) with U, W
 // read back from xref: every FP32 product is then of increment size.
 template <typename T, int P, int R, int EM>
-__global__ void __launch_bounds__(R * Plan<P>::N1, (EM == EV_INC ? 3 : XWaves<T, P>::value))
-    k_x_nonlinear(const cx<T>* __restrict__ w2, cx<T>* __restrict__ w3,
+__device__ __forceinline__ void x_nonlinear_body(const cx<T>* __restrict__ w2, cx<T>* __restrict__ w3,
                   const cx<T>* __restrict__ tw, const double* __restrict__ kx_tab,
                   const double* __restrict__ base, const double* __restrict__ cbase,
                   const double* __restrict__ /* nbase: U x curl U, implied */, int nx, int Py,
@@ -693,6 +775,16 @@ __global__ void __launch_bounds__(R * Plan<P>::N1, (EM == EV_INC ? 3 : XWaves<T,
     }
   }
 }
+template <typename T, int P, int R, int EM>
+__global__ void __launch_bounds__(R * Plan<P>::N1, (EM == EV_INC ? 3 : XWaves<T, P>::value)) k_x_nonlinear(XArgs a) {
+  x_nonlinear_body<T, P, R, EM>(static_cast<const cx<T>*>(a.w2), static_cast<cx<T>*>(a.w3), static_cast<const cx<T>*>(a.tw), a.kx, a.base, a.cbase, a.nbase, a.nx, a.Py, a.Pz, a.ybs, a.act, a.xref);
+}
+template <typename T, int P, int R, int EM>
+__global__ void __launch_bounds__(R * Plan<P>::N1, (EM == EV_INC ? 3 : XWaves<T, P>::value)) k_x_nonlinear_b(const XArgs* __restrict__ t) {
+  const XArgs a = t[blockIdx.z];
+  x_nonlinear_body<T, P, R, EM>(static_cast<const cx<T>*>(a.w2), static_cast<cx<T>*>(a.w3), static_cast<const cx<T>*>(a.tw), a.kx, a.base, a.cbase, a.nbase, a.nx, a.Py, a.Pz, a.ybs, a.act, a.xref);
+}
+
 
 // K3, split form (N2 even): 2 N1 threads per row.  The inverse step B
 // (a DFT-N2 per k1) is split by decimation in frequency between the thread
@@ -702,8 +794,7 @@ __global__ void __launch_bounds__(R * Plan<P>::N1, (EM == EV_INC ? 3 : XWaves<T,
 // distribution of the forward plan (N2/2, 2 N1); its step B (DFT-2N1) is
 // split the same way.  Both threads of a pair read the whole LDS column.
 template <typename T, int P, int R>
-__global__ void __launch_bounds__(R * 2 * Plan<P>::N1)
-    k_x_nonlinear_split(const cx<T>* __restrict__ w2, cx<T>* __restrict__ w3,
+__device__ __forceinline__ void x_nonlinear_split_body(const cx<T>* __restrict__ w2, cx<T>* __restrict__ w3,
                         const cx<T>* __restrict__ tw, const double* __restrict__ kx_tab,
                         const double* __restrict__ base, const double* __restrict__ cbase,
                         const double* __restrict__ nbase, int nx, int Py, int Pz,
@@ -848,11 +939,20 @@ __global__ void __launch_bounds__(R * 2 * Plan<P>::N1)
     __syncthreads();
   }
 }
+template <typename T, int P, int R>
+__global__ void __launch_bounds__(R * 2 * Plan<P>::N1) k_x_nonlinear_split(XArgs a) {
+  x_nonlinear_split_body<T, P, R>(static_cast<const cx<T>*>(a.w2), static_cast<cx<T>*>(a.w3), static_cast<const cx<T>*>(a.tw), a.kx, a.base, a.cbase, a.nbase, a.nx, a.Py, a.Pz, a.ybs, a.act);
+}
+template <typename T, int P, int R>
+__global__ void __launch_bounds__(R * 2 * Plan<P>::N1) k_x_nonlinear_split_b(const XArgs* __restrict__ t) {
+  const XArgs a = t[blockIdx.z];
+  x_nonlinear_split_body<T, P, R>(static_cast<const cx<T>*>(a.w2), static_cast<cx<T>*>(a.w3), static_cast<const cx<T>*>(a.tw), a.kx, a.base, a.cbase, a.nbase, a.nx, a.Py, a.Pz, a.ybs, a.act);
+}
+
 
 // ------------------------------------------------------------------ K4
 template <typename T, int P, int TC>
-__global__ void __launch_bounds__(TC * ColPlan<P>::threads)
-    k_z_forward(const cx<T>* __restrict__ w3, cx<T>* __restrict__ w4, const cx<T>* __restrict__ tw,
+__device__ __forceinline__ void z_forward_body(const cx<T>* __restrict__ w3, cx<T>* __restrict__ w4, const cx<T>* __restrict__ tw,
                 int nz, int Py, int Nx1, const int* __restrict__ act) {
   if (act && *act == 0) return;
   constexpr int N1 = ColPlan<P>::N1, N2 = ColPlan<P>::N2;
@@ -894,11 +994,20 @@ __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
     __syncthreads();
   }
 }
+template <typename T, int P, int TC>
+__global__ void __launch_bounds__(TC * ColPlan<P>::threads) k_z_forward(ZFwdArgs a) {
+  z_forward_body<T, P, TC>(static_cast<const cx<T>*>(a.w3), static_cast<cx<T>*>(a.w4), static_cast<const cx<T>*>(a.tw), a.nz, a.Py, a.Nx1, a.act);
+}
+template <typename T, int P, int TC>
+__global__ void __launch_bounds__(TC * ColPlan<P>::threads) k_z_forward_b(const ZFwdArgs* __restrict__ t) {
+  const ZFwdArgs a = t[blockIdx.z];
+  z_forward_body<T, P, TC>(static_cast<const cx<T>*>(a.w3), static_cast<cx<T>*>(a.w4), static_cast<const cx<T>*>(a.tw), a.nz, a.Py, a.Nx1, a.act);
+}
+
 
 // ------------------------------------------------------------------ K5
 template <typename T, int P, int TC, int MODE, bool PADY>
-__global__ void __launch_bounds__(TC * ColPlan<P>::threads)
-    k_y_forward(const cx<T>* __restrict__ w4, const cx<T>* __restrict__ tw,
+__device__ __forceinline__ void y_forward_body(const cx<T>* __restrict__ w4, const cx<T>* __restrict__ tw,
                 const double* __restrict__ kx_tab, const double* __restrict__ ky_tab,
                 const double* __restrict__ kz_tab, int ny, int Nz1, int Nx1, StepArgs args) {
   if (args.act && *args.act == 0) return;
@@ -1018,6 +1127,16 @@ __global__ void __launch_bounds__(TC * ColPlan<P>::threads)
     if (threadIdx.x == 0) args.partials[blockIdx.x] = s;
   }
 }
+template <typename T, int P, int TC, int MODE, bool PADY>
+__global__ void __launch_bounds__(TC * ColPlan<P>::threads) k_y_forward(YFwdArgs a) {
+  y_forward_body<T, P, TC, MODE, PADY>(static_cast<const cx<T>*>(a.w4), static_cast<const cx<T>*>(a.tw), a.kx, a.ky, a.kz, a.ny, a.Nz1, a.Nx1, a.s);
+}
+template <typename T, int P, int TC, int MODE, bool PADY>
+__global__ void __launch_bounds__(TC * ColPlan<P>::threads) k_y_forward_b(const YFwdArgs* __restrict__ t) {
+  const YFwdArgs a = t[blockIdx.z];
+  y_forward_body<T, P, TC, MODE, PADY>(static_cast<const cx<T>*>(a.w4), static_cast<const cx<T>*>(a.tw), a.kx, a.ky, a.kz, a.ny, a.Nz1, a.Nx1, a.s);
+}
+
 
 // ------------------------------------------------------------------ pointwise
 struct PointArgs {
@@ -1034,9 +1153,25 @@ struct PointArgs {
 
 enum { PW_PREDICT = 0, PW_CORRECT = 1, PW_NORM = 2, PW_DIVFREE = 3, PW_DIVFREE_NORM = 4 };
 
+struct PwArgs {
+  PointArgs p;
+  const double *kx, *ky, *kz;
+};
+struct ZmArgs {
+  double* u;
+  long long n;
+};
+struct FinArgs {
+  const double* partials;
+  int n;
+  double* err;
+  const int* act;
+  StepCtl* ctl;
+  CtlRule rule;
+};
+
 template <int OP>
-__global__ void __launch_bounds__(256)
-    k_pointwise(PointArgs p, const double* __restrict__ kx_tab, const double* __restrict__ ky_tab,
+__device__ __forceinline__ void pointwise_body(PointArgs p, const double* __restrict__ kx_tab, const double* __restrict__ ky_tab,
                 const double* __restrict__ kz_tab) {
   __shared__ double red[16];
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1098,17 +1233,35 @@ __global__ void __launch_bounds__(256)
     if (threadIdx.x == 0) p.partials[blockIdx.x] = s;
   }
 }
+template <int OP>
+__global__ void __launch_bounds__(256) k_pointwise(PwArgs a) {
+  pointwise_body<OP>(a.p, a.kx, a.ky, a.kz);
+}
+template <int OP>
+__global__ void __launch_bounds__(256) k_pointwise_b(const PwArgs* __restrict__ t) {
+  const PwArgs a = t[blockIdx.z];
+  pointwise_body<OP>(a.p, a.kx, a.ky, a.kz);
+}
 
-__global__ void __launch_bounds__(64) k_zero_mean(double* __restrict__ u, long long n) {
+
+__device__ __forceinline__ void zero_mean_body(double* __restrict__ u, long long n) {
   if (threadIdx.x < 6) u[2 * (threadIdx.x / 2) * n + (threadIdx.x & 1)] = 0.0;
 }
+__global__ void __launch_bounds__(64) k_zero_mean(ZmArgs a) {
+  zero_mean_body(a.u, a.n);
+}
+__global__ void __launch_bounds__(64) k_zero_mean_b(const ZmArgs* __restrict__ t) {
+  const ZmArgs a = t[blockIdx.z];
+  zero_mean_body(a.u, a.n);
+}
+
 
 // The norm from its per-block partials, summed in a fixed order.  With a
 // control block (the corrector slots of a device step, whose err it writes)
 // it then applies the corrector rule, one launch instead of two.  Every
 // thread reads *act before the reduction's barriers, so the rule's update of
 // `active` (the same word) cannot race with that read.
-__global__ void __launch_bounds__(256) k_finalize_norm(const double* __restrict__ partials, int n,
+__device__ __forceinline__ void finalize_norm_body(const double* __restrict__ partials, int n,
                                                        double* err, const int* act, StepCtl* ctl,
                                                        CtlRule rule) {
   if (act && *act == 0) return;
@@ -1122,12 +1275,20 @@ __global__ void __launch_bounds__(256) k_finalize_norm(const double* __restrict_
     if (ctl) ctl_after_iteration(ctl, rule, e);
   }
 }
+__global__ void __launch_bounds__(256) k_finalize_norm(FinArgs a) {
+  finalize_norm_body(a.partials, a.n, a.err, a.act, a.ctl, a.rule);
+}
+__global__ void __launch_bounds__(256) k_finalize_norm_b(const FinArgs* __restrict__ t) {
+  const FinArgs a = t[blockIdx.z];
+  finalize_norm_body(a.partials, a.n, a.err, a.act, a.ctl, a.rule);
+}
+
 
 int finalize_norm(Ctx* c, int nblocks, double* err_dev, hipStream_t s) {
   ProfScope ps(c, KID_FINALIZE, s);
   StepCtl* ctl = c->fuse_ctl ? reinterpret_cast<StepCtl*>(c->ctl) : nullptr;
-  hipLaunchKernelGGL(k_finalize_norm, dim3(1), dim3(256), 0, s, c->partials, nblocks, err_dev, c->act,
-                     ctl, c->rule);
+  launch(c, &k_finalize_norm, &k_finalize_norm_b, dim3(1), dim3(256), s,
+         FinArgs{c->partials, nblocks, err_dev, c->act, ctl, c->rule});
   return check_launch(c, "finalize_norm");
 }
 
@@ -1170,20 +1331,15 @@ static void launch_y_inverse_p(Ctx* c, dim3 grid, const cx<double>* u, cx<T>* w1
                                hipStream_t s) {
   const dim3 blk(TC * ColPlan<P>::threads);
   const bool pady = c->Py != c->ny;  // y padded by the 3/2 rule
-  const cx<double>* uref = reinterpret_cast<const cx<double>*>(c->uref);
-  cx<double>* ucopy = reinterpret_cast<cx<double>*>(c->uref);
+  const YInvArgs a{u, w1, tw, c->ky, c->ny, c->Nz1, c->Nx1, c->act, c->uref, c->uref};
   if (c->yinv_seq && pady)
-    hipLaunchKernelGGL((k_y_inverse_seq<T, P, TC, true, EM>), grid, blk, 0, s, u, w1, tw, c->ky,
-                       c->ny, c->Nz1, c->Nx1, c->act, uref, ucopy);
+    launch(c, &k_y_inverse_seq<T, P, TC, true, EM>, &k_y_inverse_seq_b<T, P, TC, true, EM>, grid, blk, s, a);
   else if (c->yinv_seq)
-    hipLaunchKernelGGL((k_y_inverse_seq<T, P, TC, false, EM>), grid, blk, 0, s, u, w1, tw, c->ky,
-                       c->ny, c->Nz1, c->Nx1, c->act, uref, ucopy);
+    launch(c, &k_y_inverse_seq<T, P, TC, false, EM>, &k_y_inverse_seq_b<T, P, TC, false, EM>, grid, blk, s, a);
   else if (pady)
-    hipLaunchKernelGGL((k_y_inverse<T, P, TC, true, EM>), grid, blk, 0, s, u, w1, tw, c->ky, c->ny,
-                       c->Nz1, c->Nx1, c->act, uref, ucopy);
+    launch(c, &k_y_inverse<T, P, TC, true, EM>, &k_y_inverse_b<T, P, TC, true, EM>, grid, blk, s, a);
   else
-    hipLaunchKernelGGL((k_y_inverse<T, P, TC, false, EM>), grid, blk, 0, s, u, w1, tw, c->ky,
-                       c->ny, c->Nz1, c->Nx1, c->act, uref, ucopy);
+    launch(c, &k_y_inverse<T, P, TC, false, EM>, &k_y_inverse_b<T, P, TC, false, EM>, grid, blk, s, a);
 }
 
 template <typename T>
@@ -1230,12 +1386,13 @@ static int launch_z_inverse(Ctx* c, hipStream_t s) {
   case P:                                                                                       \
     with_tc(c, [&](auto tcv) {                                                                  \
       constexpr int TC = decltype(tcv)::value;                                                  \
+      const ZInvArgs a{b.A, b.B, b.twz, c->kz, c->nz, c->Py, c->Nx1, c->act};                   \
       if (c->zinv_seq)                                                                          \
-        hipLaunchKernelGGL((k_z_inverse_seq<T, P, TC>), grid, dim3(TC * ColPlan<P>::threads), 0, \
-                           s, b.A, b.B, b.twz, c->kz, c->nz, c->Py, c->Nx1, c->act);            \
+        launch(c, &k_z_inverse_seq<T, P, TC>, &k_z_inverse_seq_b<T, P, TC>, grid,               \
+               dim3(TC * ColPlan<P>::threads), s, a);                                           \
       else                                                                                      \
-        hipLaunchKernelGGL((k_z_inverse<T, P, TC>), grid, dim3(TC * ColPlan<P>::threads), 0, s,  \
-                           b.A, b.B, b.twz, c->kz, c->nz, c->Py, c->Nx1, c->act);               \
+        launch(c, &k_z_inverse<T, P, TC>, &k_z_inverse_b<T, P, TC>, grid,                       \
+               dim3(TC * ColPlan<P>::threads), s, a);                                           \
     });                                                                                         \
     break;
     DNSMI_FOR_EACH_P(X)
@@ -1261,23 +1418,21 @@ static void launch_x_len(Ctx* c, const Bufs<T>& b, const cx<T>* xin, cx<T>* xout
   const dim3 grid((nrows + R - 1) / R);
   bool split = false;
   if constexpr (XSplit<P>::value) split = c->xpass_split && em == EV_PLAIN;
+  const XArgs a{xin, xout, b.twx, c->kx, base, cbase, nbase, c->nx, ny, c->Pz, ny, c->act, c->xref};
   if (split) {
     if constexpr (XSplit<P>::value)
-      hipLaunchKernelGGL((k_x_nonlinear_split<T, P, R>), grid, dim3(R * XSplit<P>::tpr), 0, s,
-                         xin, xout, b.twx, c->kx, base, cbase, nbase, c->nx, ny, c->Pz, ny, c->act);
+      launch(c, &k_x_nonlinear_split<T, P, R>, &k_x_nonlinear_split_b<T, P, R>, grid,
+             dim3(R * XSplit<P>::tpr), s, a);
   } else {
     constexpr int R1 = kRows;
     const dim3 grid1((nrows + R1 - 1) / R1), blk(R1 * Plan<P>::N1);
     if (em == EV_PLAIN) {
-      hipLaunchKernelGGL((k_x_nonlinear<T, P, R1, EV_PLAIN>), grid1, blk, 0, s, xin, xout, b.twx,
-                         c->kx, base, cbase, nbase, c->nx, ny, c->Pz, ny, c->act, c->xref);
+      launch(c, &k_x_nonlinear<T, P, R1, EV_PLAIN>, &k_x_nonlinear_b<T, P, R1, EV_PLAIN>, grid1, blk, s, a);
     } else if constexpr (std::is_same<T, float>::value) {
       if (em == EV_RECORD)
-        hipLaunchKernelGGL((k_x_nonlinear<T, P, R1, EV_RECORD>), grid1, blk, 0, s, xin, xout, b.twx,
-                           c->kx, base, cbase, nbase, c->nx, ny, c->Pz, ny, c->act, c->xref);
+        launch(c, &k_x_nonlinear<T, P, R1, EV_RECORD>, &k_x_nonlinear_b<T, P, R1, EV_RECORD>, grid1, blk, s, a);
       else
-        hipLaunchKernelGGL((k_x_nonlinear<T, P, R1, EV_INC>), grid1, blk, 0, s, xin, xout, b.twx,
-                           c->kx, base, cbase, nbase, c->nx, ny, c->Pz, ny, c->act, c->xref);
+        launch(c, &k_x_nonlinear<T, P, R1, EV_INC>, &k_x_nonlinear_b<T, P, R1, EV_INC>, grid1, blk, s, a);
     }
   }
 }
@@ -1327,8 +1482,8 @@ int launch_z_forward_buf(Ctx* c, const cx<T>* in, cx<T>* out, int ny, int kid, h
   case P:                                                                                      \
     with_tc(c, [&](auto tcv) {                                                                 \
       constexpr int TC = decltype(tcv)::value;                                                 \
-      hipLaunchKernelGGL((k_z_forward<T, P, TC>), grid, dim3(TC * ColPlan<P>::threads), 0, s,  \
-                         in, out, b.twz, c->nz, ny, c->Nx1, c->act);                           \
+      launch(c, &k_z_forward<T, P, TC>, &k_z_forward_b<T, P, TC>, grid,                         \
+             dim3(TC * ColPlan<P>::threads), s, ZFwdArgs{in, out, b.twz, c->nz, ny, c->Nx1, c->act}); \
     });                                                                                        \
     break;
     DNSMI_FOR_EACH_P(X)
@@ -1362,14 +1517,13 @@ static int launch_y_forward(Ctx* c, const StepArgs& a, hipStream_t s, int* nbloc
   case P:                                                                                      \
     with_tc(c, [&](auto tcv) {                                                                 \
       constexpr int TC = decltype(tcv)::value;                                                 \
+      const YFwdArgs ya{b.B, b.twy, c->kx, c->ky, c->kz, c->ny, c->Nz1, c->Nx1, a};           \
       if (c->Py != c->ny)                                                                      \
-        hipLaunchKernelGGL((k_y_forward<T, P, TC, MODE, true>), dim3(nb),                     \
-                           dim3(TC * ColPlan<P>::threads), 0, s, b.B, b.twy, c->kx, c->ky,     \
-                           c->kz, c->ny, c->Nz1, c->Nx1, a);                                   \
+        launch(c, &k_y_forward<T, P, TC, MODE, true>, &k_y_forward_b<T, P, TC, MODE, true>,    \
+               dim3(nb), dim3(TC * ColPlan<P>::threads), s, ya);                               \
       else                                                                                     \
-        hipLaunchKernelGGL((k_y_forward<T, P, TC, MODE, false>), dim3(nb),                    \
-                           dim3(TC * ColPlan<P>::threads), 0, s, b.B, b.twy, c->kx, c->ky,     \
-                           c->kz, c->ny, c->Nz1, c->Nx1, a);                                   \
+        launch(c, &k_y_forward<T, P, TC, MODE, false>, &k_y_forward_b<T, P, TC, MODE, false>,  \
+               dim3(nb), dim3(TC * ColPlan<P>::threads), s, ya);                               \
     });                                                                                        \
     break;
     DNSMI_FOR_EACH_P(X)
@@ -1523,8 +1677,8 @@ int periodic_predict(Ctx* c, const double* state, const double* rhs, double* pre
   p.b = rhs;
   p.out = pred;
   ProfScope ps(c, KID_POINTWISE, s);
-  hipLaunchKernelGGL(k_pointwise<PW_PREDICT>, dim3(pw_blocks(p)), dim3(256), 0, s, p, c->kx,
-                     c->ky, c->kz);
+  launch(c, &k_pointwise<PW_PREDICT>, &k_pointwise_b<PW_PREDICT>, dim3(pw_blocks(p)), dim3(256), s,
+         PwArgs{p, c->kx, c->ky, c->kz});
   return check_launch(c, "predict");
 }
 
@@ -1537,8 +1691,8 @@ int periodic_correct(Ctx* c, const double* pred_in, const double* rhs_prev,
   p.out = pred_out;
   p.out2 = corr;
   ProfScope ps(c, KID_POINTWISE, s);
-  hipLaunchKernelGGL(k_pointwise<PW_CORRECT>, dim3(pw_blocks(p)), dim3(256), 0, s, p, c->kx,
-                     c->ky, c->kz);
+  launch(c, &k_pointwise<PW_CORRECT>, &k_pointwise_b<PW_CORRECT>, dim3(pw_blocks(p)), dim3(256), s,
+         PwArgs{p, c->kx, c->ky, c->kz});
   return check_launch(c, "correct");
 }
 
@@ -1548,7 +1702,7 @@ int periodic_norm(Ctx* c, const double* corr, double* err, hipStream_t s) {
   const int nb = pw_blocks(p);
   if (nb > c->max_partials) return c->fail(DNSMI_EINVAL, "norm: too many blocks");
   ProfScope ps(c, KID_POINTWISE, s);
-  hipLaunchKernelGGL(k_pointwise<PW_NORM>, dim3(nb), dim3(256), 0, s, p, c->kx, c->ky, c->kz);
+  launch(c, &k_pointwise<PW_NORM>, &k_pointwise_b<PW_NORM>, dim3(nb), dim3(256), s, PwArgs{p, c->kx, c->ky, c->kz});
   int rc = check_launch(c, "norm");
   if (rc) return rc;
   return finalize_norm(c, nb, err, s);
@@ -1564,20 +1718,21 @@ int periodic_correct_velocity(Ctx* c, double* state, double* err, hipStream_t s)
     // call can sit inside a captured graph)
     if (err) return c->fail(DNSMI_EINVAL, "measure_corrections needs correct_divergence");
     ProfScope ps(c, KID_POINTWISE, s);
-    hipLaunchKernelGGL(k_zero_mean, dim3(1), dim3(64), 0, s, state, p.n);
+    launch(c, &k_zero_mean, &k_zero_mean_b, dim3(1), dim3(64), s, ZmArgs{state, p.n});
     return check_launch(c, "zero_mean");
   }
   if (err) {
     if (nb > c->max_partials) return c->fail(DNSMI_EINVAL, "too many blocks");
     ProfScope ps(c, KID_POINTWISE, s);
-  hipLaunchKernelGGL(k_pointwise<PW_DIVFREE_NORM>, dim3(nb), dim3(256), 0, s, p, c->kx, c->ky,
-                       c->kz);
+    launch(c, &k_pointwise<PW_DIVFREE_NORM>, &k_pointwise_b<PW_DIVFREE_NORM>, dim3(nb), dim3(256), s,
+           PwArgs{p, c->kx, c->ky, c->kz});
     int rc = check_launch(c, "correct_velocity");
     if (rc) return rc;
     return finalize_norm(c, nb, err, s);
   }
   ProfScope ps(c, KID_POINTWISE, s);
-  hipLaunchKernelGGL(k_pointwise<PW_DIVFREE>, dim3(nb), dim3(256), 0, s, p, c->kx, c->ky, c->kz);
+  launch(c, &k_pointwise<PW_DIVFREE>, &k_pointwise_b<PW_DIVFREE>, dim3(nb), dim3(256), s,
+         PwArgs{p, c->kx, c->ky, c->kz});
   return check_launch(c, "correct_velocity");
 }
 

@@ -37,7 +37,20 @@ int walls_iterate_correction(Ctx* c, const double* state_prev, double* pred,
                              const double* rhs_prev, double* rhs_next, double* err,
                              hipStream_t s);
 
-__global__ void k_ctl_begin(StepCtl* ctl, double tol, int max_iter, double far) {
+struct CtlBeginArgs {
+  StepCtl* ctl;
+  double tol;
+  int max_iter;
+  double far;
+};
+struct CtlEndArgs {
+  StepCtl* ctl;
+  int* counts;
+  int counts_cap;
+  double tol;
+};
+
+__device__ inline void ctl_begin_body(StepCtl* ctl, double tol, int max_iter, double far) {
   if (threadIdx.x != 0) return;
   const double e = ctl->err;
   ctl->c = 0;
@@ -51,6 +64,11 @@ __global__ void k_ctl_begin(StepCtl* ctl, double tol, int max_iter, double far) 
   ctl_clear_slots(ctl);
   ctl->rp[1] = ctl->active;
 }
+__global__ void k_ctl_begin(CtlBeginArgs a) { ctl_begin_body(a.ctl, a.tol, a.max_iter, a.far); }
+__global__ void k_ctl_begin_b(const CtlBeginArgs* __restrict__ t) {
+  const CtlBeginArgs a = t[blockIdx.z];
+  ctl_begin_body(a.ctl, a.tol, a.max_iter, a.far);
+}
 
 // The re-centring threshold of a step (split evaluations only).
 static double recenter_far(const Ctx* c, double tol) {
@@ -58,7 +76,7 @@ static double recenter_far(const Ctx* c, double tol) {
                                                                   : (double)INFINITY;
 }
 
-__global__ void k_ctl_end(StepCtl* ctl, int* counts, int counts_cap, double tol) {
+__device__ inline void ctl_end_body(StepCtl* ctl, int* counts, int counts_cap, double tol) {
   if (threadIdx.x != 0) return;
   const double e = ctl->err;
   if (counts && ctl->steps < counts_cap) counts[ctl->steps] = ctl->c;
@@ -66,6 +84,11 @@ __global__ void k_ctl_end(StepCtl* ctl, int* counts, int counts_cap, double tol)
   if (!(e <= tol)) ctl->unconverged += 1;
   if (e != e) ctl->nan += 1;
   ctl->steps += 1;
+}
+__global__ void k_ctl_end(CtlEndArgs a) { ctl_end_body(a.ctl, a.counts, a.counts_cap, a.tol); }
+__global__ void k_ctl_end_b(const CtlEndArgs* __restrict__ t) {
+  const CtlEndArgs a = t[blockIdx.z];
+  ctl_end_body(a.ctl, a.counts, a.counts_cap, a.tol);
 }
 
 // Before the FP64 escalation slots: the passes alternate between the two
@@ -153,11 +176,11 @@ int step_device(Ctx* c, const double* state, double* pred, double* rhs_a, double
   if (rc) return rc;
   c->rule = CtlRule{tol, max_iter, esc_slots > 0 ? esc_ratio : 0.0, esc_cap,
                     esc_slots > 0 ? slots : max_iter, recenter_far(c, tol), c->cfg.recenter_near * tol};
-  hipLaunchKernelGGL(k_ctl_begin, dim3(1), dim3(64), 0, s, ctl, tol, max_iter, c->rule.far);
+  launch(c, &k_ctl_begin, &k_ctl_begin_b, dim3(1), dim3(64), s, CtlBeginArgs{ctl, tol, max_iter, c->rule.far});
   if ((rc = check_launch(c, "ctl_begin"))) return rc;
   if ((rc = step_slots(c, state, pred, rhs_a, rhs_b, slots, esc_slots, s))) return rc;
   if (!c->walls && (rc = periodic_correct_velocity(c, pred, nullptr, s))) return rc;
-  hipLaunchKernelGGL(k_ctl_end, dim3(1), dim3(64), 0, s, ctl, counts, counts_cap, tol);
+  launch(c, &k_ctl_end, &k_ctl_end_b, dim3(1), dim3(64), s, CtlEndArgs{ctl, counts, counts_cap, tol});
   return check_launch(c, "ctl_end");
 }
 
@@ -212,7 +235,7 @@ int step_device_guess(Ctx* c, const double* state, double* pred, double* rhs_a, 
                     esc_slots > 0 ? slots : max_pass, recenter_far(c, tol), c->cfg.recenter_near * tol};
   if ((rc = step_slots(c, state, pred, rhs_a, rhs_b, slots, esc_slots, s))) return rc;
   if (!c->walls && (rc = periodic_correct_velocity(c, pred, nullptr, s))) return rc;
-  hipLaunchKernelGGL(k_ctl_end, dim3(1), dim3(64), 0, s, ctl, counts, counts_cap, tol);
+  launch(c, &k_ctl_end, &k_ctl_end_b, dim3(1), dim3(64), s, CtlEndArgs{ctl, counts, counts_cap, tol});
   return check_launch(c, "ctl_end");
 }
 

@@ -21,6 +21,18 @@ from __future__ import annotations
 
 import torch
 
+from .params import PERIODIC_SYSTEMS
+
+
+class _Solo:
+    """A member outside every batch class, enqueued on its own."""
+
+    def __init__(self, member, parity):
+        self.member, self.parity = member, parity
+
+    def enqueue(self):
+        self.member.enqueue(self.parity)
+
 
 class _Member:
     def __init__(self, flow, state, slots, capacity, esc_slots=None):
@@ -47,9 +59,14 @@ class Ensemble:
     member, or one per member (a member's skipped slots still launch their
     kernels, so each can hold just what its own trajectory needs).
     `esc_slots`: the mixed form's FP64 escalation slots per step (None: the
-    library default)."""
+    library default).  `batched`: the periodic members of one size,
+    precision and kernel variants (``groups()``) advance by one launch per
+    kernel over all of them (``dnsmi_batch_plan``; their slot count is the
+    class's largest) instead of one branch per member; every member's
+    results are the same bit for bit."""
 
-    def __init__(self, flows, states, n_streams: int = 8, slots=None, capacity: int = 4096, esc_slots=None):
+    def __init__(self, flows, states, n_streams: int = 8, slots=None, capacity: int = 4096, esc_slots=None,
+                 batched: bool = False):
         if len(flows) != len(states) or not flows:
             raise ValueError("need one state per flow")
         if len({id(f._native) for f in flows}) != len(flows):
@@ -63,19 +80,56 @@ class Ensemble:
         self.capacity = capacity
         self.parity = 0
         self.steps = 0
+        self.plans = None
+        if batched:
+            self.plans = [self._plan(parity, esc_slots) for parity in (0, 1)]
         self.graphs = [self._capture(0, n_streams), self._capture(1, n_streams)]
+
+    def groups(self):
+        """Member indices per batch class: periodic members of one size,
+        precision, kernel variants and corrector limits (members of a class
+        share one launch per kernel in the batched form)."""
+        out = {}
+        for i, m in enumerate(self.members):
+            p = m.flow.params
+            key = (m.nat.shape, p.precision, m.nat.variants(), p.corrector_tolerance, p.max_corrector_iterations,
+                   p.system in PERIODIC_SYSTEMS)
+            out.setdefault(key, []).append(i)
+        return list(out.values())
+
+    def _plan(self, parity: int, esc_slots):
+        from .native import BatchPlan
+
+        plans = []
+        for idx in self.groups():
+            ms = [self.members[i] for i in idx]
+            if ms[0].flow.params.system not in PERIODIC_SYSTEMS:
+                plans += [_Solo(m, parity) for m in ms]  # wall-bounded members keep their own launches
+                continue
+            slots = max(m.slots for m in ms)
+            for m in ms:
+                m.slots = slots
+            plans.append(BatchPlan([m.nat for m in ms], [m.bufs[parity] for m in ms],
+                                   [m.bufs[1 - parity] for m in ms], [m.rhs[0] for m in ms],
+                                   [m.rhs[1] for m in ms], slots, counts=[m.counts for m in ms],
+                                   esc_slots=esc_slots))
+        return plans
 
     def _capture(self, parity: int, n_streams: int):
         g = torch.cuda.CUDAGraph()
-        side = [torch.cuda.Stream() for _ in range(min(n_streams, len(self.members)))]
+        work = self.plans[parity] if self.plans is not None else self.members
+        side = [torch.cuda.Stream() for _ in range(min(n_streams, len(work)))]
         torch.cuda.synchronize()
         with torch.cuda.graph(g):
             main = torch.cuda.current_stream()
             for s in side:
                 s.wait_stream(main)
-            for i, m in enumerate(self.members):
+            for i, w in enumerate(work):
                 with torch.cuda.stream(side[i % len(side)]):
-                    m.enqueue(parity)
+                    if self.plans is not None:
+                        w.enqueue()
+                    else:
+                        w.enqueue(parity)
             for s in side:
                 main.wait_stream(s)
         return g

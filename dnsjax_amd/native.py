@@ -228,6 +228,12 @@ class NativeContext:
     def tuning(self) -> str:
         return self.lib.dnsmi_tuning(self.handle).decode()
 
+    def variants(self) -> tuple[int, int]:
+        """(tile_cols, variants) the context runs, in Params' convention."""
+        tc, v = ctypes.c_int(), ctypes.c_int()
+        self._chk(self.lib.dnsmi_variants(self.handle, ctypes.byref(tc), ctypes.byref(v)))
+        return tc.value, v.value
+
     def walls_table(self, which: int, shape: tuple):
         """Host copy of a wall-bounded set-up table (see include/dnsmi.h)."""
         import numpy as np
@@ -254,3 +260,54 @@ class NativeContext:
         err = self.scalar() if measure else None
         self._chk(self.lib.dnsmi_correct_velocity(self.handle, _ptr(state), _ptr(err), _stream()))
         return state, err
+
+
+class BatchPlan:
+    """One device step of several periodic contexts of one size class, merged
+    into one launch per kernel over the members (``dnsmi_batch_plan``).  The
+    members' buffers are fixed at planning; ``enqueue`` issues the step of
+    all of them on torch's current stream (capturable)."""
+
+    def __init__(self, natives, states, nexts, rhs_a, rhs_b, slots, counts=None, esc_slots=None):
+        n = len(natives)
+        if not n or not (len(states) == len(nexts) == len(rhs_a) == len(rhs_b) == n):
+            raise ValueError("need one state, next, rhs_a and rhs_b per member")
+        lib = natives[0].lib
+        p = natives[0].params
+        tol = p.corrector_tolerance
+        max_iter = p.max_corrector_iterations
+        n_esc, ratio, cap = natives[0]._esc_args(esc_slots)
+        for nat, *bufs in zip(natives, states, nexts, rhs_a, rhs_b):
+            for t, name in zip(bufs, ("state", "next", "rhs_a", "rhs_b")):
+                nat.check_field(t, name)
+            q = nat.params
+            if (q.corrector_tolerance, q.max_corrector_iterations) != (tol, max_iter):
+                raise ValueError("members of a batch share the corrector tolerance and iteration limit")
+        P = ctypes.c_void_p
+        arr = lambda ts: (P * n)(*[t.data_ptr() for t in ts])  # noqa: E731
+        ctxs = (P * n)(*[nat.handle.value for nat in natives])
+        cnt = None if counts is None else (P * n)(*[c.data_ptr() for c in counts])
+        ncap = 0 if counts is None else min(c.numel() for c in counts)
+        handle = ctypes.c_void_p()
+        rc = lib.dnsmi_batch_plan(ctxs, n, arr(states), arr(nexts), arr(rhs_a), arr(rhs_b), slots, n_esc, cnt, ncap,
+                                  tol, max_iter, ratio, cap, ctypes.byref(handle))
+        _lib.check(rc, natives[0].handle)
+        self.lib, self.handle, self.members = lib, handle, n
+        self._keep = (list(states), list(nexts), list(rhs_a), list(rhs_b), counts)
+
+    @property
+    def launches(self) -> int:
+        return self.lib.dnsmi_batch_launches(self.handle)
+
+    def enqueue(self) -> None:
+        _lib.check(self.lib.dnsmi_batch_enqueue(self.handle, _stream()), None)
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            try:
+                torch.cuda.synchronize()
+            except Exception:
+                return
+            self.lib.dnsmi_batch_destroy(h)
+            self.handle = None

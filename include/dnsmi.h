@@ -156,6 +156,12 @@ int dnsmi_correct_velocity(dnsmi_ctx* ctx, double* state, double* err_dev, void*
  * "name=base|alt(us base/us alt) ..." (no counterpart in the reference). */
 const char* dnsmi_tuning(const dnsmi_ctx* ctx);
 
+/* The kernel variants the context runs, in the config's convention
+ * (tile_cols 8 or 16; variants bit mask as for dnsmi_config.variants), so
+ * that further contexts can be created with the same choice (no
+ * counterpart in the reference). */
+int dnsmi_variants(const dnsmi_ctx* ctx, int* tile_cols, int* variants);
+
 /* Host copy (out: n host doubles) of a wall-bounded set-up table in the
  * reference layout, for inspection (no counterpart in the reference, which
  * keeps these as module attributes, flows/plane_couette.py:139-219):
@@ -364,6 +370,28 @@ int dnsmi_guess_profile(dnsmi_ctx* ctx, const double* state, double* pred, doubl
 int dnsmi_profile_begin(dnsmi_ctx* ctx);
 int dnsmi_profile_end(dnsmi_ctx* ctx, int n_ids, double* total_ms, long long* launches);
 const char* dnsmi_kernel_name(int id);
+
+/* Batched device steps of an ensemble (no counterpart in the reference,
+ * which runs one trajectory, __main__.py:125-203): for n periodic contexts
+ * with the same sizes, precision and kernel variants (each with its own
+ * Re, dt, tolerance and buffers), dnsmi_batch_plan records the launches of
+ * one dnsmi_step_device per member (member i: states[i] -> nexts[i], work
+ * arrays rhs_a[i], rhs_b[i], iteration counts into counts[i] when counts
+ * is given) and merges them into one launch per kernel over the members;
+ * dnsmi_batch_enqueue issues the whole step of all members on `stream`
+ * (capturable).  Each member's results equal its own dnsmi_step_device's
+ * bit for bit.  A plan keeps the buffers' addresses (plan each buffer
+ * parity once) and its argument tables (a few KB of device memory) until
+ * dnsmi_batch_destroy.  Errors of a plan are reported on ctxs[0]. */
+typedef struct dnsmi_batch dnsmi_batch;
+int dnsmi_batch_plan(dnsmi_ctx* const* ctxs, int n, const double* const* states, double* const* nexts,
+                     double* const* rhs_a, double* const* rhs_b, int slots, int esc_slots,
+                     int* const* counts, int counts_cap, double tol, int max_iter, double esc_ratio,
+                     double esc_cap, dnsmi_batch** out);
+int dnsmi_batch_enqueue(dnsmi_batch* b, void* stream);
+/* Launches one dnsmi_batch_enqueue issues. */
+int dnsmi_batch_launches(const dnsmi_batch* b);
+void dnsmi_batch_destroy(dnsmi_batch* b);
 
 #ifdef __cplusplus
 }

@@ -21,22 +21,42 @@ from oracle import stepper as S  # noqa: E402
 SIZES = [32] * 16 + [48] * 16 + [64] * 16 + [96] * 16
 
 
-def _flows(res):
+def _flows(res, pinned=None):
+    """One flow per member; with `pinned` ({size: (tile_cols, variants)}),
+    every member of a size runs that size's kernel variants."""
     from dnsjax_amd.flows import TriplyPeriodicFlow
     from dnsjax_amd.params import kolmogorov_2pi
 
-    return [TriplyPeriodicFlow(kolmogorov_2pi(n, float(r), 0.01, corrector_tolerance=1e-10,
-                                              max_corrector_iterations=10, precision="mixed"))
-            for n, r in zip(SIZES, res)]
+    out = []
+    for n, r in zip(SIZES, res):
+        kw = {} if pinned is None else dict(zip(("tile_cols", "variants"), pinned[n]))
+        out.append(TriplyPeriodicFlow(kolmogorov_2pi(n, float(r), 0.01, corrector_tolerance=1e-10,
+                                                     max_corrector_iterations=10, precision="mixed", **kw)))
+    return out
 
 
-def test_64_member_ensemble_matches_eager_runs_and_the_oracle():
+@pytest.mark.parametrize("batched", [False, True], ids=["branches", "batched"])
+def test_64_member_ensemble_matches_eager_runs_and_the_oracle(batched):
+    """`batched`: each size class advances by one launch per kernel over its
+    16 members (their variants pinned to the autotuner's choice for the
+    size), instead of one branch per member."""
     from dnsjax_amd.driver import run
     from dnsjax_amd.graph import Ensemble
 
     res = np.random.default_rng(3).uniform(20.0, 80.0, len(SIZES))
     n = 3
-    eager, graphed = _flows(res), _flows(res)
+    pinned = None
+    if batched:
+        from dnsjax_amd.flows import TriplyPeriodicFlow
+        from dnsjax_amd.params import kolmogorov_2pi
+
+        pinned = {}
+        for s in (32, 48, 64, 96):  # the autotuner's choice per size
+            f = TriplyPeriodicFlow(kolmogorov_2pi(s, 50.0, 0.01, corrector_tolerance=1e-10,
+                                                  max_corrector_iterations=10, precision="mixed"))
+            pinned[s] = f._native.variants()
+            del f
+    eager, graphed = _flows(res, pinned), _flows(res, pinned)
     states0 = [f.init_state(seed=300 + i, rms=0.05) for i, f in enumerate(eager)]
     refs = []
     for f, u0 in zip(eager, states0):
@@ -45,7 +65,10 @@ def test_64_member_ensemble_matches_eager_runs_and_the_oracle():
         assert r.converged and r.escalations == 0
         refs.append((traj, r.counts))
     slots = [min(max(c) + 1, 10) for _, c in refs]
-    ens = Ensemble(graphed, states0, n_streams=8, slots=slots, capacity=n)
+    ens = Ensemble(graphed, states0, n_streams=8, slots=slots, capacity=n, batched=batched)
+    if batched:
+        assert sorted(len(g) for g in ens.groups()) == [16, 16, 16, 16]
+        assert all(pl.members == 16 for pl in ens.plans[0])
     ens.step(n)
     torch.cuda.synchronize()
     for m, (traj, counts) in enumerate(refs):

@@ -95,6 +95,37 @@ def test_ensemble_members_match_single_runs():
         assert ens.counts()[m] == counts
 
 
+def test_batched_ensemble_members_match_single_runs():
+    """Batched classes (three 16^3 members, two 32^3, one 48^3, plus an FP64
+    16^3 member and a plane Couette member outside every class): each member
+    equals its own eager run, and a class of k members issues the launches
+    of one."""
+    from dnsjax_amd.graph import Ensemble
+    from dnsjax_amd.native import BatchPlan
+
+    specs = [(16, 25.0, "mixed"), (32, 40.0, "mixed"), (16, 55.0, "mixed"), (48, 70.0, "mixed"),
+             (16, 35.0, "mixed"), (32, 20.0, "mixed"), (16, 45.0, "fp64")]
+    n = 4
+    flows = [periodic(sz, re, prec) for sz, re, prec in specs] + [walls(16, 33, 16)]
+    states0 = [f.init_state(seed=320 + i, rms=0.05) for i, f in enumerate(flows)]
+    refs = [eager_trajectory(periodic(sz, re, prec), u0, n) for (sz, re, prec), u0 in zip(specs, states0)]
+    refs.append(eager_trajectory(walls(16, 33, 16), states0[-1], n))
+    ens = Ensemble(flows, states0, n_streams=3, batched=True)
+    assert sorted(len(g) for g in ens.groups()) == [1, 1, 1, 2, 3]
+    plans = [w for w in ens.plans[0] if isinstance(w, BatchPlan)]
+    assert sorted(pl.members for pl in plans) == [1, 1, 2, 3]
+    # one launch sequence per precision, whatever the class's size and members
+    assert len({pl.launches for pl in plans}) == 2 and min(pl.launches for pl in plans) > 0
+    for i in range(n):
+        ens.step(1)
+        torch.cuda.synchronize()
+        for m, (states, _) in enumerate(refs):
+            assert torch.equal(ens.states()[m], states[i]), (m, i)
+    for m, (_, counts) in enumerate(refs):
+        assert ens.counts()[m] == counts
+    assert all(x["unconverged"] == 0 for x in ens.step_stats())
+
+
 def test_ensemble_with_per_member_slots():
     """One slot count per member, each one more than its own eager run's
     largest count: every member still equals its eager run, with no step
