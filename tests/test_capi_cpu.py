@@ -43,6 +43,25 @@ def test_library_exports_every_declared_symbol():
     assert set(_header_functions()) <= set(_lib.SIGNATURES)
 
 
+def test_config_struct_matches_the_header():
+    """The ctypes DnsmiConfig has every field of dnsmi_config, in order and
+    with its C type (the library reads the whole struct)."""
+    txt = open(os.path.join(ROOT, "include", "dnsmi.h")).read()
+    body = re.search(r"typedef struct dnsmi_config \{(.*?)\} dnsmi_config;", txt, re.S).group(1)
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    fields = []
+    for decl in body.split(";"):
+        decl = decl.strip()
+        if not decl:
+            continue
+        ctype, names = decl.split(None, 1)
+        fields += [(n.strip(), ctype) for n in names.split(",")]
+    want = {"int": ctypes.c_int, "double": ctypes.c_double}
+    got = [(n, t) for n, t in _lib.DnsmiConfig._fields_]
+    assert [n for n, _ in got] == [n for n, _ in fields]
+    assert all(t is want[c] for (_, t), (_, c) in zip(got, fields))
+
+
 def test_create_rejects_null_and_reports():
     if not os.path.exists(_lib.LIB_PATH):
         pytest.skip("libdnsmi.so not built")
