@@ -1522,15 +1522,23 @@ __device__ __forceinline__ void map_offsets(unsigned (&o)[W], int U, int u, int 
   for (int k = 0; k < W; ++k) o[k] = 8u * (unsigned)map_word_at(k, u, tt, LAG, W, U);
 }
 
-// Lane t's row of chunk c's map (chunks past the last repeat it): a
-// wave-uniform base and the lane's offsets, no per-load arithmetic.
+// Lane t's row of chunk c's map (chunks past the last repeat it): buffer
+// loads from a resource over the chunk's maps (base and size in scalar
+// registers) at the lane's 32-bit offsets.  Flat 64-bit addresses would need
+// a pair of address registers per load, and the compiler, reusing those of
+// loads still in flight, waited for every outstanding load at each chunk
+// step, which undid the prefetch; a buffer load needs none.  Offsets past
+// the chunk's maps (none, by construction) would read zero.
 template <int W, int LAG>
 __device__ __forceinline__ void map_row(double (&f)[W], const double* __restrict__ map,
                                         const unsigned (&o)[W], int U, int c, int nC) {
-  const char* p = reinterpret_cast<const char*>(map + (size_t)(c < nC ? c : nC - 1) * U *
-                                                          map_pre(W, LAG, W));
+  const int words = U * map_pre(W, LAG, W);
+  const double* base = map + (size_t)(c < nC ? c : nC - 1) * words;
+  const __amdgpu_buffer_rsrc_t r =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(base), 0, words * (int)sizeof(double), 0x00020000);
 #pragma unroll
-  for (int k = 0; k < W; ++k) f[k] = *reinterpret_cast<const double*>(p + o[k]);
+  for (int k = 0; k < W; ++k)
+    f[k] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)o[k], 0, 0));
 }
 
 // acc[r] += sum_k map[t][k] win[r][k] in order k = 0, 1, ...; the window
