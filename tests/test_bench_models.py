@@ -30,8 +30,11 @@ def mb(x):
 def test_periodic_rhs_model_matches_the_stated_figures():
     assert mb(bench.rhs_model_bytes(128, 16)) == pytest.approx(1228.02, abs=0.01)
     assert mb(bench.rhs_model_bytes(32, 16)) == pytest.approx(18.95, abs=0.01)
-    # FP32 intermediates halve every term of the model
-    assert bench.rhs_model_bytes(128, 8) * 2 == bench.rhs_model_bytes(128, 16)
+    # mixed: FP32 intermediates, the FP64 state read and RHS written
+    assert mb(bench.rhs_model_bytes(128, 8)) == pytest.approx(663.56, abs=0.01)
+    S = 3 * 127 * 127 * 64 * 16
+    assert bench.rhs_model_bytes(128, 16) - bench.rhs_model_bytes(128, 8) == \
+        (bench.rhs_model_bytes(128, 16) - 2 * S) // 2
 
 
 def test_periodic_kernel_bytes_parts():
