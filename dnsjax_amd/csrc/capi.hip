@@ -177,6 +177,7 @@ int dnsmi_create(const dnsmi_config* cfg, dnsmi_ctx** out) {
       c->xpass_split = (cfg->variants & 4) != 0;
       c->sweep_seq = (cfg->variants & 8) != 0;
       c->imm_fused = (cfg->variants & 16) != 0 && !c->sweep_seq;
+      c->zinv_nt = (cfg->variants & 32) != 0;
     }
     if (cfg->system == DNSMI_PLANE_COUETTE) {
       c->walls = true;
@@ -526,7 +527,7 @@ int dnsmi_variants(const dnsmi_ctx* ctx, int* tile_cols, int* variants) {
   const Ctx* c = &ctx->impl;
   *tile_cols = c->tile_cols;
   *variants = (c->yinv_seq ? 1 : 0) | (c->zinv_seq ? 2 : 0) | (c->xpass_split ? 4 : 0) | (c->sweep_seq ? 8 : 0) |
-              (c->imm_fused ? 16 : 0);
+              (c->imm_fused ? 16 : 0) | (c->zinv_nt ? 32 : 0);
   return DNSMI_OK;
 }
 

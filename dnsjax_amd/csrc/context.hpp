@@ -132,6 +132,7 @@ struct Ctx {
   bool xpass_split = false;
   bool yinv_seq = false;
   bool zinv_seq = false;
+  bool zinv_nt = false;  // z-inverse output through non-temporal stores
   bool sweep_seq = false;  // wall-bounded: sequential band sweeps instead of chunk maps
   bool imm_fused = false;  // wall-bounded, chunk maps: the two-launch fused IMM
   std::string tuning;

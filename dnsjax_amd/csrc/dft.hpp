@@ -22,6 +22,25 @@ struct alignas(2 * sizeof(T)) cx {
   T x, y;
 };
 
+// A store marked non-temporal: the line is written to memory without being
+// kept in the caches, which then keep what the writing kernel still reads.
+template <typename T>
+__device__ __forceinline__ void st_stream(cx<T>* p, cx<T> v) {
+  typedef T v2 __attribute__((ext_vector_type(2)));
+  v2 w;
+  w.x = v.x;
+  w.y = v.y;
+  __builtin_nontemporal_store(w, reinterpret_cast<v2*>(p));
+}
+// A plain or a non-temporal store, as a wave-uniform flag selects.
+template <typename T>
+__device__ __forceinline__ void st_out(cx<T>* p, cx<T> v, bool nt) {
+  if (nt)
+    st_stream(p, v);
+  else
+    *p = v;
+}
+
 template <typename T>
 __device__ __forceinline__ cx<T> cadd(cx<T> a, cx<T> b) { return {a.x + b.x, a.y + b.y}; }
 template <typename T>

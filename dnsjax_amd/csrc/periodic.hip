@@ -74,6 +74,7 @@ struct ZInvArgs {
   const double* kz;
   int nz, Py, Nx1;
   const int* act;
+  int stream_out;  // non-temporal stores of the output (an autotuned choice)
 };
 struct XArgs {
   const void* w2;
@@ -196,7 +197,7 @@ __global__ void __launch_bounds__(TC * ColPlan<P>::threads) k_y_inverse_b(const 
 // ------------------------------------------------------------------ K2
 template <typename T, int P, int TC>
 __device__ __forceinline__ void z_inverse_body(const cx<T>* __restrict__ w1, cx<T>* __restrict__ w2, const cx<T>* __restrict__ tw,
-                const double* __restrict__ kz_tab, int nz, int Py, int Nx1, const int* __restrict__ act) {
+                const double* __restrict__ kz_tab, int nz, int Py, int Nx1, const int* __restrict__ act, bool nt) {
   if (act && *act == 0) return;
   constexpr int N1 = ColPlan<P>::N1, N2 = ColPlan<P>::N2;
   __shared__ cx<T> lds[TC * P];
@@ -255,7 +256,7 @@ __device__ __forceinline__ void z_inverse_body(const cx<T>* __restrict__ w1, cx<
       step_b_load<P, N1, N2, +1, TC>(o, j, col, lds);
 #pragma unroll
       for (int k2 = 0; k2 < N2; ++k2)
-        w2[fa * out_f + out_off + (size_t)(j + N1 * k2) * Nx1] = o[k2];
+        st_out(&w2[fa * out_f + out_off + (size_t)(j + N1 * k2) * Nx1], o[k2], nt);
     }
     __syncthreads();
     if (pass < 2) {
@@ -266,7 +267,7 @@ __device__ __forceinline__ void z_inverse_body(const cx<T>* __restrict__ w1, cx<
         step_b_load<P, N1, N2, +1, TC>(o, j, col, lds);
 #pragma unroll
         for (int k2 = 0; k2 < N2; ++k2)
-          w2[fb * out_f + out_off + (size_t)(j + N1 * k2) * Nx1] = o[k2];
+          st_out(&w2[fb * out_f + out_off + (size_t)(j + N1 * k2) * Nx1], o[k2], nt);
       }
       __syncthreads();
     }
@@ -274,12 +275,12 @@ __device__ __forceinline__ void z_inverse_body(const cx<T>* __restrict__ w1, cx<
 }
 template <typename T, int P, int TC>
 __global__ void __launch_bounds__(TC * ColPlan<P>::threads) k_z_inverse(ZInvArgs a) {
-  z_inverse_body<T, P, TC>(static_cast<const cx<T>*>(a.w1), static_cast<cx<T>*>(a.w2), static_cast<const cx<T>*>(a.tw), a.kz, a.nz, a.Py, a.Nx1, a.act);
+  z_inverse_body<T, P, TC>(static_cast<const cx<T>*>(a.w1), static_cast<cx<T>*>(a.w2), static_cast<const cx<T>*>(a.tw), a.kz, a.nz, a.Py, a.Nx1, a.act, a.stream_out != 0);
 }
 template <typename T, int P, int TC>
 __global__ void __launch_bounds__(TC * ColPlan<P>::threads) k_z_inverse_b(const ZInvArgs* __restrict__ t) {
   const ZInvArgs a = t[blockIdx.z];
-  z_inverse_body<T, P, TC>(static_cast<const cx<T>*>(a.w1), static_cast<cx<T>*>(a.w2), static_cast<const cx<T>*>(a.tw), a.kz, a.nz, a.Py, a.Nx1, a.act);
+  z_inverse_body<T, P, TC>(static_cast<const cx<T>*>(a.w1), static_cast<cx<T>*>(a.w2), static_cast<const cx<T>*>(a.tw), a.kz, a.nz, a.Py, a.Nx1, a.act, a.stream_out != 0);
 }
 
 
@@ -364,7 +365,7 @@ __global__ void __launch_bounds__(TC * ColPlan<P>::threads) k_y_inverse_seq_b(co
 template <typename T, int P, int TC>
 __device__ __forceinline__ void z_inverse_seq_body(const cx<T>* __restrict__ w1, cx<T>* __restrict__ w2,
                     const cx<T>* __restrict__ tw, const double* __restrict__ kz_tab, int nz, int Py,
-                    int Nx1, const int* __restrict__ act) {
+                    int Nx1, const int* __restrict__ act, bool nt) {
   if (act && *act == 0) return;
   constexpr int N1 = ColPlan<P>::N1, N2 = ColPlan<P>::N2;
   __shared__ cx<T> lds[TC * P];
@@ -424,19 +425,19 @@ __device__ __forceinline__ void z_inverse_seq_body(const cx<T>* __restrict__ w1,
       step_b_load<P, N1, N2, +1, TC>(o, j, col, lds);
 #pragma unroll
       for (int k2 = 0; k2 < N2; ++k2)
-        w2[fo * out_f + out_off + (size_t)(j + N1 * k2) * Nx1] = o[k2];
+        st_out(&w2[fo * out_f + out_off + (size_t)(j + N1 * k2) * Nx1], o[k2], nt);
     }
     __syncthreads();
   }
 }
 template <typename T, int P, int TC>
 __global__ void __launch_bounds__(TC * ColPlan<P>::threads) k_z_inverse_seq(ZInvArgs a) {
-  z_inverse_seq_body<T, P, TC>(static_cast<const cx<T>*>(a.w1), static_cast<cx<T>*>(a.w2), static_cast<const cx<T>*>(a.tw), a.kz, a.nz, a.Py, a.Nx1, a.act);
+  z_inverse_seq_body<T, P, TC>(static_cast<const cx<T>*>(a.w1), static_cast<cx<T>*>(a.w2), static_cast<const cx<T>*>(a.tw), a.kz, a.nz, a.Py, a.Nx1, a.act, a.stream_out != 0);
 }
 template <typename T, int P, int TC>
 __global__ void __launch_bounds__(TC * ColPlan<P>::threads) k_z_inverse_seq_b(const ZInvArgs* __restrict__ t) {
   const ZInvArgs a = t[blockIdx.z];
-  z_inverse_seq_body<T, P, TC>(static_cast<const cx<T>*>(a.w1), static_cast<cx<T>*>(a.w2), static_cast<const cx<T>*>(a.tw), a.kz, a.nz, a.Py, a.Nx1, a.act);
+  z_inverse_seq_body<T, P, TC>(static_cast<const cx<T>*>(a.w1), static_cast<cx<T>*>(a.w2), static_cast<const cx<T>*>(a.tw), a.kz, a.nz, a.Py, a.Nx1, a.act, a.stream_out != 0);
 }
 
 
@@ -1386,7 +1387,7 @@ static int launch_z_inverse(Ctx* c, hipStream_t s) {
   case P:                                                                                       \
     with_tc(c, [&](auto tcv) {                                                                  \
       constexpr int TC = decltype(tcv)::value;                                                  \
-      const ZInvArgs a{b.A, b.B, b.twz, c->kz, c->nz, c->Py, c->Nx1, c->act};                   \
+      const ZInvArgs a{b.A, b.B, b.twz, c->kz, c->nz, c->Py, c->Nx1, c->act, c->zinv_nt};      \
       if (c->zinv_seq)                                                                          \
         launch(c, &k_z_inverse_seq<T, P, TC>, &k_z_inverse_seq_b<T, P, TC>, grid,               \
                dim3(TC * ColPlan<P>::threads), s, a);                                           \
@@ -1866,10 +1867,33 @@ static int periodic_autotune_t(Ctx* c) {
       t8 = std::min(t8, time_launches(rhs, reps));
     }
     c->tile_cols = t8 < t16 ? kTCMin : kTC;
-    c->release(out);
     char buf[160];
     snprintf(buf, sizeof buf, "tile_cols=%d(16: %.1f / 8: %.1f us) ", c->tile_cols, 1e3 * t16 / reps,
              1e3 * t8 / reps);
+    log += buf;
+    // non-temporal stores of the z-inverse output: they keep the caches for
+    // the lines that pass still reads, but the next pass then reads all of
+    // its input from memory, so the choice is timed over a whole RHS.  FP64
+    // only: the FP32 output (half the bytes) is read back from the caches
+    // by the x pass, and in FP32 the isolated timings are near ties that
+    // the stepping loop, with its recorded reference fields, does not bear
+    // out.
+    double tp = 1e30, tn = 1e30;
+    if (sizeof(T) == 8) {
+      for (int round = 0; round < 3; ++round) {
+        c->zinv_nt = false;
+        tp = std::min(tp, time_launches(rhs, reps));
+        c->zinv_nt = true;
+        tn = std::min(tn, time_launches(rhs, reps));
+      }
+    }
+    c->zinv_nt = tn < tp;
+    c->release(out);
+    if (sizeof(T) == 8)
+      snprintf(buf, sizeof buf, "z_inverse_stream=%s(%.1f/%.1f us) ", c->zinv_nt ? "alt" : "base",
+               1e3 * tp / reps, 1e3 * tn / reps);
+    else
+      snprintf(buf, sizeof buf, "z_inverse_stream=base(fp32) ");
     log += buf;
   }
   c->tuning = log;

@@ -242,7 +242,7 @@ def test_rejects_bad_configs_and_aliasing():
         flow.get_rhs(u.to(torch.complex64))
 
 
-@pytest.mark.parametrize("variants", [0, 1, 2, 4, 7])
+@pytest.mark.parametrize("variants", [0, 1, 2, 4, 7, 32, 34])
 @pytest.mark.parametrize("case", ["periodic_16_kolmogorov", "periodic_16x8x32_waleffe_ly4"])
 def test_every_kernel_variant_matches(case, variants):
     """Each exact kernel variant (as the autotuner may pick) on the golden step."""
@@ -258,7 +258,26 @@ def test_every_kernel_variant_matches(case, variants):
 def test_autotuner_reports_choices():
     flow = make("periodic_16_kolmogorov")
     t = flow._native.tuning
-    assert "y_inverse_seq=" in t and "z_inverse_seq=" in t and "x_split=" in t
+    assert "y_inverse_seq=" in t and "z_inverse_seq=" in t and "x_split=" in t and "z_inverse_stream=" in t
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+@pytest.mark.parametrize("seq", [0, 2], ids=["paired", "sequential"])
+def test_streaming_z_inverse_stores_are_bit_identical(seq, prec):
+    """Variant bit 32 (non-temporal stores of the z-inverse output) changes
+    where the lines are cached, not the values: at 64^3 the RHS and a
+    predict-and-correct equal the plain stores' bit for bit."""
+    from dnsjax_amd.flows import TriplyPeriodicFlow
+    from dnsjax_amd.params import kolmogorov_2pi
+
+    flows = [TriplyPeriodicFlow(kolmogorov_2pi(64, 60.0, 0.01, corrector_tolerance=1e-10, precision=prec,
+                                               tile_cols=16, variants=seq | nt)) for nt in (0, 32)]
+    assert [f._native.variants()[1] for f in flows] == [seq, seq | 32]
+    u = flows[0].init_state(seed=3, rms=0.1)
+    r = [f.get_rhs(u) for f in flows]
+    assert torch.equal(r[0], r[1])
+    p = [f.predict_and_correct(u) for f in flows]
+    assert torch.equal(p[0][0], p[1][0]) and torch.equal(p[0][1], p[1][1])
 
 
 @pytest.mark.parametrize("case", list(CASES))
