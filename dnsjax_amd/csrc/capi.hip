@@ -527,7 +527,7 @@ int dnsmi_variants(const dnsmi_ctx* ctx, int* tile_cols, int* variants) {
   const Ctx* c = &ctx->impl;
   *tile_cols = c->tile_cols;
   *variants = (c->yinv_seq ? 1 : 0) | (c->zinv_seq ? 2 : 0) | (c->xpass_split ? 4 : 0) | (c->sweep_seq ? 8 : 0) |
-              (c->imm_fused ? 16 : 0) | (c->zinv_nt ? 32 : 0);
+              (c->imm_fused ? 16 : 0) | (c->zinv_nt ? 32 : 0) | (c->walls && c->curl_rows == 64 ? 64 : 0);
   return DNSMI_OK;
 }
 

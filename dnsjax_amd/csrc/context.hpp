@@ -133,6 +133,7 @@ struct Ctx {
   bool yinv_seq = false;
   bool zinv_seq = false;
   bool zinv_nt = false;  // z-inverse output through non-temporal stores
+  int curl_rows = 32;    // walls: y points per FD-curl workgroup (32 or 64)
   bool sweep_seq = false;  // wall-bounded: sequential band sweeps instead of chunk maps
   bool imm_fused = false;  // wall-bounded, chunk maps: the two-launch fused IMM
   std::string tuning;

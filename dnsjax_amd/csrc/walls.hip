@@ -44,6 +44,7 @@ constexpr int kImmT = 128;  // threads per mode in the parallel phases
 
 struct WallTables {
   int p = 0, Ny = 0, Nkz = 0, Nkx = 0, M = 0;
+  int curl_ty = 32;  // y points per curl workgroup (walls_curl_ty)
   // device
   double* d1w = nullptr;  // (Ny, p+1)
   int* d1j0 = nullptr;    // (Ny)
@@ -71,6 +72,8 @@ struct WallTables {
   std::vector<double> h_ys;
   std::vector<int> h_umode;
 };
+
+inline int walls_curl_ty(int Ny, int M);  // the curl kernel's tile height
 
 // ------------------------------------------------------------------ host set-up
 // Fornberg (1998), SIAM Rev. 40, 685-691: weights of derivatives 0..m at z
@@ -690,6 +693,9 @@ int walls_setup(Ctx* c) {
   t->Nkz = Nkz;
   t->Nkx = Nkx;
   t->M = M;
+  // forced variants (tile_cols != 0): bit 64 selects the 64-row curl tiles
+  t->curl_ty = g.tile_cols != 0 ? ((g.variants & 64) ? 64 : 32) : walls_curl_ty(Ny, M);
+  c->curl_rows = t->curl_ty;
 
   // buffers: WT1 out (6,Ny,Nkz,Nkx) | z-inverse out (6,Ny,Pz,Nkx) | x-pass out
   // (3,Ny,Pz,Nkx) | z-forward out (3,Ny,Nkz,Nkx)
@@ -987,7 +993,8 @@ int walls_setup(Ctx* c) {
   const double setup_ms =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   char note[96];
-  std::snprintf(note, sizeof note, "imm_setup=%s(%.1f ms) ", on_gpu ? "gpu" : "host", setup_ms);
+  std::snprintf(note, sizeof note, "imm_setup=%s(%.1f ms) curl_rows=%d ", on_gpu ? "gpu" : "host", setup_ms,
+                t->curl_ty);
   if (want_maps) {
     // the chunked solve's mode order: by map, so that the two modes of a
     // wavefront share theirs wherever two modes have the same k^2
@@ -1024,27 +1031,37 @@ void walls_release(Ctx* c) {
 // WT1: FD curl + transpose.  In: u (3, M, Ny) with M = Nkz*Nkx modes, FP64.
 // Out: V (6, Ny, Nkz, Nkx) in T: u_x, u_y, u_z, w_x = D1 u_z - i kz u_y,
 // i kz u_x, -D1 u_x (the i kx parts of w_y, w_z are added by the x pass).
-// A block handles kCurlTY y-points x TM modes; loads run along y, stores
+// A block handles TY y-points x kTM modes; loads run along y, stores
 // along the mode index.  The curl's tiles are taller than the transposes'
-// (kTY): each re-reads a halo of S1 rows on either side.
-constexpr int kTY = 16, kTM = 16, kCurlTY = 32;
+// (kTY): each re-reads a halo of S1 rows on either side, so taller tiles
+// re-read less but make fewer workgroups (walls_curl_ty chooses).
+constexpr int kTY = 16, kTM = 16;
+
+// The curl's tile height: 64 rows where that still gives at least four
+// workgroups per CU (128 x 256 x 128: 2032; the RHS 356 against 386 us
+// FP64, 211 against 240 us mixed), else 32 (64 x 128 x 64: 252 workgroups
+// of 64 rows would leave CUs idle; 64 against 73 us).
+inline int walls_curl_ty(int Ny, int M) {
+  const long long blocks64 = (long long)((Ny + 63) / 64) * ((M + kTM - 1) / kTM);
+  return blocks64 >= 4 * 256 ? 64 : 32;
+}
 
 // EM (EvalMode): EV_RECORD also copies u to ucopy (u_ref); EV_INC takes the
 // curl of d = u - uref, formed in FP64.
-template <typename T, int S1, int EM>
-__global__ void __launch_bounds__(kCurlTY* kTM)
+template <typename T, int S1, int EM, int TY>
+__global__ void __launch_bounds__(TY* kTM)
     k_walls_curl_t(const cx<double>* __restrict__ u, cx<T>* __restrict__ v,
                    const double* __restrict__ d1w, const int* __restrict__ d1j0,
                    const double* __restrict__ mkz, int Ny, int M, const int* __restrict__ act,
                    const cx<double>* __restrict__ uref, cx<double>* __restrict__ ucopy) {
   if (act && *act == 0) return;
   constexpr int HALO = S1;
-  __shared__ cx<double> tile[3][kTM][kCurlTY + 2 * HALO + 1];
-  const int ntile_y = (Ny + kCurlTY - 1) / kCurlTY;
-  const int y0 = (blockIdx.x % ntile_y) * kCurlTY;
+  __shared__ cx<double> tile[3][kTM][TY + 2 * HALO + 1];
+  const int ntile_y = (Ny + TY - 1) / TY;
+  const int y0 = (blockIdx.x % ntile_y) * TY;
   const int m0 = (blockIdx.x / ntile_y) * kTM;
   const int ylo = max(0, y0 - HALO);
-  const int yhi = min(Ny, y0 + kCurlTY + HALO);
+  const int yhi = min(Ny, y0 + TY + HALO);
   const int span = yhi - ylo;
   const size_t comp = (size_t)M * Ny;
   // load: lanes along y
@@ -1059,7 +1076,7 @@ __global__ void __launch_bounds__(kCurlTY* kTM)
       val = u[e];
       if constexpr (EM == EV_RECORD) {
         const int yy = ylo + yi;  // the tile's own rows (not its halo)
-        if (yy >= y0 && yy < y0 + kCurlTY) ucopy[e] = val;
+        if (yy >= y0 && yy < y0 + TY) ucopy[e] = val;
       }
       if constexpr (EM == EV_INC) {
         const cx<double> r = uref[e];
@@ -1952,21 +1969,29 @@ __global__ void __launch_bounds__(256)
 }
 
 // ------------------------------------------------------------------ launchers
-template <typename T, int EM>
-static void launch_walls_curl(Ctx* c, dim3 grid, const cx<double>* u, cx<T>* A, hipStream_t s) {
+template <typename T, int EM, int TY>
+static void launch_walls_curl_ty(Ctx* c, const cx<double>* u, cx<T>* A, hipStream_t s) {
   WallTables* t = c->wt;
   const cx<double>* uref = reinterpret_cast<const cx<double>*>(c->uref);
   cx<double>* ucopy = reinterpret_cast<cx<double>*>(c->uref);
+  const dim3 grid(((t->Ny + TY - 1) / TY) * ((t->M + kTM - 1) / kTM));
   switch (t->p) {
 #define CURL(PO)                                                                                \
   case PO:                                                                                      \
-    hipLaunchKernelGGL((k_walls_curl_t<T, PO + 1, EM>), grid, dim3(kCurlTY * kTM), 0, s, u, A,  \
+    hipLaunchKernelGGL((k_walls_curl_t<T, PO + 1, EM, TY>), grid, dim3(TY * kTM), 0, s, u, A,   \
                        t->d1w, t->d1j0, t->mkz, t->Ny, t->M, c->act, uref, ucopy);              \
     break;
     CURL(2) CURL(4) CURL(6) CURL(8)
 #undef CURL
     default: break;
   }
+}
+template <typename T, int EM>
+static void launch_walls_curl(Ctx* c, const cx<double>* u, cx<T>* A, hipStream_t s) {
+  if (c->wt->curl_ty == 64)
+    launch_walls_curl_ty<T, EM, 64>(c, u, A, s);
+  else
+    launch_walls_curl_ty<T, EM, 32>(c, u, A, s);
 }
 
 // em: EvalMode; EV_RECORD / EV_INC only in FP32 with the reference buffers.
@@ -1980,16 +2005,15 @@ static int walls_rhs_t(Ctx* c, const double* state, double* rhs, hipStream_t s, 
   const int ntile_y = (Ny + kTY - 1) / kTY, ntile_m = (M + kTM - 1) / kTM;
   {
     ProfScope ps(c, KID_W_Z_INVERSE, s);
-    const dim3 grid(((Ny + kCurlTY - 1) / kCurlTY) * ntile_m);
     const cx<double>* u = reinterpret_cast<const cx<double>*>(state);
     if (t->p != 2 && t->p != 4 && t->p != 6 && t->p != 8) return c->fail(DNSMI_EINVAL, "fd_order");
     if (em == EV_PLAIN)
-      launch_walls_curl<T, EV_PLAIN>(c, grid, u, A, s);
+      launch_walls_curl<T, EV_PLAIN>(c, u, A, s);
     else if constexpr (std::is_same<T, float>::value) {
       if (em == EV_RECORD)
-        launch_walls_curl<T, EV_RECORD>(c, grid, u, A, s);
+        launch_walls_curl<T, EV_RECORD>(c, u, A, s);
       else
-        launch_walls_curl<T, EV_INC>(c, grid, u, A, s);
+        launch_walls_curl<T, EV_INC>(c, u, A, s);
     }
     int rc = check_launch(c, "walls_curl");
     if (rc) return rc;

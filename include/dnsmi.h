@@ -79,7 +79,8 @@ typedef struct dnsmi_config {
                               8 sequential band sweeps (walls; else chunk maps),
                               16 the chunk maps' fused two-launch influence-
                               matrix step (walls, without 8), 32 non-temporal
-                              stores of the z-inverse output (periodic) */
+                              stores of the z-inverse output (periodic), 64
+                              64-row workgroups of the FD curl (walls) */
   int plain_mixed;         /* DNSMI_MIXED only: 0 (default) evaluates every
                               corrector pass of a step as an FP32 increment over
                               the step's first evaluation (split precision);

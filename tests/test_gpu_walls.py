@@ -109,10 +109,13 @@ def test_gpu_setup_equals_host_setup(shape):
 
 
 @pytest.mark.parametrize("case", list(CASES))
-@pytest.mark.parametrize("variants", [None, 0, 16, 8], ids=["autotuned", "chunked", "fused", "sequential"])
+@pytest.mark.parametrize("variants", [None, 0, 16, 8, 64], ids=["autotuned", "chunked", "fused", "sequential",
+                                                                "curl64"])
 def test_golden_rhs_and_imm_step(case, variants):
     """The band-solve forms (chunk maps in five launches or fused into two;
-    sequential sweeps) and the autotuner's choice against the golden steps."""
+    sequential sweeps), the FD curl in 64-row workgroups (variant bit 64; the
+    automatic choice at large shapes) and the autotuner's choice against the
+    golden steps."""
     g = np.load(os.path.join(GOLDEN, case + ".npz"))
     flow = make(case) if variants is None else make(case, tile_cols=16, variants=variants)
     u = flow.to_device(g["state"])

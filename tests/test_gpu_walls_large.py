@@ -65,6 +65,21 @@ def test_gpu_setup_equals_host_setup_at_config_shape():
     assert ms(G.tuning) < ms(H.tuning)
 
 
+def test_tall_curl_tiles_equal_the_short_ones():
+    """At this shape the context picks 64-row FD-curl workgroups (32 at the
+    smaller configured shapes; the golden tests run both forms against the
+    oracle there): the RHS equals that of forced 32-row workgroups bit for
+    bit, in FP64 and in the mixed form."""
+    for prec in ("fp64", "mixed"):
+        auto = make(precision=prec)
+        assert "curl_rows=64" in auto._native.tuning and auto._native.variants()[1] & 64
+        short = make(precision=prec, tile_cols=16, variants=auto._native.variants()[1] & ~64)
+        assert "curl_rows=32" in short._native.tuning
+        u = auto.init_state(seed=4, rms=0.05)
+        assert torch.equal(auto.get_rhs(u), short.get_rhs(u)), prec
+        del auto, short
+
+
 def test_band_solve_forms_agree_with_wall_invariants():
     from dnsjax_amd.fields import wall_field
 
