@@ -220,7 +220,7 @@ def test_walls_stats_match_oracle(case):
 
 
 @pytest.mark.parametrize("p", [2, 8])
-@pytest.mark.parametrize("variants", [0, 16, 8], ids=["chunked", "fused", "sequential"])
+@pytest.mark.parametrize("variants", [0, 16, 8, 64], ids=["chunked", "fused", "sequential", "curl64"])
 def test_fd_orders_2_and_8_against_oracle(p, variants):
     """The remaining stencil orders (the goldens hold 4 and 6): 8 x 33 x 8,
     Re 400, dt 0.01.  The RHS to 1e-12, one predictor-corrector step and
